@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Plain-apply throughput of the stretched-leaf path on one MI355X.
+
+Workload: 20 deg polar cap, lmax = 2048, M = 2.0e6 area-uniform points, tol = 1e-8, ntrans = 1,
+FP64 throughout; alm complex with Re/Im ~ N(0, C_l/2), C_l = (l+1)^-2 (a_l0 real); f complex
+N(0, 1). One step = one synthesis mul!(f, plan, alm) plus one adjoint mul!(alm, plan', f), inputs
+resident in HBM. `value` counts applies (either direction) per second.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line (the last line of stdout). Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "partialfsht.jl_amd"))
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def cap_points(rng, deg, M):
+    ts = np.deg2rad(deg)
+    th = np.arccos(rng.uniform(np.cos(ts), 1.0, M))
+    ph = rng.uniform(0.0, 2 * np.pi, M)
+    return th, ph
+
+
+def rand_alm(L, rng, cl_power=2.0):
+    rows = np.arange(L + 1)[:, None]
+    cols = np.arange(2 * L + 1)[None, :]
+    l = rows + (cols + 1) // 2
+    z = rng.standard_normal((L + 1, 2 * L + 1)) + 1j * rng.standard_normal((L + 1, 2 * L + 1))
+    z *= np.sqrt(0.5 * (l + 1.0) ** (-cl_power))
+    z[:, 0] = z[:, 0].real
+    z[l > L] = 0
+    return np.asfortranarray(z)
+
+
+def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
+    """The oracle's threaded restatement of the reference host path on this host (test
+    infrastructure, timed as the baseline only): one synthesis and one adjoint at full size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    t0 = time.perf_counter()
+    p = O.Plan(th, ph, L, tol=tol, method="stretch", nufft="es", nthreads=nthreads)
+    t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p.synth(alm)
+    t_s = time.perf_counter() - t0
+    st = p.last_times()
+    t0 = time.perf_counter()
+    p.adjoint(f)
+    t_a = time.perf_counter() - t0
+    return {"value": 2.0 / (t_s + t_a), "unit": "applies/s", "cores": nthreads, "kind": "port",
+            "sample": f"1 synthesis + 1 adjoint at the full workload (lmax {L}, M {len(th)}); "
+                      f"ring pass threaded over interleaved orders, ES-kernel NUFFT, {nthreads} threads",
+            "synth_s": t_s, "adjoint_s": t_a, "plan_s": t_plan,
+            "synth_stage_s": {"ring": st[0], "grid": st[1], "nufft": st[2]},
+            "ns_per_triple_ring_synth": st[0] * 1e9 / p.info["ring_work"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lmax", type=int, default=2048)
+    ap.add_argument("--npts", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (collectives are out of "
+              "scope); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L, tol, M = args.lmax, 1e-8, args.npts
+    rng = np.random.default_rng(2048)
+    th, ph = cap_points(rng, 20, M)
+    alm = rand_alm(L, rng)
+    f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+
+    log(f"plan lmax={L} M={M} tol={tol}")
+    t0 = time.perf_counter()
+    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method="stretch")
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    info = plan.info
+    log(f"plan built in {t_plan:.2f}s: nth={info['nth']} nph={info['nph']} mmax={info['mmax']} "
+        f"ring_work={info['ring_work']} fine={info['es_n1']}x{info['es_n2']} w={info['es_w']}")
+
+    alm_d = pfsht.to_device(alm)
+    f_d = pfsht.to_device(f)
+    f_out = torch.empty(M, dtype=torch.complex128, device="cuda")
+    a_out = torch.empty(alm.size, dtype=torch.complex128, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        plan.synth(f_out, alm_d, stream)
+        plan.adjoint(a_out, f_d, stream)
+    torch.cuda.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+
+    plan.set_timing(True)
+    plan.stage_times("synth", reset=True)
+    plan.stage_times("adjoint", reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.synth(f_out, alm_d, stream)
+        plan.adjoint(a_out, f_d, stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.set_timing(False)
+    elapsed = t1 - t0
+    ms_step = elapsed / args.steps * 1e3
+    applies_s = 2 * args.steps / elapsed
+    ss = plan.stage_times("synth")
+    sa = plan.stage_times("adjoint")
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {applies_s:.1f} applies/s")
+
+    W = info["ring_work"]
+    nu_b = 2 * 16 * M + 16 * info["es_n1"] * info["es_n2"] + 4 * M  # points + f, fine grid, perm
+    stages = {}
+    for name, d, flops, bytes_ in (
+            ("ring_synth", ss["ring"], 12.0 * W, None),
+            ("ring_adjoint", sa["ring"], 12.0 * W, None),
+            ("nufft_interp", ss["nufft"], None, nu_b),
+            ("nufft_spread", sa["nufft"], None, nu_b),
+            ("fine_fft_synth", ss["fine_fft"], None, None),
+            ("fine_fft_adjoint", sa["fine_fft"], None, None)):
+        ms_avg = d[0] / max(d[1], 1)
+        e = {"ms": ms_avg}
+        if flops:
+            e["tflops"] = flops / (ms_avg * 1e-3) / 1e12
+            e["frac_fp64_peak"] = e["tflops"] / PEAK_FP64_TFLOPS
+        if bytes_:
+            e["gbs"] = bytes_ / (ms_avg * 1e-3) / 1e9
+            e["frac_hbm_peak"] = e["gbs"] / PEAK_HBM_GBS
+        stages[name] = e
+    per_stage = {"synth": {k: v[0] / max(v[1], 1) for k, v in ss.items()},
+                 "adjoint": {k: v[0] / max(v[1], 1) for k, v in sa.items()}}
+    ring = stages["ring_synth"]
+    roofline = {"kernel": "ring_fwd_kernel (synthesis ring pass)", "bound": "mfma",
+                "achieved": ring["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ring["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
+                "algorithmic": f"12 FP64 flops per retained (l,m,ring) triple x {W} triples per launch; "
+                               "FP64 VALU peak = FP64 matrix peak on MI355X"}
+
+    cpu = None
+    if not args.no_cpu:
+        nt = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline (oracle, {nt} threads)")
+        cpu = cpu_baseline(th, ph, L, tol, alm, f, nt)
+        log(f"cpu baseline: {cpu['value']:.3f} applies/s")
+
+    out = {"metric": "applies/s at lmax=2048 tol=1e-8 (synth+adjoint); served p50/p99 latency",
+           "value": applies_s, "unit": "applies/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"plain apply, 20deg polar cap, lmax={L}, M={M} area-uniform, "
+                                  f"tol={tol}, ntrans=1, one synthesis + one adjoint per step",
+                      "lmax": L, "npts": M, "tol": tol, "ntrans": 1,
+                      "nth": info["nth"], "nph": info["nph"], "mmax": info["mmax"],
+                      "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
+                      "es_w": info["es_w"]},
+           "triples_per_s": W * 2 * args.steps / elapsed,
+           "plan_s": t_plan,
+           "stage_ms": per_stage,
+           "stages": stages,
+           "roofline": roofline,
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+/*
+ * pfsht.h — C ABI of libpfsht_hip.so, the MI355X (gfx950) implementation of PartialFSHT.jl's
+ * stretched-leaf apply path: synthesis mul!(f, plan, alm) and its exact adjoint
+ * mul!(alm, plan', f) on a Plan(theta, phi; lmax, tol) in sph_mode layout.
+ *
+ * Plain C types only. Complex arrays are interleaved (re, im) doubles, which is the memory of
+ * Julia's ComplexF64 and of C99 double _Complex. Layouts are the reference's column-major ones:
+ *   alm : (lmax+1) x (2 lmax+1) x ntrans, entry (l, m) at row l-|m|, column 2|m| + (m >= 0) - 1
+ *         (0-based), i.e. FastSphericalHarmonics sph_mode (src/Layout.jl:13,20);
+ *   f   : npts x ntrans (src/Layout.jl:47-56);
+ *   G   : nrings x (2 lmax+1) x ntrans, column m + lmax (src/Legendre.jl:306).
+ * Every call returns PFSHT_OK or an error code, with a message in pfsht_last_error() (per thread).
+ * Nothing aborts. A plan owns all device memory it uses; applies allocate nothing.
+ */
+#ifndef PFSHT_H
+#define PFSHT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: the Julia extension maps E_DIM to DimensionMismatch, E_ARG / E_GEOM_REFUSED to
+ * ArgumentError (src/Layout.jl:35-56; src/Plan.jl:248-252; src/Windows.jl:66-74). */
+enum {
+    PFSHT_OK = 0,
+    PFSHT_E_DIM = 1,
+    PFSHT_E_ARG = 2,
+    PFSHT_E_GEOM_REFUSED = 3,
+    PFSHT_E_NOMEM = 4,
+    PFSHT_E_HIP = 5,
+    PFSHT_E_UNSUPPORTED = 6
+};
+
+/* Leaf selection, the AbstractAlgorithm tags of src/Cost.jl:15-24. */
+enum { PFSHT_METHOD_AUTO = 0, PFSHT_METHOD_DIRECT = 1, PFSHT_METHOD_STRETCH = 2 };
+
+/* Where apply buffers live: host memory (copied through the plan's staging) or HBM. */
+enum { PFSHT_WHERE_HOST = 0, PFSHT_WHERE_DEVICE = 1 };
+
+/* Arithmetic: FP64 throughout, or FP64 recurrence with FP32 accumulation/FFT/NUFFT (tol >= 1e-6). */
+enum { PFSHT_PREC_FP64 = 0, PFSHT_PREC_MIXED = 1 };
+
+/* Periodisation kinds (src/Windows.jl:34-36); -1 for a Direct leaf. */
+enum { PFSHT_KIND_POLE = 0, PFSHT_KIND_INTERIOR = 1, PFSHT_KIND_FULL = 2 };
+
+typedef struct pfsht_plan_s *pfsht_plan_t;
+typedef struct pfsht_rings_s *pfsht_rings_t;
+
+typedef struct {
+    int32_t method;    /* PFSHT_METHOD_DIRECT or PFSHT_METHOD_STRETCH (as chosen) */
+    int32_t kind;      /* PFSHT_KIND_* */
+    int32_t flip;      /* south-pole periodisation reflected north (src/Stretch.jl:151-158) */
+    int32_t lmax, ntrans, precision;
+    int32_t nth_half;  /* rings the ring engine evaluates */
+    int32_t nth, nph;  /* stretched grid */
+    int32_t mmax;      /* largest retained order */
+    int32_t es_w, es_n1, es_n2; /* NUFFT kernel width and fine grid */
+    int64_t npts;
+    int64_t ring_work; /* retained (l, m, ring) triples per apply (src/Legendre.jl:286-293) */
+    int64_t device_bytes;
+    double tol, es_beta;
+} pfsht_info;
+
+/* Number of timed stages reported by pfsht_stage_times. */
+#define PFSHT_NSTAGES 8
+/* Stage order: 0 ring pass, 1 pack/unpack+window, 2 phi FFT, 3 DFS double/fold,
+ * 4 2-D FFT, 5 deconvolve/phase, 6 fine FFT, 7 NUFFT interpolate/spread(+merge). */
+
+const char *pfsht_last_error(void);
+const char *pfsht_version(void);
+
+/* Plan(theta, phi; lmax, tol, ntrans, method) — replaces the constructor of src/Plan.jl:234-266
+ * with the StretchLeaf of src/Stretch.jl:129-248 (or DirectLeaf, src/Direct.jl:241-267) built on
+ * the device. theta, phi: host arrays of npts colatitudes in [0, pi] and longitudes. */
+int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                      int32_t ntrans, int32_t method, int32_t precision, pfsht_plan_t *out);
+void pfsht_plan_destroy(pfsht_plan_t plan);
+int pfsht_plan_info(pfsht_plan_t plan, pfsht_info *info);
+
+/* mul!(f, plan, alm) — src/Plan.jl:287-291 -> synthesize_stretch! (src/Stretch.jl:262-277).
+ * Overwrites f. stream: a hipStream_t or NULL for the null stream; the call is asynchronous
+ * on that stream when where == PFSHT_WHERE_DEVICE. */
+int pfsht_synth(pfsht_plan_t plan, const void *alm, void *f, int32_t where, void *stream);
+
+/* mul!(alm, plan', f) — src/Plan.jl:293-298 -> adjoint_stretch! (src/Stretch.jl:279-295).
+ * Overwrites alm, padding slots included (src/Legendre.jl:338). */
+int pfsht_adjoint(pfsht_plan_t plan, const void *f, void *alm, int32_t where, void *stream);
+
+/* RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255. theta sorted
+ * ascending, host memory. */
+int pfsht_rings_create(const double *theta, int32_t nrings, int32_t lmax, double eps, int32_t ntrans,
+                       int32_t reflect, pfsht_rings_t *out);
+void pfsht_rings_destroy(pfsht_rings_t rings);
+int64_t pfsht_rings_work(pfsht_rings_t rings);
+int32_t pfsht_rings_mmax(pfsht_rings_t rings);
+/* rings_from_alm!(G, alm, eng) — src/Legendre.jl:310-323; G overwritten (zero off the retained set). */
+int pfsht_rings_from_alm(pfsht_rings_t rings, const void *alm, void *G, int32_t where, void *stream);
+/* alm_from_rings!(alm, G, eng) — src/Legendre.jl:332-341; alm overwritten. */
+int pfsht_alm_from_rings(pfsht_rings_t rings, const void *G, void *alm, int32_t where, void *stream);
+
+/* The plan's own ring engine, for stage-level parity of a stretched plan. */
+int pfsht_plan_rings(pfsht_plan_t plan, pfsht_rings_t *out);
+
+/* Per-stage device time of the plan's applies, recorded with HIP events on the apply stream while
+ * timing is enabled. ms[PFSHT_NSTAGES] are summed milliseconds, counts[PFSHT_NSTAGES] launches
+ * timed; reset clears both. Synthesis and adjoint are reported separately (dir 0 / 1). */
+int pfsht_set_timing(pfsht_plan_t plan, int32_t enable);
+int pfsht_stage_times(pfsht_plan_t plan, int32_t dir, double *ms, int64_t *counts, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

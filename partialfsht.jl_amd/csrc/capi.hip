@@ -1,0 +1,733 @@
+// capi.hip — plan construction, apply orchestration and the C ABI of include/pfsht.h.
+//
+// A plan is the device analogue of StretchLeaf (src/Stretch.jl:94-248): geometry, ring tables,
+// window, bin-sorted points, NUFFT kernel tables, work buffers and rocFFT plans are built once and
+// stay resident in HBM; an apply is a fixed sequence of launches on the caller's stream and
+// allocates nothing.
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/pfsht.h"
+#include "pfsht_common.hpp"
+
+namespace pfsht {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+template <class T>
+static Status dalloc(T **p, size_t count, int64_t &bytes) {
+    *p = nullptr;
+    if (count == 0) return kOk;
+    const size_t nb = count * sizeof(T);
+    if (std::getenv("PFSHT_FAIL_ALLOC")) {  // fault injection for the out-of-memory path
+        set_error("device allocation failed (PFSHT_FAIL_ALLOC set)");
+        return kENomem;
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), nb);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        set_error(std::string("device allocation of ") + std::to_string(nb) + " bytes failed: " + hipGetErrorString(e));
+        return kENomem;
+    }
+    bytes += int64_t(nb);
+    return kOk;
+}
+
+template <class T>
+static Status upload(T **p, const std::vector<T> &h, int64_t &bytes) {
+    Status st = dalloc(p, h.size(), bytes);
+    if (st != kOk || h.empty()) return st;
+    PFSHT_HIP_TRY(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return kOk;
+}
+
+template <class T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// RAII device guard: make the plan's device current for the call, restore the caller's after.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+static std::once_flag g_rocfft_once;
+static rocfft_status g_rocfft_setup = rocfft_status_success;
+
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+};
+
+static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, size_t batch, int64_t &bytes) {
+    std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
+    if (g_rocfft_setup != rocfft_status_success) { set_error("rocfft_setup failed"); return kEHip; }
+    size_t lengths[2] = {n_fast, n_slow};
+    const size_t dims = n_slow > 0 ? 2 : 1;
+    rocfft_status s = rocfft_plan_create(&fp.plan, rocfft_placement_inplace,
+                                         forward ? rocfft_transform_type_complex_forward
+                                                 : rocfft_transform_type_complex_inverse,
+                                         rocfft_precision_double, dims, lengths, batch, nullptr);
+    if (s != rocfft_status_success) { set_error("rocfft_plan_create failed"); return kEHip; }
+    rocfft_plan_get_work_buffer_size(fp.plan, &fp.work_bytes);
+    if (rocfft_execution_info_create(&fp.info) != rocfft_status_success) {
+        set_error("rocfft_execution_info_create failed");
+        return kEHip;
+    }
+    if (fp.work_bytes) {
+        Status st = dalloc(reinterpret_cast<char **>(&fp.work), fp.work_bytes, bytes);
+        if (st != kOk) return st;
+        rocfft_execution_info_set_work_buffer(fp.info, fp.work, fp.work_bytes);
+    }
+    return kOk;
+}
+
+static void fft_free(FftPlan &fp) {
+    if (fp.plan) rocfft_plan_destroy(fp.plan);
+    if (fp.info) rocfft_execution_info_destroy(fp.info);
+    if (fp.work) (void)hipFree(fp.work);
+    fp = FftPlan{};
+}
+
+static Status fft_exec(FftPlan &fp, void *buf, hipStream_t s) {
+    rocfft_execution_info_set_stream(fp.info, s);
+    void *in[1] = {buf};
+    if (rocfft_execute(fp.plan, in, nullptr, fp.info) != rocfft_status_success) {
+        set_error("rocfft_execute failed");
+        return kEHip;
+    }
+    return kOk;
+}
+
+}  // namespace pfsht
+
+using namespace pfsht;
+
+// ------------------------------------------------------------------------------------------------
+// Ring engine
+// ------------------------------------------------------------------------------------------------
+struct pfsht_rings_s {
+    int dev = 0;
+    RingDev d;
+    std::vector<int> jlo, jhi;
+    int64_t work = 0;
+    int64_t bytes = 0;
+    double2 *Gd = nullptr;                    // [B][2 mmax + 1][n]
+    double2 *alm_stage = nullptr, *G_stage = nullptr;  // for host-memory calls
+    std::mutex mu;
+
+    ~pfsht_rings_s() {
+        dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.abofs);
+        dfree(d.jlo); dfree(d.jhi); dfree(d.items); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
+        dfree(Gd); dfree(alm_stage); dfree(G_stage);
+    }
+};
+
+static Status rings_build(const double *theta, int n, int L, double eps, int B, bool reflect, pfsht_rings_s &r) {
+    if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
+    if (B < 1) { set_error("ntrans must be >= 1"); return kEArg; }
+    if (n < 1) { set_error("need at least one ring"); return kEArg; }
+    for (int j = 1; j < n; ++j)
+        if (!(theta[j] >= theta[j - 1])) { set_error("ring colatitudes must be sorted ascending"); return kEArg; }
+    (void)hipGetDevice(&r.dev);
+    const double csgn = reflect ? -1.0 : 1.0;
+    std::vector<double> costh(n), log2sin(n);
+    std::vector<int> mlim(n);
+    int mmax = 0;
+    for (int j = 0; j < n; ++j) {
+        const double s = std::sin(theta[j]), c = std::cos(theta[j]);
+        const double as = std::fabs(s);
+        costh[j] = csgn * c;
+        log2sin[j] = as > 0 ? std::log2(as) : -INFINITY;
+        mlim[j] = polar_mmax(theta[j], L, eps);
+        mmax = std::max(mmax, mlim[j]);
+    }
+    std::string msg;
+    if (!nested_ranges(mlim, L, r.jlo, r.jhi, msg)) { set_error(msg); return kEArg; }
+    std::vector<double> log2c;
+    sectorial_log2c(L, log2c);
+    RingDev &d = r.d;
+    d.n = n; d.L = L; d.B = B; d.mmax = mmax;
+    // recurrence coefficients, evaluated in the reference's order (src/Legendre.jl:406-408)
+    std::vector<int64_t> abofs(mmax + 1);
+    int64_t nab = 0;
+    for (int m = 0; m <= mmax; ++m) { abofs[m] = nab; nab += std::max(0, L - m - 1); }
+    std::vector<double2> ab(std::max<int64_t>(nab, 1));
+    for (int m = 0; m <= mmax; ++m)
+        for (int l = m + 2; l <= L; ++l) {
+            const double a = std::sqrt(double(2 * l - 1) * double(2 * l + 1) / (double(l - m) * double(l + m)));
+            const double bb = std::sqrt(double(2 * l + 1) / double(2 * l - 3) * double(l - m - 1) / double(l - m) *
+                                        double(l + m - 1) / double(l + m));
+            ab[abofs[m] + (l - m - 2)] = make_double2(a, bb);
+        }
+    // work items: tiles of <= 64 retained rings per order, longest first
+    std::vector<RingItem> items;
+    std::vector<int32_t> ntiles(mmax + 1, 0);
+    std::vector<int64_t> pofs(mmax + 1, 0);
+    int64_t np = 0;
+    r.work = 0;
+    for (int m = 0; m <= mmax; ++m) {
+        pofs[m] = np;
+        const int lo = r.jlo[m], hi = r.jhi[m];
+        if (hi < lo) continue;
+        r.work += int64_t(hi - lo + 1) * (L - m + 1);
+        int t = 0;
+        for (int j0 = lo; j0 <= hi; j0 += 64, ++t) items.push_back({m, j0, std::min(64, hi - j0 + 1), t});
+        ntiles[m] = t;
+        np += int64_t(t) * (L - m + 1) * B;
+    }
+    std::stable_sort(items.begin(), items.end(), [L](const RingItem &a, const RingItem &b) {
+        return int64_t(L - a.m + 1) * a.nr > int64_t(L - b.m + 1) * b.nr;
+    });
+    d.nitems = int(items.size());
+    d.npartials = np;
+    std::vector<int32_t> jlo32(r.jlo.begin(), r.jlo.begin() + mmax + 1), jhi32(r.jhi.begin(), r.jhi.begin() + mmax + 1);
+    Status st;
+    int64_t &by = r.bytes;
+    if ((st = upload(&d.costh, costh, by)) != kOk) return st;
+    if ((st = upload(&d.log2sin, log2sin, by)) != kOk) return st;
+    if ((st = upload(&d.log2c, log2c, by)) != kOk) return st;
+    if ((st = upload(&d.ab, ab, by)) != kOk) return st;
+    if ((st = upload(&d.abofs, abofs, by)) != kOk) return st;
+    if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
+    if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
+    if ((st = upload(&d.items, items, by)) != kOk) return st;
+    if ((st = upload(&d.pofs, pofs, by)) != kOk) return st;
+    if ((st = upload(&d.ntiles, ntiles, by)) != kOk) return st;
+    if ((st = dalloc(&d.partials, size_t(std::max<int64_t>(np, 1)), by)) != kOk) return st;
+    if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
+    return kOk;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Plan
+// ------------------------------------------------------------------------------------------------
+namespace {
+constexpr int kStages = PFSHT_NSTAGES;
+// Events in execution order; interval k (ev[k] -> ev[k+1]) is stage stage_of[k].
+struct TimedApply {
+    int dir;
+    hipEvent_t ev[kStages + 1];
+    int stage_of[kStages];
+};
+}  // namespace
+
+struct pfsht_plan_s {
+    int dev = 0;
+    int method = PFSHT_METHOD_STRETCH, precision = PFSHT_PREC_FP64;
+    int L = 0, B = 1;
+    int64_t M = 0;
+    double tol = 0;
+    Geom geom;
+    int flip = 0, nh = 0, nth = 0, nph = 0, mmax = 0;
+    pfsht_rings_s *rings = nullptr;
+    double *wth = nullptr;
+    NufftDev nu;
+    double2 *Fp = nullptr, *Ft = nullptr, *U = nullptr;
+    FftPlan phi_bwd, phi_fwd, grid_fwd, grid_bwd, fine_bwd, fine_fwd;
+    double2 *alm_stage = nullptr, *f_stage = nullptr;
+    int64_t bytes = 0;
+    std::mutex mu;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<TimedApply> pending;
+    double ms[2][kStages] = {};
+    int64_t counts[2][kStages] = {};
+
+    ~pfsht_plan_s() {
+        delete rings;
+        dfree(wth);
+        dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c2);
+        dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
+        dfree(Fp); dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage);
+        fft_free(phi_bwd); fft_free(phi_fwd); fft_free(grid_fwd); fft_free(grid_bwd);
+        fft_free(fine_bwd); fft_free(fine_fwd);
+        for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi, int64_t M, int L, double tol, int B,
+                         int method) {
+    if (M < 1) { set_error("need at least one point"); return kEArg; }
+    if (M > INT32_MAX) { set_error("more than 2^31-1 points per plan is not supported"); return kEArg; }
+    if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
+    if (B < 1) { set_error("ntrans must be >= 1"); return kEArg; }
+    if (!(tol > 0) || !(tol < 1)) { set_error("tol must lie in (0, 1)"); return kEArg; }
+    (void)hipGetDevice(&p.dev);
+    p.L = L; p.B = B; p.M = M; p.tol = tol;
+    // bounding box in colatitude (src/Box.jl:43-73)
+    double thlo = INFINITY, thhi = -INFINITY;
+    for (int64_t i = 0; i < M; ++i) { thlo = std::min(thlo, theta[i]); thhi = std::max(thhi, theta[i]); }
+    thlo = std::max(thlo, 0.0);
+    thhi = std::min(thhi, kPi);
+    if (method == PFSHT_METHOD_AUTO) {
+        const double cd = cost_direct(M, L, B), cs = cost_stretch(thlo, thhi, M, L, B, tol);
+        method = cs < cd ? PFSHT_METHOD_STRETCH : PFSHT_METHOD_DIRECT;
+    }
+    p.method = method;
+    if (method == PFSHT_METHOD_DIRECT) {
+        set_error("the Direct leaf (src/Direct.jl) is not available on the device in this build; the stretched "
+                  "construction is degenerate or more expensive for this point set");
+        return kEUnsupported;
+    }
+    if (method != PFSHT_METHOD_STRETCH) { set_error("unknown method"); return kEArg; }
+    // geometry and southern reflection (src/Stretch.jl:151-158)
+    std::string msg;
+    const double tiny = std::sqrt(DBL_EPSILON);
+    Geom g0;
+    int rc = stretch_geometry(thlo, thhi, L, tol, g0, msg);
+    if (rc != 0) { set_error(msg); return rc < 0 ? kEArg : kEGeom; }
+    p.flip = g0.kind == kPole && g0.thhi >= kPi - tiny && g0.thlo > tiny;
+    if (p.flip) {
+        rc = stretch_geometry(kPi - thhi, kPi - thlo, L, tol, p.geom, msg);
+        if (rc != 0) { set_error(msg); return rc < 0 ? kEArg : kEGeom; }
+    } else {
+        p.geom = g0;
+    }
+    // sizing (src/Stretch.jl:164-171)
+    const int Kt = p.geom.K + kKMargin;
+    if (p.geom.kind == kInterior) { p.nth = int(next_smooth(2 * Kt + 1)); p.nh = p.nth; }
+    else { p.nh = int(next_smooth(Kt + 1)); p.nth = 2 * p.nh; }
+    std::vector<double> th, Th;
+    stretch_rings(p.geom, p.nh, th, Th);
+    if (!std::is_sorted(th.begin(), th.end())) { set_error("ring colatitudes are not ascending"); return kEArg; }
+    p.rings = new pfsht_rings_s;
+    Status st = rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings);
+    if (st != kOk) return st;
+    p.mmax = p.rings->d.mmax;
+    p.nph = int(next_smooth_even(2 * int64_t(p.mmax) + 1));
+    int64_t &by = p.bytes;
+    std::vector<double> wth(p.nh);
+    for (int i = 0; i < p.nh; ++i) wth[i] = stretch_window(p.geom, Th[i]);
+    if ((st = upload(&p.wth, wth, by)) != kOk) return st;
+
+    // NUFFT: kernel, fine grid, bin-sorted points (src/Stretch.jl:202-213, 235-238)
+    NufftDev &u = p.nu;
+    u.B = B; u.n1 = p.nth; u.n2 = p.nph; u.M = M;
+    es_params(tol, u.w, u.beta);
+    u.N1 = es_fine_size(p.nth, u.w);
+    u.N2 = es_fine_size(p.nph, u.w);
+    u.S = 17 - u.w;
+    const int S4 = 4 * u.S;
+    u.ns1 = (u.N1 + S4 - 1) / S4;
+    u.ns2 = (u.N2 + S4 - 1) / S4;
+    u.win = 3 * u.S + 16;
+    std::vector<double> X1(M), X2(M);
+    std::vector<int32_t> key(M);
+    const double half = 0.5 * u.w;
+    for (int64_t i = 0; i < M; ++i) {
+        const double t = theta[i];
+        const double Tst = theta_to_stretch(p.geom, p.flip ? kPi - t : t);
+        const double ph = jmod(phi[i], 2 * kPi);
+        double x1 = Tst * (double(u.N1) / (2 * kPi));
+        double x2 = ph * (double(u.N2) / (2 * kPi));
+        x1 -= std::floor(x1 / u.N1) * u.N1;
+        x2 -= std::floor(x2 / u.N2) * u.N2;
+        int b1 = int(std::ceil(x1 - half)), b2 = int(std::ceil(x2 - half));
+        if (b1 < 0) { x1 += u.N1; b1 = int(std::ceil(x1 - half)); }
+        if (b2 < 0) { x2 += u.N2; b2 = int(std::ceil(x2 - half)); }
+        if (b1 < 0 || b1 >= u.N1 || b2 < 0 || b2 >= u.N2) { set_error("point maps outside the fine grid"); return kEArg; }
+        X1[i] = x1;
+        X2[i] = x2;
+        const int s1 = b1 / S4, s2 = b2 / S4;
+        const int lb = ((b1 % S4) / u.S) * 4 + (b2 % S4) / u.S;
+        key[i] = (s1 * u.ns2 + s2) * 16 + lb;
+    }
+    const int nkeys = u.ns1 * u.ns2 * 16;
+    std::vector<int64_t> cnt(int64_t(nkeys) + 1, 0);
+    for (int64_t i = 0; i < M; ++i) cnt[key[i] + 1]++;
+    for (int k = 0; k < nkeys; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int32_t> perm(M);
+    {
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int64_t i = 0; i < M; ++i) perm[pos[key[i]]++] = int32_t(i);
+    }
+    std::vector<double> X1s(M), X2s(M);
+    for (int64_t q = 0; q < M; ++q) { X1s[q] = X1[perm[q]]; X2s[q] = X2[perm[q]]; }
+    // Non-empty super-tiles, and per (slot, local bin) the start of its point range: a slot's bins
+    // are contiguous in the sorted order and empty super-tiles own no points, so bo[q + 1] is the
+    // end of range q for every q, with bo[nst * 16] = M.
+    std::vector<int32_t> st_list, st_slot(size_t(u.ns1) * u.ns2, -1);
+    for (int s = 0; s < u.ns1 * u.ns2; ++s) {
+        if (cnt[int64_t(s) * 16 + 16] == cnt[int64_t(s) * 16]) continue;
+        st_slot[s] = int32_t(st_list.size());
+        st_list.push_back(s);
+    }
+    std::vector<int32_t> bo(st_list.size() * 16 + 1);
+    for (size_t slot = 0; slot < st_list.size(); ++slot)
+        for (int lb = 0; lb < 16; ++lb) bo[slot * 16 + lb] = int32_t(cnt[int64_t(st_list[slot]) * 16 + lb]);
+    bo[st_list.size() * 16] = int32_t(M);
+    u.nst = int(st_list.size());
+    // pc1[i] = phase[i] * h1/psihat1(k_i), phase = cis(-k shift)/(nth nph) (src/Stretch.jl:229-233)
+    std::vector<double> c1, c2;
+    es_corr(p.nth, u.N1, u.w, u.beta, c1);
+    es_corr(p.nph, u.N2, u.w, u.beta, c2);
+    const double shift = p.geom.kind == kInterior ? 0.0 : kPi / double(p.nth);
+    const double Nn = double(p.nth) * double(p.nph);
+    std::vector<double2> pc1(p.nth);
+    for (int i = 0; i < p.nth; ++i) {
+        const double ang = -double(mode_k(i, p.nth)) * shift;
+        pc1[i] = make_double2(std::cos(ang) / Nn * c1[i], std::sin(ang) / Nn * c1[i]);
+    }
+    if ((st = upload(&u.X1, X1s, by)) != kOk) return st;
+    if ((st = upload(&u.X2, X2s, by)) != kOk) return st;
+    if ((st = upload(&u.perm, perm, by)) != kOk) return st;
+    if ((st = upload(&u.pc1, pc1, by)) != kOk) return st;
+    if ((st = upload(&u.c2, c2, by)) != kOk) return st;
+    if ((st = upload(&u.st_list, st_list, by)) != kOk) return st;
+    if ((st = upload(&u.st_slot, st_slot, by)) != kOk) return st;
+    if ((st = upload(&u.bin_ofs, bo, by)) != kOk) return st;
+    if ((st = dalloc(&u.Wst, size_t(B) * std::max(u.nst, 1) * u.win * u.win, by)) != kOk) return st;
+    // work buffers
+    if ((st = dalloc(&p.Fp, size_t(B) * p.nh * p.nph, by)) != kOk) return st;
+    if ((st = dalloc(&p.Ft, size_t(B) * p.nth * p.nph, by)) != kOk) return st;
+    if ((st = dalloc(&p.U, size_t(B) * u.N1 * u.N2, by)) != kOk) return st;
+    // rocFFT plans (row-major, contiguous; lengths fastest first)
+    if ((st = fft_make(p.phi_bwd, false, p.nph, 0, size_t(B) * p.nh, by)) != kOk) return st;
+    if ((st = fft_make(p.phi_fwd, true, p.nph, 0, size_t(B) * p.nh, by)) != kOk) return st;
+    if ((st = fft_make(p.grid_fwd, true, p.nph, p.nth, B, by)) != kOk) return st;
+    if ((st = fft_make(p.grid_bwd, false, p.nph, p.nth, B, by)) != kOk) return st;
+    if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;
+    if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
+    PFSHT_HIP_TRY(hipDeviceSynchronize());
+    return kOk;
+}
+
+static GridDims dims_of(const pfsht_plan_s &p) {
+    GridDims g;
+    g.B = p.B; g.nh = p.nh; g.nth = p.nth; g.nph = p.nph; g.mmax = p.mmax; g.L = p.L; g.kind = p.geom.kind;
+    return g;
+}
+
+// stage timing: events recorded on the apply stream, drained by pfsht_stage_times
+static void tmark(pfsht_plan_s &p, TimedApply *t, int k, hipStream_t s) {
+    if (t) (void)hipEventRecord(t->ev[k], s);
+}
+
+static TimedApply *timing_begin(pfsht_plan_s &p, int dir) {
+    if (!p.timing) return nullptr;
+    TimedApply t;
+    t.dir = dir;
+    // synthesis runs stages 0..7 in order; the adjoint runs them transposed, 7..0
+    for (int k = 0; k < kStages; ++k) t.stage_of[k] = dir == 0 ? k : kStages - 1 - k;
+    for (int k = 0; k <= kStages; ++k) {
+        if (p.pool.empty()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            p.pool.push_back(e);
+        }
+        t.ev[k] = p.pool.back();
+        p.pool.pop_back();
+    }
+    p.pending.push_back(t);
+    return &p.pending.back();
+}
+
+static void timing_drain(pfsht_plan_s &p) {
+    for (auto &t : p.pending) {
+        (void)hipEventSynchronize(t.ev[kStages]);
+        for (int k = 0; k < kStages; ++k) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]) == hipSuccess) {
+                p.ms[t.dir][t.stage_of[k]] += ms;
+                p.counts[t.dir][t.stage_of[k]] += 1;
+            }
+        }
+        for (auto e : t.ev) p.pool.push_back(e);
+    }
+    p.pending.clear();
+}
+
+static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+    if (p.timing && p.pending.size() >= 256) timing_drain(p);
+    TimedApply *t = timing_begin(p, 0);
+    const GridDims g = dims_of(p);
+    pfsht_rings_s &r = *p.rings;
+    tmark(p, t, 0, s);
+    launch_ring_fwd(r.d, alm, r.Gd, s);
+    tmark(p, t, 1, s);
+    launch_pack_window(g, r.d, r.Gd, p.wth, p.Fp, s);
+    tmark(p, t, 2, s);
+    Status st = fft_exec(p.phi_bwd, p.Fp, s);
+    if (st != kOk) return st;
+    tmark(p, t, 3, s);
+    double2 *Ft = p.Ft;
+    if (p.geom.kind == kInterior) Ft = p.Fp;
+    else launch_dfs_double(g, p.Fp, p.Ft, s);
+    tmark(p, t, 4, s);
+    if ((st = fft_exec(p.grid_fwd, Ft, s)) != kOk) return st;
+    tmark(p, t, 5, s);
+    launch_deconv_pad(p.nu, Ft, p.U, s);
+    tmark(p, t, 6, s);
+    if ((st = fft_exec(p.fine_bwd, p.U, s)) != kOk) return st;
+    tmark(p, t, 7, s);
+    launch_interp(p.nu, p.U, f, s);
+    tmark(p, t, 8, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+    if (p.timing && p.pending.size() >= 256) timing_drain(p);
+    TimedApply *t = timing_begin(p, 1);
+    const GridDims g = dims_of(p);
+    pfsht_rings_s &r = *p.rings;
+    tmark(p, t, 0, s);
+    launch_spread(p.nu, f, p.U, s);
+    tmark(p, t, 1, s);
+    Status st = fft_exec(p.fine_fwd, p.U, s);
+    if (st != kOk) return st;
+    tmark(p, t, 2, s);
+    double2 *Ft = p.geom.kind == kInterior ? p.Fp : p.Ft;
+    launch_deconv_trunc(p.nu, p.U, Ft, s);
+    tmark(p, t, 3, s);
+    if ((st = fft_exec(p.grid_bwd, Ft, s)) != kOk) return st;
+    tmark(p, t, 4, s);
+    if (p.geom.kind != kInterior) launch_dfs_fold(g, p.Ft, p.Fp, s);
+    tmark(p, t, 5, s);
+    if ((st = fft_exec(p.phi_fwd, p.Fp, s)) != kOk) return st;
+    tmark(p, t, 6, s);
+    launch_unpack_window(g, p.Fp, p.wth, r.Gd, s);
+    tmark(p, t, 7, s);
+    launch_ring_adj(r.d, r.Gd, alm, s);
+    tmark(p, t, 8, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *pfsht_last_error(void) { return g_error.c_str(); }
+const char *pfsht_version(void) { return "pfsht-hip 0.1.0 (gfx950)"; }
+
+int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol, int32_t ntrans,
+                      int32_t method, int32_t precision, pfsht_plan_t *out) {
+    if (!out) { set_error("out is NULL"); return kEArg; }
+    *out = nullptr;
+    if (!theta || !phi) { set_error("theta/phi is NULL"); return kEArg; }
+    if (precision != PFSHT_PREC_FP64) {
+        set_error("only PFSHT_PREC_FP64 is available in this build");
+        return kEUnsupported;
+    }
+    auto *p = new (std::nothrow) pfsht_plan_s;
+    if (!p) { set_error("host allocation failed"); return kENomem; }
+    p->precision = precision;
+    Status st;
+    try {
+        st = plan_build(*p, theta, phi, npts, lmax, tol, ntrans, method);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        st = kENomem;
+    }
+    if (st != kOk) {
+        delete p;
+        return st;
+    }
+    *out = p;
+    return kOk;
+}
+
+void pfsht_plan_destroy(pfsht_plan_t p) {
+    if (!p) return;
+    DeviceGuard dg(p->dev);
+    (void)hipDeviceSynchronize();
+    delete p;
+}
+
+int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
+    if (!p || !info) { set_error("NULL argument"); return kEArg; }
+    std::memset(info, 0, sizeof *info);
+    info->method = p->method;
+    info->kind = p->geom.kind;
+    info->flip = p->flip;
+    info->lmax = p->L;
+    info->ntrans = p->B;
+    info->precision = p->precision;
+    info->nth_half = p->nh;
+    info->nth = p->nth;
+    info->nph = p->nph;
+    info->mmax = p->mmax;
+    info->es_w = p->nu.w;
+    info->es_n1 = p->nu.N1;
+    info->es_n2 = p->nu.N2;
+    info->npts = p->M;
+    info->ring_work = p->rings ? p->rings->work : 0;
+    info->device_bytes = p->bytes + (p->rings ? p->rings->bytes : 0);
+    info->tol = p->tol;
+    info->es_beta = p->nu.beta;
+    return kOk;
+}
+
+static Status stage_in(void **stage, size_t bytes, const void *src, hipStream_t s, int64_t &acct) {
+    if (!*stage) {
+        Status st = dalloc(reinterpret_cast<char **>(stage), bytes, acct);
+        if (st != kOk) return st;
+    }
+    if (src) PFSHT_HIP_TRY(hipMemcpyAsync(*stage, src, bytes, hipMemcpyHostToDevice, s));
+    return kOk;
+}
+
+int pfsht_synth(pfsht_plan_t p, const void *alm, void *f, int32_t where, void *stream) {
+    if (!p || !alm || !f) { set_error("NULL argument"); return kEArg; }
+    if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t alm_bytes = sizeof(double2) * size_t(p->L + 1) * (2 * p->L + 1) * p->B;
+    const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;
+    if (where == PFSHT_WHERE_DEVICE) return synth_impl(*p, static_cast<const double2 *>(alm), static_cast<double2 *>(f), s);
+    Status st;
+    if ((st = stage_in(reinterpret_cast<void **>(&p->alm_stage), alm_bytes, alm, s, p->bytes)) != kOk) return st;
+    if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), f_bytes, nullptr, s, p->bytes)) != kOk) return st;
+    if ((st = synth_impl(*p, p->alm_stage, p->f_stage, s)) != kOk) return st;
+    PFSHT_HIP_TRY(hipMemcpyAsync(f, p->f_stage, f_bytes, hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void *stream) {
+    if (!p || !alm || !f) { set_error("NULL argument"); return kEArg; }
+    if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t alm_bytes = sizeof(double2) * size_t(p->L + 1) * (2 * p->L + 1) * p->B;
+    const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;
+    if (where == PFSHT_WHERE_DEVICE)
+        return adjoint_impl(*p, static_cast<const double2 *>(f), static_cast<double2 *>(alm), s);
+    Status st;
+    if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), f_bytes, f, s, p->bytes)) != kOk) return st;
+    if ((st = stage_in(reinterpret_cast<void **>(&p->alm_stage), alm_bytes, nullptr, s, p->bytes)) != kOk) return st;
+    if ((st = adjoint_impl(*p, p->f_stage, p->alm_stage, s)) != kOk) return st;
+    PFSHT_HIP_TRY(hipMemcpyAsync(alm, p->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int pfsht_rings_create(const double *theta, int32_t nrings, int32_t lmax, double eps, int32_t ntrans, int32_t reflect,
+                       pfsht_rings_t *out) {
+    if (!out || !theta) { set_error("NULL argument"); return kEArg; }
+    *out = nullptr;
+    auto *r = new (std::nothrow) pfsht_rings_s;
+    if (!r) { set_error("host allocation failed"); return kENomem; }
+    Status st = rings_build(theta, nrings, lmax, eps, ntrans, reflect != 0, *r);
+    if (st == kOk) st = hipDeviceSynchronize() == hipSuccess ? kOk : kEHip;
+    if (st != kOk) { delete r; return st; }
+    *out = r;
+    return kOk;
+}
+
+void pfsht_rings_destroy(pfsht_rings_t r) {
+    if (!r) return;
+    DeviceGuard dg(r->dev);
+    (void)hipDeviceSynchronize();
+    delete r;
+}
+
+int64_t pfsht_rings_work(pfsht_rings_t r) { return r ? r->work : -1; }
+int32_t pfsht_rings_mmax(pfsht_rings_t r) { return r ? r->d.mmax : -1; }
+
+int pfsht_plan_rings(pfsht_plan_t p, pfsht_rings_t *out) {
+    if (!p || !out) { set_error("NULL argument"); return kEArg; }
+    *out = p->rings;
+    return kOk;
+}
+
+int pfsht_rings_from_alm(pfsht_rings_t r, const void *alm, void *G, int32_t where, void *stream) {
+    if (!r || !alm || !G) { set_error("NULL argument"); return kEArg; }
+    std::lock_guard<std::mutex> lk(r->mu);
+    DeviceGuard dg(r->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t alm_bytes = sizeof(double2) * size_t(r->d.L + 1) * (2 * r->d.L + 1) * r->d.B;
+    const size_t G_bytes = sizeof(double2) * size_t(r->d.n) * (2 * r->d.L + 1) * r->d.B;
+    const double2 *a = static_cast<const double2 *>(alm);
+    double2 *Gout = static_cast<double2 *>(G);
+    Status st;
+    if (where == PFSHT_WHERE_HOST) {
+        if ((st = stage_in(reinterpret_cast<void **>(&r->alm_stage), alm_bytes, alm, s, r->bytes)) != kOk) return st;
+        if ((st = stage_in(reinterpret_cast<void **>(&r->G_stage), G_bytes, nullptr, s, r->bytes)) != kOk) return st;
+        a = r->alm_stage;
+        Gout = r->G_stage;
+    }
+    launch_ring_fwd(r->d, a, r->Gd, s);
+    launch_gd_to_ref(r->d, r->Gd, Gout, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    if (where == PFSHT_WHERE_HOST) {
+        PFSHT_HIP_TRY(hipMemcpyAsync(G, r->G_stage, G_bytes, hipMemcpyDeviceToHost, s));
+        PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
+int pfsht_alm_from_rings(pfsht_rings_t r, const void *G, void *alm, int32_t where, void *stream) {
+    if (!r || !alm || !G) { set_error("NULL argument"); return kEArg; }
+    std::lock_guard<std::mutex> lk(r->mu);
+    DeviceGuard dg(r->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t alm_bytes = sizeof(double2) * size_t(r->d.L + 1) * (2 * r->d.L + 1) * r->d.B;
+    const size_t G_bytes = sizeof(double2) * size_t(r->d.n) * (2 * r->d.L + 1) * r->d.B;
+    const double2 *Gin = static_cast<const double2 *>(G);
+    double2 *aout = static_cast<double2 *>(alm);
+    Status st;
+    if (where == PFSHT_WHERE_HOST) {
+        if ((st = stage_in(reinterpret_cast<void **>(&r->G_stage), G_bytes, G, s, r->bytes)) != kOk) return st;
+        if ((st = stage_in(reinterpret_cast<void **>(&r->alm_stage), alm_bytes, nullptr, s, r->bytes)) != kOk) return st;
+        Gin = r->G_stage;
+        aout = r->alm_stage;
+    }
+    launch_ref_to_gd(r->d, Gin, r->Gd, s);
+    launch_ring_adj(r->d, r->Gd, aout, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    if (where == PFSHT_WHERE_HOST) {
+        PFSHT_HIP_TRY(hipMemcpyAsync(alm, r->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
+        PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
+int pfsht_set_timing(pfsht_plan_t p, int32_t enable) {
+    if (!p) { set_error("NULL plan"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->timing = enable != 0;
+    return kOk;
+}
+
+int pfsht_stage_times(pfsht_plan_t p, int32_t dir, double *ms, int64_t *counts, int32_t reset) {
+    if (!p || dir < 0 || dir > 1) { set_error("bad argument"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    timing_drain(*p);
+    for (int k = 0; k < kStages; ++k) {
+        if (ms) ms[k] = p->ms[dir][k];
+        if (counts) counts[k] = p->counts[dir][k];
+    }
+    if (reset) {
+        std::memset(p->ms[dir], 0, sizeof p->ms[dir]);
+        std::memset(p->counts[dir], 0, sizeof p->counts[dir]);
+    }
+    return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,263 @@
+// nufft_kernels.hip — the exponential-of-semicircle NUFFT on the stretched torus.
+//
+// Synthesis needs a type-2 NUFFT with iflag +1 (src/Stretch.jl:274; definition src/NUFFT.jl:155-176),
+// the adjoint a type-1 NUFFT with iflag -1 (src/Stretch.jl:286; src/NUFFT.jl:177-194). With kernel
+// psi(x) = phi(x / alpha), phi(z) = exp(beta (sqrt(1 - z^2) - 1)), alpha = w h / 2, h = 2 pi / N:
+//   type 2:  u = IFFT_N( c_k * h/psihat(k) zero-padded ),  f_j = sum_{w x w cells} psi psi u;
+//   type 1:  exactly the transpose — spread, FFT_N, multiply by h/psihat(k) and truncate.
+// Interpolation and spreading evaluate the kernel with the same expression on the same cells,
+// so ⟨type2 c, f⟩ = ⟨c, type1 f⟩ holds to round-off.
+//
+// Points are bin-sorted at plan time. Spreading is deterministic without atomics: a workgroup owns
+// a super-tile of 4 x 4 bins (S x S base cells each) and accumulates each bin's 16 x 16 footprint
+// window as an FP64 MFMA outer-product sum (v_mfma_f64_16x16x4f64: four points per instruction,
+// A = f_p * psi_row, B = psi_col). Bins of one parity class are 2S apart and their windows cannot
+// overlap, so the four classes are added into the LDS super-tile window one after another, in a
+// fixed order. A merge pass then sums, for every fine-grid cell, the (at most 2 x 2) super-tile
+// windows covering it in a fixed order.
+#include "pfsht_common.hpp"
+
+namespace pfsht {
+
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double es_kernel(double z, double beta) {
+    const double zz = 1.0 - z * z;
+    return zz >= 0.0 ? exp(beta * (sqrt(zz) - 1.0)) : 0.0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Deconvolution / padding
+// ------------------------------------------------------------------------------------------------
+__global__ void deconv_pad_kernel(const double2 *__restrict__ Fhat, const double2 *__restrict__ pc1,
+                                  const double *__restrict__ c2, double2 *__restrict__ U, int n1, int n2, int N1,
+                                  int N2) {
+    const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l1 = blockIdx.y;
+    const int b = blockIdx.z;
+    if (l2 >= N2) return;
+    int i = -1, k = -1;
+    if (l1 < (n1 + 1) / 2) i = l1;
+    else if (l1 >= N1 - n1 / 2) i = l1 - N1 + n1;
+    if (l2 < (n2 + 1) / 2) k = l2;
+    else if (l2 >= N2 - n2 / 2) k = l2 - N2 + n2;
+    double2 v = make_double2(0.0, 0.0);
+    if (i >= 0 && k >= 0) {
+        const double2 F = Fhat[(int64_t(b) * n1 + i) * n2 + k];
+        const double2 p = pc1[i];
+        const double s = c2[k];
+        v = make_double2((F.x * p.x - F.y * p.y) * s, (F.x * p.y + F.y * p.x) * s);
+    }
+    U[(int64_t(b) * N1 + l1) * N2 + l2] = v;
+}
+
+__global__ void deconv_trunc_kernel(const double2 *__restrict__ U, const double2 *__restrict__ pc1,
+                                    const double *__restrict__ c2, double2 *__restrict__ Fhat, int n1, int n2, int N1,
+                                    int N2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int b = blockIdx.z;
+    if (k >= n2) return;
+    const int l1 = i < (n1 + 1) / 2 ? i : i - n1 + N1;
+    const int l2 = k < (n2 + 1) / 2 ? k : k - n2 + N2;
+    const double2 u = U[(int64_t(b) * N1 + l1) * N2 + l2];
+    const double2 p = pc1[i];  // conj(p) for the adjoint
+    const double s = c2[k];
+    Fhat[(int64_t(b) * n1 + i) * n2 + k] = make_double2((u.x * p.x + u.y * p.y) * s, (u.y * p.x - u.x * p.y) * s);
+}
+
+void launch_deconv_pad(const NufftDev &u, const double2 *Fhat, double2 *U, hipStream_t s) {
+    dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
+    deconv_pad_kernel<<<grid, 256, 0, s>>>(Fhat, u.pc1, u.c2, U, u.n1, u.n2, u.N1, u.N2);
+}
+
+void launch_deconv_trunc(const NufftDev &u, const double2 *U, double2 *Fhat, hipStream_t s) {
+    dim3 grid((u.n2 + 255) / 256, u.n1, u.B);
+    deconv_trunc_kernel<<<grid, 256, 0, s>>>(U, u.pc1, u.c2, Fhat, u.n1, u.n2, u.N1, u.N2);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Interpolation (type 2): one thread per bin-sorted point, w x w gather.
+// ------------------------------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(256)
+interp_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
+              const double2 *__restrict__ U, double2 *__restrict__ f, int64_t M, int N1, int N2, double beta) {
+    const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+    if (p >= M) return;
+    const int b = blockIdx.y;
+    const double x1 = X1[p], x2 = X2[p];
+    const int b1 = int(ceil(x1 - 0.5 * W));
+    const int b2 = int(ceil(x2 - 0.5 * W));
+    const double inv = 2.0 / W;
+    double k1[W], k2[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        k1[t] = es_kernel((double(b1 + t) - x1) * inv, beta);
+        k2[t] = es_kernel((double(b2 + t) - x2) * inv, beta);
+    }
+    int c2i[W];
+#pragma unroll
+    for (int t = 0; t < W; ++t) {
+        const int c = b2 + t;
+        c2i[t] = c < N2 ? c : c - N2;
+    }
+    const double2 *__restrict__ Ub = U + int64_t(b) * N1 * N2;
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int t1 = 0; t1 < W; ++t1) {
+        const int r = b1 + t1;
+        const double2 *__restrict__ row = Ub + int64_t(r < N1 ? r : r - N1) * N2;
+        double rr = 0.0, ri = 0.0;
+#pragma unroll
+        for (int t2 = 0; t2 < W; ++t2) {
+            const double2 v = row[c2i[t2]];
+            rr = fma(k2[t2], v.x, rr);
+            ri = fma(k2[t2], v.y, ri);
+        }
+        ar = fma(k1[t1], rr, ar);
+        ai = fma(k1[t1], ri, ai);
+    }
+    f[int64_t(perm[p]) + int64_t(b) * M] = make_double2(ar, ai);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Spreading (type 1): MFMA super-tiles + ordered merge.
+// ------------------------------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(256)
+spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
+                 const double2 *__restrict__ f, const int32_t *__restrict__ st_list,
+                 const int32_t *__restrict__ bin_ofs, double2 *__restrict__ Wst, int64_t M, int S, int ns2, int win,
+                 int nst, double beta) {
+    extern __shared__ double2 lw[];
+    const int slot = blockIdx.x;
+    const int b = blockIdx.y;
+    const int st = st_list[slot];
+    const int x0 = (st / ns2) * 4 * S, y0 = (st % ns2) * 4 * S;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ii = lane & 15, kq = lane >> 4;
+    const double inv = 2.0 / W;
+    for (int q = tid; q < win * win; q += 256) lw[q] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int phase = 0; phase < 4; ++phase) {
+        const int bi = (phase >> 1) + 2 * (wave >> 1);
+        const int bj = (phase & 1) + 2 * (wave & 1);
+        const int lb = bi * 4 + bj;
+        const int start = bin_ofs[slot * 16 + lb], end = bin_ofs[slot * 16 + lb + 1];
+        const int bx0 = x0 + bi * S, by0 = y0 + bj * S;
+        double4v dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
+        for (int p0 = start; p0 < end; p0 += 4) {
+            const int p = p0 + kq;
+            double are = 0.0, aim = 0.0, c = 0.0;
+            if (p < end) {
+                const double x1 = X1[p], x2 = X2[p];
+                const double2 fv = f[int64_t(perm[p]) + int64_t(b) * M];
+                const int b1 = int(ceil(x1 - 0.5 * W));
+                const int b2 = int(ceil(x2 - 0.5 * W));
+                const int t1 = bx0 + ii - b1;
+                const int t2 = by0 + ii - b2;
+                if (t1 >= 0 && t1 < W) {
+                    const double a = es_kernel((double(b1 + t1) - x1) * inv, beta);
+                    are = a * fv.x;
+                    aim = a * fv.y;
+                }
+                if (t2 >= 0 && t2 < W) c = es_kernel((double(b2 + t2) - x2) * inv, beta);
+            }
+            dr = __builtin_amdgcn_mfma_f64_16x16x4f64(are, c, dr, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di, 0, 0, 0);
+        }
+        // D[i][j]: lane holds column j = lane % 16 and rows i = 4 * (lane / 16) + r
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = bi * S + 4 * kq + r;
+            const int col = bj * S + ii;
+            double2 &cell = lw[row * win + col];
+            cell.x += dr[r];
+            cell.y += di[r];
+        }
+        __syncthreads();
+    }
+    double2 *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
+    for (int q = tid; q < win * win; q += 256) out[q] = lw[q];
+}
+
+__global__ void spread_merge_kernel(const double2 *__restrict__ Wst, const int32_t *__restrict__ st_slot,
+                                    double2 *__restrict__ U, int N1, int N2, int S4, int ns1, int ns2, int win, int w,
+                                    int nst) {
+    const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l1 = blockIdx.y;
+    const int b = blockIdx.z;
+    if (l2 >= N2) return;
+    const int q1 = l1 / S4, q2 = l2 / S4;
+    double sr = 0.0, si = 0.0;
+    for (int d1 = 0; d1 < 3 && d1 < ns1; ++d1) {
+        const int s1 = ((q1 - d1) % ns1 + ns1) % ns1;
+        const int wh1 = min(S4, N1 - s1 * S4) + w - 1;
+        int o1 = (l1 - s1 * S4) % N1;
+        if (o1 < 0) o1 += N1;
+        for (; o1 < wh1; o1 += N1) {
+            for (int d2 = 0; d2 < 3 && d2 < ns2; ++d2) {
+                const int s2 = ((q2 - d2) % ns2 + ns2) % ns2;
+                const int slot = st_slot[s1 * ns2 + s2];
+                if (slot < 0) continue;
+                const int wh2 = min(S4, N2 - s2 * S4) + w - 1;
+                int o2 = (l2 - s2 * S4) % N2;
+                if (o2 < 0) o2 += N2;
+                for (; o2 < wh2; o2 += N2) {
+                    const double2 v = Wst[((int64_t(b) * nst + slot) * win + o1) * win + o2];
+                    sr += v.x;
+                    si += v.y;
+                }
+            }
+        }
+    }
+    U[(int64_t(b) * N1 + l1) * N2 + l2] = make_double2(sr, si);
+}
+
+template <int W>
+static void interp_w(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s) {
+    dim3 grid(unsigned((u.M + 255) / 256), u.B);
+    interp_kernel<W><<<grid, 256, 0, s>>>(u.X1, u.X2, u.perm, U, f, u.M, u.N1, u.N2, u.beta);
+}
+
+template <int W>
+static void spread_w(const NufftDev &u, const double2 *f, hipStream_t s) {
+    if (u.nst == 0) return;
+    dim3 grid(u.nst, u.B);
+    const size_t lds = sizeof(double2) * size_t(u.win) * u.win;
+    spread_st_kernel<W><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, u.Wst, u.M, u.S, u.ns2,
+                                               u.win, u.nst, u.beta);
+}
+
+#define PFSHT_W_SWITCH(w, FN, ...)                 \
+    switch (w) {                                   \
+        case 2: FN<2>(__VA_ARGS__); break;         \
+        case 3: FN<3>(__VA_ARGS__); break;         \
+        case 4: FN<4>(__VA_ARGS__); break;         \
+        case 5: FN<5>(__VA_ARGS__); break;         \
+        case 6: FN<6>(__VA_ARGS__); break;         \
+        case 7: FN<7>(__VA_ARGS__); break;         \
+        case 8: FN<8>(__VA_ARGS__); break;         \
+        case 9: FN<9>(__VA_ARGS__); break;         \
+        case 10: FN<10>(__VA_ARGS__); break;       \
+        case 11: FN<11>(__VA_ARGS__); break;       \
+        case 12: FN<12>(__VA_ARGS__); break;       \
+        case 13: FN<13>(__VA_ARGS__); break;       \
+        case 14: FN<14>(__VA_ARGS__); break;       \
+        case 15: FN<15>(__VA_ARGS__); break;       \
+        default: FN<16>(__VA_ARGS__); break;       \
+    }
+
+void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s) {
+    PFSHT_W_SWITCH(u.w, interp_w, u, U, f, s);
+}
+
+void launch_spread(const NufftDev &u, const double2 *f, double2 *U, hipStream_t s) {
+    PFSHT_W_SWITCH(u.w, spread_w, u, f, s);
+    dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
+    spread_merge_kernel<<<grid, 256, 0, s>>>(u.Wst, u.st_slot, U, u.N1, u.N2, 4 * u.S, u.ns1, u.ns2, u.win, u.w,
+                                             u.nst);
+}
+
+}  // namespace pfsht

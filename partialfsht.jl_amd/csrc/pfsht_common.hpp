@@ -1,0 +1,92 @@
+// pfsht_common.hpp — shared declarations of libpfsht_hip: device table structs, kernel launchers,
+// error plumbing. gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace pfsht {
+
+void set_error(const std::string &msg);
+
+// Status propagated through the C ABI (values of the PFSHT_E_* codes in include/pfsht.h).
+enum Status { kOk = 0, kEDim = 1, kEArg = 2, kEGeom = 3, kENomem = 4, kEHip = 5, kEUnsupported = 6 };
+
+#define PFSHT_HIP_TRY(expr)                                                                       \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess) {                                                                   \
+            ::pfsht::set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr); \
+            return ::pfsht::kEHip;                                                                \
+        }                                                                                         \
+    } while (0)
+
+// Ring-pass work item: one wavefront of up to 64 consecutive retained rings of order m.
+struct RingItem {
+    int32_t m;      // order
+    int32_t j0;     // first ring of the tile
+    int32_t nr;     // rings in the tile (1..64)
+    int32_t tile;   // tile index within order m
+};
+
+// Device-resident tables of a ring engine (RingLegendre, src/Legendre.jl:174-255).
+struct RingDev {
+    int n = 0, L = 0, B = 1, mmax = 0;
+    double *costh = nullptr;    // [n] cos theta, sign-flipped when reflected
+    double *log2sin = nullptr;  // [n]
+    double *log2c = nullptr;    // [L+1] log2 sectorial prefactor
+    double2 *ab = nullptr;      // recurrence coefficients (a_l^m, b_l^m), l = m+2..L, per m
+    int64_t *abofs = nullptr;   // [mmax+1] offset of order m in ab
+    int32_t *jlo = nullptr, *jhi = nullptr;  // [mmax+1]
+    RingItem *items = nullptr;  // work items, descending cost
+    int nitems = 0;
+    int64_t *pofs = nullptr;    // [mmax+1] offset of order m's adjoint partials (in double4 units)
+    int32_t *ntiles = nullptr;  // [mmax+1]
+    double4 *partials = nullptr;  // adjoint per-tile partial sums: (re+, im+, re-, im-) per (tile, l)
+    int64_t npartials = 0;
+};
+
+// Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
+// Gd is the internal ring buffer [b][m + mmax][j].
+void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s);
+void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s);
+void launch_gd_to_ref(const RingDev &r, const double2 *Gd, double2 *G, hipStream_t s);
+void launch_ref_to_gd(const RingDev &r, const double2 *G, double2 *Gd, hipStream_t s);
+
+struct GridDims {
+    int B, nh, nth, nph, mmax, L, kind;
+};
+void launch_pack_window(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, double2 *Fp,
+                        hipStream_t s);
+void launch_unpack_window(const GridDims &g, const double2 *Fp, const double *wth, double2 *Gd, hipStream_t s);
+void launch_dfs_double(const GridDims &g, const double2 *Fp, double2 *Ft, hipStream_t s);
+void launch_dfs_fold(const GridDims &g, const double2 *Ft, double2 *Fp, hipStream_t s);
+
+// ES NUFFT on the stretched torus.
+struct NufftDev {
+    int B = 1, n1 = 0, n2 = 0;     // modes (nth, nph)
+    int N1 = 0, N2 = 0, w = 0;     // fine grid and kernel width
+    double beta = 0;
+    int64_t M = 0;
+    double *X1 = nullptr, *X2 = nullptr;  // sorted points in fine-grid units, base cell in [0, N)
+    int32_t *perm = nullptr;              // sorted position -> user index
+    double2 *pc1 = nullptr;               // [n1] phase * h1/psihat1 (synthesis; adjoint uses conj)
+    double *c2 = nullptr;                 // [n2] h2/psihat2
+    // spreading super-tiles
+    int S = 0, ns1 = 0, ns2 = 0, win = 0;  // bin size, super-tile grid, LDS window edge
+    int nst = 0;                          // non-empty super-tiles
+    int32_t *st_list = nullptr;           // [nst] super-tile ids (s1 * ns2 + s2)
+    int32_t *st_slot = nullptr;           // [ns1*ns2] slot in Wst or -1 when empty
+    int32_t *bin_ofs = nullptr;           // [nst*16 + 1] point ranges per (super-tile, local bin)
+    double2 *Wst = nullptr;               // [B][nst][win][win] super-tile windows
+};
+void launch_deconv_pad(const NufftDev &u, const double2 *Fhat, double2 *U, hipStream_t s);
+void launch_deconv_trunc(const NufftDev &u, const double2 *U, double2 *Fhat, hipStream_t s);
+void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s);
+void launch_spread(const NufftDev &u, const double2 *f, double2 *U, hipStream_t s);
+
+}  // namespace pfsht

@@ -1,0 +1,382 @@
+"""pfsht — Python host-side mirror of PartialFSHT.jl's operator API over libpfsht_hip.so.
+
+    plan = Plan(theta, phi, lmax=..., tol=1e-10, ntrans=1, method="auto")
+    mul(f, plan, alm)          # mul!(f, plan, alm)        src/Plan.jl:287-291
+    mul(alm, plan.H, f)        # mul!(alm, plan', f)       src/Plan.jl:293-298
+    f = plan @ alm ; alm = plan.H @ f                      # Base.:* (src/Plan.jl:340-349)
+
+Arrays are the reference's column-major layouts (src/Layout.jl:13-56). Host arrays are numpy
+complex128 (Fortran order for alm); device arrays are torch complex128 CUDA tensors whose memory
+is the column-major layout (the helpers `alm_device`/`field_device` make such tensors). Shape
+errors raise ValueError (DimensionMismatch), bad arguments ValueError (ArgumentError). Every call
+goes to the HIP library; there is no CPU fallback, and importing raises when the library is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["Plan", "AdjointPlan", "Rings", "mul", "coeff_size", "sph_mode", "zeros_alm",
+           "PFSHTError", "DimensionMismatch", "ArgumentError", "lib", "library_path"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(os.path.dirname(_HERE), "lib", "libpfsht_hip.so")
+
+
+class PFSHTError(RuntimeError):
+    pass
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class ArgumentError(ValueError):
+    pass
+
+
+class GeometryRefused(ArgumentError):
+    pass
+
+
+class InfoC(C.Structure):
+    _fields_ = [("method", C.c_int32), ("kind", C.c_int32), ("flip", C.c_int32),
+                ("lmax", C.c_int32), ("ntrans", C.c_int32), ("precision", C.c_int32),
+                ("nth_half", C.c_int32), ("nth", C.c_int32), ("nph", C.c_int32),
+                ("mmax", C.c_int32), ("es_w", C.c_int32), ("es_n1", C.c_int32),
+                ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
+                ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double)]
+
+
+NSTAGES = 8
+STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft"]
+METHODS = {"auto": 0, "direct": 1, "stretch": 2}
+KINDS = {0: "pole", 1: "interior", 2: "full"}
+WHERE_HOST, WHERE_DEVICE = 0, 1
+
+EXPORTS = {
+    "pfsht_last_error": (C.c_char_p, []),
+    "pfsht_version": (C.c_char_p, []),
+    "pfsht_plan_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double,
+                                    C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_plan_destroy": (None, [C.c_void_p]),
+    "pfsht_plan_info": (C.c_int, [C.c_void_p, C.POINTER(InfoC)]),
+    "pfsht_synth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_rings_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                     C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_rings_destroy": (None, [C.c_void_p]),
+    "pfsht_rings_work": (C.c_int64, [C.c_void_p]),
+    "pfsht_rings_mmax": (C.c_int32, [C.c_void_p]),
+    "pfsht_rings_from_alm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_alm_from_rings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_plan_rings": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pfsht_stage_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return _LIB
+
+
+def lib():
+    """Load libpfsht_hip.so. Raises when it has not been built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise ImportError(f"{_LIB} is missing: build it with partialfsht.jl_amd/build.py")
+        L = C.CDLL(_LIB)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc == 0:
+        return
+    msg = lib().pfsht_last_error().decode()
+    if rc == 1:
+        raise DimensionMismatch(msg)
+    if rc == 3:
+        raise GeometryRefused(msg)
+    if rc in (2, 6):
+        raise ArgumentError(msg)
+    raise PFSHTError(f"error {rc}: {msg}")
+
+
+def coeff_size(lmax: int):
+    """(lmax+1, 2lmax+1) — src/Layout.jl:13."""
+    return (int(lmax) + 1, 2 * int(lmax) + 1)
+
+
+def sph_mode(l: int, m: int):
+    """0-based (row, column) of (l, m) — src/Layout.jl:20."""
+    am = abs(int(m))
+    return (int(l) - am, 2 * am + (1 if m >= 0 else 0) - 1)
+
+
+def zeros_alm(lmax: int, ntrans: int = 1):
+    shape = coeff_size(lmax) if ntrans == 1 else (*coeff_size(lmax), ntrans)
+    return np.zeros(shape, np.complex128, order="F")
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _buf(x, count: int, what: str):
+    """(pointer, where) of a complex128 buffer of `count` elements in column-major order."""
+    if _is_torch(x):
+        import torch
+        if x.dtype != torch.complex128:
+            raise ArgumentError(f"{what} must be complex128")
+        if not x.is_contiguous():
+            raise ArgumentError(f"{what} must be contiguous (column-major data in a contiguous tensor)")
+        if x.numel() != count:
+            raise DimensionMismatch(f"{what} has {x.numel()} elements, expected {count}")
+        return C.c_void_p(x.data_ptr()), (WHERE_DEVICE if x.is_cuda else WHERE_HOST)
+    if not isinstance(x, np.ndarray) or x.dtype != np.complex128:
+        raise ArgumentError(f"{what} must be a complex128 numpy array or torch tensor")
+    if x.size != count:
+        raise DimensionMismatch(f"{what} has shape {x.shape}, expected {count} elements")
+    if not (x.flags.f_contiguous or x.ndim <= 1):
+        raise ArgumentError(f"{what} must be Fortran-ordered (column-major)")
+    return C.c_void_p(x.ctypes.data), WHERE_HOST
+
+
+def _assert_coeffs(alm, L: int, B: int):
+    """src/Layout.jl:35-45."""
+    shp = tuple(alm.shape)
+    want = coeff_size(L)
+    if B == 1 and len(shp) == 2:
+        ok = shp == want
+    else:
+        ok = len(shp) == 3 and shp == (*want, B)
+    if _is_torch(alm) and alm.dim() == 1:
+        ok = alm.numel() == want[0] * want[1] * B
+    if not ok:
+        raise DimensionMismatch(f"coefficient array is {shp}, expected {(*want, B) if B > 1 else want}")
+
+
+def _assert_field(f, M: int, B: int):
+    """src/Layout.jl:47-56."""
+    shp = tuple(f.shape)
+    if B == 1 and len(shp) == 1:
+        ok = shp[0] == M
+    else:
+        ok = len(shp) == 2 and shp == (M, B)
+    if _is_torch(f) and f.dim() == 1:
+        ok = f.numel() == M * B
+    if not ok:
+        raise DimensionMismatch(f"field is {shp}, expected {(M, B) if B > 1 else (M,)}")
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except ImportError:
+            pass
+        return C.c_void_p(0)
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+class Plan:
+    """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:234-266."""
+
+    def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
+                 method: str = "auto", precision: str = "fp64"):
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
+        ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
+        if th.shape != ph.shape or th.ndim != 1:
+            raise DimensionMismatch("theta and phi must have the same length")
+        if lmax < 0:
+            raise ArgumentError("lmax must be nonnegative")
+        if ntrans < 1:
+            raise ArgumentError("ntrans must be >= 1")
+        self.lmax, self.ntrans, self.tol, self.npts = int(lmax), int(ntrans), float(tol), len(th)
+        h = C.c_void_p()
+        _check(lib().pfsht_plan_create(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
+                                       len(th), self.lmax, self.tol, self.ntrans, METHODS[method],
+                                       {"fp64": 0, "mixed": 1}[precision], C.byref(h)))
+        self._h = h
+        inf = InfoC()
+        _check(lib().pfsht_plan_info(self._h, C.byref(inf)))
+        self.info = {f: getattr(inf, f) for f, _ in InfoC._fields_}
+        self.info["kind_name"] = KINDS.get(inf.kind, "direct")
+        self.theta, self.phi = th, ph
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pfsht_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, "
+                f"algorithm=StretchAlgorithm, kind={self.info['kind_name']})")
+
+    @property
+    def H(self) -> "AdjointPlan":
+        """plan' — src/Plan.jl:172."""
+        return AdjointPlan(self)
+
+    @property
+    def ring_work(self) -> int:
+        return int(self.info["ring_work"])
+
+    def synth(self, f, alm, stream=None):
+        """mul!(f, plan, alm): overwrites f."""
+        L, B, M = self.lmax, self.ntrans, self.npts
+        _assert_coeffs(alm, L, B)
+        _assert_field(f, M, B)
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        pf, wf = _buf(f, M * B, "f")
+        if wa != wf:
+            raise ArgumentError("alm and f must both be host or both be device arrays")
+        _check(lib().pfsht_synth(self._h, pa, pf, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return f
+
+    def adjoint(self, alm, f, stream=None):
+        """mul!(alm, plan', f): overwrites alm, padding included."""
+        L, B, M = self.lmax, self.ntrans, self.npts
+        _assert_coeffs(alm, L, B)
+        _assert_field(f, M, B)
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        pf, wf = _buf(f, M * B, "f")
+        if wa != wf:
+            raise ArgumentError("alm and f must both be host or both be device arrays")
+        _check(lib().pfsht_adjoint(self._h, pf, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return alm
+
+    def __matmul__(self, alm):
+        """plan * alm (src/Plan.jl:340-343): a new field, host or device like alm."""
+        B = self.ntrans
+        if _is_torch(alm):
+            import torch
+            f = torch.empty(self.npts * B, dtype=torch.complex128, device=alm.device)
+            return self.synth(f, alm)
+        f = np.zeros((self.npts,) if B == 1 else (self.npts, B), np.complex128, order="F")
+        return self.synth(f, alm)
+
+    def set_timing(self, on: bool = True):
+        _check(lib().pfsht_set_timing(self._h, 1 if on else 0))
+
+    def stage_times(self, direction: str = "synth", reset: bool = False):
+        ms = np.zeros(NSTAGES)
+        cnt = np.zeros(NSTAGES, np.int64)
+        _check(lib().pfsht_stage_times(self._h, 0 if direction == "synth" else 1,
+                                       ms.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p),
+                                       1 if reset else 0))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(STAGES)}
+
+    def rings(self) -> "Rings":
+        h = C.c_void_p()
+        _check(lib().pfsht_plan_rings(self._h, C.byref(h)))
+        return Rings._borrow(h, self.lmax, self.info["nth_half"], self.ntrans)
+
+
+class AdjointPlan:
+    """AdjointPlan — src/Plan.jl:168-173."""
+
+    def __init__(self, parent: Plan):
+        self.parent = parent
+
+    @property
+    def H(self) -> Plan:
+        return self.parent
+
+    def __matmul__(self, f):
+        p = self.parent
+        alm = zeros_alm(p.lmax, p.ntrans)
+        return p.adjoint(alm, f)
+
+
+def mul(out, op, x, stream=None):
+    """mul!(out, op, x) for op a Plan or an AdjointPlan."""
+    if isinstance(op, Plan):
+        return op.synth(out, x, stream)
+    if isinstance(op, AdjointPlan):
+        return op.parent.adjoint(out, x, stream)
+    raise ArgumentError("op must be a Plan or plan.H")
+
+
+class Rings:
+    """RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255."""
+
+    def __init__(self, theta, lmax: int, eps: float, ntrans: int = 1, reflect: bool = False):
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
+        h = C.c_void_p()
+        _check(lib().pfsht_rings_create(th.ctypes.data_as(C.c_void_p), len(th), int(lmax), float(eps),
+                                        int(ntrans), 1 if reflect else 0, C.byref(h)))
+        self._h, self._owned = h, True
+        self.lmax, self.n, self.ntrans = int(lmax), len(th), int(ntrans)
+
+    @classmethod
+    def _borrow(cls, h, lmax, n, ntrans):
+        r = cls.__new__(cls)
+        r._h, r._owned = h, False
+        r.lmax, r.n, r.ntrans = int(lmax), int(n), int(ntrans)
+        return r
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "_h", None):
+            lib().pfsht_rings_destroy(self._h)
+            self._h = None
+
+    @property
+    def work(self) -> int:
+        return int(lib().pfsht_rings_work(self._h))
+
+    @property
+    def mmax(self) -> int:
+        return int(lib().pfsht_rings_mmax(self._h))
+
+    def rings_from_alm(self, G, alm, stream=None):
+        L, B = self.lmax, self.ntrans
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        pg, wg = _buf(G, self.n * (2 * L + 1) * B, "G")
+        if wa != wg:
+            raise ArgumentError("alm and G must both be host or both be device arrays")
+        _check(lib().pfsht_rings_from_alm(self._h, pa, pg, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return G
+
+    def alm_from_rings(self, alm, G, stream=None):
+        L, B = self.lmax, self.ntrans
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        pg, wg = _buf(G, self.n * (2 * L + 1) * B, "G")
+        if wa != wg:
+            raise ArgumentError("alm and G must both be host or both be device arrays")
+        _check(lib().pfsht_alm_from_rings(self._h, pg, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return alm
+
+
+# --- device helpers: torch tensors holding column-major data ------------------------------------
+def to_device(a: np.ndarray, device="cuda"):
+    """Copy a column-major numpy array into a flat contiguous torch CUDA tensor (same memory order)."""
+    import torch
+    flat = np.asarray(a, dtype=np.complex128).ravel(order="F")
+    return torch.from_numpy(flat.copy()).to(device)
+
+
+def from_device(t, shape) -> np.ndarray:
+    """Flat device tensor -> column-major numpy array of `shape`."""
+    return t.detach().cpu().numpy().reshape(shape, order="F")

@@ -1,0 +1,255 @@
+"""Device parity of libpfsht_hip against the CPU oracle, through the C ABI.
+
+Mirrors gpu/runtests.jl of the reference (ring pass forward/adjoint, end-to-end device vs host,
+adjoint identity on device) and the north-star checks: synthesis and adjoint against the oracle's
+stretched leaf on the same inputs (within tol), against the FP64 Direct oracle (<= 10 tol at
+small sizes, <= 1e-8 at lmax 2048, M = 2e6), the adjoint identity to round-off, and bit-identical
+results run to run. Tolerances are written in each test.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import cap_points, relmax
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as T
+    assert T.cuda.is_available(), "a GPU is required for the gpu-marked tests"
+    print(f"\ndevice: {T.cuda.get_device_name(0)}", flush=True)
+    return T
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pfsht
+    pfsht.lib()
+    return pfsht
+
+
+def dev(torch, a):
+    import pfsht
+    return pfsht.to_device(a)
+
+
+def host(t, shape):
+    import pfsht
+    return pfsht.from_device(t, shape)
+
+
+def valid(O, L):
+    return O.valid_mask(L)
+
+
+def rel_modes(O, a, b, L):
+    m = valid(O, L)
+    a = np.asarray(a).reshape(m.shape + (-1,), order="F")[m]
+    b = np.asarray(b).reshape(m.shape + (-1,), order="F")[m]
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+# --------------------------------------------------------------------------------------------
+# Ring engine (gpu/runtests.jl:42-66; test_legendre.jl:87-98)
+# --------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("L", [128, 512])
+def test_ring_forward_and_adjoint_vs_oracle(torch, P, O, rng, L):
+    th = np.sort(0.02 + 0.5 * rng.uniform(size=64))
+    a = O.rand_alm_fast(L, rng)
+    Gh = O.Rings(th, L, 1e-8).rings_from_alm(a)
+    eng = P.Rings(th, L, 1e-8)
+    assert eng.work == O.Rings(th, L, 1e-8).work and eng.mmax == O.Rings(th, L, 1e-8).mmax
+    Gd = torch.zeros(Gh.size, dtype=torch.complex128, device="cuda")
+    eng.rings_from_alm(Gd, dev(torch, a))
+    torch.cuda.synchronize()
+    G = host(Gd, Gh.shape)
+    # per-ring exponents and FMA contraction versus the host's shared chunk exponent: round-off
+    assert relmax(G, Gh) < 1e-12
+    assert np.all(G[Gh == 0] == 0)
+    Gr = rng.standard_normal(Gh.shape) + 1j * rng.standard_normal(Gh.shape)
+    ah = O.Rings(th, L, 1e-8).alm_from_rings(Gr)
+    ad = torch.zeros(ah.size, dtype=torch.complex128, device="cuda")
+    eng.alm_from_rings(ad, dev(torch, Gr))
+    assert relmax(host(ad, ah.shape), ah) <= 1e-12
+
+
+def test_ring_engine_matches_alf(torch, P, O):
+    L = 48
+    th = np.sort([0.02, 0.11, 0.37, 0.8, 1.4, 2.7])
+    a = np.zeros((L + 1, 2 * L + 1), complex, order="F")
+    a[P.sph_mode(30, 7)] = 1.0
+    G = np.zeros((len(th), 2 * L + 1, 1), complex, order="F")
+    P.Rings(th, L, 1e-13).rings_from_alm(G, a)
+    for j, t in enumerate(th):
+        assert abs(G[j, 7 + L, 0] - O.alf(30, 7, t)) <= 1e-13
+
+
+def test_ring_engine_reflect_and_equator(torch, P, O, rng):
+    L = 256
+    for th, refl in ((np.linspace(0.5, 2.6, 77), False), (np.sort(rng.uniform(0.1, 0.9, 40)), True)):
+        a = O.rand_alm_fast(L, rng)
+        Gh = O.Rings(th, L, 1e-8, reflect=refl).rings_from_alm(a)
+        G = np.zeros(Gh.shape, complex, order="F")
+        P.Rings(th, L, 1e-8, reflect=refl).rings_from_alm(G, a)
+        assert relmax(G, Gh) < 1e-12
+
+
+# --------------------------------------------------------------------------------------------
+# Stretched plan end to end
+# --------------------------------------------------------------------------------------------
+def _plans(P, O, th, ph, L, tol, B=1):
+    pg = P.Plan(th, ph, lmax=L, tol=tol, ntrans=B, method="stretch")
+    po = O.Plan(th, ph, L, tol=tol, ntrans=B, method="stretch", nufft="es", nthreads=NT)
+    for k in ("kind", "flip", "nth_half", "nth", "nph", "mmax", "ring_work", "es_w", "es_n1", "es_n2"):
+        assert pg.info[k] == po.info[k], k
+    return pg, po
+
+
+def test_config1_synth_adjoint_vs_oracle(torch, P, O, rng):
+    # 20 deg cap, lmax 256, M = 4 (L theta*)^2, tol 1e-8, B = 1
+    L, tol = 256, 1e-8
+    M = int(round(4 * (L * np.deg2rad(20)) ** 2))
+    th, ph = cap_points(rng, 20, M)
+    pg, po = _plans(P, O, th, ph, L, tol)
+    a = O.rand_alm_fast(L, rng)
+    fd = torch.zeros(M, dtype=torch.complex128, device="cuda")
+    ad = dev(torch, a)
+    pg.synth(fd, ad)
+    fg = host(fd, (M,))
+    fo = po.synth(a)
+    # same algorithm and kernel parameters as the oracle: agreement far inside tol
+    assert relmax(fg, fo) < 1e-11
+    # against the FP64 Direct oracle on a subset of points (test_transforms.jl:59-82: 10 tol)
+    sub = rng.choice(M, 300, replace=False)
+    pd = O.Plan(th[sub], ph[sub], L, method="direct", nthreads=NT)
+    assert relmax(fg[sub], pd.synth(a)) < 10 * tol
+    # adjoint
+    f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    alm_d = torch.zeros(a.size, dtype=torch.complex128, device="cuda")
+    pg.adjoint(alm_d, dev(torch, f))
+    ag = host(alm_d, a.shape)
+    ao = po.adjoint(f)
+    assert rel_modes(O, ag, ao, L) < 1e-11
+    assert np.all(ag[~O.valid_mask(L)] == 0)  # padding overwritten with zeros
+    # adjoint identity <Y a, f> = <a, Y' f> to round-off (north star ~1e-13; reference gate 1e-12)
+    lhs = np.vdot(f, fg)
+    rhs = np.vdot(ag, a)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+def test_bit_identical_run_to_run(torch, P, O, rng):
+    L, tol = 256, 1e-8
+    th, ph = cap_points(rng, 20, 20000)
+    pg = P.Plan(th, ph, lmax=L, tol=tol)
+    a = dev(torch, O.rand_alm_fast(L, rng))
+    f = dev(torch, rng.standard_normal(20000) + 1j * rng.standard_normal(20000))
+    outs_f, outs_a = [], []
+    for _ in range(3):
+        fo = torch.empty(20000, dtype=torch.complex128, device="cuda")
+        ao = torch.empty(a.numel(), dtype=torch.complex128, device="cuda")
+        pg.synth(fo, a)
+        pg.adjoint(ao, f)
+        outs_f.append(fo)
+        outs_a.append(ao)
+    torch.cuda.synchronize()
+    for k in (1, 2):
+        assert torch.equal(outs_f[0], outs_f[k])
+        assert torch.equal(outs_a[0], outs_a[k])
+
+
+@pytest.mark.parametrize("L,tc,kind,flip", [(256, 0.9, "pole", 0), (256, np.pi - 0.3, "pole", 1),
+                                             (512, 1.2, "interior", 0)])
+def test_off_pole_south_interior(torch, P, O, rng, L, tc, kind, flip):
+    # test_transforms.jl:125-140
+    tol, n = 1e-8, 150
+    th = tc + 0.12 * (2 * rng.uniform(size=n) - 1)
+    ph = 1.0 + 0.4 * rng.uniform(size=n)
+    pg, po = _plans(P, O, th, ph, L, tol)
+    assert pg.info["kind_name"] == kind and pg.info["flip"] == flip
+    a = O.rand_alm_fast(L, rng)
+    fg = pg @ a
+    pd = O.Plan(th, ph, L, method="direct")
+    assert relmax(fg, pd.synth(a)) < 10 * tol
+    assert relmax(fg, po.synth(a)) < 1e-11
+    f = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ag = pg.H @ f
+    assert abs(np.vdot(f, fg) - np.vdot(ag, a)) / abs(np.vdot(f, fg)) < 1e-12
+    assert rel_modes(O, ag, po.adjoint(f), L) < 1e-11
+
+
+def test_ntrans_columns(torch, P, O, rng):
+    # test_transforms.jl:142-156
+    L, B, n = 256, 3, 4000
+    th, ph = cap_points(rng, 20, n)
+    p1 = P.Plan(th, ph, lmax=L, tol=1e-8)
+    pB = P.Plan(th, ph, lmax=L, tol=1e-8, ntrans=B)
+    aB = O.rand_alm_fast(L, rng, ntrans=B)
+    fB = pB @ aB
+    fs = np.stack([p1 @ np.asfortranarray(aB[:, :, b]) for b in range(B)], axis=1)
+    assert relmax(fB, fs) < 1e-13
+    f = rng.standard_normal((n, B)) + 1j * rng.standard_normal((n, B))
+    f = np.asfortranarray(f)
+    aBg = pB.H @ f
+    for b in range(B):
+        assert rel_modes(O, aBg[:, :, b], p1.H @ np.ascontiguousarray(f[:, b]), L) < 1e-13
+
+
+def test_shape_errors(torch, P, rng):
+    th, ph = cap_points(rng, 20, 1000)
+    p = P.Plan(th, ph, lmax=64, tol=1e-8)
+    with pytest.raises(P.DimensionMismatch):
+        p.synth(np.zeros(999, complex), P.zeros_alm(64))
+    with pytest.raises(P.DimensionMismatch):
+        p.synth(np.zeros(1000, complex), P.zeros_alm(63))
+
+
+def test_refused_geometry(torch, P, rng):
+    th, ph = cap_points(rng, 5, 2000)  # 5 deg cap at lmax 256, tol 1e-8: no admissible stretch
+    with pytest.raises(P.ArgumentError):
+        P.Plan(th, ph, lmax=256, tol=1e-8, method="stretch")
+
+
+# --------------------------------------------------------------------------------------------
+# Full size: lmax 2048, M = 2e6 (the benchmark workload)
+# --------------------------------------------------------------------------------------------
+def test_full_size_config2(torch, P, O, rng):
+    L, tol, M = 2048, 1e-8, 2_000_000
+    th, ph = cap_points(rng, 20, M)
+    print("building lmax 2048 plan", flush=True)
+    pg = P.Plan(th, ph, lmax=L, tol=tol, method="stretch")
+    a = O.rand_alm_fast(L, rng, cl_power=2.0)
+    ad = dev(torch, a)
+    fd = torch.zeros(M, dtype=torch.complex128, device="cuda")
+    pg.synth(fd, ad)
+    fg = host(fd, (M,))
+    sub = rng.choice(M, 256, replace=False)
+    print("direct oracle on a 256-point subset", flush=True)
+    pd = O.Plan(th[sub], ph[sub], L, method="direct", nthreads=NT)
+    fdir = pd.synth(a)
+    err_s = relmax(fg[sub], fdir)
+    print(f"synthesis vs Direct: {err_s:.3e}", flush=True)
+    assert err_s <= 1e-8
+    # adjoint with f supported on the subset: Y'f then involves only those points
+    f = np.zeros(M, complex)
+    f[sub] = rng.standard_normal(256) + 1j * rng.standard_normal(256)
+    alm_d = torch.zeros(a.size, dtype=torch.complex128, device="cuda")
+    pg.adjoint(alm_d, dev(torch, f))
+    ag = host(alm_d, a.shape)
+    adir = pd.adjoint(f[sub])
+    err_a = rel_modes(O, ag, adir, L)
+    print(f"adjoint vs Direct: {err_a:.3e}", flush=True)
+    assert err_a <= 1e-8
+    # adjoint identity on a dense random field
+    g = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    pg.adjoint(alm_d, dev(torch, g))
+    ag = host(alm_d, a.shape)
+    lhs, rhs = np.vdot(g, fg), np.vdot(ag, a)
+    rid = abs(lhs - rhs) / abs(lhs)
+    print(f"adjoint identity: {rid:.3e}", flush=True)
+    assert rid < 1e-12
