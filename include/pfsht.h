@@ -99,6 +99,12 @@ int pfsht_rings_from_alm(pfsht_rings_t rings, const void *alm, void *G, int32_t 
 /* alm_from_rings!(alm, G, eng) — src/Legendre.jl:332-341; alm overwritten. */
 int pfsht_alm_from_rings(pfsht_rings_t rings, const void *G, void *alm, int32_t where, void *stream);
 
+/* The plan's NUFFT on its stretched points — the seam _nufft_exec!(plan, in, out) of
+ * src/NUFFT.jl:17-25 with the two plans a StretchLeaf builds (src/Stretch.jl:235-238):
+ *   type 2 (iflag +1): in = modes nth x nph x ntrans (column-major, FFT order), out = f (npts x ntrans);
+ *   type 1 (iflag -1): in = f, out = modes. Outputs are overwritten. */
+int pfsht_plan_nufft(pfsht_plan_t plan, int32_t type, const void *in, void *out, int32_t where, void *stream);
+
 /* The plan's own ring engine, for stage-level parity of a stretched plan. */
 int pfsht_plan_rings(pfsht_plan_t plan, pfsht_rings_t *out);
 

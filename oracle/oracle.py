@@ -95,6 +95,7 @@ def lib():
             "or_plan_last_G": (_I, [_P, _P]),
             "or_plan_last_Fhat": (_I, [_P, _P]),
             "or_plan_last_times": (None, [_P, _P]),
+            "or_plan_points": (_I, [_P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -380,6 +381,13 @@ class Plan:
         F = np.zeros((self.info["nth"], self.info["nph"], self.B), np.complex128, order="F")
         lib().or_plan_last_Fhat(self.h, _ptr(F))
         return F
+
+    def points(self):
+        T = np.zeros(self.M)
+        p = np.zeros(self.M)
+        if lib().or_plan_points(self.h, _ptr(T), _ptr(p)) != 0:
+            raise ValueError("not a stretched plan")
+        return T, p
 
     def last_times(self):
         t = np.zeros(3)

@@ -1360,3 +1360,10 @@ int or_plan_last_Fhat(const or_plan *p, or_cplx *Fhat) {
     return 0;
 }
 void or_plan_last_times(const or_plan *p, double *t3) { memcpy(t3, p->times, sizeof p->times); }
+
+int or_plan_points(const or_plan *p, double *Theta, double *phi) {
+    if (p->method != 2) return -1;
+    memcpy(Theta, p->Tpts, sizeof(double) * (size_t)p->M);
+    memcpy(phi, p->ppts, sizeof(double) * (size_t)p->M);
+    return 0;
+}

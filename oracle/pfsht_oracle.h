@@ -98,6 +98,8 @@ int or_plan_last_G(const or_plan *p, or_cplx *G);
 int or_plan_last_Fhat(const or_plan *p, or_cplx *Fhat);
 /* stage timings of the last apply, seconds: [ring, grid(fft+dfs+window), nufft] */
 void or_plan_last_times(const or_plan *p, double *t3);
+/* the stretched coordinates (Theta_i, phi_i) of the plan's points (src/Stretch.jl:202-213) */
+int or_plan_points(const or_plan *p, double *Theta, double *phi);
 
 #ifdef __cplusplus
 }

@@ -242,7 +242,7 @@ struct pfsht_plan_s {
     NufftDev nu;
     double2 *Fp = nullptr, *Ft = nullptr, *U = nullptr;
     FftPlan phi_bwd, phi_fwd, grid_fwd, grid_bwd, fine_bwd, fine_fwd;
-    double2 *alm_stage = nullptr, *f_stage = nullptr;
+    double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     int64_t bytes = 0;
     std::mutex mu;
     // timing
@@ -255,9 +255,9 @@ struct pfsht_plan_s {
     ~pfsht_plan_s() {
         delete rings;
         dfree(wth);
-        dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c2);
+        dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c1); dfree(nu.c2);
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
-        dfree(Fp); dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage);
+        dfree(Fp); dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
         fft_free(phi_bwd); fft_free(phi_fwd); fft_free(grid_fwd); fft_free(grid_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
@@ -383,15 +383,17 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     es_corr(p.nph, u.N2, u.w, u.beta, c2);
     const double shift = p.geom.kind == kInterior ? 0.0 : kPi / double(p.nth);
     const double Nn = double(p.nth) * double(p.nph);
-    std::vector<double2> pc1(p.nth);
+    std::vector<double2> pc1(p.nth), c1c(p.nth);
     for (int i = 0; i < p.nth; ++i) {
         const double ang = -double(mode_k(i, p.nth)) * shift;
         pc1[i] = make_double2(std::cos(ang) / Nn * c1[i], std::sin(ang) / Nn * c1[i]);
+        c1c[i] = make_double2(c1[i], 0.0);
     }
     if ((st = upload(&u.X1, X1s, by)) != kOk) return st;
     if ((st = upload(&u.X2, X2s, by)) != kOk) return st;
     if ((st = upload(&u.perm, perm, by)) != kOk) return st;
     if ((st = upload(&u.pc1, pc1, by)) != kOk) return st;
+    if ((st = upload(&u.c1, c1c, by)) != kOk) return st;
     if ((st = upload(&u.c2, c2, by)) != kOk) return st;
     if ((st = upload(&u.st_list, st_list, by)) != kOk) return st;
     if ((st = upload(&u.st_slot, st_slot, by)) != kOk) return st;
@@ -476,7 +478,7 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     tmark(p, t, 4, s);
     if ((st = fft_exec(p.grid_fwd, Ft, s)) != kOk) return st;
     tmark(p, t, 5, s);
-    launch_deconv_pad(p.nu, Ft, p.U, s);
+    launch_deconv_pad(p.nu, p.nu.pc1, Ft, p.U, s);
     tmark(p, t, 6, s);
     if ((st = fft_exec(p.fine_bwd, p.U, s)) != kOk) return st;
     tmark(p, t, 7, s);
@@ -498,7 +500,7 @@ static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     if (st != kOk) return st;
     tmark(p, t, 2, s);
     double2 *Ft = p.geom.kind == kInterior ? p.Fp : p.Ft;
-    launch_deconv_trunc(p.nu, p.U, Ft, s);
+    launch_deconv_trunc(p.nu, p.nu.pc1, p.U, Ft, s);
     tmark(p, t, 3, s);
     if ((st = fft_exec(p.grid_bwd, Ft, s)) != kOk) return st;
     tmark(p, t, 4, s);
@@ -623,6 +625,53 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
     if ((st = adjoint_impl(*p, p->f_stage, p->alm_stage, s)) != kOk) return st;
     PFSHT_HIP_TRY(hipMemcpyAsync(alm, p->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
     PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, int32_t where, void *stream) {
+    if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
+    if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }
+    if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const NufftDev &u = p->nu;
+    const size_t modes_bytes = sizeof(double2) * size_t(p->nth) * p->nph * p->B;
+    const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;
+    // the modes buffer in the reference layout lives in Fp's place when the caller's memory is on
+    // the host; the plan's Ft holds the internal [i][k] layout.
+    double2 *modes_ref = nullptr, *fbuf = nullptr;
+    Status st;
+    const void *in_d = in;
+    void *out_d = out;
+    if (where == PFSHT_WHERE_HOST) {
+        if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), f_bytes, type == 1 ? in : nullptr, s, p->bytes)) != kOk)
+            return st;
+        if ((st = stage_in(reinterpret_cast<void **>(&p->modes_stage), modes_bytes, type == 2 ? in : nullptr, s,
+                           p->bytes)) != kOk)
+            return st;
+        fbuf = p->f_stage;
+        modes_ref = p->modes_stage;
+        in_d = type == 1 ? static_cast<const void *>(fbuf) : static_cast<const void *>(modes_ref);
+        out_d = type == 1 ? static_cast<void *>(modes_ref) : static_cast<void *>(fbuf);
+    }
+    if (type == 2) {
+        // reference column-major nth x nph == row-major [k][i]; internal is [i][k]
+        launch_transpose(static_cast<const double2 *>(in_d), p->Ft, p->nph, p->nth, p->B, s);
+        launch_deconv_pad(u, u.c1, p->Ft, p->U, s);
+        if ((st = fft_exec(p->fine_bwd, p->U, s)) != kOk) return st;
+        launch_interp(u, p->U, static_cast<double2 *>(out_d), s);
+    } else {
+        launch_spread(u, static_cast<const double2 *>(in_d), p->U, s);
+        if ((st = fft_exec(p->fine_fwd, p->U, s)) != kOk) return st;
+        launch_deconv_trunc(u, u.c1, p->U, p->Ft, s);
+        launch_transpose(p->Ft, static_cast<double2 *>(out_d), p->nth, p->nph, p->B, s);
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
+    if (where == PFSHT_WHERE_HOST) {
+        PFSHT_HIP_TRY(hipMemcpyAsync(out, out_d, type == 1 ? modes_bytes : f_bytes, hipMemcpyDeviceToHost, s));
+        PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    }
     return kOk;
 }
 

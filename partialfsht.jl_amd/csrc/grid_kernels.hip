@@ -135,4 +135,27 @@ void launch_dfs_fold(const GridDims &g, const double2 *Ft, double2 *Fp, hipStrea
     dfs_fold_kernel<<<grid, 256, 0, s>>>(Ft, Fp, g.nh, g.nph);
 }
 
+// Batched complex transpose through an LDS tile (layout conversion at the C-ABI boundary).
+__global__ void __launch_bounds__(256)
+transpose_kernel(const double2 *__restrict__ in, double2 *__restrict__ out, int rows, int cols) {
+    __shared__ double2 tile[kTile][kTile + 1];
+    const int c0 = blockIdx.x * kTile, r0 = blockIdx.y * kTile, b = blockIdx.z;
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const int64_t base = int64_t(b) * rows * cols;
+    for (int rr = ty; rr < kTile; rr += 8) {
+        const int r = r0 + rr, c = c0 + tx;
+        if (r < rows && c < cols) tile[rr][tx] = in[base + int64_t(r) * cols + c];
+    }
+    __syncthreads();
+    for (int cc = ty; cc < kTile; cc += 8) {
+        const int c = c0 + cc, r = r0 + tx;
+        if (c < cols && r < rows) out[base + int64_t(c) * rows + r] = tile[tx][cc];
+    }
+}
+
+void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s) {
+    dim3 grid((cols + kTile - 1) / kTile, (rows + kTile - 1) / kTile, B);
+    transpose_kernel<<<grid, 256, 0, s>>>(in, out, rows, cols);
+}
+
 }  // namespace pfsht

@@ -66,14 +66,14 @@ __global__ void deconv_trunc_kernel(const double2 *__restrict__ U, const double2
     Fhat[(int64_t(b) * n1 + i) * n2 + k] = make_double2((u.x * p.x + u.y * p.y) * s, (u.y * p.x - u.x * p.y) * s);
 }
 
-void launch_deconv_pad(const NufftDev &u, const double2 *Fhat, double2 *U, hipStream_t s) {
+void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s) {
     dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
-    deconv_pad_kernel<<<grid, 256, 0, s>>>(Fhat, u.pc1, u.c2, U, u.n1, u.n2, u.N1, u.N2);
+    deconv_pad_kernel<<<grid, 256, 0, s>>>(Fhat, pc1, u.c2, U, u.n1, u.n2, u.N1, u.N2);
 }
 
-void launch_deconv_trunc(const NufftDev &u, const double2 *U, double2 *Fhat, hipStream_t s) {
+void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s) {
     dim3 grid((u.n2 + 255) / 256, u.n1, u.B);
-    deconv_trunc_kernel<<<grid, 256, 0, s>>>(U, u.pc1, u.c2, Fhat, u.n1, u.n2, u.N1, u.N2);
+    deconv_trunc_kernel<<<grid, 256, 0, s>>>(U, pc1, u.c2, Fhat, u.n1, u.n2, u.N1, u.N2);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -167,10 +167,11 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             dr = __builtin_amdgcn_mfma_f64_16x16x4f64(are, c, dr, 0, 0, 0);
             di = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di, 0, 0, 0);
         }
-        // D[i][j]: lane holds column j = lane % 16 and rows i = 4 * (lane / 16) + r
+        // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds
+        // row i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = bi * S + 4 * kq + r;
+            const int row = bi * S + kq + 4 * r;
             const int col = bj * S + ii;
             double2 &cell = lw[row * win + col];
             cell.x += dr[r];

@@ -75,6 +75,7 @@ struct NufftDev {
     double *X1 = nullptr, *X2 = nullptr;  // sorted points in fine-grid units, base cell in [0, N)
     int32_t *perm = nullptr;              // sorted position -> user index
     double2 *pc1 = nullptr;               // [n1] phase * h1/psihat1 (synthesis; adjoint uses conj)
+    double2 *c1 = nullptr;                // [n1] h1/psihat1 as complex (the bare NUFFT, no phase)
     double *c2 = nullptr;                 // [n2] h2/psihat2
     // spreading super-tiles
     int S = 0, ns1 = 0, ns2 = 0, win = 0;  // bin size, super-tile grid, LDS window edge
@@ -84,8 +85,11 @@ struct NufftDev {
     int32_t *bin_ofs = nullptr;           // [nst*16 + 1] point ranges per (super-tile, local bin)
     double2 *Wst = nullptr;               // [B][nst][win][win] super-tile windows
 };
-void launch_deconv_pad(const NufftDev &u, const double2 *Fhat, double2 *U, hipStream_t s);
-void launch_deconv_trunc(const NufftDev &u, const double2 *U, double2 *Fhat, hipStream_t s);
+// pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
+void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
+void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
+// out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
+void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
 void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s);
 void launch_spread(const NufftDev &u, const double2 *f, double2 *U, hipStream_t s);
 

@@ -74,6 +74,7 @@ EXPORTS = {
     "pfsht_rings_from_alm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_alm_from_rings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_plan_rings": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pfsht_plan_nufft": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "pfsht_stage_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
 }
@@ -287,6 +288,19 @@ class Plan:
                                        ms.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p),
                                        1 if reset else 0))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(STAGES)}
+
+    def nufft(self, typ: int, out, inp, stream=None):
+        """_nufft_exec!(nufft_typ, inp, out) on the plan's stretched points (src/NUFFT.jl:17-25):
+        type 2 (iflag +1) modes -> f, type 1 (iflag -1) f -> modes; modes are nth x nph x ntrans,
+        column-major, FFT order."""
+        nm = self.info["nth"] * self.info["nph"] * self.ntrans
+        nf = self.npts * self.ntrans
+        pi, wi = _buf(inp, nm if typ == 2 else nf, "in")
+        po, wo = _buf(out, nf if typ == 2 else nm, "out")
+        if wi != wo:
+            raise ArgumentError("in and out must both be host or both be device arrays")
+        _check(lib().pfsht_plan_nufft(self._h, int(typ), pi, po, wi, _stream_ptr(stream) if wi else C.c_void_p(0)))
+        return out
 
     def rings(self) -> "Rings":
         h = C.c_void_p()

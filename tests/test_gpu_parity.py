@@ -111,6 +111,28 @@ def _plans(P, O, th, ph, L, tol, B=1):
     return pg, po
 
 
+def test_nufft_seam_vs_oracle_and_direct_sum(torch, P, O, rng):
+    # _nufft_exec! on the plan's stretched points (src/NUFFT.jl:17-25,155-199): type 2 (iflag +1)
+    # and type 1 (iflag -1) against the oracle's ES NUFFT (same kernel, <1e-12) and against the
+    # reference's direct sum (200 tol, test_backends.jl:1-19); and the pair is an exact adjoint.
+    L, tol, M = 96, 1e-8, 700
+    th, ph = cap_points(rng, 20, M)
+    pg, po = _plans(P, O, th, ph, L, tol)
+    T, p = po.points()
+    n1, n2 = pg.info["nth"], pg.info["nph"]
+    c = np.asfortranarray(rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2)))
+    f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    out2 = np.zeros(M, complex)
+    pg.nufft(2, out2, c)
+    assert relmax(out2, O.nufft_es(2, n1, n2, +1, T, p, c, tol)[:, 0]) < 1e-12
+    assert relmax(out2, O.nufft_direct(2, n1, n2, +1, T, p, c)[:, 0]) < 200 * tol
+    out1 = np.zeros((n1, n2), complex, order="F")
+    pg.nufft(1, out1, f)
+    assert relmax(out1, O.nufft_es(1, n1, n2, -1, T, p, f, tol)[:, :, 0]) < 1e-12
+    assert relmax(out1, O.nufft_direct(1, n1, n2, -1, T, p, f)[:, :, 0]) < 200 * tol
+    assert abs(np.vdot(f, out2) - np.vdot(out1, c)) < 1e-13 * abs(np.vdot(f, out2))
+
+
 def test_config1_synth_adjoint_vs_oracle(torch, P, O, rng):
     # 20 deg cap, lmax 256, M = 4 (L theta*)^2, tol 1e-8, B = 1
     L, tol = 256, 1e-8
