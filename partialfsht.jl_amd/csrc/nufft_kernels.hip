@@ -9,12 +9,12 @@
 // so ⟨type2 c, f⟩ = ⟨c, type1 f⟩ holds to round-off.
 //
 // Points are bin-sorted at plan time. Spreading is deterministic without atomics: a workgroup owns
-// a super-tile of 4 x 4 bins (S x S base cells each) and accumulates each bin's 16 x 16 footprint
-// window as an FP64 MFMA outer-product sum (v_mfma_f64_16x16x4f64: four points per instruction,
-// A = f_p * psi_row, B = psi_col). Bins of one parity class are 2S apart and their windows cannot
-// overlap, so the four classes are added into the LDS super-tile window one after another, in a
-// fixed order. A merge pass then sums, for every fine-grid cell, the (at most 2 x 2) super-tile
-// windows covering it in a fixed order.
+// a super-tile of 4 x 4 bins (S x S base cells each, S = 17 - w so a bin's footprint is 16 x 16)
+// and accumulates each bin's footprint window as an FP64 MFMA outer-product sum
+// (v_mfma_f64_16x16x4f64: four points per instruction, A = f_p * psi_row, B = psi_col). The four
+// waves compute four bins' windows concurrently, then add them into the LDS super-tile window one
+// wave after another, so every cell is summed in a fixed order. A merge pass then sums, for every
+// fine-grid cell, the (at most 2 x 2) super-tile windows covering it in a fixed order.
 #include "pfsht_common.hpp"
 
 namespace pfsht {
@@ -168,16 +168,22 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             di = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di, 0, 0, 0);
         }
         // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds
-        // row i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe)
+        // row i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Windows of
+        // different waves may overlap (16 > 2S when w > 9), so the waves add into LDS one at a
+        // time, in wave order: race-free and a fixed summation order.
+        for (int turn = 0; turn < 4; ++turn) {
+            if (wave == turn) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = bi * S + kq + 4 * r;
-            const int col = bj * S + ii;
-            double2 &cell = lw[row * win + col];
-            cell.x += dr[r];
-            cell.y += di[r];
+                for (int r = 0; r < 4; ++r) {
+                    const int row = bi * S + kq + 4 * r;
+                    const int col = bj * S + ii;
+                    double2 &cell = lw[row * win + col];
+                    cell.x += dr[r];
+                    cell.y += di[r];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     double2 *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
     for (int q = tid; q < win * win; q += 256) out[q] = lw[q];

@@ -54,17 +54,92 @@ __device__ __forceinline__ void seed(double e0, double &q, int &E) {
 // ------------------------------------------------------------------------------------------------
 // Forward
 // ------------------------------------------------------------------------------------------------
+// One degree sweep of order m feeding NB coefficient sets (transforms b0 .. b0 + NB - 1): the
+// recurrence is computed once per ring and reused for every transform. NEG: m > 0, so the order
+// -m column is accumulated too.
+template <int NB, bool NEG>
+__device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *const *cp,
+                                          const double2 *const *cm, double x, double q2, int E, int m, int L,
+                                          double (&ar)[NB], double (&ai)[NB], double (&br)[NB], double (&bi)[NB],
+                                          int &Eout) {
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        const double2 a0 = cp[t][m];
+        ar[t] = a0.x * q2;
+        ai[t] = a0.y * q2;
+        br[t] = bi[t] = 0.0;
+        if (NEG) {
+            const double2 c0 = cm[t][m];
+            br[t] = c0.x * q2;
+            bi[t] = c0.y * q2;
+        }
+    }
+    double q1 = 0.0;
+    if (m < L) {
+        q1 = sqrt(double(2 * m + 3)) * x * q2;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const double2 a1 = cp[t][m + 1];
+            ar[t] = fma(a1.x, q1, ar[t]);
+            ai[t] = fma(a1.y, q1, ai[t]);
+            if (NEG) {
+                const double2 c1 = cm[t][m + 1];
+                br[t] = fma(c1.x, q1, br[t]);
+                bi[t] = fma(c1.y, q1, bi[t]);
+            }
+        }
+    }
+    int l = m + 2;
+    while (l <= L) {
+        const int lend = min(L, l + kRescaleStride - 1);
+#pragma unroll 4
+        for (; l <= lend; ++l) {
+            const double2 c = abm[l];
+            const double q = fma(c.x * x, q1, -(c.y * q2));
+            q2 = q1;
+            q1 = q;
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const double2 a = cp[t][l];
+                ar[t] = fma(a.x, q, ar[t]);
+                ai[t] = fma(a.y, q, ai[t]);
+                if (NEG) {
+                    const double2 d = cm[t][l];
+                    br[t] = fma(d.x, q, br[t]);
+                    bi[t] = fma(d.y, q, bi[t]);
+                }
+            }
+        }
+        if (l <= L) {
+            const int s = renorm_shift(q1, q2);
+            q1 = ldexp(q1, -s);
+            q2 = ldexp(q2, -s);
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                ar[t] = ldexp(ar[t], -s);
+                ai[t] = ldexp(ai[t], -s);
+                if (NEG) {
+                    br[t] = ldexp(br[t], -s);
+                    bi[t] = ldexp(bi[t], -s);
+                }
+            }
+            E += s;
+        }
+    }
+    Eout = E;
+}
+
+template <int NB>
 __global__ void __launch_bounds__(64)
 ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
-                const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax) {
+                const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax, int b0) {
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int lane = threadIdx.x;
     const bool active = lane < it.nr;
     const int j = it.j0 + (active ? lane : 0);
-    const int b = blockIdx.y;
 
     const double x = costh[j];
     const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
@@ -74,102 +149,59 @@ ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     if (!active) q2 = 0.0;
 
     const int64_t LB = int64_t(L + 1) * (2 * L + 1);
-    // column of order m (row l - m holds degree l) and of order -m
-    const double2 *__restrict__ cp = alm + b * LB + int64_t(2 * m) * (L + 1) - m;
-    const double2 *__restrict__ cm = alm + b * LB + int64_t(2 * m - 1) * (L + 1) - m;
-
-    double ar = 0, ai = 0, br = 0, bi = 0;  // accumulators for +m and -m
-    {
-        const double2 a0 = cp[m];
-        ar = a0.x * q2;
-        ai = a0.y * q2;
-        if (m > 0) {
-            const double2 c0 = cm[m];
-            br = c0.x * q2;
-            bi = c0.y * q2;
-        }
-    }
-    double q1 = 0.0;
-    if (m < L) {
-        q1 = sqrt(double(2 * m + 3)) * x * q2;
-        const double2 a1 = cp[m + 1];
-        ar = fma(a1.x, q1, ar);
-        ai = fma(a1.y, q1, ai);
-        if (m > 0) {
-            const double2 c1 = cm[m + 1];
-            br = fma(c1.x, q1, br);
-            bi = fma(c1.y, q1, bi);
-        }
+    // column of order m (row l - m holds degree l) and of order -m, per transform
+    const double2 *cp[NB];
+    const double2 *cm[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        cp[t] = alm + (b0 + t) * LB + int64_t(2 * m) * (L + 1) - m;
+        cm[t] = alm + (b0 + t) * LB + int64_t(2 * m - 1) * (L + 1) - m;
     }
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);  // abm[l] for l = m+2..L
 
-    int l = m + 2;
-    if (m > 0) {
-        while (l <= L) {
-            const int lend = min(L, l + kRescaleStride - 1);
-#pragma unroll 4
-            for (; l <= lend; ++l) {
-                const double2 c = abm[l];
-                const double q = fma(c.x * x, q1, -(c.y * q2));
-                q2 = q1;
-                q1 = q;
-                const double2 a = cp[l];
-                const double2 d = cm[l];
-                ar = fma(a.x, q, ar);
-                ai = fma(a.y, q, ai);
-                br = fma(d.x, q, br);
-                bi = fma(d.y, q, bi);
-            }
-            if (l <= L) {
-                const int s = renorm_shift(q1, q2);
-                q1 = ldexp(q1, -s);
-                q2 = ldexp(q2, -s);
-                ar = ldexp(ar, -s);
-                ai = ldexp(ai, -s);
-                br = ldexp(br, -s);
-                bi = ldexp(bi, -s);
-                E += s;
-            }
-        }
-    } else {
-        while (l <= L) {
-            const int lend = min(L, l + kRescaleStride - 1);
-#pragma unroll 4
-            for (; l <= lend; ++l) {
-                const double2 c = abm[l];
-                const double q = fma(c.x * x, q1, -(c.y * q2));
-                q2 = q1;
-                q1 = q;
-                const double2 a = cp[l];
-                ar = fma(a.x, q, ar);
-                ai = fma(a.y, q, ai);
-            }
-            if (l <= L) {
-                const int s = renorm_shift(q1, q2);
-                q1 = ldexp(q1, -s);
-                q2 = ldexp(q2, -s);
-                ar = ldexp(ar, -s);
-                ai = ldexp(ai, -s);
-                E += s;
-            }
-        }
-    }
+    double ar[NB], ai[NB], br[NB], bi[NB];
+    int Ef;
+    if (m > 0) fwd_sweep<NB, true>(abm, cp, cm, x, q2, E, m, L, ar, ai, br, bi, Ef);
+    else fwd_sweep<NB, false>(abm, cp, cm, x, q2, E, m, L, ar, ai, br, bi, Ef);
     if (!active) return;
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int64_t ncol = 2 * mmax + 1;
-    double2 *gp = Gd + (int64_t(b) * ncol + (mmax + m)) * n + j;
-    *gp = make_double2(sg * ldexp(ar, E), sg * ldexp(ai, E));
-    if (m > 0) {
-        double2 *gm = Gd + (int64_t(b) * ncol + (mmax - m)) * n + j;
-        *gm = make_double2(ldexp(br, E), ldexp(bi, E));
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        double2 *gp = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + j;
+        *gp = make_double2(sg * ldexp(ar[t], Ef), sg * ldexp(ai[t], Ef));
+        if (m > 0) {
+            double2 *gm = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + j;
+            *gm = make_double2(ldexp(br[t], Ef), ldexp(bi[t], Ef));
+        }
     }
+}
+
+// Transform groups of 8, 4, 2 and 1 cover any ntrans exactly with at most four launches.
+template <class F>
+static void for_transform_groups(int B, F &&launch) {
+    int b0 = 0;
+    for (int nb : {8, 4, 2, 1})
+        while (B - b0 >= nb) {
+            launch(nb, b0);
+            b0 += nb;
+        }
 }
 
 void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s) {
     if (r.nitems == 0) return;
-    dim3 grid(r.nitems, r.B);
-    ring_fwd_kernel<<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n,
-                                        r.mmax);
+    for_transform_groups(r.B, [&](int nb, int b0) {
+        dim3 grid(r.nitems);
+#define PFSHT_FWD(NB) \
+    ring_fwd_kernel<NB><<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0)
+        switch (nb) {
+            case 8: PFSHT_FWD(8); break;
+            case 4: PFSHT_FWD(4); break;
+            case 2: PFSHT_FWD(2); break;
+            default: PFSHT_FWD(1); break;
+        }
+#undef PFSHT_FWD
+    });
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -177,12 +209,13 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
 // ------------------------------------------------------------------------------------------------
 constexpr int kTL = 64;  // degrees per LDS tile
 
+template <int NB>
 __global__ void __launch_bounds__(64)
 ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
                 const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
-                double4 *__restrict__ partials, int L, int n, int mmax, int B) {
+                double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0) {
     __shared__ double W[kTL][kTL + 1];
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
@@ -191,7 +224,6 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const int lane = threadIdx.x;
     const bool active = lane < nr;
     const int j = it.j0 + (active ? lane : 0);
-    const int b = blockIdx.y;
 
     const double x = costh[j];
     const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
@@ -201,11 +233,16 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     if (!active) qs = 0.0;
 
     const int64_t ncol = 2 * mmax + 1;
-    const double2 *__restrict__ gpc = Gd + (int64_t(b) * ncol + (mmax + m)) * n + it.j0;
-    const double2 *__restrict__ gmc = Gd + (int64_t(b) * ncol + (mmax - m)) * n + it.j0;
+    const double2 *gpc[NB];
+    const double2 *gmc[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+        gpc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + it.j0;
+        gmc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + it.j0;
+    }
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
     const int nl = L - m + 1;
-    double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b;
+    double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
 
     double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
     for (int l0 = m; l0 <= L; l0 += kTL) {
@@ -242,28 +279,37 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
         __syncthreads();
         // phase 2: lanes = degree offsets, sum over the tile's rings in ring order
         if (lane < tl) {
-            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            double s0[NB], s1[NB], s2[NB], s3[NB];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) s0[u] = s1[u] = s2[u] = s3[u] = 0.0;
             if (m > 0) {
-#pragma unroll 4
+#pragma unroll 2
                 for (int r = 0; r < nr; ++r) {
                     const double wv = W[lane][r];
-                    const double2 gp = gpc[r];
-                    const double2 gm = gmc[r];
-                    s0 = fma(wv, gp.x, s0);
-                    s1 = fma(wv, gp.y, s1);
-                    s2 = fma(wv, gm.x, s2);
-                    s3 = fma(wv, gm.y, s3);
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const double2 gp = gpc[u][r];
+                        const double2 gm = gmc[u][r];
+                        s0[u] = fma(wv, gp.x, s0[u]);
+                        s1[u] = fma(wv, gp.y, s1[u]);
+                        s2[u] = fma(wv, gm.x, s2[u]);
+                        s3[u] = fma(wv, gm.y, s3[u]);
+                    }
                 }
             } else {
-#pragma unroll 4
+#pragma unroll 2
                 for (int r = 0; r < nr; ++r) {
                     const double wv = W[lane][r];
-                    const double2 gp = gpc[r];
-                    s0 = fma(wv, gp.x, s0);
-                    s1 = fma(wv, gp.y, s1);
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        const double2 gp = gpc[u][r];
+                        s0[u] = fma(wv, gp.x, s0[u]);
+                        s1[u] = fma(wv, gp.y, s1[u]);
+                    }
                 }
             }
-            out[int64_t(l0 - m + lane) * B] = make_double4(s0, s1, s2, s3);
+#pragma unroll
+            for (int u = 0; u < NB; ++u) out[int64_t(l0 - m + lane) * B + u] = make_double4(s0[u], s1[u], s2[u], s3[u]);
         }
         __syncthreads();
     }
@@ -304,11 +350,20 @@ __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_
 }
 
 void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s) {
-    if (r.nitems > 0) {
-        dim3 grid(r.nitems, r.B);
-        ring_adj_kernel<<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, r.pofs, Gd,
-                                            r.partials, r.L, r.n, r.mmax, r.B);
-    }
+    if (r.nitems > 0)
+        for_transform_groups(r.B, [&](int nb, int b0) {
+            dim3 grid(r.nitems);
+#define PFSHT_ADJ(NB)                                                                                      \
+    ring_adj_kernel<NB><<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, r.pofs, Gd, \
+                                            r.partials, r.L, r.n, r.mmax, r.B, b0)
+            switch (nb) {
+                case 8: PFSHT_ADJ(8); break;
+                case 4: PFSHT_ADJ(4); break;
+                case 2: PFSHT_ADJ(2); break;
+                default: PFSHT_ADJ(1); break;
+            }
+#undef PFSHT_ADJ
+        });
     const int64_t total = int64_t(r.L + 1) * (2 * r.L + 1) * r.B;
     const int threads = 256;
     const int64_t blocks = std::min<int64_t>((total + threads - 1) / threads, 65536);

@@ -1,0 +1,141 @@
+"""
+    PartialFSHTHIPExt
+
+MI355X (gfx950) stretched leaf for `PartialFSHT.Plan`, loaded by `using AMDGPU`. A plan whose
+node arrays are `ROCVector`s builds a `HIPStretchLeaf`: every apply runs on the device through
+`libpfsht_hip.so`'s C ABI (`include/pfsht.h`), with coefficients and fields left in HBM.
+
+    using PartialFSHT, AMDGPU
+    plan = PartialFSHT.Plan(ROCArray(θ), ROCArray(φ); lmax = 2048, tol = 1e-8,
+                            method = PartialFSHT.StretchAlgorithm(),
+                            nufft = PartialFSHT.DirectSumBackend())   # any resolvable backend
+    mul!(f, plan, alm)            # f, alm :: ROCArray{ComplexF64}
+    mul!(alm, plan', f)
+
+The NUFFT lives inside the library, so the `nufft` keyword only has to resolve
+(`src/NUFFT.jl:131-142`); `DirectSumBackend()` always does. Dispatch is on the node array type,
+through the leaf seam `_build_leaf` / `_synthesize!` / `_adjoint_apply!` (`src/Plan.jl:118-135,
+181-188`), the same seam `PartialFSHTNUFSHTExt` uses to add a leaf: nothing in the package is
+overwritten and no `method=` tag is added.
+"""
+module PartialFSHTHIPExt
+
+using PartialFSHT: PartialFSHT
+using AMDGPU: AMDGPU, ROCArray, ROCVector
+using LinearAlgebra: LinearAlgebra
+
+const libpfsht = get(ENV, "PFSHT_HIP_LIB",
+                     normpath(joinpath(@__DIR__, "..", "..", "lib", "libpfsht_hip.so")))
+
+const PFSHT_OK = Cint(0)
+const PFSHT_E_DIM = Cint(1)
+const PFSHT_E_ARG = Cint(2)
+const PFSHT_E_GEOM_REFUSED = Cint(3)
+const PFSHT_METHOD_STRETCH = Cint(2)
+const PFSHT_PREC_FP64 = Cint(0)
+const PFSHT_WHERE_DEVICE = Cint(1)
+
+struct PfshtInfo              # pfsht_info, include/pfsht.h
+    method::Int32
+    kind::Int32
+    flip::Int32
+    lmax::Int32
+    ntrans::Int32
+    precision::Int32
+    nth_half::Int32
+    nth::Int32
+    nph::Int32
+    mmax::Int32
+    es_w::Int32
+    es_n1::Int32
+    es_n2::Int32
+    npts::Int64
+    ring_work::Int64
+    device_bytes::Int64
+    tol::Float64
+    es_beta::Float64
+end
+
+_last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
+
+# Status codes map onto the exceptions the host path throws (src/Layout.jl:35-56;
+# src/Plan.jl:248-252; src/Windows.jl:66-74).
+function _check(rc::Cint)
+    rc == PFSHT_OK && return nothing
+    msg = _last_error()
+    rc == PFSHT_E_DIM && throw(DimensionMismatch(msg))
+    (rc == PFSHT_E_ARG || rc == PFSHT_E_GEOM_REFUSED) && throw(ArgumentError(msg))
+    error("libpfsht_hip: $msg (code $rc)")
+end
+
+"""
+    HIPStretchLeaf
+
+Handle of a device plan: geometry, ring tables, bin-sorted points, NUFFT kernel tables, rocFFT
+plans and work buffers, all resident in HBM (the device analogue of `StretchLeaf`,
+`src/Stretch.jl:94-127`). Freed by a finalizer.
+"""
+mutable struct HIPStretchLeaf
+    handle::Ptr{Cvoid}
+    info::PfshtInfo
+end
+
+function HIPStretchLeaf(θ::AbstractVector, φ::AbstractVector, lmax::Int, ntrans::Int, tol::Real)
+    θh = Vector{Float64}(PartialFSHT._host(θ))
+    φh = Vector{Float64}(PartialFSHT._host(φ))
+    length(θh) == length(φh) || throw(DimensionMismatch("θ and φ must have the same length"))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:pfsht_plan_create, libpfsht), Cint,
+                 (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Int32, Int32,
+                  Ref{Ptr{Cvoid}}),
+                 θh, φh, length(θh), lmax, tol, ntrans, PFSHT_METHOD_STRETCH, PFSHT_PREC_FP64, h))
+    info = Ref{PfshtInfo}()
+    _check(ccall((:pfsht_plan_info, libpfsht), Cint, (Ptr{Cvoid}, Ref{PfshtInfo}), h[], info))
+    leaf = HIPStretchLeaf(h[], info[])
+    finalizer(leaf) do l
+        l.handle == C_NULL || ccall((:pfsht_plan_destroy, libpfsht), Cvoid, (Ptr{Cvoid},), l.handle)
+        l.handle = C_NULL
+    end
+    return leaf
+end
+
+PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
+PartialFSHT.ring_work(leaf::HIPStretchLeaf) = Int(leaf.info.ring_work)
+
+# Leaf seam (src/Plan.jl:275-279): a stretched plan on ROCVector nodes builds the device leaf.
+PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
+                        spin, tol, nufft_b, opts) =
+    spin == 0 ? HIPStretchLeaf(θ, φ, lmax, ntrans, tol) :
+    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+
+_stream() = AMDGPU.stream().stream   # hipStream_t of the task-local AMDGPU stream
+
+function _device_ptr(x::ROCArray{ComplexF64})
+    return reinterpret(Ptr{Cvoid}, pointer(x))
+end
+
+# src/Plan.jl:181-188 → synthesize_stretch!/adjoint_stretch! (src/Stretch.jl:262-295)
+function PartialFSHT._synthesize!(f::ROCArray{ComplexF64}, alm::ROCArray{ComplexF64},
+                                  leaf::HIPStretchLeaf, ::PartialFSHT.Plan)
+    _check(ccall((:pfsht_synth, libpfsht), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                 leaf.handle, _device_ptr(alm), _device_ptr(f), PFSHT_WHERE_DEVICE, _stream()))
+    return f
+end
+
+function PartialFSHT._adjoint_apply!(alm::ROCArray{ComplexF64}, f::ROCArray{ComplexF64},
+                                     leaf::HIPStretchLeaf, ::PartialFSHT.Plan)
+    _check(ccall((:pfsht_adjoint, libpfsht), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                 leaf.handle, _device_ptr(f), _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream()))
+    return alm
+end
+
+# A host array handed to a device leaf is refused, as the reference refuses the converse
+# (ext/PartialFSHTKernelAbstractionsExt.jl:266-271).
+PartialFSHT._synthesize!(f, alm, ::HIPStretchLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
+    "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
+PartialFSHT._adjoint_apply!(alm, f, ::HIPStretchLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
+    "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
+
+end

@@ -115,7 +115,7 @@ def test_nufft_seam_vs_oracle_and_direct_sum(torch, P, O, rng):
     # _nufft_exec! on the plan's stretched points (src/NUFFT.jl:17-25,155-199): type 2 (iflag +1)
     # and type 1 (iflag -1) against the oracle's ES NUFFT (same kernel, <1e-12) and against the
     # reference's direct sum (200 tol, test_backends.jl:1-19); and the pair is an exact adjoint.
-    L, tol, M = 96, 1e-8, 700
+    L, tol, M = 160, 1e-8, 700
     th, ph = cap_points(rng, 20, M)
     pg, po = _plans(P, O, th, ph, L, tol)
     T, p = po.points()
