@@ -135,7 +135,7 @@ struct pfsht_rings_s {
 
     ~pfsht_rings_s() {
         dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.abofs);
-        dfree(d.jlo); dfree(d.jhi); dfree(d.items); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
+        dfree(d.jlo); dfree(d.jhi); dfree(d.items); dfree(d.aitems); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
     }
 };
@@ -177,25 +177,42 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
                                         double(l + m - 1) / double(l + m));
             ab[abofs[m] + (l - m - 2)] = make_double2(a, bb);
         }
-    // work items: tiles of <= 64 retained rings per order, longest first
-    std::vector<RingItem> items;
+    // work blocks, longest first: forward = all tiles of an order (up to kFwdMaxWaves); adjoint =
+    // up to kAdjWaves tiles. Partials of the adjoint are indexed by the order's 64-ring tile.
+    std::vector<RingItem> items, aitems;
     std::vector<int32_t> ntiles(mmax + 1, 0);
     std::vector<int64_t> pofs(mmax + 1, 0);
     int64_t np = 0;
     r.work = 0;
+    int fw = 1, aw = 1;
     for (int m = 0; m <= mmax; ++m) {
         pofs[m] = np;
         const int lo = r.jlo[m], hi = r.jhi[m];
         if (hi < lo) continue;
-        r.work += int64_t(hi - lo + 1) * (L - m + 1);
-        int t = 0;
-        for (int j0 = lo; j0 <= hi; j0 += 64, ++t) items.push_back({m, j0, std::min(64, hi - j0 + 1), t});
-        ntiles[m] = t;
-        np += int64_t(t) * (L - m + 1) * B;
+        const int nj = hi - lo + 1;
+        r.work += int64_t(nj) * (L - m + 1);
+        const int nt = (nj + 63) / 64;
+        for (int t0 = 0; t0 < nt; t0 += kFwdMaxWaves) {
+            const int j0 = lo + 64 * t0, nr = std::min(64 * kFwdMaxWaves, hi - j0 + 1);
+            items.push_back({m, j0, nr, t0});
+            fw = std::max(fw, (nr + 63) / 64);
+        }
+        for (int t0 = 0; t0 < nt; t0 += kAdjWaves) {
+            const int j0 = lo + 64 * t0, nr = std::min(64 * kAdjWaves, hi - j0 + 1);
+            aitems.push_back({m, j0, nr, t0});
+            aw = std::max(aw, (nr + 63) / 64);
+        }
+        ntiles[m] = nt;
+        np += int64_t(nt) * (L - m + 1) * B;
     }
-    std::stable_sort(items.begin(), items.end(), [L](const RingItem &a, const RingItem &b) {
-        return int64_t(L - a.m + 1) * a.nr > int64_t(L - b.m + 1) * b.nr;
-    });
+    auto by_cost = [L](const RingItem &a, const RingItem &b) {
+        return int64_t(L - a.m + 1) * std::min(a.nr, 64) > int64_t(L - b.m + 1) * std::min(b.nr, 64);
+    };
+    std::stable_sort(items.begin(), items.end(), by_cost);
+    std::stable_sort(aitems.begin(), aitems.end(), by_cost);
+    d.fwd_waves = fw;
+    d.adj_waves = aw;
+    d.naitems = int(aitems.size());
     d.nitems = int(items.size());
     d.npartials = np;
     std::vector<int32_t> jlo32(r.jlo.begin(), r.jlo.begin() + mmax + 1), jhi32(r.jhi.begin(), r.jhi.begin() + mmax + 1);
@@ -209,6 +226,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
     if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
     if ((st = upload(&d.items, items, by)) != kOk) return st;
+    if ((st = upload(&d.aitems, aitems, by)) != kOk) return st;
     if ((st = upload(&d.pofs, pofs, by)) != kOk) return st;
     if ((st = upload(&d.ntiles, ntiles, by)) != kOk) return st;
     if ((st = dalloc(&d.partials, size_t(std::max<int64_t>(np, 1)), by)) != kOk) return st;

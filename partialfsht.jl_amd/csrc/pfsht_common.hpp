@@ -25,12 +25,14 @@ enum Status { kOk = 0, kEDim = 1, kEArg = 2, kEGeom = 3, kENomem = 4, kEHip = 5,
         }                                                                                         \
     } while (0)
 
-// Ring-pass work item: one wavefront of up to 64 consecutive retained rings of order m.
+// Ring-pass work block: retained rings [j0, j0 + nr) of order m. Wavefront w of the block owns
+// the 64-ring tile tile0 + w (rings j0 + 64 w ..), so all tiles of one order share a compute unit
+// and its scalar/L2 caches for the order's coefficient streams.
 struct RingItem {
     int32_t m;      // order
-    int32_t j0;     // first ring of the tile
-    int32_t nr;     // rings in the tile (1..64)
-    int32_t tile;   // tile index within order m
+    int32_t j0;     // first ring of the block
+    int32_t nr;     // rings in the block
+    int32_t tile0;  // tile index (within order m) of wavefront 0
 };
 
 // Device-resident tables of a ring engine (RingLegendre, src/Legendre.jl:174-255).
@@ -42,13 +44,18 @@ struct RingDev {
     double2 *ab = nullptr;      // recurrence coefficients (a_l^m, b_l^m), l = m+2..L, per m
     int64_t *abofs = nullptr;   // [mmax+1] offset of order m in ab
     int32_t *jlo = nullptr, *jhi = nullptr;  // [mmax+1]
-    RingItem *items = nullptr;  // work items, descending cost
-    int nitems = 0;
+    RingItem *items = nullptr;  // forward blocks (all tiles of an order), descending cost
+    int nitems = 0, fwd_waves = 1;
+    RingItem *aitems = nullptr;  // adjoint blocks (<= kAdjWaves tiles of an order), descending cost
+    int naitems = 0, adj_waves = 1;
     int64_t *pofs = nullptr;    // [mmax+1] offset of order m's adjoint partials (in double4 units)
     int32_t *ntiles = nullptr;  // [mmax+1]
     double4 *partials = nullptr;  // adjoint per-tile partial sums: (re+, im+, re-, im-) per (tile, l)
     int64_t npartials = 0;
 };
+
+constexpr int kFwdMaxWaves = 16;  // forward block: up to 1024 rings of one order
+constexpr int kAdjWaves = 4;      // adjoint block: LDS holds one 64 x 65 FP64 tile per wavefront
 
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
 // Gd is the internal ring buffer [b][m + mmax][j].

@@ -130,16 +130,19 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
 }
 
 template <int NB>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kFwdMaxWaves)
 ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
                 const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax, int b0) {
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
-    const int lane = threadIdx.x;
-    const bool active = lane < it.nr;
-    const int j = it.j0 + (active ? lane : 0);
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nrw = __builtin_amdgcn_readfirstlane(it.nr) - 64 * wave;  // rings of this wavefront's tile
+    if (nrw <= 0) return;
+    const bool active = lane < nrw;
+    const int j = it.j0 + 64 * wave + (active ? lane : 0);
 
     const double x = costh[j];
     const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
@@ -193,7 +196,7 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
     for_transform_groups(r.B, [&](int nb, int b0) {
         dim3 grid(r.nitems);
 #define PFSHT_FWD(NB) \
-    ring_fwd_kernel<NB><<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0)
+    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0)
         switch (nb) {
             case 8: PFSHT_FWD(8); break;
             case 4: PFSHT_FWD(4); break;
@@ -207,23 +210,31 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
 // ------------------------------------------------------------------------------------------------
 // Adjoint
 // ------------------------------------------------------------------------------------------------
-constexpr int kTL = 64;  // degrees per LDS tile
+constexpr int kTL = 64;  // degrees per LDS tile (one 64 x 65 FP64 tile per wavefront)
+
+// Wavefront-private ordering of LDS traffic: a wavefront's LDS operations complete in issue
+// order, so a wavefront reading back its own tile needs only a compiler barrier.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 template <int NB>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kAdjWaves)
 ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
                 const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
                 double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0) {
-    __shared__ double W[kTL][kTL + 1];
+    extern __shared__ double lds_adj[];
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
-    const int nr = __builtin_amdgcn_readfirstlane(it.nr);
-    const int tile = __builtin_amdgcn_readfirstlane(it.tile);
-    const int lane = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    const int nr = min(64, __builtin_amdgcn_readfirstlane(it.nr) - 64 * wave);  // rings of this tile
+    if (nr <= 0) return;
+    const int tile = __builtin_amdgcn_readfirstlane(it.tile0) + wave;
+    const int jt = it.j0 + 64 * wave;  // first ring of the tile
+    double(*W)[kTL + 1] = reinterpret_cast<double(*)[kTL + 1]>(lds_adj + wave * kTL * (kTL + 1));
     const bool active = lane < nr;
-    const int j = it.j0 + (active ? lane : 0);
+    const int j = jt + (active ? lane : 0);
 
     const double x = costh[j];
     const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
@@ -237,12 +248,13 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const double2 *gmc[NB];
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-        gpc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + it.j0;
-        gmc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + it.j0;
+        gpc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + jt;
+        gmc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + jt;
     }
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
     const int nl = L - m + 1;
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
+    constexpr int U = NB >= 4 ? 2 : (NB == 2 ? 4 : 8);  // rings per hoisted group in phase 2
 
     double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
     for (int l0 = m; l0 <= L; l0 += kTL) {
@@ -276,86 +288,94 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
             q2 = ldexp(q2, -s);
             E += s;
         }
-        __syncthreads();
-        // phase 2: lanes = degree offsets, sum over the tile's rings in ring order
+        wave_lds_order();
+        // phase 2: lanes = degree offsets; the tile's rings are summed in ring order, U rings at a
+        // time with their LDS reads and scalar loads issued before the FMAs that use them
         if (lane < tl) {
             double s0[NB], s1[NB], s2[NB], s3[NB];
 #pragma unroll
             for (int u = 0; u < NB; ++u) s0[u] = s1[u] = s2[u] = s3[u] = 0.0;
-            if (m > 0) {
-#pragma unroll 2
-                for (int r = 0; r < nr; ++r) {
-                    const double wv = W[lane][r];
+            int r = 0;
+            for (; r + U <= nr; r += U) {
+                double wv[U];
+                double2 gp[U][NB], gm[U][NB];
+#pragma unroll
+                for (int v = 0; v < U; ++v) {
+                    wv[v] = W[lane][r + v];
 #pragma unroll
                     for (int u = 0; u < NB; ++u) {
-                        const double2 gp = gpc[u][r];
-                        const double2 gm = gmc[u][r];
-                        s0[u] = fma(wv, gp.x, s0[u]);
-                        s1[u] = fma(wv, gp.y, s1[u]);
-                        s2[u] = fma(wv, gm.x, s2[u]);
-                        s3[u] = fma(wv, gm.y, s3[u]);
+                        gp[v][u] = gpc[u][r + v];
+                        gm[v][u] = m > 0 ? gmc[u][r + v] : make_double2(0.0, 0.0);
                     }
                 }
-            } else {
-#pragma unroll 2
-                for (int r = 0; r < nr; ++r) {
-                    const double wv = W[lane][r];
+#pragma unroll
+                for (int v = 0; v < U; ++v)
 #pragma unroll
                     for (int u = 0; u < NB; ++u) {
-                        const double2 gp = gpc[u][r];
-                        s0[u] = fma(wv, gp.x, s0[u]);
-                        s1[u] = fma(wv, gp.y, s1[u]);
+                        s0[u] = fma(wv[v], gp[v][u].x, s0[u]);
+                        s1[u] = fma(wv[v], gp[v][u].y, s1[u]);
+                        s2[u] = fma(wv[v], gm[v][u].x, s2[u]);
+                        s3[u] = fma(wv[v], gm[v][u].y, s3[u]);
                     }
+            }
+            for (; r < nr; ++r) {
+                const double wv = W[lane][r];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const double2 gp = gpc[u][r];
+                    const double2 gm = m > 0 ? gmc[u][r] : make_double2(0.0, 0.0);
+                    s0[u] = fma(wv, gp.x, s0[u]);
+                    s1[u] = fma(wv, gp.y, s1[u]);
+                    s2[u] = fma(wv, gm.x, s2[u]);
+                    s3[u] = fma(wv, gm.y, s3[u]);
                 }
             }
 #pragma unroll
             for (int u = 0; u < NB; ++u) out[int64_t(l0 - m + lane) * B + u] = make_double4(s0[u], s1[u], s2[u], s3[u]);
         }
-        __syncthreads();
+        wave_lds_order();
     }
 }
 
 // Writes every alm slot: the tile-ordered sum of the partials where (l, m) is retained, zero in
-// the padding and for unretained orders (the adjoint overwrites alm, src/Legendre.jl:338).
+// the padding and for unretained orders (the adjoint overwrites alm, src/Legendre.jl:338). One
+// thread owns row l - |m| of the column pair (2|m|, 2|m| - 1), reading each partial once.
 __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_t *__restrict__ ntiles,
                                   const double4 *__restrict__ partials, double2 *__restrict__ alm, int L,
                                   int mmax, int B) {
-    const int64_t nrow = L + 1, ncol = 2 * L + 1;
-    const int64_t total = nrow * ncol * B;
-    for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < total;
-         idx += int64_t(gridDim.x) * blockDim.x) {
-        const int row = int(idx % nrow);
-        const int64_t rest = idx / nrow;
-        const int col = int(rest % ncol);
-        const int b = int(rest / ncol);
-        const int am = (col + 1) / 2;
-        const bool neg = (col & 1) != 0;
-        const int l = row + am;
-        double2 v = make_double2(0.0, 0.0);
-        if (l <= L && am <= mmax) {
-            const int nt = ntiles[am];
-            const int nl = L - am + 1;
-            const double4 *p = partials + pofs[am] + int64_t(l - am) * B + b;
-            double sr = 0, si = 0;
-            for (int t = 0; t < nt; ++t) {
-                const double4 q = p[int64_t(t) * nl * B];
-                sr += neg ? q.z : q.x;
-                si += neg ? q.w : q.y;
-            }
-            if (!neg && (am & 1)) { sr = -sr; si = -si; }
-            v = make_double2(sr, si);
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int am = blockIdx.y;
+    const int b = blockIdx.z;
+    if (row > L) return;
+    const int64_t nrow = L + 1;
+    const int64_t base = int64_t(b) * nrow * (2 * L + 1);
+    const int l = row + am;
+    double pr = 0, pi = 0, nr_ = 0, ni = 0;
+    if (l <= L && am <= mmax) {
+        const int nt = ntiles[am];
+        const int nl = L - am + 1;
+        const double4 *p = partials + pofs[am] + int64_t(l - am) * B + b;
+        for (int t = 0; t < nt; ++t) {
+            const double4 q = p[int64_t(t) * nl * B];
+            pr += q.x;
+            pi += q.y;
+            nr_ += q.z;
+            ni += q.w;
         }
-        alm[idx] = v;
+        if (am & 1) { pr = -pr; pi = -pi; }
     }
+    alm[base + int64_t(2 * am) * nrow + row] = make_double2(pr, pi);
+    if (am > 0) alm[base + int64_t(2 * am - 1) * nrow + row] = make_double2(nr_, ni);
 }
 
 void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s) {
-    if (r.nitems > 0)
+    if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
-            dim3 grid(r.nitems);
+            dim3 grid(r.naitems);
+            const size_t lds = sizeof(double) * kTL * (kTL + 1) * r.adj_waves;
 #define PFSHT_ADJ(NB)                                                                                      \
-    ring_adj_kernel<NB><<<grid, 64, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, r.pofs, Gd, \
-                                            r.partials, r.L, r.n, r.mmax, r.B, b0)
+    ring_adj_kernel<NB><<<grid, 64 * r.adj_waves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab,      \
+                                                            r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0)
             switch (nb) {
                 case 8: PFSHT_ADJ(8); break;
                 case 4: PFSHT_ADJ(4); break;
@@ -364,11 +384,8 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
             }
 #undef PFSHT_ADJ
         });
-    const int64_t total = int64_t(r.L + 1) * (2 * r.L + 1) * r.B;
-    const int threads = 256;
-    const int64_t blocks = std::min<int64_t>((total + threads - 1) / threads, 65536);
-    ring_adj_finalize<<<dim3(unsigned(blocks)), threads, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax,
-                                                                  r.B);
+    dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, r.B);
+    ring_adj_finalize<<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.B);
 }
 
 // ------------------------------------------------------------------------------------------------

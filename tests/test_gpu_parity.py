@@ -224,11 +224,13 @@ def test_ntrans_columns(torch, P, O, rng):
 
 def test_shape_errors(torch, P, rng):
     th, ph = cap_points(rng, 20, 1000)
-    p = P.Plan(th, ph, lmax=64, tol=1e-8)
+    p = P.Plan(th, ph, lmax=160, tol=1e-8, method="stretch")
     with pytest.raises(P.DimensionMismatch):
-        p.synth(np.zeros(999, complex), P.zeros_alm(64))
+        p.synth(np.zeros(999, complex), P.zeros_alm(160))
     with pytest.raises(P.DimensionMismatch):
-        p.synth(np.zeros(1000, complex), P.zeros_alm(63))
+        p.synth(np.zeros(1000, complex), P.zeros_alm(159))
+    with pytest.raises(P.DimensionMismatch):
+        p.adjoint(P.zeros_alm(160, ntrans=2), np.zeros(1000, complex))
 
 
 def test_refused_geometry(torch, P, rng):
