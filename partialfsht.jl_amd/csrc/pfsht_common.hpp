@@ -54,7 +54,7 @@ struct RingDev {
     int64_t npartials = 0;
 };
 
-constexpr int kFwdMaxWaves = 16;  // forward block: up to 1024 rings of one order
+constexpr int kFwdMaxWaves = 8;   // forward block: up to 512 rings of one order
 constexpr int kAdjWaves = 4;      // adjoint block: LDS holds one 64 x 65 FP64 tile per wavefront
 
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;

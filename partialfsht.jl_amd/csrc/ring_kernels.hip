@@ -49,6 +49,27 @@ __device__ __forceinline__ void seed(double e0, double &q, int &E) {
     }
 }
 
+// Pulls the cache lines of up to 64 byte ranges into L2 without any register dependence: lane i
+// fetches 4 bytes at addr (when addr is not null) into a wavefront-private LDS sink that is never
+// read, so the load retires asynchronously and no instruction waits on it. Scalar loads of the
+// same lines a block later then hit L2 instead of HBM.
+__device__ __forceinline__ void l2_prefetch(const void *addr, int *sink) {
+    if (addr)
+        __builtin_amdgcn_global_load_lds(addr, (__attribute__((address_space(3))) void *)(sink), 4,
+                                         0, 0);
+}
+
+// Address of the 128-byte line `k` of a stream starting at element `first`, clamped to the
+// stream's last valid element `last` (never past the end of the allocation).
+template <class T>
+__device__ __forceinline__ const void *line_addr(const T *first, const T *last, int k) {
+    const char *a = reinterpret_cast<const char *>(first) + 128 * k;
+    const char *e = reinterpret_cast<const char *>(last);
+    return a <= e ? a : (reinterpret_cast<const char *>(first) <= e ? e : nullptr);
+}
+
+constexpr int kPrefetchAhead = 128;  // degrees ahead of the sweep
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
@@ -58,18 +79,19 @@ __device__ __forceinline__ void seed(double e0, double &q, int &E) {
 // recurrence is computed once per ring and reused for every transform. NEG: m > 0, so the order
 // -m column is accumulated too.
 template <int NB, bool NEG>
-__device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *const *cp,
-                                          const double2 *const *cm, double x, double q2, int E, int m, int L,
-                                          double (&ar)[NB], double (&ai)[NB], double (&br)[NB], double (&bi)[NB],
-                                          int &Eout) {
+__device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ cp0,
+                                          const double2 *__restrict__ cm0, int64_t LB, double x, double q2, int E,
+                                          int m, int L, double (&ar)[NB], double (&ai)[NB], double (&br)[NB],
+                                          double (&bi)[NB], int &Eout, int *sink) {
+    // transform t's order-m column is cp0 + t LB (row l - m holds degree l), order -m likewise
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-        const double2 a0 = cp[t][m];
+        const double2 a0 = cp0[t * LB + m];
         ar[t] = a0.x * q2;
         ai[t] = a0.y * q2;
         br[t] = bi[t] = 0.0;
         if (NEG) {
-            const double2 c0 = cm[t][m];
+            const double2 c0 = cm0[t * LB + m];
             br[t] = c0.x * q2;
             bi[t] = c0.y * q2;
         }
@@ -79,19 +101,46 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
         q1 = sqrt(double(2 * m + 3)) * x * q2;
 #pragma unroll
         for (int t = 0; t < NB; ++t) {
-            const double2 a1 = cp[t][m + 1];
+            const double2 a1 = cp0[t * LB + m + 1];
             ar[t] = fma(a1.x, q1, ar[t]);
             ai[t] = fma(a1.y, q1, ai[t]);
             if (NEG) {
-                const double2 c1 = cm[t][m + 1];
+                const double2 c1 = cm0[t * LB + m + 1];
                 br[t] = fma(c1.x, q1, br[t]);
                 bi[t] = fma(c1.y, q1, bi[t]);
             }
         }
     }
     int l = m + 2;
+    const int lane = threadIdx.x & 63;
+    // prefetch the first kPrefetchAhead degrees' lines at once, then one 64-degree block per block
+    {
+        const int lp = l;
+        // 16 lines of 128 B cover 128 degrees of one 16-byte stream
+        for (int k0 = 0; k0 < 16 * (1 + 2 * NB); k0 += 64) {
+            const int k = k0 + lane;
+            const void *a = nullptr;
+            if (k < 16) a = line_addr(abm + lp, abm + L, k);
+            else if (k < 16 + 16 * NB) { const int64_t o = int64_t((k - 16) / 16) * LB; a = line_addr(cp0 + o + lp, cp0 + o + L, (k - 16) % 16); }
+            else if (NEG && k < 16 + 32 * NB) { const int64_t o = int64_t((k - 16 - 16 * NB) / 16) * LB; a = line_addr(cm0 + o + lp, cm0 + o + L, (k - 16 - 16 * NB) % 16); }
+            l2_prefetch(a, sink);
+        }
+    }
     while (l <= L) {
         const int lend = min(L, l + kRescaleStride - 1);
+        {
+            const int lp = l + kPrefetchAhead;  // 8 lines per stream for this block's successor
+            for (int k0 = 0; k0 < 8 * (1 + 2 * NB); k0 += 64) {
+                const int k = k0 + lane;
+                const void *a = nullptr;
+                if (lp <= L) {
+                    if (k < 8) a = line_addr(abm + lp, abm + L, k);
+                    else if (k < 8 + 8 * NB) { const int64_t o = int64_t((k - 8) / 8) * LB; a = line_addr(cp0 + o + lp, cp0 + o + L, (k - 8) % 8); }
+                    else if (NEG && k < 8 + 16 * NB) { const int64_t o = int64_t((k - 8 - 8 * NB) / 8) * LB; a = line_addr(cm0 + o + lp, cm0 + o + L, (k - 8 - 8 * NB) % 8); }
+                }
+                l2_prefetch(a, sink);
+            }
+        }
 #pragma unroll 4
         for (; l <= lend; ++l) {
             const double2 c = abm[l];
@@ -100,11 +149,11 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
             q1 = q;
 #pragma unroll
             for (int t = 0; t < NB; ++t) {
-                const double2 a = cp[t][l];
+                const double2 a = cp0[t * LB + l];
                 ar[t] = fma(a.x, q, ar[t]);
                 ai[t] = fma(a.y, q, ai[t]);
                 if (NEG) {
-                    const double2 d = cm[t][l];
+                    const double2 d = cm0[t * LB + l];
                     br[t] = fma(d.x, q, br[t]);
                     bi[t] = fma(d.y, q, bi[t]);
                 }
@@ -152,20 +201,17 @@ ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     if (!active) q2 = 0.0;
 
     const int64_t LB = int64_t(L + 1) * (2 * L + 1);
-    // column of order m (row l - m holds degree l) and of order -m, per transform
-    const double2 *cp[NB];
-    const double2 *cm[NB];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-        cp[t] = alm + (b0 + t) * LB + int64_t(2 * m) * (L + 1) - m;
-        cm[t] = alm + (b0 + t) * LB + int64_t(2 * m - 1) * (L + 1) - m;
-    }
+    // column of order m (row l - m holds degree l) and of order -m of transform b0; transform
+    // b0 + t is LB further on
+    const double2 *__restrict__ cp0 = alm + b0 * LB + int64_t(2 * m) * (L + 1) - m;
+    const double2 *__restrict__ cm0 = alm + b0 * LB + int64_t(2 * m - 1) * (L + 1) - m;
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);  // abm[l] for l = m+2..L
 
+    __shared__ int sink[kFwdMaxWaves][64];  // prefetch target, never read
     double ar[NB], ai[NB], br[NB], bi[NB];
     int Ef;
-    if (m > 0) fwd_sweep<NB, true>(abm, cp, cm, x, q2, E, m, L, ar, ai, br, bi, Ef);
-    else fwd_sweep<NB, false>(abm, cp, cm, x, q2, E, m, L, ar, ai, br, bi, Ef);
+    if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink[wave]);
+    else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink[wave]);
     if (!active) return;
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int64_t ncol = 2 * mmax + 1;
@@ -256,9 +302,13 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
     constexpr int U = NB >= 4 ? 2 : (NB == 2 ? 4 : 8);  // rings per hoisted group in phase 2
 
+    int *sink = reinterpret_cast<int *>(lds_adj + kAdjWaves * kTL * (kTL + 1)) + 64 * wave;
+    // coefficient lines of the first two tiles, then two tiles ahead of the sweep
+    if (lane < 16) l2_prefetch(line_addr(abm + m + 2, abm + L, lane), sink);
     double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
     for (int l0 = m; l0 <= L; l0 += kTL) {
         const int tl = min(kTL, L - l0 + 1);
+        if (lane < 8 && l0 + 2 * kTL <= L) l2_prefetch(line_addr(abm + l0 + 2 * kTL, abm + L, lane), sink);
         // phase 1: lanes = rings, t = degree offset
         int t = 0;
         if (l0 == m) {
@@ -273,7 +323,7 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
                 t = 2;
             }
         }
-#pragma unroll 4
+#pragma unroll 8
         for (; t < tl; ++t) {
             const int l = l0 + t;
             const double2 c = abm[l];
@@ -372,7 +422,7 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
     if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
             dim3 grid(r.naitems);
-            const size_t lds = sizeof(double) * kTL * (kTL + 1) * r.adj_waves;
+            const size_t lds = sizeof(double) * kTL * (kTL + 1) * kAdjWaves + sizeof(int) * 64 * kAdjWaves;
 #define PFSHT_ADJ(NB)                                                                                      \
     ring_adj_kernel<NB><<<grid, 64 * r.adj_waves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab,      \
                                                             r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0)
