@@ -49,26 +49,20 @@ __device__ __forceinline__ void seed(double e0, double &q, int &E) {
     }
 }
 
-// Pulls the cache lines of up to 64 byte ranges into L2 without any register dependence: lane i
-// fetches 4 bytes at addr (when addr is not null) into a wavefront-private LDS sink that is never
-// read, so the load retires asynchronously and no instruction waits on it. Scalar loads of the
-// same lines a block later then hit L2 instead of HBM.
-__device__ __forceinline__ void l2_prefetch(const void *addr, int *sink) {
-    if (addr)
-        __builtin_amdgcn_global_load_lds(addr, (__attribute__((address_space(3))) void *)(sink), 4,
-                                         0, 0);
-}
-
-// Address of the 128-byte line `k` of a stream starting at element `first`, clamped to the
-// stream's last valid element `last` (never past the end of the allocation).
-template <class T>
-__device__ __forceinline__ const void *line_addr(const T *first, const T *last, int k) {
-    const char *a = reinterpret_cast<const char *>(first) + 128 * k;
-    const char *e = reinterpret_cast<const char *>(last);
-    return a <= e ? a : (reinterpret_cast<const char *>(first) <= e ? e : nullptr);
+// L2 prefetch without losing scalar loads. Each lane reads one 4-byte word of a 128-byte line at
+// uniform_base + voff (voff per lane, loop-invariant, so its VGPR is never rewritten while the
+// load is in flight) into a register that is folded into `acc` one block later: the wait for
+// it lands a whole block of FMAs after the issue. A plain load cannot clobber memory, so the
+// compiler still proves the coefficient streams read-only and keeps them on scalar loads.
+__device__ __forceinline__ int l2_touch(const char *base, int voff, bool on) {
+    return on ? *reinterpret_cast<const int *>(base + voff) : 0;
 }
 
 constexpr int kPrefetchAhead = 128;  // degrees ahead of the sweep
+// Value the prefetch accumulator is compared with to keep its loads alive: an XOR of 4-byte words
+// of coefficient data cannot be predicted by the compiler, and a match only rewrites one element
+// of an output the kernel writes anyway.
+constexpr int kNeverSink = 0x7f3c91e5;
 
 }  // namespace
 
@@ -82,7 +76,7 @@ template <int NB, bool NEG>
 __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ cp0,
                                           const double2 *__restrict__ cm0, int64_t LB, double x, double q2, int E,
                                           int m, int L, double (&ar)[NB], double (&ai)[NB], double (&br)[NB],
-                                          double (&bi)[NB], int &Eout, int *sink) {
+                                          double (&bi)[NB], int &Eout, int &sink) {
     // transform t's order-m column is cp0 + t LB (row l - m holds degree l), order -m likewise
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
@@ -113,32 +107,46 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
     }
     int l = m + 2;
     const int lane = threadIdx.x & 63;
-    // prefetch the first kPrefetchAhead degrees' lines at once, then one 64-degree block per block
-    {
-        const int lp = l;
-        // 16 lines of 128 B cover 128 degrees of one 16-byte stream
-        for (int k0 = 0; k0 < 16 * (1 + 2 * NB); k0 += 64) {
-            const int k = k0 + lane;
-            const void *a = nullptr;
-            if (k < 16) a = line_addr(abm + lp, abm + L, k);
-            else if (k < 16 + 16 * NB) { const int64_t o = int64_t((k - 16) / 16) * LB; a = line_addr(cp0 + o + lp, cp0 + o + L, (k - 16) % 16); }
-            else if (NEG && k < 16 + 32 * NB) { const int64_t o = int64_t((k - 16 - 16 * NB) / 16) * LB; a = line_addr(cm0 + o + lp, cm0 + o + L, (k - 16 - 16 * NB) % 16); }
-            l2_prefetch(a, sink);
+    // Prefetch lanes: 8 lines of 128 B cover a 64-degree block of one 16-byte stream. Stream 0 is
+    // the (a, b) row, streams 1..NB the +m columns and NB+1..2NB the -m columns (NEG only).
+    constexpr int NS = 1 + 2 * NB;
+    constexpr int NCH = (8 * NS + 63) / 64;
+    int voff[NCH];
+    bool isab[NCH], used[NCH];
+    int lstart[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        const int k = 64 * c + lane, st = k >> 3, line = k & 7;
+        lstart[c] = 8 * line;  // degree offset of the line within the block
+        isab[c] = st == 0;
+        used[c] = st < (NEG ? NS : 1 + NB);
+        int64_t o = 128 * line;
+        if (st >= 1 && st <= NB) o += int64_t(st - 1) * LB * 16;
+        if (st > NB) o += int64_t(st - 1 - NB) * LB * 16 + (reinterpret_cast<const char *>(cm0) - reinterpret_cast<const char *>(cp0));
+        voff[c] = int(o);
+    }
+    int pending = 0;  // prefetched words of the previous block, folded into `sink` one block later
+    // block b starts at degree m + 2 + 64 b; prefetch blocks 0 and 1 now, block b + 2 at block b
+    for (int pb = 0; pb < 2; ++pb) {
+        const int lp = l + 64 * pb;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            const bool on = used[c] && lp + lstart[c] <= L;
+            pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
+                                voff[c], on);
         }
     }
     while (l <= L) {
         const int lend = min(L, l + kRescaleStride - 1);
         {
-            const int lp = l + kPrefetchAhead;  // 8 lines per stream for this block's successor
-            for (int k0 = 0; k0 < 8 * (1 + 2 * NB); k0 += 64) {
-                const int k = k0 + lane;
-                const void *a = nullptr;
-                if (lp <= L) {
-                    if (k < 8) a = line_addr(abm + lp, abm + L, k);
-                    else if (k < 8 + 8 * NB) { const int64_t o = int64_t((k - 8) / 8) * LB; a = line_addr(cp0 + o + lp, cp0 + o + L, (k - 8) % 8); }
-                    else if (NEG && k < 8 + 16 * NB) { const int64_t o = int64_t((k - 8 - 8 * NB) / 8) * LB; a = line_addr(cm0 + o + lp, cm0 + o + L, (k - 8 - 8 * NB) % 8); }
-                }
-                l2_prefetch(a, sink);
+            sink ^= pending;  // the previous block's prefetch has long completed
+            pending = 0;
+            const int lp = l + kPrefetchAhead;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                const bool on = used[c] && lp + lstart[c] <= L;
+                pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
+                                    voff[c], on);
             }
         }
 #pragma unroll 4
@@ -175,6 +183,7 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
             E += s;
         }
     }
+    sink ^= pending;
     Eout = E;
 }
 
@@ -183,7 +192,8 @@ __global__ void __launch_bounds__(64 * kFwdMaxWaves)
 ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
-                const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax, int b0) {
+                const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax, int b0,
+                int never) {
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
@@ -207,11 +217,11 @@ ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const double2 *__restrict__ cm0 = alm + b0 * LB + int64_t(2 * m - 1) * (L + 1) - m;
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);  // abm[l] for l = m+2..L
 
-    __shared__ int sink[kFwdMaxWaves][64];  // prefetch target, never read
     double ar[NB], ai[NB], br[NB], bi[NB];
-    int Ef;
-    if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink[wave]);
-    else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink[wave]);
+    int Ef, sink = 0;
+    if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink);
+    else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink);
+    if (sink == never) Gd[j] = make_double2(0.0, 0.0);  // keeps the prefetch loads; never true
     if (!active) return;
     const double sg = (m & 1) ? -1.0 : 1.0;
     const int64_t ncol = 2 * mmax + 1;
@@ -242,7 +252,7 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
     for_transform_groups(r.B, [&](int nb, int b0) {
         dim3 grid(r.nitems);
 #define PFSHT_FWD(NB) \
-    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0)
+    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0, kNeverSink)
         switch (nb) {
             case 8: PFSHT_FWD(8); break;
             case 4: PFSHT_FWD(4); break;
@@ -268,7 +278,7 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
                 const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
-                double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0) {
+                double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never) {
     extern __shared__ double lds_adj[];
     const RingItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
@@ -302,13 +312,17 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
     constexpr int U = NB >= 4 ? 2 : (NB == 2 ? 4 : 8);  // rings per hoisted group in phase 2
 
-    int *sink = reinterpret_cast<int *>(lds_adj + kAdjWaves * kTL * (kTL + 1)) + 64 * wave;
-    // coefficient lines of the first two tiles, then two tiles ahead of the sweep
-    if (lane < 16) l2_prefetch(line_addr(abm + m + 2, abm + L, lane), sink);
+    // (a, b) lines of the first two tiles, then two tiles ahead of the sweep: lane i < 16 owns the
+    // line at byte offset 128 i of a 128-degree window
+    const int pvoff = 128 * (lane & 15);
+    int sink = 0, pending = l2_touch(reinterpret_cast<const char *>(abm + m + 2), pvoff,
+                                     lane < 16 && m + 2 + 8 * lane <= L);
     double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
     for (int l0 = m; l0 <= L; l0 += kTL) {
         const int tl = min(kTL, L - l0 + 1);
-        if (lane < 8 && l0 + 2 * kTL <= L) l2_prefetch(line_addr(abm + l0 + 2 * kTL, abm + L, lane), sink);
+        sink ^= pending;
+        const int lp = l0 + 2 * kTL;
+        pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= L);
         // phase 1: lanes = rings, t = degree offset
         int t = 0;
         if (l0 == m) {
@@ -385,6 +399,7 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
         }
         wave_lds_order();
     }
+    if ((sink ^ pending) == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true
 }
 
 // Writes every alm slot: the tile-ordered sum of the partials where (l, m) is retained, zero in
@@ -422,10 +437,10 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
     if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
             dim3 grid(r.naitems);
-            const size_t lds = sizeof(double) * kTL * (kTL + 1) * kAdjWaves + sizeof(int) * 64 * kAdjWaves;
+            const size_t lds = sizeof(double) * kTL * (kTL + 1) * kAdjWaves;
 #define PFSHT_ADJ(NB)                                                                                      \
     ring_adj_kernel<NB><<<grid, 64 * r.adj_waves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab,      \
-                                                            r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0)
+                                                            r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0, kNeverSink)
             switch (nb) {
                 case 8: PFSHT_ADJ(8); break;
                 case 4: PFSHT_ADJ(4); break;
