@@ -126,7 +126,7 @@ using namespace pfsht;
 struct pfsht_rings_s {
     int dev = 0;
     RingDev d;
-    std::vector<int> jlo, jhi;
+    std::vector<int> jlo, jhi, mlim;
     int64_t work = 0;
     int64_t bytes = 0;
     double2 *Gd = nullptr;                    // [B][2 mmax + 1][n]
@@ -161,6 +161,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     }
     std::string msg;
     if (!nested_ranges(mlim, L, r.jlo, r.jhi, msg)) { set_error(msg); return kEArg; }
+    r.mlim = mlim;
     std::vector<double> log2c;
     sectorial_log2c(L, log2c);
     RingDev &d = r.d;
@@ -261,6 +262,9 @@ struct pfsht_plan_s {
     double2 *Fp = nullptr, *Ft = nullptr, *U = nullptr;
     FftPlan phi_bwd, phi_fwd, grid_fwd, grid_bwd, fine_bwd, fine_fwd;
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
+    // Direct leaf: theta-sorted points as rings
+    double *dphi = nullptr;
+    int32_t *dperm = nullptr, *dmlim = nullptr;
     int64_t bytes = 0;
     std::mutex mu;
     // timing
@@ -276,6 +280,7 @@ struct pfsht_plan_s {
         dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c1); dfree(nu.c2);
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
         dfree(Fp); dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
+        dfree(dphi); dfree(dperm); dfree(dmlim);
         fft_free(phi_bwd); fft_free(phi_fwd); fft_free(grid_fwd); fft_free(grid_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
@@ -303,9 +308,26 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     }
     p.method = method;
     if (method == PFSHT_METHOD_DIRECT) {
-        set_error("the Direct leaf (src/Direct.jl) is not available on the device in this build; the stretched "
-                  "construction is degenerate or more expensive for this point set");
-        return kEUnsupported;
+        // DirectLeaf (src/Direct.jl:241-267): points sorted by colatitude (stable), ring engine at
+        // eps = 0 so polar_mmax keeps every order that is not below floatmin.
+        std::vector<int32_t> perm(M);
+        for (int64_t i = 0; i < M; ++i) perm[i] = int32_t(i);
+        std::stable_sort(perm.begin(), perm.end(), [theta](int32_t a, int32_t b) { return theta[a] < theta[b]; });
+        std::vector<double> ths(M), phs(M);
+        for (int64_t i = 0; i < M; ++i) { ths[i] = theta[perm[i]]; phs[i] = phi[perm[i]]; }
+        p.rings = new pfsht_rings_s;
+        Status st = rings_build(ths.data(), int(M), L, 0.0, B, false, *p.rings);
+        if (st != kOk) return st;
+        p.mmax = p.rings->d.mmax;
+        p.nh = int(M);
+        std::vector<int32_t> ml(p.rings->mlim.begin(), p.rings->mlim.end());
+        int64_t &by = p.bytes;
+        if ((st = upload(&p.dphi, phs, by)) != kOk) return st;
+        if ((st = upload(&p.dperm, perm, by)) != kOk) return st;
+        if ((st = upload(&p.dmlim, ml, by)) != kOk) return st;
+        p.geom.kind = -1;
+        PFSHT_HIP_TRY(hipDeviceSynchronize());
+        return kOk;
     }
     if (method != PFSHT_METHOD_STRETCH) { set_error("unknown method"); return kEArg; }
     // geometry and southern reflection (src/Stretch.jl:151-158)
@@ -480,6 +502,16 @@ static void timing_drain(pfsht_plan_s &p) {
 static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
+    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:276-302
+        pfsht_rings_s &r = *p.rings;
+        for (int k = 0; k <= 0; ++k) tmark(p, t, k, s);
+        launch_ring_fwd(r.d, alm, r.Gd, s);
+        for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
+        launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, p.B, s);
+        tmark(p, t, 8, s);
+        PFSHT_HIP_TRY(hipGetLastError());
+        return kOk;
+    }
     const GridDims g = dims_of(p);
     pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
@@ -509,6 +541,16 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
 static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 1);
+    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:310-333
+        pfsht_rings_s &r = *p.rings;
+        tmark(p, t, 0, s);
+        launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);
+        for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
+        launch_ring_adj(r.d, r.Gd, alm, s);
+        tmark(p, t, 8, s);
+        PFSHT_HIP_TRY(hipGetLastError());
+        return kOk;
+    }
     const GridDims g = dims_of(p);
     pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
@@ -580,7 +622,7 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     if (!p || !info) { set_error("NULL argument"); return kEArg; }
     std::memset(info, 0, sizeof *info);
     info->method = p->method;
-    info->kind = p->geom.kind;
+    info->kind = p->method == PFSHT_METHOD_DIRECT ? -1 : p->geom.kind;
     info->flip = p->flip;
     info->lmax = p->L;
     info->ntrans = p->B;
@@ -649,6 +691,7 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
 int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, int32_t where, void *stream) {
     if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
     if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }
+    if (p->method != PFSHT_METHOD_STRETCH) { set_error("a Direct plan has no NUFFT"); return kEArg; }
     if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard dg(p->dev);

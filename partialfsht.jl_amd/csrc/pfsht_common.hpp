@@ -73,6 +73,12 @@ void launch_unpack_window(const GridDims &g, const double2 *Fp, const double *wt
 void launch_dfs_double(const GridDims &g, const double2 *Fp, double2 *Ft, hipStream_t s);
 void launch_dfs_fold(const GridDims &g, const double2 *Ft, double2 *Fp, hipStream_t s);
 
+// Direct leaf: phase sums over the ring-engine output of theta-sorted points.
+void launch_direct_synth(const double2 *Gd, const double *phis, const int32_t *mlim, const int32_t *perm,
+                         double2 *f, int64_t M, int mmax, int B, hipStream_t s);
+void launch_direct_adjoint(const double2 *f, const double *phis, const int32_t *mlim, const int32_t *perm,
+                           double2 *Gd, int64_t M, int mmax, int B, hipStream_t s);
+
 // ES NUFFT on the stretched torus.
 struct NufftDev {
     int B = 1, n1 = 0, n2 = 0;     // modes (nth, nph)

@@ -54,7 +54,7 @@ class InfoC(C.Structure):
 NSTAGES = 8
 STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft"]
 METHODS = {"auto": 0, "direct": 1, "stretch": 2}
-KINDS = {0: "pole", 1: "interior", 2: "full"}
+KINDS = {0: "pole", 1: "interior", 2: "full", -1: "direct"}
 WHERE_HOST, WHERE_DEVICE = 0, 1
 
 EXPORTS = {
@@ -232,8 +232,14 @@ class Plan:
             pass
 
     def __repr__(self):
+        alg = "DirectAlgorithm" if self.info["method"] == 1 else "StretchAlgorithm"
         return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, "
-                f"algorithm=StretchAlgorithm, kind={self.info['kind_name']})")
+                f"algorithm={alg}, kind={self.info['kind_name']})")
+
+    @property
+    def algorithm(self) -> str:
+        """algorithm(plan) — src/Plan.jl:186: "direct" or "stretch"."""
+        return "direct" if self.info["method"] == 1 else "stretch"
 
     @property
     def H(self) -> "AdjointPlan":

@@ -240,6 +240,52 @@ def test_refused_geometry(torch, P, rng):
 
 
 # --------------------------------------------------------------------------------------------
+# Direct leaf on the device (src/Direct.jl; test_transforms.jl:1-57)
+# --------------------------------------------------------------------------------------------
+def test_direct_leaf_vs_oracle(torch, P, O, rng):
+    # whole-sphere points with both poles, the equator and repeated colatitudes, B = 2
+    L, B, n = 128, 2, 600
+    th = np.arccos(2 * rng.uniform(size=n) - 1)
+    th[:4] = [0.0, np.pi, np.pi / 2, np.pi / 2]
+    th[4:8] = th[8]
+    ph = 2 * np.pi * rng.uniform(size=n)
+    pg = P.Plan(th, ph, lmax=L, ntrans=B, method="direct")
+    po = O.Plan(th, ph, L, ntrans=B, method="direct", nthreads=NT)
+    assert pg.algorithm == "direct" and pg.info["kind_name"] == "direct"
+    assert pg.info["mmax"] == L
+    a = O.rand_alm_fast(L, rng, ntrans=B)
+    fg = pg @ a
+    fo = po.synth(a)
+    assert relmax(fg, fo) < 1e-12
+    f = np.asfortranarray(rng.standard_normal((n, B)) + 1j * rng.standard_normal((n, B)))
+    ag = pg.H @ f
+    ao = po.adjoint(f)
+    for b in range(B):
+        assert rel_modes(O, ag[:, :, b], ao[:, :, b], L) < 1e-12
+        lhs = np.vdot(f[:, b], fg[:, b])
+        assert abs(lhs - np.vdot(ag[:, :, b], a[:, :, b])) / abs(lhs) < 1e-12
+    assert np.all(ag[~O.valid_mask(L)] == 0)
+
+
+def test_auto_selection_matches_oracle(torch, P, O, rng):
+    # src/Plan.jl:234-266 with src/Cost.jl:174-201: a tiny set goes Direct, a cap the stretch
+    # refuses goes Direct, a large cap goes Stretch; the device plan agrees with the oracle's
+    # planner and with its Direct results.
+    cases = [(64, 2, 20), (256, 5, 2000), (256, 20, 30000)]
+    for L, deg, n in cases:
+        th, ph = cap_points(rng, deg, n)
+        pg = P.Plan(th, ph, lmax=L, tol=1e-8)
+        po = O.Plan(th, ph, L, tol=1e-8, nthreads=NT)
+        assert pg.algorithm == po.method, (L, deg, n)
+    th, ph = cap_points(rng, 5, 2000)
+    pg = P.Plan(th, ph, lmax=256, tol=1e-8)
+    assert pg.algorithm == "direct"
+    a = O.rand_alm_fast(256, rng)
+    pd = O.Plan(th, ph, 256, method="direct", nthreads=NT)
+    assert relmax(pg @ a, pd.synth(a)) < 1e-12
+
+
+# --------------------------------------------------------------------------------------------
 # Full size: lmax 2048, M = 2e6 (the benchmark workload)
 # --------------------------------------------------------------------------------------------
 def test_full_size_config2(torch, P, O, rng):
