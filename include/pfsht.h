@@ -21,7 +21,7 @@ extern "C" {
 #endif
 
 /* Error codes: the Julia extension maps E_DIM to DimensionMismatch, E_ARG / E_GEOM_REFUSED to
- * ArgumentError (src/Layout.jl:35-56; src/Plan.jl:248-252; src/Windows.jl:66-74). */
+ * ArgumentError (src/Layout.jl:35-56; src/Plan.jl:98-102; src/Windows.jl:66-74). */
 enum {
     PFSHT_OK = 0,
     PFSHT_E_DIM = 1,
@@ -70,20 +70,20 @@ typedef struct {
 const char *pfsht_last_error(void);
 const char *pfsht_version(void);
 
-/* Plan(theta, phi; lmax, tol, ntrans, method) — replaces the constructor of src/Plan.jl:234-266
- * with the StretchLeaf of src/Stretch.jl:129-248 (or DirectLeaf, src/Direct.jl:241-267) built on
+/* Plan(theta, phi; lmax, tol, ntrans, method) — replaces the constructor of src/Plan.jl:84-116
+ * with the StretchLeaf of src/Stretch.jl:129-248 (or DirectLeaf, src/Direct.jl:25-51) built on
  * the device. theta, phi: host arrays of npts colatitudes in [0, pi] and longitudes. */
 int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
                       int32_t ntrans, int32_t method, int32_t precision, pfsht_plan_t *out);
 void pfsht_plan_destroy(pfsht_plan_t plan);
 int pfsht_plan_info(pfsht_plan_t plan, pfsht_info *info);
 
-/* mul!(f, plan, alm) — src/Plan.jl:287-291 -> synthesize_stretch! (src/Stretch.jl:262-277).
+/* mul!(f, plan, alm) — src/Plan.jl:137-141 -> synthesize_stretch! (src/Stretch.jl:262-277).
  * Overwrites f. stream: a hipStream_t or NULL for the null stream; the call is asynchronous
  * on that stream when where == PFSHT_WHERE_DEVICE. */
 int pfsht_synth(pfsht_plan_t plan, const void *alm, void *f, int32_t where, void *stream);
 
-/* mul!(alm, plan', f) — src/Plan.jl:293-298 -> adjoint_stretch! (src/Stretch.jl:279-295).
+/* mul!(alm, plan', f) — src/Plan.jl:143-148 -> adjoint_stretch! (src/Stretch.jl:279-295).
  * Overwrites alm, padding slots included (src/Legendre.jl:338). */
 int pfsht_adjoint(pfsht_plan_t plan, const void *f, void *alm, int32_t where, void *stream);
 

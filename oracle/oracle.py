@@ -331,7 +331,7 @@ NUFFTS = {"es": 0, "direct": 1}
 
 class Plan:
     """Plan(theta, phi; lmax, tol, ntrans, method) with synth (mul!(f, plan, alm)) and
-    adjoint (mul!(alm, plan', f)) — Plan.jl:234-298."""
+    adjoint (mul!(alm, plan', f)) — Plan.jl:84-148."""
 
     def __init__(self, theta, phi, lmax, tol=1e-10, ntrans=1, method="auto", nufft="es",
                  nthreads=1):

@@ -11,10 +11,10 @@
  *
  * Deliberate deviations from the reference, each tolerance-neutral:
  *  - the order-0 seed exponent is log2c[0] even on a ring exactly at a pole, as in `alf`
- *    (Legendre.jl:565) and the device kernel (KAExt.jl:152); the host ring engine
+ *    (Legendre.jl:565) and the device kernel (ext/PartialFSHTKernelAbstractionsExt.jl:96,179); the host ring engine
  *    (Legendre.jl:381,478) would form 0*(-Inf) = NaN there;
  *  - @simd reductions in the adjoint (Legendre.jl:437-443,506-509) are summed in ring order;
- *  - the NUFFT is either the reference's own direct sum (NUFFT.jl:155-199) or an
+ *  - the NUFFT is either the reference's own direct sum (NUFFT.jl:85-129) or an
  *    exponential-of-semicircle NUFFT written from the published method (Barnett, Magland &
  *    af Klinteberg, SISC 41, 2019), standing in for FINUFFT, which is not vendored;
  *  - the planner compares Direct against Stretch only: the multi-box partition
@@ -793,7 +793,7 @@ void or_fft2(or_cplx *x, int n1, int n2, int B, int sign, int nthreads) {
 }
 
 /* ---------------------------------------------------------------------------------------------
- * NUFFT.jl:144-153 (_mode_k, modeord 1) and the direct sum NUFFT.jl:155-199.
+ * NUFFT.jl:74-83 (_mode_k, modeord 1) and the direct sum NUFFT.jl:85-129.
  * ------------------------------------------------------------------------------------------- */
 static inline int mode_k(int i0, int n) { /* 0-based i0 */
     int k = i0;
@@ -1028,7 +1028,7 @@ int or_nufft_es(int type, int n1, int n2, int iflag, int B, const double *x, con
 }
 
 /* ---------------------------------------------------------------------------------------------
- * Plan (Plan.jl:234-266), planner (Cost.jl:104-140,174-201), leaves (Stretch.jl, Direct.jl).
+ * Plan (Plan.jl:84-116), planner (Cost.jl:104-140,174-201), leaves (Stretch.jl, Direct.jl).
  * ------------------------------------------------------------------------------------------- */
 struct or_plan {
     int L, B, method, nufft_kind, nthreads;
@@ -1130,7 +1130,7 @@ or_plan *or_plan_create(const double *theta, const double *phi, int64_t M, int L
         method = cs < cd ? 2 : 1;
     }
     p->method = method;
-    if (method == 1) { /* Direct.jl:241-267 */
+    if (method == 1) { /* Direct.jl:25-51 */
         p->perm = malloc(sizeof(int64_t) * M);
         int64_t *tmp = malloc(sizeof(int64_t) * M);
         for (int64_t i = 0; i < M; i++) p->perm[i] = i;
@@ -1239,7 +1239,7 @@ static int run_nufft(or_plan *p, int type, int iflag, const or_cplx *in, or_cplx
     return or_nufft_es(type, p->nth, p->nph, iflag, p->B, p->Tpts, p->ppts, p->M, p->tol, in, out, p->nthreads);
 }
 
-/* Stretch.jl:262-277 / Direct.jl:276-302 */
+/* Stretch.jl:262-277 / Direct.jl:60-91 */
 int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     int L = p->L, B = p->B;
     int64_t M = p->M;
@@ -1292,7 +1292,7 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     return rc;
 }
 
-/* Stretch.jl:279-295 / Direct.jl:310-333 */
+/* Stretch.jl:279-295 / Direct.jl:94-117 */
 int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
     int L = p->L, B = p->B;
     int64_t M = p->M;

@@ -308,7 +308,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     }
     p.method = method;
     if (method == PFSHT_METHOD_DIRECT) {
-        // DirectLeaf (src/Direct.jl:241-267): points sorted by colatitude (stable), ring engine at
+        // DirectLeaf (src/Direct.jl:25-51): points sorted by colatitude (stable), ring engine at
         // eps = 0 so polar_mmax keeps every order that is not below floatmin.
         std::vector<int32_t> perm(M);
         for (int64_t i = 0; i < M; ++i) perm[i] = int32_t(i);
@@ -502,7 +502,7 @@ static void timing_drain(pfsht_plan_s &p) {
 static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
-    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:276-302
+    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
         pfsht_rings_s &r = *p.rings;
         for (int k = 0; k <= 0; ++k) tmark(p, t, k, s);
         launch_ring_fwd(r.d, alm, r.Gd, s);
@@ -541,7 +541,7 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
 static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 1);
-    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:310-333
+    if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:94-117
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
         launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);

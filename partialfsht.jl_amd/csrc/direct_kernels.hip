@@ -7,7 +7,7 @@
 namespace pfsht {
 
 // f[perm[j]] = sum_{|m| <= mlim_j} G[j, m] cis(m phi_j), the phase formed exactly per order rather
-// than by a running product (src/Direct.jl:285-299).
+// than by a running product (src/Direct.jl:69-88).
 __global__ void direct_synth_kernel(const double2 *__restrict__ Gd, const double *__restrict__ phis,
                                     const int32_t *__restrict__ mlim, const int32_t *__restrict__ perm,
                                     double2 *__restrict__ f, int64_t M, int mmax) {
@@ -33,7 +33,7 @@ __global__ void direct_synth_kernel(const double2 *__restrict__ Gd, const double
 }
 
 // G[j, m] = f_j conj(cis(m phi_j)), G[j, -m] = f_j cis(m phi_j) on the orders ring j retains
-// (src/Direct.jl:310-333); alm_from_rings! then reads exactly those entries.
+// (src/Direct.jl:94-117); alm_from_rings! then reads exactly those entries.
 __global__ void direct_adjoint_kernel(const double2 *__restrict__ f, const double *__restrict__ phis,
                                       const int32_t *__restrict__ mlim, const int32_t *__restrict__ perm,
                                       double2 *__restrict__ Gd, int64_t M, int mmax) {

@@ -45,7 +45,7 @@ void stretch_rings(const Geom &g, int n, std::vector<double> &theta, std::vector
 // src/Stretch.jl:52-81
 int64_t next_smooth(int64_t n);
 int64_t next_smooth_even(int64_t n);
-// src/NUFFT.jl:144-153 (modeord 1, 0-based index)
+// src/NUFFT.jl:74-83 (modeord 1, 0-based index)
 inline int mode_k(int i, int n) { return i < (n + 1) / 2 ? i : i - n; }
 
 // Julia's floating mod (result carries the divisor's sign).

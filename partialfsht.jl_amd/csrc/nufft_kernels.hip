@@ -1,7 +1,7 @@
 // nufft_kernels.hip — the exponential-of-semicircle NUFFT on the stretched torus.
 //
-// Synthesis needs a type-2 NUFFT with iflag +1 (src/Stretch.jl:274; definition src/NUFFT.jl:155-176),
-// the adjoint a type-1 NUFFT with iflag -1 (src/Stretch.jl:286; src/NUFFT.jl:177-194). With kernel
+// Synthesis needs a type-2 NUFFT with iflag +1 (src/Stretch.jl:274; definition src/NUFFT.jl:85-106),
+// the adjoint a type-1 NUFFT with iflag -1 (src/Stretch.jl:286; src/NUFFT.jl:107-124). With kernel
 // psi(x) = phi(x / alpha), phi(z) = exp(beta (sqrt(1 - z^2) - 1)), alpha = w h / 2, h = 2 pi / N:
 //   type 2:  u = IFFT_N( c_k * h/psihat(k) zero-padded ),  f_j = sum_{w x w cells} psi psi u;
 //   type 1:  exactly the transpose — spread, FFT_N, multiply by h/psihat(k) and truncate.

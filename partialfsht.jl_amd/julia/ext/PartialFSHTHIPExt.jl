@@ -59,7 +59,7 @@ end
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
 
 # Status codes map onto the exceptions the host path throws (src/Layout.jl:35-56;
-# src/Plan.jl:248-252; src/Windows.jl:66-74).
+# src/Plan.jl:98-102; src/Windows.jl:66-74).
 function _check(rc::Cint)
     rc == PFSHT_OK && return nothing
     msg = _last_error()
@@ -102,7 +102,7 @@ end
 PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
 PartialFSHT.ring_work(leaf::HIPStretchLeaf) = Int(leaf.info.ring_work)
 
-# Leaf seam (src/Plan.jl:275-279): a stretched plan on ROCVector nodes builds the device leaf.
+# Leaf seam (src/Plan.jl:125-129): a stretched plan on ROCVector nodes builds the device leaf.
 PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
     spin == 0 ? HIPStretchLeaf(θ, φ, lmax, ntrans, tol) :

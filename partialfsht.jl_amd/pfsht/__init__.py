@@ -1,9 +1,9 @@
 """pfsht — Python host-side mirror of PartialFSHT.jl's operator API over libpfsht_hip.so.
 
     plan = Plan(theta, phi, lmax=..., tol=1e-10, ntrans=1, method="auto")
-    mul(f, plan, alm)          # mul!(f, plan, alm)        src/Plan.jl:287-291
-    mul(alm, plan.H, f)        # mul!(alm, plan', f)       src/Plan.jl:293-298
-    f = plan @ alm ; alm = plan.H @ f                      # Base.:* (src/Plan.jl:340-349)
+    mul(f, plan, alm)          # mul!(f, plan, alm)        src/Plan.jl:137-141
+    mul(alm, plan.H, f)        # mul!(alm, plan', f)       src/Plan.jl:143-148
+    f = plan @ alm ; alm = plan.H @ f                      # Base.:* (src/Plan.jl:190-199)
 
 Arrays are the reference's column-major layouts (src/Layout.jl:13-56). Host arrays are numpy
 complex128 (Fortran order for alm); device arrays are torch complex128 CUDA tensors whose memory
@@ -196,7 +196,7 @@ def _stream_ptr(stream):
 
 
 class Plan:
-    """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:234-266."""
+    """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116."""
 
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
                  method: str = "auto", precision: str = "fp64"):
@@ -275,7 +275,7 @@ class Plan:
         return alm
 
     def __matmul__(self, alm):
-        """plan * alm (src/Plan.jl:340-343): a new field, host or device like alm."""
+        """plan * alm (src/Plan.jl:190-193): a new field, host or device like alm."""
         B = self.ntrans
         if _is_torch(alm):
             import torch

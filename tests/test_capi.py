@@ -45,7 +45,7 @@ def test_version_and_null_argument_errors():
 
 
 def test_host_validation_before_device():
-    # argument checks happen before any device work (src/Plan.jl:248-252)
+    # argument checks happen before any device work (src/Plan.jl:98-102)
     import pfsht
     L = pfsht.lib()
     th = np.zeros(4)

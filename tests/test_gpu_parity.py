@@ -112,7 +112,7 @@ def _plans(P, O, th, ph, L, tol, B=1):
 
 
 def test_nufft_seam_vs_oracle_and_direct_sum(torch, P, O, rng):
-    # _nufft_exec! on the plan's stretched points (src/NUFFT.jl:17-25,155-199): type 2 (iflag +1)
+    # _nufft_exec! on the plan's stretched points (src/NUFFT.jl:17-25,85-129): type 2 (iflag +1)
     # and type 1 (iflag -1) against the oracle's ES NUFFT (same kernel, <1e-12) and against the
     # reference's direct sum (200 tol, test_backends.jl:1-19); and the pair is an exact adjoint.
     L, tol, M = 160, 1e-8, 700
@@ -268,7 +268,7 @@ def test_direct_leaf_vs_oracle(torch, P, O, rng):
 
 
 def test_auto_selection_matches_oracle(torch, P, O, rng):
-    # src/Plan.jl:234-266 with src/Cost.jl:174-201: a tiny set goes Direct, a cap the stretch
+    # src/Plan.jl:84-116 with src/Cost.jl:174-201: a tiny set goes Direct, a cap the stretch
     # refuses goes Direct, a large cap goes Stretch; the device plan agrees with the oracle's
     # planner and with its Direct results.
     cases = [(64, 2, 20), (256, 5, 2000), (256, 20, 30000)]
