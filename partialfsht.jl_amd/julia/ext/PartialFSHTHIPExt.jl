@@ -1,9 +1,10 @@
 """
     PartialFSHTHIPExt
 
-MI355X (gfx950) stretched leaf for `PartialFSHT.Plan`, loaded by `using AMDGPU`. A plan whose
-node arrays are `ROCVector`s builds a `HIPStretchLeaf`: every apply runs on the device through
-`libpfsht_hip.so`'s C ABI (`include/pfsht.h`), with coefficients and fields left in HBM.
+MI355X (gfx950) leaves for `PartialFSHT.Plan`, loaded by `using AMDGPU`. A plan whose node arrays
+are `ROCVector`s builds a `HIPStretchLeaf` (or, when the planner picks Direct, a `HIPDirectLeaf`):
+every apply runs on the device through `libpfsht_hip.so`'s C ABI (`include/pfsht.h`), with
+coefficients and fields left in HBM.
 
     using PartialFSHT, AMDGPU
     plan = PartialFSHT.Plan(ROCArray(θ), ROCArray(φ); lmax = 2048, tol = 1e-8,
@@ -31,6 +32,7 @@ const PFSHT_OK = Cint(0)
 const PFSHT_E_DIM = Cint(1)
 const PFSHT_E_ARG = Cint(2)
 const PFSHT_E_GEOM_REFUSED = Cint(3)
+const PFSHT_METHOD_DIRECT = Cint(1)
 const PFSHT_METHOD_STRETCH = Cint(2)
 const PFSHT_PREC_FP64 = Cint(0)
 const PFSHT_WHERE_DEVICE = Cint(1)
@@ -69,18 +71,27 @@ function _check(rc::Cint)
 end
 
 """
-    HIPStretchLeaf
+    HIPStretchLeaf, HIPDirectLeaf
 
-Handle of a device plan: geometry, ring tables, bin-sorted points, NUFFT kernel tables, rocFFT
-plans and work buffers, all resident in HBM (the device analogue of `StretchLeaf`,
-`src/Stretch.jl:94-127`). Freed by a finalizer.
+Handles of device plans. A stretched plan holds its geometry, ring tables, bin-sorted points, NUFFT
+kernel tables, rocFFT plans and work buffers in HBM (the device analogue of `StretchLeaf`,
+`src/Stretch.jl:94-127`). A Direct plan holds the colatitude-sorted points and the ring engine at
+ε = 0 (`DirectLeaf`, `src/Direct.jl:15-51`). Both are freed by a finalizer.
 """
-mutable struct HIPStretchLeaf
+abstract type AbstractHIPLeaf end
+
+mutable struct HIPStretchLeaf <: AbstractHIPLeaf
     handle::Ptr{Cvoid}
     info::PfshtInfo
 end
 
-function HIPStretchLeaf(θ::AbstractVector, φ::AbstractVector, lmax::Int, ntrans::Int, tol::Real)
+mutable struct HIPDirectLeaf <: AbstractHIPLeaf
+    handle::Ptr{Cvoid}
+    info::PfshtInfo
+end
+
+function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVector, lmax::Int,
+                 ntrans::Int, tol::Real) where {LT<:AbstractHIPLeaf}
     θh = Vector{Float64}(PartialFSHT._host(θ))
     φh = Vector{Float64}(PartialFSHT._host(φ))
     length(θh) == length(φh) || throw(DimensionMismatch("θ and φ must have the same length"))
@@ -88,10 +99,10 @@ function HIPStretchLeaf(θ::AbstractVector, φ::AbstractVector, lmax::Int, ntran
     _check(ccall((:pfsht_plan_create, libpfsht), Cint,
                  (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Int32, Int32,
                   Ref{Ptr{Cvoid}}),
-                 θh, φh, length(θh), lmax, tol, ntrans, PFSHT_METHOD_STRETCH, PFSHT_PREC_FP64, h))
+                 θh, φh, length(θh), lmax, tol, ntrans, method, PFSHT_PREC_FP64, h))
     info = Ref{PfshtInfo}()
     _check(ccall((:pfsht_plan_info, libpfsht), Cint, (Ptr{Cvoid}, Ref{PfshtInfo}), h[], info))
-    leaf = HIPStretchLeaf(h[], info[])
+    leaf = LT(h[], info[])
     finalizer(leaf) do l
         l.handle == C_NULL || ccall((:pfsht_plan_destroy, libpfsht), Cvoid, (Ptr{Cvoid},), l.handle)
         l.handle = C_NULL
@@ -99,13 +110,24 @@ function HIPStretchLeaf(θ::AbstractVector, φ::AbstractVector, lmax::Int, ntran
     return leaf
 end
 
-PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
-PartialFSHT.ring_work(leaf::HIPStretchLeaf) = Int(leaf.info.ring_work)
+HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real) =
+    _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol)
+HIPDirectLeaf(θ, φ, lmax::Int, ntrans::Int) =
+    _create(HIPDirectLeaf, PFSHT_METHOD_DIRECT, θ, φ, lmax, ntrans, 1e-10)
 
-# Leaf seam (src/Plan.jl:125-129): a stretched plan on ROCVector nodes builds the device leaf.
+PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
+PartialFSHT.algorithm(::HIPDirectLeaf) = PartialFSHT.DirectAlgorithm()
+PartialFSHT.ring_work(leaf::AbstractHIPLeaf) = Int(leaf.info.ring_work)
+
+# Leaf seam (src/Plan.jl:118-129): a plan on ROCVector nodes builds the device leaf of the chosen
+# algorithm.
 PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
     spin == 0 ? HIPStretchLeaf(θ, φ, lmax, ntrans, tol) :
+    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+PartialFSHT._build_leaf(::PartialFSHT.DirectAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
+                        spin, tol, nufft_b, opts) =
+    spin == 0 ? HIPDirectLeaf(θ, φ, lmax, ntrans) :
     throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
 
 _stream() = AMDGPU.stream().stream   # hipStream_t of the task-local AMDGPU stream
@@ -114,9 +136,10 @@ function _device_ptr(x::ROCArray{ComplexF64})
     return reinterpret(Ptr{Cvoid}, pointer(x))
 end
 
-# src/Plan.jl:181-188 → synthesize_stretch!/adjoint_stretch! (src/Stretch.jl:262-295)
+# src/Plan.jl:181-188 → synthesize_stretch!/adjoint_stretch! (src/Stretch.jl:262-295) or
+# synthesize_direct!/adjoint_direct! (src/Direct.jl:60-117)
 function PartialFSHT._synthesize!(f::ROCArray{ComplexF64}, alm::ROCArray{ComplexF64},
-                                  leaf::HIPStretchLeaf, ::PartialFSHT.Plan)
+                                  leaf::AbstractHIPLeaf, ::PartialFSHT.Plan)
     _check(ccall((:pfsht_synth, libpfsht), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
                  leaf.handle, _device_ptr(alm), _device_ptr(f), PFSHT_WHERE_DEVICE, _stream()))
@@ -124,7 +147,7 @@ function PartialFSHT._synthesize!(f::ROCArray{ComplexF64}, alm::ROCArray{Complex
 end
 
 function PartialFSHT._adjoint_apply!(alm::ROCArray{ComplexF64}, f::ROCArray{ComplexF64},
-                                     leaf::HIPStretchLeaf, ::PartialFSHT.Plan)
+                                     leaf::AbstractHIPLeaf, ::PartialFSHT.Plan)
     _check(ccall((:pfsht_adjoint, libpfsht), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
                  leaf.handle, _device_ptr(f), _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream()))
@@ -133,9 +156,9 @@ end
 
 # A host array handed to a device leaf is refused, as the reference refuses the converse
 # (ext/PartialFSHTKernelAbstractionsExt.jl:266-271).
-PartialFSHT._synthesize!(f, alm, ::HIPStretchLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
+PartialFSHT._synthesize!(f, alm, ::AbstractHIPLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
     "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
-PartialFSHT._adjoint_apply!(alm, f, ::HIPStretchLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
+PartialFSHT._adjoint_apply!(alm, f, ::AbstractHIPLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
     "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
 
 end
