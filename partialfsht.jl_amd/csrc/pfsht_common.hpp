@@ -55,7 +55,7 @@ struct RingDev {
 };
 
 constexpr int kFwdMaxWaves = 8;   // forward block: up to 512 rings of one order
-constexpr int kAdjWaves = 4;      // adjoint block: LDS holds one 64 x 65 FP64 tile per wavefront
+constexpr int kAdjWaves = 4;      // adjoint block: up to 4 tiles of an order, one 16 x 64 LDS tile each
 
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
 // Gd is the internal ring buffer [b][m + mmax][j].

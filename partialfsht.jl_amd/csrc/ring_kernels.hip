@@ -17,10 +17,10 @@
 // Pbar_l^m = (-1)^m q 2^E, so G[j, m] = (-1)^m sum a_lm q 2^E and G[j, -m] = sum a_l,-m q 2^E.
 //
 // Adjoint: alm[l, m] = sum_j Pbar_l^m(x_j) G[j, m] is a reduction across lanes. Each wavefront
-// computes a 64-degree tile of w = q 2^E with lanes over rings, transposes it through LDS, and
-// then lets lanes run over degrees, summing the tile's rings in ring order with the ring data as
-// scalar operands. Per-tile partials go to a plan-owned buffer and a finalize kernel adds the
-// tiles of each order in tile order: no float atomics, bit-identical run to run.
+// computes 16-degree sub-tiles of w = q 2^E with lanes over rings, transposes them through LDS,
+// and sums them against G with row-broadcast FMAs (see the adjoint section). Per-tile partials go
+// to a plan-owned buffer and a finalize kernel adds the tiles of each order in tile order: no
+// float atomics, bit-identical run to run.
 #include "pfsht_common.hpp"
 
 namespace pfsht {
@@ -266,11 +266,159 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
 // ------------------------------------------------------------------------------------------------
 // Adjoint
 // ------------------------------------------------------------------------------------------------
-constexpr int kTL = 64;  // degrees per LDS tile (one 64 x 65 FP64 tile per wavefront)
+// alm[l, m] = sum_j Pbar_l^m(x_j) G[j, m] reduces across the rings a wavefront holds in its lanes.
+// The wavefront walks the degrees in sub-tiles of 16:
+//   phase 1 (lanes = rings) advances the recurrence 16 degrees and stores w = q 2^E to a 16 x 64
+//           LDS tile;
+//   phase 2 (lane = 16 row + t) gives each 16-lane row 16 of the tile's rings: lane t of row r
+//           sums w[t][16 r + k] G[16 r + k] over k, the G operand taken from lane k of its own row
+//           by a DPP row broadcast on the FMA (G stays in the registers it was loaded into);
+//   the four row sums are then added across rows ((r0 + r2) + (r1 + r3)) with lane swaps.
+// The tile is 8.6 KB per wavefront, so four wavefronts share a SIMD and hide each other's LDS and
+// scalar-load latency. Sums have a fixed order: bit-identical run to run.
+constexpr int kSub = 16;  // degrees per LDS sub-tile
+constexpr int kWS = 69;   // LDS row stride in doubles; ring rho sits at column rho + rho / 16
 
 // Wavefront-private ordering of LDS traffic: a wavefront's LDS operations complete in issue
 // order, so a wavefront reading back its own tile needs only a compiler barrier.
 __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// a_c += g_c(lane 16 row + K) * w for the (up to) four components of one transform. The DPP source
+// must not have been written by a VALU instruction in the two preceding cycles: s_nop 1.
+template <int K, bool NEG>
+__device__ __forceinline__ void fmac_row_bcast(double (&a)[4], const double (&g)[4], double w) {
+    if (NEG) {
+        asm("s_nop 1\n\t"
+            "v_fmac_f64_dpp %0, %4, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %5, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %2, %6, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %3, %7, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf"
+            : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3])
+            : "v"(g[0]), "v"(g[1]), "v"(g[2]), "v"(g[3]), "v"(w), "n"(K));
+    } else {
+        asm("s_nop 1\n\t"
+            "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+            : "+v"(a[0]), "+v"(a[1])
+            : "v"(g[0]), "v"(g[1]), "v"(w), "n"(K));
+    }
+}
+
+template <int K, int NB, bool NEG>
+struct RowSweep {
+    static __device__ __forceinline__ void run(const double *__restrict__ wr, double (&a)[NB][4],
+                                               const double (&g)[NB][4]) {
+        const double w = wr[K];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) fmac_row_bcast<K, NEG>(a[u], g[u], w);
+        RowSweep<K + 1, NB, NEG>::run(wr, a, g);
+    }
+};
+template <int NB, bool NEG>
+struct RowSweep<kSub, NB, NEG> {
+    static __device__ __forceinline__ void run(const double *__restrict__, double (&)[NB][4], const double (&)[NB][4]) {}
+};
+
+// Sum of v over the four 16-lane rows, the same value (bitwise) in every lane: lane swaps across
+// the wave halves and then across row pairs; each stage adds the lower partner first.
+__device__ __forceinline__ double row_sum4(double v) {
+    unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    v = __hiloint2double(int(b[0]), int(a[0])) + __hiloint2double(int(b[1]), int(a[1]));
+    lo = __double2loint(v);
+    hi = __double2hiint(v);
+    a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return __hiloint2double(int(b[0]), int(a[0])) + __hiloint2double(int(b[1]), int(a[1]));
+}
+
+template <int NB, bool NEG>
+__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ gpc,
+                                          const double2 *__restrict__ gmc, int64_t gstride, int nr, double x,
+                                          double qs, int E, int m, int L, int B, double (*W)[kWS],
+                                          double4 *__restrict__ out, int &sink) {
+    const int lane = threadIdx.x & 63;
+    const bool active = lane < nr;
+    // this lane's ring (phase-1 layout) for every transform: (re, im) of orders +m and -m
+    double g[NB][4];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const double2 p = active ? gpc[u * gstride + lane] : make_double2(0.0, 0.0);
+        const double2 q = (NEG && active) ? gmc[u * gstride + lane] : make_double2(0.0, 0.0);
+        g[u][0] = p.x;
+        g[u][1] = p.y;
+        g[u][2] = q.x;
+        g[u][3] = q.y;
+    }
+    const int t = lane & 15, row = lane >> 4;
+    const int wcol = lane + (lane >> 4);
+    const double *__restrict__ wr = &W[t][17 * row];
+
+    // (a, b) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
+    // i < 16 owns the line at byte offset 128 i of a 128-degree window
+    const int pvoff = 128 * (lane & 15);
+    int pending = l2_touch(reinterpret_cast<const char *>(abm + m + 2), pvoff, lane < 16 && m + 2 + 8 * lane <= L);
+    double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
+    for (int l0 = m; l0 <= L; l0 += kSub) {
+        const int tl = min(kSub, L - l0 + 1);
+        if (((l0 - m) & 63) == 0) {
+            sink ^= pending;
+            const int lp = l0 + 128;
+            pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= L);
+        }
+        // phase 1: lanes = rings
+        int tt = 0;
+        if (l0 == m) {
+            q1 = qs;
+            W[0][wcol] = ldexp(qs, E);
+            tt = 1;
+            if (m < L) {
+                const double q = sqrt(double(2 * m + 3)) * x * qs;
+                q2 = q1;
+                q1 = q;
+                W[1][wcol] = ldexp(q, E);
+                tt = 2;
+            }
+        }
+#pragma unroll 8
+        for (; tt < tl; ++tt) {
+            const double2 c = abm[l0 + tt];
+            const double q = fma(c.x * x, q1, -(c.y * q2));
+            q2 = q1;
+            q1 = q;
+            W[tt][wcol] = ldexp(q, E);
+        }
+        if (((l0 + kSub - m) & 63) == 0) {  // renormalise every 64 degrees
+            const int s = renorm_shift(q1, q2);
+            q1 = ldexp(q1, -s);
+            q2 = ldexp(q2, -s);
+            E += s;
+        }
+        wave_lds_order();
+        // phase 2: lane t of row r sums rings 16 r .. 16 r + 15 of degree l0 + t
+        double a[NB][4];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) a[u][0] = a[u][1] = a[u][2] = a[u][3] = 0.0;
+        RowSweep<0, NB, NEG>::run(wr, a, g);
+        wave_lds_order();
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            a[u][0] = row_sum4(a[u][0]);
+            a[u][1] = row_sum4(a[u][1]);
+            if (NEG) {
+                a[u][2] = row_sum4(a[u][2]);
+                a[u][3] = row_sum4(a[u][3]);
+            }
+        }
+        if (row == 0 && t < tl) {
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+                out[int64_t(l0 - m + t) * B + u] = make_double4(a[u][0], a[u][1], NEG ? a[u][2] : 0.0, NEG ? a[u][3] : 0.0);
+        }
+    }
+    sink ^= pending;
+}
 
 template <int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
@@ -285,121 +433,30 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int nr = min(64, __builtin_amdgcn_readfirstlane(it.nr) - 64 * wave);  // rings of this tile
-    if (nr <= 0) return;
+    if (nr <= 0) return;  // wave-uniform: the whole wavefront leaves
     const int tile = __builtin_amdgcn_readfirstlane(it.tile0) + wave;
     const int jt = it.j0 + 64 * wave;  // first ring of the tile
-    double(*W)[kTL + 1] = reinterpret_cast<double(*)[kTL + 1]>(lds_adj + wave * kTL * (kTL + 1));
-    const bool active = lane < nr;
-    const int j = jt + (active ? lane : 0);
+    double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds_adj + wave * kSub * kWS);
+    const int j = jt + (lane < nr ? lane : 0);
 
     const double x = costh[j];
     const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
     double qs;
     int E;
     seed(e0, qs, E);
-    if (!active) qs = 0.0;
+    if (lane >= nr) qs = 0.0;
 
     const int64_t ncol = 2 * mmax + 1;
-    const double2 *gpc[NB];
-    const double2 *gmc[NB];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-        gpc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + jt;
-        gmc[t] = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + jt;
-    }
+    const int64_t gstride = ncol * n;  // next transform
+    const double2 *gpc = Gd + (int64_t(b0) * ncol + (mmax + m)) * n + jt;
+    const double2 *gmc = Gd + (int64_t(b0) * ncol + (mmax - m)) * n + jt;
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
     const int nl = L - m + 1;
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
-    constexpr int U = NB >= 4 ? 2 : (NB == 2 ? 4 : 8);  // rings per hoisted group in phase 2
-
-    // (a, b) lines of the first two tiles, then two tiles ahead of the sweep: lane i < 16 owns the
-    // line at byte offset 128 i of a 128-degree window
-    const int pvoff = 128 * (lane & 15);
-    int sink = 0, pending = l2_touch(reinterpret_cast<const char *>(abm + m + 2), pvoff,
-                                     lane < 16 && m + 2 + 8 * lane <= L);
-    double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
-    for (int l0 = m; l0 <= L; l0 += kTL) {
-        const int tl = min(kTL, L - l0 + 1);
-        sink ^= pending;
-        const int lp = l0 + 2 * kTL;
-        pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= L);
-        // phase 1: lanes = rings, t = degree offset
-        int t = 0;
-        if (l0 == m) {
-            q1 = qs;
-            W[0][lane] = ldexp(qs, E);
-            t = 1;
-            if (m < L) {
-                const double q = sqrt(double(2 * m + 3)) * x * qs;
-                q2 = q1;
-                q1 = q;
-                W[1][lane] = ldexp(q, E);
-                t = 2;
-            }
-        }
-#pragma unroll 8
-        for (; t < tl; ++t) {
-            const int l = l0 + t;
-            const double2 c = abm[l];
-            const double q = fma(c.x * x, q1, -(c.y * q2));
-            q2 = q1;
-            q1 = q;
-            W[t][lane] = ldexp(q, E);
-        }
-        {
-            const int s = renorm_shift(q1, q2);
-            q1 = ldexp(q1, -s);
-            q2 = ldexp(q2, -s);
-            E += s;
-        }
-        wave_lds_order();
-        // phase 2: lanes = degree offsets; the tile's rings are summed in ring order, U rings at a
-        // time with their LDS reads and scalar loads issued before the FMAs that use them
-        if (lane < tl) {
-            double s0[NB], s1[NB], s2[NB], s3[NB];
-#pragma unroll
-            for (int u = 0; u < NB; ++u) s0[u] = s1[u] = s2[u] = s3[u] = 0.0;
-            int r = 0;
-            for (; r + U <= nr; r += U) {
-                double wv[U];
-                double2 gp[U][NB], gm[U][NB];
-#pragma unroll
-                for (int v = 0; v < U; ++v) {
-                    wv[v] = W[lane][r + v];
-#pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-                        gp[v][u] = gpc[u][r + v];
-                        gm[v][u] = m > 0 ? gmc[u][r + v] : make_double2(0.0, 0.0);
-                    }
-                }
-#pragma unroll
-                for (int v = 0; v < U; ++v)
-#pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-                        s0[u] = fma(wv[v], gp[v][u].x, s0[u]);
-                        s1[u] = fma(wv[v], gp[v][u].y, s1[u]);
-                        s2[u] = fma(wv[v], gm[v][u].x, s2[u]);
-                        s3[u] = fma(wv[v], gm[v][u].y, s3[u]);
-                    }
-            }
-            for (; r < nr; ++r) {
-                const double wv = W[lane][r];
-#pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const double2 gp = gpc[u][r];
-                    const double2 gm = m > 0 ? gmc[u][r] : make_double2(0.0, 0.0);
-                    s0[u] = fma(wv, gp.x, s0[u]);
-                    s1[u] = fma(wv, gp.y, s1[u]);
-                    s2[u] = fma(wv, gm.x, s2[u]);
-                    s3[u] = fma(wv, gm.y, s3[u]);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < NB; ++u) out[int64_t(l0 - m + lane) * B + u] = make_double4(s0[u], s1[u], s2[u], s3[u]);
-        }
-        wave_lds_order();
-    }
-    if ((sink ^ pending) == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true
+    int sink = 0;
+    if (m > 0) adj_sweep<NB, true>(abm, gpc, gmc, gstride, nr, x, qs, E, m, L, B, W, out, sink);
+    else adj_sweep<NB, false>(abm, gpc, gmc, gstride, nr, x, qs, E, m, L, B, W, out, sink);
+    if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
 // Writes every alm slot: the tile-ordered sum of the partials where (l, m) is retained, zero in
@@ -437,7 +494,7 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
     if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
             dim3 grid(r.naitems);
-            const size_t lds = sizeof(double) * kTL * (kTL + 1) * kAdjWaves;
+            const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
 #define PFSHT_ADJ(NB)                                                                                      \
     ring_adj_kernel<NB><<<grid, 64 * r.adj_waves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab,      \
                                                             r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0, kNeverSink)
