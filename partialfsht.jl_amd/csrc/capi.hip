@@ -371,6 +371,14 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     u.ns1 = (u.N1 + S4 - 1) / S4;
     u.ns2 = (u.N2 + S4 - 1) / S4;
     u.win = 3 * u.S + 16;
+    {
+        int maxlds = 65536;
+        (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p.dev);
+        if (nufft_window_lds_bytes(u.win) > size_t(maxlds)) {
+            set_error("the NUFFT super-tile window does not fit in LDS at this tolerance (tol >= 1e-3)");
+            return kEUnsupported;
+        }
+    }
     std::vector<double> X1(M), X2(M);
     std::vector<int32_t> key(M);
     const double half = 0.5 * u.w;

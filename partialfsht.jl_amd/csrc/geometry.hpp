@@ -62,4 +62,5 @@ int es_fine_size(int n, int w);
 // corr[i] = h / psihat(k_i): deconvolution factors for the modes of a length-n axis in FFT order.
 void es_corr(int n, int N, int w, double beta, std::vector<double> &corr);
 
+
 }  // namespace pfsht

@@ -8,22 +8,73 @@
 // Interpolation and spreading evaluate the kernel with the same expression on the same cells,
 // so ⟨type2 c, f⟩ = ⟨c, type1 f⟩ holds to round-off.
 //
-// Points are bin-sorted at plan time. Spreading is deterministic without atomics: a workgroup owns
+// Points are bin-sorted at plan time. Interpolation stages each super-tile's window of the fine
+// grid in LDS and gathers from there. Spreading is deterministic without atomics: a workgroup owns
 // a super-tile of 4 x 4 bins (S x S base cells each, S = 17 - w so a bin's footprint is 16 x 16)
 // and accumulates each bin's footprint window as an FP64 MFMA outer-product sum
 // (v_mfma_f64_16x16x4f64: four points per instruction, A = f_p * psi_row, B = psi_col). The four
 // waves compute four bins' windows concurrently, then add them into the LDS super-tile window one
 // wave after another, so every cell is summed in a fixed order. A merge pass then sums, for every
 // fine-grid cell, the (at most 2 x 2) super-tile windows covering it in a fixed order.
+#include <utility>
+
 #include "pfsht_common.hpp"
 
 namespace pfsht {
 
 typedef double double4v __attribute__((ext_vector_type(4)));
 
+// The ES kernel evaluated directly (spreading evaluates one cell per lane).
 __device__ __forceinline__ double es_kernel(double z, double beta) {
     const double zz = 1.0 - z * z;
     return zz >= 0.0 ? exp(beta * (sqrt(zz) - 1.0)) : 0.0;
+}
+
+// Wavefront-private ordering of LDS traffic: a wavefront's LDS operations complete in issue
+// order, so a wavefront reading back what it wrote needs only a compiler barrier.
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// The ES kernel as piecewise polynomials (EsHorner<W>, generated by gen_es_horner.py): the
+// coefficients are compile-time constants, so they are immediate operands of the FMAs (materialised
+// in SGPRs where they are used), never loaded.
+#include "es_horner_tables.inc"
+
+template <int W, int Off, int N, int... J>
+__device__ __forceinline__ double horner_seq(double x, std::integer_sequence<int, J...>) {
+    double acc = EsHorner<W>::c[Off + N - 1];
+    ((acc = fma(acc, x, EsHorner<W>::c[Off + N - 2 - J])), ...);
+    return acc;
+}
+template <int W, int Off, int N>
+__device__ __forceinline__ double horner(double x) {
+    return horner_seq<W, Off, N>(x, std::make_integer_sequence<int, N - 1>{});
+}
+template <int W, int... T>
+__device__ __forceinline__ void es_interior(double u, double (&k)[W], std::integer_sequence<int, T...>) {
+    ((k[T + 1] = horner<W, T * kHornInt, kHornInt>(u)), ...);
+}
+
+// The W kernel values of a point at x with base cell b (cells b .. b + W - 1): W - 2 interior
+// polynomials in u = 2 (b - x) + W - 1 and the edge polynomial, in e = 2 sqrt((1 -+ u)/2) - 1, for
+// the two end cells.
+template <int W>
+__device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
+    const double u = 2.0 * (double(b) - x) + double(W - 1);
+    es_interior<W>(u, k, std::make_integer_sequence<int, W - 2>{});
+    k[0] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + u)) - 1.0);
+    k[W - 1] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 - u)) - 1.0);
+}
+
+// Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
+// v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2).
+template <int W, int... T>
+__device__ __forceinline__ void es_half_interior(double v, double *k, std::integer_sequence<int, T...>) {
+    ((k[T + 1] = horner<W, T * kHornInt, kHornInt>(v)), ...);
+}
+template <int W>
+__device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2]) {
+    k[0] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
+    es_half_interior<W>(v, k, std::make_integer_sequence<int, (W + 1) / 2 - 1>{});
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -77,48 +128,50 @@ void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U
 }
 
 // ------------------------------------------------------------------------------------------------
-// Interpolation (type 2): one thread per bin-sorted point, w x w gather.
+// Interpolation (type 2): one workgroup per non-empty super-tile. The super-tile's window of the
+// fine grid (4 S + w - 1 cells square, wrapped periodically) is staged in LDS with coalesced row
+// reads, then each thread interpolates bin-sorted points of the super-tile from LDS.
 // ------------------------------------------------------------------------------------------------
 template <int W>
 __global__ void __launch_bounds__(256)
-interp_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
-              const double2 *__restrict__ U, double2 *__restrict__ f, int64_t M, int N1, int N2, double beta) {
-    const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-    if (p >= M) return;
+interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
+                 const double2 *__restrict__ U, double2 *__restrict__ f, const int32_t *__restrict__ st_list,
+                 const int32_t *__restrict__ bin_ofs, int64_t M, int N1, int N2, int S4, int ns2, int win) {
+    extern __shared__ double2 lw[];
+    const int slot = blockIdx.x;
     const int b = blockIdx.y;
-    const double x1 = X1[p], x2 = X2[p];
-    const int b1 = int(ceil(x1 - 0.5 * W));
-    const int b2 = int(ceil(x2 - 0.5 * W));
-    const double inv = 2.0 / W;
-    double k1[W], k2[W];
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-        k1[t] = es_kernel((double(b1 + t) - x1) * inv, beta);
-        k2[t] = es_kernel((double(b2 + t) - x2) * inv, beta);
-    }
-    int c2i[W];
-#pragma unroll
-    for (int t = 0; t < W; ++t) {
-        const int c = b2 + t;
-        c2i[t] = c < N2 ? c : c - N2;
-    }
+    const int st = st_list[slot];
+    const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
     const double2 *__restrict__ Ub = U + int64_t(b) * N1 * N2;
-    double ar = 0.0, ai = 0.0;
-#pragma unroll
-    for (int t1 = 0; t1 < W; ++t1) {
-        const int r = b1 + t1;
-        const double2 *__restrict__ row = Ub + int64_t(r < N1 ? r : r - N1) * N2;
-        double rr = 0.0, ri = 0.0;
-#pragma unroll
-        for (int t2 = 0; t2 < W; ++t2) {
-            const double2 v = row[c2i[t2]];
-            rr = fma(k2[t2], v.x, rr);
-            ri = fma(k2[t2], v.y, ri);
-        }
-        ar = fma(k1[t1], rr, ar);
-        ai = fma(k1[t1], ri, ai);
+    for (int r = threadIdx.x >> 6; r < win; r += 4) {
+        const int row = (x0 + r) % N1;
+        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(row) * N2 + (y0 + c) % N2];
     }
-    f[int64_t(perm[p]) + int64_t(b) * M] = make_double2(ar, ai);
+    __syncthreads();
+    const int start = bin_ofs[slot * 16], end = bin_ofs[slot * 16 + 16];
+    for (int p = start + int(threadIdx.x); p < end; p += 256) {
+        const double x1 = X1[p], x2 = X2[p];
+        const int b1 = int(ceil(x1 - 0.5 * W));
+        const int b2 = int(ceil(x2 - 0.5 * W));
+        double k1[W], k2[W];
+        es_row<W>(x1, b1, k1);
+        es_row<W>(x2, b2, k2);
+        const double2 *__restrict__ wrow = lw + (b1 - x0) * win + (b2 - y0);
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int t1 = 0; t1 < W; ++t1) {
+            double rr = 0.0, ri = 0.0;
+#pragma unroll
+            for (int t2 = 0; t2 < W; ++t2) {
+                const double2 v = wrow[t1 * win + t2];
+                rr = fma(k2[t2], v.x, rr);
+                ri = fma(k2[t2], v.y, ri);
+            }
+            ar = fma(k1[t1], rr, ar);
+            ai = fma(k1[t1], ri, ai);
+        }
+        f[int64_t(perm[p]) + int64_t(b) * M] = make_double2(ar, ai);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -224,8 +277,11 @@ __global__ void spread_merge_kernel(const double2 *__restrict__ Wst, const int32
 
 template <int W>
 static void interp_w(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s) {
-    dim3 grid(unsigned((u.M + 255) / 256), u.B);
-    interp_kernel<W><<<grid, 256, 0, s>>>(u.X1, u.X2, u.perm, U, f, u.M, u.N1, u.N2, u.beta);
+    if (u.nst == 0) return;
+    dim3 grid(u.nst, u.B);
+    const size_t lds = sizeof(double2) * size_t(u.win) * u.win;
+    interp_st_kernel<W><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list, u.bin_ofs, u.M, u.N1, u.N2,
+                                               4 * u.S, u.ns2, u.win);
 }
 
 template <int W>

@@ -1,0 +1,72 @@
+"""The device NUFFT's kernel tables (partialfsht.jl_amd/csrc/es_horner_tables.inc) against the
+exponential-of-semicircle kernel they stand for, phi(z) = exp(beta (sqrt(1 - z^2) - 1)), evaluated
+exactly as the oracle's NUFFT does. Every width the library can pick (2..16) is checked over a fine
+sweep of point offsets, each of the w cells a point touches, evaluating the polynomials in double
+precision by Horner exactly as the kernels do. A dropped coefficient, a wrong cell index or a
+mirrored edge fails these bounds by orders of magnitude."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+INC = os.path.join(ROOT, "partialfsht.jl_amd", "csrc", "es_horner_tables.inc")
+
+
+def _tables():
+    txt = open(INC).read()
+    hint = int(re.search(r"kHornInt = (\d+)", txt).group(1))
+    hedge = int(re.search(r"kHornEdge = (\d+)", txt).group(1))
+    out = {}
+    for m in re.finditer(r"struct EsHorner<(\d+)> \{\s*static constexpr double beta = (\S+);\s*"
+                         r"static constexpr double c\[(\d+)\] = \{([^}]*)\}", txt):
+        w = int(m.group(1))
+        vals = [float.fromhex(v.strip()) for v in m.group(4).split(",")]
+        assert len(vals) == int(m.group(3)) == (w - 2) * hint + hedge
+        out[w] = (float.fromhex(m.group(2)), vals)
+    return hint, hedge, out
+
+
+def _horner(c, x):
+    acc = np.zeros_like(x)
+    for a in reversed(c):
+        acc = acc * x + a
+    return acc
+
+
+def _beta_of(w):
+    bo = {2: 2.20, 3: 2.26, 4: 2.38}.get(w, 2.30)
+    return bo * w
+
+
+# bounds on the largest absolute error (phi peaks at 1); the edge cells of the narrow kernels (loose
+# tolerances) converge slowest
+BOUND = {2: 2e-14, 3: 2e-14, 4: 5e-14, 5: 1e-14, 6: 4e-15}
+
+
+@pytest.mark.parametrize("w", range(2, 17))
+def test_tables_match_kernel(w):
+    hint, hedge, tabs = _tables()
+    beta, c = tabs[w]
+    assert beta == _beta_of(w)  # the shape parameter es_params chooses for this width
+    x = 1000.0 + np.linspace(0.0, 1.0, 40001, endpoint=False)
+    b = np.ceil(x - 0.5 * w)
+    u = 2.0 * (b - x) + (w - 1)
+    assert u.min() >= -1.0 and u.max() < 1.0
+    worst = 0.0
+    for t in range(w):
+        # the kernel in 80-bit extended precision: a double-precision exp(beta (...)) is itself
+        # off by ~beta * 1e-16
+        z = (b.astype(np.longdouble) + t - x) * (np.longdouble(2) / w)
+        exact = np.exp(np.longdouble(beta) * (np.sqrt(np.maximum(1 - z * z, 0)) - 1))
+        if 1 <= t <= w - 2:
+            got = _horner(c[(t - 1) * hint:t * hint], u)
+        else:
+            s = 1.0 if t == 0 else -1.0
+            e = 2.0 * np.sqrt(0.5 * (1.0 + s * u)) - 1.0
+            got = _horner(c[(w - 2) * hint:], e)
+        worst = max(worst, float(np.abs(got.astype(np.longdouble) - exact).max()))
+    assert worst < BOUND.get(w, 1e-15), (w, worst)
