@@ -374,7 +374,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     {
         int maxlds = 65536;
         (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p.dev);
-        if (nufft_window_lds_bytes(u.win) > size_t(maxlds)) {
+        if (spread_lds_bytes(u.win) > size_t(maxlds)) {
             set_error("the NUFFT super-tile window does not fit in LDS at this tolerance (tol >= 1e-3)");
             return kEUnsupported;
         }

@@ -39,6 +39,23 @@ __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOM
 // in SGPRs where they are used), never loaded.
 #include "es_horner_tables.inc"
 
+// A coefficient passed through an empty asm: inside a loop its materialisation cannot be hoisted
+// out (where, with all of a row's coefficients live at once, they would be parked in VGPRs).
+__device__ __forceinline__ double launder(double c) {
+    asm volatile("" : "+s"(c));
+    return c;
+}
+template <int W, int Off, int N, int... J>
+__device__ __forceinline__ double horner_seq_l(double x, std::integer_sequence<int, J...>) {
+    double acc = launder(EsHorner<W>::c[Off + N - 1]);
+    ((acc = fma(acc, x, launder(EsHorner<W>::c[Off + N - 2 - J]))), ...);
+    return acc;
+}
+template <int W, int Off, int N>
+__device__ __forceinline__ double horner_l(double x) {
+    return horner_seq_l<W, Off, N>(x, std::make_integer_sequence<int, N - 1>{});
+}
+
 template <int W, int Off, int N, int... J>
 __device__ __forceinline__ double horner_seq(double x, std::integer_sequence<int, J...>) {
     double acc = EsHorner<W>::c[Off + N - 1];
@@ -66,14 +83,15 @@ __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
 }
 
 // Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
-// v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2).
+// v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2). Coefficients are
+// laundered: this is evaluated inside the spreading chunk loop.
 template <int W, int... T>
 __device__ __forceinline__ void es_half_interior(double v, double *k, std::integer_sequence<int, T...>) {
-    ((k[T + 1] = horner<W, T * kHornInt, kHornInt>(v)), ...);
+    ((k[T + 1] = horner_l<W, T * kHornInt, kHornInt>(v)), ...);
 }
 template <int W>
 __device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2]) {
-    k[0] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
+    k[0] = horner_l<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
     es_half_interior<W>(v, k, std::make_integer_sequence<int, (W + 1) / 2 - 1>{});
 }
 
@@ -177,66 +195,112 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 // ------------------------------------------------------------------------------------------------
 // Spreading (type 1): MFMA super-tiles + ordered merge.
 // ------------------------------------------------------------------------------------------------
+// Spreading: one workgroup per non-empty super-tile; wave w owns bins w, w+4, w+8, w+12 and takes
+// each bin's points 16 at a time. For such a chunk the wave first evaluates the kernel rows once per
+// point: lane (p, axis, h) evaluates the cells t < ceil(W/2) of its point and axis from the left
+// (h = 0) or, through phi(-z) = phi(z), from the right (h = 1), so all lanes use the same
+// polynomial coefficients (scalar operands). The rows go to a wave-private LDS staging area at the
+// point's offset within its bin's 16-cell window, zero elsewhere, with f at the point. The wave then
+// accumulates the chunk on FP64 MFMA, four points per instruction (A = f psi_row, B = psi_col),
+// keeping each of its bins' 16 x 16 tiles in registers. No workgroup barrier until the end, where
+// the tiles are added into the LDS window wave after wave (bin windows overlap) and the window is
+// written out for the merge pass.
 template <int W>
 __global__ void __launch_bounds__(256)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  const double2 *__restrict__ f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, double2 *__restrict__ Wst, int64_t M, int S, int ns2, int win,
-                 int nst, double beta) {
-    extern __shared__ double2 lw[];
+                 const int32_t *__restrict__ bin_ofs, double2 *__restrict__ Wst, int64_t M, int S, int ns2,
+                 int win, int nst) {
+    constexpr int NH = (W + 1) / 2;  // cells per half row
+    extern __shared__ double lds_sp[];
+    double2 *lw = reinterpret_cast<double2 *>(lds_sp);  // [win][win]
     const int slot = blockIdx.x;
     const int b = blockIdx.y;
     const int st = st_list[slot];
-    const int x0 = (st / ns2) * 4 * S, y0 = (st % ns2) * 4 * S;
+    const int S4 = 4 * S;
+    const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int ii = lane & 15, kq = lane >> 4;
-    const double inv = 2.0 / W;
+    // rows are kRowPad doubles apart so that the 16 lanes writing one entry of 16 rows, and the
+    // MFMA operand reads, fall in different LDS banks
+    double *K1 = lds_sp + 2 * win * win + wave * kSpreadStage;  // [16][kRowPad] row kernel
+    double *K2 = K1 + 16 * kRowPad;                              // [16][kRowPad] column kernel
+    double2 *Fc = reinterpret_cast<double2 *>(K2 + 16 * kRowPad);  // [16]
+    const int ii = lane & 15, kq = lane >> 4;                    // MFMA operand layout
+    const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;  // kernel-row evaluation
     for (int q = tid; q < win * win; q += 256) lw[q] = make_double2(0.0, 0.0);
-    __syncthreads();
-    for (int phase = 0; phase < 4; ++phase) {
-        const int bi = (phase >> 1) + 2 * (wave >> 1);
-        const int bj = (phase & 1) + 2 * (wave & 1);
-        const int lb = bi * 4 + bj;
-        const int start = bin_ofs[slot * 16 + lb], end = bin_ofs[slot * 16 + lb + 1];
-        const int bx0 = x0 + bi * S, by0 = y0 + bj * S;
-        double4v dr = {0.0, 0.0, 0.0, 0.0}, di = {0.0, 0.0, 0.0, 0.0};
-        for (int p0 = start; p0 < end; p0 += 4) {
-            const int p = p0 + kq;
-            double are = 0.0, aim = 0.0, c = 0.0;
-            if (p < end) {
-                const double x1 = X1[p], x2 = X2[p];
-                const double2 fv = f[int64_t(perm[p]) + int64_t(b) * M];
-                const int b1 = int(ceil(x1 - 0.5 * W));
-                const int b2 = int(ceil(x2 - 0.5 * W));
-                const int t1 = bx0 + ii - b1;
-                const int t2 = by0 + ii - b2;
-                if (t1 >= 0 && t1 < W) {
-                    const double a = es_kernel((double(b1 + t1) - x1) * inv, beta);
+    const int32_t *__restrict__ bo = bin_ofs + slot * 16;
+    // the bin being accumulated is always tile 0; after each bin the tiles rotate by one, so after
+    // four bins tile k holds bin wave + 4 k again
+    double4v dr[4], di[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dr[k] = double4v{0.0, 0.0, 0.0, 0.0};
+        di[k] = double4v{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        const int lb = wave + 4 * k;
+        const int ps = bo[lb], pe = bo[lb + 1];
+        for (int c0 = ps; c0 < pe; c0 += 16) {
+            const int n = min(16, pe - c0);
+            if (pa < n) {
+                const int p = c0 + pa;
+                const double x = axis == 0 ? X1[p] : X2[p];
+                const int bc = int(ceil(x - 0.5 * W));
+                const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
+                const double u = 2.0 * (double(bc) - x) + double(W - 1);
+                double kv[NH];
+                es_half_row<W>(h == 0 ? u : -u, kv);
+                double *row = (axis == 0 ? K1 : K2) + pa * kRowPad;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0;
+                // the middle cell of an odd W is written by the left half only
+#pragma unroll
+                for (int t = 0; t < NH; ++t)
+                    if (h == 0) row[o + t] = kv[t];
+                    else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
+                if (axis == 0 && h == 0) Fc[pa] = f[int64_t(perm[p]) + int64_t(b) * M];
+            }
+            wave_lds_order();
+            for (int g = 0; g < n; g += 4) {
+                const int q = g + kq;
+                double are = 0.0, aim = 0.0, c = 0.0;
+                if (q < n) {
+                    const double a = K1[q * kRowPad + ii];
+                    const double2 fv = Fc[q];
                     are = a * fv.x;
                     aim = a * fv.y;
+                    c = K2[q * kRowPad + ii];
                 }
-                if (t2 >= 0 && t2 < W) c = es_kernel((double(b2 + t2) - x2) * inv, beta);
+                dr[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, c, dr[0], 0, 0, 0);
+                di[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di[0], 0, 0, 0);
             }
-            dr = __builtin_amdgcn_mfma_f64_16x16x4f64(are, c, dr, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di, 0, 0, 0);
+            wave_lds_order();
         }
-        // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds
-        // row i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Windows of
-        // different waves may overlap (16 > 2S when w > 9), so the waves add into LDS one at a
-        // time, in wave order: race-free and a fixed summation order.
-        for (int turn = 0; turn < 4; ++turn) {
-            if (wave == turn) {
+        const double4v tr = dr[0], ti = di[0];
+        dr[0] = dr[1]; dr[1] = dr[2]; dr[2] = dr[3]; dr[3] = tr;
+        di[0] = di[1]; di[1] = di[2]; di[2] = di[3]; di[3] = ti;
+    }
+    __syncthreads();
+    // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds row
+    // i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Bin windows overlap
+    // (16 > S), so the waves add into LDS one at a time, in wave order, each wave its bins in
+    // order: race-free and a fixed summation order.
+    for (int turn = 0; turn < 4; ++turn) {
+        if (wave == turn) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int lb = wave + 4 * k;
+                const int bi = lb >> 2, bj = lb & 3;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = bi * S + kq + 4 * r;
-                    const int col = bj * S + ii;
-                    double2 &cell = lw[row * win + col];
-                    cell.x += dr[r];
-                    cell.y += di[r];
+                    double2 &cell = lw[(bi * S + kq + 4 * r) * win + bj * S + ii];
+                    cell.x += dr[k][r];
+                    cell.y += di[k][r];
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
     double2 *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
     for (int q = tid; q < win * win; q += 256) out[q] = lw[q];
@@ -288,9 +352,8 @@ template <int W>
 static void spread_w(const NufftDev &u, const double2 *f, hipStream_t s) {
     if (u.nst == 0) return;
     dim3 grid(u.nst, u.B);
-    const size_t lds = sizeof(double2) * size_t(u.win) * u.win;
-    spread_st_kernel<W><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, u.Wst, u.M, u.S, u.ns2,
-                                               u.win, u.nst, u.beta);
+    spread_st_kernel<W><<<grid, 256, spread_lds_bytes(u.win), s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs,
+                                                                   u.Wst, u.M, u.S, u.ns2, u.win, u.nst);
 }
 
 #define PFSHT_W_SWITCH(w, FN, ...)                 \

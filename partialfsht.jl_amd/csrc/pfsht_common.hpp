@@ -98,8 +98,13 @@ struct NufftDev {
     int32_t *bin_ofs = nullptr;           // [nst*16 + 1] point ranges per (super-tile, local bin)
     double2 *Wst = nullptr;               // [B][nst][win][win] super-tile windows
 };
-// LDS of one interpolation or spreading workgroup: the super-tile window of the fine grid.
+// LDS of one interpolation workgroup: the super-tile window of the fine grid.
 inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(win) * win; }
+// LDS of one spreading workgroup: the window plus, per wavefront, the staging of one 16-point chunk
+// (two 16-row kernel tiles with rows kRowPad doubles apart, and 16 complex f values), in doubles.
+constexpr int kRowPad = 17;
+constexpr int kSpreadStage = 2 * 16 * kRowPad + 2 * 16;
+inline size_t spread_lds_bytes(int win) { return nufft_window_lds_bytes(win) + sizeof(double) * 4 * kSpreadStage; }
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
