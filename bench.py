@@ -79,7 +79,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lmax", type=int, default=2048)
     ap.add_argument("--npts", type=int, default=2_000_000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy three times (HBM counter calibration "
+                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(1 << 28, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 1 << 28, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (collectives are out of "
               "scope); run with --gpus 1", file=sys.stderr)
@@ -154,12 +166,22 @@ def main():
         stages[name] = e
     per_stage = {"synth": {k: v[0] / max(v[1], 1) for k, v in ss.items()},
                  "adjoint": {k: v[0] / max(v[1], 1) for k, v in sa.items()}}
-    ring = stages["ring_synth"]
-    roofline = {"kernel": "ring_fwd_kernel (synthesis ring pass)", "bound": "mfma",
+    # the dominant kernel of the step: the adjoint ring pass (ring_adj_kernel), timed alone
+    ring = stages["ring_adjoint"]
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic_lmax2048_M2e6.json")
+    if os.path.exists(tf) and L == 2048 and M == 2_000_000:
+        t = json.load(open(tf))
+        k = t.get("kernels", {}).get("ring_adj_kernel")
+        if k:
+            traffic, traffic_src = k["read_bytes"] + k["write_bytes"], "profiles/pmc_traffic_lmax2048_M2e6.json"
+    roofline = {"kernel": "ring_adj_kernel (adjoint ring pass)", "bound": "mfma",
                 "achieved": ring["tflops"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ring["tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
+                "frac": ring["tflops"] / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "algorithmic": f"12 FP64 flops per retained (l,m,ring) triple x {W} triples per launch; "
-                               "FP64 VALU peak = FP64 matrix peak on MI355X"}
+                               "the bound is FP64 FMA throughput (VALU), whose MI355X peak equals the "
+                               "FP64 matrix peak"}
 
     cpu = None
     if not args.no_cpu:

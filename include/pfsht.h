@@ -63,9 +63,10 @@ typedef struct {
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */
-#define PFSHT_NSTAGES 8
-/* Stage order: 0 ring pass, 1 pack/unpack+window, 2 phi FFT, 3 DFS double/fold,
- * 4 2-D FFT, 5 deconvolve/phase, 6 fine FFT, 7 NUFFT interpolate/spread(+merge). */
+#define PFSHT_NSTAGES 9
+/* Stage order: 0 ring pass (adjoint: per-tile partial sums), 1 pack/unpack+window, 2 phi FFT,
+ * 3 DFS double/fold, 4 2-D FFT, 5 deconvolve/phase, 6 fine FFT, 7 NUFFT interpolate/spread(+merge),
+ * 8 adjoint ring partial reduction (empty in synthesis). */
 
 const char *pfsht_last_error(void);
 const char *pfsht_version(void);

@@ -477,8 +477,9 @@ static TimedApply *timing_begin(pfsht_plan_s &p, int dir) {
     if (!p.timing) return nullptr;
     TimedApply t;
     t.dir = dir;
-    // synthesis runs stages 0..7 in order; the adjoint runs them transposed, 7..0
-    for (int k = 0; k < kStages; ++k) t.stage_of[k] = dir == 0 ? k : kStages - 1 - k;
+    // synthesis runs stages 0..7 in order (8 is empty); the adjoint runs them transposed, 7..0,
+    // then the ring partial reduction, 8
+    for (int k = 0; k < kStages; ++k) t.stage_of[k] = (dir == 0 || k == kStages - 1) ? k : kStages - 2 - k;
     for (int k = 0; k <= kStages; ++k) {
         if (p.pool.empty()) {
             hipEvent_t e;
@@ -512,11 +513,12 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     TimedApply *t = timing_begin(p, 0);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
         pfsht_rings_s &r = *p.rings;
-        for (int k = 0; k <= 0; ++k) tmark(p, t, k, s);
+        tmark(p, t, 0, s);
         launch_ring_fwd(r.d, alm, r.Gd, s);
         for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
         launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, p.B, s);
         tmark(p, t, 8, s);
+        tmark(p, t, 9, s);
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
     }
@@ -542,6 +544,7 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     tmark(p, t, 7, s);
     launch_interp(p.nu, p.U, f, s);
     tmark(p, t, 8, s);
+    tmark(p, t, 9, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
@@ -554,8 +557,10 @@ static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
         tmark(p, t, 0, s);
         launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);
         for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
-        launch_ring_adj(r.d, r.Gd, alm, s);
+        launch_ring_adj_tiles(r.d, r.Gd, s);
         tmark(p, t, 8, s);
+        launch_ring_adj_finalize(r.d, alm, s);
+        tmark(p, t, 9, s);
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
     }
@@ -578,8 +583,10 @@ static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     tmark(p, t, 6, s);
     launch_unpack_window(g, p.Fp, p.wth, r.Gd, s);
     tmark(p, t, 7, s);
-    launch_ring_adj(r.d, r.Gd, alm, s);
+    launch_ring_adj_tiles(r.d, r.Gd, s);
     tmark(p, t, 8, s);
+    launch_ring_adj_finalize(r.d, alm, s);
+    tmark(p, t, 9, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }

@@ -61,6 +61,9 @@ constexpr int kAdjWaves = 4;      // adjoint block: up to 4 tiles of an order, o
 // Gd is the internal ring buffer [b][m + mmax][j].
 void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s);
 void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s);
+// the two launches of launch_ring_adj: per-tile partial sums, then their ordered reduction into alm
+void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s);
+void launch_ring_adj_finalize(const RingDev &r, double2 *alm, hipStream_t s);
 void launch_gd_to_ref(const RingDev &r, const double2 *Gd, double2 *G, hipStream_t s);
 void launch_ref_to_gd(const RingDev &r, const double2 *G, double2 *Gd, hipStream_t s);
 

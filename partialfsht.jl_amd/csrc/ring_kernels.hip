@@ -491,6 +491,11 @@ __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_
 }
 
 void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s) {
+    launch_ring_adj_tiles(r, Gd, s);
+    launch_ring_adj_finalize(r, alm, s);
+}
+
+void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
     if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
             dim3 grid(r.naitems);
@@ -506,6 +511,9 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
             }
 #undef PFSHT_ADJ
         });
+}
+
+void launch_ring_adj_finalize(const RingDev &r, double2 *alm, hipStream_t s) {
     dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, r.B);
     ring_adj_finalize<<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.B);
 }

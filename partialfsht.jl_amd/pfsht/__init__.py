@@ -51,8 +51,8 @@ class InfoC(C.Structure):
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double)]
 
 
-NSTAGES = 8
-STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft"]
+NSTAGES = 9
+STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft", "ring_reduce"]
 METHODS = {"auto": 0, "direct": 1, "stretch": 2}
 KINDS = {0: "pole", 1: "interior", 2: "full", -1: "direct"}
 WHERE_HOST, WHERE_DEVICE = 0, 1
