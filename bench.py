@@ -73,6 +73,10 @@ def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", choices=["plain", "batched"], default="plain",
+                    help="plain: the 20 deg cap at lmax 2048, M 2e6 (default, the headline workload); "
+                         "batched: the same cap at lmax 1024, M 5.1e5, ntrans 16, against one-at-a-time "
+                         "applies of the same plan geometry")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -97,6 +101,12 @@ def main():
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "batched":
+        return run_batched(args)
+    return run_plain(args)
+
+
+def run_plain(args):
     import torch
     import pfsht
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -114,6 +124,15 @@ def main():
     info = plan.info
     log(f"plan built in {t_plan:.2f}s: nth={info['nth']} nph={info['nph']} mmax={info['mmax']} "
         f"ring_work={info['ring_work']} fine={info['es_n1']}x{info['es_n2']} w={info['es_w']}")
+    # the first plan of a process pays HIP start-up and rocFFT's run-time kernel compilation; a second
+    # plan on new points of the same sizes shows the plan construction itself
+    th2, ph2 = cap_points(np.random.default_rng(4096), 20, M)
+    t0 = time.perf_counter()
+    plan2 = pfsht.Plan(th2, ph2, lmax=L, tol=tol, method="stretch")
+    torch.cuda.synchronize()
+    t_plan_warm = time.perf_counter() - t0
+    del plan2
+    log(f"second plan (new points) built in {t_plan_warm:.3f}s")
 
     alm_d = pfsht.to_device(alm)
     f_d = pfsht.to_device(f)
@@ -202,10 +221,88 @@ def main():
                       "es_w": info["es_w"]},
            "triples_per_s": W * 2 * args.steps / elapsed,
            "plan_s": t_plan,
+           "plan_s_warm": t_plan_warm,
            "stage_ms": per_stage,
            "stages": stages,
            "roofline": roofline,
            "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def run_batched(args):
+    """ntrans = 16 independent coefficient sets on one plan (lmax 1024, M = 5.1e5, 20 deg cap,
+    tol 1e-8, C_l = (l+1)^-2 spectra with distinct seeds), against the same applies made one at a time
+    on an ntrans = 1 plan of the same points. value counts single-transform applies per second."""
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L, tol, M, B = 1024, 1e-8, 510_000, 16
+    rng = np.random.default_rng(1024)
+    th, ph = cap_points(rng, 20, M)
+    alm = np.asfortranarray(np.stack([rand_alm(L, np.random.default_rng(5000 + b)) for b in range(B)], axis=2))
+    f = np.asfortranarray(rng.standard_normal((M, B)) + 1j * rng.standard_normal((M, B)))
+    log(f"plans lmax={L} M={M} ntrans={B} and 1")
+    pB = pfsht.Plan(th, ph, lmax=L, tol=tol, ntrans=B, method="stretch")
+    p1 = pfsht.Plan(th, ph, lmax=L, tol=tol, ntrans=1, method="stretch")
+    torch.cuda.synchronize()
+    W = pB.info["ring_work"]
+    a_d = pfsht.to_device(alm)
+    f_d = pfsht.to_device(f)
+    fo = torch.empty(M * B, dtype=torch.complex128, device="cuda")
+    ao = torch.empty(alm.size, dtype=torch.complex128, device="cuda")
+    LB = (L + 1) * (2 * L + 1)
+    a_cols = [a_d[b * LB:(b + 1) * LB] for b in range(B)]
+    f_cols = [f_d[b * M:(b + 1) * M] for b in range(B)]
+    fo1 = torch.empty(M, dtype=torch.complex128, device="cuda")
+    ao1 = torch.empty(LB, dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        pB.synth(fo, a_d, s)
+        pB.adjoint(ao, f_d, s)
+        p1.synth(fo1, a_cols[0], s)
+        p1.adjoint(ao1, f_cols[0], s)
+    torch.cuda.synchronize()
+    steps = max(1, args.steps // 4)
+    pB.set_timing(True)
+    pB.stage_times("synth", reset=True)
+    pB.stage_times("adjoint", reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pB.synth(fo, a_d, s)
+        pB.adjoint(ao, f_d, s)
+    torch.cuda.synchronize()
+    tB = time.perf_counter() - t0
+    pB.set_timing(False)
+    ss, sa = pB.stage_times("synth"), pB.stage_times("adjoint")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for b in range(B):
+            p1.synth(fo1, a_cols[b], s)
+            p1.adjoint(ao1, f_cols[b], s)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter() - t0
+    batched = 2 * B * steps / tB
+    single = 2 * B * steps / t1
+    log(f"batched {batched:.1f} applies/s, one at a time {single:.1f} applies/s, ratio {batched / single:.2f}")
+    ring_ms = sa["ring"][0] / max(sa["ring"][1], 1)
+    flops = (4.0 + 8.0 * B) * W
+    tf = flops / (ring_ms * 1e-3) / 1e12
+    out = {"metric": "applies/s at lmax=1024 tol=1e-8, ntrans=16 (synth+adjoint)", "value": batched,
+           "unit": "applies/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": tB / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"batched apply, 20deg polar cap, lmax={L}, M={M} area-uniform, tol={tol}, "
+                                  f"ntrans={B}, one synthesis + one adjoint of all {B} transforms per step",
+                      "lmax": L, "npts": M, "ntrans": B, "ring_work_per_transform": W},
+           "one_at_a_time_applies_s": single, "batched_over_single": batched / single,
+           "stage_ms": {"synth": {k: v[0] / max(v[1], 1) for k, v in ss.items()},
+                        "adjoint": {k: v[0] / max(v[1], 1) for k, v in sa.items()}},
+           "roofline": {"kernel": f"ring_adj_kernel (adjoint ring pass, {B} transforms)", "bound": "mfma",
+                        "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tf / PEAK_FP64_TFLOPS, "traffic": None,
+                        "algorithmic": f"(4 + 8 x {B}) FP64 flops per retained triple x {W} triples"},
+           "cpu_baseline": None}
     print(json.dumps(out))
 
 

@@ -222,6 +222,23 @@ def test_ntrans_columns(torch, P, O, rng):
         assert rel_modes(O, aBg[:, :, b], p1.H @ np.ascontiguousarray(f[:, b]), L) < 1e-13
 
 
+def test_ntrans16_columns(torch, P, O, rng):
+    # the batched form (two groups of 8 transforms per ring launch) equals sixteen single applies
+    L, B, n = 128, 16, 2500
+    th, ph = cap_points(rng, 20, n)
+    p1 = P.Plan(th, ph, lmax=L, tol=1e-8, method="stretch")
+    pB = P.Plan(th, ph, lmax=L, tol=1e-8, ntrans=B, method="stretch")
+    aB = O.rand_alm_fast(L, rng, ntrans=B)
+    fB = pB @ aB
+    for b in range(B):
+        assert relmax(fB[:, b], p1 @ np.asfortranarray(aB[:, :, b])) < 1e-13
+    f = np.asfortranarray(rng.standard_normal((n, B)) + 1j * rng.standard_normal((n, B)))
+    aBg = pB.H @ f
+    for b in range(B):
+        assert rel_modes(O, aBg[:, :, b], p1.H @ np.ascontiguousarray(f[:, b]), L) < 1e-13
+    assert np.all(aBg[~O.valid_mask(L)] == 0)
+
+
 def test_shape_errors(torch, P, rng):
     th, ph = cap_points(rng, 20, 1000)
     p = P.Plan(th, ph, lmax=160, tol=1e-8, method="stretch")
