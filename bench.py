@@ -73,10 +73,16 @@ def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", choices=["plain", "batched"], default="plain",
+    ap.add_argument("--config", choices=["plain", "batched", "cap30", "equatorial", "cap15at60", "serve"],
+                    default="plain",
                     help="plain: the 20 deg cap at lmax 2048, M 2e6 (default, the headline workload); "
                          "batched: the same cap at lmax 1024, M 5.1e5, ntrans 16, against one-at-a-time "
-                         "applies of the same plan geometry")
+                         "applies of the same plan geometry; cap30: 30 deg polar cap, lmax 2048, "
+                         "M = 4 (L theta*)^2, tol 1e-5; equatorial / cap15at60: interior caps (30 deg "
+                         "around the equator, 15 deg around colatitude 60 deg) at lmax 2048, tol 1e-8; "
+                         "these three also report the error against the Direct leaf on 1e4 points; "
+                         "serve: a Poisson request stream over 32 recurring geometries through the "
+                         "serving runtime (plan cache, HBM admission, 4 streams), p50/p99 latency")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -103,6 +109,10 @@ def main():
 
     if args.config == "batched":
         return run_batched(args)
+    if args.config in ("cap30", "equatorial", "cap15at60"):
+        return run_geometry(args)
+    if args.config == "serve":
+        return run_serve(args)
     return run_plain(args)
 
 
@@ -226,6 +236,188 @@ def run_plain(args):
            "stages": stages,
            "roofline": roofline,
            "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def centered_cap(rng, deg, M, theta_c, phi_c=0.0):
+    """Area-uniform points on the cap of radius deg degrees around (theta_c, phi_c)."""
+    th, ph = cap_points(rng, deg, M)
+    x, y, z = np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)
+    c, s = np.cos(theta_c), np.sin(theta_c)
+    x, z = c * x + s * z, -s * x + c * z
+    return np.arccos(np.clip(z, -1.0, 1.0)), np.mod(np.arctan2(y, x) + phi_c, 2 * np.pi)
+
+
+def run_geometry(args):
+    """Larger caps, one plan each, synthesis + adjoint per step, and the delivered error against the
+    device Direct leaf on a 1e4-point random subset: synthesis on the subset, and the adjoint of a
+    field supported on the subset (for which Direct is exact and cheap)."""
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L = 2048
+    geo = {"cap30": (30.0, 0.0, 1e-5, "30deg polar cap"),
+           "equatorial": (30.0, np.pi / 2, 1e-8, "30deg cap centred on the equator (interior)"),
+           "cap15at60": (15.0, np.deg2rad(60.0), 1e-8, "15deg cap centred at colatitude 60deg (interior)")}
+    deg, tc, tol, desc = geo[args.config]
+    M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
+    rng = np.random.default_rng(30)
+    th, ph = centered_cap(rng, deg, M, tc, 0.5)
+    alm = rand_alm(L, rng)
+    f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    log(f"{args.config}: plan lmax={L} M={M} tol={tol}")
+    t0 = time.perf_counter()
+    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method="stretch")
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    info = plan.info
+    log(f"plan {t_plan:.2f}s kind={info['kind_name']} nth={info['nth']} nph={info['nph']} "
+        f"ring_work={info['ring_work']} fine={info['es_n1']}x{info['es_n2']} w={info['es_w']}")
+    a_d, f_d = pfsht.to_device(alm), pfsht.to_device(f)
+    fo = torch.empty(M, dtype=torch.complex128, device="cuda")
+    ao = torch.empty(alm.size, dtype=torch.complex128, device="cuda")
+    s = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        plan.synth(fo, a_d, s)
+        plan.adjoint(ao, f_d, s)
+    torch.cuda.synchronize()
+    steps = max(1, args.steps // 2)
+    plan.set_timing(True)
+    plan.stage_times("synth", reset=True)
+    plan.stage_times("adjoint", reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.synth(fo, a_d, s)
+        plan.adjoint(ao, f_d, s)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    plan.set_timing(False)
+    ss, sa = plan.stage_times("synth"), plan.stage_times("adjoint")
+    # error against the device Direct leaf on 1e4 points
+    sub = np.random.default_rng(31).choice(M, 10_000, replace=False)
+    pdir = pfsht.Plan(th[sub], ph[sub], lmax=L, method="direct")
+    fsyn = pfsht.from_device(fo, (M,))
+    fdir = pdir @ alm
+    err_s = float(np.abs(fsyn[sub] - fdir).max() / np.abs(fdir).max())
+    fsub = np.random.default_rng(32).standard_normal(10_000) + 1j * np.random.default_rng(33).standard_normal(10_000)
+    fz = np.zeros(M, complex)
+    fz[sub] = fsub
+    ast_ = plan.H @ fz
+    adir = pdir.H @ fsub
+    err_a = float(np.abs(ast_ - adir).max() / np.abs(adir).max())
+    del pdir
+    log(f"{steps} steps: {el / steps * 1e3:.3f} ms/step; error vs Direct: synthesis {err_s:.2e}, adjoint {err_a:.2e}")
+    W = info["ring_work"]
+    ring_ms = sa["ring"][0] / max(sa["ring"][1], 1)
+    tf = 12.0 * W / (ring_ms * 1e-3) / 1e12
+    out = {"metric": f"applies/s at lmax=2048 tol={tol:g} (synth+adjoint)", "value": 2 * steps / el,
+           "unit": "applies/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": el / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{desc}, lmax={L}, M={M} area-uniform, tol={tol:g}, ntrans=1, "
+                                  "one synthesis + one adjoint per step",
+                      "lmax": L, "npts": M, "tol": tol, "kind": info["kind_name"], "nth": info["nth"],
+                      "nph": info["nph"], "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
+                      "es_w": info["es_w"]},
+           "error_vs_direct_1e4": {"synthesis": err_s, "adjoint": err_a},
+           "plan_s": t_plan,
+           "stage_ms": {"synth": {k: v[0] / max(v[1], 1) for k, v in ss.items()},
+                        "adjoint": {k: v[0] / max(v[1], 1) for k, v in sa.items()}},
+           "roofline": {"kernel": "ring_adj_kernel (adjoint ring pass)", "bound": "mfma", "achieved": tf,
+                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
+                        "traffic": None,
+                        "algorithmic": f"12 FP64 flops per retained triple x {W} triples"},
+           "cpu_baseline": None}
+    print(json.dumps(out))
+
+
+def run_serve(args):
+    """Serving: 32 recurring geometries, each a cap of radius U(5, 30) deg around a random centre
+    on the sphere, lmax from {256, 512, 1024, 2048}, M = 4 (L theta*)^2 area-uniform points, ntrans
+    U{1..8}, tol from {1e-8, 1e-5}, method chosen by the planner (Direct where the stretch is refused
+    or dearer). Requests draw a geometry and a direction (50/50) and arrive as a Poisson stream at a
+    fixed offered load of 70% of the measured closed-loop capacity; plans live in the server's cache
+    under a 200 GB HBM budget and applies go over 4 streams. Latency = completion - arrival (host
+    clock, completions polled). value = transforms applied per second (a request applies ntrans)."""
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    rng = np.random.default_rng(555)
+    geos = []
+    for g in range(32):
+        deg = rng.uniform(5.0, 30.0)
+        tc, pc = np.arccos(rng.uniform(-1.0, 1.0)), rng.uniform(0.0, 2 * np.pi)
+        L = int(rng.choice([256, 512, 1024, 2048]))
+        M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
+        B = int(rng.integers(1, 9))
+        tol = float(rng.choice([1e-8, 1e-5]))
+        th, ph = centered_cap(rng, deg, M, tc, pc)
+        geos.append(dict(deg=deg, tc=tc, L=L, M=M, B=B, tol=tol, th=th, ph=ph))
+    srv = pfsht.Server(nstreams=4, hbm_budget_bytes=200 * 10**9)
+    t0 = time.perf_counter()
+    methods = []
+    for g, G in enumerate(geos):
+        info, _ = srv.plan(g, G["th"], G["ph"], G["L"], G["tol"], G["B"], "auto")
+        methods.append("direct" if info["method"] == 1 else "stretch")
+        LB = (G["L"] + 1) * (2 * G["L"] + 1) * G["B"]
+        G["alm"] = torch.randn(LB, dtype=torch.complex128, device="cuda")
+        G["f"] = torch.randn(G["M"] * G["B"], dtype=torch.complex128, device="cuda")
+        G["alm_out"] = torch.empty_like(G["alm"])
+        G["f_out"] = torch.empty_like(G["f"])
+    torch.cuda.synchronize()
+    t_plans = time.perf_counter() - t0
+    log(f"32 plans built in {t_plans:.1f}s ({methods.count('direct')} Direct); stats {srv.stats()}")
+
+    def submit(g, d):
+        G = geos[g]
+        if d == 0:
+            return srv.submit(g, "synth", G["alm"], G["f_out"])
+        return srv.submit(g, "adjoint", G["f"], G["alm_out"])
+
+    for g in range(32):  # warm every plan in both directions
+        srv.wait(submit(g, 0))
+        srv.wait(submit(g, 1))
+    nreq = max(200, 40 * args.steps)
+    req = [(int(rng.integers(0, 32)), int(rng.integers(0, 2))) for _ in range(nreq)]
+    # closed loop: all requests in flight at once -> capacity in requests/s
+    t0 = time.perf_counter()
+    tick = [submit(g, d) for g, d in req[:nreq // 2]]
+    for t in tick:
+        srv.wait(t)
+    cap_rps = (nreq // 2) / (time.perf_counter() - t0)
+    lam = 0.7 * cap_rps
+    log(f"closed-loop capacity {cap_rps:.1f} requests/s; offered load {lam:.1f} requests/s")
+    gaps = rng.exponential(1.0 / lam, nreq)
+    arrive = np.cumsum(gaps)
+    # open loop: submit each request at its arrival time; completion times come from the device
+    # clock (server epoch), so no polling perturbs the arrivals
+    srv.epoch()
+    start = time.perf_counter()
+    tickets = []
+    for i in range(nreq):
+        while time.perf_counter() - start < arrive[i]:
+            pass
+        g, d = req[i]
+        tickets.append(submit(g, d))
+    ends = np.array([srv.times(t)[1] for t in tickets]) * 1e-3
+    el = float(ends.max())
+    lat_ms = (ends - arrive) * 1e3
+    applied = sum(geos[g]["B"] for g, _ in req)
+    st = srv.stats()
+    out = {"metric": "served p50/p99 latency and applies/s (mixed lmax 256-2048, tol 1e-8/1e-5, ntrans 1-8)",
+           "value": applied / el, "unit": "applies/s", "n_gpus": 1, "steps": nreq, "warmup": 64,
+           "ms_per_step": el / nreq * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "serving: Poisson arrivals over 32 recurring geometries (cap radius U(5,30) "
+                                  "deg, random centre, lmax in {256,512,1024,2048}, M = 4 (L theta*)^2, ntrans "
+                                  "U{1..8}, tol in {1e-8,1e-5}, synthesis/adjoint 50/50), 4 streams, 200 GB "
+                                  "HBM budget", "requests": nreq, "offered_rps": lam,
+                      "closed_loop_capacity_rps": cap_rps, "direct_plans": methods.count("direct")},
+           "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
+                          "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max())},
+           "requests_per_s": nreq / el, "plan_build_s": t_plans, "server_stats": st,
+           "roofline": None, "cpu_baseline": None}
     print(json.dumps(out))
 
 

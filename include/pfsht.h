@@ -115,6 +115,43 @@ int pfsht_plan_rings(pfsht_plan_t plan, pfsht_rings_t *out);
 int pfsht_set_timing(pfsht_plan_t plan, int32_t enable);
 int pfsht_stage_times(pfsht_plan_t plan, int32_t dir, double *ms, int64_t *counts, int32_t reset);
 
+/* ---------------------------------------------------------------------------------------------
+ * Serving runtime. No counterpart in the reference, whose plans are built and applied by their
+ * caller (src/Plan.jl:84-148); it is for services answering requests on recurring point sets.
+ * A server keeps plans resident in HBM under caller-chosen 64-bit keys, builds a plan on a miss and
+ * evicts least-recently-used plans whose last apply has finished when the resident plans would
+ * exceed hbm_budget_bytes. Applies go round-robin over nstreams HIP streams (default priority);
+ * applies of one plan stay ordered (a plan orders its own applies across streams). All calls are
+ * thread-safe.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct pfsht_server_s *pfsht_server_t;
+
+typedef struct {
+    int64_t hits, misses, evictions, rejected; /* plan lookups; plans refused for lack of HBM */
+    int64_t submitted;                          /* applies enqueued */
+    int64_t resident_bytes, budget_bytes, plans;
+} pfsht_server_stats;
+
+int pfsht_server_create(int32_t nstreams, int64_t hbm_budget_bytes, pfsht_server_t *out);
+void pfsht_server_destroy(pfsht_server_t srv);
+/* The plan cached under key, built (method, FP64) on a miss; *hit (may be NULL) tells which. The
+ * plan stays owned by the server: do not destroy it. */
+int pfsht_server_plan(pfsht_server_t srv, uint64_t key, const double *theta, const double *phi, int64_t npts,
+                      int32_t lmax, double tol, int32_t ntrans, int32_t method, pfsht_plan_t *plan, int32_t *hit);
+/* Enqueue one apply of the plan under key: dir 0 = mul!(out = f, plan, in = alm), dir 1 =
+ * mul!(out = alm, plan', in = f). in/out are device buffers; returns a ticket. */
+int pfsht_server_submit(pfsht_server_t srv, uint64_t key, int32_t dir, const void *in, void *out, int64_t *ticket);
+/* *done = 1 once the apply has finished; *device_ms (may be NULL) is then its time on its stream. */
+int pfsht_server_query(pfsht_server_t srv, int64_t ticket, int32_t *done, double *device_ms);
+int pfsht_server_wait(pfsht_server_t srv, int64_t ticket, double *device_ms);
+int pfsht_server_get_stats(pfsht_server_t srv, pfsht_server_stats *stats);
+/* Device clock: pfsht_server_epoch records a reference event and waits for it; pfsht_server_times
+ * then gives a ticket's start (reached on its stream) and end, in ms after the epoch, waiting for
+ * the apply to finish. With the host clock read right after the epoch, request latencies follow
+ * without polling. */
+int pfsht_server_epoch(pfsht_server_t srv);
+int pfsht_server_times(pfsht_server_t srv, int64_t ticket, double *start_ms, double *end_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,6 +267,11 @@ struct pfsht_plan_s {
     int32_t *dperm = nullptr, *dmlim = nullptr;
     int64_t bytes = 0;
     std::mutex mu;
+    // Applies of one plan share its work buffers, so they are ordered across streams: every apply
+    // records `done` on its stream, and an apply on another stream first waits for it.
+    hipEvent_t done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool has_done = false;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> pool;
@@ -285,8 +290,22 @@ struct pfsht_plan_s {
         fft_free(fine_bwd); fft_free(fine_fwd);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
     }
 };
+
+// Orders an apply on stream s after the plan's previous apply (on any stream).
+static Status apply_begin(pfsht_plan_s &p, hipStream_t s) {
+    if (!p.done) PFSHT_HIP_TRY(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
+    if (p.has_done && s != p.last_stream) PFSHT_HIP_TRY(hipStreamWaitEvent(s, p.done, 0));
+    return kOk;
+}
+static Status apply_end(pfsht_plan_s &p, hipStream_t s) {
+    PFSHT_HIP_TRY(hipEventRecord(p.done, s));
+    p.has_done = true;
+    p.last_stream = s;
+    return kOk;
+}
 
 static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi, int64_t M, int L, double tol, int B,
                          int method) {
@@ -508,7 +527,7 @@ static void timing_drain(pfsht_plan_s &p) {
     p.pending.clear();
 }
 
-static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
@@ -549,7 +568,7 @@ static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     return kOk;
 }
 
-static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 1);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:94-117
@@ -589,6 +608,20 @@ static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     tmark(p, t, 9, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
+}
+
+static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+    Status st = apply_begin(p, s);
+    if (st == kOk) st = synth_body(p, alm, f, s);
+    if (st == kOk) st = apply_end(p, s);
+    return st;
+}
+
+static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+    Status st = apply_begin(p, s);
+    if (st == kOk) st = adjoint_body(p, f, alm, s);
+    if (st == kOk) st = apply_end(p, s);
+    return st;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -676,6 +709,7 @@ int pfsht_synth(pfsht_plan_t p, const void *alm, void *f, int32_t where, void *s
     const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;
     if (where == PFSHT_WHERE_DEVICE) return synth_impl(*p, static_cast<const double2 *>(alm), static_cast<double2 *>(f), s);
     Status st;
+    if ((st = apply_begin(*p, s)) != kOk) return st;  // the staging buffers belong to the plan too
     if ((st = stage_in(reinterpret_cast<void **>(&p->alm_stage), alm_bytes, alm, s, p->bytes)) != kOk) return st;
     if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), f_bytes, nullptr, s, p->bytes)) != kOk) return st;
     if ((st = synth_impl(*p, p->alm_stage, p->f_stage, s)) != kOk) return st;
@@ -695,6 +729,7 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
     if (where == PFSHT_WHERE_DEVICE)
         return adjoint_impl(*p, static_cast<const double2 *>(f), static_cast<double2 *>(alm), s);
     Status st;
+    if ((st = apply_begin(*p, s)) != kOk) return st;  // the staging buffers belong to the plan too
     if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), f_bytes, f, s, p->bytes)) != kOk) return st;
     if ((st = stage_in(reinterpret_cast<void **>(&p->alm_stage), alm_bytes, nullptr, s, p->bytes)) != kOk) return st;
     if ((st = adjoint_impl(*p, p->f_stage, p->alm_stage, s)) != kOk) return st;
@@ -718,6 +753,7 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
     // the host; the plan's Ft holds the internal [i][k] layout.
     double2 *modes_ref = nullptr, *fbuf = nullptr;
     Status st;
+    if ((st = apply_begin(*p, s)) != kOk) return st;
     const void *in_d = in;
     void *out_d = out;
     if (where == PFSHT_WHERE_HOST) {
@@ -744,6 +780,7 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
         launch_transpose(p->Ft, static_cast<double2 *>(out_d), p->nth, p->nph, p->B, s);
     }
     PFSHT_HIP_TRY(hipGetLastError());
+    if ((st = apply_end(*p, s)) != kOk) return st;
     if (where == PFSHT_WHERE_HOST) {
         PFSHT_HIP_TRY(hipMemcpyAsync(out, out_d, type == 1 ? modes_bytes : f_bytes, hipMemcpyDeviceToHost, s));
         PFSHT_HIP_TRY(hipStreamSynchronize(s));

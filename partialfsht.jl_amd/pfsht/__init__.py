@@ -77,6 +77,18 @@ EXPORTS = {
     "pfsht_plan_nufft": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "pfsht_stage_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pfsht_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "pfsht_server_destroy": (None, [C.c_void_p]),
+    "pfsht_server_plan": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
+                                    C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_int32)]),
+    "pfsht_server_submit": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_int64)]),
+    "pfsht_server_query": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "pfsht_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
+    "pfsht_server_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pfsht_server_epoch": (C.c_int, [C.c_void_p]),
+    "pfsht_server_times": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -328,6 +340,81 @@ class AdjointPlan:
         p = self.parent
         alm = zeros_alm(p.lmax, p.ntrans)
         return p.adjoint(alm, f)
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("hits", C.c_int64), ("misses", C.c_int64), ("evictions", C.c_int64),
+                ("rejected", C.c_int64), ("submitted", C.c_int64), ("resident_bytes", C.c_int64),
+                ("budget_bytes", C.c_int64), ("plans", C.c_int64)]
+
+
+class Server:
+    """Serving runtime over pfsht_server_* (include/pfsht.h): plans cached in HBM under integer keys
+    with an HBM budget and LRU eviction, applies spread over nstreams streams, completion tickets.
+    Inputs and outputs are device (torch) buffers."""
+
+    def __init__(self, nstreams: int = 4, hbm_budget_bytes: int = 200 * 2**30):
+        h = C.c_void_p()
+        _check(lib().pfsht_server_create(int(nstreams), int(hbm_budget_bytes), C.byref(h)))
+        self._h = h
+        self._plans = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pfsht_server_destroy(self._h)
+            self._h = None
+
+    def plan(self, key: int, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
+             method: str = "auto"):
+        """The plan cached under key (built on a miss). Returns (info dict, hit)."""
+        th = np.ascontiguousarray(theta, np.float64)
+        ph = np.ascontiguousarray(phi, np.float64)
+        if th.shape != ph.shape:
+            raise DimensionMismatch("theta and phi must have the same length")
+        hp = C.c_void_p()
+        hit = C.c_int32()
+        _check(lib().pfsht_server_plan(self._h, C.c_uint64(key), th.ctypes.data, ph.ctypes.data, len(th),
+                                       int(lmax), float(tol), int(ntrans), METHODS[method], C.byref(hp),
+                                       C.byref(hit)))
+        inf = InfoC()
+        _check(lib().pfsht_plan_info(hp, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in InfoC._fields_}, bool(hit.value)
+
+    def submit(self, key: int, direction: str, inp, out) -> int:
+        """Enqueue mul!(out, plan, inp) ("synth") or mul!(out, plan', inp) ("adjoint"); returns a
+        ticket."""
+        t = C.c_int64()
+        d = 0 if direction == "synth" else 1
+        _check(lib().pfsht_server_submit(self._h, C.c_uint64(key), d, C.c_void_p(inp.data_ptr()),
+                                         C.c_void_p(out.data_ptr()), C.byref(t)))
+        return t.value
+
+    def query(self, ticket: int):
+        """(done, device_ms) of a ticket, without waiting."""
+        done = C.c_int32()
+        ms = C.c_double(-1.0)
+        _check(lib().pfsht_server_query(self._h, int(ticket), C.byref(done), C.byref(ms)))
+        return bool(done.value), ms.value
+
+    def wait(self, ticket: int) -> float:
+        ms = C.c_double(-1.0)
+        _check(lib().pfsht_server_wait(self._h, int(ticket), C.byref(ms)))
+        return ms.value
+
+    def epoch(self):
+        """Record the device-clock reference of times() and wait for it."""
+        _check(lib().pfsht_server_epoch(self._h))
+
+    def times(self, ticket: int):
+        """(start_ms, end_ms) of a ticket after the epoch, on the device clock (waits for it)."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().pfsht_server_times(self._h, int(ticket), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def stats(self) -> dict:
+        st = ServerStatsC()
+        _check(lib().pfsht_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStatsC._fields_}
 
 
 def mul(out, op, x, stream=None):

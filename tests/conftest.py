@@ -40,6 +40,16 @@ def cap_points(rng, deg, M):
     return th, ph
 
 
+def centered_cap_points(rng, deg, M, theta_c, phi_c=0.0):
+    """Area-uniform points on the cap of radius deg degrees around (theta_c, phi_c): a polar cap
+    rotated about the y axis by theta_c, then about z by phi_c."""
+    th, ph = cap_points(rng, deg, M)
+    x, y, z = np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)
+    c, s = np.cos(theta_c), np.sin(theta_c)
+    x, z = c * x + s * z, -s * x + c * z
+    return np.arccos(np.clip(z, -1.0, 1.0)), np.mod(np.arctan2(y, x) + phi_c, 2 * np.pi)
+
+
 def relmax(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

@@ -58,3 +58,19 @@ def test_host_validation_before_device():
     assert L.pfsht_plan_create(p, p, 4, 8, 1e-8, 1, 0, 7, C.byref(out)) == 6       # precision
     with pytest.raises(pfsht.DimensionMismatch):
         pfsht.Plan(np.zeros(3), np.zeros(4), lmax=8)
+
+
+def test_server_argument_errors():
+    # serving runtime: argument checks come back as codes before any device work
+    import pfsht
+    L = pfsht.lib()
+    out = C.c_void_p()
+    assert L.pfsht_server_create(0, 1 << 30, C.byref(out)) == 2      # no streams
+    assert L.pfsht_server_create(65, 1 << 30, C.byref(out)) == 2     # too many streams
+    assert L.pfsht_server_create(2, 0, C.byref(out)) == 2            # no budget
+    assert L.pfsht_server_create(2, 1 << 30, None) == 2
+    t = C.c_int64()
+    assert L.pfsht_server_submit(None, 0, 0, None, None, C.byref(t)) == 2
+    done = C.c_int32()
+    assert L.pfsht_server_query(None, 0, C.byref(done), None) == 2
+    assert L.pfsht_server_get_stats(None, None) == 2
