@@ -156,9 +156,6 @@ def run_plain(args):
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} steps)")
 
-    plan.set_timing(True)
-    plan.stage_times("synth", reset=True)
-    plan.stage_times("adjoint", reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -166,8 +163,26 @@ def run_plain(args):
         plan.adjoint(a_out, f_d, stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    plan.set_timing(False)
     elapsed = t1 - t0
+    # per-stage device times from HIP events on the apply stream, in a separate pass (recording
+    # events costs device time, so the timed steps above run without them); also the per-apply
+    # latency distribution from events around each apply
+    plan.set_timing(True)
+    plan.stage_times("synth", reset=True)
+    plan.stage_times("adjoint", reset=True)
+    ev = []
+    for _ in range(args.steps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        plan.synth(f_out, alm_d, stream)
+        e[1].record(stream)
+        plan.adjoint(a_out, f_d, stream)
+        e[2].record(stream)
+        ev.append(e)
+    torch.cuda.synchronize()
+    plan.set_timing(False)
+    lat_s = np.array([e[0].elapsed_time(e[1]) for e in ev])
+    lat_a = np.array([e[1].elapsed_time(e[2]) for e in ev])
     ms_step = elapsed / args.steps * 1e3
     applies_s = 2 * args.steps / elapsed
     ss = plan.stage_times("synth")
@@ -230,6 +245,11 @@ def run_plain(args):
                       "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
                       "es_w": info["es_w"]},
            "triples_per_s": W * 2 * args.steps / elapsed,
+           "apply_latency_ms": {"synthesis": {"p50": float(np.percentile(lat_s, 50)),
+                                              "p99": float(np.percentile(lat_s, 99))},
+                                "adjoint": {"p50": float(np.percentile(lat_a, 50)),
+                                            "p99": float(np.percentile(lat_a, 99))},
+                                "note": "device time of single applies on an idle GPU, stage events on"},
            "plan_s": t_plan,
            "plan_s_warm": t_plan_warm,
            "stage_ms": per_stage,

@@ -241,9 +241,12 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
 namespace {
 constexpr int kStages = PFSHT_NSTAGES;
 // Events in execution order; interval k (ev[k] -> ev[k+1]) is stage stage_of[k].
+// Boundaries with no work in between are not recorded (an event costs a few microseconds of device
+// time): recorded[k] says whether ev[k] was; an interval ending at an unrecorded boundary is empty.
 struct TimedApply {
     int dir;
     hipEvent_t ev[kStages + 1];
+    bool recorded[kStages + 1];
     int stage_of[kStages];
 };
 }  // namespace
@@ -259,8 +262,10 @@ struct pfsht_plan_s {
     pfsht_rings_s *rings = nullptr;
     double *wth = nullptr;
     NufftDev nu;
-    double2 *Fp = nullptr, *Ft = nullptr, *U = nullptr;
-    FftPlan phi_bwd, phi_fwd, grid_fwd, grid_bwd, fine_bwd, fine_fwd;
+    // Ft: the periodised grid, either in order-column layout [b][m + mmax][i] (the apply) or
+    // [b][i][k] (the NUFFT seam); U: the fine grid [b][l1][l2]
+    double2 *Ft = nullptr, *U = nullptr;
+    FftPlan th_fwd, th_bwd, fine_bwd, fine_fwd;
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     // Direct leaf: theta-sorted points as rings
     double *dphi = nullptr;
@@ -284,9 +289,9 @@ struct pfsht_plan_s {
         dfree(wth);
         dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c1); dfree(nu.c2);
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
-        dfree(Fp); dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
+        dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
         dfree(dphi); dfree(dperm); dfree(dmlim);
-        fft_free(phi_bwd); fft_free(phi_fwd); fft_free(grid_fwd); fft_free(grid_bwd);
+        fft_free(th_fwd); fft_free(th_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -467,14 +472,13 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if ((st = upload(&u.bin_ofs, bo, by)) != kOk) return st;
     if ((st = dalloc(&u.Wst, size_t(B) * std::max(u.nst, 1) * u.win * u.win, by)) != kOk) return st;
     // work buffers
-    if ((st = dalloc(&p.Fp, size_t(B) * p.nh * p.nph, by)) != kOk) return st;
-    if ((st = dalloc(&p.Ft, size_t(B) * p.nth * p.nph, by)) != kOk) return st;
+    if ((st = dalloc(&p.Ft, size_t(B) * p.nth * p.nph, by)) != kOk) return st;  // nph >= 2 mmax + 1
     if ((st = dalloc(&p.U, size_t(B) * u.N1 * u.N2, by)) != kOk) return st;
     // rocFFT plans (row-major, contiguous; lengths fastest first)
-    if ((st = fft_make(p.phi_bwd, false, p.nph, 0, size_t(B) * p.nh, by)) != kOk) return st;
-    if ((st = fft_make(p.phi_fwd, true, p.nph, 0, size_t(B) * p.nh, by)) != kOk) return st;
-    if ((st = fft_make(p.grid_fwd, true, p.nph, p.nth, B, by)) != kOk) return st;
-    if ((st = fft_make(p.grid_bwd, false, p.nph, p.nth, B, by)) != kOk) return st;
+    // colatitude FFTs of the order columns (the longitude FFT pair of the reference cancels)
+    const size_t ncols = size_t(B) * (2 * p.mmax + 1);
+    if ((st = fft_make(p.th_fwd, true, p.nth, 0, ncols, by)) != kOk) return st;
+    if ((st = fft_make(p.th_bwd, false, p.nth, 0, ncols, by)) != kOk) return st;
     if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;
     if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
     PFSHT_HIP_TRY(hipDeviceSynchronize());
@@ -489,13 +493,14 @@ static GridDims dims_of(const pfsht_plan_s &p) {
 
 // stage timing: events recorded on the apply stream, drained by pfsht_stage_times
 static void tmark(pfsht_plan_s &p, TimedApply *t, int k, hipStream_t s) {
-    if (t) (void)hipEventRecord(t->ev[k], s);
+    if (t) t->recorded[k] = hipEventRecord(t->ev[k], s) == hipSuccess;
 }
 
 static TimedApply *timing_begin(pfsht_plan_s &p, int dir) {
     if (!p.timing) return nullptr;
     TimedApply t;
     t.dir = dir;
+    for (int k = 0; k <= kStages; ++k) t.recorded[k] = false;
     // synthesis runs stages 0..7 in order (8 is empty); the adjoint runs them transposed, 7..0,
     // then the ring partial reduction, 8
     for (int k = 0; k < kStages; ++k) t.stage_of[k] = (dir == 0 || k == kStages - 1) ? k : kStages - 2 - k;
@@ -514,13 +519,18 @@ static TimedApply *timing_begin(pfsht_plan_s &p, int dir) {
 
 static void timing_drain(pfsht_plan_s &p) {
     for (auto &t : p.pending) {
-        (void)hipEventSynchronize(t.ev[kStages]);
+        int last = kStages;
+        while (last > 0 && !t.recorded[last]) --last;
+        (void)hipEventSynchronize(t.ev[last]);
+        int from = 0;  // last recorded boundary
         for (int k = 0; k < kStages; ++k) {
             float ms = 0;
-            if (hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]) == hipSuccess) {
-                p.ms[t.dir][t.stage_of[k]] += ms;
-                p.counts[t.dir][t.stage_of[k]] += 1;
+            if (t.recorded[k + 1]) {
+                if (hipEventElapsedTime(&ms, t.ev[from], t.ev[k + 1]) != hipSuccess) ms = 0;
+                from = k + 1;
             }
+            p.ms[t.dir][t.stage_of[k]] += ms;
+            p.counts[t.dir][t.stage_of[k]] += 1;
         }
         for (auto e : t.ev) p.pool.push_back(e);
     }
@@ -534,36 +544,30 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
         launch_ring_fwd(r.d, alm, r.Gd, s);
-        for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
+        tmark(p, t, 1, s);
         launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, p.B, s);
-        tmark(p, t, 8, s);
-        tmark(p, t, 9, s);
+        tmark(p, t, 2, s);  // the phase sums are timed as stage 1
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
     }
+    // src/Stretch.jl:262-277: ring pass, window + DFS (in longitude-spectral space), colatitude FFT,
+    // phase and deconvolution into the fine grid, fine inverse FFT, interpolation
     const GridDims g = dims_of(p);
     pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
     launch_ring_fwd(r.d, alm, r.Gd, s);
     tmark(p, t, 1, s);
-    launch_pack_window(g, r.d, r.Gd, p.wth, p.Fp, s);
-    tmark(p, t, 2, s);
-    Status st = fft_exec(p.phi_bwd, p.Fp, s);
+    launch_spec_dfs(g, r.d, r.Gd, p.wth, p.Ft, s);
+    tmark(p, t, 2, s);  // stages 2 and 3 (longitude FFT, DFS) have no work of their own
+    Status st = fft_exec(p.th_fwd, p.Ft, s);
     if (st != kOk) return st;
-    tmark(p, t, 3, s);
-    double2 *Ft = p.Ft;
-    if (p.geom.kind == kInterior) Ft = p.Fp;
-    else launch_dfs_double(g, p.Fp, p.Ft, s);
-    tmark(p, t, 4, s);
-    if ((st = fft_exec(p.grid_fwd, Ft, s)) != kOk) return st;
     tmark(p, t, 5, s);
-    launch_deconv_pad(p.nu, p.nu.pc1, Ft, p.U, s);
+    launch_deconv_pad_cols(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ft, p.U, s);
     tmark(p, t, 6, s);
     if ((st = fft_exec(p.fine_bwd, p.U, s)) != kOk) return st;
     tmark(p, t, 7, s);
     launch_interp(p.nu, p.U, f, s);
-    tmark(p, t, 8, s);
-    tmark(p, t, 9, s);
+    tmark(p, t, 8, s);  // stage 8 is the adjoint's
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
@@ -575,7 +579,7 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
         launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);
-        for (int k = 1; k <= 7; ++k) tmark(p, t, k, s);
+        tmark(p, t, 7, s);  // the phase products are timed as stage 1 (adjoint order)
         launch_ring_adj_tiles(r.d, r.Gd, s);
         tmark(p, t, 8, s);
         launch_ring_adj_finalize(r.d, alm, s);
@@ -583,6 +587,7 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
     }
+    // src/Stretch.jl:279-295, the exact transpose of the synthesis
     const GridDims g = dims_of(p);
     pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
@@ -591,16 +596,11 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     Status st = fft_exec(p.fine_fwd, p.U, s);
     if (st != kOk) return st;
     tmark(p, t, 2, s);
-    double2 *Ft = p.geom.kind == kInterior ? p.Fp : p.Ft;
-    launch_deconv_trunc(p.nu, p.nu.pc1, p.U, Ft, s);
+    launch_deconv_trunc_cols(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.U, p.Ft, s);
     tmark(p, t, 3, s);
-    if ((st = fft_exec(p.grid_bwd, Ft, s)) != kOk) return st;
+    if ((st = fft_exec(p.th_bwd, p.Ft, s)) != kOk) return st;
     tmark(p, t, 4, s);
-    if (p.geom.kind != kInterior) launch_dfs_fold(g, p.Ft, p.Fp, s);
-    tmark(p, t, 5, s);
-    if ((st = fft_exec(p.phi_fwd, p.Fp, s)) != kOk) return st;
-    tmark(p, t, 6, s);
-    launch_unpack_window(g, p.Fp, p.wth, r.Gd, s);
+    launch_spec_fold(g, p.Ft, p.wth, r.Gd, s);
     tmark(p, t, 7, s);
     launch_ring_adj_tiles(r.d, r.Gd, s);
     tmark(p, t, 8, s);

@@ -135,6 +135,89 @@ __global__ void deconv_trunc_kernel(const double2 *__restrict__ U, const double2
     Fhat[(int64_t(b) * n1 + i) * n2 + k] = make_double2((u.x * p.x + u.y * p.y) * s, (u.y * p.x - u.x * p.y) * s);
 }
 
+// The same two maps from the order-column layout F^[b][m + mmax][k1] (k1 < n1 in FFT order) of the
+// spectral periodisation, through a 32 x 32 LDS tile so that both sides are read and written along
+// their contiguous index. `scale` carries the longitude FFT pair the spectral path skips (nph).
+constexpr int kDT = 32;
+
+__device__ __forceinline__ int fine_to_mode(int l, int n, int N) {
+    if (l < (n + 1) / 2) return l;
+    if (l >= N - n / 2) return l - N + n;
+    return -1;
+}
+
+__global__ void __launch_bounds__(256)
+deconv_pad_cols_kernel(const double2 *__restrict__ Fc, const double2 *__restrict__ pc1, const double *__restrict__ c2,
+                       double2 *__restrict__ U, int n1, int mmax, int nph, int N1, int N2, double scale) {
+    __shared__ double2 tile[kDT][kDT + 1];
+    const int l2_0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
+    const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;  // 32 x 8
+    const int64_t ncol = 2 * mmax + 1;
+    // stage: rows = l2 (orders), columns = l1 (colatitude modes, contiguous in Fc)
+    for (int rr = ty; rr < kDT; rr += 8) {
+        const int l2 = l2_0 + rr, l1 = l1_0 + tx;
+        double2 v = make_double2(0.0, 0.0);
+        if (l2 < N2 && l1 < N1) {
+            const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
+            const int i = fine_to_mode(l1, n1, N1);
+            if (m != INT32_MIN && i >= 0) {
+                const double2 F = Fc[(int64_t(b) * ncol + (m + mmax)) * n1 + i];
+                const double2 p = pc1[i];
+                const double sc = c2[m >= 0 ? m : m + nph] * scale;
+                v = make_double2((F.x * p.x - F.y * p.y) * sc, (F.x * p.y + F.y * p.x) * sc);
+            }
+        }
+        tile[rr][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < kDT; rr += 8) {
+        const int l1 = l1_0 + rr, l2 = l2_0 + tx;
+        if (l1 < N1 && l2 < N2) U[(int64_t(b) * N1 + l1) * N2 + l2] = tile[tx][rr];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+deconv_trunc_cols_kernel(const double2 *__restrict__ U, const double2 *__restrict__ pc1, const double *__restrict__ c2,
+                         double2 *__restrict__ Fc, int n1, int mmax, int nph, int N1, int N2, double scale) {
+    __shared__ double2 tile[kDT][kDT + 1];
+    const int c0 = blockIdx.x * kDT, i0 = blockIdx.y * kDT, b = blockIdx.z;
+    const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
+    const int ncol = 2 * mmax + 1;
+    // stage: rows = i (colatitude modes), columns = c (orders; their fine-grid columns are contiguous
+    // within each sign of m)
+    for (int rr = ty; rr < kDT; rr += 8) {
+        const int i = i0 + rr, c = c0 + tx;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n1 && c < ncol) {
+            const int m = c - mmax;
+            const int l1 = i < (n1 + 1) / 2 ? i : i - n1 + N1;
+            const int l2 = m >= 0 ? m : m + N2;
+            const double2 u = U[(int64_t(b) * N1 + l1) * N2 + l2];
+            const double2 p = pc1[i];  // conj(p) for the adjoint
+            const double sc = c2[m >= 0 ? m : m + nph] * scale;
+            v = make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc);
+        }
+        tile[rr][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < kDT; rr += 8) {
+        const int c = c0 + rr, i = i0 + tx;
+        if (c < ncol && i < n1) Fc[(int64_t(b) * ncol + c) * n1 + i] = tile[tx][rr];
+    }
+}
+
+void launch_deconv_pad_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                            const double2 *Fc, double2 *U, hipStream_t s) {
+    dim3 grid((u.N2 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
+    deconv_pad_cols_kernel<<<grid, 256, 0, s>>>(Fc, pc1, u.c2, U, u.n1, mmax, nph, u.N1, u.N2, scale);
+}
+
+void launch_deconv_trunc_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                              const double2 *U, double2 *Fc, hipStream_t s) {
+    dim3 grid((2 * mmax + 1 + kDT - 1) / kDT, (u.n1 + kDT - 1) / kDT, u.B);
+    deconv_trunc_cols_kernel<<<grid, 256, 0, s>>>(U, pc1, u.c2, Fc, u.n1, mmax, nph, u.N1, u.N2, scale);
+}
+
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s) {
     dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
     deconv_pad_kernel<<<grid, 256, 0, s>>>(Fhat, pc1, u.c2, U, u.n1, u.n2, u.N1, u.N2);

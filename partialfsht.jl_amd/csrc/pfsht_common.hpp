@@ -70,11 +70,10 @@ void launch_ref_to_gd(const RingDev &r, const double2 *G, double2 *Gd, hipStream
 struct GridDims {
     int B, nh, nth, nph, mmax, L, kind;
 };
-void launch_pack_window(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, double2 *Fp,
-                        hipStream_t s);
-void launch_unpack_window(const GridDims &g, const double2 *Fp, const double *wth, double2 *Gd, hipStream_t s);
-void launch_dfs_double(const GridDims &g, const double2 *Fp, double2 *Ft, hipStream_t s);
-void launch_dfs_fold(const GridDims &g, const double2 *Ft, double2 *Fp, hipStream_t s);
+// Periodisation in longitude-spectral space, order-column layout Ds [b][m + mmax][i] (i < nth).
+void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, double2 *Ds,
+                     hipStream_t s);
+void launch_spec_fold(const GridDims &g, const double2 *Ds, const double *wth, double2 *Gd, hipStream_t s);
 
 // Direct leaf: phase sums over the ring-engine output of theta-sorted points.
 void launch_direct_synth(const double2 *Gd, const double *phis, const int32_t *mlim, const int32_t *perm,
@@ -111,6 +110,12 @@ inline size_t spread_lds_bytes(int win) { return nufft_window_lds_bytes(win) + s
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
+// The phase/deconvolution maps between the order-column layout F^[b][m + mmax][k1] and the fine grid;
+// scale multiplies every mode (the longitude FFT pair the spectral periodisation skips).
+void launch_deconv_pad_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                            const double2 *Fc, double2 *U, hipStream_t s);
+void launch_deconv_trunc_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                              const double2 *U, double2 *Fc, hipStream_t s);
 // out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
 void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
 void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s);
