@@ -265,7 +265,8 @@ struct pfsht_plan_s {
     // Ft: the periodised grid, either in order-column layout [b][m + mmax][i] (the apply) or
     // [b][i][k] (the NUFFT seam); U: the fine grid [b][l1][l2]
     double2 *Ft = nullptr, *U = nullptr;
-    FftPlan th_fwd, th_bwd, fine_bwd, fine_fwd;
+    double2 *Ut = nullptr;  // nonzero fine columns, order-column layout [b][m + mmax][l1]
+    FftPlan th_fwd, th_bwd, fine_bwd, fine_fwd, col_bwd, col_fwd, row_bwd, row_fwd;
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     // Direct leaf: theta-sorted points as rings
     double *dphi = nullptr;
@@ -289,10 +290,11 @@ struct pfsht_plan_s {
         dfree(wth);
         dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c1); dfree(nu.c2);
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
-        dfree(Ft); dfree(U); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
+        dfree(Ft); dfree(U); dfree(Ut); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
         dfree(dphi); dfree(dperm); dfree(dmlim);
         fft_free(th_fwd); fft_free(th_bwd);
-        fft_free(fine_bwd); fft_free(fine_fwd);
+        fft_free(fine_bwd); fft_free(fine_fwd); fft_free(col_bwd); fft_free(col_fwd);
+        fft_free(row_bwd); fft_free(row_fwd);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
@@ -479,8 +481,14 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     const size_t ncols = size_t(B) * (2 * p.mmax + 1);
     if ((st = fft_make(p.th_fwd, true, p.nth, 0, ncols, by)) != kOk) return st;
     if ((st = fft_make(p.th_bwd, false, p.nth, 0, ncols, by)) != kOk) return st;
-    if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;
+    if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;  // the NUFFT seam
     if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
+    // the apply's pruned fine transform: colatitude FFTs of the nonzero columns, longitude FFTs of rows
+    if ((st = dalloc(&p.Ut, ncols * u.N1, by)) != kOk) return st;
+    if ((st = fft_make(p.col_bwd, false, u.N1, 0, ncols, by)) != kOk) return st;
+    if ((st = fft_make(p.col_fwd, true, u.N1, 0, ncols, by)) != kOk) return st;
+    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.N1, by)) != kOk) return st;
+    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.N1, by)) != kOk) return st;
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
@@ -562,9 +570,11 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     Status st = fft_exec(p.th_fwd, p.Ft, s);
     if (st != kOk) return st;
     tmark(p, t, 5, s);
-    launch_deconv_pad_cols(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ft, p.U, s);
+    launch_deconv_pad_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ft, p.Ut, s);
     tmark(p, t, 6, s);
-    if ((st = fft_exec(p.fine_bwd, p.U, s)) != kOk) return st;
+    if ((st = fft_exec(p.col_bwd, p.Ut, s)) != kOk) return st;
+    launch_ut_to_rows(p.nu, p.mmax, p.Ut, p.U, s);
+    if ((st = fft_exec(p.row_bwd, p.U, s)) != kOk) return st;
     tmark(p, t, 7, s);
     launch_interp(p.nu, p.U, f, s);
     tmark(p, t, 8, s);  // stage 8 is the adjoint's
@@ -593,10 +603,12 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     tmark(p, t, 0, s);
     launch_spread(p.nu, f, p.U, s);
     tmark(p, t, 1, s);
-    Status st = fft_exec(p.fine_fwd, p.U, s);
+    Status st = fft_exec(p.row_fwd, p.U, s);
     if (st != kOk) return st;
+    launch_rows_to_ut(p.nu, p.mmax, p.U, p.Ut, s);
+    if ((st = fft_exec(p.col_fwd, p.Ut, s)) != kOk) return st;
     tmark(p, t, 2, s);
-    launch_deconv_trunc_cols(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.U, p.Ft, s);
+    launch_deconv_trunc_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ut, p.Ft, s);
     tmark(p, t, 3, s);
     if ((st = fft_exec(p.th_bwd, p.Ft, s)) != kOk) return st;
     tmark(p, t, 4, s);

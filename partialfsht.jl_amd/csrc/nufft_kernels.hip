@@ -135,9 +135,8 @@ __global__ void deconv_trunc_kernel(const double2 *__restrict__ U, const double2
     Fhat[(int64_t(b) * n1 + i) * n2 + k] = make_double2((u.x * p.x + u.y * p.y) * s, (u.y * p.x - u.x * p.y) * s);
 }
 
-// The same two maps from the order-column layout F^[b][m + mmax][k1] (k1 < n1 in FFT order) of the
-// spectral periodisation, through a 32 x 32 LDS tile so that both sides are read and written along
-// their contiguous index. `scale` carries the longitude FFT pair the spectral path skips (nph).
+// The apply reaches the fine grid from the order-column layout F^[b][m + mmax][k1] (k1 < n1 in FFT
+// order) of the spectral periodisation; `scale` carries the longitude FFT pair that path skips (nph).
 constexpr int kDT = 32;
 
 __device__ __forceinline__ int fine_to_mode(int l, int n, int N) {
@@ -146,26 +145,63 @@ __device__ __forceinline__ int fine_to_mode(int l, int n, int N) {
     return -1;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Pruned fine-grid transform. After the deconvolution only the 2 mmax + 1 fine columns of retained
+// orders are nonzero, so the fine 2-D FFT runs as a colatitude FFT over those columns alone (order-
+// column layout Ut [b][m + mmax][l1], contiguous) and a longitude FFT over all rows, with a
+// transpose that inserts the zero columns between them (synthesis), or drops them (adjoint).
+// ------------------------------------------------------------------------------------------------
+// Ut[b][c][l1] = F^[b][c][k1(l1)] pc1[k1] c2[m] scale, zero in the colatitude padding.
+__global__ void deconv_pad_ut_kernel(const double2 *__restrict__ Fc, const double2 *__restrict__ pc1,
+                                     const double *__restrict__ c2, double2 *__restrict__ Ut, int n1, int mmax,
+                                     int nph, int N1, double scale) {
+    const int l1 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b = blockIdx.z;
+    if (l1 >= N1) return;
+    const int64_t col = int64_t(b) * (2 * mmax + 1) + c;
+    const int i = fine_to_mode(l1, n1, N1);
+    double2 v = make_double2(0.0, 0.0);
+    if (i >= 0) {
+        const int m = c - mmax;
+        const double2 F = Fc[col * n1 + i];
+        const double2 p = pc1[i];
+        const double sc = c2[m >= 0 ? m : m + nph] * scale;
+        v = make_double2((F.x * p.x - F.y * p.y) * sc, (F.x * p.y + F.y * p.x) * sc);
+    }
+    Ut[col * N1 + l1] = v;
+}
+
+// F^[b][c][k1] = Ut[b][c][l1(k1)] conj(pc1[k1]) c2[m] scale.
+__global__ void deconv_trunc_ut_kernel(const double2 *__restrict__ Ut, const double2 *__restrict__ pc1,
+                                       const double *__restrict__ c2, double2 *__restrict__ Fc, int n1, int mmax,
+                                       int nph, int N1, double scale) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b = blockIdx.z;
+    if (i >= n1) return;
+    const int64_t col = int64_t(b) * (2 * mmax + 1) + c;
+    const int m = c - mmax;
+    const int l1 = i < (n1 + 1) / 2 ? i : i - n1 + N1;
+    const double2 u = Ut[col * N1 + l1];
+    const double2 p = pc1[i];  // conj(p) for the adjoint
+    const double sc = c2[m >= 0 ? m : m + nph] * scale;
+    Fc[col * n1 + i] = make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc);
+}
+
+// U[b][l1][l2] = Ut[b][c(l2)][l1] for the fine columns of retained orders, zero elsewhere.
 __global__ void __launch_bounds__(256)
-deconv_pad_cols_kernel(const double2 *__restrict__ Fc, const double2 *__restrict__ pc1, const double *__restrict__ c2,
-                       double2 *__restrict__ U, int n1, int mmax, int nph, int N1, int N2, double scale) {
+ut_to_rows_kernel(const double2 *__restrict__ Ut, double2 *__restrict__ U, int mmax, int N1, int N2) {
     __shared__ double2 tile[kDT][kDT + 1];
     const int l2_0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
-    const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;  // 32 x 8
+    const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
     const int64_t ncol = 2 * mmax + 1;
-    // stage: rows = l2 (orders), columns = l1 (colatitude modes, contiguous in Fc)
     for (int rr = ty; rr < kDT; rr += 8) {
         const int l2 = l2_0 + rr, l1 = l1_0 + tx;
         double2 v = make_double2(0.0, 0.0);
         if (l2 < N2 && l1 < N1) {
             const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
-            const int i = fine_to_mode(l1, n1, N1);
-            if (m != INT32_MIN && i >= 0) {
-                const double2 F = Fc[(int64_t(b) * ncol + (m + mmax)) * n1 + i];
-                const double2 p = pc1[i];
-                const double sc = c2[m >= 0 ? m : m + nph] * scale;
-                v = make_double2((F.x * p.x - F.y * p.y) * sc, (F.x * p.y + F.y * p.x) * sc);
-            }
+            if (m != INT32_MIN) v = Ut[(int64_t(b) * ncol + (m + mmax)) * N1 + l1];
         }
         tile[rr][tx] = v;
     }
@@ -176,46 +212,49 @@ deconv_pad_cols_kernel(const double2 *__restrict__ Fc, const double2 *__restrict
     }
 }
 
+// Ut[b][c][l1] = U[b][l1][l2(c)]: the transpose, keeping only the fine columns of retained orders.
 __global__ void __launch_bounds__(256)
-deconv_trunc_cols_kernel(const double2 *__restrict__ U, const double2 *__restrict__ pc1, const double *__restrict__ c2,
-                         double2 *__restrict__ Fc, int n1, int mmax, int nph, int N1, int N2, double scale) {
+rows_to_ut_kernel(const double2 *__restrict__ U, double2 *__restrict__ Ut, int mmax, int N1, int N2) {
     __shared__ double2 tile[kDT][kDT + 1];
-    const int c0 = blockIdx.x * kDT, i0 = blockIdx.y * kDT, b = blockIdx.z;
+    const int c0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
     const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
     const int ncol = 2 * mmax + 1;
-    // stage: rows = i (colatitude modes), columns = c (orders; their fine-grid columns are contiguous
-    // within each sign of m)
     for (int rr = ty; rr < kDT; rr += 8) {
-        const int i = i0 + rr, c = c0 + tx;
+        const int l1 = l1_0 + rr, c = c0 + tx;
         double2 v = make_double2(0.0, 0.0);
-        if (i < n1 && c < ncol) {
+        if (l1 < N1 && c < ncol) {
             const int m = c - mmax;
-            const int l1 = i < (n1 + 1) / 2 ? i : i - n1 + N1;
-            const int l2 = m >= 0 ? m : m + N2;
-            const double2 u = U[(int64_t(b) * N1 + l1) * N2 + l2];
-            const double2 p = pc1[i];  // conj(p) for the adjoint
-            const double sc = c2[m >= 0 ? m : m + nph] * scale;
-            v = make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc);
+            v = U[(int64_t(b) * N1 + l1) * N2 + (m >= 0 ? m : m + N2)];
         }
         tile[rr][tx] = v;
     }
     __syncthreads();
     for (int rr = ty; rr < kDT; rr += 8) {
-        const int c = c0 + rr, i = i0 + tx;
-        if (c < ncol && i < n1) Fc[(int64_t(b) * ncol + c) * n1 + i] = tile[tx][rr];
+        const int c = c0 + rr, l1 = l1_0 + tx;
+        if (c < ncol && l1 < N1) Ut[(int64_t(b) * ncol + c) * N1 + l1] = tile[tx][rr];
     }
 }
 
-void launch_deconv_pad_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                            const double2 *Fc, double2 *U, hipStream_t s) {
-    dim3 grid((u.N2 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
-    deconv_pad_cols_kernel<<<grid, 256, 0, s>>>(Fc, pc1, u.c2, U, u.n1, mmax, nph, u.N1, u.N2, scale);
+void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const double2 *Fc,
+                          double2 *Ut, hipStream_t s) {
+    dim3 grid((u.N1 + 255) / 256, 2 * mmax + 1, u.B);
+    deconv_pad_ut_kernel<<<grid, 256, 0, s>>>(Fc, pc1, u.c2, Ut, u.n1, mmax, nph, u.N1, scale);
 }
 
-void launch_deconv_trunc_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                              const double2 *U, double2 *Fc, hipStream_t s) {
-    dim3 grid((2 * mmax + 1 + kDT - 1) / kDT, (u.n1 + kDT - 1) / kDT, u.B);
-    deconv_trunc_cols_kernel<<<grid, 256, 0, s>>>(U, pc1, u.c2, Fc, u.n1, mmax, nph, u.N1, u.N2, scale);
+void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                            const double2 *Ut, double2 *Fc, hipStream_t s) {
+    dim3 grid((u.n1 + 255) / 256, 2 * mmax + 1, u.B);
+    deconv_trunc_ut_kernel<<<grid, 256, 0, s>>>(Ut, pc1, u.c2, Fc, u.n1, mmax, nph, u.N1, scale);
+}
+
+void launch_ut_to_rows(const NufftDev &u, int mmax, const double2 *Ut, double2 *U, hipStream_t s) {
+    dim3 grid((u.N2 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
+    ut_to_rows_kernel<<<grid, 256, 0, s>>>(Ut, U, mmax, u.N1, u.N2);
+}
+
+void launch_rows_to_ut(const NufftDev &u, int mmax, const double2 *U, double2 *Ut, hipStream_t s) {
+    dim3 grid((2 * mmax + 1 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
+    rows_to_ut_kernel<<<grid, 256, 0, s>>>(U, Ut, mmax, u.N1, u.N2);
 }
 
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s) {

@@ -110,12 +110,13 @@ inline size_t spread_lds_bytes(int win) { return nufft_window_lds_bytes(win) + s
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
-// The phase/deconvolution maps between the order-column layout F^[b][m + mmax][k1] and the fine grid;
-// scale multiplies every mode (the longitude FFT pair the spectral periodisation skips).
-void launch_deconv_pad_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                            const double2 *Fc, double2 *U, hipStream_t s);
-void launch_deconv_trunc_cols(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                              const double2 *U, double2 *Fc, hipStream_t s);
+// The pruned fine transform: order-column layout Ut [b][m + mmax][l1] of the nonzero fine columns.
+void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const double2 *Fc,
+                          double2 *Ut, hipStream_t s);
+void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
+                            const double2 *Ut, double2 *Fc, hipStream_t s);
+void launch_ut_to_rows(const NufftDev &u, int mmax, const double2 *Ut, double2 *U, hipStream_t s);
+void launch_rows_to_ut(const NufftDev &u, int mmax, const double2 *U, double2 *Ut, hipStream_t s);
 // out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
 void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
 void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s);
