@@ -317,7 +317,7 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 // ------------------------------------------------------------------------------------------------
 // Spreading (type 1): MFMA super-tiles + ordered merge.
 // ------------------------------------------------------------------------------------------------
-// Spreading: one workgroup per non-empty super-tile; wave w owns bins w, w+4, w+8, w+12 and takes
+// Spreading: one workgroup per non-empty super-tile; wave w owns bins w + kSpreadWaves k and takes
 // each bin's points 16 at a time. For such a chunk the wave first evaluates the kernel rows once per
 // point: lane (p, axis, h) evaluates the cells t < ceil(W/2) of its point and axis from the left
 // (h = 0) or, through phi(-z) = phi(z), from the right (h = 1), so all lanes use the same
@@ -328,7 +328,7 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 // the tiles are added into the LDS window wave after wave (bin windows overlap) and the window is
 // written out for the merge pass.
 template <int W>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * kSpreadWaves)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  const double2 *__restrict__ f, const int32_t *__restrict__ st_list,
                  const int32_t *__restrict__ bin_ofs, double2 *__restrict__ Wst, int64_t M, int S, int ns2,
@@ -349,19 +349,20 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     double2 *Fc = reinterpret_cast<double2 *>(K2 + 16 * kRowPad);  // [16]
     const int ii = lane & 15, kq = lane >> 4;                    // MFMA operand layout
     const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;  // kernel-row evaluation
-    for (int q = tid; q < win * win; q += 256) lw[q] = make_double2(0.0, 0.0);
+    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_double2(0.0, 0.0);
     const int32_t *__restrict__ bo = bin_ofs + slot * 16;
     // the bin being accumulated is always tile 0; after each bin the tiles rotate by one, so after
-    // four bins tile k holds bin wave + 4 k again
-    double4v dr[4], di[4];
+    // all of its bins tile k holds bin wave + kSpreadWaves k again
+    constexpr int NBW = 16 / kSpreadWaves;  // bins per wavefront
+    double4v dr[NBW], di[NBW];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NBW; ++k) {
         dr[k] = double4v{0.0, 0.0, 0.0, 0.0};
         di[k] = double4v{0.0, 0.0, 0.0, 0.0};
     }
 #pragma unroll 1
-    for (int k = 0; k < 4; ++k) {
-        const int lb = wave + 4 * k;
+    for (int k = 0; k < NBW; ++k) {
+        const int lb = wave + kSpreadWaves * k;
         const int ps = bo[lb], pe = bo[lb + 1];
         for (int c0 = ps; c0 < pe; c0 += 16) {
             const int n = min(16, pe - c0);
@@ -400,19 +401,24 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             wave_lds_order();
         }
         const double4v tr = dr[0], ti = di[0];
-        dr[0] = dr[1]; dr[1] = dr[2]; dr[2] = dr[3]; dr[3] = tr;
-        di[0] = di[1]; di[1] = di[2]; di[2] = di[3]; di[3] = ti;
+#pragma unroll
+        for (int q = 0; q + 1 < NBW; ++q) {
+            dr[q] = dr[q + 1];
+            di[q] = di[q + 1];
+        }
+        dr[NBW - 1] = tr;
+        di[NBW - 1] = ti;
     }
     __syncthreads();
     // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds row
     // i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Bin windows overlap
     // (16 > S), so the waves add into LDS one at a time, in wave order, each wave its bins in
     // order: race-free and a fixed summation order.
-    for (int turn = 0; turn < 4; ++turn) {
+    for (int turn = 0; turn < kSpreadWaves; ++turn) {
         if (wave == turn) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int lb = wave + 4 * k;
+            for (int k = 0; k < NBW; ++k) {
+                const int lb = wave + kSpreadWaves * k;
                 const int bi = lb >> 2, bj = lb & 3;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -425,7 +431,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         __syncthreads();
     }
     double2 *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
-    for (int q = tid; q < win * win; q += 256) out[q] = lw[q];
+    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) out[q] = lw[q];
 }
 
 __global__ void spread_merge_kernel(const double2 *__restrict__ Wst, const int32_t *__restrict__ st_slot,
@@ -474,7 +480,7 @@ template <int W>
 static void spread_w(const NufftDev &u, const double2 *f, hipStream_t s) {
     if (u.nst == 0) return;
     dim3 grid(u.nst, u.B);
-    spread_st_kernel<W><<<grid, 256, spread_lds_bytes(u.win), s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs,
+    spread_st_kernel<W><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win), s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs,
                                                                    u.Wst, u.M, u.S, u.ns2, u.win, u.nst);
 }
 

@@ -106,7 +106,10 @@ inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(
 // (two 16-row kernel tiles with rows kRowPad doubles apart, and 16 complex f values), in doubles.
 constexpr int kRowPad = 17;
 constexpr int kSpreadStage = 2 * 16 * kRowPad + 2 * 16;
-inline size_t spread_lds_bytes(int win) { return nufft_window_lds_bytes(win) + sizeof(double) * 4 * kSpreadStage; }
+constexpr int kSpreadWaves = 8;  // wavefronts per spreading workgroup: two of the 16 bins each
+inline size_t spread_lds_bytes(int win) {
+    return nufft_window_lds_bytes(win) + sizeof(double) * kSpreadWaves * kSpreadStage;
+}
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
