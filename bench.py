@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lmax", type=int, default=2048)
     ap.add_argument("--npts", type=int, default=2_000_000)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="cap30 / equatorial / cap15at60 only: mixed = FP64 ring recurrence with the grid "
+                         "buffers and FFTs in FP32 (needs tol >= 1e-6)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy three times (HBM counter calibration "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -285,9 +288,12 @@ def run_geometry(args):
     th, ph = centered_cap(rng, deg, M, tc, 0.5)
     alm = rand_alm(L, rng)
     f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
-    log(f"{args.config}: plan lmax={L} M={M} tol={tol}")
+    mixed = args.precision == "mixed"
+    if mixed and tol < 1e-6:
+        tol = 1e-6  # the tightest tolerance mixed precision serves
+    log(f"{args.config}: plan lmax={L} M={M} tol={tol} precision={args.precision}")
     t0 = time.perf_counter()
-    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method="stretch")
+    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method="stretch", precision=args.precision)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
     info = plan.info
@@ -334,9 +340,10 @@ def run_geometry(args):
     out = {"metric": f"applies/s at lmax=2048 tol={tol:g} (synth+adjoint)", "value": 2 * steps / el,
            "unit": "applies/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
+           "dtype": "f64 recurrence, f32 grid and FFT" if mixed else "f64", "data": "synthetic",
            "config": {"workload": f"{desc}, lmax={L}, M={M} area-uniform, tol={tol:g}, ntrans=1, "
-                                  "one synthesis + one adjoint per step",
+                                  f"{args.precision}, one synthesis + one adjoint per step",
+                      "precision": args.precision,
                       "lmax": L, "npts": M, "tol": tol, "kind": info["kind_name"], "nth": info["nth"],
                       "nph": info["nph"], "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
                       "es_w": info["es_w"]},

@@ -76,7 +76,9 @@ struct FftPlan {
     size_t work_bytes = 0;
 };
 
-static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, size_t batch, int64_t &bytes) {
+// single: FP32 data (the grid buffers of a mixed-precision plan), else FP64.
+static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, size_t batch, int64_t &bytes,
+                       bool single = false) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     if (g_rocfft_setup != rocfft_status_success) { set_error("rocfft_setup failed"); return kEHip; }
     size_t lengths[2] = {n_fast, n_slow};
@@ -84,7 +86,8 @@ static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, 
     rocfft_status s = rocfft_plan_create(&fp.plan, rocfft_placement_inplace,
                                          forward ? rocfft_transform_type_complex_forward
                                                  : rocfft_transform_type_complex_inverse,
-                                         rocfft_precision_double, dims, lengths, batch, nullptr);
+                                         single ? rocfft_precision_single : rocfft_precision_double, dims, lengths,
+                                         batch, nullptr);
     if (s != rocfft_status_success) { set_error("rocfft_plan_create failed"); return kEHip; }
     rocfft_plan_get_work_buffer_size(fp.plan, &fp.work_bytes);
     if (rocfft_execution_info_create(&fp.info) != rocfft_status_success) {
@@ -263,9 +266,10 @@ struct pfsht_plan_s {
     double *wth = nullptr;
     NufftDev nu;
     // Ft: the periodised grid, either in order-column layout [b][m + mmax][i] (the apply) or
-    // [b][i][k] (the NUFFT seam); U: the fine grid [b][l1][l2]
-    double2 *Ft = nullptr, *U = nullptr;
-    double2 *Ut = nullptr;  // nonzero fine columns, order-column layout [b][m + mmax][l1]
+    // [b][i][k] (the NUFFT seam); U: the fine grid [b][l1][l2]; Ut: the nonzero fine columns in
+    // order-column layout [b][m + mmax][l1]. Elements are double2, or float2 in a mixed-precision
+    // plan (the NUFFT seam is FP64 only).
+    void *Ft = nullptr, *U = nullptr, *Ut = nullptr;
     FftPlan th_fwd, th_bwd, fine_bwd, fine_fwd, col_bwd, col_fwd, row_bwd, row_fwd;
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     // Direct leaf: theta-sorted points as rings
@@ -472,23 +476,28 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if ((st = upload(&u.st_list, st_list, by)) != kOk) return st;
     if ((st = upload(&u.st_slot, st_slot, by)) != kOk) return st;
     if ((st = upload(&u.bin_ofs, bo, by)) != kOk) return st;
-    if ((st = dalloc(&u.Wst, size_t(B) * std::max(u.nst, 1) * u.win * u.win, by)) != kOk) return st;
+    const bool single = p.precision == PFSHT_PREC_MIXED;
+    const size_t csz = single ? sizeof(float2) : sizeof(double2);
+    auto galloc = [&](void **q, size_t count) { return dalloc(reinterpret_cast<char **>(q), count * csz, by); };
+    if ((st = galloc(&u.Wst, size_t(B) * std::max(u.nst, 1) * u.win * u.win)) != kOk) return st;
     // work buffers
-    if ((st = dalloc(&p.Ft, size_t(B) * p.nth * p.nph, by)) != kOk) return st;  // nph >= 2 mmax + 1
-    if ((st = dalloc(&p.U, size_t(B) * u.N1 * u.N2, by)) != kOk) return st;
+    if ((st = galloc(&p.Ft, size_t(B) * p.nth * p.nph)) != kOk) return st;  // nph >= 2 mmax + 1
+    if ((st = galloc(&p.U, size_t(B) * u.N1 * u.N2)) != kOk) return st;
     // rocFFT plans (row-major, contiguous; lengths fastest first)
     // colatitude FFTs of the order columns (the longitude FFT pair of the reference cancels)
     const size_t ncols = size_t(B) * (2 * p.mmax + 1);
-    if ((st = fft_make(p.th_fwd, true, p.nth, 0, ncols, by)) != kOk) return st;
-    if ((st = fft_make(p.th_bwd, false, p.nth, 0, ncols, by)) != kOk) return st;
-    if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;  // the NUFFT seam
-    if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
+    if ((st = fft_make(p.th_fwd, true, p.nth, 0, ncols, by, single)) != kOk) return st;
+    if ((st = fft_make(p.th_bwd, false, p.nth, 0, ncols, by, single)) != kOk) return st;
+    if (!single) {  // the NUFFT seam
+        if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;
+        if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
+    }
     // the apply's pruned fine transform: colatitude FFTs of the nonzero columns, longitude FFTs of rows
-    if ((st = dalloc(&p.Ut, ncols * u.N1, by)) != kOk) return st;
-    if ((st = fft_make(p.col_bwd, false, u.N1, 0, ncols, by)) != kOk) return st;
-    if ((st = fft_make(p.col_fwd, true, u.N1, 0, ncols, by)) != kOk) return st;
-    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.N1, by)) != kOk) return st;
-    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.N1, by)) != kOk) return st;
+    if ((st = galloc(&p.Ut, ncols * u.N1)) != kOk) return st;
+    if ((st = fft_make(p.col_bwd, false, u.N1, 0, ncols, by, single)) != kOk) return st;
+    if ((st = fft_make(p.col_fwd, true, u.N1, 0, ncols, by, single)) != kOk) return st;
+    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.N1, by, single)) != kOk) return st;
+    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.N1, by, single)) != kOk) return st;
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
@@ -545,6 +554,48 @@ static void timing_drain(pfsht_plan_s &p) {
     p.pending.clear();
 }
 
+// The grid part of the stretched synthesis, from the ring buffer to the points, with the grid
+// buffers stored as C (stages 1-7).
+template <class C>
+static Status synth_grid(pfsht_plan_s &p, TimedApply *t, double2 *f, hipStream_t s) {
+    const GridDims g = dims_of(p);
+    C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
+    launch_spec_dfs(g, p.rings->d, p.rings->Gd, p.wth, Ft, s);
+    tmark(p, t, 2, s);  // stages 2 and 3 (longitude FFT, DFS) have no work of their own
+    Status st = fft_exec(p.th_fwd, Ft, s);
+    if (st != kOk) return st;
+    tmark(p, t, 5, s);
+    launch_deconv_pad_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ft), Ut, s);
+    tmark(p, t, 6, s);
+    if ((st = fft_exec(p.col_bwd, Ut, s)) != kOk) return st;
+    launch_ut_to_rows(p.nu, p.mmax, static_cast<const C *>(Ut), U, s);
+    if ((st = fft_exec(p.row_bwd, U, s)) != kOk) return st;
+    tmark(p, t, 7, s);
+    launch_interp(p.nu, static_cast<const C *>(U), f, s);
+    tmark(p, t, 8, s);  // stage 8 is the adjoint's
+    return kOk;
+}
+
+// The grid part of the stretched adjoint, from the points to the ring buffer (stages 1-6).
+template <class C>
+static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, const double2 *f, hipStream_t s) {
+    const GridDims g = dims_of(p);
+    C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
+    launch_spread(p.nu, f, U, s);
+    tmark(p, t, 1, s);
+    Status st = fft_exec(p.row_fwd, U, s);
+    if (st != kOk) return st;
+    launch_rows_to_ut(p.nu, p.mmax, static_cast<const C *>(U), Ut, s);
+    if ((st = fft_exec(p.col_fwd, Ut, s)) != kOk) return st;
+    tmark(p, t, 2, s);
+    launch_deconv_trunc_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ut), Ft, s);
+    tmark(p, t, 3, s);
+    if ((st = fft_exec(p.th_bwd, Ft, s)) != kOk) return st;
+    tmark(p, t, 4, s);
+    launch_spec_fold(g, static_cast<const C *>(Ft), p.wth, p.rings->Gd, s);
+    return kOk;
+}
+
 static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
@@ -560,24 +611,11 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     }
     // src/Stretch.jl:262-277: ring pass, window + DFS (in longitude-spectral space), colatitude FFT,
     // phase and deconvolution into the fine grid, fine inverse FFT, interpolation
-    const GridDims g = dims_of(p);
-    pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
-    launch_ring_fwd(r.d, alm, r.Gd, s);
+    launch_ring_fwd(p.rings->d, alm, p.rings->Gd, s);
     tmark(p, t, 1, s);
-    launch_spec_dfs(g, r.d, r.Gd, p.wth, p.Ft, s);
-    tmark(p, t, 2, s);  // stages 2 and 3 (longitude FFT, DFS) have no work of their own
-    Status st = fft_exec(p.th_fwd, p.Ft, s);
+    Status st = p.precision == PFSHT_PREC_MIXED ? synth_grid<float2>(p, t, f, s) : synth_grid<double2>(p, t, f, s);
     if (st != kOk) return st;
-    tmark(p, t, 5, s);
-    launch_deconv_pad_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ft, p.Ut, s);
-    tmark(p, t, 6, s);
-    if ((st = fft_exec(p.col_bwd, p.Ut, s)) != kOk) return st;
-    launch_ut_to_rows(p.nu, p.mmax, p.Ut, p.U, s);
-    if ((st = fft_exec(p.row_bwd, p.U, s)) != kOk) return st;
-    tmark(p, t, 7, s);
-    launch_interp(p.nu, p.U, f, s);
-    tmark(p, t, 8, s);  // stage 8 is the adjoint's
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
@@ -598,21 +636,10 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
         return kOk;
     }
     // src/Stretch.jl:279-295, the exact transpose of the synthesis
-    const GridDims g = dims_of(p);
     pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
-    launch_spread(p.nu, f, p.U, s);
-    tmark(p, t, 1, s);
-    Status st = fft_exec(p.row_fwd, p.U, s);
+    Status st = p.precision == PFSHT_PREC_MIXED ? adjoint_grid<float2>(p, t, f, s) : adjoint_grid<double2>(p, t, f, s);
     if (st != kOk) return st;
-    launch_rows_to_ut(p.nu, p.mmax, p.U, p.Ut, s);
-    if ((st = fft_exec(p.col_fwd, p.Ut, s)) != kOk) return st;
-    tmark(p, t, 2, s);
-    launch_deconv_trunc_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), p.Ut, p.Ft, s);
-    tmark(p, t, 3, s);
-    if ((st = fft_exec(p.th_bwd, p.Ft, s)) != kOk) return st;
-    tmark(p, t, 4, s);
-    launch_spec_fold(g, p.Ft, p.wth, r.Gd, s);
     tmark(p, t, 7, s);
     launch_ring_adj_tiles(r.d, r.Gd, s);
     tmark(p, t, 8, s);
@@ -649,9 +676,10 @@ int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int3
     if (!out) { set_error("out is NULL"); return kEArg; }
     *out = nullptr;
     if (!theta || !phi) { set_error("theta/phi is NULL"); return kEArg; }
-    if (precision != PFSHT_PREC_FP64) {
-        set_error("only PFSHT_PREC_FP64 is available in this build");
-        return kEUnsupported;
+    if (precision != PFSHT_PREC_FP64 && precision != PFSHT_PREC_MIXED) { set_error("unknown precision"); return kEArg; }
+    if (precision == PFSHT_PREC_MIXED && !(tol >= 1e-6)) {
+        set_error("mixed precision needs tol >= 1e-6");
+        return kEArg;
     }
     auto *p = new (std::nothrow) pfsht_plan_s;
     if (!p) { set_error("host allocation failed"); return kENomem; }
@@ -754,11 +782,13 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
     if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
     if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }
     if (p->method != PFSHT_METHOD_STRETCH) { set_error("a Direct plan has no NUFFT"); return kEArg; }
+    if (p->precision != PFSHT_PREC_FP64) { set_error("the NUFFT seam is FP64 only"); return kEUnsupported; }
     if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard dg(p->dev);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const NufftDev &u = p->nu;
+    double2 *Ft = static_cast<double2 *>(p->Ft), *U = static_cast<double2 *>(p->U);
     const size_t modes_bytes = sizeof(double2) * size_t(p->nth) * p->nph * p->B;
     const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;
     // the modes buffer in the reference layout lives in Fp's place when the caller's memory is on
@@ -781,15 +811,15 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
     }
     if (type == 2) {
         // reference column-major nth x nph == row-major [k][i]; internal is [i][k]
-        launch_transpose(static_cast<const double2 *>(in_d), p->Ft, p->nph, p->nth, p->B, s);
-        launch_deconv_pad(u, u.c1, p->Ft, p->U, s);
-        if ((st = fft_exec(p->fine_bwd, p->U, s)) != kOk) return st;
-        launch_interp(u, p->U, static_cast<double2 *>(out_d), s);
+        launch_transpose(static_cast<const double2 *>(in_d), Ft, p->nph, p->nth, p->B, s);
+        launch_deconv_pad(u, u.c1, Ft, U, s);
+        if ((st = fft_exec(p->fine_bwd, U, s)) != kOk) return st;
+        launch_interp(u, static_cast<const double2 *>(U), static_cast<double2 *>(out_d), s);
     } else {
-        launch_spread(u, static_cast<const double2 *>(in_d), p->U, s);
-        if ((st = fft_exec(p->fine_fwd, p->U, s)) != kOk) return st;
-        launch_deconv_trunc(u, u.c1, p->U, p->Ft, s);
-        launch_transpose(p->Ft, static_cast<double2 *>(out_d), p->nth, p->nph, p->B, s);
+        launch_spread(u, static_cast<const double2 *>(in_d), U, s);
+        if ((st = fft_exec(p->fine_fwd, U, s)) != kOk) return st;
+        launch_deconv_trunc(u, u.c1, U, Ft, s);
+        launch_transpose(Ft, static_cast<double2 *>(out_d), p->nth, p->nph, p->B, s);
     }
     PFSHT_HIP_TRY(hipGetLastError());
     if ((st = apply_end(*p, s)) != kOk) return st;

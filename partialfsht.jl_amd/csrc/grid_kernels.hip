@@ -20,9 +20,10 @@ constexpr int kTile = 32;
 
 // Ds[b][c][i] = w[j] G[j, m] for i < nh (j = i), and (-1)^m w[j] G[j, m] for the mirrored rows
 // i >= nh of a pole/full periodisation (j = nth - 1 - i); zero off the retained ranges.
+template <class C>
 __global__ void spec_dfs_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo,
                                 const int32_t *__restrict__ jhi, const double *__restrict__ wth,
-                                double2 *__restrict__ Ds, int nh, int nth, int mmax) {
+                                C *__restrict__ Ds, int nh, int nth, int mmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -37,12 +38,13 @@ __global__ void spec_dfs_kernel(const double2 *__restrict__ Gd, const int32_t *_
         if (i >= nh && (am & 1)) w = -w;
         v = make_double2(g.x * w, g.y * w);
     }
-    Ds[(int64_t(b) * ncol + c) * nth + i] = v;
+    stc(Ds + (int64_t(b) * ncol + c) * nth + i, v);
 }
 
 // The transpose: Gd[b][c][j] = w[j] (Ds[c][j] + (-1)^m Ds[c][nth - 1 - j]) (pole/full) or
 // w[j] Ds[c][j] (interior, nth = nh), for every ring j < nh.
-__global__ void spec_fold_kernel(const double2 *__restrict__ Ds, const double *__restrict__ wth,
+template <class C>
+__global__ void spec_fold_kernel(const C *__restrict__ Ds, const double *__restrict__ wth,
                                  double2 *__restrict__ Gd, int nh, int nth, int mmax) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
@@ -50,10 +52,10 @@ __global__ void spec_fold_kernel(const double2 *__restrict__ Ds, const double *_
     if (j >= nh) return;
     const int64_t ncol = 2 * mmax + 1;
     const int m = c - mmax, am = m < 0 ? -m : m;
-    const double2 *col = Ds + (int64_t(b) * ncol + c) * nth;
-    double2 v = col[j];
+    const C *col = Ds + (int64_t(b) * ncol + c) * nth;
+    double2 v = ldc(col + j);
     if (nth > nh) {
-        const double2 r = col[nth - 1 - j];
+        const double2 r = ldc(col + nth - 1 - j);
         if (am & 1) v = make_double2(v.x - r.x, v.y - r.y);
         else v = make_double2(v.x + r.x, v.y + r.y);
     }
@@ -61,16 +63,24 @@ __global__ void spec_fold_kernel(const double2 *__restrict__ Ds, const double *_
     Gd[(int64_t(b) * ncol + c) * nh + j] = make_double2(v.x * w, v.y * w);
 }
 
-void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, double2 *Ds,
-                     hipStream_t s) {
+template <class C>
+void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, C *Ds, hipStream_t s) {
     dim3 grid((g.nth + 255) / 256, 2 * g.mmax + 1, g.B);
-    spec_dfs_kernel<<<grid, 256, 0, s>>>(Gd, r.jlo, r.jhi, wth, Ds, g.nh, g.nth, g.mmax);
+    spec_dfs_kernel<C><<<grid, 256, 0, s>>>(Gd, r.jlo, r.jhi, wth, Ds, g.nh, g.nth, g.mmax);
 }
 
-void launch_spec_fold(const GridDims &g, const double2 *Ds, const double *wth, double2 *Gd, hipStream_t s) {
+template <class C>
+void launch_spec_fold(const GridDims &g, const C *Ds, const double *wth, double2 *Gd, hipStream_t s) {
     dim3 grid((g.nh + 255) / 256, 2 * g.mmax + 1, g.B);
-    spec_fold_kernel<<<grid, 256, 0, s>>>(Ds, wth, Gd, g.nh, g.nth, g.mmax);
+    spec_fold_kernel<C><<<grid, 256, 0, s>>>(Ds, wth, Gd, g.nh, g.nth, g.mmax);
 }
+
+template void launch_spec_dfs<double2>(const GridDims &, const RingDev &, const double2 *, const double *, double2 *,
+                                       hipStream_t);
+template void launch_spec_dfs<float2>(const GridDims &, const RingDev &, const double2 *, const double *, float2 *,
+                                      hipStream_t);
+template void launch_spec_fold<double2>(const GridDims &, const double2 *, const double *, double2 *, hipStream_t);
+template void launch_spec_fold<float2>(const GridDims &, const float2 *, const double *, double2 *, hipStream_t);
 
 // Batched complex transpose through an LDS tile (layout conversion at the C-ABI boundary).
 __global__ void __launch_bounds__(256)

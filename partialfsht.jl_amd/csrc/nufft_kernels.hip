@@ -152,9 +152,10 @@ __device__ __forceinline__ int fine_to_mode(int l, int n, int N) {
 // transpose that inserts the zero columns between them (synthesis), or drops them (adjoint).
 // ------------------------------------------------------------------------------------------------
 // Ut[b][c][l1] = F^[b][c][k1(l1)] pc1[k1] c2[m] scale, zero in the colatitude padding.
-__global__ void deconv_pad_ut_kernel(const double2 *__restrict__ Fc, const double2 *__restrict__ pc1,
-                                     const double *__restrict__ c2, double2 *__restrict__ Ut, int n1, int mmax,
-                                     int nph, int N1, double scale) {
+template <class C>
+__global__ void deconv_pad_ut_kernel(const C *__restrict__ Fc, const double2 *__restrict__ pc1,
+                                     const double *__restrict__ c2, C *__restrict__ Ut, int n1, int mmax, int nph,
+                                     int N1, double scale) {
     const int l1 = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -164,18 +165,19 @@ __global__ void deconv_pad_ut_kernel(const double2 *__restrict__ Fc, const doubl
     double2 v = make_double2(0.0, 0.0);
     if (i >= 0) {
         const int m = c - mmax;
-        const double2 F = Fc[col * n1 + i];
+        const double2 F = ldc(Fc + col * n1 + i);
         const double2 p = pc1[i];
         const double sc = c2[m >= 0 ? m : m + nph] * scale;
         v = make_double2((F.x * p.x - F.y * p.y) * sc, (F.x * p.y + F.y * p.x) * sc);
     }
-    Ut[col * N1 + l1] = v;
+    stc(Ut + col * N1 + l1, v);
 }
 
 // F^[b][c][k1] = Ut[b][c][l1(k1)] conj(pc1[k1]) c2[m] scale.
-__global__ void deconv_trunc_ut_kernel(const double2 *__restrict__ Ut, const double2 *__restrict__ pc1,
-                                       const double *__restrict__ c2, double2 *__restrict__ Fc, int n1, int mmax,
-                                       int nph, int N1, double scale) {
+template <class C>
+__global__ void deconv_trunc_ut_kernel(const C *__restrict__ Ut, const double2 *__restrict__ pc1,
+                                       const double *__restrict__ c2, C *__restrict__ Fc, int n1, int mmax, int nph,
+                                       int N1, double scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -183,22 +185,23 @@ __global__ void deconv_trunc_ut_kernel(const double2 *__restrict__ Ut, const dou
     const int64_t col = int64_t(b) * (2 * mmax + 1) + c;
     const int m = c - mmax;
     const int l1 = i < (n1 + 1) / 2 ? i : i - n1 + N1;
-    const double2 u = Ut[col * N1 + l1];
+    const double2 u = ldc(Ut + col * N1 + l1);
     const double2 p = pc1[i];  // conj(p) for the adjoint
     const double sc = c2[m >= 0 ? m : m + nph] * scale;
-    Fc[col * n1 + i] = make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc);
+    stc(Fc + col * n1 + i, make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc));
 }
 
 // U[b][l1][l2] = Ut[b][c(l2)][l1] for the fine columns of retained orders, zero elsewhere.
+template <class C>
 __global__ void __launch_bounds__(256)
-ut_to_rows_kernel(const double2 *__restrict__ Ut, double2 *__restrict__ U, int mmax, int N1, int N2) {
-    __shared__ double2 tile[kDT][kDT + 1];
+ut_to_rows_kernel(const C *__restrict__ Ut, C *__restrict__ U, int mmax, int N1, int N2) {
+    __shared__ C tile[kDT][kDT + 1];
     const int l2_0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
     const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
     const int64_t ncol = 2 * mmax + 1;
     for (int rr = ty; rr < kDT; rr += 8) {
         const int l2 = l2_0 + rr, l1 = l1_0 + tx;
-        double2 v = make_double2(0.0, 0.0);
+        C v = {};
         if (l2 < N2 && l1 < N1) {
             const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
             if (m != INT32_MIN) v = Ut[(int64_t(b) * ncol + (m + mmax)) * N1 + l1];
@@ -213,15 +216,16 @@ ut_to_rows_kernel(const double2 *__restrict__ Ut, double2 *__restrict__ U, int m
 }
 
 // Ut[b][c][l1] = U[b][l1][l2(c)]: the transpose, keeping only the fine columns of retained orders.
+template <class C>
 __global__ void __launch_bounds__(256)
-rows_to_ut_kernel(const double2 *__restrict__ U, double2 *__restrict__ Ut, int mmax, int N1, int N2) {
-    __shared__ double2 tile[kDT][kDT + 1];
+rows_to_ut_kernel(const C *__restrict__ U, C *__restrict__ Ut, int mmax, int N1, int N2) {
+    __shared__ C tile[kDT][kDT + 1];
     const int c0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
     const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
     const int ncol = 2 * mmax + 1;
     for (int rr = ty; rr < kDT; rr += 8) {
         const int l1 = l1_0 + rr, c = c0 + tx;
-        double2 v = make_double2(0.0, 0.0);
+        C v = {};
         if (l1 < N1 && c < ncol) {
             const int m = c - mmax;
             v = U[(int64_t(b) * N1 + l1) * N2 + (m >= 0 ? m : m + N2)];
@@ -235,26 +239,30 @@ rows_to_ut_kernel(const double2 *__restrict__ U, double2 *__restrict__ Ut, int m
     }
 }
 
-void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const double2 *Fc,
-                          double2 *Ut, hipStream_t s) {
+template <class C>
+void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const C *Fc, C *Ut,
+                          hipStream_t s) {
     dim3 grid((u.N1 + 255) / 256, 2 * mmax + 1, u.B);
-    deconv_pad_ut_kernel<<<grid, 256, 0, s>>>(Fc, pc1, u.c2, Ut, u.n1, mmax, nph, u.N1, scale);
+    deconv_pad_ut_kernel<C><<<grid, 256, 0, s>>>(Fc, pc1, u.c2, Ut, u.n1, mmax, nph, u.N1, scale);
 }
 
-void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                            const double2 *Ut, double2 *Fc, hipStream_t s) {
+template <class C>
+void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const C *Ut, C *Fc,
+                            hipStream_t s) {
     dim3 grid((u.n1 + 255) / 256, 2 * mmax + 1, u.B);
-    deconv_trunc_ut_kernel<<<grid, 256, 0, s>>>(Ut, pc1, u.c2, Fc, u.n1, mmax, nph, u.N1, scale);
+    deconv_trunc_ut_kernel<C><<<grid, 256, 0, s>>>(Ut, pc1, u.c2, Fc, u.n1, mmax, nph, u.N1, scale);
 }
 
-void launch_ut_to_rows(const NufftDev &u, int mmax, const double2 *Ut, double2 *U, hipStream_t s) {
+template <class C>
+void launch_ut_to_rows(const NufftDev &u, int mmax, const C *Ut, C *U, hipStream_t s) {
     dim3 grid((u.N2 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
-    ut_to_rows_kernel<<<grid, 256, 0, s>>>(Ut, U, mmax, u.N1, u.N2);
+    ut_to_rows_kernel<C><<<grid, 256, 0, s>>>(Ut, U, mmax, u.N1, u.N2);
 }
 
-void launch_rows_to_ut(const NufftDev &u, int mmax, const double2 *U, double2 *Ut, hipStream_t s) {
+template <class C>
+void launch_rows_to_ut(const NufftDev &u, int mmax, const C *U, C *Ut, hipStream_t s) {
     dim3 grid((2 * mmax + 1 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
-    rows_to_ut_kernel<<<grid, 256, 0, s>>>(U, Ut, mmax, u.N1, u.N2);
+    rows_to_ut_kernel<C><<<grid, 256, 0, s>>>(U, Ut, mmax, u.N1, u.N2);
 }
 
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s) {
@@ -272,17 +280,18 @@ void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U
 // fine grid (4 S + w - 1 cells square, wrapped periodically) is staged in LDS with coalesced row
 // reads, then each thread interpolates bin-sorted points of the super-tile from LDS.
 // ------------------------------------------------------------------------------------------------
-template <int W>
+template <int W, class C>
 __global__ void __launch_bounds__(256)
 interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
-                 const double2 *__restrict__ U, double2 *__restrict__ f, const int32_t *__restrict__ st_list,
+                 const C *__restrict__ U, double2 *__restrict__ f, const int32_t *__restrict__ st_list,
                  const int32_t *__restrict__ bin_ofs, int64_t M, int N1, int N2, int S4, int ns2, int win) {
-    extern __shared__ double2 lw[];
+    extern __shared__ __align__(16) unsigned char lds_ip_raw[];
+    C *lw = reinterpret_cast<C *>(lds_ip_raw);
     const int slot = blockIdx.x;
     const int b = blockIdx.y;
     const int st = st_list[slot];
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
-    const double2 *__restrict__ Ub = U + int64_t(b) * N1 * N2;
+    const C *__restrict__ Ub = U + int64_t(b) * N1 * N2;
     for (int r = threadIdx.x >> 6; r < win; r += 4) {
         const int row = (x0 + r) % N1;
         for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(row) * N2 + (y0 + c) % N2];
@@ -296,14 +305,14 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         double k1[W], k2[W];
         es_row<W>(x1, b1, k1);
         es_row<W>(x2, b2, k2);
-        const double2 *__restrict__ wrow = lw + (b1 - x0) * win + (b2 - y0);
+        const C *__restrict__ wrow = lw + (b1 - x0) * win + (b2 - y0);
         double ar = 0.0, ai = 0.0;
 #pragma unroll
         for (int t1 = 0; t1 < W; ++t1) {
             double rr = 0.0, ri = 0.0;
 #pragma unroll
             for (int t2 = 0; t2 < W; ++t2) {
-                const double2 v = wrow[t1 * win + t2];
+                const double2 v = ldc(wrow + t1 * win + t2);
                 rr = fma(k2[t2], v.x, rr);
                 ri = fma(k2[t2], v.y, ri);
             }
@@ -327,12 +336,12 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 // keeping each of its bins' 16 x 16 tiles in registers. No workgroup barrier until the end, where
 // the tiles are added into the LDS window wave after wave (bin windows overlap) and the window is
 // written out for the merge pass.
-template <int W>
+template <int W, class C>
 __global__ void __launch_bounds__(64 * kSpreadWaves)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  const double2 *__restrict__ f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, double2 *__restrict__ Wst, int64_t M, int S, int ns2,
-                 int win, int nst) {
+                 const int32_t *__restrict__ bin_ofs, C *__restrict__ Wst, int64_t M, int S, int ns2, int win,
+                 int nst) {
     constexpr int NH = (W + 1) / 2;  // cells per half row
     extern __shared__ double lds_sp[];
     double2 *lw = reinterpret_cast<double2 *>(lds_sp);  // [win][win]
@@ -430,12 +439,13 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         }
         __syncthreads();
     }
-    double2 *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
-    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) out[q] = lw[q];
+    C *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
+    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) stc(out + q, lw[q]);
 }
 
-__global__ void spread_merge_kernel(const double2 *__restrict__ Wst, const int32_t *__restrict__ st_slot,
-                                    double2 *__restrict__ U, int N1, int N2, int S4, int ns1, int ns2, int win, int w,
+template <class C>
+__global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__restrict__ st_slot,
+                                    C *__restrict__ U, int N1, int N2, int S4, int ns1, int ns2, int win, int w,
                                     int nst) {
     const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
     const int l1 = blockIdx.y;
@@ -457,61 +467,77 @@ __global__ void spread_merge_kernel(const double2 *__restrict__ Wst, const int32
                 int o2 = (l2 - s2 * S4) % N2;
                 if (o2 < 0) o2 += N2;
                 for (; o2 < wh2; o2 += N2) {
-                    const double2 v = Wst[((int64_t(b) * nst + slot) * win + o1) * win + o2];
+                    const double2 v = ldc(Wst + ((int64_t(b) * nst + slot) * win + o1) * win + o2);
                     sr += v.x;
                     si += v.y;
                 }
             }
         }
     }
-    U[(int64_t(b) * N1 + l1) * N2 + l2] = make_double2(sr, si);
+    stc(U + (int64_t(b) * N1 + l1) * N2 + l2, make_double2(sr, si));
 }
 
-template <int W>
-static void interp_w(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s) {
+template <int W, class C>
+static void interp_w(const NufftDev &u, const C *U, double2 *f, hipStream_t s) {
     if (u.nst == 0) return;
     dim3 grid(u.nst, u.B);
-    const size_t lds = sizeof(double2) * size_t(u.win) * u.win;
-    interp_st_kernel<W><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list, u.bin_ofs, u.M, u.N1, u.N2,
-                                               4 * u.S, u.ns2, u.win);
+    const size_t lds = sizeof(C) * size_t(u.win) * u.win;
+    interp_st_kernel<W, C><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list, u.bin_ofs, u.M, u.N1, u.N2,
+                                                  4 * u.S, u.ns2, u.win);
 }
 
-template <int W>
-static void spread_w(const NufftDev &u, const double2 *f, hipStream_t s) {
+template <int W, class C>
+static void spread_w(const NufftDev &u, const double2 *f, C *Wst, hipStream_t s) {
     if (u.nst == 0) return;
     dim3 grid(u.nst, u.B);
-    spread_st_kernel<W><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win), s>>>(u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs,
-                                                                   u.Wst, u.M, u.S, u.ns2, u.win, u.nst);
+    spread_st_kernel<W, C><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win), s>>>(
+        u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.M, u.S, u.ns2, u.win, u.nst);
 }
 
 #define PFSHT_W_SWITCH(w, FN, ...)                 \
     switch (w) {                                   \
-        case 2: FN<2>(__VA_ARGS__); break;         \
-        case 3: FN<3>(__VA_ARGS__); break;         \
-        case 4: FN<4>(__VA_ARGS__); break;         \
-        case 5: FN<5>(__VA_ARGS__); break;         \
-        case 6: FN<6>(__VA_ARGS__); break;         \
-        case 7: FN<7>(__VA_ARGS__); break;         \
-        case 8: FN<8>(__VA_ARGS__); break;         \
-        case 9: FN<9>(__VA_ARGS__); break;         \
-        case 10: FN<10>(__VA_ARGS__); break;       \
-        case 11: FN<11>(__VA_ARGS__); break;       \
-        case 12: FN<12>(__VA_ARGS__); break;       \
-        case 13: FN<13>(__VA_ARGS__); break;       \
-        case 14: FN<14>(__VA_ARGS__); break;       \
-        case 15: FN<15>(__VA_ARGS__); break;       \
-        default: FN<16>(__VA_ARGS__); break;       \
+        case 2: FN<2, C>(__VA_ARGS__); break;      \
+        case 3: FN<3, C>(__VA_ARGS__); break;      \
+        case 4: FN<4, C>(__VA_ARGS__); break;      \
+        case 5: FN<5, C>(__VA_ARGS__); break;      \
+        case 6: FN<6, C>(__VA_ARGS__); break;      \
+        case 7: FN<7, C>(__VA_ARGS__); break;      \
+        case 8: FN<8, C>(__VA_ARGS__); break;      \
+        case 9: FN<9, C>(__VA_ARGS__); break;      \
+        case 10: FN<10, C>(__VA_ARGS__); break;    \
+        case 11: FN<11, C>(__VA_ARGS__); break;    \
+        case 12: FN<12, C>(__VA_ARGS__); break;    \
+        case 13: FN<13, C>(__VA_ARGS__); break;    \
+        case 14: FN<14, C>(__VA_ARGS__); break;    \
+        case 15: FN<15, C>(__VA_ARGS__); break;    \
+        default: FN<16, C>(__VA_ARGS__); break;    \
     }
 
-void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s) {
+template <class C>
+void launch_interp(const NufftDev &u, const C *U, double2 *f, hipStream_t s) {
     PFSHT_W_SWITCH(u.w, interp_w, u, U, f, s);
 }
 
-void launch_spread(const NufftDev &u, const double2 *f, double2 *U, hipStream_t s) {
-    PFSHT_W_SWITCH(u.w, spread_w, u, f, s);
+template <class C>
+void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s) {
+    C *Wst = static_cast<C *>(u.Wst);
+    PFSHT_W_SWITCH(u.w, spread_w, u, f, Wst, s);
     dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
-    spread_merge_kernel<<<grid, 256, 0, s>>>(u.Wst, u.st_slot, U, u.N1, u.N2, 4 * u.S, u.ns1, u.ns2, u.win, u.w,
-                                             u.nst);
+    spread_merge_kernel<C><<<grid, 256, 0, s>>>(Wst, u.st_slot, U, u.N1, u.N2, 4 * u.S, u.ns1, u.ns2, u.win, u.w,
+                                                u.nst);
 }
+
+#define PFSHT_INSTANTIATE(C)                                                                                   \
+    template void launch_deconv_pad_ut<C>(const NufftDev &, const double2 *, int, int, double, const C *, C *,  \
+                                          hipStream_t);                                                         \
+    template void launch_deconv_trunc_ut<C>(const NufftDev &, const double2 *, int, int, double, const C *, C *,\
+                                            hipStream_t);                                                       \
+    template void launch_ut_to_rows<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
+    template void launch_rows_to_ut<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
+    template void launch_interp<C>(const NufftDev &, const C *, double2 *, hipStream_t);                        \
+    template void launch_spread<C>(const NufftDev &, const double2 *, C *, hipStream_t);
+PFSHT_INSTANTIATE(double2)
+PFSHT_INSTANTIATE(float2)
+#undef PFSHT_INSTANTIATE
 
 }  // namespace pfsht

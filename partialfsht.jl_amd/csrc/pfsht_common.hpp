@@ -13,6 +13,17 @@ namespace pfsht {
 
 void set_error(const std::string &msg);
 
+// Storage of the grid buffers: complex FP64 (double2) or, in mixed precision, FP32 (float2).
+// Kernels load into FP64, compute in FP64 and round on store.
+__device__ __forceinline__ double2 ldc(const double2 *p) { return *p; }
+__device__ __forceinline__ double2 ldc(const float2 *p) {
+    const float2 v = *p;
+    return make_double2(double(v.x), double(v.y));
+}
+__device__ __forceinline__ void stc(double2 *p, double2 v) { *p = v; }
+__device__ __forceinline__ void stc(float2 *p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
+
+
 // Status propagated through the C ABI (values of the PFSHT_E_* codes in include/pfsht.h).
 enum Status { kOk = 0, kEDim = 1, kEArg = 2, kEGeom = 3, kENomem = 4, kEHip = 5, kEUnsupported = 6 };
 
@@ -71,9 +82,11 @@ struct GridDims {
     int B, nh, nth, nph, mmax, L, kind;
 };
 // Periodisation in longitude-spectral space, order-column layout Ds [b][m + mmax][i] (i < nth).
-void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, double2 *Ds,
-                     hipStream_t s);
-void launch_spec_fold(const GridDims &g, const double2 *Ds, const double *wth, double2 *Gd, hipStream_t s);
+// C is the storage of the grid buffers: double2, or float2 in mixed precision.
+template <class C>
+void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, C *Ds, hipStream_t s);
+template <class C>
+void launch_spec_fold(const GridDims &g, const C *Ds, const double *wth, double2 *Gd, hipStream_t s);
 
 // Direct leaf: phase sums over the ring-engine output of theta-sorted points.
 void launch_direct_synth(const double2 *Gd, const double *phis, const int32_t *mlim, const int32_t *perm,
@@ -98,7 +111,7 @@ struct NufftDev {
     int32_t *st_list = nullptr;           // [nst] super-tile ids (s1 * ns2 + s2)
     int32_t *st_slot = nullptr;           // [ns1*ns2] slot in Wst or -1 when empty
     int32_t *bin_ofs = nullptr;           // [nst*16 + 1] point ranges per (super-tile, local bin)
-    double2 *Wst = nullptr;               // [B][nst][win][win] super-tile windows
+    void *Wst = nullptr;                  // [B][nst][win][win] super-tile windows (grid storage type)
 };
 // LDS of one interpolation workgroup: the super-tile window of the fine grid.
 inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(win) * win; }
@@ -114,15 +127,21 @@ inline size_t spread_lds_bytes(int win) {
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);
 // The pruned fine transform: order-column layout Ut [b][m + mmax][l1] of the nonzero fine columns.
-void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const double2 *Fc,
-                          double2 *Ut, hipStream_t s);
-void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale,
-                            const double2 *Ut, double2 *Fc, hipStream_t s);
-void launch_ut_to_rows(const NufftDev &u, int mmax, const double2 *Ut, double2 *U, hipStream_t s);
-void launch_rows_to_ut(const NufftDev &u, int mmax, const double2 *U, double2 *Ut, hipStream_t s);
+template <class C>
+void launch_deconv_pad_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const C *Fc, C *Ut,
+                          hipStream_t s);
+template <class C>
+void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int nph, double scale, const C *Ut, C *Fc,
+                            hipStream_t s);
+template <class C>
+void launch_ut_to_rows(const NufftDev &u, int mmax, const C *Ut, C *U, hipStream_t s);
+template <class C>
+void launch_rows_to_ut(const NufftDev &u, int mmax, const C *U, C *Ut, hipStream_t s);
 // out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
 void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
-void launch_interp(const NufftDev &u, const double2 *U, double2 *f, hipStream_t s);
-void launch_spread(const NufftDev &u, const double2 *f, double2 *U, hipStream_t s);
+template <class C>
+void launch_interp(const NufftDev &u, const C *U, double2 *f, hipStream_t s);
+template <class C>
+void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s);
 
 }  // namespace pfsht

@@ -55,7 +55,8 @@ def test_host_validation_before_device():
     assert L.pfsht_plan_create(p, p, 4, 8, 1e-8, 0, 0, 0, C.byref(out)) == 2       # ntrans < 1
     assert L.pfsht_plan_create(p, p, 0, 8, 1e-8, 1, 0, 0, C.byref(out)) == 2       # no points
     assert L.pfsht_plan_create(p, p, 4, 8, 2.0, 1, 0, 0, C.byref(out)) == 2        # tol out of range
-    assert L.pfsht_plan_create(p, p, 4, 8, 1e-8, 1, 0, 7, C.byref(out)) == 6       # precision
+    assert L.pfsht_plan_create(p, p, 4, 8, 1e-8, 1, 0, 7, C.byref(out)) == 2       # unknown precision
+    assert L.pfsht_plan_create(p, p, 4, 8, 1e-8, 1, 0, 1, C.byref(out)) == 2       # mixed needs tol >= 1e-6
     with pytest.raises(pfsht.DimensionMismatch):
         pfsht.Plan(np.zeros(3), np.zeros(4), lmax=8)
 

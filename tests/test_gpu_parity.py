@@ -266,6 +266,50 @@ def test_ntrans16_columns(torch, P, O, rng):
     assert np.all(aBg[~O.valid_mask(L)] == 0)
 
 
+@pytest.mark.parametrize("tol,deg,tc,ntrans", [(1e-6, 20, 0.0, 1), (1e-5, 25, 1.1, 2)])
+def test_mixed_precision(torch, P, O, rng, tol, deg, tc, ntrans):
+    # mixed precision (tol >= 1e-6): FP64 ring recurrence, grid buffers and FFTs in FP32. Against
+    # the FP64 plan at the same tol (within tol), the FP64 Direct oracle on a subset (10 tol, the
+    # bar of test_transforms.jl:59-82), and the adjoint identity (within tol: FP32 round-off).
+    L = 256
+    M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
+    th, ph = centered_cap_points(rng, deg, M, tc, 0.5) if tc else cap_points(rng, deg, M)
+    p64 = P.Plan(th, ph, lmax=L, tol=tol, ntrans=ntrans, method="stretch")
+    pmx = P.Plan(th, ph, lmax=L, tol=tol, ntrans=ntrans, method="stretch", precision="mixed")
+    assert pmx.info["precision"] == 1 and pmx.info["kind"] == p64.info["kind"]
+    a = O.rand_alm_fast(L, rng, ntrans=ntrans)
+    fm = pmx @ a
+    f64 = p64 @ a
+    e64 = relmax(fm, f64)
+    sub = rng.choice(M, 200, replace=False)
+    pd = O.Plan(th[sub], ph[sub], L, method="direct", nthreads=NT)
+    cols = [np.asfortranarray(a[:, :, b]) for b in range(ntrans)] if ntrans > 1 else [a]
+    fd = np.stack([pd.synth(c) for c in cols], axis=1).reshape(fm[sub].shape)
+    ed = relmax(fm[sub], fd)
+    fs = rng.standard_normal((200, ntrans)) + 1j * rng.standard_normal((200, ntrans))
+    f = np.zeros((M, ntrans), complex)
+    f[sub] = fs
+    f = np.asfortranarray(f if ntrans > 1 else f[:, 0])
+    am = pmx.H @ f
+    ea = max(rel_modes(O, (am[:, :, b] if ntrans > 1 else am),
+                       pd.adjoint(np.ascontiguousarray(fs[:, b])), L) for b in range(ntrans))
+    g = np.asfortranarray(rng.standard_normal(f.shape) + 1j * rng.standard_normal(f.shape))
+    ag = pmx.H @ g
+    lhs = np.vdot(g, fm)
+    eid = abs(lhs - np.vdot(ag, a)) / abs(lhs)
+    print(f"mixed tol={tol}: vs fp64 plan {e64:.2e}, vs Direct {ed:.2e}, adjoint vs Direct {ea:.2e}, "
+          f"identity {eid:.2e}", flush=True)
+    assert e64 < tol
+    assert ed < 10 * tol
+    assert ea < 10 * tol
+    assert eid < tol
+    assert np.all(am[~O.valid_mask(L)] == 0)
+    with pytest.raises(P.ArgumentError):   # the NUFFT seam is FP64 only
+        pmx.nufft(2, np.zeros(M * ntrans, complex), np.zeros(pmx.info["nth"] * pmx.info["nph"] * ntrans, complex))
+    with pytest.raises(P.ArgumentError):
+        P.Plan(th, ph, lmax=L, tol=1e-8, method="stretch", precision="mixed")
+
+
 def test_shape_errors(torch, P, rng):
     th, ph = cap_points(rng, 20, 1000)
     p = P.Plan(th, ph, lmax=160, tol=1e-8, method="stretch")
