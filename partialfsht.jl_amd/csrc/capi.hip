@@ -138,7 +138,7 @@ struct pfsht_rings_s {
 
     ~pfsht_rings_s() {
         dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.abofs);
-        dfree(d.jlo); dfree(d.jhi); dfree(d.items); dfree(d.aitems); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
+        dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
     }
 };
@@ -181,43 +181,80 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
                                         double(l + m - 1) / double(l + m));
             ab[abofs[m] + (l - m - 2)] = make_double2(a, bb);
         }
-    // work blocks, longest first: forward = all tiles of an order (up to kFwdMaxWaves); adjoint =
-    // up to kAdjWaves tiles. Partials of the adjoint are indexed by the order's 64-ring tile.
-    std::vector<RingItem> items, aitems;
+    // Work items: one per (order, 64-ring tile), with the order's degree segments; checkpoint rows
+    // for segments 1.. of every tile. Partials of the adjoint are indexed by the order's tile.
+    std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
     std::vector<int32_t> ntiles(mmax + 1, 0);
-    std::vector<int64_t> pofs(mmax + 1, 0);
-    int64_t np = 0;
+    std::vector<int64_t> pofs(mmax + 1, 0), cost(mmax + 1, 0);
+    int64_t np = 0, nck = 0;
     r.work = 0;
-    int fw = 1, aw = 1;
+    int fw = 1;
     for (int m = 0; m <= mmax; ++m) {
         pofs[m] = np;
         const int lo = r.jlo[m], hi = r.jhi[m];
         if (hi < lo) continue;
         const int nj = hi - lo + 1;
         r.work += int64_t(nj) * (L - m + 1);
-        const int nt = (nj + 63) / 64;
-        for (int t0 = 0; t0 < nt; t0 += kFwdMaxWaves) {
-            const int j0 = lo + 64 * t0, nr = std::min(64 * kFwdMaxWaves, hi - j0 + 1);
-            items.push_back({m, j0, nr, t0});
-            fw = std::max(fw, (nr + 63) / 64);
-        }
-        for (int t0 = 0; t0 < nt; t0 += kAdjWaves) {
-            const int j0 = lo + 64 * t0, nr = std::min(64 * kAdjWaves, hi - j0 + 1);
-            aitems.push_back({m, j0, nr, t0});
-            aw = std::max(aw, (nr + 63) / 64);
+        cost[m] = int64_t(nj) * (L - m + 1);
+        const int nt = (nj + 63) / 64, nseg = seg_count(m, L);
+        fw = std::max(fw, nseg);
+        for (int t = 0; t < nt; ++t) {
+            const int j0 = lo + 64 * t;
+            tiles_of[m].push_back({m, j0, std::min(64, hi - j0 + 1), t, nseg, 0, nck});
+            nck += nseg - 1;
         }
         ntiles[m] = nt;
         np += int64_t(nt) * (L - m + 1) * B;
     }
-    auto by_cost = [L](const RingItem &a, const RingItem &b) {
-        return int64_t(L - a.m + 1) * std::min(a.nr, 64) > int64_t(L - b.m + 1) * std::min(b.nr, 64);
+    // Orders are dealt to the eight XCDs, longest first onto the least-loaded one, and workgroup i
+    // is laid out for XCD i % 8 (the hardware deals consecutive workgroups round-robin): the tiles
+    // and segments of an order then share one XCD's L2 for the order's coefficient streams.
+    constexpr int kXcd = 8;
+    std::vector<int> order;
+    for (int m = 0; m <= mmax; ++m)
+        if (!tiles_of[m].empty()) order.push_back(m);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<int>> xq(kXcd);
+    std::vector<int64_t> xload(kXcd, 0);
+    for (int m : order) {
+        const int x = int(std::min_element(xload.begin(), xload.end()) - xload.begin());
+        xq[x].push_back(m);
+        xload[x] += cost[m];
+    }
+    const SegItem empty{0, 0, 0, 0, 1, 0, 0};
+    // forward: one workgroup per tile
+    std::vector<std::vector<SegItem>> fq(kXcd);
+    for (int x = 0; x < kXcd; ++x)
+        for (int m : xq[x])
+            for (const SegItem &t : tiles_of[m]) fq[x].push_back(t);
+    // adjoint: one wavefront per (tile, segment), kAdjWaves per workgroup
+    std::vector<std::vector<SegItem>> aq(kXcd);
+    for (int x = 0; x < kXcd; ++x) {
+        for (int m : xq[x])
+            for (const SegItem &t : tiles_of[m])
+                for (int sg = 0; sg < t.nseg; ++sg) {
+                    SegItem w = t;
+                    w.seg = sg;
+                    aq[x].push_back(w);
+                }
+        while (aq[x].size() % kAdjWaves) aq[x].push_back(empty);
+    }
+    auto interleave = [&](const std::vector<std::vector<SegItem>> &q, size_t group) {
+        size_t rounds = 0;
+        for (const auto &v : q) rounds = std::max(rounds, v.size() / group);
+        std::vector<SegItem> out;
+        out.reserve(rounds * kXcd * group);
+        for (size_t i = 0; i < rounds; ++i)
+            for (int x = 0; x < kXcd; ++x)
+                for (size_t g = 0; g < group; ++g)
+                    out.push_back((i + 1) * group <= q[x].size() ? q[x][i * group + g] : empty);
+        return out;
     };
-    std::stable_sort(items.begin(), items.end(), by_cost);
-    std::stable_sort(aitems.begin(), aitems.end(), by_cost);
+    std::vector<SegItem> fitems = interleave(fq, 1), aitems = interleave(aq, kAdjWaves);
     d.fwd_waves = fw;
-    d.adj_waves = aw;
+    d.nfitems = int(fitems.size());
     d.naitems = int(aitems.size());
-    d.nitems = int(items.size());
+    d.nck = nck;
     d.npartials = np;
     std::vector<int32_t> jlo32(r.jlo.begin(), r.jlo.begin() + mmax + 1), jhi32(r.jhi.begin(), r.jhi.begin() + mmax + 1);
     Status st;
@@ -229,12 +266,17 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.abofs, abofs, by)) != kOk) return st;
     if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
     if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
-    if ((st = upload(&d.items, items, by)) != kOk) return st;
+    if ((st = upload(&d.fitems, fitems, by)) != kOk) return st;
     if ((st = upload(&d.aitems, aitems, by)) != kOk) return st;
+    if ((st = dalloc(&d.ckq, size_t(nck) * 64, by)) != kOk) return st;
+    if ((st = dalloc(&d.ckE, size_t(nck) * 64, by)) != kOk) return st;
     if ((st = upload(&d.pofs, pofs, by)) != kOk) return st;
     if ((st = upload(&d.ntiles, ntiles, by)) != kOk) return st;
     if ((st = dalloc(&d.partials, size_t(std::max<int64_t>(np, 1)), by)) != kOk) return st;
     if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
+    launch_ring_checkpoints(d, nullptr);
+    PFSHT_HIP_TRY(hipGetLastError());
+    PFSHT_HIP_TRY(hipStreamSynchronize(nullptr));
     return kOk;
 }
 

@@ -36,15 +36,36 @@ enum Status { kOk = 0, kEDim = 1, kEArg = 2, kEGeom = 3, kENomem = 4, kEHip = 5,
         }                                                                                         \
     } while (0)
 
-// Ring-pass work block: retained rings [j0, j0 + nr) of order m. Wavefront w of the block owns
-// the 64-ring tile tile0 + w (rings j0 + 64 w ..), so all tiles of one order share a compute unit
-// and its scalar/L2 caches for the order's coefficient streams.
-struct RingItem {
-    int32_t m;      // order
-    int32_t j0;     // first ring of the block
-    int32_t nr;     // rings in the block
-    int32_t tile0;  // tile index (within order m) of wavefront 0
+// Ring-pass work. Order m's degrees m..L are cut into nseg segments of whole 64-degree blocks
+// (block k = degrees m + 64 k .. m + 64 k + 63); the recurrence is renormalised after the last
+// degree of every block, and segment s >= 1 starts from the state (q_{l-1}, q_{l-2}, E) that a
+// plan-time sweep stored for it (the checkpoints). One item is one 64-ring tile of one order:
+// the forward runs it as a workgroup with one wavefront per segment, the adjoint as nseg
+// independent wavefronts (one item each, seg set).
+struct SegItem {
+    int32_t m;     // order
+    int32_t j0;    // first ring of the tile
+    int32_t nr;    // rings in the tile, <= 64 (0: an empty slot)
+    int32_t tile;  // tile index within order m
+    int32_t nseg;  // degree segments of order m
+    int32_t seg;   // adjoint items: this wavefront's segment
+    int64_t ck;    // checkpoint row of segment 1 of this tile; segment s uses row ck + s - 1
 };
+
+constexpr int kSegMax = 8;     // segments per order (forward workgroup: one wavefront each)
+constexpr int kSegBlocks = 4;  // target 64-degree blocks per segment
+constexpr int kAdjWaves = 4;   // adjoint workgroup: 4 wavefronts, one 16 x 64 LDS tile each
+
+// First degree of segment s of order m (s = nseg gives a degree past L).
+__host__ __device__ inline int seg_begin(int m, int L, int nseg, int s) {
+    const int nblk = (L - m + 64) >> 6;  // ceil((L - m + 1) / 64)
+    return m + 64 * ((s * nblk) / nseg);
+}
+__host__ __device__ inline int seg_count(int m, int L) {
+    const int nblk = (L - m + 64) >> 6;
+    const int k = (nblk + kSegBlocks - 1) / kSegBlocks;
+    return k < 1 ? 1 : (k > kSegMax ? kSegMax : k);
+}
 
 // Device-resident tables of a ring engine (RingLegendre, src/Legendre.jl:174-255).
 struct RingDev {
@@ -55,18 +76,21 @@ struct RingDev {
     double2 *ab = nullptr;      // recurrence coefficients (a_l^m, b_l^m), l = m+2..L, per m
     int64_t *abofs = nullptr;   // [mmax+1] offset of order m in ab
     int32_t *jlo = nullptr, *jhi = nullptr;  // [mmax+1]
-    RingItem *items = nullptr;  // forward blocks (all tiles of an order), descending cost
-    int nitems = 0, fwd_waves = 1;
-    RingItem *aitems = nullptr;  // adjoint blocks (<= kAdjWaves tiles of an order), descending cost
-    int naitems = 0, adj_waves = 1;
+    SegItem *fitems = nullptr;  // forward workgroups (one tile, nseg wavefronts), XCD-interleaved
+    int nfitems = 0, fwd_waves = 1;
+    SegItem *aitems = nullptr;  // adjoint wavefronts, kAdjWaves per workgroup, XCD-interleaved
+    int naitems = 0;            // a multiple of kAdjWaves
+    double2 *ckq = nullptr;     // [nck][64] checkpoint (q_{l-1}, q_{l-2}) per lane
+    int32_t *ckE = nullptr;     // [nck][64] checkpoint exponent
+    int64_t nck = 0;
     int64_t *pofs = nullptr;    // [mmax+1] offset of order m's adjoint partials (in double4 units)
     int32_t *ntiles = nullptr;  // [mmax+1]
     double4 *partials = nullptr;  // adjoint per-tile partial sums: (re+, im+, re-, im-) per (tile, l)
     int64_t npartials = 0;
 };
 
-constexpr int kFwdMaxWaves = 8;   // forward block: up to 512 rings of one order
-constexpr int kAdjWaves = 4;      // adjoint block: up to 4 tiles of an order, one 16 x 64 LDS tile each
+// The plan-time sweep that stores the checkpoints of every forward item (default stream).
+void launch_ring_checkpoints(const RingDev &r, hipStream_t s);
 
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
 // Gd is the internal ring buffer [b][m + mmax][j].

@@ -66,48 +66,85 @@ constexpr int kNeverSink = 0x7f3c91e5;
 
 }  // namespace
 
+namespace {
+// q_l from q_{l-1}, q_{l-2} (src/Legendre.jl:406-408): the one expression every sweep uses, so the
+// checkpoint sweep and the segments reproduce the same bits.
+__device__ __forceinline__ double rec_step(double2 c, double x, double q1, double q2) {
+    return fma(c.x * x, q1, -(c.y * q2));
+}
+// q_{m+1} from q_m
+__device__ __forceinline__ double first_step(int m, double x, double q) { return sqrt(double(2 * m + 3)) * x * q; }
+// last degree of the 64-degree block holding degree l
+__device__ __forceinline__ int block_end(int m, int l) { return m + ((l - m) | 63); }
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Checkpoints
+// ------------------------------------------------------------------------------------------------
+// One wavefront per forward item sweeps the recurrence of its 64 rings from the seed and stores the
+// state at the start of segments 1 .. nseg - 1 (after the renormalisation that closes the previous
+// block). Runs once per plan.
+__global__ void __launch_bounds__(64)
+ring_ckpt_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                 const double2 *__restrict__ ab, const int64_t *__restrict__ abofs, double2 *__restrict__ ckq,
+                 int32_t *__restrict__ ckE, int L) {
+    const SegItem it = items[blockIdx.x];
+    const int m = __builtin_amdgcn_readfirstlane(it.m);
+    const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
+    const int nr = __builtin_amdgcn_readfirstlane(it.nr);
+    if (nr <= 0 || nseg <= 1) return;
+    const int lane = threadIdx.x & 63;
+    const bool active = lane < nr;
+    const int j = it.j0 + (active ? lane : 0);
+    const double x = costh[j];
+    double q2;
+    int E;
+    seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], q2, E);
+    if (!active) q2 = 0.0;
+    double q1 = m < L ? first_step(m, x, q2) : 0.0;
+    const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
+    int sgi = 1, lnext = seg_begin(m, L, nseg, 1);
+    for (int l = m + 2; l <= L; ++l) {
+        const double q = rec_step(abm[l], x, q1, q2);
+        q2 = q1;
+        q1 = q;
+        if (l == block_end(m, l)) {
+            const int sh = renorm_shift(q1, q2);
+            q1 = ldexp(q1, -sh);
+            q2 = ldexp(q2, -sh);
+            E += sh;
+        }
+        if (l + 1 == lnext) {
+            const int64_t row = (it.ck + sgi - 1) * 64 + lane;
+            ckq[row] = make_double2(q1, q2);
+            ckE[row] = E;
+            if (++sgi == nseg) break;
+            lnext = seg_begin(m, L, nseg, sgi);
+        }
+    }
+}
+
+void launch_ring_checkpoints(const RingDev &r, hipStream_t s) {
+    if (r.nfitems == 0 || r.nck == 0) return;
+    ring_ckpt_kernel<<<r.nfitems, 64, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, r.ckq, r.ckE,
+                                              r.L);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Forward
 // ------------------------------------------------------------------------------------------------
-// One degree sweep of order m feeding NB coefficient sets (transforms b0 .. b0 + NB - 1): the
+// Degrees l .. le of order m feeding NB coefficient sets (transforms b0 .. b0 + NB - 1): the
 // recurrence is computed once per ring and reused for every transform. NEG: m > 0, so the order
-// -m column is accumulated too.
+// -m column is accumulated too. The accumulators share the exponent E of (q1, q2).
 template <int NB, bool NEG>
 __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ cp0,
-                                          const double2 *__restrict__ cm0, int64_t LB, double x, double q2, int E,
-                                          int m, int L, double (&ar)[NB], double (&ai)[NB], double (&br)[NB],
-                                          double (&bi)[NB], int &Eout, int &sink) {
+                                          const double2 *__restrict__ cm0, int64_t LB, double x, double q1,
+                                          double q2, int &E, int l, int le, int m, double (&ar)[NB],
+                                          double (&ai)[NB], double (&br)[NB], double (&bi)[NB], int &sink) {
     // transform t's order-m column is cp0 + t LB (row l - m holds degree l), order -m likewise
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-        const double2 a0 = cp0[t * LB + m];
-        ar[t] = a0.x * q2;
-        ai[t] = a0.y * q2;
-        br[t] = bi[t] = 0.0;
-        if (NEG) {
-            const double2 c0 = cm0[t * LB + m];
-            br[t] = c0.x * q2;
-            bi[t] = c0.y * q2;
-        }
-    }
-    double q1 = 0.0;
-    if (m < L) {
-        q1 = sqrt(double(2 * m + 3)) * x * q2;
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-            const double2 a1 = cp0[t * LB + m + 1];
-            ar[t] = fma(a1.x, q1, ar[t]);
-            ai[t] = fma(a1.y, q1, ai[t]);
-            if (NEG) {
-                const double2 c1 = cm0[t * LB + m + 1];
-                br[t] = fma(c1.x, q1, br[t]);
-                bi[t] = fma(c1.y, q1, bi[t]);
-            }
-        }
-    }
-    int l = m + 2;
     const int lane = threadIdx.x & 63;
-    // Prefetch lanes: 8 lines of 128 B cover a 64-degree block of one 16-byte stream. Stream 0 is
+    // Prefetch lanes: 8 lines of 128 B cover a 64-degree run of one 16-byte stream. Stream 0 is
     // the (a, b) row, streams 1..NB the +m columns and NB+1..2NB the -m columns (NEG only).
     constexpr int NS = 1 + 2 * NB;
     constexpr int NCH = (8 * NS + 63) / 64;
@@ -117,7 +154,7 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         const int k = 64 * c + lane, st = k >> 3, line = k & 7;
-        lstart[c] = 8 * line;  // degree offset of the line within the block
+        lstart[c] = 8 * line;  // degree offset of the line within the run
         isab[c] = st == 0;
         used[c] = st < (NEG ? NS : 1 + NB);
         int64_t o = 128 * line;
@@ -126,33 +163,32 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
         voff[c] = int(o);
     }
     int pending = 0;  // prefetched words of the previous block, folded into `sink` one block later
-    // block b starts at degree m + 2 + 64 b; prefetch blocks 0 and 1 now, block b + 2 at block b
+    // the first two 64-degree runs now, then the run 128 degrees ahead at every block
     for (int pb = 0; pb < 2; ++pb) {
         const int lp = l + 64 * pb;
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-            const bool on = used[c] && lp + lstart[c] <= L;
+            const bool on = used[c] && lp + lstart[c] <= le;
             pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
                                 voff[c], on);
         }
     }
-    while (l <= L) {
-        const int lend = min(L, l + kRescaleStride - 1);
+    while (l <= le) {
+        const int lend = min(le, block_end(m, l));
         {
             sink ^= pending;  // the previous block's prefetch has long completed
             pending = 0;
             const int lp = l + kPrefetchAhead;
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                const bool on = used[c] && lp + lstart[c] <= L;
+                const bool on = used[c] && lp + lstart[c] <= le;
                 pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
                                     voff[c], on);
             }
         }
 #pragma unroll 4
         for (; l <= lend; ++l) {
-            const double2 c = abm[l];
-            const double q = fma(c.x * x, q1, -(c.y * q2));
+            const double q = rec_step(abm[l], x, q1, q2);
             q2 = q1;
             q1 = q;
 #pragma unroll
@@ -167,7 +203,7 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
                 }
             }
         }
-        if (l <= L) {
+        if (l <= le) {  // renormalise at the block end (not after the segment's last degree)
             const int s = renorm_shift(q1, q2);
             q1 = ldexp(q1, -s);
             q2 = ldexp(q2, -s);
@@ -184,31 +220,28 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
         }
     }
     sink ^= pending;
-    Eout = E;
 }
 
+// One workgroup per 64-ring tile of one order; wavefront s sweeps degree segment s. Segment 0
+// starts from the seed, the others from their checkpoints; the segment sums, scaled to plain
+// doubles, are added in segment order through LDS and wavefront 0 stores the tile's ring values.
 template <int NB>
-__global__ void __launch_bounds__(64 * kFwdMaxWaves)
-ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
+__global__ void __launch_bounds__(64 * kSegMax)
+ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
-                const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L, int n, int mmax, int b0,
-                int never) {
-    const RingItem it = items[blockIdx.x];
+                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                const int32_t *__restrict__ ckE, const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L,
+                int n, int mmax, int b0, int never) {
+    __shared__ double red[kSegMax - 1][4][64];
+    const SegItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
+    const int nr = __builtin_amdgcn_readfirstlane(it.nr);
+    if (nr <= 0) return;  // an empty slot: the whole workgroup leaves
+    const int sg = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);  // this wavefront's segment
     const int lane = threadIdx.x & 63;
-    const int nrw = __builtin_amdgcn_readfirstlane(it.nr) - 64 * wave;  // rings of this wavefront's tile
-    if (nrw <= 0) return;
-    const bool active = lane < nrw;
-    const int j = it.j0 + 64 * wave + (active ? lane : 0);
-
-    const double x = costh[j];
-    const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
-    double q2;
-    int E;
-    seed(e0, q2, E);
-    if (!active) q2 = 0.0;
+    const bool active = lane < nr;
+    const int j = it.j0 + (active ? lane : 0);
 
     const int64_t LB = int64_t(L + 1) * (2 * L + 1);
     // column of order m (row l - m holds degree l) and of order -m of transform b0; transform
@@ -218,20 +251,90 @@ ring_fwd_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);  // abm[l] for l = m+2..L
 
     double ar[NB], ai[NB], br[NB], bi[NB];
-    int Ef, sink = 0;
-    if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink);
-    else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q2, E, m, L, ar, ai, br, bi, Ef, sink);
+    int sink = 0;
+    if (sg < nseg) {
+        const double x = costh[j];
+        double q1, q2;
+        int E, l;
+        if (sg == 0) {
+            seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], q2, E);
+            if (!active) q2 = 0.0;
+            q1 = m < L ? first_step(m, x, q2) : 0.0;
+#pragma unroll
+            for (int t = 0; t < NB; ++t) {
+                const double2 a0 = cp0[t * LB + m];
+                ar[t] = a0.x * q2;
+                ai[t] = a0.y * q2;
+                br[t] = bi[t] = 0.0;
+                if (m > 0) {
+                    const double2 c0 = cm0[t * LB + m];
+                    br[t] = c0.x * q2;
+                    bi[t] = c0.y * q2;
+                }
+                if (m < L) {
+                    const double2 a1 = cp0[t * LB + m + 1];
+                    ar[t] = fma(a1.x, q1, ar[t]);
+                    ai[t] = fma(a1.y, q1, ai[t]);
+                    if (m > 0) {
+                        const double2 c1 = cm0[t * LB + m + 1];
+                        br[t] = fma(c1.x, q1, br[t]);
+                        bi[t] = fma(c1.y, q1, bi[t]);
+                    }
+                }
+            }
+            l = m + 2;
+        } else {
+            const int64_t row = (it.ck + sg - 1) * 64 + lane;
+            const double2 c = ckq[row];
+            q1 = c.x;
+            q2 = c.y;
+            E = ckE[row];
+#pragma unroll
+            for (int t = 0; t < NB; ++t) ar[t] = ai[t] = br[t] = bi[t] = 0.0;
+            l = seg_begin(m, L, nseg, sg);
+        }
+        const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
+        if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+        else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            ar[t] = ldexp(ar[t], E);
+            ai[t] = ldexp(ai[t], E);
+            br[t] = ldexp(br[t], E);
+            bi[t] = ldexp(bi[t], E);
+        }
+    }
     if (sink == never) Gd[j] = make_double2(0.0, 0.0);  // keeps the prefetch loads; never true
-    if (!active) return;
-    const double sg = (m & 1) ? -1.0 : 1.0;
+    if (nseg > 1) {
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            if (sg > 0 && sg < nseg) {
+                red[sg - 1][0][lane] = ar[t];
+                red[sg - 1][1][lane] = ai[t];
+                red[sg - 1][2][lane] = br[t];
+                red[sg - 1][3][lane] = bi[t];
+            }
+            __syncthreads();
+            if (sg == 0)
+                for (int k = 0; k < nseg - 1; ++k) {
+                    ar[t] += red[k][0][lane];
+                    ai[t] += red[k][1][lane];
+                    br[t] += red[k][2][lane];
+                    bi[t] += red[k][3][lane];
+                }
+            __syncthreads();
+        }
+    }
+    if (sg != 0 || !active) return;
+    const double sgn = (m & 1) ? -1.0 : 1.0;
     const int64_t ncol = 2 * mmax + 1;
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
         double2 *gp = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + j;
-        *gp = make_double2(sg * ldexp(ar[t], Ef), sg * ldexp(ai[t], Ef));
+        *gp = make_double2(sgn * ar[t], sgn * ai[t]);
         if (m > 0) {
             double2 *gm = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + j;
-            *gm = make_double2(ldexp(br[t], Ef), ldexp(bi[t], Ef));
+            *gm = make_double2(br[t], bi[t]);
         }
     }
 }
@@ -248,11 +351,12 @@ static void for_transform_groups(int B, F &&launch) {
 }
 
 void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s) {
-    if (r.nitems == 0) return;
+    if (r.nfitems == 0) return;
     for_transform_groups(r.B, [&](int nb, int b0) {
-        dim3 grid(r.nitems);
-#define PFSHT_FWD(NB) \
-    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.items, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, alm, Gd, r.L, r.n, r.mmax, b0, kNeverSink)
+        dim3 grid(r.nfitems);
+#define PFSHT_FWD(NB)                                                                                         \
+    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, \
+                                                          r.ckq, r.ckE, alm, Gd, r.L, r.n, r.mmax, b0, kNeverSink)
         switch (nb) {
             case 8: PFSHT_FWD(8); break;
             case 4: PFSHT_FWD(4); break;
@@ -333,11 +437,13 @@ __device__ __forceinline__ double row_sum4(double v) {
     return __hiloint2double(int(b[0]), int(a[0])) + __hiloint2double(int(b[1]), int(a[1]));
 }
 
+// Degrees ls .. le of order m. Segment 0 (ls = m) starts from the seed qs; a later segment from
+// its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}). ls - m is a multiple of 64.
 template <int NB, bool NEG>
 __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ gpc,
                                           const double2 *__restrict__ gmc, int64_t gstride, int nr, double x,
-                                          double qs, int E, int m, int L, int B, double (*W)[kWS],
-                                          double4 *__restrict__ out, int &sink) {
+                                          double qs, double q1, double q2, int E, int ls, int le, int m, int L,
+                                          int B, double (*W)[kWS], double4 *__restrict__ out, int &sink) {
     const int lane = threadIdx.x & 63;
     const bool active = lane < nr;
     // this lane's ring (phase-1 layout) for every transform: (re, im) of orders +m and -m
@@ -358,14 +464,14 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
     // (a, b) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
     // i < 16 owns the line at byte offset 128 i of a 128-degree window
     const int pvoff = 128 * (lane & 15);
-    int pending = l2_touch(reinterpret_cast<const char *>(abm + m + 2), pvoff, lane < 16 && m + 2 + 8 * lane <= L);
-    double q1 = 0.0, q2 = 0.0;  // q_{l-1}, q_{l-2}
-    for (int l0 = m; l0 <= L; l0 += kSub) {
-        const int tl = min(kSub, L - l0 + 1);
+    const int lp0 = ls == m ? m + 2 : ls;
+    int pending = l2_touch(reinterpret_cast<const char *>(abm + lp0), pvoff, lane < 16 && lp0 + 8 * lane <= le);
+    for (int l0 = ls; l0 <= le; l0 += kSub) {
+        const int tl = min(kSub, le - l0 + 1);
         if (((l0 - m) & 63) == 0) {
             sink ^= pending;
             const int lp = l0 + 128;
-            pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= L);
+            pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= le);
         }
         // phase 1: lanes = rings
         int tt = 0;
@@ -374,7 +480,7 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
             W[0][wcol] = ldexp(qs, E);
             tt = 1;
             if (m < L) {
-                const double q = sqrt(double(2 * m + 3)) * x * qs;
+                const double q = first_step(m, x, qs);
                 q2 = q1;
                 q1 = q;
                 W[1][wcol] = ldexp(q, E);
@@ -383,13 +489,12 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
         }
 #pragma unroll 8
         for (; tt < tl; ++tt) {
-            const double2 c = abm[l0 + tt];
-            const double q = fma(c.x * x, q1, -(c.y * q2));
+            const double q = rec_step(abm[l0 + tt], x, q1, q2);
             q2 = q1;
             q1 = q;
             W[tt][wcol] = ldexp(q, E);
         }
-        if (((l0 + kSub - m) & 63) == 0) {  // renormalise every 64 degrees
+        if (((l0 + kSub - m) & 63) == 0) {  // renormalise at the end of every 64-degree block
             const int s = renorm_shift(q1, q2);
             q1 = ldexp(q1, -s);
             q2 = ldexp(q2, -s);
@@ -420,31 +525,44 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
     sink ^= pending;
 }
 
+// Workgroups of kAdjWaves wavefronts; each wavefront is one item: one degree segment of one
+// 64-ring tile of one order, writing that tile's partials for the segment's degrees.
 template <int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
-ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ costh,
+ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs,
-                const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
+                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                const int32_t *__restrict__ ckE, const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
                 double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never) {
     extern __shared__ double lds_adj[];
-    const RingItem it = items[blockIdx.x];
-    const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const SegItem it = items[blockIdx.x * kAdjWaves + wave];
+    const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int lane = threadIdx.x & 63;
-    const int nr = min(64, __builtin_amdgcn_readfirstlane(it.nr) - 64 * wave);  // rings of this tile
-    if (nr <= 0) return;  // wave-uniform: the whole wavefront leaves
-    const int tile = __builtin_amdgcn_readfirstlane(it.tile0) + wave;
-    const int jt = it.j0 + 64 * wave;  // first ring of the tile
+    const int nr = __builtin_amdgcn_readfirstlane(it.nr);
+    if (nr <= 0) return;  // an empty slot (wave-uniform): the whole wavefront leaves
+    const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
+    const int sg = __builtin_amdgcn_readfirstlane(it.seg);
+    const int tile = __builtin_amdgcn_readfirstlane(it.tile);
+    const int jt = it.j0;  // first ring of the tile
     double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds_adj + wave * kSub * kWS);
     const int j = jt + (lane < nr ? lane : 0);
 
     const double x = costh[j];
-    const double e0 = m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m];
-    double qs;
+    double qs = 0.0, q1 = 0.0, q2 = 0.0;
     int E;
-    seed(e0, qs, E);
-    if (lane >= nr) qs = 0.0;
+    if (sg == 0) {
+        seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], qs, E);
+        if (lane >= nr) qs = 0.0;
+    } else {
+        const int64_t row = (it.ck + sg - 1) * 64 + lane;
+        const double2 c = ckq[row];
+        q1 = c.x;
+        q2 = c.y;
+        E = ckE[row];
+    }
+    const int ls = seg_begin(m, L, nseg, sg);
+    const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
 
     const int64_t ncol = 2 * mmax + 1;
     const int64_t gstride = ncol * n;  // next transform
@@ -454,8 +572,8 @@ ring_adj_kernel(const RingItem *__restrict__ items, const double *__restrict__ c
     const int nl = L - m + 1;
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
     int sink = 0;
-    if (m > 0) adj_sweep<NB, true>(abm, gpc, gmc, gstride, nr, x, qs, E, m, L, B, W, out, sink);
-    else adj_sweep<NB, false>(abm, gpc, gmc, gstride, nr, x, qs, E, m, L, B, W, out, sink);
+    if (m > 0) adj_sweep<NB, true>(abm, gpc, gmc, gstride, nr, x, qs, q1, q2, E, ls, le, m, L, B, W, out, sink);
+    else adj_sweep<NB, false>(abm, gpc, gmc, gstride, nr, x, qs, q1, q2, E, ls, le, m, L, B, W, out, sink);
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
@@ -498,11 +616,12 @@ void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStrea
 void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
     if (r.naitems > 0)
         for_transform_groups(r.B, [&](int nb, int b0) {
-            dim3 grid(r.naitems);
+            dim3 grid(r.naitems / kAdjWaves);
             const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
 #define PFSHT_ADJ(NB)                                                                                      \
-    ring_adj_kernel<NB><<<grid, 64 * r.adj_waves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab,      \
-                                                            r.abofs, r.pofs, Gd, r.partials, r.L, r.n, r.mmax, r.B, b0, kNeverSink)
+    ring_adj_kernel<NB><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, \
+                                                          r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, r.mmax,  \
+                                                          r.B, b0, kNeverSink)
             switch (nb) {
                 case 8: PFSHT_ADJ(8); break;
                 case 4: PFSHT_ADJ(4); break;
