@@ -189,6 +189,9 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     int64_t np = 0, nck = 0;
     r.work = 0;
     int fw = 1;
+    // PFSHT_SEG_BLOCKS (1..64) overrides the target segment length, for tuning
+    int per = kSegBlocks;
+    if (const char *e = std::getenv("PFSHT_SEG_BLOCKS")) per = std::min(64, std::max(1, std::atoi(e)));
     for (int m = 0; m <= mmax; ++m) {
         pofs[m] = np;
         const int lo = r.jlo[m], hi = r.jhi[m];
@@ -196,7 +199,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         const int nj = hi - lo + 1;
         r.work += int64_t(nj) * (L - m + 1);
         cost[m] = int64_t(nj) * (L - m + 1);
-        const int nt = (nj + 63) / 64, nseg = seg_count(m, L);
+        const int nt = (nj + 63) / 64, nseg = seg_count(m, L, per);
         fw = std::max(fw, nseg);
         for (int t = 0; t < nt; ++t) {
             const int j0 = lo + 64 * t;
@@ -483,6 +486,25 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     }
     std::vector<double> X1s(M), X2s(M);
     for (int64_t q = 0; q < M; ++q) { X1s[q] = X1[perm[q]]; X2s[q] = X2[perm[q]]; }
+    // Fine-grid rows the kernels reach: the apply stores and transforms only the smallest cyclic
+    // interval holding them (about half of the rows for a polar cap, whose points fill half the
+    // doubled colatitude period).
+    {
+        std::vector<char> on(u.N1, 0);
+        for (int64_t i = 0; i < M; ++i) {
+            const int b1 = int(std::ceil(X1[i] - half));
+            for (int t = 0; t < u.w; ++t) on[(b1 + t) % u.N1] = 1;
+        }
+        int best = 0, best_end = 0;  // longest cyclic run of unreached rows, and the row after it
+        for (int pass = 0, run = 0; pass < 2; ++pass)
+            for (int r = 0; r < u.N1; ++r) {
+                run = on[r] ? 0 : std::min(run + 1, u.N1);
+                if (run > best) { best = run; best_end = (r + 1) % u.N1; }
+            }
+        if (best >= u.N1) best = 0;  // no point at all cannot happen (M >= 1)
+        u.r0 = best > 0 ? best_end : 0;
+        u.nR = u.N1 - best;
+    }
     // Non-empty super-tiles, and per (slot, local bin) the start of its point range: a slot's bins
     // are contiguous in the sorted order and empty super-tiles own no points, so bo[q + 1] is the
     // end of range q for every q, with bo[nst * 16] = M.
@@ -538,8 +560,8 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if ((st = galloc(&p.Ut, ncols * u.N1)) != kOk) return st;
     if ((st = fft_make(p.col_bwd, false, u.N1, 0, ncols, by, single)) != kOk) return st;
     if ((st = fft_make(p.col_fwd, true, u.N1, 0, ncols, by, single)) != kOk) return st;
-    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.N1, by, single)) != kOk) return st;
-    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.N1, by, single)) != kOk) return st;
+    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.nR, by, single)) != kOk) return st;
+    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.nR, by, single)) != kOk) return st;
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
@@ -829,7 +851,9 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard dg(p->dev);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const NufftDev &u = p->nu;
+    NufftDev u = p->nu;  // the seam works on the whole fine grid
+    u.r0 = 0;
+    u.nR = u.N1;
     double2 *Ft = static_cast<double2 *>(p->Ft), *U = static_cast<double2 *>(p->U);
     const size_t modes_bytes = sizeof(double2) * size_t(p->nth) * p->nph * p->B;
     const size_t f_bytes = sizeof(double2) * size_t(p->M) * p->B;

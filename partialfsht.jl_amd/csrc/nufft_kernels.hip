@@ -191,34 +191,37 @@ __global__ void deconv_trunc_ut_kernel(const C *__restrict__ Ut, const double2 *
     stc(Fc + col * n1 + i, make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc));
 }
 
-// U[b][l1][l2] = Ut[b][c(l2)][l1] for the fine columns of retained orders, zero elsewhere.
+// U[b][pr][l2] = Ut[b][c(l2)][(r0 + pr) mod N1] for the fine columns of retained orders, zero
+// elsewhere, for the stored rows pr < nR.
 template <class C>
 __global__ void __launch_bounds__(256)
-ut_to_rows_kernel(const C *__restrict__ Ut, C *__restrict__ U, int mmax, int N1, int N2) {
+ut_to_rows_kernel(const C *__restrict__ Ut, C *__restrict__ U, int mmax, int N1, int N2, int r0, int nR) {
     __shared__ C tile[kDT][kDT + 1];
-    const int l2_0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
+    const int l2_0 = blockIdx.x * kDT, pr_0 = blockIdx.y * kDT, b = blockIdx.z;
     const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
     const int64_t ncol = 2 * mmax + 1;
     for (int rr = ty; rr < kDT; rr += 8) {
-        const int l2 = l2_0 + rr, l1 = l1_0 + tx;
+        const int l2 = l2_0 + rr, pr = pr_0 + tx;
         C v = {};
-        if (l2 < N2 && l1 < N1) {
+        if (l2 < N2 && pr < nR) {
             const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
+            const int l1 = (r0 + pr) % N1;
             if (m != INT32_MIN) v = Ut[(int64_t(b) * ncol + (m + mmax)) * N1 + l1];
         }
         tile[rr][tx] = v;
     }
     __syncthreads();
     for (int rr = ty; rr < kDT; rr += 8) {
-        const int l1 = l1_0 + rr, l2 = l2_0 + tx;
-        if (l1 < N1 && l2 < N2) U[(int64_t(b) * N1 + l1) * N2 + l2] = tile[tx][rr];
+        const int pr = pr_0 + rr, l2 = l2_0 + tx;
+        if (pr < nR && l2 < N2) U[(int64_t(b) * nR + pr) * N2 + l2] = tile[tx][rr];
     }
 }
 
-// Ut[b][c][l1] = U[b][l1][l2(c)]: the transpose, keeping only the fine columns of retained orders.
+// Ut[b][c][l1] = U[b][l1][l2(c)]: the transpose, keeping only the fine columns of retained orders;
+// rows that are not stored are zero.
 template <class C>
 __global__ void __launch_bounds__(256)
-rows_to_ut_kernel(const C *__restrict__ U, C *__restrict__ Ut, int mmax, int N1, int N2) {
+rows_to_ut_kernel(const C *__restrict__ U, C *__restrict__ Ut, int mmax, int N1, int N2, int r0, int nR) {
     __shared__ C tile[kDT][kDT + 1];
     const int c0 = blockIdx.x * kDT, l1_0 = blockIdx.y * kDT, b = blockIdx.z;
     const int tx = threadIdx.x % kDT, ty = threadIdx.x / kDT;
@@ -226,9 +229,11 @@ rows_to_ut_kernel(const C *__restrict__ U, C *__restrict__ Ut, int mmax, int N1,
     for (int rr = ty; rr < kDT; rr += 8) {
         const int l1 = l1_0 + rr, c = c0 + tx;
         C v = {};
-        if (l1 < N1 && c < ncol) {
+        int pr = l1 - r0;
+        if (pr < 0) pr += N1;
+        if (l1 < N1 && c < ncol && pr < nR) {
             const int m = c - mmax;
-            v = U[(int64_t(b) * N1 + l1) * N2 + (m >= 0 ? m : m + N2)];
+            v = U[(int64_t(b) * nR + pr) * N2 + (m >= 0 ? m : m + N2)];
         }
         tile[rr][tx] = v;
     }
@@ -255,14 +260,14 @@ void launch_deconv_trunc_ut(const NufftDev &u, const double2 *pc1, int mmax, int
 
 template <class C>
 void launch_ut_to_rows(const NufftDev &u, int mmax, const C *Ut, C *U, hipStream_t s) {
-    dim3 grid((u.N2 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
-    ut_to_rows_kernel<C><<<grid, 256, 0, s>>>(Ut, U, mmax, u.N1, u.N2);
+    dim3 grid((u.N2 + kDT - 1) / kDT, (u.nR + kDT - 1) / kDT, u.B);
+    ut_to_rows_kernel<C><<<grid, 256, 0, s>>>(Ut, U, mmax, u.N1, u.N2, u.r0, u.nR);
 }
 
 template <class C>
 void launch_rows_to_ut(const NufftDev &u, int mmax, const C *U, C *Ut, hipStream_t s) {
     dim3 grid((2 * mmax + 1 + kDT - 1) / kDT, (u.N1 + kDT - 1) / kDT, u.B);
-    rows_to_ut_kernel<C><<<grid, 256, 0, s>>>(U, Ut, mmax, u.N1, u.N2);
+    rows_to_ut_kernel<C><<<grid, 256, 0, s>>>(U, Ut, mmax, u.N1, u.N2, u.r0, u.nR);
 }
 
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s) {
@@ -284,17 +289,20 @@ template <int W, class C>
 __global__ void __launch_bounds__(256)
 interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  const C *__restrict__ U, double2 *__restrict__ f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, int64_t M, int N1, int N2, int S4, int ns2, int win) {
+                 const int32_t *__restrict__ bin_ofs, int64_t M, int N1, int N2, int S4, int ns2, int win, int r0,
+                 int nR) {
     extern __shared__ __align__(16) unsigned char lds_ip_raw[];
     C *lw = reinterpret_cast<C *>(lds_ip_raw);
     const int slot = blockIdx.x;
     const int b = blockIdx.y;
     const int st = st_list[slot];
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
-    const C *__restrict__ Ub = U + int64_t(b) * N1 * N2;
+    const C *__restrict__ Ub = U + int64_t(b) * nR * N2;
     for (int r = threadIdx.x >> 6; r < win; r += 4) {
-        const int row = (x0 + r) % N1;
-        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(row) * N2 + (y0 + c) % N2];
+        int pr = (x0 + r) % N1 - r0;
+        if (pr < 0) pr += N1;
+        if (pr >= nR) continue;  // a row that no point's kernel reaches is neither stored nor read
+        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + (y0 + c) % N2];
     }
     __syncthreads();
     const int start = bin_ofs[slot * 16], end = bin_ofs[slot * 16 + 16];
@@ -333,9 +341,10 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 // polynomial coefficients (scalar operands). The rows go to a wave-private LDS staging area at the
 // point's offset within its bin's 16-cell window, zero elsewhere, with f at the point. The wave then
 // accumulates the chunk on FP64 MFMA, four points per instruction (A = f psi_row, B = psi_col),
-// keeping each of its bins' 16 x 16 tiles in registers. No workgroup barrier until the end, where
-// the tiles are added into the LDS window wave after wave (bin windows overlap) and the window is
-// written out for the merge pass.
+// keeping each of its bins' 16 x 16 tiles in registers. The inputs of the next chunk (its
+// coordinates and its f values) are loaded while the current chunk is evaluated and accumulated. No workgroup barrier until the end, where the tiles are added
+// into the LDS window wave after wave (bin windows overlap) and the window is written out for the
+// merge pass.
 template <int W, class C>
 __global__ void __launch_bounds__(64 * kSpreadWaves)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
@@ -358,6 +367,8 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     double2 *Fc = reinterpret_cast<double2 *>(K2 + 16 * kRowPad);  // [16]
     const int ii = lane & 15, kq = lane >> 4;                    // MFMA operand layout
     const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;  // kernel-row evaluation
+    const double *__restrict__ Xa = axis == 0 ? X1 : X2;
+    const double2 *__restrict__ fb = f + int64_t(b) * M;
     for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_double2(0.0, 0.0);
     const int32_t *__restrict__ bo = bin_ofs + slot * 16;
     // the bin being accumulated is always tile 0; after each bin the tiles rotate by one, so after
@@ -369,54 +380,97 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         dr[k] = double4v{0.0, 0.0, 0.0, 0.0};
         di[k] = double4v{0.0, 0.0, 0.0, 0.0};
     }
-#pragma unroll 1
+    int bs[NBW], be[NBW];  // point range [bs[k], be[k]) of this wave's bin k
+#pragma unroll
     for (int k = 0; k < NBW; ++k) {
-        const int lb = wave + kSpreadWaves * k;
-        const int ps = bo[lb], pe = bo[lb + 1];
-        for (int c0 = ps; c0 < pe; c0 += 16) {
-            const int n = min(16, pe - c0);
-            if (pa < n) {
-                const int p = c0 + pa;
-                const double x = axis == 0 ? X1[p] : X2[p];
-                const int bc = int(ceil(x - 0.5 * W));
-                const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
-                const double u = 2.0 * (double(bc) - x) + double(W - 1);
-                double kv[NH];
-                es_half_row<W>(h == 0 ? u : -u, kv);
-                double *row = (axis == 0 ? K1 : K2) + pa * kRowPad;
+        bs[k] = bo[wave + kSpreadWaves * k];
+        be[k] = bo[wave + kSpreadWaves * k + 1];
+    }
+    // first chunk: (k, c) with c the chunk start in bin k; k = NBW when none is left
+    int k = 0, c = bs[0];
+    while (k < NBW && c >= be[k]) {
+        ++k;
+        if (k < NBW) c = bs[k];
+    }
+    // one rotation per empty bin skipped (all NBW, an identity, when the wave has no points)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0;
-                // the middle cell of an odd W is written by the left half only
+    for (int r = 0; r < NBW; ++r)
+        if (r < k) {
+            const double4v tr = dr[0], ti = di[0];
 #pragma unroll
-                for (int t = 0; t < NH; ++t)
-                    if (h == 0) row[o + t] = kv[t];
-                    else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
-                if (axis == 0 && h == 0) Fc[pa] = f[int64_t(perm[p]) + int64_t(b) * M];
+            for (int q = 0; q + 1 < NBW; ++q) {
+                dr[q] = dr[q + 1];
+                di[q] = di[q + 1];
             }
-            wave_lds_order();
-            for (int g = 0; g < n; g += 4) {
-                const int q = g + kq;
-                double are = 0.0, aim = 0.0, c = 0.0;
-                if (q < n) {
-                    const double a = K1[q * kRowPad + ii];
-                    const double2 fv = Fc[q];
-                    are = a * fv.x;
-                    aim = a * fv.y;
-                    c = K2[q * kRowPad + ii];
+            dr[NBW - 1] = tr;
+            di[NBW - 1] = ti;
+        }
+    double xn = 0.0;
+    double2 fn = make_double2(0.0, 0.0);
+    if (k < NBW && c + pa < be[k]) {
+        xn = Xa[c + pa];
+        if (axis == 0 && h == 0) fn = fb[perm[c + pa]];
+    }
+    while (k < NBW) {
+        const int n = min(16, be[k] - c);
+        const double x = xn;
+        const double2 fv = fn;
+        // the next chunk, possibly in a later bin
+        int k2 = k, c2 = c + 16;
+        while (k2 < NBW && c2 >= be[k2]) {
+            ++k2;
+            if (k2 < NBW) c2 = bs[k2];
+        }
+        if (k2 < NBW && c2 + pa < be[k2]) {
+            xn = Xa[c2 + pa];
+            if (axis == 0 && h == 0) fn = fb[perm[c2 + pa]];
+        }
+        if (pa < n) {
+            const int bc = int(ceil(x - 0.5 * W));
+            const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
+            const double u = 2.0 * (double(bc) - x) + double(W - 1);
+            double kv[NH];
+            es_half_row<W>(h == 0 ? u : -u, kv);
+            double *row = (axis == 0 ? K1 : K2) + pa * kRowPad;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0;
+            // the middle cell of an odd W is written by the left half only
+#pragma unroll
+            for (int t = 0; t < NH; ++t)
+                if (h == 0) row[o + t] = kv[t];
+                else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
+            if (axis == 0 && h == 0) Fc[pa] = fv;
+        }
+        wave_lds_order();
+        for (int g = 0; g < n; g += 4) {
+            const int q = g + kq;
+            double are = 0.0, aim = 0.0, cc = 0.0;
+            if (q < n) {
+                const double a = K1[q * kRowPad + ii];
+                const double2 fq = Fc[q];
+                are = a * fq.x;
+                aim = a * fq.y;
+                cc = K2[q * kRowPad + ii];
+            }
+            dr[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, cc, dr[0], 0, 0, 0);
+            di[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, cc, di[0], 0, 0, 0);
+        }
+        wave_lds_order();
+        // one rotation per bin finished (the current one and any empty ones skipped)
+#pragma unroll
+        for (int r = 0; r < NBW; ++r)
+            if (r < k2 - k) {
+                const double4v tr = dr[0], ti = di[0];
+#pragma unroll
+                for (int q = 0; q + 1 < NBW; ++q) {
+                    dr[q] = dr[q + 1];
+                    di[q] = di[q + 1];
                 }
-                dr[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, c, dr[0], 0, 0, 0);
-                di[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, c, di[0], 0, 0, 0);
+                dr[NBW - 1] = tr;
+                di[NBW - 1] = ti;
             }
-            wave_lds_order();
-        }
-        const double4v tr = dr[0], ti = di[0];
-#pragma unroll
-        for (int q = 0; q + 1 < NBW; ++q) {
-            dr[q] = dr[q + 1];
-            di[q] = di[q + 1];
-        }
-        dr[NBW - 1] = tr;
-        di[NBW - 1] = ti;
+        k = k2;
+        c = c2;
     }
     __syncthreads();
     // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds row
@@ -446,9 +500,10 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
 template <class C>
 __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__restrict__ st_slot,
                                     C *__restrict__ U, int N1, int N2, int S4, int ns1, int ns2, int win, int w,
-                                    int nst) {
+                                    int nst, int r0, int nR) {
     const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l1 = blockIdx.y;
+    const int pr = blockIdx.y;  // stored row
+    const int l1 = (r0 + pr) % N1;
     const int b = blockIdx.z;
     if (l2 >= N2) return;
     const int q1 = l1 / S4, q2 = l2 / S4;
@@ -474,7 +529,7 @@ __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__
             }
         }
     }
-    stc(U + (int64_t(b) * N1 + l1) * N2 + l2, make_double2(sr, si));
+    stc(U + (int64_t(b) * nR + pr) * N2 + l2, make_double2(sr, si));
 }
 
 template <int W, class C>
@@ -483,7 +538,7 @@ static void interp_w(const NufftDev &u, const C *U, double2 *f, hipStream_t s) {
     dim3 grid(u.nst, u.B);
     const size_t lds = sizeof(C) * size_t(u.win) * u.win;
     interp_st_kernel<W, C><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list, u.bin_ofs, u.M, u.N1, u.N2,
-                                                  4 * u.S, u.ns2, u.win);
+                                                  4 * u.S, u.ns2, u.win, u.r0, u.nR);
 }
 
 template <int W, class C>
@@ -522,9 +577,9 @@ template <class C>
 void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s) {
     C *Wst = static_cast<C *>(u.Wst);
     PFSHT_W_SWITCH(u.w, spread_w, u, f, Wst, s);
-    dim3 grid((u.N2 + 255) / 256, u.N1, u.B);
+    dim3 grid((u.N2 + 255) / 256, u.nR, u.B);
     spread_merge_kernel<C><<<grid, 256, 0, s>>>(Wst, u.st_slot, U, u.N1, u.N2, 4 * u.S, u.ns1, u.ns2, u.win, u.w,
-                                                u.nst);
+                                                u.nst, u.r0, u.nR);
 }
 
 #define PFSHT_INSTANTIATE(C)                                                                                   \

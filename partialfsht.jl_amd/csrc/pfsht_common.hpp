@@ -53,7 +53,7 @@ struct SegItem {
 };
 
 constexpr int kSegMax = 8;     // segments per order (forward workgroup: one wavefront each)
-constexpr int kSegBlocks = 4;  // target 64-degree blocks per segment
+constexpr int kSegBlocks = 4;  // default target of 64-degree blocks per segment
 constexpr int kAdjWaves = 4;   // adjoint workgroup: 4 wavefronts, one 16 x 64 LDS tile each
 
 // First degree of segment s of order m (s = nseg gives a degree past L).
@@ -61,9 +61,10 @@ __host__ __device__ inline int seg_begin(int m, int L, int nseg, int s) {
     const int nblk = (L - m + 64) >> 6;  // ceil((L - m + 1) / 64)
     return m + 64 * ((s * nblk) / nseg);
 }
-__host__ __device__ inline int seg_count(int m, int L) {
+// Segments of order m for a target of `per` blocks per segment.
+inline int seg_count(int m, int L, int per) {
     const int nblk = (L - m + 64) >> 6;
-    const int k = (nblk + kSegBlocks - 1) / kSegBlocks;
+    const int k = (nblk + per - 1) / per;
     return k < 1 ? 1 : (k > kSegMax ? kSegMax : k);
 }
 
@@ -122,6 +123,10 @@ void launch_direct_adjoint(const double2 *f, const double *phis, const int32_t *
 struct NufftDev {
     int B = 1, n1 = 0, n2 = 0;     // modes (nth, nph)
     int N1 = 0, N2 = 0, w = 0;     // fine grid and kernel width
+    // Rows of the fine grid that kernels of the points reach: the cyclic interval r0 .. r0 + nR - 1
+    // (mod N1). The apply stores only those rows, U [b][pr][l2] with row (r0 + pr) mod N1; the
+    // NUFFT seam uses r0 = 0, nR = N1.
+    int r0 = 0, nR = 0;
     double beta = 0;
     int64_t M = 0;
     double *X1 = nullptr, *X2 = nullptr;  // sorted points in fine-grid units, base cell in [0, N)
