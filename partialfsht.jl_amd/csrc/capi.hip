@@ -137,7 +137,7 @@ struct pfsht_rings_s {
     std::mutex mu;
 
     ~pfsht_rings_s() {
-        dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.abofs);
+        dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.gc); dfree(d.abofs);
         dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
     }
@@ -181,6 +181,19 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
                                         double(l + m - 1) / double(l + m));
             ab[abofs[m] + (l - m - 2)] = make_double2(a, bb);
         }
+    // the rescaled forward's (gamma_l, C_l) = (b_l / (a_l a_{l-1}), prod of a_k over the degrees of
+    // l's 16-degree group up to l), with a_{m+1} = sqrt(2m + 3) and group 0 starting at m + 2
+    std::vector<double2> gc(ab.size());
+    for (int m = 0; m <= mmax; ++m) {
+        double aprev = std::sqrt(double(2 * m + 3)), C = 1.0;
+        for (int l = m + 2; l <= L; ++l) {
+            const double2 c = ab[abofs[m] + (l - m - 2)];
+            if (((l - m) & 15) == 0) C = 1.0;
+            C *= c.x;
+            gc[abofs[m] + (l - m - 2)] = make_double2(c.y / (c.x * aprev), C);
+            aprev = c.x;
+        }
+    }
     // Work items: one per (order, 64-ring tile), with the order's degree segments; checkpoint rows
     // for segments 1.. of every tile. Partials of the adjoint are indexed by the order's tile.
     std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
@@ -266,6 +279,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.log2sin, log2sin, by)) != kOk) return st;
     if ((st = upload(&d.log2c, log2c, by)) != kOk) return st;
     if ((st = upload(&d.ab, ab, by)) != kOk) return st;
+    if ((st = upload(&d.gc, gc, by)) != kOk) return st;
     if ((st = upload(&d.abofs, abofs, by)) != kOk) return st;
     if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
     if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;

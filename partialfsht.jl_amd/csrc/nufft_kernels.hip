@@ -442,16 +442,24 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             if (axis == 0 && h == 0) Fc[pa] = fv;
         }
         wave_lds_order();
-        for (int g = 0; g < n; g += 4) {
-            const int q = g + kq;
-            double are = 0.0, aim = 0.0, cc = 0.0;
-            if (q < n) {
-                const double a = K1[q * kRowPad + ii];
-                const double2 fq = Fc[q];
-                are = a * fq.x;
-                aim = a * fq.y;
-                cc = K2[q * kRowPad + ii];
-            }
+        // all operands of the chunk first (one LDS wait), then up to four MFMA pairs; staging rows of
+        // points q >= n hold stale values and are selected away, never multiplied
+        double av[4], cv[4];
+        double2 fq[4];
+#pragma unroll
+        for (int gi = 0; gi < 4; ++gi) {
+            const int q = 4 * gi + kq;
+            av[gi] = K1[q * kRowPad + ii];
+            cv[gi] = K2[q * kRowPad + ii];
+            fq[gi] = Fc[q];
+        }
+#pragma unroll
+        for (int gi = 0; gi < 4; ++gi) {
+            if (4 * gi >= n) break;  // wave-uniform
+            const bool ok = 4 * gi + kq < n;
+            const double are = ok ? av[gi] * fq[gi].x : 0.0;
+            const double aim = ok ? av[gi] * fq[gi].y : 0.0;
+            const double cc = ok ? cv[gi] : 0.0;
             dr[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, cc, dr[0], 0, 0, 0);
             di[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, cc, di[0], 0, 0, 0);
         }

@@ -75,6 +75,7 @@ struct RingDev {
     double *log2sin = nullptr;  // [n]
     double *log2c = nullptr;    // [L+1] log2 sectorial prefactor
     double2 *ab = nullptr;      // recurrence coefficients (a_l^m, b_l^m), l = m+2..L, per m
+    double2 *gc = nullptr;      // the rescaled forward's (gamma_l, C_l), same indexing as ab
     int64_t *abofs = nullptr;   // [mmax+1] offset of order m in ab
     int32_t *jlo = nullptr, *jhi = nullptr;  // [mmax+1]
     SegItem *fitems = nullptr;  // forward workgroups (one tile, nseg wavefronts), XCD-interleaved

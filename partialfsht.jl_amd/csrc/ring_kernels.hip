@@ -222,14 +222,140 @@ __device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const
     sink ^= pending;
 }
 
+// Single-transform sweep in the rescaled form. Degrees run in groups of 16 aligned at m (group 0
+// of segment 0 starts at m + 2). Within a group r_l = q_l / C_l with C_l the product of a_k over
+// the group's degrees up to l, so that
+//     r_l = x r_{l-1} - gamma_l r_{l-2},   gamma_l = b_l / (a_l a_{l-1}),
+// and the accumulators take a_lm C_l r_l = a_lm q_l. The plan holds (gamma_l, C_l) per (m, l).
+// Lane k of every 16-lane row holds degree g + k of the group (its gamma, C and a_lm), loaded with
+// vector loads one group ahead; the FMAs read them through DPP row broadcasts (row_newbcast:k).
+// At the group end (r_{l-1}, r_{l-2}) times C of the last degree gives the next group's start
+// (q_{l-1}, a_{l-1} q_{l-2}); renormalisation follows at 64-degree block ends as in the scalar
+// sweep. 6 FP64 FMAs per degree and ring, no scalar loads.
+template <int K, bool NEG>
+__device__ __forceinline__ double dpp_step(double x, double r1, double r2, double ng, const double (&c)[4],
+                                           double (&acc)[4]) {
+    double t;
+    // s_nop 1: a DPP source must not be written by VALU in the two instructions before its use
+    if (NEG)
+        asm("s_nop 1\n\t"
+            "v_mul_f64 %0, %5, %6\n\t"
+            "v_fmac_f64_dpp %0, %7, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %10, %0 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %2, %11, %0 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %3, %12, %0 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %4, %13, %0 row_newbcast:%9 row_mask:0xf bank_mask:0xf"
+            : "=&v"(t), "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])
+            : "v"(x), "v"(r1), "v"(ng), "v"(r2), "n"(K), "v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]));
+    else
+        asm("s_nop 1\n\t"
+            "v_mul_f64 %0, %3, %4\n\t"
+            "v_fmac_f64_dpp %0, %5, %6 row_newbcast:%7 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %8, %0 row_newbcast:%7 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %2, %9, %0 row_newbcast:%7 row_mask:0xf bank_mask:0xf"
+            : "=&v"(t), "+v"(acc[0]), "+v"(acc[1])
+            : "v"(x), "v"(r1), "v"(ng), "v"(r2), "n"(K), "v"(c[0]), "v"(c[1]));
+    return t;
+}
+
+// Steps K .. 15 of a group; GUARD: only degrees g + K with kfirst <= K < kend.
+template <int K, bool GUARD, bool NEG>
+struct DppGroup {
+    static __device__ __forceinline__ void run(double x, double &r1, double &r2, double ng, const double (&c)[4],
+                                               double (&acc)[4], int kfirst, int kend) {
+        if (!GUARD || (K >= kfirst && K < kend)) {
+            const double t = dpp_step<K, NEG>(x, r1, r2, ng, c, acc);
+            r2 = r1;
+            r1 = t;
+        }
+        DppGroup<K + 1, GUARD, NEG>::run(x, r1, r2, ng, c, acc, kfirst, kend);
+    }
+};
+template <bool GUARD, bool NEG>
+struct DppGroup<16, GUARD, NEG> {
+    static __device__ __forceinline__ void run(double, double &, double &, double, const double (&)[4], double (&)[4],
+                                               int, int) {}
+};
+
+// Degrees l .. le (l = m + 2, or a 64-degree block start) from (q1, q2) = (q_{l-1}, q_{l-2}) with
+// exponent E; aprev = a_{l-1} (sqrt(2m + 3) for l = m + 2). gcm[l] = (gamma_l, C_l).
+template <bool NEG>
+__device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, const double2 *__restrict__ cp0,
+                                              const double2 *__restrict__ cm0, double x, double q1, double q2,
+                                              double aprev, int &E, int l, int le, int m, int L,
+                                              double (&acc)[4], int &sink) {
+    const int lane = threadIdx.x & 63, k = lane & 15;
+    double r1 = q1, r2 = q2 * aprev;
+    int g = m + ((l - m) & ~15);  // group start
+    int kfirst = l - g;
+    // Every load is unconditional, at an index clamped into the order's tables, so that the
+    // compiler can count them and wait only for the ones in use: a lane outside [kfirst, kend) of
+    // a group loads a neighbouring degree and is never broadcast.
+    auto group_load = [&](int gs, double2 &gcv, double2 &ap, double2 &am) {
+        const int lc = min(max(gs + k, m + 2), L);
+        gcv = gcm[lc];
+        ap = cp0[lc];
+        if (NEG) am = cm0[lc];
+    };
+    // L2 prefetch of the run two 64-degree blocks ahead, at every block start: lane i touches line
+    // i % 8 of stream (i / 8) % 3 (gamma/C, a_lm of +m, of -m); the word read is folded into `sink`
+    // one block later, so its wait never stalls the sweep
+    const int pst = (lane >> 3) % (NEG ? 3 : 2), pline = lane & 7;
+    const char *pbase = pst == 0 ? reinterpret_cast<const char *>(gcm)
+                                 : reinterpret_cast<const char *>(pst == 1 ? cp0 : cm0);
+    auto touch = [&](int lp) {
+        const int lt = min(max(lp + 8 * pline, m + 2), L);
+        return *reinterpret_cast<const int *>(pbase + int64_t(lt) * 16);
+    };
+    int pa = 0, pb = touch(l + 64);
+    double2 gcv, ap, am = make_double2(0.0, 0.0);
+    group_load(g, gcv, ap, am);
+    while (g <= le) {
+        const int gn = g + 16;
+        double2 gcn, apn, amn = make_double2(0.0, 0.0);
+        group_load(gn, gcn, apn, amn);
+        if (((g - m) & 63) == 0) {
+            sink ^= pa;
+            pa = pb;
+            pb = touch(g + kPrefetchAhead);
+        }
+        const double C = gcv.y, ng = -gcv.x;
+        double c[4] = {ap.x * C, ap.y * C, am.x * C, am.y * C};
+        const int kend = min(16, le - g + 1);
+        if (kend == 16 && kfirst == 0) DppGroup<0, false, NEG>::run(x, r1, r2, ng, c, acc, 0, 16);
+        else if (kend == 16 && kfirst == 2) DppGroup<2, false, NEG>::run(x, r1, r2, ng, c, acc, 2, 16);
+        else DppGroup<0, true, NEG>::run(x, r1, r2, ng, c, acc, kfirst, kend);
+        // back to the q scale at the group end: C of the group's last degree, from lane kend - 1
+        const int last = kend - 1;
+        const double Ce = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(C), last),
+                                           __builtin_amdgcn_readlane(__double2loint(C), last));
+        r1 *= Ce;
+        r2 *= Ce;
+        g = gn;
+        kfirst = 0;
+        if (g <= le && ((g - m) & 63) == 0) {  // renormalise at the block end
+            const int s = renorm_shift(r1, r2);
+            r1 = ldexp(r1, -s);
+            r2 = ldexp(r2, -s);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] = ldexp(acc[u], -s);
+            E += s;
+        }
+        gcv = gcn;
+        ap = apn;
+        am = amn;
+    }
+    sink ^= pa ^ pb;
+}
+
 // One workgroup per 64-ring tile of one order; wavefront s sweeps degree segment s. Segment 0
 // starts from the seed, the others from their checkpoints; the segment sums, scaled to plain
 // doubles, are added in segment order through LDS and wavefront 0 stores the tile's ring values.
 template <int NB>
-__global__ void __launch_bounds__(64 * kSegMax)
-ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+__device__ __forceinline__ void ring_fwd_body(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
+                const double2 *__restrict__ ckq,
                 const int32_t *__restrict__ ckE, const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L,
                 int n, int mmax, int b0, int never) {
     __shared__ double red[kSegMax - 1][4][64];
@@ -294,8 +420,23 @@ ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
             l = seg_begin(m, L, nseg, sg);
         }
         const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
-        if (m > 0) fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
-        else fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+        if (NB == 1) {
+            if (l <= le) {
+                const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
+                const double aprev = l == m + 2 ? sqrt(double(2 * m + 3)) : abm[l - 1].x;
+                double acc[4] = {ar[0], ai[0], br[0], bi[0]};
+                if (m > 0) fwd_sweep_dpp<true>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
+                else fwd_sweep_dpp<false>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
+                ar[0] = acc[0];
+                ai[0] = acc[1];
+                br[0] = acc[2];
+                bi[0] = acc[3];
+            }
+        } else if (m > 0) {
+            fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+        } else {
+            fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+        }
 #pragma unroll
         for (int t = 0; t < NB; ++t) {
             ar[t] = ldexp(ar[t], E);
@@ -339,6 +480,27 @@ ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     }
 }
 
+// The single-transform sweep holds the DPP operands of two groups; six wavefronts per SIMD fit
+// its registers.
+__global__ void __launch_bounds__(64 * kSegMax) __attribute__((amdgpu_waves_per_eu(6)))
+ring_fwd1_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                 const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
+                 const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, const double2 *__restrict__ alm,
+                 double2 *__restrict__ Gd, int L, int n, int mmax, int b0, int never) {
+    ring_fwd_body<1>(items, costh, log2sin, log2c, ab, gc, abofs, ckq, ckE, alm, Gd, L, n, mmax, b0, never);
+}
+
+template <int NB>
+__global__ void __launch_bounds__(64 * kSegMax)
+ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+                const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
+                const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, const double2 *__restrict__ alm,
+                double2 *__restrict__ Gd, int L, int n, int mmax, int b0, int never) {
+    ring_fwd_body<NB>(items, costh, log2sin, log2c, ab, gc, abofs, ckq, ckE, alm, Gd, L, n, mmax, b0, never);
+}
+
 // Transform groups of 8, 4, 2 and 1 cover any ntrans exactly with at most four launches.
 template <class F>
 static void for_transform_groups(int B, F &&launch) {
@@ -355,13 +517,17 @@ void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStrea
     for_transform_groups(r.B, [&](int nb, int b0) {
         dim3 grid(r.nfitems);
 #define PFSHT_FWD(NB)                                                                                         \
-    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, \
+    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, \
                                                           r.ckq, r.ckE, alm, Gd, r.L, r.n, r.mmax, b0, kNeverSink)
         switch (nb) {
             case 8: PFSHT_FWD(8); break;
             case 4: PFSHT_FWD(4); break;
             case 2: PFSHT_FWD(2); break;
-            default: PFSHT_FWD(1); break;
+            default:
+                ring_fwd1_kernel<<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,
+                                                                  r.abofs, r.ckq, r.ckE, alm, Gd, r.L, r.n, r.mmax, b0,
+                                                                  kNeverSink);
+                break;
         }
 #undef PFSHT_FWD
     });

@@ -23,7 +23,8 @@ __all__ = ["Plan", "AdjointPlan", "Rings", "mul", "coeff_size", "sph_mode", "zer
            "PFSHTError", "DimensionMismatch", "ArgumentError", "lib", "library_path"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(os.path.dirname(_HERE), "lib", "libpfsht_hip.so")
+# PFSHT_HIP_LIB names another build of the library (the Julia extension reads the same variable)
+_LIB = os.environ.get("PFSHT_HIP_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libpfsht_hip.so")
 
 
 class PFSHTError(RuntimeError):
