@@ -116,6 +116,52 @@ int pfsht_set_timing(pfsht_plan_t plan, int32_t enable);
 int pfsht_stage_times(pfsht_plan_t plan, int32_t dir, double *ms, int64_t *counts, int32_t reset);
 
 /* ---------------------------------------------------------------------------------------------
+ * Bounding box and analysis on a plan (src/Box.jl, src/Analysis.jl). The iterative callers keep
+ * every vector in HBM and reuse a workspace the plan allocates at the first call (AnalysisWorkspace,
+ * src/Analysis.jl:45-84): rhs, r, p, Ap in the alm layout and two fields. Dot products are
+ * deterministic (fixed-order reductions), so results are bit-identical run to run. For
+ * where == PFSHT_WHERE_HOST every array argument is host memory; for PFSHT_WHERE_DEVICE every
+ * one is device memory except prior, which is always host memory. The calls return when their
+ * outputs are ready.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    double thlo, thhi;       /* colatitude range, clamped to [0, pi] */
+    double phistart, phiext; /* smallest longitude arc holding the points; 2 pi when touching a pole */
+    int32_t touches_north, touches_south;
+    double fsky;             /* box_fsky: (cos thlo - cos thhi) phiext / 4 pi */
+    double dl;               /* spectral_resolution: pi / max(theta extent, phiext sin(mid theta)) */
+} pfsht_box;
+
+/* bounding_box(theta, phi) — src/Box.jl:43-73 (with _circular_extent, :80-102), box_fsky :106-108,
+ * spectral_resolution :148-153. Host arrays; no device work. */
+int pfsht_bounding_box(const double *theta, const double *phi, int64_t npts, pfsht_box *out);
+
+/* apply_resolution!(out, plan, alm, w, ws) — src/Analysis.jl:96-104: out = A' (w .* (A alm)).
+ * w: npts real weights (NULL: all ones). out overwritten, padding included. */
+int pfsht_apply_resolution(pfsht_plan_t plan, const void *alm, const double *w, void *out, int32_t where,
+                           void *stream);
+
+/* pseudo_spectrum!(C, plan, f, w, ws; batch) — src/Analysis.jl:286-308:
+ * C[l] = sum_m |a~_lm|^2 / (2l + 1) with a~ = A' (w .* f) of transform `batch` (1-based). C has
+ * lmax + 1 entries. */
+int pfsht_pseudo_spectrum(pfsht_plan_t plan, const void *f, const double *w, int32_t batch, double *C,
+                          int32_t where, void *stream);
+
+/* solve!(alm, plan, f; mode, rtol, maxiter, prior, w) — src/Analysis.jl:375-436. Conjugate
+ * gradients on the normal equations A' W A alm = A' W f from alm = 0, stopped when
+ * |r| / |A' W f| <= rtol (min(rtol, 1e-12) for MINNORM); WIENER adds diag(1 / prior_l) (prior_l <= 0
+ * adds nothing) and needs prior (lmax + 1 host doubles). */
+enum { PFSHT_SOLVE_CG = 0, PFSHT_SOLVE_WIENER = 1, PFSHT_SOLVE_MINNORM = 2 };
+typedef struct {
+    double shannon;  /* (lmax + 1)^2 fsky of the points' bounding box */
+    double dl;       /* the box's spectral resolution */
+    double relres;   /* final |r| / |A' W f| */
+    int32_t iters, mode;
+} pfsht_solve_info;
+int pfsht_solve(pfsht_plan_t plan, const void *f, const double *w, int32_t mode, double rtol, int32_t maxiter,
+                const double *prior, void *alm, int32_t where, void *stream, pfsht_solve_info *info);
+
+/* ---------------------------------------------------------------------------------------------
  * Serving runtime. No counterpart in the reference, whose plans are built and applied by their
  * caller (src/Plan.jl:84-148); it is for services answering requests on recurring point sets.
  * A server keeps plans resident in HBM under caller-chosen 64-bit keys, builds a plan on a miss and

@@ -439,3 +439,107 @@ def valid_mask(L: int) -> np.ndarray:
     rows = np.arange(L + 1)[:, None]
     cols = np.arange(2 * L + 1)[None, :]
     return rows + (cols + 1) // 2 <= L
+
+
+# ------------------------------------------------------------------------------------------------
+# Bounding box and analysis (src/Box.jl, src/Analysis.jl), restated in numpy over an oracle Plan.
+# ------------------------------------------------------------------------------------------------
+def bounding_box(theta, phi) -> dict:
+    """bounding_box (Box.jl:43-73) with the circular extent found by sorting (the reference's
+    bucket scan, Box.jl:246-296, returns the same largest gap), box_fsky (:106-108) and
+    spectral_resolution (:148-153)."""
+    th = np.asarray(theta, np.float64)
+    ph = np.asarray(phi, np.float64)
+    thlo = max(float(th.min()), 0.0)
+    thhi = min(float(th.max()), np.pi)
+    ptol = np.sqrt(np.finfo(np.float64).eps)
+    north, south = thlo <= ptol, thhi >= np.pi - ptol
+    if north or south:
+        start, ext = 0.0, 2 * np.pi
+    else:
+        ps = np.mod(ph, 2 * np.pi)
+        if len(ps) == 1:
+            start, ext = float(ps[0]), 0.0
+        else:
+            s = np.sort(ps)
+            d = np.diff(s)
+            k = int(np.argmax(d)) if len(d) else 0
+            gap, at = (float(d[k]), float(s[k + 1])) if len(d) and d[k] > 0 else (0.0, float(s[-1]))
+            wrap = 2 * np.pi - s[-1] + s[0]
+            start = at
+            if wrap >= gap:
+                gap, start = wrap, float(s[0])
+            ext = 2 * np.pi - gap
+            if ext <= 0:
+                start, ext = float(s[0]), 0.0
+            elif ext >= 2 * np.pi:
+                start, ext = 0.0, 2 * np.pi
+    fsky = (np.cos(thlo) - np.cos(thhi)) * ext / (4 * np.pi)
+    e = np.sqrt(np.finfo(np.float64).eps)
+    mid = min(max(0.5 * (thlo + thhi), e), np.pi - e)
+    lam = max(thhi - thlo, ext * np.sin(mid))
+    dl = np.inf if lam <= 0 else np.pi / lam
+    return dict(thlo=thlo, thhi=thhi, phistart=start, phiext=ext, touches_north=north, touches_south=south,
+                fsky=fsky, dl=dl)
+
+
+def _weight(f, w):
+    return f if w is None else (f * (w if f.ndim == 1 else w[:, None]))
+
+
+def apply_resolution(plan: "Plan", alm, w=None) -> np.ndarray:
+    """apply_resolution! (Analysis.jl:96-104): A' (w .* (A alm))."""
+    return plan.adjoint(np.asfortranarray(_weight(plan.synth(alm), w)))
+
+
+def pseudo_spectrum(plan: "Plan", f, w=None, batch: int = 1) -> np.ndarray:
+    """pseudo_spectrum! (Analysis.jl:286-308)."""
+    a = plan.adjoint(np.asfortranarray(_weight(np.asarray(f), w)))
+    if a.ndim == 3:
+        a = a[:, :, batch - 1]
+    L = plan.L
+    C = np.zeros(L + 1)
+    for l in range(L + 1):
+        acc = abs(a[l, 0]) ** 2
+        for m in range(1, l + 1):
+            acc += abs(a[l - m, 2 * m]) ** 2 + abs(a[l - m, 2 * m - 1]) ** 2
+        C[l] = acc / (2 * l + 1)
+    return C
+
+
+def solve(plan: "Plan", f, mode="cg", rtol=1e-6, maxiter=500, prior=None, w=None):
+    """solve! (Analysis.jl:375-436): CG on the normal equations, optional Wiener prior."""
+    L = plan.L
+    fw = np.asfortranarray(_weight(np.asarray(f), w))
+    rhs = plan.adjoint(fw)
+    reg = None
+    if mode == "wiener":
+        reg = np.array([1.0 / c if c > 0 else 0.0 for c in prior[:L + 1]])
+    tol = min(rtol, 1e-12) if mode == "minnorm" else rtol
+    mask = valid_mask(L)
+    lgrid = np.arange(L + 1)[:, None] + (np.arange(2 * L + 1)[None, :] + 1) // 2
+    x = np.zeros_like(rhs)
+    r = rhs.copy()
+    p = r.copy()
+    rsold = np.vdot(r, r)
+    rhsn = np.sqrt(abs(np.vdot(rhs, rhs)))
+    if rhsn == 0:
+        return x, dict(iters=0, relres=0.0)
+    iters, rel = 0, 1.0
+    for i in range(1, maxiter + 1):
+        iters = i
+        Ap = plan.adjoint(np.asfortranarray(_weight(plan.synth(p), w)))
+        if reg is not None:
+            s = np.where(mask, reg[np.minimum(lgrid, L)], 0.0)
+            Ap = Ap + (s[..., None] * p if p.ndim == 3 else s * p)
+        pAp = np.vdot(p, Ap)
+        alpha = 0.0 if pAp == 0 else rsold / pAp
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rsnew = np.vdot(r, r)
+        rel = np.sqrt(abs(rsnew)) / rhsn
+        if rel <= tol:
+            break
+        p = r + (rsnew / rsold) * p
+        rsold = rsnew
+    return np.asfortranarray(x), dict(iters=iters, relres=float(rel))

@@ -7,6 +7,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <complex>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -341,6 +342,15 @@ struct pfsht_plan_s {
     hipEvent_t done = nullptr;
     hipStream_t last_stream = nullptr;
     bool has_done = false;
+    // analysis workspace (AnalysisWorkspace, src/Analysis.jl:45-84), allocated at the first analysis
+    // call: rhs, r, p, Ap and a staging alm in the alm layout, two fields, weights, the Wiener
+    // prior, the spectrum, dot-product partials and a pinned host slot for scalars
+    double2 *aw_rhs = nullptr, *aw_r = nullptr, *aw_p = nullptr, *aw_Ap = nullptr, *aw_x = nullptr;
+    double2 *aw_f = nullptr, *aw_f2 = nullptr, *aw_part = nullptr, *aw_dot = nullptr, *aw_hdot = nullptr;
+    double *aw_w = nullptr, *aw_reg = nullptr, *aw_C = nullptr;
+    double th_min = 0, th_max = 0;  // colatitude range of the points
+    bool box_ready = false;
+    Box box;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> pool;
@@ -355,6 +365,9 @@ struct pfsht_plan_s {
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
         dfree(Ft); dfree(U); dfree(Ut); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
         dfree(dphi); dfree(dperm); dfree(dmlim);
+        dfree(aw_rhs); dfree(aw_r); dfree(aw_p); dfree(aw_Ap); dfree(aw_x); dfree(aw_f); dfree(aw_f2);
+        dfree(aw_part); dfree(aw_dot); dfree(aw_w); dfree(aw_reg); dfree(aw_C);
+        if (aw_hdot) (void)hipHostFree(aw_hdot);
         fft_free(th_fwd); fft_free(th_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd); fft_free(col_bwd); fft_free(col_fwd);
         fft_free(row_bwd); fft_free(row_fwd);
@@ -389,6 +402,8 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     // bounding box in colatitude (src/Box.jl:43-73)
     double thlo = INFINITY, thhi = -INFINITY;
     for (int64_t i = 0; i < M; ++i) { thlo = std::min(thlo, theta[i]); thhi = std::max(thhi, theta[i]); }
+    p.th_min = thlo;
+    p.th_max = thhi;
     thlo = std::max(thlo, 0.0);
     thhi = std::min(thhi, kPi);
     if (method == PFSHT_METHOD_AUTO) {
@@ -742,9 +757,236 @@ static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
 }
 
 // ------------------------------------------------------------------------------------------------
+// Analysis on a plan (src/Analysis.jl)
+// ------------------------------------------------------------------------------------------------
+static size_t alm_count(const pfsht_plan_s &p) { return size_t(p.L + 1) * (2 * p.L + 1) * p.B; }
+
+static Status aw_alloc(pfsht_plan_s &p) {
+    if (p.aw_rhs) return kOk;
+    const size_t na = alm_count(p), nf = size_t(p.M) * p.B;
+    int64_t &by = p.bytes;
+    Status st;
+    for (double2 **q : {&p.aw_rhs, &p.aw_r, &p.aw_p, &p.aw_Ap, &p.aw_x})
+        if ((st = dalloc(q, na, by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_f, nf, by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_f2, nf, by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_part, size_t(kDotBlocks), by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_dot, 1, by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_w, size_t(p.M), by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_reg, size_t(p.L + 1), by)) != kOk) return st;
+    if ((st = dalloc(&p.aw_C, size_t(p.L + 1), by)) != kOk) return st;
+    PFSHT_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&p.aw_hdot), sizeof(double2)));
+    return kOk;
+}
+
+// The points' bounding box, computed once: longitudes read back from the plan's device copy
+// (the Direct leaf's sorted phi, or the stretched leaf's fine-grid x2 scaled back to radians).
+static Status aw_box(pfsht_plan_s &p) {
+    if (p.box_ready) return kOk;
+    std::vector<double> ph(p.M);
+    if (p.method == PFSHT_METHOD_DIRECT) {
+        PFSHT_HIP_TRY(hipMemcpy(ph.data(), p.dphi, sizeof(double) * p.M, hipMemcpyDeviceToHost));
+    } else {
+        PFSHT_HIP_TRY(hipMemcpy(ph.data(), p.nu.X2, sizeof(double) * p.M, hipMemcpyDeviceToHost));
+        const double sc = 2 * kPi / double(p.nu.N2);
+        for (auto &v : ph) v *= sc;
+    }
+    p.box = bounding_box_range(p.th_min, p.th_max, ph.data(), p.M);
+    p.box_ready = true;
+    return kOk;
+}
+
+// LinearAlgebra.dot(a, b) = sum conj(a) b over n complex entries, read back to the host
+static Status cdot(pfsht_plan_s &p, const double2 *a, const double2 *b, size_t n, hipStream_t s,
+                   std::complex<double> &out) {
+    launch_cdot(a, b, int64_t(n), p.aw_part, p.aw_dot, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    PFSHT_HIP_TRY(hipMemcpyAsync(p.aw_hdot, p.aw_dot, sizeof(double2), hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    out = std::complex<double>(p.aw_hdot->x, p.aw_hdot->y);
+    return kOk;
+}
+
+// The caller's weights as a device pointer (staged for host calls), or nullptr for none.
+static Status weights_dev(pfsht_plan_s &p, const double *w, int32_t where, hipStream_t s, const double *&wd) {
+    wd = nullptr;
+    if (!w) return kOk;
+    if (where == PFSHT_WHERE_DEVICE) { wd = w; return kOk; }
+    PFSHT_HIP_TRY(hipMemcpyAsync(p.aw_w, w, sizeof(double) * p.M, hipMemcpyHostToDevice, s));
+    wd = p.aw_w;
+    return kOk;
+}
+
+// A field argument in a workspace buffer: a copy of the caller's f with the weights applied.
+static Status field_in(pfsht_plan_s &p, const void *f, const double *wd, int32_t where, double2 *dst, hipStream_t s) {
+    const size_t bytes = sizeof(double2) * size_t(p.M) * p.B;
+    PFSHT_HIP_TRY(hipMemcpyAsync(dst, f, bytes, where == PFSHT_WHERE_DEVICE ? hipMemcpyDeviceToDevice
+                                                                              : hipMemcpyHostToDevice, s));
+    if (wd) launch_weight_field(dst, wd, p.M, p.B, s);
+    return kOk;
+}
+
+static Status analysis_check(pfsht_plan_s *p, int32_t where) {
+    if (!p) { set_error("plan is NULL"); return kEArg; }
+    if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
+    return kOk;
+}
+
+// ------------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------------
 extern "C" {
+
+int pfsht_bounding_box(const double *theta, const double *phi, int64_t npts, pfsht_box *out) {
+    if (!theta || !phi || !out) { set_error("NULL argument"); return kEArg; }
+    if (npts < 1) { set_error("need at least one point"); return kEArg; }
+    const Box b = bounding_box(theta, phi, npts);
+    out->thlo = b.thlo;
+    out->thhi = b.thhi;
+    out->phistart = b.phistart;
+    out->phiext = b.phiext;
+    out->touches_north = b.north;
+    out->touches_south = b.south;
+    out->fsky = box_fsky(b);
+    out->dl = spectral_resolution(b);
+    return kOk;
+}
+
+int pfsht_apply_resolution(pfsht_plan_t p, const void *alm, const double *w, void *out, int32_t where,
+                           void *stream) {
+    Status st = analysis_check(p, where);
+    if (st != kOk) return st;
+    if (!alm || !out) { set_error("NULL argument"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if ((st = aw_alloc(*p)) != kOk) return st;
+    const size_t abytes = sizeof(double2) * alm_count(*p);
+    const double2 *a = static_cast<const double2 *>(alm);
+    double2 *o = static_cast<double2 *>(out);
+    if (where == PFSHT_WHERE_HOST) {
+        PFSHT_HIP_TRY(hipMemcpyAsync(p->aw_x, alm, abytes, hipMemcpyHostToDevice, s));
+        a = p->aw_x;
+        o = p->aw_Ap;
+    }
+    const double *wd;
+    if ((st = weights_dev(*p, w, where, s, wd)) != kOk) return st;
+    // src/Analysis.jl:96-104: f = A alm; f .*= w; out = A' f
+    if ((st = synth_impl(*p, a, p->aw_f, s)) != kOk) return st;
+    if (wd) launch_weight_field(p->aw_f, wd, p->M, p->B, s);
+    if ((st = adjoint_impl(*p, p->aw_f, o, s)) != kOk) return st;
+    if (where == PFSHT_WHERE_HOST) PFSHT_HIP_TRY(hipMemcpyAsync(out, o, abytes, hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int pfsht_pseudo_spectrum(pfsht_plan_t p, const void *f, const double *w, int32_t batch, double *C, int32_t where,
+                          void *stream) {
+    Status st = analysis_check(p, where);
+    if (st != kOk) return st;
+    if (!f || !C) { set_error("NULL argument"); return kEArg; }
+    if (batch < 1 || batch > p->B) {
+        set_error("batch = " + std::to_string(batch) + " outside 1:" + std::to_string(p->B));
+        return kEArg;
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if ((st = aw_alloc(*p)) != kOk) return st;
+    const double *wd;
+    if ((st = weights_dev(*p, w, where, s, wd)) != kOk) return st;
+    // src/Analysis.jl:286-308: a~ = A' (w .* f), then C_l over the orders of transform `batch`
+    if ((st = field_in(*p, f, wd, where, p->aw_f, s)) != kOk) return st;
+    if ((st = adjoint_impl(*p, p->aw_f, p->aw_x, s)) != kOk) return st;
+    double *Cd = where == PFSHT_WHERE_DEVICE ? C : p->aw_C;
+    launch_spectrum(p->aw_x + size_t(batch - 1) * (size_t(p->L + 1) * (2 * p->L + 1)), p->L, Cd, s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    if (where == PFSHT_WHERE_HOST)
+        PFSHT_HIP_TRY(hipMemcpyAsync(C, Cd, sizeof(double) * (p->L + 1), hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    return kOk;
+}
+
+int pfsht_solve(pfsht_plan_t p, const void *f, const double *w, int32_t mode, double rtol, int32_t maxiter,
+                const double *prior, void *alm, int32_t where, void *stream, pfsht_solve_info *info) {
+    Status st = analysis_check(p, where);
+    if (st != kOk) return st;
+    if (!f || !alm) { set_error("NULL argument"); return kEArg; }
+    if (mode != PFSHT_SOLVE_CG && mode != PFSHT_SOLVE_WIENER && mode != PFSHT_SOLVE_MINNORM) {
+        set_error("mode must be CG, WIENER or MINNORM");
+        return kEArg;
+    }
+    if (mode == PFSHT_SOLVE_WIENER && !prior) {
+        set_error("WIENER needs a prior C_l: that prior is what makes the problem well posed, so it cannot be "
+                  "defaulted");
+        return kEArg;
+    }
+    if (maxiter < 0) { set_error("maxiter must be nonnegative"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if ((st = aw_alloc(*p)) != kOk) return st;
+    if ((st = aw_box(*p)) != kOk) return st;
+    const size_t n = alm_count(*p), abytes = sizeof(double2) * n;
+    double2 *x = where == PFSHT_WHERE_DEVICE ? static_cast<double2 *>(alm) : p->aw_x;
+    const double *wd;
+    if ((st = weights_dev(*p, w, where, s, wd)) != kOk) return st;
+    // src/Analysis.jl:391-409: rhs = A' (w .* f); reg = 1 / prior (0 where prior <= 0)
+    if ((st = field_in(*p, f, wd, where, p->aw_f2, s)) != kOk) return st;
+    if ((st = adjoint_impl(*p, p->aw_f2, p->aw_rhs, s)) != kOk) return st;
+    const bool wiener = mode == PFSHT_SOLVE_WIENER;
+    if (wiener) {
+        std::vector<double> reg(p->L + 1);
+        for (int l = 0; l <= p->L; ++l) reg[l] = prior[l] > 0 ? 1.0 / prior[l] : 0.0;
+        PFSHT_HIP_TRY(hipMemcpyAsync(p->aw_reg, reg.data(), sizeof(double) * reg.size(), hipMemcpyHostToDevice, s));
+        PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    const double tol = mode == PFSHT_SOLVE_MINNORM ? std::min(rtol, 1e-12) : rtol;
+    // src/Analysis.jl:401-436: CG from alm = 0
+    PFSHT_HIP_TRY(hipMemsetAsync(x, 0, abytes, s));
+    PFSHT_HIP_TRY(hipMemcpyAsync(p->aw_r, p->aw_rhs, abytes, hipMemcpyDeviceToDevice, s));
+    PFSHT_HIP_TRY(hipMemcpyAsync(p->aw_p, p->aw_r, abytes, hipMemcpyDeviceToDevice, s));
+    std::complex<double> rsold, rr;
+    if ((st = cdot(*p, p->aw_r, p->aw_r, n, s, rsold)) != kOk) return st;
+    if ((st = cdot(*p, p->aw_rhs, p->aw_rhs, n, s, rr)) != kOk) return st;
+    const double rhsn = std::sqrt(std::abs(rr));
+    pfsht_solve_info inf{};
+    inf.shannon = double(p->L + 1) * double(p->L + 1) * box_fsky(p->box);
+    inf.dl = spectral_resolution(p->box);
+    inf.mode = mode;
+    int iters = 0;
+    double rel = 1.0;
+    if (rhsn == 0) {
+        rel = 0.0;
+    } else {
+        for (int i = 1; i <= maxiter; ++i) {
+            iters = i;
+            if ((st = synth_impl(*p, p->aw_p, p->aw_f, s)) != kOk) return st;
+            if (wd) launch_weight_field(p->aw_f, wd, p->M, p->B, s);
+            if ((st = adjoint_impl(*p, p->aw_f, p->aw_Ap, s)) != kOk) return st;
+            if (wiener) launch_add_prior(p->aw_Ap, p->aw_p, p->aw_reg, p->L, p->B, s);
+            std::complex<double> pAp, rsnew;
+            if ((st = cdot(*p, p->aw_p, p->aw_Ap, n, s, pAp)) != kOk) return st;
+            const std::complex<double> alpha = pAp == 0.0 ? std::complex<double>(0.0) : rsold / pAp;
+            launch_zaxpy(x, p->aw_p, make_double2(alpha.real(), alpha.imag()), int64_t(n), s);
+            launch_zaxpy(p->aw_r, p->aw_Ap, make_double2(-alpha.real(), -alpha.imag()), int64_t(n), s);
+            if ((st = cdot(*p, p->aw_r, p->aw_r, n, s, rsnew)) != kOk) return st;
+            rel = std::sqrt(std::abs(rsnew)) / rhsn;
+            if (rel <= tol) break;
+            const std::complex<double> beta = rsnew / rsold;
+            launch_zxpby(p->aw_p, p->aw_r, make_double2(beta.real(), beta.imag()), int64_t(n), s);
+            rsold = rsnew;
+        }
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
+    if (where == PFSHT_WHERE_HOST) PFSHT_HIP_TRY(hipMemcpyAsync(alm, x, abytes, hipMemcpyDeviceToHost, s));
+    PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    inf.iters = iters;
+    inf.relres = rel;
+    if (info) *info = inf;
+    return kOk;
+}
+
 
 const char *pfsht_last_error(void) { return g_error.c_str(); }
 const char *pfsht_version(void) { return "pfsht-hip 0.1.0 (gfx950)"; }

@@ -202,6 +202,65 @@ double jmod(double x, double y) {
     return r;
 }
 
+Box bounding_box(const double *theta, const double *phi, int64_t n) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) { lo = std::min(lo, theta[i]); hi = std::max(hi, theta[i]); }
+    return bounding_box_range(lo, hi, phi, n);
+}
+
+Box bounding_box_range(double lo, double hi, const double *phi, int64_t n) {
+    Box b;
+    b.thlo = std::max(lo, 0.0);
+    b.thhi = std::min(hi, kPi);
+    const double ptol = std::sqrt(2.220446049250313e-16);
+    b.north = b.thlo <= ptol;
+    b.south = b.thhi >= kPi - ptol;
+    if (b.north || b.south) { b.phistart = 0.0; b.phiext = 2 * kPi; return b; }
+    std::vector<double> ps(n);
+    for (int64_t i = 0; i < n; ++i) ps[i] = jmod(phi[i], 2 * kPi);
+    if (n == 1) { b.phistart = ps[0]; b.phiext = 0.0; return b; }
+    // largest gap between consecutive values: n - 1 equal buckets hold it between two of them
+    double plo = ps[0], phi_ = ps[0];
+    for (int64_t i = 1; i < n; ++i) { plo = std::min(plo, ps[i]); phi_ = std::max(phi_, ps[i]); }
+    double gap = 0.0, at = plo;
+    if (phi_ > plo) {
+        const int64_t nb = n - 1;
+        const double w = (phi_ - plo) / double(nb);
+        std::vector<std::pair<double, double>> bk(nb, {INFINITY, -INFINITY});  // (min, max) per bucket
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t k = int64_t(std::floor((ps[i] - plo) / w));
+            k = std::min(std::max(k, int64_t(0)), nb - 1);
+            bk[k].first = std::min(bk[k].first, ps[i]);
+            bk[k].second = std::max(bk[k].second, ps[i]);
+        }
+        at = phi_;
+        double prev = plo;
+        for (int64_t k = 0; k < nb; ++k) {
+            if (!(bk[k].first <= bk[k].second)) continue;  // empty bucket
+            const double d = bk[k].first - prev;
+            if (d > gap) { gap = d; at = bk[k].first; }
+            prev = bk[k].second;
+        }
+    }
+    const double wrap = 2 * kPi - phi_ + plo;
+    double start = at;
+    if (wrap >= gap) { gap = wrap; start = plo; }
+    const double ext = 2 * kPi - gap;
+    if (ext <= 0) { b.phistart = plo; b.phiext = 0.0; }
+    else if (ext >= 2 * kPi) { b.phistart = 0.0; b.phiext = 2 * kPi; }
+    else { b.phistart = start; b.phiext = ext; }
+    return b;
+}
+
+double box_fsky(const Box &b) { return (std::cos(b.thlo) - std::cos(b.thhi)) * b.phiext / (4 * kPi); }
+
+double spectral_resolution(const Box &b) {
+    const double e = std::sqrt(2.220446049250313e-16);
+    const double mid = std::min(std::max(0.5 * (b.thlo + b.thhi), e), kPi - e);
+    const double lam = std::max(b.thhi - b.thlo, b.phiext * std::sin(mid));
+    return lam <= 0 ? INFINITY : kPi / lam;
+}
+
 static double fftunits(int64_t n) { return n <= 1 ? 0.0 : double(n) * std::log2(double(n)); }
 
 double cost_direct(int64_t npts, int L, int B) {

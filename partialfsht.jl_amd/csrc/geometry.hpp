@@ -51,6 +51,20 @@ inline int mode_k(int i, int n) { return i < (n + 1) / 2 ? i : i - n; }
 // Julia's floating mod (result carries the divisor's sign).
 double jmod(double x, double y);
 
+// bounding_box (src/Box.jl:43-73): colatitude range clamped to [0, pi], pole-touch flags, and the
+// smallest longitude arc holding the points (src/Box.jl:80-102; 0 .. 2 pi when a pole is touched),
+// by the O(n) maximum-gap bucket scan (src/Box.jl:246-296).
+struct Box {
+    double thlo = 0, thhi = 0, phistart = 0, phiext = 0;
+    bool north = false, south = false;
+};
+Box bounding_box(const double *theta, const double *phi, int64_t n);
+// the same from a known colatitude range (unclamped min and max) and the longitudes
+Box bounding_box_range(double thmin, double thmax, const double *phi, int64_t n);
+// box_fsky (src/Box.jl:106-108) and spectral_resolution (src/Box.jl:148-153)
+double box_fsky(const Box &b);
+double spectral_resolution(const Box &b);
+
 // Cost model restricted to Direct vs Stretch (src/Cost.jl:104-140,174-201).
 double cost_direct(int64_t npts, int L, int B);
 double cost_stretch(double thlo, double thhi, int64_t npts, int L, int B, double tol);

@@ -174,4 +174,14 @@ void launch_interp(const NufftDev &u, const C *U, double2 *f, hipStream_t s);
 template <class C>
 void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s);
 
+// Iterative callers on a plan (analysis_kernels.hip). launch_cdot writes sum conj(a) b to *out
+// through kDotBlocks partial sums in part, in a fixed order.
+constexpr int kDotBlocks = 1024;
+void launch_weight_field(double2 *f, const double *w, int64_t M, int B, hipStream_t s);
+void launch_cdot(const double2 *a, const double2 *b, int64_t n, double2 *part, double2 *out, hipStream_t s);
+void launch_zaxpy(double2 *y, const double2 *x, double2 a, int64_t n, hipStream_t s);
+void launch_zxpby(double2 *p, const double2 *r, double2 beta, int64_t n, hipStream_t s);
+void launch_add_prior(double2 *Ap, const double2 *p, const double *reg, int L, int B, hipStream_t s);
+void launch_spectrum(const double2 *alm, int L, double *C, hipStream_t s);
+
 }  // namespace pfsht

@@ -52,6 +52,19 @@ class InfoC(C.Structure):
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double)]
 
 
+class BoxC(C.Structure):
+    _fields_ = [("thlo", C.c_double), ("thhi", C.c_double), ("phistart", C.c_double), ("phiext", C.c_double),
+                ("touches_north", C.c_int32), ("touches_south", C.c_int32), ("fsky", C.c_double),
+                ("dl", C.c_double)]
+
+
+class SolveInfoC(C.Structure):
+    _fields_ = [("shannon", C.c_double), ("dl", C.c_double), ("relres", C.c_double), ("iters", C.c_int32),
+                ("mode", C.c_int32)]
+
+
+SOLVE_MODES = {"cg": 0, "wiener": 1, "minnorm": 2}
+
 NSTAGES = 9
 STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft", "ring_reduce"]
 METHODS = {"auto": 0, "direct": 1, "stretch": 2}
@@ -78,6 +91,12 @@ EXPORTS = {
     "pfsht_plan_nufft": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "pfsht_stage_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pfsht_bounding_box": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(BoxC)]),
+    "pfsht_apply_resolution": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_pseudo_spectrum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
+                                        C.c_void_p]),
+    "pfsht_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_void_p,
+                              C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(SolveInfoC)]),
     "pfsht_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
     "pfsht_server_destroy": (None, [C.c_void_p]),
     "pfsht_server_plan": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
@@ -208,6 +227,38 @@ def _stream_ptr(stream):
     return C.c_void_p(int(stream))
 
 
+def bounding_box(theta, phi) -> dict:
+    """bounding_box(θ, φ) (src/Box.jl:43-73) with box_fsky and spectral_resolution: host arrays."""
+    th = np.ascontiguousarray(theta, np.float64)
+    ph = np.ascontiguousarray(phi, np.float64)
+    if th.shape != ph.shape:
+        raise DimensionMismatch("theta and phi must have the same length")
+    out = BoxC()
+    _check(lib().pfsht_bounding_box(th.ctypes.data, ph.ctypes.data, len(th), C.byref(out)))
+    d = {k: getattr(out, k) for k, _ in BoxC._fields_}
+    d["touches_north"], d["touches_south"] = bool(d["touches_north"]), bool(d["touches_south"])
+    return d
+
+
+def _weights(w, M: int, where: int):
+    """(pointer, keep-alive) of npts float64 weights on the same side as the other arrays."""
+    if w is None:
+        return C.c_void_p(0), None
+    if _is_torch(w):
+        import torch
+        if w.dtype != torch.float64 or not w.is_contiguous() or w.numel() != M:
+            raise ArgumentError("w must be a contiguous float64 tensor of npts entries")
+        if (WHERE_DEVICE if w.is_cuda else WHERE_HOST) != where:
+            raise ArgumentError("w must live where the other arrays do")
+        return C.c_void_p(w.data_ptr()), w
+    if where != WHERE_HOST:
+        raise ArgumentError("w must live where the other arrays do")
+    wa = np.ascontiguousarray(w, np.float64)
+    if wa.size != M:
+        raise DimensionMismatch(f"weights have {wa.size} entries, field has {M}")
+    return C.c_void_p(wa.ctypes.data), wa
+
+
 class Plan:
     """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116."""
 
@@ -320,6 +371,71 @@ class Plan:
             raise ArgumentError("in and out must both be host or both be device arrays")
         _check(lib().pfsht_plan_nufft(self._h, int(typ), pi, po, wi, _stream_ptr(stream) if wi else C.c_void_p(0)))
         return out
+
+    # -- analysis (src/Analysis.jl) ---------------------------------------------------------------
+    def _new_alm(self, like):
+        L, B = self.lmax, self.ntrans
+        if _is_torch(like):
+            import torch
+            return torch.zeros((L + 1) * (2 * L + 1) * B, dtype=torch.complex128, device=like.device)
+        return np.zeros((L + 1, 2 * L + 1) if B == 1 else (L + 1, 2 * L + 1, B), np.complex128, order="F")
+
+    def apply_resolution(self, alm, w=None, out=None, stream=None):
+        """apply_resolution!(out, plan, alm, w) (src/Analysis.jl:96-104): out = A' (w .* (A alm))."""
+        L, B = self.lmax, self.ntrans
+        _assert_coeffs(alm, L, B)
+        out = self._new_alm(alm) if out is None else out
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        po, wo = _buf(out, (L + 1) * (2 * L + 1) * B, "out")
+        if wa != wo:
+            raise ArgumentError("alm and out must both be host or both be device arrays")
+        pw, keep = _weights(w, self.npts, wa)
+        _check(lib().pfsht_apply_resolution(self._h, pa, pw, po, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return out
+
+    def pseudo_spectrum(self, f, w=None, batch: int = 1, stream=None):
+        """pseudo_spectrum!(C, plan, f, w; batch) (src/Analysis.jl:286-308): C_l of A' (w .* f)."""
+        M, B, L = self.npts, self.ntrans, self.lmax
+        _assert_field(f, M, B)
+        pf, wf = _buf(f, M * B, "f")
+        pw, keep = _weights(w, M, wf)
+        if wf == WHERE_DEVICE:
+            import torch
+            Cl = torch.zeros(L + 1, dtype=torch.float64, device=f.device)
+            pc = C.c_void_p(Cl.data_ptr())
+        else:
+            Cl = np.zeros(L + 1)
+            pc = C.c_void_p(Cl.ctypes.data)
+        _check(lib().pfsht_pseudo_spectrum(self._h, pf, pw, int(batch), pc, wf,
+                                           _stream_ptr(stream) if wf else C.c_void_p(0)))
+        return Cl
+
+    def solve(self, f, mode: str = "cg", rtol: float = 1e-6, maxiter: int = 500, prior=None, w=None,
+              alm=None, stream=None):
+        """solve!(alm, plan, f; mode, rtol, maxiter, prior, w) (src/Analysis.jl:375-436).
+        Returns (alm, info) with info = {shannon, dl, iters, relres, mode}."""
+        if mode not in SOLVE_MODES:
+            raise ArgumentError(f"mode must be cg, wiener or minnorm, got {mode}")
+        M, B, L = self.npts, self.ntrans, self.lmax
+        _assert_field(f, M, B)
+        pf, wf = _buf(f, M * B, "f")
+        alm = self._new_alm(f) if alm is None else alm
+        _assert_coeffs(alm, L, B)
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        if wa != wf:
+            raise ArgumentError("f and alm must both be host or both be device arrays")
+        pw, keep = _weights(w, M, wf)
+        pp = C.c_void_p(0)
+        if prior is not None:
+            pr = np.ascontiguousarray(prior, np.float64)
+            if pr.size < L + 1:
+                raise DimensionMismatch(f"prior must cover l = 0:{L}, got {pr.size} entries")
+            pp = C.c_void_p(pr.ctypes.data)
+        inf = SolveInfoC()
+        _check(lib().pfsht_solve(self._h, pf, pw, SOLVE_MODES[mode], float(rtol), int(maxiter), pp, pa, wf,
+                                 _stream_ptr(stream) if wf else C.c_void_p(0), C.byref(inf)))
+        info = {"shannon": inf.shannon, "dl": inf.dl, "iters": int(inf.iters), "relres": inf.relres, "mode": mode}
+        return alm, info
 
     def rings(self) -> "Rings":
         h = C.c_void_p()
