@@ -603,13 +603,53 @@ __device__ __forceinline__ double row_sum4(double v) {
     return __hiloint2double(int(b[0]), int(a[0])) + __hiloint2double(int(b[1]), int(a[1]));
 }
 
+// Phase-1 step K of the rescaled recurrence (as in the forward): t = x r1 - gamma_K r2, with gamma_K
+// broadcast from lane K of the row.
+template <int K>
+__device__ __forceinline__ double dpp_rstep(double x, double r1, double r2, double ng) {
+    double t;
+    asm("s_nop 1\n\t"
+        "v_mul_f64 %0, %1, %2\n\t"
+        "v_fmac_f64_dpp %0, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+        : "=&v"(t)
+        : "v"(x), "v"(r1), "v"(ng), "v"(r2), "n"(K));
+    return t;
+}
+
+// Steps K .. 15 of a sub-tile, each storing r (SCALED: r 2^E) to row K of the LDS tile; GUARD:
+// only kfirst <= K < kend.
+template <int K, bool GUARD, bool SCALED>
+struct AdjPhase1 {
+    static __device__ __forceinline__ void run(double x, double &r1, double &r2, double ng, int E, double (*W)[kWS],
+                                               int wcol, int kfirst, int kend) {
+        if (!GUARD || (K >= kfirst && K < kend)) {
+            const double t = dpp_rstep<K>(x, r1, r2, ng);
+            r2 = r1;
+            r1 = t;
+            W[K][wcol] = SCALED ? ldexp(t, E) : t;
+        }
+        AdjPhase1<K + 1, GUARD, SCALED>::run(x, r1, r2, ng, E, W, wcol, kfirst, kend);
+    }
+};
+template <bool GUARD, bool SCALED>
+struct AdjPhase1<kSub, GUARD, SCALED> {
+    static __device__ __forceinline__ void run(double, double &, double &, double, int, double (*)[kWS], int, int,
+                                               int) {}
+};
+
 // Degrees ls .. le of order m. Segment 0 (ls = m) starts from the seed qs; a later segment from
-// its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}). ls - m is a multiple of 64.
+// its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}), aprev = a_{ls-1}. ls - m is a multiple of 64.
+// Phase 1 runs the rescaled recurrence of the forward sweep over each 16-degree sub-tile (gamma_l
+// and C_l of lane k of a row give degree l0 + k), and the ring sums of degree l are multiplied by
+// C_l once, after the reduction. For one or two transforms the ring's exponent goes into its G
+// operands (G 2^E, exact, refreshed when E changes at a block end) and the tile holds r; with more
+// transforms (registers) the tile holds r 2^E.
 template <int NB, bool NEG>
-__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ gpc,
+__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const double2 *__restrict__ gpc,
                                           const double2 *__restrict__ gmc, int64_t gstride, int nr, double x,
-                                          double qs, double q1, double q2, int E, int ls, int le, int m, int L,
-                                          int B, double (*W)[kWS], double4 *__restrict__ out, int &sink) {
+                                          double qs, double q1, double q2, double aprev, int E, int ls, int le,
+                                          int m, int L, int B, double (*W)[kWS], double4 *__restrict__ out,
+                                          int &sink) {
     const int lane = threadIdx.x & 63;
     const bool active = lane < nr;
     // this lane's ring (phase-1 layout) for every transform: (re, im) of orders +m and -m
@@ -626,44 +666,70 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
     const int t = lane & 15, row = lane >> 4;
     const int wcol = lane + (lane >> 4);
     const double *__restrict__ wr = &W[t][17 * row];
+    constexpr bool kScaleG = NB <= 2;
+    constexpr int NG = kScaleG ? NB : 1;
+    double gs[NG][4];  // G 2^E
+    auto scale_g = [&]() {
+        if (kScaleG)
+#pragma unroll
+            for (int u = 0; u < NG; ++u)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) gs[u][c] = ldexp(g[u][c], E);
+    };
+    scale_g();
 
-    // (a, b) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
+    // (gamma, C) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
     // i < 16 owns the line at byte offset 128 i of a 128-degree window
     const int pvoff = 128 * (lane & 15);
     const int lp0 = ls == m ? m + 2 : ls;
-    int pending = l2_touch(reinterpret_cast<const char *>(abm + lp0), pvoff, lane < 16 && lp0 + 8 * lane <= le);
+    int pending = l2_touch(reinterpret_cast<const char *>(gcm + lp0), pvoff, lane < 16 && lp0 + 8 * lane <= le);
+    // (gamma, C) of degree l0 + t, loaded at a clamped index one sub-tile ahead; orders m >= L - 1
+    // have no table (and no recurrence step)
+    const bool has_tab = m + 2 <= L;
+    auto coef = [&](int l0) { return has_tab ? gcm[min(max(l0 + t, m + 2), L)] : make_double2(0.0, 1.0); };
+    double2 gcv = coef(ls);
+    double r1, r2;
+    int kfirst = 0;
+    if (ls == m) {  // seed degrees m and m + 1, stored as q (q 2^E)
+        W[0][wcol] = kScaleG ? qs : ldexp(qs, E);
+        kfirst = 1;
+        r1 = qs;
+        r2 = 0.0;
+        if (m < L) {
+            const double q = first_step(m, x, qs);
+            W[1][wcol] = kScaleG ? q : ldexp(q, E);
+            kfirst = 2;
+            r1 = q;
+            r2 = qs * sqrt(double(2 * m + 3));
+        }
+    } else {
+        r1 = q1;
+        r2 = q2 * aprev;
+    }
     for (int l0 = ls; l0 <= le; l0 += kSub) {
         const int tl = min(kSub, le - l0 + 1);
         if (((l0 - m) & 63) == 0) {
             sink ^= pending;
             const int lp = l0 + 128;
-            pending = l2_touch(reinterpret_cast<const char *>(abm + lp), pvoff, lane < 8 && lp + 8 * lane <= le);
+            pending = l2_touch(reinterpret_cast<const char *>(gcm + lp), pvoff, lane < 8 && lp + 8 * lane <= le);
         }
+        const double2 gcn = coef(l0 + kSub);
+        const double ng = -gcv.x, Cc = gcv.y;
+        const double cmul = t < kfirst ? 1.0 : Cc;  // the seed degrees are stored unscaled
         // phase 1: lanes = rings
-        int tt = 0;
-        if (l0 == m) {
-            q1 = qs;
-            W[0][wcol] = ldexp(qs, E);
-            tt = 1;
-            if (m < L) {
-                const double q = first_step(m, x, qs);
-                q2 = q1;
-                q1 = q;
-                W[1][wcol] = ldexp(q, E);
-                tt = 2;
-            }
-        }
-#pragma unroll 8
-        for (; tt < tl; ++tt) {
-            const double q = rec_step(abm[l0 + tt], x, q1, q2);
-            q2 = q1;
-            q1 = q;
-            W[tt][wcol] = ldexp(q, E);
-        }
+        if (tl == kSub && kfirst == 0) AdjPhase1<0, false, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, 0, kSub);
+        else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, 2, kSub);
+        else AdjPhase1<0, true, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, kfirst, tl);
+        // back to the q scale: C of the sub-tile's last degree (the seed-only sub-tile ends the sweep)
+        const int last = tl - 1;
+        const double Ce = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(Cc), last),
+                                           __builtin_amdgcn_readlane(__double2loint(Cc), last));
+        r1 *= Ce;
+        r2 *= Ce;
         if (((l0 + kSub - m) & 63) == 0) {  // renormalise at the end of every 64-degree block
-            const int s = renorm_shift(q1, q2);
-            q1 = ldexp(q1, -s);
-            q2 = ldexp(q2, -s);
+            const int s = renorm_shift(r1, r2);
+            r1 = ldexp(r1, -s);
+            r2 = ldexp(r2, -s);
             E += s;
         }
         wave_lds_order();
@@ -671,7 +737,9 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
         double a[NB][4];
 #pragma unroll
         for (int u = 0; u < NB; ++u) a[u][0] = a[u][1] = a[u][2] = a[u][3] = 0.0;
-        RowSweep<0, NB, NEG>::run(wr, a, g);
+        if constexpr (kScaleG) RowSweep<0, NB, NEG>::run(wr, a, gs);
+        else RowSweep<0, NB, NEG>::run(wr, a, g);
+        if (((l0 + kSub - m) & 63) == 0) scale_g();  // the next sub-tile's r carry the new exponent
         wave_lds_order();
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -685,8 +753,11 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ abm, const
         if (row == 0 && t < tl) {
 #pragma unroll
             for (int u = 0; u < NB; ++u)
-                out[int64_t(l0 - m + t) * B + u] = make_double4(a[u][0], a[u][1], NEG ? a[u][2] : 0.0, NEG ? a[u][3] : 0.0);
+                out[int64_t(l0 - m + t) * B + u] =
+                    make_double4(a[u][0] * cmul, a[u][1] * cmul, NEG ? a[u][2] * cmul : 0.0, NEG ? a[u][3] * cmul : 0.0);
         }
+        gcv = gcn;
+        kfirst = 0;
     }
     sink ^= pending;
 }
@@ -697,7 +768,8 @@ template <int NB>
 __global__ void __launch_bounds__(64 * kAdjWaves)
 ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
+                const double2 *__restrict__ ckq,
                 const int32_t *__restrict__ ckE, const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
                 double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never) {
     extern __shared__ double lds_adj[];
@@ -738,8 +810,10 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     const int nl = L - m + 1;
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
     int sink = 0;
-    if (m > 0) adj_sweep<NB, true>(abm, gpc, gmc, gstride, nr, x, qs, q1, q2, E, ls, le, m, L, B, W, out, sink);
-    else adj_sweep<NB, false>(abm, gpc, gmc, gstride, nr, x, qs, q1, q2, E, ls, le, m, L, B, W, out, sink);
+    const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
+    const double aprev = sg == 0 ? 0.0 : abm[ls - 1].x;
+    if (m > 0) adj_sweep<NB, true>(gcm, gpc, gmc, gstride, nr, x, qs, q1, q2, aprev, E, ls, le, m, L, B, W, out, sink);
+    else adj_sweep<NB, false>(gcm, gpc, gmc, gstride, nr, x, qs, q1, q2, aprev, E, ls, le, m, L, B, W, out, sink);
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
@@ -785,7 +859,7 @@ void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
             dim3 grid(r.naitems / kAdjWaves);
             const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
 #define PFSHT_ADJ(NB)                                                                                      \
-    ring_adj_kernel<NB><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.abofs, \
+    ring_adj_kernel<NB><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, \
                                                           r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, r.mmax,  \
                                                           r.B, b0, kNeverSink)
             switch (nb) {
