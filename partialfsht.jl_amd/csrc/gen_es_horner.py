@@ -8,17 +8,21 @@ For each kernel width w (2..16) at the shape parameter the library uses for it (
 b = ceil(x - w/2), put u = 2 (b - x) + w - 1 in [-1, 1). Cell t of the point sits at
 z_t = (u + 1 - w + 2 t) / w and the kernel there is phi(z) = exp(beta (sqrt(1 - z^2) - 1)).
 
-  interior cells 1 <= t <= w - 2: phi(z_t) as a degree-14 polynomial in u;
+  interior cells 1 <= t <= w - 2: phi(z_t) as a polynomial in u;
   edge cells t = 0 and t = w - 1:  phi has a square-root branch point at |z| = 1, so the edge
-      polynomial (degree 22) is in e = 2 sqrt((1 + s u) / 2) - 1 (s = +1 for t = 0, -1 for
+      polynomial is in e = 2 sqrt((1 + s u) / 2) - 1 (s = +1 for t = 0, -1 for
       t = w - 1), i.e. v = sqrt(1 - |z|) rescaled to [-1, 1], in which
       phi = exp(beta (v sqrt(2 - v^2) - 1)) is analytic. By phi(-z) = phi(z) one edge table serves
       both ends, and p_{w-1-t}(u) = p_t(-u) for the interior ones.
 
 Each polynomial is the Chebyshev interpolant at Chebyshev points of the first kind, computed in
 60-digit arithmetic and converted to monomial coefficients (highest first for Horner), then rounded
-to double. The tables are compile-time constants of the device code, so the coefficients are
-immediate operands and never loaded.
+to double. The degrees are the smallest (interior at most 14, edge at most 22) whose largest error
+over the cells, sampled at 801 points, is within target(w) = max(2e-16, 1e-5 tol_w) of the kernel
+(which peaks at 1), tol_w = 10^-(w-2) being the tightest tolerance that picks width w: the
+kernel's own truncation error at that tolerance stays far above the polynomial's. The tables
+are compile-time constants of the device code, so the coefficients are immediate operands and
+never loaded.
 """
 import os
 
@@ -26,7 +30,30 @@ import mpmath as mp
 
 mp.mp.dps = 60
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEG_INT, DEG_EDGE = 14, 22
+DEG_INT, DEG_EDGE = 14, 22  # largest degrees
+
+
+def target(w):
+    return max(2e-16, 1e-5 * 10.0 ** (-(w - 2)))
+
+
+def _horner(c, x):
+    acc = 0.0
+    for a in reversed(c):
+        acc = acc * x + a
+    return acc
+
+
+def _maxerr(fn, c):
+    xs = [-1 + 2 * i / 800 for i in range(801)]
+    return max(abs(_horner(c, x) - float(fn(mp.mpf(x)))) for x in xs)
+
+
+def _fit(fns, dmax, tgt):
+    for d in range(4, dmax + 1):
+        cs = [cheb_monomial(fn, d) for fn in fns]
+        if d == dmax or max(_maxerr(fn, c) for fn, c in zip(fns, cs)) <= tgt:
+            return cs
 
 
 def es_beta(w):
@@ -72,35 +99,38 @@ def cheb_monomial(fn, d):
 
 def tables(w):
     beta = mp.mpf(es_beta(w))
-    ints = []
+    fns = []
     for t in range(1, w - 1):
         def fn(u, t=t):
             z = (u + 1 - w + 2 * t) / mp.mpf(w)
             zz = 1 - z * z
             return mp.e ** (beta * (mp.sqrt(zz) - 1)) if zz > 0 else mp.mpf(0)
-        ints.append(cheb_monomial(fn, DEG_INT))
+        fns.append(fn)
     vmax = mp.sqrt(mp.mpf(2) / w)
 
     def edge(e):
         v = (e + 1) / 2 * vmax
         return mp.e ** (beta * (v * mp.sqrt(2 - v * v) - 1))
-    return ints, cheb_monomial(edge, DEG_EDGE)
+    ints = _fit(fns, DEG_INT, target(w)) if fns else []
+    return ints, _fit([edge], DEG_EDGE, target(w))[0]
 
 
 def main():
     out = ["// es_horner_tables.inc — generated by gen_es_horner.py; do not edit.",
            "// EsHorner<w>: the ES kernel of width w as piecewise polynomials (see the generator).",
-           "// c: interior cells t = 1..w-2, 15 monomial coefficients each (constant term first), then",
-           "// the edge polynomial, 23 coefficients.",
-           f"constexpr int kHornInt = {DEG_INT + 1}, kHornEdge = {DEG_EDGE + 1};",
+           "// c: interior cells t = 1..w-2, nint monomial coefficients each (constant term first), then",
+           "// the edge polynomial, nedge coefficients.",
            "template <int W> struct EsHorner;"]
     for w in range(2, 17):
         ints, edge = tables(w)
+        nint = len(ints[0]) if ints else 1
         vals = [x for p in ints for x in p] + edge
         body = ",\n        ".join(", ".join(v.hex() for v in vals[i:i + 3]) for i in range(0, len(vals), 3))
         out.append(f"template <> struct EsHorner<{w}> {{\n"
+                   f"    static constexpr int nint = {nint}, nedge = {len(edge)};\n"
                    f"    static constexpr double beta = {es_beta(w).hex()};\n"
                    f"    static constexpr double c[{len(vals)}] = {{\n        {body}}};\n}};")
+        print(f"w {w}: interior degree {nint - 1}, edge degree {len(edge) - 1}", flush=True)
     with open(os.path.join(HERE, "es_horner_tables.inc"), "w") as fh:
         fh.write("\n".join(out) + "\n")
 

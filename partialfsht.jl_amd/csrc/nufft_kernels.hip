@@ -68,7 +68,7 @@ __device__ __forceinline__ double horner(double x) {
 }
 template <int W, int... T>
 __device__ __forceinline__ void es_interior(double u, double (&k)[W], std::integer_sequence<int, T...>) {
-    ((k[T + 1] = horner<W, T * kHornInt, kHornInt>(u)), ...);
+    ((k[T + 1] = horner<W, T * EsHorner<W>::nint, EsHorner<W>::nint>(u)), ...);
 }
 
 // The W kernel values of a point at x with base cell b (cells b .. b + W - 1): W - 2 interior
@@ -78,8 +78,8 @@ template <int W>
 __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
     const double u = 2.0 * (double(b) - x) + double(W - 1);
     es_interior<W>(u, k, std::make_integer_sequence<int, W - 2>{});
-    k[0] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + u)) - 1.0);
-    k[W - 1] = horner<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 - u)) - 1.0);
+    k[0] = horner<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 + u)) - 1.0);
+    k[W - 1] = horner<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 - u)) - 1.0);
 }
 
 // Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
@@ -87,11 +87,11 @@ __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
 // laundered: this is evaluated inside the spreading chunk loop.
 template <int W, int... T>
 __device__ __forceinline__ void es_half_interior(double v, double *k, std::integer_sequence<int, T...>) {
-    ((k[T + 1] = horner_l<W, T * kHornInt, kHornInt>(v)), ...);
+    ((k[T + 1] = horner_l<W, T * EsHorner<W>::nint, EsHorner<W>::nint>(v)), ...);
 }
 template <int W>
 __device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2]) {
-    k[0] = horner_l<W, (W - 2) * kHornInt, kHornEdge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
+    k[0] = horner_l<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
     es_half_interior<W>(v, k, std::make_integer_sequence<int, (W + 1) / 2 - 1>{});
 }
 
