@@ -245,7 +245,7 @@ def run_plain(args):
                                   f"tol={tol}, ntrans=1, one synthesis + one adjoint per step",
                       "lmax": L, "npts": M, "tol": tol, "ntrans": 1,
                       "nth": info["nth"], "nph": info["nph"], "mmax": info["mmax"],
-                      "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
+                      "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]], "fine_rows_stored": info["es_rows"],
                       "es_w": info["es_w"]},
            "triples_per_s": W * 2 * args.steps / elapsed,
            "apply_latency_ms": {"synthesis": {"p50": float(np.percentile(lat_s, 50)),
@@ -345,7 +345,7 @@ def run_geometry(args):
                                   f"{args.precision}, one synthesis + one adjoint per step",
                       "precision": args.precision,
                       "lmax": L, "npts": M, "tol": tol, "kind": info["kind_name"], "nth": info["nth"],
-                      "nph": info["nph"], "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]],
+                      "nph": info["nph"], "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]], "fine_rows_stored": info["es_rows"],
                       "es_w": info["es_w"]},
            "error_vs_direct_1e4": {"synthesis": err_s, "adjoint": err_a},
            "plan_s": t_plan,

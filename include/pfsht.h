@@ -60,6 +60,7 @@ typedef struct {
     int64_t ring_work; /* retained (l, m, ring) triples per apply (src/Legendre.jl:286-293) */
     int64_t device_bytes;
     double tol, es_beta;
+    int32_t es_rows;   /* fine-grid rows the apply stores: those the points' kernels reach */
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */

@@ -1047,6 +1047,7 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     info->device_bytes = p->bytes + (p->rings ? p->rings->bytes : 0);
     info->tol = p->tol;
     info->es_beta = p->nu.beta;
+    info->es_rows = p->nu.nR;
     return kOk;
 }
 

@@ -56,6 +56,7 @@ struct PfshtInfo              # pfsht_info, include/pfsht.h
     device_bytes::Int64
     tol::Float64
     es_beta::Float64
+    es_rows::Int32
 end
 
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
@@ -152,6 +153,75 @@ function PartialFSHT._adjoint_apply!(alm::ROCArray{ComplexF64}, f::ROCArray{Comp
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
                  leaf.handle, _device_ptr(f), _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream()))
     return alm
+end
+
+# ---------------------------------------------------------------------------------------------
+# Analysis on a device plan (src/Analysis.jl): the iterative callers keep every vector in HBM.
+# ---------------------------------------------------------------------------------------------
+const HIPPlan = PartialFSHT.Plan{<:Any, <:AbstractHIPLeaf}
+const _SOLVE_MODE = Dict(:cg => Int32(0), :wiener => Int32(1), :minnorm => Int32(2))
+
+struct PfshtSolveInfo         # pfsht_solve_info, include/pfsht.h
+    shannon::Float64
+    dl::Float64
+    relres::Float64
+    iters::Int32
+    mode::Int32
+end
+
+_weights_dev(w::Nothing) = nothing
+_weights_dev(w::ROCArray{Float64}) = w
+_weights_dev(w::AbstractVector) = ROCArray{Float64}(Vector{Float64}(w))
+_wptr(w::Nothing) = Ptr{Float64}(C_NULL)
+_wptr(w::ROCArray{Float64}) = reinterpret(Ptr{Float64}, pointer(w))
+
+# apply_resolution!(out, plan, alm, w, ws) — src/Analysis.jl:96-104
+function PartialFSHT.apply_resolution!(out::ROCArray{ComplexF64}, plan::HIPPlan, alm::ROCArray{ComplexF64},
+                                       w::AbstractVector, ws = nothing)
+    wd = _weights_dev(w)
+    GC.@preserve wd _check(ccall((:pfsht_apply_resolution, libpfsht), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                                 plan.leaf.handle, _device_ptr(alm), _wptr(wd), _device_ptr(out),
+                                 PFSHT_WHERE_DEVICE, _stream()))
+    return out
+end
+
+# pseudo_spectrum!(C, plan, f, w, ws; batch) — src/Analysis.jl:286-308
+function PartialFSHT.pseudo_spectrum!(C::AbstractVector, plan::HIPPlan, f::ROCArray{ComplexF64},
+                                      w::AbstractVector, ws = nothing; batch::Integer = 1)
+    length(C) == plan.lmax + 1 ||
+        throw(DimensionMismatch("C has $(length(C)) entries, expected $(plan.lmax + 1)"))
+    wd = _weights_dev(w)
+    Cd = ROCArray{Float64}(undef, plan.lmax + 1)
+    GC.@preserve wd _check(ccall((:pfsht_pseudo_spectrum, libpfsht), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Int32, Ptr{Cvoid}),
+                                 plan.leaf.handle, _device_ptr(f), _wptr(wd), batch,
+                                 reinterpret(Ptr{Float64}, pointer(Cd)), PFSHT_WHERE_DEVICE, _stream()))
+    copyto!(C, Array(Cd))
+    return C
+end
+
+# solve!(alm, plan, f; mode, rtol, maxiter, prior, w, ws) — src/Analysis.jl:375-436; info carries
+# shannon, Δℓ, mode, iters and relres as the reference's does
+function PartialFSHT.solve!(alm::ROCArray{ComplexF64}, plan::HIPPlan, f::ROCArray{ComplexF64};
+                            mode::Symbol = :cg, rtol::Real = 1e-6, maxiter::Integer = 500,
+                            prior = nothing, w = nothing, ws = nothing)
+    haskey(_SOLVE_MODE, mode) ||
+        throw(ArgumentError("mode must be :cg, :wiener or :minnorm, got :$mode"))
+    mode === :wiener && prior === nothing &&
+        throw(ArgumentError(":wiener needs a prior C_ℓ — that prior is what makes the " *
+                            "problem well posed, so it cannot be defaulted."))
+    wd = _weights_dev(w)
+    pr = prior === nothing ? nothing : Vector{Float64}(prior)
+    pp = pr === nothing ? Ptr{Float64}(C_NULL) : pointer(pr)
+    info = Ref{PfshtSolveInfo}()
+    GC.@preserve wd pr _check(ccall((:pfsht_solve, libpfsht), Cint,
+                                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int32, Float64, Int32, Ptr{Float64},
+                                     Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ref{PfshtSolveInfo}),
+                                    plan.leaf.handle, _device_ptr(f), _wptr(wd), _SOLVE_MODE[mode], rtol,
+                                    maxiter, pp, _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream(), info))
+    i = info[]
+    return alm, (shannon = i.shannon, Δℓ = i.dl, mode = mode, iters = Int(i.iters), relres = i.relres)
 end
 
 # A host array handed to a device leaf is refused, as the reference refuses the converse

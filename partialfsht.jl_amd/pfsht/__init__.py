@@ -49,7 +49,8 @@ class InfoC(C.Structure):
                 ("nth_half", C.c_int32), ("nth", C.c_int32), ("nph", C.c_int32),
                 ("mmax", C.c_int32), ("es_w", C.c_int32), ("es_n1", C.c_int32),
                 ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
-                ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double)]
+                ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double),
+                ("es_rows", C.c_int32)]
 
 
 class BoxC(C.Structure):
