@@ -32,8 +32,12 @@ enum {
     PFSHT_E_UNSUPPORTED = 6
 };
 
-/* Leaf selection, the AbstractAlgorithm tags of src/Cost.jl:15-24. */
-enum { PFSHT_METHOD_AUTO = 0, PFSHT_METHOD_DIRECT = 1, PFSHT_METHOD_STRETCH = 2 };
+/* Leaf selection, the AbstractAlgorithm tags of src/Cost.jl:15-24. AUTO follows select_method
+ * (src/Cost.jl:174-201): the cheaper of Direct and one stretched box, or several boxes when
+ * partition_boxes splits the points and the split is cheaper. MULTIBOX is MultiBoxLeaf
+ * (src/MultiBox.jl:20-103): one stretched plan per box, a disjoint union in synthesis and a sum
+ * in the adjoint. */
+enum { PFSHT_METHOD_AUTO = 0, PFSHT_METHOD_DIRECT = 1, PFSHT_METHOD_STRETCH = 2, PFSHT_METHOD_MULTIBOX = 3 };
 
 /* Where apply buffers live: host memory (copied through the plan's staging) or HBM. */
 enum { PFSHT_WHERE_HOST = 0, PFSHT_WHERE_DEVICE = 1 };
@@ -41,7 +45,7 @@ enum { PFSHT_WHERE_HOST = 0, PFSHT_WHERE_DEVICE = 1 };
 /* Arithmetic: FP64 throughout, or FP64 recurrence with FP32 accumulation/FFT/NUFFT (tol >= 1e-6). */
 enum { PFSHT_PREC_FP64 = 0, PFSHT_PREC_MIXED = 1 };
 
-/* Periodisation kinds (src/Windows.jl:34-36); -1 for a Direct leaf. */
+/* Periodisation kinds (src/Windows.jl:34-36); -1 for a Direct leaf, -2 for a multi-box plan. */
 enum { PFSHT_KIND_POLE = 0, PFSHT_KIND_INTERIOR = 1, PFSHT_KIND_FULL = 2 };
 
 typedef struct pfsht_plan_s *pfsht_plan_t;
@@ -61,6 +65,7 @@ typedef struct {
     int64_t device_bytes;
     double tol, es_beta;
     int32_t es_rows;   /* fine-grid rows the apply stores: those the points' kernels reach */
+    int32_t nboxes;    /* boxes of a multi-box plan (1 otherwise) */
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */
@@ -136,6 +141,12 @@ typedef struct {
 /* bounding_box(theta, phi) — src/Box.jl:43-73 (with _circular_extent, :80-102), box_fsky :106-108,
  * spectral_resolution :148-153. Host arrays; no device work. */
 int pfsht_bounding_box(const double *theta, const double *phi, int64_t npts, pfsht_box *out);
+
+/* partition_boxes(θ, φ, costfn; maxboxes) with costfn = cost_stretch(box, n, lmax, ntrans, tol)
+ * — src/Box.jl:159-235, src/Cost.jl:193. Host arrays; group_of[i] receives the box of point i
+ * (boxes numbered in the reference's order of creation), *nboxes their number. No device work. */
+int pfsht_partition_boxes(const double *theta, const double *phi, int64_t npts, int32_t lmax, int32_t ntrans,
+                          double tol, int32_t maxboxes, int32_t *group_of, int32_t *nboxes);
 
 /* apply_resolution!(out, plan, alm, w, ws) — src/Analysis.jl:96-104: out = A' (w .* (A alm)).
  * w: npts real weights (NULL: all ones). out overwritten, padding included. */

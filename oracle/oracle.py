@@ -62,6 +62,8 @@ def lib():
             "or_sph_mode": (_I64, [_I, _I, _I]),
             "or_polar_pad_factor": (_D, []),
             "or_polar_mmax": (_I, [_D, _I, _D]),
+            "or_cost_stretch": (_D, [_D, _D, _I64, _I, _I, _D]),
+            "or_cost_direct": (_D, [_I64, _I, _I]),
             "or_sectorial_log2c": (None, [_I, _P]),
             "or_nested_ranges": (_I, [_P, _I, _I, _P, _P]),
             "or_retained_triples": (_I64, [_P, _I, _I]),
@@ -543,3 +545,109 @@ def solve(plan: "Plan", f, mode="cg", rtol=1e-6, maxiter=500, prior=None, w=None
         p = r + (rsnew / rsold) * p
         rsold = rsnew
     return np.asfortranarray(x), dict(iters=iters, relres=float(rel))
+
+
+# ------------------------------------------------------------------------------------------------
+# Multi-box (src/Box.jl:159-374, src/MultiBox.jl, src/Cost.jl:174-201), restated over oracle plans.
+# ------------------------------------------------------------------------------------------------
+def _max_gap(v):
+    """_max_gap (Box.jl:257-296) by sorting: (lo, hi, gap, at) with `at` the upper edge of the first
+    largest gap in increasing order."""
+    s = np.sort(v)
+    lo, hi = float(s[0]), float(s[-1])
+    if len(s) < 2 or hi <= lo:
+        return lo, hi, 0.0, lo
+    d = np.diff(s)
+    k = int(np.argmax(d))
+    return lo, hi, float(d[k]), float(s[k + 1])
+
+
+def _group_box(th, ph):
+    """_group_box (Box.jl:298-319): (thlo, thhi) unclamped; the longitude arc is not needed by the cost."""
+    return float(th.min()), float(th.max())
+
+
+def partition_boxes(theta, phi, L, B=1, tol=1e-10, maxboxes=8):
+    """partition_boxes_costed (Box.jl:173-235) with costfn = cost_stretch (Cost.jl:193)."""
+    th = np.asarray(theta, np.float64)
+    ph = np.mod(np.asarray(phi, np.float64), 2 * np.pi)
+
+    def cost(g):
+        lo, hi = _group_box(th[g], ph[g])
+        return lib().or_cost_stretch(lo, hi, len(g), L, B, float(tol))
+
+    groups = [np.arange(len(th))]
+    costs = [cost(groups[0])]
+    e = np.sqrt(np.finfo(np.float64).eps)
+    while len(groups) < maxboxes:
+        best_gain, best_at, best = 0.0, -1, None
+        for gi, g in enumerate(groups):
+            if len(g) < 2:
+                continue
+            tlo, thi, gt, at_t = _max_gap(th[g])
+            _, _, gp, at_p = _max_gap(ph[g])
+            wphi = gp * np.sin(min(max((tlo + thi) / 2, e), np.pi - e))
+            if gt >= wphi:
+                if not gt > 0:
+                    continue
+                mask = th[g] < at_t
+            else:
+                if not gp > 0:
+                    continue
+                mask = ph[g] < at_p
+            ga, gb = g[mask], g[~mask]
+            if len(ga) == 0 or len(gb) == 0:
+                continue
+            gain = costs[gi] - (cost(ga) + cost(gb))
+            if gain > best_gain:
+                best_gain, best_at, best = gain, gi, (ga, gb)
+        if best_at < 0:
+            break
+        del groups[best_at]
+        del costs[best_at]
+        for g in best:
+            groups.append(g)
+            costs.append(cost(g))
+    return groups, costs
+
+
+def select_method(theta, phi, L, tol=1e-10, B=1):
+    """select_method (Cost.jl:174-201) without the NUFSHT extension: the cheapest of Direct, the
+    stretched leaf over the bounding box, and the multi-box leaf when the partition has more than
+    one box. Returns "direct", "stretch" or "multibox"."""
+    th = np.asarray(theta, np.float64)
+    n = len(th)
+    best, bestc = "direct", lib().or_cost_direct(n, L, B)
+    cs = lib().or_cost_stretch(max(float(th.min()), 0.0), min(float(th.max()), np.pi), n, L, B, float(tol))
+    if cs < bestc:
+        best, bestc = "stretch", cs
+    groups, costs = partition_boxes(th, phi, L, B, tol)
+    if len(groups) > 1 and sum(costs) < bestc:
+        best = "multibox"
+    return best
+
+
+class MultiBox:
+    """MultiBoxLeaf (MultiBox.jl:20-144) over oracle stretched plans: a disjoint union in synthesis,
+    the sum over boxes (in box order) in the adjoint."""
+
+    def __init__(self, theta, phi, lmax, tol=1e-10, ntrans=1, maxboxes=8, nthreads=1):
+        th, ph = np.asarray(theta, np.float64), np.asarray(phi, np.float64)
+        self.L, self.B, self.M = lmax, ntrans, len(th)
+        self.groups, _ = partition_boxes(th, ph, lmax, ntrans, tol, maxboxes)
+        self.plans = [Plan(th[g], ph[g], lmax, tol=tol, ntrans=ntrans, method="stretch", nufft="es",
+                           nthreads=nthreads) for g in self.groups]
+
+    def synth(self, alm):
+        f = np.zeros((self.M, self.B), np.complex128, order="F")
+        for g, p in zip(self.groups, self.plans):
+            f[g, :] = p.synth(alm).reshape(len(g), self.B, order="F")
+        return f[:, 0] if self.B == 1 else f
+
+    def adjoint(self, f):
+        f = np.asarray(f).reshape(self.M, self.B, order="F")
+        alm = None
+        for g, p in zip(self.groups, self.plans):
+            a = p.adjoint(np.asfortranarray(f[g, :] if self.B > 1 else f[g, 0]))
+            alm = a if alm is None else alm + a
+        return alm

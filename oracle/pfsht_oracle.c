@@ -1085,6 +1085,12 @@ static double cost_stretch(double thlo, double thhi, int64_t npts, int L, int B,
     return B * (COST_RING * (double)rings + COST_FFT * fft + COST_SPREAD * spread);
 }
 
+/* Cost.jl:114-143, for the partition restatement of oracle.py */
+double or_cost_stretch(double thlo, double thhi, int64_t npts, int L, int B, double tol) {
+    return cost_stretch(thlo, thhi, npts, L, B, tol);
+}
+double or_cost_direct(int64_t npts, int L, int B) { return cost_direct(npts, L, B); }
+
 static double jmod(double x, double y) { /* Julia mod for floats */
     double r = fmod(x, y);
     if (r != 0 && ((r < 0) != (y < 0))) r += y;

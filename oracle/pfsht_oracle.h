@@ -49,6 +49,8 @@ int or_polar_mmax(double theta, int L, double eps);
 void or_sectorial_log2c(int L, double *out);
 int or_nested_ranges(const int32_t *mlim, int n, int L, int32_t *jlo, int32_t *jhi);
 int64_t or_retained_triples(const int32_t *mlim, int n, int L);
+double or_cost_stretch(double thlo, double thhi, int64_t npts, int L, int B, double tol);
+double or_cost_direct(int64_t npts, int L, int B);
 double or_alf(int l, int m, double theta);
 
 typedef struct or_rings or_rings;

@@ -139,3 +139,33 @@ void launch_spectrum(const double2 *alm, int L, double *C, hipStream_t s) {
 }
 
 }  // namespace pfsht
+
+namespace pfsht {
+
+// Multi-box applies (src/MultiBox.jl:83-144): f[b][idx[j]] = fg[b][j] and fg[b][j] = f[b][idx[j]]
+// for the points of one box.
+__global__ void scatter_field_kernel(double2 *__restrict__ f, const double2 *__restrict__ fg,
+                                     const int32_t *__restrict__ idx, int64_t Mg, int64_t M) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    if (j < Mg) f[b * M + idx[j]] = fg[b * Mg + j];
+}
+
+__global__ void gather_field_kernel(double2 *__restrict__ fg, const double2 *__restrict__ f,
+                                    const int32_t *__restrict__ idx, int64_t Mg, int64_t M) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    if (j < Mg) fg[b * Mg + j] = f[b * M + idx[j]];
+}
+
+void launch_scatter_field(double2 *f, const double2 *fg, const int32_t *idx, int64_t Mg, int64_t M, int B,
+                          hipStream_t s) {
+    if (Mg > 0) scatter_field_kernel<<<dim3(unsigned((Mg + 255) / 256), B), 256, 0, s>>>(f, fg, idx, Mg, M);
+}
+
+void launch_gather_field(double2 *fg, const double2 *f, const int32_t *idx, int64_t Mg, int64_t M, int B,
+                         hipStream_t s) {
+    if (Mg > 0) gather_field_kernel<<<dim3(unsigned((Mg + 255) / 256), B), 256, 0, s>>>(fg, f, idx, Mg, M);
+}
+
+}  // namespace pfsht

@@ -342,6 +342,13 @@ struct pfsht_plan_s {
     hipEvent_t done = nullptr;
     hipStream_t last_stream = nullptr;
     bool has_done = false;
+    // multi-box plan (MultiBoxLeaf, src/MultiBox.jl:20-73): one stretched plan per box, the
+    // points of each box (device indices into the caller's field), a box's field, and the
+    // accumulator the adjoint sums the boxes' coefficients through
+    std::vector<pfsht_plan_s *> boxes;
+    std::vector<int32_t *> box_idx;
+    std::vector<int64_t> box_npts;
+    double2 *mb_f = nullptr, *mb_acc = nullptr;
     // analysis workspace (AnalysisWorkspace, src/Analysis.jl:45-84), allocated at the first analysis
     // call: rhs, r, p, Ap and a staging alm in the alm layout, two fields, weights, the Wiener
     // prior, the spectrum, dot-product partials and a pinned host slot for scalars
@@ -368,6 +375,9 @@ struct pfsht_plan_s {
         dfree(aw_rhs); dfree(aw_r); dfree(aw_p); dfree(aw_Ap); dfree(aw_x); dfree(aw_f); dfree(aw_f2);
         dfree(aw_part); dfree(aw_dot); dfree(aw_w); dfree(aw_reg); dfree(aw_C);
         if (aw_hdot) (void)hipHostFree(aw_hdot);
+        for (auto *q : boxes) delete q;
+        for (auto *q : box_idx) dfree(q);
+        dfree(mb_f); dfree(mb_acc);
         fft_free(th_fwd); fft_free(th_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd); fft_free(col_bwd); fft_free(col_fwd);
         fft_free(row_bwd); fft_free(row_fwd);
@@ -406,11 +416,50 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     p.th_max = thhi;
     thlo = std::max(thlo, 0.0);
     thhi = std::min(thhi, kPi);
-    if (method == PFSHT_METHOD_AUTO) {
+    Partition part;
+    bool parted = false;
+    if (method == PFSHT_METHOD_AUTO) {  // select_method, src/Cost.jl:174-201
         const double cd = cost_direct(M, L, B), cs = cost_stretch(thlo, thhi, M, L, B, tol);
-        method = cs < cd ? PFSHT_METHOD_STRETCH : PFSHT_METHOD_DIRECT;
+        double best = cd;
+        method = PFSHT_METHOD_DIRECT;
+        if (cs < best) { method = PFSHT_METHOD_STRETCH; best = cs; }
+        part = partition_boxes(theta, phi, M, L, B, tol, kMaxBoxes);
+        parted = true;
+        if (part.groups.size() > 1) {
+            double cm = 0;
+            for (double c : part.costs) cm += c;
+            if (cm < best) method = PFSHT_METHOD_MULTIBOX;
+        }
     }
     p.method = method;
+    if (method == PFSHT_METHOD_MULTIBOX) {
+        // MultiBoxLeaf (src/MultiBox.jl:30-62): a stretched plan per box of the partition
+        if (!parted) part = partition_boxes(theta, phi, M, L, B, tol, kMaxBoxes);
+        p.geom.kind = -2;
+        int64_t &by = p.bytes, maxg = 0;
+        for (const auto &g : part.groups) {
+            const int64_t Mg = int64_t(g.size());
+            std::vector<double> tg(Mg), pg(Mg);
+            for (int64_t j = 0; j < Mg; ++j) { tg[j] = theta[g[j]]; pg[j] = phi[g[j]]; }
+            auto *q = new (std::nothrow) pfsht_plan_s;
+            if (!q) { set_error("host allocation failed"); return kENomem; }
+            q->precision = p.precision;
+            p.boxes.push_back(q);
+            Status st = plan_build(*q, tg.data(), pg.data(), Mg, L, tol, B, PFSHT_METHOD_STRETCH);
+            if (st != kOk) return st;
+            int32_t *idx = nullptr;
+            if ((st = upload(&idx, g, by)) != kOk) return st;
+            p.box_idx.push_back(idx);
+            p.box_npts.push_back(Mg);
+            p.mmax = std::max(p.mmax, q->mmax);
+            maxg = std::max(maxg, Mg);
+        }
+        Status st;
+        if ((st = dalloc(&p.mb_f, size_t(maxg) * B, by)) != kOk) return st;
+        if ((st = dalloc(&p.mb_acc, size_t(L + 1) * (2 * L + 1) * B, by)) != kOk) return st;
+        PFSHT_HIP_TRY(hipDeviceSynchronize());
+        return kOk;
+    }
     if (method == PFSHT_METHOD_DIRECT) {
         // DirectLeaf (src/Direct.jl:25-51): points sorted by colatitude (stable), ring engine at
         // eps = 0 so polar_mmax keeps every order that is not below floatmin.
@@ -689,7 +738,36 @@ static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, const double2 *f, hip
     return kOk;
 }
 
+static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s);
+static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s);
+
+// synthesize_multibox! / adjoint_multibox! (src/MultiBox.jl:83-112): every point belongs to one
+// box, so the synthesis writes each of f's entries once; the adjoint adds the boxes' coefficients
+// in box order.
+static Status multibox_synth(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+    for (size_t g = 0; g < p.boxes.size(); ++g) {
+        Status st = synth_impl(*p.boxes[g], alm, p.mb_f, s);
+        if (st != kOk) return st;
+        launch_scatter_field(f, p.mb_f, p.box_idx[g], p.box_npts[g], p.M, p.B, s);
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+static Status multibox_adjoint(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+    const int64_t n = int64_t(p.L + 1) * (2 * p.L + 1) * p.B;
+    for (size_t g = 0; g < p.boxes.size(); ++g) {
+        launch_gather_field(p.mb_f, f, p.box_idx[g], p.box_npts[g], p.M, p.B, s);
+        Status st = adjoint_impl(*p.boxes[g], p.mb_f, g == 0 ? alm : p.mb_acc, s);
+        if (st != kOk) return st;
+        if (g > 0) launch_zaxpy(alm, p.mb_acc, make_double2(1.0, 0.0), n, s);
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
 static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_synth(p, alm, f, s);
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
@@ -714,6 +792,7 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
 }
 
 static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_adjoint(p, f, alm, s);
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 1);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:94-117
@@ -786,6 +865,14 @@ static Status aw_box(pfsht_plan_s &p) {
     std::vector<double> ph(p.M);
     if (p.method == PFSHT_METHOD_DIRECT) {
         PFSHT_HIP_TRY(hipMemcpy(ph.data(), p.dphi, sizeof(double) * p.M, hipMemcpyDeviceToHost));
+    } else if (p.method == PFSHT_METHOD_MULTIBOX) {
+        int64_t o = 0;
+        for (auto *q : p.boxes) {
+            PFSHT_HIP_TRY(hipMemcpy(ph.data() + o, q->nu.X2, sizeof(double) * q->M, hipMemcpyDeviceToHost));
+            const double sc = 2 * kPi / double(q->nu.N2);
+            for (int64_t j = 0; j < q->M; ++j) ph[o + j] *= sc;
+            o += q->M;
+        }
     } else {
         PFSHT_HIP_TRY(hipMemcpy(ph.data(), p.nu.X2, sizeof(double) * p.M, hipMemcpyDeviceToHost));
         const double sc = 2 * kPi / double(p.nu.N2);
@@ -849,6 +936,24 @@ int pfsht_bounding_box(const double *theta, const double *phi, int64_t npts, pfs
     out->touches_south = b.south;
     out->fsky = box_fsky(b);
     out->dl = spectral_resolution(b);
+    return kOk;
+}
+
+int pfsht_partition_boxes(const double *theta, const double *phi, int64_t npts, int32_t lmax, int32_t ntrans,
+                          double tol, int32_t maxboxes, int32_t *group_of, int32_t *nboxes) {
+    if (!theta || !phi || !group_of || !nboxes) { set_error("NULL argument"); return kEArg; }
+    if (npts < 1) { set_error("need at least one point"); return kEArg; }
+    if (npts > INT32_MAX) { set_error("more than 2^31-1 points is not supported"); return kEArg; }
+    if (lmax < 0 || ntrans < 1 || maxboxes < 1 || !(tol > 0) || !(tol < 1)) { set_error("bad argument"); return kEArg; }
+    try {
+        const Partition P = partition_boxes(theta, phi, npts, lmax, ntrans, tol, maxboxes);
+        for (size_t g = 0; g < P.groups.size(); ++g)
+            for (int32_t i : P.groups[g]) group_of[i] = int32_t(g);
+        *nboxes = int32_t(P.groups.size());
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return kENomem;
+    }
     return kOk;
 }
 
@@ -1031,6 +1136,7 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     std::memset(info, 0, sizeof *info);
     info->method = p->method;
     info->kind = p->method == PFSHT_METHOD_DIRECT ? -1 : p->geom.kind;
+    info->nboxes = p->method == PFSHT_METHOD_MULTIBOX ? int32_t(p->boxes.size()) : 1;
     info->flip = p->flip;
     info->lmax = p->L;
     info->ntrans = p->B;
@@ -1045,6 +1151,10 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     info->npts = p->M;
     info->ring_work = p->rings ? p->rings->work : 0;
     info->device_bytes = p->bytes + (p->rings ? p->rings->bytes : 0);
+    for (auto *q : p->boxes) {
+        info->ring_work += q->rings ? q->rings->work : 0;
+        info->device_bytes += q->bytes + (q->rings ? q->rings->bytes : 0);
+    }
     info->tol = p->tol;
     info->es_beta = p->nu.beta;
     info->es_rows = p->nu.nR;
@@ -1102,7 +1212,7 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
 int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, int32_t where, void *stream) {
     if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
     if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }
-    if (p->method != PFSHT_METHOD_STRETCH) { set_error("a Direct plan has no NUFFT"); return kEArg; }
+    if (p->method != PFSHT_METHOD_STRETCH) { set_error("only a stretched plan has a NUFFT"); return kEArg; }
     if (p->precision != PFSHT_PREC_FP64) { set_error("the NUFFT seam is FP64 only"); return kEUnsupported; }
     if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
     std::lock_guard<std::mutex> lk(p->mu);
@@ -1178,6 +1288,7 @@ int32_t pfsht_rings_mmax(pfsht_rings_t r) { return r ? r->d.mmax : -1; }
 
 int pfsht_plan_rings(pfsht_plan_t p, pfsht_rings_t *out) {
     if (!p || !out) { set_error("NULL argument"); return kEArg; }
+    if (p->method == PFSHT_METHOD_MULTIBOX) { set_error("a multi-box plan has one ring engine per box"); return kEArg; }
     *out = p->rings;
     return kOk;
 }

@@ -261,6 +261,140 @@ double spectral_resolution(const Box &b) {
     return lam <= 0 ? INFINITY : kPi / lam;
 }
 
+namespace {
+struct GapResult {
+    double lo, hi, gap, at;
+};
+
+// _max_gap over the values v[g[i]] (src/Box.jl:257-296); bk is bucket scratch of >= |g| - 1 pairs
+GapResult max_gap(const std::vector<double> &v, const std::vector<int32_t> &g,
+                  std::vector<std::pair<double, double>> &bk) {
+    const int64_t n = int64_t(g.size());
+    double lo = v[g[0]], hi = v[g[0]];
+    for (int64_t i = 1; i < n; ++i) {
+        lo = std::min(lo, v[g[i]]);
+        hi = std::max(hi, v[g[i]]);
+    }
+    if (n < 2 || hi <= lo) return {lo, hi, 0.0, lo};
+    const int64_t nb = n - 1;
+    // bucket k ~ [lo + k w, lo + (k + 1) w): a value within rounding of a boundary may land in the
+    // neighbouring bucket, which cannot move a gap of at least w, and the largest gap is at least w
+    const double iw = double(nb) / (hi - lo);
+    for (int64_t k = 0; k < nb; ++k) bk[k] = {INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; ++i) {
+        const double x = v[g[i]];
+        int64_t k = int64_t((x - lo) * iw);
+        k = std::min(k, nb - 1);
+        bk[k].first = std::min(bk[k].first, x);
+        bk[k].second = std::max(bk[k].second, x);
+    }
+    double gap = 0.0, at = hi, prev = lo;
+    for (int64_t k = 0; k < nb; ++k) {
+        if (!(bk[k].first <= bk[k].second)) continue;
+        const double d = bk[k].first - prev;
+        if (d > gap) { gap = d; at = bk[k].first; }
+        prev = bk[k].second;
+    }
+    return {lo, hi, gap, at};
+}
+
+// _group_box (src/Box.jl:298-319): colatitude ends (not clamped) and the longitude arc
+Box group_box(const std::vector<double> &th, const std::vector<double> &ph, const std::vector<int32_t> &g,
+              std::vector<std::pair<double, double>> &bk) {
+    double lo = th[g[0]], hi = th[g[0]];  // only the colatitude ends are needed here
+    for (int32_t i : g) {
+        lo = std::min(lo, th[i]);
+        hi = std::max(hi, th[i]);
+    }
+    Box b;
+    b.thlo = lo;
+    b.thhi = hi;
+    const double ptol = std::sqrt(2.220446049250313e-16);
+    b.north = lo <= ptol;
+    b.south = hi >= kPi - ptol;
+    if (b.north || b.south) { b.phistart = 0.0; b.phiext = 2 * kPi; return b; }
+    const GapResult f = max_gap(ph, g, bk);
+    const double wrap = 2 * kPi - f.hi + f.lo;
+    double gap = f.gap, start = f.at;
+    if (wrap >= gap) { gap = wrap; start = f.lo; }
+    const double ext = 2 * kPi - gap;
+    if (ext <= 0) { b.phistart = f.lo; b.phiext = 0.0; }
+    else if (ext >= 2 * kPi) { b.phistart = 0.0; b.phiext = 2 * kPi; }
+    else { b.phistart = start; b.phiext = ext; }
+    return b;
+}
+
+// _split_below (src/Box.jl:356-374): false when either side would be empty
+bool split_below(const std::vector<double> &v, const std::vector<int32_t> &g, double thresh,
+                 std::vector<int32_t> &a, std::vector<int32_t> &b) {
+    a.clear();
+    b.clear();
+    for (int32_t i : g) (v[i] < thresh ? a : b).push_back(i);
+    return !a.empty() && !b.empty();
+}
+
+// _best_gap_split (src/Box.jl:325-343): the larger of the colatitude gap and the longitude gap
+// weighted by the ring radius
+bool best_gap_split(const std::vector<double> &th, const std::vector<double> &ph, const std::vector<int32_t> &g,
+                    std::vector<std::pair<double, double>> &bk, std::vector<int32_t> &a, std::vector<int32_t> &b) {
+    const GapResult t = max_gap(th, g, bk);
+    const GapResult f = max_gap(ph, g, bk);
+    const double e = std::sqrt(2.220446049250313e-16);
+    const double mid = std::min(std::max(0.5 * (t.lo + t.hi), e), kPi - e);
+    const double wphi = f.gap * std::sin(mid);
+    if (t.gap >= wphi) {
+        if (!(t.gap > 0)) return false;
+        return split_below(th, g, t.at, a, b);
+    }
+    if (!(f.gap > 0)) return false;
+    return split_below(ph, g, f.at, a, b);
+}
+}  // namespace
+
+Partition partition_boxes(const double *theta, const double *phi, int64_t n, int L, int B, double tol,
+                          int maxboxes) {
+    std::vector<double> th(theta, theta + n), ph(n);
+    for (int64_t i = 0; i < n; ++i) ph[i] = jmod(phi[i], 2 * kPi);
+    std::vector<std::pair<double, double>> bk(size_t(std::max<int64_t>(n, 1)));
+    auto cost = [&](const Box &b, int64_t np) { return cost_stretch(b.thlo, b.thhi, np, L, B, tol); };
+    Partition P;
+    std::vector<int32_t> all(n);
+    for (int64_t i = 0; i < n; ++i) all[i] = int32_t(i);
+    P.groups.push_back(std::move(all));
+    P.costs.push_back(cost(group_box(th, ph, P.groups[0], bk), n));
+    std::vector<int32_t> a, b;
+    while (int(P.groups.size()) < maxboxes) {
+        double best_gain = 0.0;
+        int best_at = -1;
+        std::vector<int32_t> ba, bb;
+        double ca = 0, cb = 0;
+        for (size_t gi = 0; gi < P.groups.size(); ++gi) {
+            const auto &g = P.groups[gi];
+            if (g.size() < 2) continue;
+            if (!best_gap_split(th, ph, g, bk, a, b)) continue;
+            const double xa = cost(group_box(th, ph, a, bk), int64_t(a.size()));
+            const double xb = cost(group_box(th, ph, b, bk), int64_t(b.size()));
+            const double gain = P.costs[gi] - (xa + xb);
+            if (gain > best_gain) {
+                best_gain = gain;
+                best_at = int(gi);
+                ba = a;
+                bb = b;
+                ca = xa;
+                cb = xb;
+            }
+        }
+        if (best_at < 0) break;
+        P.groups.erase(P.groups.begin() + best_at);
+        P.costs.erase(P.costs.begin() + best_at);
+        P.groups.push_back(std::move(ba));
+        P.costs.push_back(ca);
+        P.groups.push_back(std::move(bb));
+        P.costs.push_back(cb);
+    }
+    return P;
+}
+
 static double fftunits(int64_t n) { return n <= 1 ? 0.0 : double(n) * std::log2(double(n)); }
 
 double cost_direct(int64_t npts, int L, int B) {

@@ -65,6 +65,19 @@ Box bounding_box_range(double thmin, double thmax, const double *phi, int64_t n)
 double box_fsky(const Box &b);
 double spectral_resolution(const Box &b);
 
+// partition_boxes_costed (src/Box.jl:159-235): split the points at the largest colatitude or
+// longitude gap of a group while the counted cost of stretched leaves falls, up to maxboxes
+// groups, with the O(n) bucket scan of _max_gap (src/Box.jl:246-296) and the group box of
+// src/Box.jl:298-319. Groups hold point indices in increasing order, in the reference's order of
+// creation; costs[g] is cost_stretch of group g.
+struct Partition {
+    std::vector<std::vector<int32_t>> groups;
+    std::vector<double> costs;
+};
+constexpr int kMaxBoxes = 8;  // partition_boxes' default (src/Box.jl:171)
+Partition partition_boxes(const double *theta, const double *phi, int64_t n, int L, int B, double tol,
+                          int maxboxes);
+
 // Cost model restricted to Direct vs Stretch (src/Cost.jl:104-140,174-201).
 double cost_direct(int64_t npts, int L, int B);
 double cost_stretch(double thlo, double thhi, int64_t npts, int L, int B, double tol);

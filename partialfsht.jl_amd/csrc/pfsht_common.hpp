@@ -183,5 +183,10 @@ void launch_zaxpy(double2 *y, const double2 *x, double2 a, int64_t n, hipStream_
 void launch_zxpby(double2 *p, const double2 *r, double2 beta, int64_t n, hipStream_t s);
 void launch_add_prior(double2 *Ap, const double2 *p, const double *reg, int L, int B, hipStream_t s);
 void launch_spectrum(const double2 *alm, int L, double *C, hipStream_t s);
+// multi-box plans: the points of one box in and out of the caller's field
+void launch_scatter_field(double2 *f, const double2 *fg, const int32_t *idx, int64_t Mg, int64_t M, int B,
+                          hipStream_t s);
+void launch_gather_field(double2 *fg, const double2 *f, const int32_t *idx, int64_t Mg, int64_t M, int B,
+                         hipStream_t s);
 
 }  // namespace pfsht

@@ -34,6 +34,7 @@ const PFSHT_E_ARG = Cint(2)
 const PFSHT_E_GEOM_REFUSED = Cint(3)
 const PFSHT_METHOD_DIRECT = Cint(1)
 const PFSHT_METHOD_STRETCH = Cint(2)
+const PFSHT_METHOD_MULTIBOX = Cint(3)
 const PFSHT_PREC_FP64 = Cint(0)
 const PFSHT_WHERE_DEVICE = Cint(1)
 
@@ -57,6 +58,7 @@ struct PfshtInfo              # pfsht_info, include/pfsht.h
     tol::Float64
     es_beta::Float64
     es_rows::Int32
+    nboxes::Int32
 end
 
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
@@ -91,6 +93,11 @@ mutable struct HIPDirectLeaf <: AbstractHIPLeaf
     info::PfshtInfo
 end
 
+mutable struct HIPMultiBoxLeaf <: AbstractHIPLeaf
+    handle::Ptr{Cvoid}
+    info::PfshtInfo
+end
+
 function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVector, lmax::Int,
                  ntrans::Int, tol::Real) where {LT<:AbstractHIPLeaf}
     θh = Vector{Float64}(PartialFSHT._host(θ))
@@ -115,9 +122,12 @@ HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real) =
     _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol)
 HIPDirectLeaf(θ, φ, lmax::Int, ntrans::Int) =
     _create(HIPDirectLeaf, PFSHT_METHOD_DIRECT, θ, φ, lmax, ntrans, 1e-10)
+HIPMultiBoxLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real) =
+    _create(HIPMultiBoxLeaf, PFSHT_METHOD_MULTIBOX, θ, φ, lmax, ntrans, tol)
 
 PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
 PartialFSHT.algorithm(::HIPDirectLeaf) = PartialFSHT.DirectAlgorithm()
+PartialFSHT.algorithm(::HIPMultiBoxLeaf) = PartialFSHT.MultiBoxAlgorithm()
 PartialFSHT.ring_work(leaf::AbstractHIPLeaf) = Int(leaf.info.ring_work)
 
 # Leaf seam (src/Plan.jl:118-129): a plan on ROCVector nodes builds the device leaf of the chosen
@@ -129,6 +139,11 @@ PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVe
 PartialFSHT._build_leaf(::PartialFSHT.DirectAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
     spin == 0 ? HIPDirectLeaf(θ, φ, lmax, ntrans) :
+    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+# src/Plan.jl:131-132: the library repeats partition_boxes on the host copy of the nodes
+PartialFSHT._build_leaf(::PartialFSHT.MultiBoxAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
+                        spin, tol, nufft_b, opts) =
+    spin == 0 ? HIPMultiBoxLeaf(θ, φ, lmax, ntrans, tol) :
     throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
 
 _stream() = AMDGPU.stream().stream   # hipStream_t of the task-local AMDGPU stream

@@ -50,7 +50,7 @@ class InfoC(C.Structure):
                 ("mmax", C.c_int32), ("es_w", C.c_int32), ("es_n1", C.c_int32),
                 ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double),
-                ("es_rows", C.c_int32)]
+                ("es_rows", C.c_int32), ("nboxes", C.c_int32)]
 
 
 class BoxC(C.Structure):
@@ -68,8 +68,8 @@ SOLVE_MODES = {"cg": 0, "wiener": 1, "minnorm": 2}
 
 NSTAGES = 9
 STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft", "ring_reduce"]
-METHODS = {"auto": 0, "direct": 1, "stretch": 2}
-KINDS = {0: "pole", 1: "interior", 2: "full", -1: "direct"}
+METHODS = {"auto": 0, "direct": 1, "stretch": 2, "multibox": 3}
+KINDS = {0: "pole", 1: "interior", 2: "full", -1: "direct", -2: "multibox"}
 WHERE_HOST, WHERE_DEVICE = 0, 1
 
 EXPORTS = {
@@ -93,6 +93,8 @@ EXPORTS = {
     "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "pfsht_stage_times": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]),
     "pfsht_bounding_box": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(BoxC)]),
+    "pfsht_partition_boxes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_double,
+                                        C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
     "pfsht_apply_resolution": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_pseudo_spectrum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
                                         C.c_void_p]),
@@ -241,6 +243,20 @@ def bounding_box(theta, phi) -> dict:
     return d
 
 
+def partition_boxes(theta, phi, lmax: int, ntrans: int = 1, tol: float = 1e-10, maxboxes: int = 8):
+    """partition_boxes(θ, φ, costfn; maxboxes) with the stretched leaf's counted cost
+    (src/Box.jl:159-235, src/Cost.jl:193): a list of index arrays, in the reference's order."""
+    th = np.ascontiguousarray(theta, np.float64)
+    ph = np.ascontiguousarray(phi, np.float64)
+    if th.shape != ph.shape:
+        raise DimensionMismatch("theta and phi must have the same length")
+    gid = np.zeros(len(th), np.int32)
+    ng = C.c_int32()
+    _check(lib().pfsht_partition_boxes(th.ctypes.data, ph.ctypes.data, len(th), int(lmax), int(ntrans),
+                                       float(tol), int(maxboxes), gid.ctypes.data, C.byref(ng)))
+    return [np.flatnonzero(gid == g) for g in range(ng.value)]
+
+
 def _weights(w, M: int, where: int):
     """(pointer, keep-alive) of npts float64 weights on the same side as the other arrays."""
     if w is None:
@@ -297,14 +313,15 @@ class Plan:
             pass
 
     def __repr__(self):
-        alg = "DirectAlgorithm" if self.info["method"] == 1 else "StretchAlgorithm"
+        alg = {"direct": "DirectAlgorithm", "stretch": "StretchAlgorithm",
+               "multibox": "MultiBoxAlgorithm"}[self.algorithm]
         return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, "
                 f"algorithm={alg}, kind={self.info['kind_name']})")
 
     @property
     def algorithm(self) -> str:
-        """algorithm(plan) — src/Plan.jl:186: "direct" or "stretch"."""
-        return "direct" if self.info["method"] == 1 else "stretch"
+        """algorithm(plan) — src/Plan.jl:32-36: "direct", "stretch" or "multibox"."""
+        return {1: "direct", 2: "stretch", 3: "multibox"}[self.info["method"]]
 
     @property
     def H(self) -> "AdjointPlan":
