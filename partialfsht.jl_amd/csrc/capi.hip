@@ -55,6 +55,21 @@ static void dfree(T *&p) {
     p = nullptr;
 }
 
+// Temporary device buffers of a plan build, freed once the device is idle when it goes out of scope.
+struct DevScratch {
+    std::vector<void *> ptrs;
+    template <class T>
+    Status take(T **p, size_t count, int64_t &bytes) {
+        Status st = dalloc(p, count, bytes);
+        if (st == kOk && *p) ptrs.push_back(*p);
+        return st;
+    }
+    ~DevScratch() {
+        if (!ptrs.empty()) (void)hipDeviceSynchronize();
+        for (void *q : ptrs) (void)hipFree(q);
+    }
+};
+
 // RAII device guard: make the plan's device current for the call, restore the caller's after.
 struct DeviceGuard {
     int prev = -1;
@@ -170,31 +185,11 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     sectorial_log2c(L, log2c);
     RingDev &d = r.d;
     d.n = n; d.L = L; d.B = B; d.mmax = mmax;
-    // recurrence coefficients, evaluated in the reference's order (src/Legendre.jl:406-408)
+    // recurrence coefficients (src/Legendre.jl:406-408) and the rescaled forward's (gamma_l, C_l),
+    // tabulated on the device per order from m + 2; abofs[m] is the order's first entry
     std::vector<int64_t> abofs(mmax + 1);
     int64_t nab = 0;
     for (int m = 0; m <= mmax; ++m) { abofs[m] = nab; nab += std::max(0, L - m - 1); }
-    std::vector<double2> ab(std::max<int64_t>(nab, 1));
-    for (int m = 0; m <= mmax; ++m)
-        for (int l = m + 2; l <= L; ++l) {
-            const double a = std::sqrt(double(2 * l - 1) * double(2 * l + 1) / (double(l - m) * double(l + m)));
-            const double bb = std::sqrt(double(2 * l + 1) / double(2 * l - 3) * double(l - m - 1) / double(l - m) *
-                                        double(l + m - 1) / double(l + m));
-            ab[abofs[m] + (l - m - 2)] = make_double2(a, bb);
-        }
-    // the rescaled forward's (gamma_l, C_l) = (b_l / (a_l a_{l-1}), prod of a_k over the degrees of
-    // l's 16-degree group up to l), with a_{m+1} = sqrt(2m + 3) and group 0 starting at m + 2
-    std::vector<double2> gc(ab.size());
-    for (int m = 0; m <= mmax; ++m) {
-        double aprev = std::sqrt(double(2 * m + 3)), C = 1.0;
-        for (int l = m + 2; l <= L; ++l) {
-            const double2 c = ab[abofs[m] + (l - m - 2)];
-            if (((l - m) & 15) == 0) C = 1.0;
-            C *= c.x;
-            gc[abofs[m] + (l - m - 2)] = make_double2(c.y / (c.x * aprev), C);
-            aprev = c.x;
-        }
-    }
     // Work items: one per (order, 64-ring tile), with the order's degree segments; checkpoint rows
     // for segments 1.. of every tile. Partials of the adjoint are indexed by the order's tile.
     std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
@@ -279,9 +274,10 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.costh, costh, by)) != kOk) return st;
     if ((st = upload(&d.log2sin, log2sin, by)) != kOk) return st;
     if ((st = upload(&d.log2c, log2c, by)) != kOk) return st;
-    if ((st = upload(&d.ab, ab, by)) != kOk) return st;
-    if ((st = upload(&d.gc, gc, by)) != kOk) return st;
     if ((st = upload(&d.abofs, abofs, by)) != kOk) return st;
+    if ((st = dalloc(&d.ab, size_t(std::max<int64_t>(nab, 1)), by)) != kOk) return st;
+    if ((st = dalloc(&d.gc, size_t(std::max<int64_t>(nab, 1)), by)) != kOk) return st;
+    launch_coef_tables(L, mmax, d.abofs, d.ab, d.gc, nullptr);
     if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
     if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
     if ((st = upload(&d.fitems, fitems, by)) != kOk) return st;
@@ -532,47 +528,66 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
             return kEUnsupported;
         }
     }
-    std::vector<double> X1(M), X2(M);
-    std::vector<int32_t> key(M);
-    const double half = 0.5 * u.w;
-    for (int64_t i = 0; i < M; ++i) {
-        const double t = theta[i];
-        const double Tst = theta_to_stretch(p.geom, p.flip ? kPi - t : t);
-        const double ph = jmod(phi[i], 2 * kPi);
-        double x1 = Tst * (double(u.N1) / (2 * kPi));
-        double x2 = ph * (double(u.N2) / (2 * kPi));
-        x1 -= std::floor(x1 / u.N1) * u.N1;
-        x2 -= std::floor(x2 / u.N2) * u.N2;
-        int b1 = int(std::ceil(x1 - half)), b2 = int(std::ceil(x2 - half));
-        if (b1 < 0) { x1 += u.N1; b1 = int(std::ceil(x1 - half)); }
-        if (b2 < 0) { x2 += u.N2; b2 = int(std::ceil(x2 - half)); }
-        if (b1 < 0 || b1 >= u.N1 || b2 < 0 || b2 >= u.N2) { set_error("point maps outside the fine grid"); return kEArg; }
-        X1[i] = x1;
-        X2[i] = x2;
-        const int s1 = b1 / S4, s2 = b2 / S4;
-        const int lb = ((b1 % S4) / u.S) * 4 + (b2 % S4) / u.S;
-        key[i] = (s1 * u.ns2 + s2) * 16 + lb;
-    }
+    // Points onto the fine grid, bin keys and counts, the reached rows: on the device, then a
+    // stable sort of the keys (points of a bin keep their input order) and the gather of the
+    // coordinates into bin order.
     const int nkeys = u.ns1 * u.ns2 * 16;
-    std::vector<int64_t> cnt(int64_t(nkeys) + 1, 0);
-    for (int64_t i = 0; i < M; ++i) cnt[key[i] + 1]++;
-    for (int k = 0; k < nkeys; ++k) cnt[k + 1] += cnt[k];
-    std::vector<int32_t> perm(M);
+    std::vector<int32_t> cnt32(nkeys);
+    std::vector<uint8_t> on(u.N1);
     {
-        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-        for (int64_t i = 0; i < M; ++i) perm[pos[key[i]]++] = int32_t(i);
+        PointMap q;
+        q.kind = p.geom.kind;
+        q.flip = p.flip;
+        q.north = p.geom.thlo <= std::sqrt(DBL_EPSILON);
+        q.thmax = p.geom.h * (1 + p.geom.epsapo);
+        q.lo = p.geom.thlo - p.geom.h * p.geom.epsapo;
+        q.period = p.geom.period;
+        q.s1 = double(u.N1) / (2 * kPi);
+        q.s2 = double(u.N2) / (2 * kPi);
+        q.half = 0.5 * u.w;
+        q.N1 = u.N1; q.N2 = u.N2; q.S = u.S; q.S4 = S4; q.ns2 = u.ns2; q.w = u.w;
+        int bits = 1;
+        while ((int64_t(1) << bits) < nkeys) ++bits;
+        DevScratch tmp;
+        double *dth = nullptr, *dph = nullptr, *X1u = nullptr, *X2u = nullptr;
+        int32_t *key = nullptr, *skey = nullptr, *idx = nullptr, *dcnt = nullptr, *err = nullptr;
+        uint8_t *don = nullptr;
+        int64_t tb = 0;
+        if ((st = tmp.take(&dth, M, tb)) != kOk || (st = tmp.take(&dph, M, tb)) != kOk ||
+            (st = tmp.take(&X1u, M, tb)) != kOk || (st = tmp.take(&X2u, M, tb)) != kOk ||
+            (st = tmp.take(&key, M, tb)) != kOk || (st = tmp.take(&skey, M, tb)) != kOk ||
+            (st = tmp.take(&idx, M, tb)) != kOk || (st = tmp.take(&dcnt, nkeys, tb)) != kOk ||
+            (st = tmp.take(&err, 1, tb)) != kOk || (st = tmp.take(&don, u.N1, tb)) != kOk)
+            return st;
+        if ((st = dalloc(&u.X1, M, by)) != kOk) return st;
+        if ((st = dalloc(&u.X2, M, by)) != kOk) return st;
+        if ((st = dalloc(&u.perm, M, by)) != kOk) return st;
+        PFSHT_HIP_TRY(hipMemcpyAsync(dth, theta, sizeof(double) * M, hipMemcpyHostToDevice, nullptr));
+        PFSHT_HIP_TRY(hipMemcpyAsync(dph, phi, sizeof(double) * M, hipMemcpyHostToDevice, nullptr));
+        PFSHT_HIP_TRY(hipMemsetAsync(dcnt, 0, sizeof(int32_t) * nkeys, nullptr));
+        PFSHT_HIP_TRY(hipMemsetAsync(don, 0, u.N1, nullptr));
+        PFSHT_HIP_TRY(hipMemsetAsync(err, 0, sizeof(int32_t), nullptr));
+        launch_map_points(dth, dph, M, q, X1u, X2u, key, idx, dcnt, don, err, nullptr);
+        PFSHT_HIP_TRY(hipGetLastError());
+        size_t sb = 0;
+        PFSHT_HIP_TRY(sort_pairs(nullptr, sb, key, skey, idx, u.perm, int(M), bits, nullptr));
+        void *sortmem = nullptr;
+        if ((st = tmp.take(reinterpret_cast<char **>(&sortmem), sb, tb)) != kOk) return st;
+        PFSHT_HIP_TRY(sort_pairs(sortmem, sb, key, skey, idx, u.perm, int(M), bits, nullptr));
+        launch_gather_points(X1u, X2u, u.perm, M, u.X1, u.X2, nullptr);
+        PFSHT_HIP_TRY(hipGetLastError());
+        int32_t herr = 0;
+        PFSHT_HIP_TRY(hipMemcpy(&herr, err, sizeof(int32_t), hipMemcpyDeviceToHost));
+        PFSHT_HIP_TRY(hipMemcpy(cnt32.data(), dcnt, sizeof(int32_t) * nkeys, hipMemcpyDeviceToHost));
+        PFSHT_HIP_TRY(hipMemcpy(on.data(), don, u.N1, hipMemcpyDeviceToHost));
+        if (herr) { set_error("point maps outside the fine grid"); return kEArg; }
     }
-    std::vector<double> X1s(M), X2s(M);
-    for (int64_t q = 0; q < M; ++q) { X1s[q] = X1[perm[q]]; X2s[q] = X2[perm[q]]; }
+    std::vector<int64_t> cnt(int64_t(nkeys) + 1, 0);
+    for (int k = 0; k < nkeys; ++k) cnt[k + 1] = cnt[k] + cnt32[k];
     // Fine-grid rows the kernels reach: the apply stores and transforms only the smallest cyclic
     // interval holding them (about half of the rows for a polar cap, whose points fill half the
     // doubled colatitude period).
     {
-        std::vector<char> on(u.N1, 0);
-        for (int64_t i = 0; i < M; ++i) {
-            const int b1 = int(std::ceil(X1[i] - half));
-            for (int t = 0; t < u.w; ++t) on[(b1 + t) % u.N1] = 1;
-        }
         int best = 0, best_end = 0;  // longest cyclic run of unreached rows, and the row after it
         for (int pass = 0, run = 0; pass < 2; ++pass)
             for (int r = 0; r < u.N1; ++r) {
@@ -609,9 +624,6 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
         pc1[i] = make_double2(std::cos(ang) / Nn * c1[i], std::sin(ang) / Nn * c1[i]);
         c1c[i] = make_double2(c1[i], 0.0);
     }
-    if ((st = upload(&u.X1, X1s, by)) != kOk) return st;
-    if ((st = upload(&u.X2, X2s, by)) != kOk) return st;
-    if ((st = upload(&u.perm, perm, by)) != kOk) return st;
     if ((st = upload(&u.pc1, pc1, by)) != kOk) return st;
     if ((st = upload(&u.c1, c1c, by)) != kOk) return st;
     if ((st = upload(&u.c2, c2, by)) != kOk) return st;

@@ -189,4 +189,21 @@ void launch_scatter_field(double2 *f, const double2 *fg, const int32_t *idx, int
 void launch_gather_field(double2 *fg, const double2 *f, const int32_t *idx, int64_t Mg, int64_t M, int B,
                          hipStream_t s);
 
+// plan construction on the device
+struct PointMap {  // theta -> fine-grid row coordinate, phi -> column coordinate, and the bin keys
+    int kind, flip, north;  // geometry kind; southern reflection; pole cap at the north pole
+    double thmax;           // pole: h (1 + epsapo)
+    double lo, period;      // interior: thlo - h epsapo, and the period
+    double s1, s2;          // N1 / (2 pi), N2 / (2 pi)
+    double half;            // w / 2
+    int N1, N2, S, S4, ns2, w;
+};
+void launch_coef_tables(int L, int mmax, const int64_t *abofs, double2 *ab, double2 *gc, hipStream_t s);
+void launch_map_points(const double *theta, const double *phi, int64_t M, const PointMap &q, double *X1, double *X2,
+                       int32_t *key, int32_t *idx, int32_t *cnt, uint8_t *on, int32_t *err, hipStream_t s);
+void launch_gather_points(const double *X1, const double *X2, const int32_t *perm, int64_t M, double *X1s,
+                          double *X2s, hipStream_t s);
+hipError_t sort_pairs(void *tmp, size_t &tmp_bytes, const int32_t *kin, int32_t *kout, const int32_t *vin,
+                      int32_t *vout, int n, int bits, hipStream_t s);
+
 }  // namespace pfsht
