@@ -7,6 +7,8 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <memory>
+#include <thread>
 
 namespace pfsht {
 
@@ -266,131 +268,274 @@ struct GapResult {
     double lo, hi, gap, at;
 };
 
-// _max_gap over the values v[g[i]] (src/Box.jl:257-296); bk is bucket scratch of >= |g| - 1 pairs
-GapResult max_gap(const std::vector<double> &v, const std::vector<int32_t> &g,
-                  std::vector<std::pair<double, double>> &bk) {
+// Worker threads for the O(n) scans of the partition: at most 16, one when the group is small.
+int scan_threads(int64_t n) {
+    if (n < (int64_t(1) << 16)) return 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return int(std::max(1u, std::min(16u, hw)));
+}
+
+template <class F>
+void run_threads(int T, F &&f) {
+    if (T <= 1) { f(0); return; }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int t = 1; t < T; ++t) th.emplace_back(f, t);
+    f(0);
+    for (auto &x : th) x.join();
+}
+
+// _max_gap over the values v[g[i]] (src/Box.jl:257-296): the largest distance between neighbours
+// in sorted order and the value above it, by n - 1 buckets of width w = (hi - lo) / (n - 1) (the
+// largest gap is at least w, a gap inside one bucket is below w). A value within rounding of a
+// bucket edge may land in the neighbouring bucket, which cannot move a gap of at least w.
+// In parallel: each thread buckets a chunk of the values and counts them per thread-owned bucket
+// range; the (bucket, value) pairs are then grouped by range; each thread fills the buckets of its
+// range and finds the first largest gap inside it; the ranges are joined left to right, so the
+// first largest gap wins as in one sequential scan.
+struct Bucket {
+    double lo, hi;
+};
+struct Item {
+    int64_t k;
+    double x;
+};
+struct GapScratch {  // n buckets, n bucket indices, n items: reused across the scans of a partition
+    std::unique_ptr<Bucket[]> bk;
+    std::unique_ptr<int64_t[]> key;
+    std::unique_ptr<Item[]> item;
+    explicit GapScratch(int64_t n)
+        : bk(new Bucket[size_t(n)]), key(new int64_t[size_t(n)]), item(new Item[size_t(n)]) {}
+};
+
+GapResult max_gap(const double *v, const std::vector<int32_t> &g, GapScratch &S) {
     const int64_t n = int64_t(g.size());
-    double lo = v[g[0]], hi = v[g[0]];
-    for (int64_t i = 1; i < n; ++i) {
-        lo = std::min(lo, v[g[i]]);
-        hi = std::max(hi, v[g[i]]);
+    const int T = scan_threads(n);
+    const int32_t *gi = g.data();
+    std::vector<double> tlo(T, INFINITY), thi(T, -INFINITY);
+    run_threads(T, [&](int t) {
+        double a = INFINITY, b = -INFINITY;
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            a = std::min(a, v[gi[i]]);
+            b = std::max(b, v[gi[i]]);
+        }
+        tlo[t] = a;
+        thi[t] = b;
+    });
+    double lo = v[gi[0]], hi = v[gi[0]];
+    for (int t = 0; t < T; ++t) {
+        lo = std::min(lo, tlo[t]);
+        hi = std::max(hi, thi[t]);
     }
     if (n < 2 || hi <= lo) return {lo, hi, 0.0, lo};
     const int64_t nb = n - 1;
-    // bucket k ~ [lo + k w, lo + (k + 1) w): a value within rounding of a boundary may land in the
-    // neighbouring bucket, which cannot move a gap of at least w, and the largest gap is at least w
     const double iw = double(nb) / (hi - lo);
-    for (int64_t k = 0; k < nb; ++k) bk[k] = {INFINITY, -INFINITY};
-    for (int64_t i = 0; i < n; ++i) {
-        const double x = v[g[i]];
-        int64_t k = int64_t((x - lo) * iw);
-        k = std::min(k, nb - 1);
-        bk[k].first = std::min(bk[k].first, x);
-        bk[k].second = std::max(bk[k].second, x);
+    // bucket range r owns buckets [k0(r), k0(r + 1)), k0(r) = ceil(r nb / T), so range(k) = k T / nb
+    auto k0 = [&](int r) { return (int64_t(r) * nb + T - 1) / T; };
+    std::vector<int64_t> cnt(size_t(T) * T, 0);  // cnt[t * T + r]: values of chunk t in range r
+    int64_t *key = S.key.get();
+    run_threads(T, [&](int t) {
+        std::vector<int64_t> c(T, 0);
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            const double x = v[gi[i]];
+            int64_t k = int64_t((x - lo) * iw);
+            k = std::min(k, nb - 1);
+            key[i] = k;
+            ++c[k * T / nb];
+        }
+        for (int r = 0; r < T; ++r) cnt[size_t(t) * T + r] = c[r];
+    });
+    std::vector<int64_t> ofs(size_t(T) * T + 1, 0);  // range-major offsets: (r, t)
+    {
+        int64_t o = 0;
+        for (int r = 0; r < T; ++r)
+            for (int t = 0; t < T; ++t) {
+                ofs[size_t(t) * T + r] = o;
+                o += cnt[size_t(t) * T + r];
+            }
+        ofs[size_t(T) * T] = o;
     }
+    Item *item = S.item.get();
+    run_threads(T, [&](int t) {
+        std::vector<int64_t> w(ofs.begin() + size_t(t) * T, ofs.begin() + size_t(t + 1) * T);
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            const int64_t k = key[i];
+            item[w[k * T / nb]++] = {k, v[gi[i]]};
+        }
+    });
+    struct Part {
+        bool any = false;
+        double first = 0, last = 0, gap = 0, at = 0;
+    };
+    std::vector<Part> part(T);
+    Bucket *bk = S.bk.get();
+    run_threads(T, [&](int r) {
+        const int64_t b0 = k0(r), b1 = k0(r + 1);
+        for (int64_t k = b0; k < b1; ++k) bk[k] = {INFINITY, -INFINITY};
+        const int64_t i0 = ofs[size_t(0) * T + r], i1 = r + 1 < T ? ofs[size_t(0) * T + r + 1] : n;
+        for (int64_t i = i0; i < i1; ++i) {
+            Bucket &b = bk[item[i].k];
+            b.lo = std::min(b.lo, item[i].x);
+            b.hi = std::max(b.hi, item[i].x);
+        }
+        Part P;
+        double prev = 0;
+        for (int64_t k = b0; k < b1; ++k) {
+            if (!(bk[k].lo <= bk[k].hi)) continue;
+            if (!P.any) {
+                P.any = true;
+                P.first = bk[k].lo;
+            } else {
+                const double d = bk[k].lo - prev;
+                if (d > P.gap) { P.gap = d; P.at = bk[k].lo; }
+            }
+            prev = bk[k].hi;
+        }
+        P.last = prev;
+        part[r] = P;
+    });
     double gap = 0.0, at = hi, prev = lo;
-    for (int64_t k = 0; k < nb; ++k) {
-        if (!(bk[k].first <= bk[k].second)) continue;
-        const double d = bk[k].first - prev;
-        if (d > gap) { gap = d; at = bk[k].first; }
-        prev = bk[k].second;
+    for (const Part &P : part) {
+        if (!P.any) continue;
+        const double d = P.first - prev;
+        if (d > gap) { gap = d; at = P.first; }
+        if (P.gap > gap) { gap = P.gap; at = P.at; }
+        prev = P.last;
     }
     return {lo, hi, gap, at};
 }
 
-// _group_box (src/Box.jl:298-319): colatitude ends (not clamped) and the longitude arc
-Box group_box(const std::vector<double> &th, const std::vector<double> &ph, const std::vector<int32_t> &g,
-              std::vector<std::pair<double, double>> &bk) {
-    double lo = th[g[0]], hi = th[g[0]];  // only the colatitude ends are needed here
-    for (int32_t i : g) {
-        lo = std::min(lo, th[i]);
-        hi = std::max(hi, th[i]);
+// _split_below (src/Box.jl:356-374) fused with the colatitude ends of the two halves (the part of
+// _group_box, src/Box.jl:298-319, that cost_stretch reads): a keeps the points below thresh, b the
+// rest, both in the group's order. False when either side would be empty.
+struct Halves {
+    std::vector<int32_t> a, b;
+    double alo = INFINITY, ahi = -INFINITY, blo = INFINITY, bhi = -INFINITY;
+};
+bool split_below(const double *v, const double *th, const std::vector<int32_t> &g, double thresh, Halves &h) {
+    const int64_t n = int64_t(g.size());
+    const int T = scan_threads(n);
+    std::vector<int64_t> na(T + 1, 0);
+    std::vector<double> alo(T, INFINITY), ahi(T, -INFINITY), blo(T, INFINITY), bhi(T, -INFINITY);
+    run_threads(T, [&](int t) {  // count and ends per chunk
+        int64_t c = 0;
+        double a0 = INFINITY, a1 = -INFINITY, b0 = INFINITY, b1 = -INFINITY;
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            const int32_t j = g[i];
+            const double x = th[j];
+            if (v[j] < thresh) {
+                ++c;
+                a0 = std::min(a0, x);
+                a1 = std::max(a1, x);
+            } else {
+                b0 = std::min(b0, x);
+                b1 = std::max(b1, x);
+            }
+        }
+        na[t + 1] = c;
+        alo[t] = a0; ahi[t] = a1; blo[t] = b0; bhi[t] = b1;
+    });
+    for (int t = 0; t < T; ++t) na[t + 1] += na[t];
+    h = Halves();
+    h.a.resize(size_t(na[T]));
+    h.b.resize(size_t(n - na[T]));
+    run_threads(T, [&](int t) {  // each chunk writes its points at its offsets, in order
+        int64_t ia = na[t], ib = n * t / T - na[t];
+        for (int64_t i = n * t / T; i < n * (t + 1) / T; ++i) {
+            const int32_t j = g[i];
+            if (v[j] < thresh) h.a[ia++] = j;
+            else h.b[ib++] = j;
+        }
+    });
+    for (int t = 0; t < T; ++t) {
+        h.alo = std::min(h.alo, alo[t]); h.ahi = std::max(h.ahi, ahi[t]);
+        h.blo = std::min(h.blo, blo[t]); h.bhi = std::max(h.bhi, bhi[t]);
     }
-    Box b;
-    b.thlo = lo;
-    b.thhi = hi;
-    const double ptol = std::sqrt(2.220446049250313e-16);
-    b.north = lo <= ptol;
-    b.south = hi >= kPi - ptol;
-    if (b.north || b.south) { b.phistart = 0.0; b.phiext = 2 * kPi; return b; }
-    const GapResult f = max_gap(ph, g, bk);
-    const double wrap = 2 * kPi - f.hi + f.lo;
-    double gap = f.gap, start = f.at;
-    if (wrap >= gap) { gap = wrap; start = f.lo; }
-    const double ext = 2 * kPi - gap;
-    if (ext <= 0) { b.phistart = f.lo; b.phiext = 0.0; }
-    else if (ext >= 2 * kPi) { b.phistart = 0.0; b.phiext = 2 * kPi; }
-    else { b.phistart = start; b.phiext = ext; }
-    return b;
-}
-
-// _split_below (src/Box.jl:356-374): false when either side would be empty
-bool split_below(const std::vector<double> &v, const std::vector<int32_t> &g, double thresh,
-                 std::vector<int32_t> &a, std::vector<int32_t> &b) {
-    a.clear();
-    b.clear();
-    for (int32_t i : g) (v[i] < thresh ? a : b).push_back(i);
-    return !a.empty() && !b.empty();
+    return !h.a.empty() && !h.b.empty();
 }
 
 // _best_gap_split (src/Box.jl:325-343): the larger of the colatitude gap and the longitude gap
 // weighted by the ring radius
-bool best_gap_split(const std::vector<double> &th, const std::vector<double> &ph, const std::vector<int32_t> &g,
-                    std::vector<std::pair<double, double>> &bk, std::vector<int32_t> &a, std::vector<int32_t> &b) {
-    const GapResult t = max_gap(th, g, bk);
-    const GapResult f = max_gap(ph, g, bk);
+bool best_gap_split(const double *th, const double *ph, const std::vector<int32_t> &g, GapScratch &S, Halves &h) {
+    const GapResult t = max_gap(th, g, S);
+    const GapResult f = max_gap(ph, g, S);
     const double e = std::sqrt(2.220446049250313e-16);
     const double mid = std::min(std::max(0.5 * (t.lo + t.hi), e), kPi - e);
     const double wphi = f.gap * std::sin(mid);
     if (t.gap >= wphi) {
         if (!(t.gap > 0)) return false;
-        return split_below(th, g, t.at, a, b);
+        return split_below(th, th, g, t.at, h);
     }
     if (!(f.gap > 0)) return false;
-    return split_below(ph, g, f.at, a, b);
+    return split_below(ph, th, g, f.at, h);
 }
+
+// A group's best cut, computed once per group: its two halves, their costs and the gain.
+struct Candidate {
+    bool done = false, ok = false;
+    double gain = 0, ca = 0, cb = 0;
+    Halves h;
+};
 }  // namespace
 
+// partition_boxes_costed (src/Box.jl:173-235): starting from one group, repeatedly make the cut
+// with the largest positive fall in counted cost over all groups (the first such group on ties),
+// up to maxboxes groups. A group's candidate cut does not change while the group does, so each is
+// evaluated once.
 Partition partition_boxes(const double *theta, const double *phi, int64_t n, int L, int B, double tol,
                           int maxboxes) {
-    std::vector<double> th(theta, theta + n), ph(n);
-    for (int64_t i = 0; i < n; ++i) ph[i] = jmod(phi[i], 2 * kPi);
-    std::vector<std::pair<double, double>> bk(size_t(std::max<int64_t>(n, 1)));
-    auto cost = [&](const Box &b, int64_t np) { return cost_stretch(b.thlo, b.thhi, np, L, B, tol); };
+    // longitudes in [0, 2 pi) (fmod returns x itself there); a copy only when some lie outside
+    std::vector<double> phc;
+    const double *th = theta, *ph = phi;
+    bool inrange = true;
+    for (int64_t i = 0; i < n && inrange; ++i) inrange = phi[i] >= 0 && phi[i] < 2 * kPi;
+    if (!inrange) {
+        phc.resize(n);
+        for (int64_t i = 0; i < n; ++i) phc[i] = jmod(phi[i], 2 * kPi);
+        ph = phc.data();
+    }
+    GapScratch scratch(std::max<int64_t>(n, 1));
+    auto cost = [&](double lo, double hi, int64_t np) { return cost_stretch(lo, hi, np, L, B, tol); };
     Partition P;
     std::vector<int32_t> all(n);
     for (int64_t i = 0; i < n; ++i) all[i] = int32_t(i);
+    double lo0 = th[0], hi0 = th[0];
+    for (int64_t i = 1; i < n; ++i) { lo0 = std::min(lo0, th[i]); hi0 = std::max(hi0, th[i]); }
     P.groups.push_back(std::move(all));
-    P.costs.push_back(cost(group_box(th, ph, P.groups[0], bk), n));
-    std::vector<int32_t> a, b;
+    P.costs.push_back(cost(lo0, hi0, n));
+    std::vector<Candidate> cand(1);
     while (int(P.groups.size()) < maxboxes) {
         double best_gain = 0.0;
         int best_at = -1;
-        std::vector<int32_t> ba, bb;
-        double ca = 0, cb = 0;
         for (size_t gi = 0; gi < P.groups.size(); ++gi) {
-            const auto &g = P.groups[gi];
-            if (g.size() < 2) continue;
-            if (!best_gap_split(th, ph, g, bk, a, b)) continue;
-            const double xa = cost(group_box(th, ph, a, bk), int64_t(a.size()));
-            const double xb = cost(group_box(th, ph, b, bk), int64_t(b.size()));
-            const double gain = P.costs[gi] - (xa + xb);
-            if (gain > best_gain) {
-                best_gain = gain;
+            Candidate &c = cand[gi];
+            if (!c.done) {
+                c.done = true;
+                const auto &g = P.groups[gi];
+                if (g.size() >= 2 && best_gap_split(th, ph, g, scratch, c.h)) {
+                    c.ok = true;
+                    c.ca = cost(c.h.alo, c.h.ahi, int64_t(c.h.a.size()));
+                    c.cb = cost(c.h.blo, c.h.bhi, int64_t(c.h.b.size()));
+                    c.gain = P.costs[gi] - (c.ca + c.cb);
+                }
+                if (!c.ok) c.h = Halves();
+            }
+            if (c.ok && c.gain > best_gain) {
+                best_gain = c.gain;
                 best_at = int(gi);
-                ba = a;
-                bb = b;
-                ca = xa;
-                cb = xb;
             }
         }
         if (best_at < 0) break;
+        Candidate c = std::move(cand[best_at]);
         P.groups.erase(P.groups.begin() + best_at);
         P.costs.erase(P.costs.begin() + best_at);
-        P.groups.push_back(std::move(ba));
-        P.costs.push_back(ca);
-        P.groups.push_back(std::move(bb));
-        P.costs.push_back(cb);
+        cand.erase(cand.begin() + best_at);
+        P.groups.push_back(std::move(c.h.a));
+        P.costs.push_back(c.ca);
+        P.groups.push_back(std::move(c.h.b));
+        P.costs.push_back(c.cb);
+        cand.emplace_back();
+        cand.emplace_back();
     }
     return P;
 }

@@ -71,3 +71,29 @@ def test_select_method_oracle(O):
     assert O.select_method(th, ph, 512, 1e-8) == "multibox"
     # one compact cap stays on the stretched leaf
     assert O.select_method(th[:n1], ph[:n1], 512, 1e-8) == "stretch"
+
+
+@pytest.mark.parametrize("name", ["antipodal", "three_patches", "cap_band"])
+def test_partition_threaded_scans_match_oracle(O, name):
+    # groups of >= 65536 points take the threaded bucket scans; the result must not change
+    rng = np.random.default_rng(23)
+    n = 240_000
+    if name == "antipodal":
+        th = np.concatenate([0.2 * np.sqrt(rng.uniform(size=n // 2)), np.pi - 0.2 * np.sqrt(rng.uniform(size=n // 2))])
+        ph = 2 * np.pi * rng.uniform(size=n)
+        L = 1024
+    elif name == "three_patches":
+        k = n // 3
+        th = np.concatenate([rng.uniform(0.5, 0.7, k), rng.uniform(1.5, 1.7, k), rng.uniform(2.5, 2.7, k)])
+        ph = np.concatenate([rng.uniform(0.5, 0.8, k), rng.uniform(3.0, 3.3, k), rng.uniform(5.0, 5.4, k)])
+        L = 1024
+    else:
+        th = np.where(np.arange(n) % 2 == 0, np.arccos(1 - rng.uniform(size=n) * (1 - np.cos(0.2))),
+                      rng.uniform(1.0, 1.3, n))
+        ph = rng.uniform(-np.pi, 3 * np.pi, n)  # longitudes outside [0, 2 pi) too
+        L = 1024
+    got = _lib_groups(th, ph, L)
+    ref, _ = O.partition_boxes(th, ph, L, 1, 1e-8)
+    assert len(got) == len(ref) >= 2
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
