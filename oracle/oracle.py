@@ -98,6 +98,15 @@ def lib():
             "or_plan_last_Fhat": (_I, [_P, _P]),
             "or_plan_last_times": (None, [_P, _P]),
             "or_plan_points": (_I, [_P, _P, _P]),
+            "or_wigner_d": (_D, [_I, _I, _I, _D]),
+            "or_sylm": (None, [_I, _I, _I, _D, _D, _P, _P]),
+            "or_spin_rings_create": (_P, [_P, _I, _I, _I, _D, _I, _I]),
+            "or_spin_rings_destroy": (None, [_P]),
+            "or_spin_rings_mmax": (_I, [_P]),
+            "or_spin_rings_work": (_I64, [_P]),
+            "or_spin_rings_from_alm": (_I, [_P, _P, _P, _I]),
+            "or_spin_alm_from_rings": (_I, [_P, _P, _P, _I]),
+            "or_plan_create_spin": (_P, [_P, _P, _I64, _I, _D, _I, _I, _I, _I, _I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -135,6 +144,71 @@ def sph_mode(l: int, m: int):
     """0-based (row, col) of (l, m) in the (L+1) x (2L+1) array (Layout.jl:20)."""
     am = abs(m)
     return (l - am, 2 * am + (1 if m >= 0 else 0) - 1)
+
+
+def spin_coeff_index(l: int, m: int, L: int):
+    """0-based (row, col) of a spin coefficient in the dense (l, m + L) layout (Layout.jl:28-29)."""
+    return (l, m + L)
+
+
+def spin_valid_mask(L: int, s: int) -> np.ndarray:
+    """Slots of the spin layout the transform reads and writes: |m| <= l and l >= |s|."""
+    l = np.arange(L + 1)[:, None]
+    m = np.arange(-L, L + 1)[None, :]
+    return (np.abs(m) <= l) & (l >= abs(s))
+
+
+# --- Spin.jl ---------------------------------------------------------------------------------
+def wigner_d(l: int, m: int, n: int, beta: float) -> float:
+    """wigner_d(l, m, n, beta) (Spin.jl:61-137)."""
+    return lib().or_wigner_d(int(l), int(m), int(n), float(beta))
+
+
+def sylm(s: int, l: int, m: int, theta: float, phi: float) -> complex:
+    """sYlm(s, l, m, theta, phi) = N_l d^l_{m,-s}(theta) e^{i m phi} (Spin.jl:144-149)."""
+    re, im = C.c_double(), C.c_double()
+    lib().or_sylm(int(s), int(l), int(m), float(theta), float(phi), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+class SpinRings:
+    """SpinRings (Spin.jl:170-270) with spin_rings_from_alm!/spin_alm_from_rings! (:316-334);
+    coefficients in the dense (l, m + L) layout."""
+
+    def __init__(self, theta, L: int, spin: int, eps: float, ntrans: int = 1, reflect: bool = False):
+        self.theta = _f64(theta)
+        self.L, self.s, self.B, self.n = L, spin, ntrans, len(self.theta)
+        self.h = lib().or_spin_rings_create(_ptr(self.theta), self.n, L, int(spin), float(eps), ntrans,
+                                            int(reflect))
+        if not self.h:
+            raise ValueError(_err())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_spin_rings_destroy(self.h)
+            self.h = None
+
+    @property
+    def work(self) -> int:
+        return int(lib().or_spin_rings_work(self.h))
+
+    @property
+    def mmax(self) -> int:
+        return int(lib().or_spin_rings_mmax(self.h))
+
+    def rings_from_alm(self, alm, nthreads: int = 1) -> np.ndarray:
+        L, B = self.L, self.B
+        a = _c128(alm, (L + 1, 2 * L + 1, B))
+        G = np.zeros((self.n, 2 * L + 1, B), np.complex128, order="F")
+        lib().or_spin_rings_from_alm(self.h, _ptr(a), _ptr(G), nthreads)
+        return G
+
+    def alm_from_rings(self, G, nthreads: int = 1) -> np.ndarray:
+        L, B = self.L, self.B
+        g = _c128(G, (self.n, 2 * L + 1, B))
+        a = np.zeros((L + 1, 2 * L + 1, B), np.complex128, order="F")
+        lib().or_spin_alm_from_rings(self.h, _ptr(g), _ptr(a), nthreads)
+        return a[:, :, 0] if B == 1 else a
 
 
 # --- Legendre.jl -----------------------------------------------------------------------------
@@ -336,14 +410,15 @@ class Plan:
     adjoint (mul!(alm, plan', f)) — Plan.jl:84-148."""
 
     def __init__(self, theta, phi, lmax, tol=1e-10, ntrans=1, method="auto", nufft="es",
-                 nthreads=1):
+                 nthreads=1, spin=0):
         self.theta, self.phi = _f64(theta), _f64(phi)
         if len(self.theta) != len(self.phi):
             raise ValueError("theta and phi must have the same length")
         self.L, self.B, self.M, self.tol = int(lmax), int(ntrans), len(self.theta), float(tol)
-        self.h = lib().or_plan_create(_ptr(self.theta), _ptr(self.phi), self.M, self.L,
-                                      self.tol, self.B, METHODS[method], NUFFTS[nufft],
-                                      int(nthreads))
+        self.spin = int(spin)
+        self.h = lib().or_plan_create_spin(_ptr(self.theta), _ptr(self.phi), self.M, self.L,
+                                           self.tol, self.B, self.spin, METHODS[method], NUFFTS[nufft],
+                                           int(nthreads))
         if not self.h:
             raise ValueError(_err())
         inf = PlanInfoC()

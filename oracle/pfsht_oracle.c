@@ -21,7 +21,12 @@
  *    (Cost.jl:192-197) and NUFSHT are outside this path.
  *
  * Parity pins (tests/test_oracle_*.py): mpmath ALF goldens, closed-form Ylm, scipy sph_harm_y,
- * DFS transpose, NUFFT vs direct sum, Stretch vs Direct, adjoint identity, counted work.
+ * DFS transpose, NUFFT vs direct sum, Stretch vs Direct, adjoint identity, counted work; for spin,
+ * Wigner-d closed forms and mpmath values of Wigner's explicit sum, the engine against single-value
+ * wigner_d, exact poles, reflection, spin Direct against explicit sYlm sums, spin Stretch vs Direct.
+ *
+ * The spin ring engine gives every ring its own power-of-two exponent where the reference shares one
+ * per chunk of rings (Spin.jl:272-310); the values agree to round-off.
  */
 #include "pfsht_oracle.h"
 
@@ -478,6 +483,280 @@ int or_alm_from_rings(const or_rings *e, const or_cplx *G, or_cplx *alm, int nth
     int64_t na = (int64_t)(e->L + 1) * (2 * e->L + 1) * e->B;
     memset(alm, 0, sizeof(or_cplx) * na);
     walk_rings(0, NULL, G, alm, NULL, e, nthreads);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Spin-weighted harmonics (Spin.jl): sY_lm(theta, phi) = N_l d^l_{m,-s}(theta) e^{i m phi},
+ * N_l = sqrt((2l+1)/4pi), with d^l by the three-term recurrence in degree at fixed (m, -s), seeded
+ * at l0 = max(|m|, |s|) from the extremal closed form in log2 magnitude.
+ * ------------------------------------------------------------------------------------------- */
+static double spin_Nl(int l) { return sqrt((double)(2 * l + 1) / (4 * PI)); } /* Spin.jl:21 */
+
+/* log sqrt(C(2l, l-a)) (Spin.jl:26-31) */
+static double half_lbinom(int l, int a) {
+    return (lgamma((double)(2 * l + 1)) - lgamma((double)(l - a + 1)) - lgamma((double)(l + a + 1))) / 2;
+}
+
+/* d^l_{a,b}(beta) at |b| = l as (log|d|, sign) (Spin.jl:39-53) */
+static double d_extremal(int l, int a, int b, double beta, double *sgn) {
+    double c = cos(beta / 2), s = sin(beta / 2);
+    double lb = half_lbinom(l, a);
+    int pc, ps;
+    double sg;
+    if (b == l) { pc = l + a; ps = l - a; sg = 1.0; }
+    else { pc = l - a; ps = l + a; sg = ((l - a) & 1) ? -1.0 : 1.0; }
+    double lc = pc == 0 ? 0.0 : (c > 0 ? (double)pc * log(c) : -INFINITY);
+    double ls = ps == 0 ? 0.0 : (s > 0 ? (double)ps * log(s) : -INFINITY);
+    *sgn = sg;
+    return lb + lc + ls;
+}
+
+/* wigner_d(l, m, n, beta) (Spin.jl:61-137): exact endpoints, extremal seed, block-scaled
+ * recurrence renormalised every RESCALE_STRIDE degrees. */
+double or_wigner_d(int l, int m, int n, double beta) {
+    int am = m < 0 ? -m : m, an = n < 0 ? -n : n;
+    if (am > l || an > l) return 0.0;
+    double tiny = sqrt(DBL_EPSILON);
+    if (beta <= tiny) return m == n ? 1.0 : 0.0;
+    if (beta >= PI - tiny) return m == -n ? (((l - n) & 1) ? -1.0 : 1.0) : 0.0;
+    int l0 = am > an ? am : an;
+    double lg, sg;
+    if (an == l0) lg = d_extremal(l0, m, n == l0 ? l0 : -l0, beta, &sg);
+    else {
+        lg = d_extremal(l0, n, m == l0 ? l0 : -l0, beta, &sg);
+        if ((n - m) & 1) sg = -sg;
+    }
+    if (!isfinite(lg)) return 0.0;
+    double E = ceil(lg / log(2.0));
+    double d1 = sg * exp(lg - E * log(2.0));
+    if (l == l0) return d1 * exp2(E);
+    double cb = cos(beta), d0 = 0.0;
+    int Lc = l0;
+    if (l0 == 0) {
+        d0 = d1;
+        d1 = d1 * cb;
+        Lc = 1;
+        if (l == 1) return d1 * exp2(E);
+    }
+    int steps = 0;
+    while (Lc < l) {
+        int L1 = Lc + 1;
+        double sm1 = sqrt(fmax(0.0, (double)(L1 * L1 - m * m)));
+        double sn1 = sqrt(fmax(0.0, (double)(L1 * L1 - n * n)));
+        double den = (double)Lc * sm1 * sn1;
+        if (den <= 0) { d0 = d1; d1 = 0.0; Lc = L1; continue; }
+        double sm0 = sqrt(fmax(0.0, (double)(Lc * Lc - m * m)));
+        double sn0 = sqrt(fmax(0.0, (double)(Lc * Lc - n * n)));
+        double c1 = (double)(2 * Lc + 1) * ((double)Lc * (double)L1 * cb - (double)m * (double)n) / den;
+        double c0 = (double)L1 * sm0 * sn0 / den;
+        double dn = c1 * d1 - c0 * d0;
+        d0 = d1;
+        d1 = dn;
+        Lc = L1;
+        if (++steps == RESCALE_STRIDE) {
+            steps = 0;
+            double mx = fmax(fabs(d1), fabs(d0));
+            if (mx > 0 && isfinite(mx)) {
+                double sh = ceil(log2(mx));
+                if (sh != 0) { double f = exp2(-sh); d1 *= f; d0 *= f; E += sh; }
+            }
+        }
+    }
+    return d1 * exp2(E);
+}
+
+/* sYlm(s, l, m, theta, phi) (Spin.jl:144-149) */
+void or_sylm(int s, int l, int m, double theta, double phi, double *re, double *im) {
+    int as = s < 0 ? -s : s;
+    if (as > l) { *re = 0; *im = 0; return; }
+    double v = spin_Nl(l) * or_wigner_d(l, m, -s, theta);
+    *re = v * cos((double)m * phi);
+    *im = v * sin((double)m * phi);
+}
+
+struct or_spin_rings { /* SpinRings (Spin.jl:170-190) */
+    int L, s, B, n, mmax;
+    double eps;
+    double *costh, *l2c, *l2s, *Nl, *lb2, *sg; /* lb2, sg per order m + L */
+    int32_t *pc, *ps, *mlim, *jlo, *jhi;       /* jlo, jhi per |m| */
+};
+
+void or_spin_rings_destroy(or_spin_rings *e) {
+    if (!e) return;
+    free(e->costh); free(e->l2c); free(e->l2s); free(e->Nl); free(e->lb2); free(e->sg);
+    free(e->pc); free(e->ps); free(e->mlim); free(e->jlo); free(e->jhi);
+    free(e);
+}
+
+/* Spin.jl:192-270: half-angle logarithms per ring (swapped under reflection, with cos theta
+ * negated), retained orders max(polar_mmax, |s|), and the seed tables per order. */
+or_spin_rings *or_spin_rings_create(const double *theta, int n, int L, int spin, double eps, int ntrans,
+                                    int reflect) {
+    if (L < 0) { set_err("lmax must be nonnegative"); return NULL; }
+    if (ntrans < 1) { set_err("ntrans must be >= 1"); return NULL; }
+    if (n < 1) { set_err("need at least one ring"); return NULL; }
+    if (spin == 0) { set_err("spin 0 uses the scalar ring engine"); return NULL; }
+    int as = spin < 0 ? -spin : spin;
+    if (as > L) { set_err("|spin| > lmax"); return NULL; }
+    for (int j = 1; j < n; j++)
+        if (theta[j] < theta[j - 1]) { set_err("ring colatitudes must be sorted ascending"); return NULL; }
+    or_spin_rings *e = calloc(1, sizeof *e);
+    e->L = L; e->s = spin; e->B = ntrans; e->n = n; e->eps = eps;
+    e->costh = malloc(sizeof(double) * n);
+    e->l2c = malloc(sizeof(double) * n);
+    e->l2s = malloc(sizeof(double) * n);
+    e->mlim = malloc(sizeof(int32_t) * n);
+    e->jlo = malloc(sizeof(int32_t) * (L + 1));
+    e->jhi = malloc(sizeof(int32_t) * (L + 1));
+    e->Nl = malloc(sizeof(double) * (L + 1));
+    e->lb2 = malloc(sizeof(double) * (2 * L + 1));
+    e->sg = malloc(sizeof(double) * (2 * L + 1));
+    e->pc = malloc(sizeof(int32_t) * (2 * L + 1));
+    e->ps = malloc(sizeof(int32_t) * (2 * L + 1));
+    double csgn = reflect ? -1.0 : 1.0;
+    e->mmax = 0;
+    for (int j = 0; j < n; j++) {
+        double t = theta[j];
+        e->costh[j] = csgn * cos(t);
+        double ch = cos(t / 2), sh = fabs(sin(t / 2));
+        if (reflect) { double q = ch; ch = sh; sh = q; }
+        e->l2c[j] = ch > 0 ? log2(ch) : -INFINITY;
+        e->l2s[j] = sh > 0 ? log2(sh) : -INFINITY;
+        int ml = or_polar_mmax(t, L, eps);
+        e->mlim[j] = ml > as ? ml : as;
+        if (e->mlim[j] > e->mmax) e->mmax = e->mlim[j];
+    }
+    if (or_nested_ranges(e->mlim, n, L, e->jlo, e->jhi) != 0) { or_spin_rings_destroy(e); return NULL; }
+    int nn = -spin, ann = as;
+    double inv_ln2 = 1.0 / log(2.0);
+    for (int m = -L; m <= L; m++) {
+        int i = m + L, am = m < 0 ? -m : m;
+        int l0 = am > ann ? am : ann;
+        if (l0 > L) { e->lb2[i] = -INFINITY; e->pc[i] = 0; e->ps[i] = 0; e->sg[i] = 1.0; continue; }
+        int a, b;
+        double extra, sgn;
+        if (ann == l0) { a = m; b = nn == l0 ? l0 : -l0; extra = 1.0; }
+        else { a = nn; b = m == l0 ? l0 : -l0; extra = ((nn - m) & 1) ? -1.0 : 1.0; }
+        if (b == l0) { e->pc[i] = l0 + a; e->ps[i] = l0 - a; sgn = 1.0; }
+        else { e->pc[i] = l0 - a; e->ps[i] = l0 + a; sgn = ((l0 - a) & 1) ? -1.0 : 1.0; }
+        e->lb2[i] = half_lbinom(l0, a) * inv_ln2;
+        e->sg[i] = sgn * extra;
+    }
+    for (int l = 0; l <= L; l++) e->Nl[l] = spin_Nl(l);
+    return e;
+}
+
+int or_spin_rings_mmax(const or_spin_rings *e) { return e->mmax; }
+int64_t or_spin_rings_work(const or_spin_rings *e) { /* retained (l, m, ring) triples over m in [-L, L] */
+    int64_t w = 0;
+    int as = e->s < 0 ? -e->s : e->s;
+    for (int m = -e->L; m <= e->L; m++) {
+        int am = m < 0 ? -m : m, l0 = am > as ? am : as;
+        int64_t nj = e->jhi[am] - e->jlo[am] + 1;
+        if (nj > 0 && l0 <= e->L) w += nj * (int64_t)(e->L - l0 + 1);
+    }
+    return w;
+}
+
+/* _spin_chunk! (Spin.jl:377-489) for one ring: the ring carries its own power-of-two exponent
+ * (the reference shares one per chunk of rings; the values agree to round-off). */
+static void spin_ring(int fwd, or_cplx *G, const or_cplx *Gin, or_cplx *alm_out, const or_cplx *alm_in,
+                      const or_spin_rings *e, int m, int j, or_cplx *acc) {
+    int L = e->L, B = e->B, n = -e->s, am = m < 0 ? -m : m, an = n < 0 ? -n : n;
+    int l0 = am > an ? am : an;
+    if (l0 > L) return;
+    int mi = m + L;
+    int64_t astride = (int64_t)(L + 1) * (2 * L + 1), col = (int64_t)(L + 1) * mi;
+    int64_t gi = j + (int64_t)e->n * mi, gstride = (int64_t)e->n * (2 * L + 1);
+    double ex = e->lb2[mi];
+    if (e->pc[mi] != 0) ex += (double)e->pc[mi] * e->l2c[j];
+    if (e->ps[mi] != 0) ex += (double)e->ps[mi] * e->l2s[j];
+    double E = isfinite(ex) ? ceil(ex) : 0.0;
+    double d1 = isfinite(ex) ? e->sg[mi] * exp2(ex - E) : 0.0, d0 = 0.0;
+    double cs = e->costh[j];
+    for (int b = 0; b < B; b++) acc[b] = fwd ? cx(0, 0) : Gin[gi + b * gstride];
+    /* _spin_touch! at l0 */
+    if (fwd) {
+        for (int b = 0; b < B; b++) acc[b] = cadd(acc[b], cscale(alm_in[col + l0 + b * astride], e->Nl[l0] * d1));
+    } else {
+        double f = exp2(E) * e->Nl[l0];
+        for (int b = 0; b < B; b++) {
+            or_cplx *o = &alm_out[col + l0 + b * astride];
+            *o = cadd(*o, cscale(cscale(acc[b], d1), f));
+        }
+    }
+    int steps = 0;
+    for (int Lc = l0; Lc < L;) {
+        int L1 = Lc + 1;
+        double sm1 = sqrt(fmax(0.0, (double)(L1 * L1 - m * m)));
+        double sn1 = sqrt(fmax(0.0, (double)(L1 * L1 - n * n)));
+        double den = (double)Lc * sm1 * sn1;
+        if (den <= 0) { d0 = d1; d1 = 0.0; Lc = L1; continue; }
+        double sm0 = sqrt(fmax(0.0, (double)(Lc * Lc - m * m)));
+        double sn0 = sqrt(fmax(0.0, (double)(Lc * Lc - n * n)));
+        double a1 = (double)(2 * Lc + 1) * (double)Lc * (double)L1 / den;
+        double a2 = (double)(2 * Lc + 1) * (double)m * (double)n / den;
+        double a0 = (double)L1 * sm0 * sn0 / den;
+        double dn = (a1 * cs - a2) * d1 - a0 * d0;
+        d0 = d1;
+        d1 = dn;
+        Lc = L1;
+        double N1 = e->Nl[L1];
+        if (fwd) {
+            for (int b = 0; b < B; b++) acc[b] = cadd(acc[b], cscale(alm_in[col + L1 + b * astride], N1 * dn));
+        } else {
+            double f = exp2(E) * N1;
+            for (int b = 0; b < B; b++) {
+                or_cplx *o = &alm_out[col + L1 + b * astride];
+                *o = cadd(*o, cscale(cscale(acc[b], dn), f));
+            }
+        }
+        if (++steps == RESCALE_STRIDE && Lc < L) {
+            steps = 0;
+            double mx = fmax(fabs(d1), fabs(d0));
+            if (mx > 0 && isfinite(mx)) {
+                double sh = ceil(log2(mx));
+                if (sh != 0) {
+                    double f = exp2(-sh);
+                    d1 *= f;
+                    d0 *= f;
+                    if (fwd)
+                        for (int b = 0; b < B; b++) acc[b] = cscale(acc[b], f);
+                    E += sh;
+                }
+            }
+        }
+    }
+    if (fwd) {
+        double f = exp2(E);
+        for (int b = 0; b < B; b++) G[gi + b * gstride] = cadd(G[gi + b * gstride], cscale(acc[b], f));
+    }
+}
+
+static void spin_walk(int fwd, or_cplx *G, const or_cplx *Gin, or_cplx *alm_out, const or_cplx *alm_in,
+                      const or_spin_rings *e, int nthreads) {
+    int L = e->L;
+    int ns = nthreads < 1 ? 1 : nthreads;
+    (void)ns;
+#pragma omp parallel for num_threads(ns) schedule(dynamic, 1)
+    for (int mi = 0; mi <= 2 * L; mi++) { /* orders are independent: each owns its alm column */
+        int m = mi - L, am = m < 0 ? -m : m;
+        or_cplx *acc = malloc(sizeof(or_cplx) * e->B);
+        for (int j = e->jlo[am]; j <= e->jhi[am]; j++) spin_ring(fwd, G, Gin, alm_out, alm_in, e, m, j, acc);
+        free(acc);
+    }
+}
+
+/* spin_rings_from_alm! / spin_alm_from_rings! (Spin.jl:316-334); alm in the dense (l, m + L) layout */
+int or_spin_rings_from_alm(const or_spin_rings *e, const or_cplx *alm, or_cplx *G, int nthreads) {
+    memset(G, 0, sizeof(or_cplx) * (size_t)e->n * (2 * e->L + 1) * e->B);
+    spin_walk(1, G, NULL, NULL, alm, e, nthreads);
+    return 0;
+}
+int or_spin_alm_from_rings(const or_spin_rings *e, const or_cplx *G, or_cplx *alm, int nthreads) {
+    memset(alm, 0, sizeof(or_cplx) * (size_t)(e->L + 1) * (2 * e->L + 1) * e->B);
+    spin_walk(0, NULL, G, alm, NULL, e, nthreads);
     return 0;
 }
 
@@ -1038,6 +1317,8 @@ struct or_plan {
     or_geom geom;
     int flip, nth_half, nth, nph, mmax;
     or_rings *eng;
+    int spin;            /* spin weight; nonzero plans use the spin engine (Spin.jl) */
+    or_spin_rings *seng;
     double *wth, *Tpts, *ppts;
     or_cplx *phase, *G, *F, *Ft, *Fhat, *fbuf;
     /* direct */
@@ -1113,6 +1394,7 @@ static void msort_idx(int64_t *idx, int64_t *tmp, const double *key, int64_t n) 
 void or_plan_destroy(or_plan *p) {
     if (!p) return;
     or_rings_destroy(p->eng);
+    or_spin_rings_destroy(p->seng);
     free(p->wth); free(p->Tpts); free(p->ppts); free(p->phase); free(p->G); free(p->F);
     free(p->Ft); free(p->Fhat); free(p->fbuf); free(p->perm); free(p->ths); free(p->phs);
     free(p);
@@ -1120,10 +1402,19 @@ void or_plan_destroy(or_plan *p) {
 
 or_plan *or_plan_create(const double *theta, const double *phi, int64_t M, int L, double tol, int ntrans,
                         int method, int nufft_kind, int nthreads) {
+    return or_plan_create_spin(theta, phi, M, L, tol, ntrans, 0, method, nufft_kind, nthreads);
+}
+
+/* Plan(θ, φ; lmax, tol, spin, ntrans, method) (Plan.jl:84-116): spin != 0 takes SpinDirectLeaf
+ * (Spin.jl:548-585) or StretchLeaf over SpinRings (Stretch.jl:190-194) */
+or_plan *or_plan_create_spin(const double *theta, const double *phi, int64_t M, int L, double tol, int ntrans,
+                             int spin, int method, int nufft_kind, int nthreads) {
     if (L < 0) { set_err("lmax must be nonnegative"); return NULL; }
     if (ntrans < 1) { set_err("ntrans must be >= 1"); return NULL; }
     if (M < 1) { set_err("need at least one point"); return NULL; }
+    if (spin != 0 && abs(spin) > L) { set_err("(|spin| = %d) > lmax = %d", abs(spin), L); return NULL; }
     or_plan *p = calloc(1, sizeof *p);
+    p->spin = spin;
     p->L = L; p->B = ntrans; p->M = M; p->tol = tol; p->nufft_kind = nufft_kind;
     p->nthreads = nthreads < 1 ? 1 : nthreads;
     /* bounding box colatitudes (Box.jl:43-73) */
@@ -1146,8 +1437,9 @@ or_plan *or_plan_create(const double *theta, const double *phi, int64_t M, int L
         p->phs = malloc(sizeof(double) * M);
         for (int64_t i = 0; i < M; i++) { p->ths[i] = theta[p->perm[i]]; p->phs[i] = phi[p->perm[i]]; }
         if (M > 0x7fffffff) { set_err("too many points for the direct oracle"); or_plan_destroy(p); return NULL; }
-        p->eng = or_rings_create(p->ths, (int)M, L, 0.0, ntrans, 0);
-        if (!p->eng) { or_plan_destroy(p); return NULL; }
+        if (spin) p->seng = or_spin_rings_create(p->ths, (int)M, L, spin, 0.0, ntrans, 0);
+        else p->eng = or_rings_create(p->ths, (int)M, L, 0.0, ntrans, 0);
+        if (!p->eng && !p->seng) { or_plan_destroy(p); return NULL; }
         p->G = calloc((size_t)M * (2 * L + 1) * ntrans, sizeof(or_cplx));
         return p;
     }
@@ -1171,10 +1463,12 @@ or_plan *or_plan_create(const double *theta, const double *phi, int64_t M, int L
     stretch_rings(&p->geom, nh, th, Th);
     for (int i = 1; i < nh; i++)
         if (th[i] < th[i - 1]) { set_err("ring colatitudes are not ascending"); free(th); free(Th); or_plan_destroy(p); return NULL; }
-    p->eng = or_rings_create(th, nh, L, tol, ntrans, flip);
-    if (!p->eng) { free(th); free(Th); or_plan_destroy(p); return NULL; }
-    p->mmax = p->eng->mmax;
-    p->nph = (int)or_next_smooth_even(2 * (int64_t)p->mmax + 1);
+    if (spin) p->seng = or_spin_rings_create(th, nh, L, spin, tol, ntrans, flip);
+    else p->eng = or_rings_create(th, nh, L, tol, ntrans, flip);
+    if (!p->eng && !p->seng) { free(th); free(Th); or_plan_destroy(p); return NULL; }
+    p->mmax = spin ? p->seng->mmax : p->eng->mmax;
+    int mphi = p->mmax > abs(spin) ? p->mmax : abs(spin);
+    p->nph = (int)or_next_smooth_even(2 * (int64_t)mphi + 1);
     p->wth = malloc(sizeof(double) * nh);
     for (int i = 0; i < nh; i++) p->wth[i] = or_stretch_window(&p->geom, Th[i]);
     free(th); free(Th);
@@ -1203,7 +1497,7 @@ void or_plan_get_info(const or_plan *p, or_plan_info *in) {
     in->kind = p->method == 2 ? p->geom.kind : -1;
     in->flip = p->flip; in->lmax = p->L; in->ntrans = p->B; in->method = p->method;
     in->nth_half = p->nth_half; in->nth = p->nth; in->nph = p->nph; in->mmax = p->mmax;
-    in->ring_work = p->method == 2 ? or_rings_work(p->eng) : 0;
+    in->ring_work = p->method == 2 ? (p->spin ? or_spin_rings_work(p->seng) : or_rings_work(p->eng)) : 0;
     in->npts = p->M; in->nufft_kind = p->nufft_kind;
     if (p->method == 2) {
         or_es_params(p->tol, &in->es_w, &in->es_beta);
@@ -1251,7 +1545,8 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     int64_t M = p->M;
     if (p->method == 1) {
         double t0 = now_s();
-        or_rings_from_alm(p->eng, alm, p->G, p->nthreads);
+        if (p->spin) or_spin_rings_from_alm(p->seng, alm, p->G, p->nthreads);
+        else or_rings_from_alm(p->eng, alm, p->G, p->nthreads);
         int64_t n = M, gB = n * (2 * L + 1);
         int off = L;
         for (int b = 0; b < B; b++)
@@ -1270,7 +1565,8 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     }
     int nh = p->nth_half, nth = p->nth, nph = p->nph;
     double t0 = now_s();
-    or_rings_from_alm(p->eng, alm, p->G, p->nthreads);
+    if (p->spin) or_spin_rings_from_alm(p->seng, alm, p->G, p->nthreads);
+    else or_rings_from_alm(p->eng, alm, p->G, p->nthreads);
     double t1 = now_s();
     pack_orders(p->F, p->G, nh, nph, L, B);
     fft_dim2(p->F, nh, nph, B, +1, 1);
@@ -1281,7 +1577,7 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
                 p->F[i] = cscale(p->F[i], p->wth[j]);
             }
     if (p->geom.kind == OR_KIND_INTERIOR) memcpy(p->Ft, p->F, sizeof(or_cplx) * (size_t)nth * nph * B);
-    else or_dfs_double(p->F, p->Ft, nh, nph, B, 0);
+    else or_dfs_double(p->F, p->Ft, nh, nph, B, p->spin);
     memcpy(p->Fhat, p->Ft, sizeof(or_cplx) * (size_t)nth * nph * B);
     or_fft2(p->Fhat, nth, nph, B, -1, 1);
     for (int b = 0; b < B; b++)
@@ -1320,7 +1616,8 @@ int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
                     Gb[i + (int64_t)(-m + off) * n] = cmul(fi, ph);
                 }
             }
-        or_alm_from_rings(p->eng, p->G, alm, p->nthreads);
+        if (p->spin) or_spin_alm_from_rings(p->seng, p->G, alm, p->nthreads);
+        else or_alm_from_rings(p->eng, p->G, alm, p->nthreads);
         p->times[0] = now_s() - t0;
         return 0;
     }
@@ -1339,7 +1636,7 @@ int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
     memcpy(p->Ft, p->Fhat, sizeof(or_cplx) * (size_t)nth * nph * B);
     or_fft2(p->Ft, nth, nph, B, +1, 1);
     if (p->geom.kind == OR_KIND_INTERIOR) memcpy(p->F, p->Ft, sizeof(or_cplx) * (size_t)nth * nph * B);
-    else or_dfs_fold(p->Ft, p->F, nh, nph, B, 0);
+    else or_dfs_fold(p->Ft, p->F, nh, nph, B, p->spin);
     for (int b = 0; b < B; b++)
         for (int k = 0; k < nph; k++)
             for (int j = 0; j < nh; j++) {
@@ -1349,7 +1646,8 @@ int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
     fft_dim2(p->F, nh, nph, B, -1, 1);
     unpack_orders(p->G, p->F, nh, nph, L, B);
     double t2 = now_s();
-    or_alm_from_rings(p->eng, p->G, alm, p->nthreads);
+    if (p->spin) or_spin_alm_from_rings(p->seng, p->G, alm, p->nthreads);
+    else or_alm_from_rings(p->eng, p->G, alm, p->nthreads);
     double t3 = now_s();
     p->times[0] = t3 - t2; p->times[1] = t2 - t1; p->times[2] = t1 - t0;
     return 0;

@@ -106,4 +106,18 @@ int or_plan_points(const or_plan *p, double *Theta, double *phi);
 #ifdef __cplusplus
 }
 #endif
+
+/* Spin-weighted transforms (Spin.jl) */
+double or_wigner_d(int l, int m, int n, double beta);
+void or_sylm(int s, int l, int m, double theta, double phi, double *re, double *im);
+typedef struct or_spin_rings or_spin_rings;
+or_spin_rings *or_spin_rings_create(const double *theta, int n, int L, int spin, double eps, int ntrans,
+                                    int reflect);
+void or_spin_rings_destroy(or_spin_rings *e);
+int or_spin_rings_mmax(const or_spin_rings *e);
+int64_t or_spin_rings_work(const or_spin_rings *e);
+int or_spin_rings_from_alm(const or_spin_rings *e, const or_cplx *alm, or_cplx *G, int nthreads);
+int or_spin_alm_from_rings(const or_spin_rings *e, const or_cplx *G, or_cplx *alm, int nthreads);
+or_plan *or_plan_create_spin(const double *theta, const double *phi, int64_t M, int L, double tol,
+                             int ntrans, int spin, int method, int nufft_kind, int nthreads);
 #endif
