@@ -66,6 +66,7 @@ typedef struct {
     double tol, es_beta;
     int32_t es_rows;   /* fine-grid rows the apply stores: those the points' kernels reach */
     int32_t nboxes;    /* boxes of a multi-box plan (1 otherwise) */
+    int32_t spin;      /* spin weight (0: scalar) */
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */
@@ -82,6 +83,14 @@ const char *pfsht_version(void);
  * the device. theta, phi: host arrays of npts colatitudes in [0, pi] and longitudes. */
 int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
                       int32_t ntrans, int32_t method, int32_t precision, pfsht_plan_t *out);
+/* Plan(theta, phi; lmax, tol, spin, ntrans, method) — src/Plan.jl:84-116 with spin != 0:
+ * SpinDirectLeaf (src/Spin.jl:548-585) or StretchLeaf over SpinRings (src/Stretch.jl:190-194),
+ * per box for a multi-box plan (src/MultiBox.jl:33-50). |spin| <= lmax (src/Plan.jl:101-102).
+ * Coefficients of a spin plan use the dense (l, m + lmax) layout of spin_coeff_index
+ * (src/Layout.jl:28-29); the analysis calls that read the sph_mode layout (pseudo-spectrum, the
+ * Wiener prior) return PFSHT_E_UNSUPPORTED on a spin plan. spin = 0 is pfsht_plan_create. */
+int pfsht_plan_create_spin(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                           int32_t ntrans, int32_t spin, int32_t method, int32_t precision, pfsht_plan_t *out);
 void pfsht_plan_destroy(pfsht_plan_t plan);
 int pfsht_plan_info(pfsht_plan_t plan, pfsht_info *info);
 
@@ -98,6 +107,11 @@ int pfsht_adjoint(pfsht_plan_t plan, const void *f, void *alm, int32_t where, vo
  * ascending, host memory. */
 int pfsht_rings_create(const double *theta, int32_t nrings, int32_t lmax, double eps, int32_t ntrans,
                        int32_t reflect, pfsht_rings_t *out);
+/* SpinRings(theta, lmax, spin, eps; ntrans, reflect) — src/Spin.jl:192-270: the same handle, whose
+ * rings_from_alm / alm_from_rings are spin_rings_from_alm! / spin_alm_from_rings!
+ * (src/Spin.jl:316-334) with coefficients in the (l, m + lmax) layout. spin != 0, |spin| <= lmax. */
+int pfsht_spin_rings_create(const double *theta, int32_t nrings, int32_t lmax, int32_t spin, double eps,
+                            int32_t ntrans, int32_t reflect, pfsht_rings_t *out);
 void pfsht_rings_destroy(pfsht_rings_t rings);
 int64_t pfsht_rings_work(pfsht_rings_t rings);
 int32_t pfsht_rings_mmax(pfsht_rings_t rings);

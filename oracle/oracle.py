@@ -706,12 +706,12 @@ class MultiBox:
     """MultiBoxLeaf (MultiBox.jl:20-144) over oracle stretched plans: a disjoint union in synthesis,
     the sum over boxes (in box order) in the adjoint."""
 
-    def __init__(self, theta, phi, lmax, tol=1e-10, ntrans=1, maxboxes=8, nthreads=1):
+    def __init__(self, theta, phi, lmax, tol=1e-10, ntrans=1, maxboxes=8, nthreads=1, spin=0):
         th, ph = np.asarray(theta, np.float64), np.asarray(phi, np.float64)
         self.L, self.B, self.M = lmax, ntrans, len(th)
         self.groups, _ = partition_boxes(th, ph, lmax, ntrans, tol, maxboxes)
         self.plans = [Plan(th[g], ph[g], lmax, tol=tol, ntrans=ntrans, method="stretch", nufft="es",
-                           nthreads=nthreads) for g in self.groups]
+                           nthreads=nthreads, spin=spin) for g in self.groups]
 
     def synth(self, alm):
         f = np.zeros((self.M, self.B), np.complex128, order="F")

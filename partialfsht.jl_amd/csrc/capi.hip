@@ -145,6 +145,8 @@ using namespace pfsht;
 struct pfsht_rings_s {
     int dev = 0;
     RingDev d;
+    int spin = 0;  // nonzero: the spin engine (sd) with d's geometry and ranges
+    SpinDev sd;
     std::vector<int> jlo, jhi, mlim;
     int64_t work = 0;
     int64_t bytes = 0;
@@ -156,6 +158,7 @@ struct pfsht_rings_s {
         dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.gc); dfree(d.abofs);
         dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
+        dfree(sd.l2c); dfree(sd.l2s); dfree(sd.seed); dfree(sd.Nl); dfree(sd.tab); dfree(sd.tofs); dfree(sd.items);
     }
 };
 
@@ -294,6 +297,105 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     return kOk;
 }
 
+// SpinRings (src/Spin.jl:192-270) on the device: the half-angle logarithms per ring (swapped, and
+// cos theta negated, for a reflected ring set, so the spin weight stays as given), retained orders
+// max(polar_mmax, |s|) per ring, the seed tables per order (src/Spin.jl:233-257), the recurrence
+// table per (|m|, l) built by a kernel, and one forward wavefront per (order, 64-ring tile).
+static Status rings_build_spin(const double *theta, int n, int L, int spin, double eps, int B, bool reflect,
+                               pfsht_rings_s &r) {
+    if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
+    if (B < 1) { set_error("ntrans must be >= 1"); return kEArg; }
+    if (n < 1) { set_error("need at least one ring"); return kEArg; }
+    const int as = spin < 0 ? -spin : spin;
+    if (spin == 0) { set_error("spin 0 uses the scalar ring engine"); return kEArg; }
+    if (as > L) { set_error("(|spin| = " + std::to_string(as) + ") > lmax = " + std::to_string(L)); return kEArg; }
+    for (int j = 1; j < n; ++j)
+        if (!(theta[j] >= theta[j - 1])) { set_error("ring colatitudes must be sorted ascending"); return kEArg; }
+    (void)hipGetDevice(&r.dev);
+    r.spin = spin;
+    const double csgn = reflect ? -1.0 : 1.0;
+    std::vector<double> costh(n), l2c(n), l2s(n);
+    std::vector<int> mlim(n);
+    int mmax = 0;
+    for (int j = 0; j < n; ++j) {
+        const double t = theta[j];
+        costh[j] = csgn * std::cos(t);
+        double ch = std::cos(t / 2), sh = std::fabs(std::sin(t / 2));
+        if (reflect) std::swap(ch, sh);
+        l2c[j] = ch > 0 ? std::log2(ch) : -INFINITY;
+        l2s[j] = sh > 0 ? std::log2(sh) : -INFINITY;
+        mlim[j] = std::max(polar_mmax(t, L, eps), as);
+        mmax = std::max(mmax, mlim[j]);
+    }
+    std::string msg;
+    if (!nested_ranges(mlim, L, r.jlo, r.jhi, msg)) { set_error(msg); return kEArg; }
+    r.mlim = mlim;
+    // seeds: l0 = max(|m|, |s|); the extremal index moves to second place through
+    // d^l_{a,b} = (-1)^(a-b) d^l_{b,a}, leaving a = m (|s| >= |m|) or a = -s
+    const int nn = -spin;
+    std::vector<double4> seed(2 * L + 1);
+    const double inv_ln2 = 1.0 / std::log(2.0);
+    for (int m = -L; m <= L; ++m) {
+        const int am = std::abs(m), l0 = std::max(am, as);
+        int a, b;
+        double extra, sgn;
+        if (as == l0) { a = m; b = nn == l0 ? l0 : -l0; extra = 1.0; }
+        else { a = nn; b = m == l0 ? l0 : -l0; extra = ((nn - m) & 1) ? -1.0 : 1.0; }
+        int pc, ps;
+        if (b == l0) { pc = l0 + a; ps = l0 - a; sgn = 1.0; }
+        else { pc = l0 - a; ps = l0 + a; sgn = ((l0 - a) & 1) ? -1.0 : 1.0; }
+        const double lb = (std::lgamma(double(2 * l0 + 1)) - std::lgamma(double(l0 - a + 1)) -
+                           std::lgamma(double(l0 + a + 1))) / 2;
+        seed[m + L] = make_double4(lb * inv_ln2, sgn * extra, double(pc), double(ps));
+    }
+    std::vector<double> Nl(L + 1);
+    for (int l = 0; l <= L; ++l) Nl[l] = std::sqrt(double(2 * l + 1) / (4 * kPi));
+    std::vector<int64_t> tofs(L + 1);
+    int64_t ntab = 0;
+    for (int am = 0; am <= L; ++am) { tofs[am] = ntab; ntab += L - std::max(am, as); }
+    std::vector<int4> items;
+    r.work = 0;
+    for (int m = -mmax; m <= mmax; ++m) {
+        const int am = std::abs(m), lo = r.jlo[am], hi = r.jhi[am];
+        if (hi < lo) continue;
+        r.work += int64_t(hi - lo + 1) * (L - std::max(am, as) + 1);
+        for (int j0 = lo; j0 <= hi; j0 += 64) items.push_back(make_int4(m, j0, std::min(64, hi - j0 + 1), 0));
+    }
+    RingDev &d = r.d;
+    SpinDev &sd = r.sd;
+    d.n = n; d.L = L; d.B = B; d.mmax = mmax;
+    sd.s = spin;
+    sd.nitems = int(items.size());
+    std::vector<int32_t> jlo32(r.jlo.begin(), r.jlo.begin() + mmax + 1), jhi32(r.jhi.begin(), r.jhi.begin() + mmax + 1);
+    Status st;
+    int64_t &by = r.bytes;
+    if ((st = upload(&d.costh, costh, by)) != kOk) return st;
+    if ((st = upload(&d.jlo, jlo32, by)) != kOk) return st;
+    if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
+    if ((st = upload(&sd.l2c, l2c, by)) != kOk) return st;
+    if ((st = upload(&sd.l2s, l2s, by)) != kOk) return st;
+    if ((st = upload(&sd.seed, seed, by)) != kOk) return st;
+    if ((st = upload(&sd.Nl, Nl, by)) != kOk) return st;
+    if ((st = upload(&sd.tofs, tofs, by)) != kOk) return st;
+    if ((st = upload(&sd.items, items, by)) != kOk) return st;
+    if ((st = dalloc(&sd.tab, size_t(std::max<int64_t>(ntab, 1)), by)) != kOk) return st;
+    if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
+    launch_spin_tables(d, sd, nullptr);
+    PFSHT_HIP_TRY(hipGetLastError());
+    PFSHT_HIP_TRY(hipStreamSynchronize(nullptr));
+    return kOk;
+}
+
+// The ring passes of an engine of either kind.
+static void ring_forward(pfsht_rings_s &r, const double2 *alm, hipStream_t s) {
+    if (r.spin) launch_spin_fwd(r.d, r.sd, alm, r.Gd, s);
+    else launch_ring_fwd(r.d, alm, r.Gd, s);
+}
+static void ring_adjoint(pfsht_rings_s &r, double2 *alm, hipStream_t s) {
+    if (r.spin) launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
+    else launch_ring_adj(r.d, r.Gd, alm, s);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Plan
 // ------------------------------------------------------------------------------------------------
@@ -313,6 +415,7 @@ struct TimedApply {
 struct pfsht_plan_s {
     int dev = 0;
     int method = PFSHT_METHOD_STRETCH, precision = PFSHT_PREC_FP64;
+    int spin = 0;  // spin weight (src/Plan.jl:87); coefficients in the dense (l, m + lmax) layout when nonzero
     int L = 0, B = 1;
     int64_t M = 0;
     double tol = 0;
@@ -397,12 +500,17 @@ static Status apply_end(pfsht_plan_s &p, hipStream_t s) {
 }
 
 static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi, int64_t M, int L, double tol, int B,
-                         int method) {
+                         int method, int spin) {
     if (M < 1) { set_error("need at least one point"); return kEArg; }
     if (M > INT32_MAX) { set_error("more than 2^31-1 points per plan is not supported"); return kEArg; }
     if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
     if (B < 1) { set_error("ntrans must be >= 1"); return kEArg; }
     if (!(tol > 0) || !(tol < 1)) { set_error("tol must lie in (0, 1)"); return kEArg; }
+    if (spin != 0 && std::abs(spin) > L) {  // src/Plan.jl:101-102
+        set_error("(|spin| = " + std::to_string(std::abs(spin)) + ") > lmax = " + std::to_string(L));
+        return kEArg;
+    }
+    p.spin = spin;
     (void)hipGetDevice(&p.dev);
     p.L = L; p.B = B; p.M = M; p.tol = tol;
     // bounding box in colatitude (src/Box.jl:43-73)
@@ -441,7 +549,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
             if (!q) { set_error("host allocation failed"); return kENomem; }
             q->precision = p.precision;
             p.boxes.push_back(q);
-            Status st = plan_build(*q, tg.data(), pg.data(), Mg, L, tol, B, PFSHT_METHOD_STRETCH);
+            Status st = plan_build(*q, tg.data(), pg.data(), Mg, L, tol, B, PFSHT_METHOD_STRETCH, spin);
             if (st != kOk) return st;
             int32_t *idx = nullptr;
             if ((st = upload(&idx, g, by)) != kOk) return st;
@@ -465,7 +573,8 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
         std::vector<double> ths(M), phs(M);
         for (int64_t i = 0; i < M; ++i) { ths[i] = theta[perm[i]]; phs[i] = phi[perm[i]]; }
         p.rings = new pfsht_rings_s;
-        Status st = rings_build(ths.data(), int(M), L, 0.0, B, false, *p.rings);
+        Status st = spin ? rings_build_spin(ths.data(), int(M), L, spin, 0.0, B, false, *p.rings)
+                         : rings_build(ths.data(), int(M), L, 0.0, B, false, *p.rings);
         if (st != kOk) return st;
         p.mmax = p.rings->d.mmax;
         p.nh = int(M);
@@ -500,10 +609,11 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     stretch_rings(p.geom, p.nh, th, Th);
     if (!std::is_sorted(th.begin(), th.end())) { set_error("ring colatitudes are not ascending"); return kEArg; }
     p.rings = new pfsht_rings_s;
-    Status st = rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings);
+    Status st = spin ? rings_build_spin(th.data(), p.nh, L, spin, tol, B, p.flip != 0, *p.rings)
+                     : rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings);
     if (st != kOk) return st;
     p.mmax = p.rings->d.mmax;
-    p.nph = int(next_smooth_even(2 * int64_t(p.mmax) + 1));
+    p.nph = int(next_smooth_even(2 * int64_t(std::max(p.mmax, std::abs(spin))) + 1));  // src/Stretch.jl:194
     int64_t &by = p.bytes;
     std::vector<double> wth(p.nh);
     for (int i = 0; i < p.nh; ++i) wth[i] = stretch_window(p.geom, Th[i]);
@@ -659,6 +769,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
 static GridDims dims_of(const pfsht_plan_s &p) {
     GridDims g;
     g.B = p.B; g.nh = p.nh; g.nth = p.nth; g.nph = p.nph; g.mmax = p.mmax; g.L = p.L; g.kind = p.geom.kind;
+    g.spin = p.spin;
     return g;
 }
 
@@ -785,7 +896,7 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
-        launch_ring_fwd(r.d, alm, r.Gd, s);
+        ring_forward(r, alm, s);
         tmark(p, t, 1, s);
         launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, p.B, s);
         tmark(p, t, 2, s);  // the phase sums are timed as stage 1
@@ -795,7 +906,7 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     // src/Stretch.jl:262-277: ring pass, window + DFS (in longitude-spectral space), colatitude FFT,
     // phase and deconvolution into the fine grid, fine inverse FFT, interpolation
     tmark(p, t, 0, s);
-    launch_ring_fwd(p.rings->d, alm, p.rings->Gd, s);
+    ring_forward(*p.rings, alm, s);
     tmark(p, t, 1, s);
     Status st = p.precision == PFSHT_PREC_MIXED ? synth_grid<float2>(p, t, f, s) : synth_grid<double2>(p, t, f, s);
     if (st != kOk) return st;
@@ -812,9 +923,14 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
         tmark(p, t, 0, s);
         launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);
         tmark(p, t, 7, s);  // the phase products are timed as stage 1 (adjoint order)
-        launch_ring_adj_tiles(r.d, r.Gd, s);
-        tmark(p, t, 8, s);
-        launch_ring_adj_finalize(r.d, alm, s);
+        if (r.spin) {
+            launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
+            tmark(p, t, 8, s);
+        } else {
+            launch_ring_adj_tiles(r.d, r.Gd, s);
+            tmark(p, t, 8, s);
+            launch_ring_adj_finalize(r.d, alm, s);
+        }
         tmark(p, t, 9, s);
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
@@ -825,9 +941,14 @@ static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipS
     Status st = p.precision == PFSHT_PREC_MIXED ? adjoint_grid<float2>(p, t, f, s) : adjoint_grid<double2>(p, t, f, s);
     if (st != kOk) return st;
     tmark(p, t, 7, s);
-    launch_ring_adj_tiles(r.d, r.Gd, s);
-    tmark(p, t, 8, s);
-    launch_ring_adj_finalize(r.d, alm, s);
+    if (r.spin) {
+        launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
+        tmark(p, t, 8, s);
+    } else {
+        launch_ring_adj_tiles(r.d, r.Gd, s);
+        tmark(p, t, 8, s);
+        launch_ring_adj_finalize(r.d, alm, s);
+    }
     tmark(p, t, 9, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
@@ -1002,6 +1123,7 @@ int pfsht_pseudo_spectrum(pfsht_plan_t p, const void *f, const double *w, int32_
     Status st = analysis_check(p, where);
     if (st != kOk) return st;
     if (!f || !C) { set_error("NULL argument"); return kEArg; }
+    if (p->spin) { set_error("the pseudo-spectrum reads the sph_mode layout; a spin plan uses (l, m + lmax)"); return kEUnsupported; }
     if (batch < 1 || batch > p->B) {
         set_error("batch = " + std::to_string(batch) + " outside 1:" + std::to_string(p->B));
         return kEArg;
@@ -1032,6 +1154,10 @@ int pfsht_solve(pfsht_plan_t p, const void *f, const double *w, int32_t mode, do
     if (mode != PFSHT_SOLVE_CG && mode != PFSHT_SOLVE_WIENER && mode != PFSHT_SOLVE_MINNORM) {
         set_error("mode must be CG, WIENER or MINNORM");
         return kEArg;
+    }
+    if (mode == PFSHT_SOLVE_WIENER && p->spin) {
+        set_error("the Wiener prior reads the sph_mode layout; a spin plan uses (l, m + lmax)");
+        return kEUnsupported;
     }
     if (mode == PFSHT_SOLVE_WIENER && !prior) {
         set_error("WIENER needs a prior C_l: that prior is what makes the problem well posed, so it cannot be "
@@ -1110,6 +1236,11 @@ const char *pfsht_version(void) { return "pfsht-hip 0.1.0 (gfx950)"; }
 
 int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol, int32_t ntrans,
                       int32_t method, int32_t precision, pfsht_plan_t *out) {
+    return pfsht_plan_create_spin(theta, phi, npts, lmax, tol, ntrans, 0, method, precision, out);
+}
+
+int pfsht_plan_create_spin(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                           int32_t ntrans, int32_t spin, int32_t method, int32_t precision, pfsht_plan_t *out) {
     if (!out) { set_error("out is NULL"); return kEArg; }
     *out = nullptr;
     if (!theta || !phi) { set_error("theta/phi is NULL"); return kEArg; }
@@ -1123,7 +1254,7 @@ int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int3
     p->precision = precision;
     Status st;
     try {
-        st = plan_build(*p, theta, phi, npts, lmax, tol, ntrans, method);
+        st = plan_build(*p, theta, phi, npts, lmax, tol, ntrans, method, spin);
     } catch (const std::bad_alloc &) {
         set_error("host allocation failed");
         st = kENomem;
@@ -1149,6 +1280,7 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     info->method = p->method;
     info->kind = p->method == PFSHT_METHOD_DIRECT ? -1 : p->geom.kind;
     info->nboxes = p->method == PFSHT_METHOD_MULTIBOX ? int32_t(p->boxes.size()) : 1;
+    info->spin = p->spin;
     info->flip = p->flip;
     info->lmax = p->L;
     info->ntrans = p->B;
@@ -1288,6 +1420,25 @@ int pfsht_rings_create(const double *theta, int32_t nrings, int32_t lmax, double
     return kOk;
 }
 
+int pfsht_spin_rings_create(const double *theta, int32_t nrings, int32_t lmax, int32_t spin, double eps,
+                            int32_t ntrans, int32_t reflect, pfsht_rings_t *out) {
+    if (!out || !theta) { set_error("NULL argument"); return kEArg; }
+    *out = nullptr;
+    auto *r = new (std::nothrow) pfsht_rings_s;
+    if (!r) { set_error("host allocation failed"); return kENomem; }
+    Status st;
+    try {
+        st = rings_build_spin(theta, nrings, lmax, spin, eps, ntrans, reflect != 0, *r);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        st = kENomem;
+    }
+    if (st == kOk) st = hipDeviceSynchronize() == hipSuccess ? kOk : kEHip;
+    if (st != kOk) { delete r; return st; }
+    *out = r;
+    return kOk;
+}
+
 void pfsht_rings_destroy(pfsht_rings_t r) {
     if (!r) return;
     DeviceGuard dg(r->dev);
@@ -1321,7 +1472,7 @@ int pfsht_rings_from_alm(pfsht_rings_t r, const void *alm, void *G, int32_t wher
         a = r->alm_stage;
         Gout = r->G_stage;
     }
-    launch_ring_fwd(r->d, a, r->Gd, s);
+    ring_forward(*r, a, s);
     launch_gd_to_ref(r->d, r->Gd, Gout, s);
     PFSHT_HIP_TRY(hipGetLastError());
     if (where == PFSHT_WHERE_HOST) {
@@ -1348,7 +1499,7 @@ int pfsht_alm_from_rings(pfsht_rings_t r, const void *G, void *alm, int32_t wher
         aout = r->alm_stage;
     }
     launch_ref_to_gd(r->d, Gin, r->Gd, s);
-    launch_ring_adj(r->d, r->Gd, aout, s);
+    ring_adjoint(*r, aout, s);
     PFSHT_HIP_TRY(hipGetLastError());
     if (where == PFSHT_WHERE_HOST) {
         PFSHT_HIP_TRY(hipMemcpyAsync(alm, r->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));

@@ -13,17 +13,18 @@ constexpr int kTile = 32;
 // The periodisation in longitude-spectral space. The stretched apply FFTs the ring grid along
 // longitude only to transform it back inside the 2-D FFT (src/Stretch.jl:270-273 after the
 // longitude bFFT of :313-318): the pair cancels to a factor nph, and the DFS half-period shift
-// k + nph/2 becomes a factor (-1)^m on order m. So the periodised grid is built directly per order,
-// in column layout Ds [b][m + mmax][i], i < nth contiguous (the ring buffer's layout), and only the
-// colatitude FFT remains.
+// k + nph/2 becomes a factor (-1)^m on order m. With spin weight s the DFS reflection carries another
+// (-1)^s (src/DFS.jl:13-31). So the periodised grid is built directly per order, in column layout
+// Ds [b][m + mmax][i], i < nth contiguous (the ring buffer's layout), and only the colatitude FFT
+// remains.
 // ------------------------------------------------------------------------------------------------
 
-// Ds[b][c][i] = w[j] G[j, m] for i < nh (j = i), and (-1)^m w[j] G[j, m] for the mirrored rows
+// Ds[b][c][i] = w[j] G[j, m] for i < nh (j = i), and (-1)^(m+s) w[j] G[j, m] for the mirrored rows
 // i >= nh of a pole/full periodisation (j = nth - 1 - i); zero off the retained ranges.
 template <class C>
 __global__ void spec_dfs_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo,
                                 const int32_t *__restrict__ jhi, const double *__restrict__ wth,
-                                C *__restrict__ Ds, int nh, int nth, int mmax) {
+                                C *__restrict__ Ds, int nh, int nth, int mmax, int spin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -35,17 +36,17 @@ __global__ void spec_dfs_kernel(const double2 *__restrict__ Gd, const int32_t *_
     if (j >= jlo[am] && j <= jhi[am]) {
         const double2 g = Gd[(int64_t(b) * ncol + c) * nh + j];
         double w = wth[j];
-        if (i >= nh && (am & 1)) w = -w;
+        if (i >= nh && ((am + spin) & 1)) w = -w;
         v = make_double2(g.x * w, g.y * w);
     }
     stc(Ds + (int64_t(b) * ncol + c) * nth + i, v);
 }
 
-// The transpose: Gd[b][c][j] = w[j] (Ds[c][j] + (-1)^m Ds[c][nth - 1 - j]) (pole/full) or
+// The transpose: Gd[b][c][j] = w[j] (Ds[c][j] + (-1)^(m+s) Ds[c][nth - 1 - j]) (pole/full) or
 // w[j] Ds[c][j] (interior, nth = nh), for every ring j < nh.
 template <class C>
 __global__ void spec_fold_kernel(const C *__restrict__ Ds, const double *__restrict__ wth,
-                                 double2 *__restrict__ Gd, int nh, int nth, int mmax) {
+                                 double2 *__restrict__ Gd, int nh, int nth, int mmax, int spin) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -56,7 +57,7 @@ __global__ void spec_fold_kernel(const C *__restrict__ Ds, const double *__restr
     double2 v = ldc(col + j);
     if (nth > nh) {
         const double2 r = ldc(col + nth - 1 - j);
-        if (am & 1) v = make_double2(v.x - r.x, v.y - r.y);
+        if ((am + spin) & 1) v = make_double2(v.x - r.x, v.y - r.y);
         else v = make_double2(v.x + r.x, v.y + r.y);
     }
     const double w = wth[j];
@@ -66,13 +67,13 @@ __global__ void spec_fold_kernel(const C *__restrict__ Ds, const double *__restr
 template <class C>
 void launch_spec_dfs(const GridDims &g, const RingDev &r, const double2 *Gd, const double *wth, C *Ds, hipStream_t s) {
     dim3 grid((g.nth + 255) / 256, 2 * g.mmax + 1, g.B);
-    spec_dfs_kernel<C><<<grid, 256, 0, s>>>(Gd, r.jlo, r.jhi, wth, Ds, g.nh, g.nth, g.mmax);
+    spec_dfs_kernel<C><<<grid, 256, 0, s>>>(Gd, r.jlo, r.jhi, wth, Ds, g.nh, g.nth, g.mmax, g.spin);
 }
 
 template <class C>
 void launch_spec_fold(const GridDims &g, const C *Ds, const double *wth, double2 *Gd, hipStream_t s) {
     dim3 grid((g.nh + 255) / 256, 2 * g.mmax + 1, g.B);
-    spec_fold_kernel<C><<<grid, 256, 0, s>>>(Ds, wth, Gd, g.nh, g.nth, g.mmax);
+    spec_fold_kernel<C><<<grid, 256, 0, s>>>(Ds, wth, Gd, g.nh, g.nth, g.mmax, g.spin);
 }
 
 template void launch_spec_dfs<double2>(const GridDims &, const RingDev &, const double2 *, const double *, double2 *,

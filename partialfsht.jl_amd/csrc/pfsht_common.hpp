@@ -91,6 +91,24 @@ struct RingDev {
     int64_t npartials = 0;
 };
 
+// Spin-weighted ring engine (src/Spin.jl:170-545), used with a RingDev that supplies n, L, B, mmax,
+// cos theta (negated when reflected) and the per-|m| ring ranges; orders m in [-mmax, mmax].
+struct SpinDev {
+    int s = 0;                  // spin weight (nonzero)
+    double *l2c = nullptr;      // [n] log2 cos(theta/2) (log2 sin(theta/2) when reflected)
+    double *l2s = nullptr;      // [n] log2 sin(theta/2) (log2 cos(theta/2) when reflected)
+    double4 *seed = nullptr;    // [2L+1] per m + L: (log2 sqrt C(2 l0, l0 - a), sign, p_cos, p_sin)
+    double *Nl = nullptr;       // [L+1] sqrt((2l+1)/4pi)
+    double4 *tab = nullptr;     // per (|m|, l >= l0): (a1, |a2|, a0, N_{l+1}) of the step l -> l+1
+    int64_t *tofs = nullptr;    // [L+1] first entry of |m| in tab
+    int4 *items = nullptr;      // forward wavefronts: (m, j0, rings, 0)
+    int nitems = 0;
+};
+void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s);
+void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, double2 *Gd, hipStream_t s);
+// writes every slot of alm (zero outside the retained modes)
+void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, double2 *alm, hipStream_t s);
+
 // The plan-time sweep that stores the checkpoints of every forward item (default stream).
 void launch_ring_checkpoints(const RingDev &r, hipStream_t s);
 
@@ -106,6 +124,7 @@ void launch_ref_to_gd(const RingDev &r, const double2 *G, double2 *Gd, hipStream
 
 struct GridDims {
     int B, nh, nth, nph, mmax, L, kind;
+    int spin = 0;  // the mirrored half takes (-1)^(m + spin)
 };
 // Periodisation in longitude-spectral space, order-column layout Ds [b][m + mmax][i] (i < nth).
 // C is the storage of the grid buffers: double2, or float2 in mixed precision.

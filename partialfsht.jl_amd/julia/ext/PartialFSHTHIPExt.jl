@@ -59,6 +59,7 @@ struct PfshtInfo              # pfsht_info, include/pfsht.h
     es_beta::Float64
     es_rows::Int32
     nboxes::Int32
+    spin::Int32
 end
 
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
@@ -98,16 +99,19 @@ mutable struct HIPMultiBoxLeaf <: AbstractHIPLeaf
     info::PfshtInfo
 end
 
+# pfsht_plan_create_spin: spin = 0 is the scalar plan; a nonzero spin builds SpinDirectLeaf /
+# StretchLeaf over SpinRings / per-box spin plans (src/Plan.jl:118-133), coefficients in the
+# spin_coeff_index layout (src/Layout.jl:28-29).
 function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVector, lmax::Int,
-                 ntrans::Int, tol::Real) where {LT<:AbstractHIPLeaf}
+                 ntrans::Int, tol::Real, spin::Integer = 0) where {LT<:AbstractHIPLeaf}
     θh = Vector{Float64}(PartialFSHT._host(θ))
     φh = Vector{Float64}(PartialFSHT._host(φ))
     length(θh) == length(φh) || throw(DimensionMismatch("θ and φ must have the same length"))
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    _check(ccall((:pfsht_plan_create, libpfsht), Cint,
-                 (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Int32, Int32,
+    _check(ccall((:pfsht_plan_create_spin, libpfsht), Cint,
+                 (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Int32, Int32, Int32,
                   Ref{Ptr{Cvoid}}),
-                 θh, φh, length(θh), lmax, tol, ntrans, method, PFSHT_PREC_FP64, h))
+                 θh, φh, length(θh), lmax, tol, ntrans, spin, method, PFSHT_PREC_FP64, h))
     info = Ref{PfshtInfo}()
     _check(ccall((:pfsht_plan_info, libpfsht), Cint, (Ptr{Cvoid}, Ref{PfshtInfo}), h[], info))
     leaf = LT(h[], info[])
@@ -118,12 +122,12 @@ function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVecto
     return leaf
 end
 
-HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real) =
-    _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol)
-HIPDirectLeaf(θ, φ, lmax::Int, ntrans::Int) =
-    _create(HIPDirectLeaf, PFSHT_METHOD_DIRECT, θ, φ, lmax, ntrans, 1e-10)
-HIPMultiBoxLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real) =
-    _create(HIPMultiBoxLeaf, PFSHT_METHOD_MULTIBOX, θ, φ, lmax, ntrans, tol)
+HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0) =
+    _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol, spin)
+HIPDirectLeaf(θ, φ, lmax::Int, ntrans::Int, spin::Integer = 0) =
+    _create(HIPDirectLeaf, PFSHT_METHOD_DIRECT, θ, φ, lmax, ntrans, 1e-10, spin)
+HIPMultiBoxLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0) =
+    _create(HIPMultiBoxLeaf, PFSHT_METHOD_MULTIBOX, θ, φ, lmax, ntrans, tol, spin)
 
 PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
 PartialFSHT.algorithm(::HIPDirectLeaf) = PartialFSHT.DirectAlgorithm()
@@ -134,17 +138,14 @@ PartialFSHT.ring_work(leaf::AbstractHIPLeaf) = Int(leaf.info.ring_work)
 # algorithm.
 PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
-    spin == 0 ? HIPStretchLeaf(θ, φ, lmax, ntrans, tol) :
-    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+    HIPStretchLeaf(θ, φ, lmax, ntrans, tol, spin)
 PartialFSHT._build_leaf(::PartialFSHT.DirectAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
-    spin == 0 ? HIPDirectLeaf(θ, φ, lmax, ntrans) :
-    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+    HIPDirectLeaf(θ, φ, lmax, ntrans, spin)
 # src/Plan.jl:131-132: the library repeats partition_boxes on the host copy of the nodes
 PartialFSHT._build_leaf(::PartialFSHT.MultiBoxAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
-    spin == 0 ? HIPMultiBoxLeaf(θ, φ, lmax, ntrans, tol) :
-    throw(ArgumentError("spin-weighted transforms are not available on the HIP leaf"))
+    HIPMultiBoxLeaf(θ, φ, lmax, ntrans, tol, spin)
 
 _stream() = AMDGPU.stream().stream   # hipStream_t of the task-local AMDGPU stream
 

@@ -50,7 +50,7 @@ class InfoC(C.Structure):
                 ("mmax", C.c_int32), ("es_w", C.c_int32), ("es_n1", C.c_int32),
                 ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double),
-                ("es_rows", C.c_int32), ("nboxes", C.c_int32)]
+                ("es_rows", C.c_int32), ("nboxes", C.c_int32), ("spin", C.c_int32)]
 
 
 class BoxC(C.Structure):
@@ -77,12 +77,16 @@ EXPORTS = {
     "pfsht_version": (C.c_char_p, []),
     "pfsht_plan_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double,
                                     C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_plan_create_spin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double,
+                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "pfsht_plan_destroy": (None, [C.c_void_p]),
     "pfsht_plan_info": (C.c_int, [C.c_void_p, C.POINTER(InfoC)]),
     "pfsht_synth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_rings_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                      C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_spin_rings_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                          C.c_int32, C.POINTER(C.c_void_p)]),
     "pfsht_rings_destroy": (None, [C.c_void_p]),
     "pfsht_rings_work": (C.c_int64, [C.c_void_p]),
     "pfsht_rings_mmax": (C.c_int32, [C.c_void_p]),
@@ -158,6 +162,11 @@ def sph_mode(l: int, m: int):
     """0-based (row, column) of (l, m) — src/Layout.jl:20."""
     am = abs(int(m))
     return (int(l) - am, 2 * am + (1 if m >= 0 else 0) - 1)
+
+
+def spin_coeff_index(l: int, m: int, lmax: int):
+    """spin_coeff_index(l, m, lmax) — src/Layout.jl:28-29, 0-based: (l, m + lmax)."""
+    return (int(l), int(m) + int(lmax))
 
 
 def zeros_alm(lmax: int, ntrans: int = 1):
@@ -280,7 +289,7 @@ class Plan:
     """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116."""
 
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
-                 method: str = "auto", precision: str = "fp64"):
+                 method: str = "auto", precision: str = "fp64", spin: int = 0):
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
         ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
         if th.shape != ph.shape or th.ndim != 1:
@@ -289,11 +298,14 @@ class Plan:
             raise ArgumentError("lmax must be nonnegative")
         if ntrans < 1:
             raise ArgumentError("ntrans must be >= 1")
+        if spin != 0 and abs(spin) > lmax:
+            raise ArgumentError(f"(|spin| = {abs(spin)}) > lmax = {lmax}")
         self.lmax, self.ntrans, self.tol, self.npts = int(lmax), int(ntrans), float(tol), len(th)
+        self.spin = int(spin)
         h = C.c_void_p()
-        _check(lib().pfsht_plan_create(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
-                                       len(th), self.lmax, self.tol, self.ntrans, METHODS[method],
-                                       {"fp64": 0, "mixed": 1}[precision], C.byref(h)))
+        _check(lib().pfsht_plan_create_spin(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
+                                            len(th), self.lmax, self.tol, self.ntrans, self.spin, METHODS[method],
+                                            {"fp64": 0, "mixed": 1}[precision], C.byref(h)))
         self._h = h
         inf = InfoC()
         _check(lib().pfsht_plan_info(self._h, C.byref(inf)))
@@ -315,7 +327,7 @@ class Plan:
     def __repr__(self):
         alg = {"direct": "DirectAlgorithm", "stretch": "StretchAlgorithm",
                "multibox": "MultiBoxAlgorithm"}[self.algorithm]
-        return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, "
+        return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, spin={self.spin}, "
                 f"algorithm={alg}, kind={self.info['kind_name']})")
 
     @property
@@ -562,13 +574,19 @@ def mul(out, op, x, stream=None):
 
 
 class Rings:
-    """RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255."""
+    """RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255; with spin != 0,
+    SpinRings(theta, lmax, spin, eps; ntrans, reflect) — src/Spin.jl:192-270 (coefficients in the
+    (l, m + lmax) layout)."""
 
-    def __init__(self, theta, lmax: int, eps: float, ntrans: int = 1, reflect: bool = False):
+    def __init__(self, theta, lmax: int, eps: float, ntrans: int = 1, reflect: bool = False, spin: int = 0):
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
         h = C.c_void_p()
-        _check(lib().pfsht_rings_create(th.ctypes.data_as(C.c_void_p), len(th), int(lmax), float(eps),
-                                        int(ntrans), 1 if reflect else 0, C.byref(h)))
+        if spin:
+            _check(lib().pfsht_spin_rings_create(th.ctypes.data_as(C.c_void_p), len(th), int(lmax), int(spin),
+                                                 float(eps), int(ntrans), 1 if reflect else 0, C.byref(h)))
+        else:
+            _check(lib().pfsht_rings_create(th.ctypes.data_as(C.c_void_p), len(th), int(lmax), float(eps),
+                                            int(ntrans), 1 if reflect else 0, C.byref(h)))
         self._h, self._owned = h, True
         self.lmax, self.n, self.ntrans = int(lmax), len(th), int(ntrans)
 
