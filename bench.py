@@ -73,7 +73,8 @@ def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", choices=["plain", "batched", "cap30", "equatorial", "cap15at60", "serve"],
+    ap.add_argument("--config", choices=["plain", "batched", "cap30", "equatorial", "cap15at60", "spin", "twocaps",
+                                         "serve"],
                     default="plain",
                     help="plain: the 20 deg cap at lmax 2048, M 2e6 (default, the headline workload); "
                          "batched: the same cap at lmax 1024, M 5.1e5, ntrans 16, against one-at-a-time "
@@ -112,7 +113,7 @@ def main():
 
     if args.config == "batched":
         return run_batched(args)
-    if args.config in ("cap30", "equatorial", "cap15at60"):
+    if args.config in ("cap30", "equatorial", "cap15at60", "spin", "twocaps"):
         return run_geometry(args)
     if args.config == "serve":
         return run_serve(args)
@@ -279,13 +280,21 @@ def run_geometry(args):
     import pfsht
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     L = 2048
-    geo = {"cap30": (30.0, 0.0, 1e-5, "30deg polar cap"),
-           "equatorial": (30.0, np.pi / 2, 1e-8, "30deg cap centred on the equator (interior)"),
-           "cap15at60": (15.0, np.deg2rad(60.0), 1e-8, "15deg cap centred at colatitude 60deg (interior)")}
-    deg, tc, tol, desc = geo[args.config]
+    geo = {"cap30": (30.0, 0.0, 1e-5, "30deg polar cap", 0),
+           "equatorial": (30.0, np.pi / 2, 1e-8, "30deg cap centred on the equator (interior)", 0),
+           "cap15at60": (15.0, np.deg2rad(60.0), 1e-8, "15deg cap centred at colatitude 60deg (interior)", 0),
+           "spin": (20.0, 0.0, 1e-8, "spin-2 field on a 20deg polar cap", 2),
+           "twocaps": (15.0, 0.0, 1e-8, "two antipodal 15deg caps (multi-box leaf)", 0)}
+    deg, tc, tol, desc, spin = geo[args.config]
     M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
     rng = np.random.default_rng(30)
     th, ph = centered_cap(rng, deg, M, tc, 0.5)
+    method = "stretch"
+    if args.config == "twocaps":
+        th2, ph2 = centered_cap(rng, deg, M, np.pi, 2.0)
+        th, ph = np.concatenate([th, th2]), np.concatenate([ph, ph2])
+        M = 2 * M
+        method = "auto"
     alm = rand_alm(L, rng)
     f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
     mixed = args.precision == "mixed"
@@ -293,12 +302,13 @@ def run_geometry(args):
         tol = 1e-6  # the tightest tolerance mixed precision serves
     log(f"{args.config}: plan lmax={L} M={M} tol={tol} precision={args.precision}")
     t0 = time.perf_counter()
-    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method="stretch", precision=args.precision)
+    plan = pfsht.Plan(th, ph, lmax=L, tol=tol, method=method, precision=args.precision, spin=spin)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
     info = plan.info
     log(f"plan {t_plan:.2f}s kind={info['kind_name']} nth={info['nth']} nph={info['nph']} "
-        f"ring_work={info['ring_work']} fine={info['es_n1']}x{info['es_n2']} w={info['es_w']}")
+        f"ring_work={info['ring_work']} fine={info['es_n1']}x{info['es_n2']} w={info['es_w']} "
+        f"nboxes={info['nboxes']} spin={info['spin']}")
     a_d, f_d = pfsht.to_device(alm), pfsht.to_device(f)
     fo = torch.empty(M, dtype=torch.complex128, device="cuda")
     ao = torch.empty(alm.size, dtype=torch.complex128, device="cuda")
@@ -322,7 +332,7 @@ def run_geometry(args):
     ss, sa = plan.stage_times("synth"), plan.stage_times("adjoint")
     # error against the device Direct leaf on 1e4 points
     sub = np.random.default_rng(31).choice(M, 10_000, replace=False)
-    pdir = pfsht.Plan(th[sub], ph[sub], lmax=L, method="direct")
+    pdir = pfsht.Plan(th[sub], ph[sub], lmax=L, method="direct", spin=spin)
     fsyn = pfsht.from_device(fo, (M,))
     fdir = pdir @ alm
     err_s = float(np.abs(fsyn[sub] - fdir).max() / np.abs(fdir).max())
@@ -335,15 +345,23 @@ def run_geometry(args):
     del pdir
     log(f"{steps} steps: {el / steps * 1e3:.3f} ms/step; error vs Direct: synthesis {err_s:.2e}, adjoint {err_a:.2e}")
     W = info["ring_work"]
-    ring_ms = sa["ring"][0] / max(sa["ring"][1], 1)
-    tf = 12.0 * W / (ring_ms * 1e-3) / 1e12
+    # the adjoint ring pass: its stage time (none for a multi-box plan, whose boxes are timed by
+    # their own plans); 12 flops per scalar triple (one triple serves orders +-m), 9 per spin triple
+    # (recurrence 5, complex accumulate 4; orders +-m are separate triples)
+    ring_ms = sa["ring"][0] / max(sa["ring"][1], 1) + (sa["ring_reduce"][0] / max(sa["ring_reduce"][1], 1) if spin else 0)
+    fl = 9.0 if spin else 12.0
+    tf = fl * W / (ring_ms * 1e-3) / 1e12 if ring_ms > 0 else None
+    kname = "spin_adj_kernel (spin adjoint ring pass)" if spin else "ring_adj_kernel (adjoint ring pass)"
+    roof = None if tf is None else {"kernel": kname, "bound": "mfma", "achieved": tf, "peak": PEAK_FP64_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS, "traffic": None,
+                                    "algorithmic": f"{fl:g} FP64 flops per retained triple x {W} triples"}
     out = {"metric": f"applies/s at lmax=2048 tol={tol:g} (synth+adjoint)", "value": 2 * steps / el,
            "unit": "applies/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64 recurrence, f32 grid and FFT" if mixed else "f64", "data": "synthetic",
            "config": {"workload": f"{desc}, lmax={L}, M={M} area-uniform, tol={tol:g}, ntrans=1, "
                                   f"{args.precision}, one synthesis + one adjoint per step",
-                      "precision": args.precision,
+                      "precision": args.precision, "spin": spin, "nboxes": info["nboxes"],
                       "lmax": L, "npts": M, "tol": tol, "kind": info["kind_name"], "nth": info["nth"],
                       "nph": info["nph"], "ring_work": W, "fine_grid": [info["es_n1"], info["es_n2"]], "fine_rows_stored": info["es_rows"],
                       "es_w": info["es_w"]},
@@ -351,10 +369,7 @@ def run_geometry(args):
            "plan_s": t_plan,
            "stage_ms": {"synth": {k: v[0] / max(v[1], 1) for k, v in ss.items()},
                         "adjoint": {k: v[0] / max(v[1], 1) for k, v in sa.items()}},
-           "roofline": {"kernel": "ring_adj_kernel (adjoint ring pass)", "bound": "mfma", "achieved": tf,
-                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
-                        "traffic": None,
-                        "algorithmic": f"12 FP64 flops per retained triple x {W} triples"},
+           "roofline": roof,
            "cpu_baseline": None}
     print(json.dumps(out))
 

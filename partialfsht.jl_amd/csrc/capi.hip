@@ -159,6 +159,7 @@ struct pfsht_rings_s {
         dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
         dfree(sd.l2c); dfree(sd.l2s); dfree(sd.seed); dfree(sd.Nl); dfree(sd.tab); dfree(sd.tofs); dfree(sd.items);
+        dfree(sd.pofs); dfree(sd.ntiles); dfree(sd.partials);
     }
 };
 
@@ -359,7 +360,18 @@ static Status rings_build_spin(const double *theta, int n, int L, int spin, doub
         const int am = std::abs(m), lo = r.jlo[am], hi = r.jhi[am];
         if (hi < lo) continue;
         r.work += int64_t(hi - lo + 1) * (L - std::max(am, as) + 1);
-        for (int j0 = lo; j0 <= hi; j0 += 64) items.push_back(make_int4(m, j0, std::min(64, hi - j0 + 1), 0));
+        for (int j0 = lo, t = 0; j0 <= hi; j0 += 64, ++t) items.push_back(make_int4(m, j0, std::min(64, hi - j0 + 1), t));
+    }
+    // adjoint partials: per order, tiles x retained degrees x transforms
+    std::vector<int64_t> spofs(2 * mmax + 1, 0);
+    std::vector<int32_t> sntiles(2 * mmax + 1, 0);
+    int64_t nsp = 0;
+    for (int m = -mmax; m <= mmax; ++m) {
+        const int am = std::abs(m), lo = r.jlo[am], hi = r.jhi[am];
+        spofs[m + mmax] = nsp;
+        if (hi < lo) continue;
+        sntiles[m + mmax] = (hi - lo + 64) / 64;
+        nsp += int64_t(sntiles[m + mmax]) * (L - std::max(am, as) + 1) * B;
     }
     RingDev &d = r.d;
     SpinDev &sd = r.sd;
@@ -379,6 +391,9 @@ static Status rings_build_spin(const double *theta, int n, int L, int spin, doub
     if ((st = upload(&sd.tofs, tofs, by)) != kOk) return st;
     if ((st = upload(&sd.items, items, by)) != kOk) return st;
     if ((st = dalloc(&sd.tab, size_t(std::max<int64_t>(ntab, 1)), by)) != kOk) return st;
+    if ((st = upload(&sd.pofs, spofs, by)) != kOk) return st;
+    if ((st = upload(&sd.ntiles, sntiles, by)) != kOk) return st;
+    if ((st = dalloc(&sd.partials, size_t(std::max<int64_t>(nsp, 1)), by)) != kOk) return st;
     if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
     launch_spin_tables(d, sd, nullptr);
     PFSHT_HIP_TRY(hipGetLastError());

@@ -101,8 +101,11 @@ struct SpinDev {
     double *Nl = nullptr;       // [L+1] sqrt((2l+1)/4pi)
     double4 *tab = nullptr;     // per (|m|, l >= l0): (a1, |a2|, a0, N_{l+1}) of the step l -> l+1
     int64_t *tofs = nullptr;    // [L+1] first entry of |m| in tab
-    int4 *items = nullptr;      // forward wavefronts: (m, j0, rings, 0)
+    int4 *items = nullptr;      // one wavefront per (order m, 64-ring tile): (m, j0, rings, tile)
     int nitems = 0;
+    int64_t *pofs = nullptr;    // [2 mmax + 1] per m + mmax: first adjoint partial of the order
+    int32_t *ntiles = nullptr;  // [2 mmax + 1] tiles of the order
+    double2 *partials = nullptr;  // adjoint per-(tile, l, b) ring sums, l >= max(|m|, |s|)
 };
 void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s);
 void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, double2 *Gd, hipStream_t s);

@@ -7,13 +7,14 @@
 // FP64 FMAs) and one complex product per transform, so the passes are FP64-issue bound like the
 // scalar ones.
 #include "pfsht_common.hpp"
+#include "adj_rows.hpp"
 
 #include <cfloat>
 
 namespace pfsht {
 
 namespace {
-constexpr int kSpinWaves = 8;  // wavefronts per adjoint workgroup: up to 8 tiles of one order at once
+constexpr int kSpinAdjWaves = 4;  // wavefronts per adjoint workgroup (each with its own LDS tile)
 
 // Seed of ring j for order m: log2 |d^{l0}_{m,-s}(theta_j)| = lb2 + pc log2 cos(theta/2) +
 // ps log2 sin(theta/2) (with the p == 0 terms left out, which makes the poles exact), split
@@ -45,37 +46,6 @@ __device__ __forceinline__ int spin_renorm(double &d1, double &d0, int &E) {
     return sh;
 }
 
-// Sum 16 values over the 64 lanes at once: each butterfly step halves the values a lane keeps
-// (exchanging the other half with lane ^ mask), then the last two steps sum across each quad.
-// Lane q ends with the total of value k = 8 b5 + 4 b4 + 2 b3 + b2 (bits of q) in v[0]; the four
-// lanes of a quad hold the same bits, and the order of every addition is fixed.
-__device__ __forceinline__ void reduce16(double (&v)[16], int lane) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const bool up = lane & 32;
-        const double keep = up ? v[8 + i] : v[i], send = up ? v[i] : v[8 + i];
-        v[i] = keep + __shfl_xor(send, 32);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const bool up = lane & 16;
-        const double keep = up ? v[4 + i] : v[i], send = up ? v[i] : v[4 + i];
-        v[i] = keep + __shfl_xor(send, 16);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const bool up = lane & 8;
-        const double keep = up ? v[2 + i] : v[i], send = up ? v[i] : v[2 + i];
-        v[i] = keep + __shfl_xor(send, 8);
-    }
-    {
-        const bool up = lane & 4;
-        const double keep = up ? v[1] : v[0], send = up ? v[0] : v[1];
-        v[0] = keep + __shfl_xor(send, 4);
-    }
-    v[0] += __shfl_xor(v[0], 2);
-    v[0] += __shfl_xor(v[0], 1);
-}
 }  // namespace
 
 // Recurrence coefficients of the step l -> l + 1 at order |m| (Spin.jl:417-432): with
@@ -176,110 +146,127 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
     }
 }
 
-// Adjoint (spin_alm_from_rings!, Spin.jl:324-334): one workgroup per order, one wavefront per
-// 64-ring tile (kSpinWaves tiles at a time); each wavefront reduces its rings' products per degree
-// in blocks of 16 degrees, and the wavefronts' sums are added into the order's column of alm in
-// wavefront order. alm must be zero on entry; every order writes only its own column.
+// Adjoint (spin_alm_from_rings!, Spin.jl:324-334): one wavefront per (order, 64-ring tile), as in
+// the forward. Per 16-degree sub-tile, phase 1 (lanes = rings) advances the recurrence and writes
+// d to the wavefront's LDS tile, phase 2 forms the tile's ring sums d G 2^E per degree (row
+// broadcast FMAs, then the sum over the four rows), and row 0 writes them to the tile's partials.
+// The ring's exponent rides on its G operands (exact, refreshed when a renormalisation every 64
+// degrees changes it), so one sub-tile shares one scale.
 template <int NB>
-__global__ void __launch_bounds__(64 * kSpinWaves)
-spin_adj_kernel(const double *__restrict__ costh, const double *__restrict__ l2c, const double *__restrict__ l2s,
-                const double4 *__restrict__ seed, const double *__restrict__ Nl, const double4 *__restrict__ tab,
-                const int64_t *__restrict__ tofs, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
-                const double2 *__restrict__ Gd, double2 *__restrict__ alm, int n, int L, int s, int mmax, int b0,
-                int nb) {
-    __shared__ double2 red[kSpinWaves][16][NB];
-    const int m = int(blockIdx.x) - mmax;
+__global__ void __launch_bounds__(64 * kSpinAdjWaves)
+spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__restrict__ costh,
+                const double *__restrict__ l2c, const double *__restrict__ l2s, const double4 *__restrict__ seed,
+                const double4 *__restrict__ tab, const int64_t *__restrict__ tofs, const int64_t *__restrict__ pofs,
+                const double2 *__restrict__ Gd, double2 *__restrict__ partials, int n, int L, int s, int mmax, int B,
+                int b0, int nb) {
+    __shared__ double lds[kSpinAdjWaves * kSub * kWS];
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int idx = int(blockIdx.x) * kSpinAdjWaves + wave;
+    if (idx >= nitems) return;  // wave-uniform; the kernel has no workgroup barrier
+    const int4 it = items[idx];
+    const int m = __builtin_amdgcn_readfirstlane(it.x), j0 = __builtin_amdgcn_readfirstlane(it.y);
+    const int nr = __builtin_amdgcn_readfirstlane(it.z), tile = __builtin_amdgcn_readfirstlane(it.w);
+    const int lane = threadIdx.x & 63;
     const int am = m < 0 ? -m : m, as = s < 0 ? -s : s;
     const int l0 = am > as ? am : as;
     if (l0 > L) return;
-    const int lo = jlo[am], hi = jhi[am];
-    if (hi < lo) return;
-    const int ntile = (hi - lo + 64) / 64;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t astride = int64_t(L + 1) * (2 * L + 1), ncol = 2 * mmax + 1;
-    double2 *col = alm + int64_t(m + L) * (L + 1) + int64_t(b0) * astride;
-    const double4 sd = seed[m + L];
-    const double4 *tb = tab + (tofs[am] - l0);
+    double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds + wave * kSub * kWS);
+    const bool act = lane < nr;
+    const int j = j0 + (act ? lane : 0);
+    const double x = costh[j];
+    double d1, d0 = 0.0;
+    int E;
+    spin_seed(seed[m + L], l2c[j], l2s[j], d1, E);
+    if (!act) d1 = 0.0;
+    const int64_t ncol = 2 * mmax + 1;
+    double g[NB][4], gs[NB][4];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        double2 G = make_double2(0.0, 0.0);
+        if (act && u < nb) G = Gd[(int64_t(b0 + u) * ncol + (m + mmax)) * n + j];
+        g[u][0] = G.x;
+        g[u][1] = G.y;
+        g[u][2] = g[u][3] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) gs[u][c] = ldexp(g[u][c], E);
+    }
+    const int t = lane & 15, row = lane >> 4;
+    const int wcol = lane + (lane >> 4);
+    const double *__restrict__ wr = &W[t][17 * row];
+    const int nl = L - l0 + 1;
+    double2 *__restrict__ out = partials + pofs[m + mmax] + (int64_t(tile) * nl) * B + b0;
+    const double4 *__restrict__ tb = tab + (tofs[am] - l0);
     const double sg2 = m * s <= 0 ? 1.0 : -1.0;
-    for (int tg = 0; tg * kSpinWaves < ntile; ++tg) {
-        const int tile = tg * kSpinWaves + w;
-        const int j = lo + tile * 64 + lane;
-        const bool ok = tile < ntile && j <= hi;
-        const int jj = ok ? j : lo;
-        const double x = costh[jj];
-        double d1 = 0.0, d0 = 0.0;
-        int E = 0;
-        if (ok) spin_seed(sd, l2c[jj], l2s[jj], d1, E);
-        double2 g[NB], gs[NB];
+    int since = 0;  // degrees since the last renormalisation
+    for (int lb = l0; lb <= L; lb += kSub) {
+        const int tl = min(kSub, L - lb + 1);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            g[b] = make_double2(0.0, 0.0);
-            if (ok && b < nb) g[b] = Gd[(int64_t(b0 + b) * ncol + (m + mmax)) * n + jj];
-            gs[b] = make_double2(ldexp(g[b].x, E), ldexp(g[b].y, E));
+        for (int k = 0; k < kSub; ++k) {
+            if (k < tl) {
+                const int l = lb + k;
+                if (l > l0) {
+                    const double4 c = tb[l - 1];
+                    const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
+                    d0 = d1;
+                    d1 = dn;
+                }
+                W[k][wcol] = d1;
+            }
         }
-        int steps = 0;
-        for (int lb = l0; lb <= L; lb += 16) {
-            double pr[NB][16], pi[NB][16];
+        wave_lds_order();
+        double a[NB][4];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int l = lb + k;
-                double val = 0.0;
-                if (l <= L) {
-                    if (l > l0) {
-                        const double4 c = tb[l - 1];
-                        const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
-                        d0 = d1;
-                        d1 = dn;
-                        if (++steps == 64) {
-                            steps = 0;
-                            if (l < L && spin_renorm(d1, d0, E) != 0) {
+        for (int u = 0; u < NB; ++u) a[u][0] = a[u][1] = a[u][2] = a[u][3] = 0.0;
+        RowSweep<0, NB, false>::run(wr, a, gs);
+        wave_lds_order();
 #pragma unroll
-                                for (int b = 0; b < NB; ++b) gs[b] = make_double2(ldexp(g[b].x, E), ldexp(g[b].y, E));
-                            }
-                        }
-                    }
-                    val = d1;
-                }
+        for (int u = 0; u < NB; ++u) {
+            a[u][0] = row_sum4(a[u][0]);
+            a[u][1] = row_sum4(a[u][1]);
+        }
+        if (row == 0 && t < tl) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    pr[b][k] = val * gs[b].x;
-                    pi[b][k] = val * gs[b].y;
-                }
+            for (int u = 0; u < NB; ++u)
+                if (u < nb) out[int64_t(lb - l0 + t) * B + u] = make_double2(a[u][0], a[u][1]);
+        }
+        since += tl;
+        if (since >= 64) {
+            since = 0;
+            if (spin_renorm(d1, d0, E) != 0) {
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) gs[u][c] = ldexp(g[u][c], E);
             }
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                reduce16(pr[b], lane);
-                reduce16(pi[b], lane);
-            }
-            if ((lane & 3) == 0) {
-                const int k = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
-#pragma unroll
-                for (int b = 0; b < NB; ++b) red[w][k][b] = make_double2(pr[b][0], pi[b][0]);
-            }
-            __syncthreads();
-            if (threadIdx.x < 16 * NB) {
-                const int k = threadIdx.x & 15, b = threadIdx.x >> 4;
-                const int l = lb + k;
-                if (l <= L && b < nb) {
-                    double sx = 0.0, sy = 0.0;
-                    for (int q = 0; q < kSpinWaves; ++q) {
-                        sx += red[q][k][b].x;
-                        sy += red[q][k][b].y;
-                    }
-                    const double N = Nl[l];
-                    double2 *o = col + b * astride + l;
-                    const double2 cur = *o;
-                    *o = make_double2(cur.x + sx * N, cur.y + sy * N);
-                }
-            }
-            __syncthreads();
         }
     }
 }
 
-__global__ void zero_fill_kernel(double2 *__restrict__ p, int64_t n) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-        p[i] = make_double2(0.0, 0.0);
+// Writes every alm slot: N_l times the tile-ordered sum of the order's partials where (l, m) is
+// retained (l >= max(|m|, |s|), |m| <= mmax), zero elsewhere (the adjoint overwrites alm).
+__global__ void spin_adj_finalize(const int64_t *__restrict__ pofs, const int32_t *__restrict__ ntiles,
+                                  const double2 *__restrict__ partials, const double *__restrict__ Nl,
+                                  double2 *__restrict__ alm, int L, int s, int mmax, int B) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b = blockIdx.z;
+    if (l > L) return;
+    const int m = c - L, am = m < 0 ? -m : m, as = s < 0 ? -s : s;
+    const int l0 = am > as ? am : as;
+    double vx = 0.0, vy = 0.0;
+    if (am <= mmax && l >= l0) {
+        const int nt = ntiles[m + mmax], nl = L - l0 + 1;
+        const double2 *p = partials + pofs[m + mmax] + int64_t(l - l0) * B + b;
+        for (int t = 0; t < nt; ++t) {
+            const double2 q = p[int64_t(t) * nl * B];
+            vx += q.x;
+            vy += q.y;
+        }
+        const double N = Nl[l];
+        vx *= N;
+        vy *= N;
+    }
+    alm[(int64_t(b) * (2 * L + 1) + c) * (L + 1) + l] = make_double2(vx, vy);
 }
 
 void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s) {
@@ -301,19 +288,22 @@ void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, do
 }
 
 void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, double2 *alm, hipStream_t s) {
-    const int64_t na = int64_t(r.L + 1) * (2 * r.L + 1) * r.B;
-    zero_fill_kernel<<<unsigned(std::min<int64_t>((na + 255) / 256, 4096)), 256, 0, s>>>(alm, na);
-    for (int b0 = 0; b0 < r.B; b0 += 2) {
-        const int nb = std::min(2, r.B - b0);
-        if (nb == 1)
-            spin_adj_kernel<1><<<2 * r.mmax + 1, 64 * kSpinWaves, 0, s>>>(
-                r.costh, sd.l2c, sd.l2s, sd.seed, sd.Nl, sd.tab, sd.tofs, r.jlo, r.jhi, Gd, alm, r.n, r.L, sd.s,
-                r.mmax, b0, nb);
-        else
-            spin_adj_kernel<2><<<2 * r.mmax + 1, 64 * kSpinWaves, 0, s>>>(
-                r.costh, sd.l2c, sd.l2s, sd.seed, sd.Nl, sd.tab, sd.tofs, r.jlo, r.jhi, Gd, alm, r.n, r.L, sd.s,
-                r.mmax, b0, nb);
+    if (sd.nitems > 0) {
+        const unsigned grid = unsigned((sd.nitems + kSpinAdjWaves - 1) / kSpinAdjWaves);
+        for (int b0 = 0; b0 < r.B; b0 += 4) {
+            const int nb = std::min(4, r.B - b0);
+#define PFSHT_SPIN_ADJ(NB)                                                                                   \
+    spin_adj_kernel<NB><<<grid, 64 * kSpinAdjWaves, 0, s>>>(sd.items, sd.nitems, r.costh, sd.l2c, sd.l2s, sd.seed, \
+                                                            sd.tab, sd.tofs, sd.pofs, Gd, sd.partials, r.n, r.L, sd.s, \
+                                                            r.mmax, r.B, b0, nb)
+            if (nb == 1) PFSHT_SPIN_ADJ(1);
+            else if (nb == 2) PFSHT_SPIN_ADJ(2);
+            else PFSHT_SPIN_ADJ(4);
+#undef PFSHT_SPIN_ADJ
+        }
     }
+    dim3 fgrid((r.L + 1 + 255) / 256, 2 * r.L + 1, r.B);
+    spin_adj_finalize<<<fgrid, 256, 0, s>>>(sd.pofs, sd.ntiles, sd.partials, sd.Nl, alm, r.L, sd.s, r.mmax, r.B);
 }
 
 }  // namespace pfsht
