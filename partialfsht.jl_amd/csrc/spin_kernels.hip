@@ -15,6 +15,7 @@ namespace pfsht {
 
 namespace {
 constexpr int kSpinAdjWaves = 4;  // wavefronts per adjoint workgroup (each with its own LDS tile)
+constexpr int kG = 4;             // degrees whose coefficients are fetched together
 
 // Seed of ring j for order m: log2 |d^{l0}_{m,-s}(theta_j)| = lb2 + pc log2 cos(theta/2) +
 // ps log2 sin(theta/2) (with the p == 0 terms left out, which makes the poles exact), split
@@ -110,28 +111,40 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
     const double4 *tb = tab + (tofs[am] - l0);
     const double sg2 = m * s <= 0 ? 1.0 : -1.0;  // sign of m n with n = -s
     int steps = 0;
-    for (int Lc = l0; Lc < L; ++Lc) {
-        const double4 c = tb[Lc];
-        const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
-        d0 = d1;
-        d1 = dn;
-        const double w = c.w * dn;
-        const double2 *cp = col + (Lc + 1);
+    // kG degrees per round: their coefficients and table entries are loaded (uniform loads, clamped
+    // in range) before any is used, so one load latency covers kG recurrence steps
+    for (int Lc = l0; Lc < L; Lc += kG) {
+        double4 c[kG];
+        double2 a[kG][NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            if (b < nb) {
-                const double2 a = cp[b * astride];
-                acc[b].x = fma(a.x, w, acc[b].x);
-                acc[b].y = fma(a.y, w, acc[b].y);
-            }
+        for (int q = 0; q < kG; ++q) {
+            const int lq = min(Lc + q, L - 1);
+            c[q] = tb[lq];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) a[q][b] = b < nb ? col[b * astride + lq + 1] : make_double2(0.0, 0.0);
         }
-        if (++steps == 64) {
-            steps = 0;
-            if (Lc + 1 < L) {
-                const int sh = spin_renorm(d1, d0, E);
-                if (sh != 0) {
 #pragma unroll
-                    for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
+        for (int q = 0; q < kG; ++q) {
+            if (Lc + q < L) {
+                const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
+                d0 = d1;
+                d1 = dn;
+                const double w = c[q].w * dn;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    acc[b].x = fma(a[q][b].x, w, acc[b].x);
+                    acc[b].y = fma(a[q][b].y, w, acc[b].y);
+                }
+                if (++steps == 64) {
+                    steps = 0;
+                    if (Lc + q + 1 < L) {
+                        const int sh = spin_renorm(d1, d0, E);
+                        if (sh != 0) {
+#pragma unroll
+                            for (int b = 0; b < NB; ++b)
+                                acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
+                        }
+                    }
                 }
             }
         }
@@ -201,16 +214,21 @@ spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__rest
     for (int lb = l0; lb <= L; lb += kSub) {
         const int tl = min(kSub, L - lb + 1);
 #pragma unroll
-        for (int k = 0; k < kSub; ++k) {
-            if (k < tl) {
-                const int l = lb + k;
-                if (l > l0) {
-                    const double4 c = tb[l - 1];
-                    const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
-                    d0 = d1;
-                    d1 = dn;
+        for (int k0 = 0; k0 < kSub; k0 += kG) {
+            double4 c[kG];  // steps to degrees lb + k0 .. lb + k0 + kG - 1, loaded first (clamped)
+#pragma unroll
+            for (int q = 0; q < kG; ++q) c[q] = tb[min(max(lb + k0 + q - 1, l0), L - 1)];
+#pragma unroll
+            for (int q = 0; q < kG; ++q) {
+                const int k = k0 + q, l = lb + k;
+                if (k < tl) {
+                    if (l > l0) {
+                        const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
+                        d0 = d1;
+                        d1 = dn;
+                    }
+                    W[k][wcol] = d1;
                 }
-                W[k][wcol] = d1;
             }
         }
         wave_lds_order();
