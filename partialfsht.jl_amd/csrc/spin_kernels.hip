@@ -108,46 +108,51 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
             }
         }
     }
-    const double4 *tb = tab + (tofs[am] - l0);
     const double sg2 = m * s <= 0 ? 1.0 : -1.0;  // sign of m n with n = -s
-    int steps = 0;
-    // kG degrees per round: their coefficients and table entries are loaded (uniform loads, clamped
-    // in range) before any is used, so one load latency covers kG recurrence steps
-    for (int Lc = l0; Lc < L; Lc += kG) {
+    // running pointers: the table entry of the step Lc -> Lc + 1 and the coefficient of degree Lc + 1
+    const double4 *__restrict__ tp = tab + tofs[am];
+    const double2 *__restrict__ ap = col + (l0 + 1);
+    auto step = [&](const double4 c, const double2 (&a)[NB]) {
+        const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
+        d0 = d1;
+        d1 = dn;
+        const double w = c.w * dn;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            acc[b].x = fma(a[b].x, w, acc[b].x);
+            acc[b].y = fma(a[b].y, w, acc[b].y);
+        }
+    };
+    // kG degrees per round, their table entries and coefficients loaded (uniform loads) before any
+    // is used, so one load latency covers kG steps; renormalised every 64 steps
+    int Lc = l0, steps = 0;
+    for (; Lc + kG <= L; Lc += kG, tp += kG, ap += kG) {
         double4 c[kG];
         double2 a[kG][NB];
 #pragma unroll
         for (int q = 0; q < kG; ++q) {
-            const int lq = min(Lc + q, L - 1);
-            c[q] = tb[lq];
+            c[q] = tp[q];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) a[q][b] = b < nb ? col[b * astride + lq + 1] : make_double2(0.0, 0.0);
+            for (int b = 0; b < NB; ++b)  // one transform per launch of the NB = 1 kernel: no test
+                a[q][b] = (NB == 1 || b < nb) ? ap[q + b * astride] : make_double2(0.0, 0.0);
         }
 #pragma unroll
-        for (int q = 0; q < kG; ++q) {
-            if (Lc + q < L) {
-                const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
-                d0 = d1;
-                d1 = dn;
-                const double w = c[q].w * dn;
+        for (int q = 0; q < kG; ++q) step(c[q], a[q]);
+        steps += kG;
+        if (steps == 64) {
+            steps = 0;
+            const int sh = spin_renorm(d1, d0, E);
+            if (sh != 0) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    acc[b].x = fma(a[q][b].x, w, acc[b].x);
-                    acc[b].y = fma(a[q][b].y, w, acc[b].y);
-                }
-                if (++steps == 64) {
-                    steps = 0;
-                    if (Lc + q + 1 < L) {
-                        const int sh = spin_renorm(d1, d0, E);
-                        if (sh != 0) {
-#pragma unroll
-                            for (int b = 0; b < NB; ++b)
-                                acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
-                        }
-                    }
-                }
+                for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
             }
         }
+    }
+    for (; Lc < L; ++Lc, ++tp, ++ap) {  // fewer than kG steps left
+        double2 a[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) a[b] = (NB == 1 || b < nb) ? ap[b * astride] : make_double2(0.0, 0.0);
+        step(*tp, a);
     }
     if (lane < nr) {
         const int64_t ncol = 2 * mmax + 1;
@@ -213,21 +218,38 @@ spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__rest
     int since = 0;  // degrees since the last renormalisation
     for (int lb = l0; lb <= L; lb += kSub) {
         const int tl = min(kSub, L - lb + 1);
+        if (tl == kSub && lb > l0) {  // a full sub-tile of steps: entries lb - 1 .. lb + 14
+            const double4 *__restrict__ tp = tb + (lb - 1);
 #pragma unroll
-        for (int k0 = 0; k0 < kSub; k0 += kG) {
-            double4 c[kG];  // steps to degrees lb + k0 .. lb + k0 + kG - 1, loaded first (clamped)
+            for (int k0 = 0; k0 < kSub; k0 += kG) {
+                double4 c[kG];
 #pragma unroll
-            for (int q = 0; q < kG; ++q) c[q] = tb[min(max(lb + k0 + q - 1, l0), L - 1)];
+                for (int q = 0; q < kG; ++q) c[q] = tp[k0 + q];
 #pragma unroll
-            for (int q = 0; q < kG; ++q) {
-                const int k = k0 + q, l = lb + k;
-                if (k < tl) {
-                    if (l > l0) {
-                        const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
-                        d0 = d1;
-                        d1 = dn;
+                for (int q = 0; q < kG; ++q) {
+                    const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
+                    d0 = d1;
+                    d1 = dn;
+                    W[k0 + q][wcol] = d1;
+                }
+            }
+        } else {  // the seed's sub-tile and the last one: guarded, entries clamped in range
+#pragma unroll
+            for (int k0 = 0; k0 < kSub; k0 += kG) {
+                double4 c[kG];
+#pragma unroll
+                for (int q = 0; q < kG; ++q) c[q] = tb[min(max(lb + k0 + q - 1, l0), L - 1)];
+#pragma unroll
+                for (int q = 0; q < kG; ++q) {
+                    const int k = k0 + q, l = lb + k;
+                    if (k < tl) {
+                        if (l > l0) {
+                            const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
+                            d0 = d1;
+                            d1 = dn;
+                        }
+                        W[k][wcol] = d1;
                     }
-                    W[k][wcol] = d1;
                 }
             }
         }
