@@ -220,7 +220,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
             nck += nseg - 1;
         }
         ntiles[m] = nt;
-        np += int64_t(nt) * (L - m + 1) * B;
+        np += int64_t(nt) * adj_rows_padded(m, L) * B;  // tile rows padded to whole sub-tiles
     }
     // Orders are dealt to the eight XCDs, longest first onto the least-loaded one, and workgroup i
     // is laid out for XCD i % 8 (the hardware deals consecutive workgroups round-robin): the tiles

@@ -112,6 +112,10 @@ void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, do
 // writes every slot of alm (zero outside the retained modes)
 void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, double2 *alm, hipStream_t s);
 
+// Degrees of one adjoint tile's partial rows for order m: L - m + 1 rounded up to whole 16-degree
+// sub-tiles, so that a sub-tile's stores never need a bounds test.
+__host__ __device__ inline int adj_rows_padded(int m, int L) { return ((L - m + 1) + 15) & ~15; }
+
 // The plan-time sweep that stores the checkpoints of every forward item (default stream).
 void launch_ring_checkpoints(const RingDev &r, hipStream_t s);
 

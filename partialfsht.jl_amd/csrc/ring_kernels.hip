@@ -694,11 +694,18 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const
                 a[u][3] = row_sum4(a[u][3]);
             }
         }
-        if (row == 0 && t < tl) {
+        {
+            // every lane stores: lane t of row r writes component r of degree l0 + t (the row sums
+            // are in every row). Degrees past the order's last (t >= tl, last sub-tile only) land in
+            // the tile's padding. No branch around the stores, so the wait for the next sub-tile's
+            // coefficients does not have to cover them.
+            double *__restrict__ od = reinterpret_cast<double *>(out);
 #pragma unroll
-            for (int u = 0; u < NB; ++u)
-                out[int64_t(l0 - m + t) * B + u] =
-                    make_double4(a[u][0] * cmul, a[u][1] * cmul, NEG ? a[u][2] * cmul : 0.0, NEG ? a[u][3] * cmul : 0.0);
+            for (int u = 0; u < NB; ++u) {
+                const double c01 = row == 0 ? a[u][0] : a[u][1];
+                const double c23 = NEG ? (row == 2 ? a[u][2] : a[u][3]) : 0.0;
+                od[(int64_t(l0 - m + t) * B + u) * 4 + row] = (row < 2 ? c01 : c23) * cmul;
+            }
         }
         gcv = gcn;
         kfirst = 0;
@@ -751,7 +758,7 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     const double2 *gpc = Gd + (int64_t(b0) * ncol + (mmax + m)) * n + jt;
     const double2 *gmc = Gd + (int64_t(b0) * ncol + (mmax - m)) * n + jt;
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
-    const int nl = L - m + 1;
+    const int nl = adj_rows_padded(m, L);
     double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
     int sink = 0;
     const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
@@ -777,7 +784,7 @@ __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_
     double pr = 0, pi = 0, nr_ = 0, ni = 0;
     if (l <= L && am <= mmax) {
         const int nt = ntiles[am];
-        const int nl = L - am + 1;
+        const int nl = adj_rows_padded(am, L);
         const double4 *p = partials + pofs[am] + int64_t(l - am) * B + b;
         for (int t = 0; t < nt; ++t) {
             const double4 q = p[int64_t(t) * nl * B];
