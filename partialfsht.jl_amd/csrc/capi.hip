@@ -195,7 +195,9 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     int64_t nab = 0;
     for (int m = 0; m <= mmax; ++m) { abofs[m] = nab; nab += std::max(0, L - m - 1); }
     // Work items: one per (order, 64-ring tile), with the order's degree segments; checkpoint rows
-    // for segments 1.. of every tile. Partials of the adjoint are indexed by the order's tile.
+    // for segments 1.. of every tile. An adjoint wavefront takes a group of tg consecutive tiles of
+    // an order (two when ntrans <= 2, which the registers allow); partials are indexed by the group.
+    const int tg = B <= 2 ? 2 : 1;
     std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
     std::vector<int32_t> ntiles(mmax + 1, 0);
     std::vector<int64_t> pofs(mmax + 1, 0), cost(mmax + 1, 0);
@@ -216,11 +218,12 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         fw = std::max(fw, nseg);
         for (int t = 0; t < nt; ++t) {
             const int j0 = lo + 64 * t;
-            tiles_of[m].push_back({m, j0, std::min(64, hi - j0 + 1), t, nseg, 0, nck});
+            tiles_of[m].push_back({m, j0, std::min(64, hi - j0 + 1), t, nseg, 0, nck, 0});
             nck += nseg - 1;
         }
-        ntiles[m] = nt;
-        np += int64_t(nt) * adj_rows_padded(m, L) * B;  // tile rows padded to whole sub-tiles
+        const int ngroups = (nt + tg - 1) / tg;
+        ntiles[m] = ngroups;
+        np += int64_t(ngroups) * adj_rows_padded(m, L) * B;  // group rows padded to whole sub-tiles
     }
     // Orders are dealt to the eight XCDs, longest first onto the least-loaded one, and workgroup i
     // is laid out for XCD i % 8 (the hardware deals consecutive workgroups round-robin): the tiles
@@ -237,22 +240,28 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         xq[x].push_back(m);
         xload[x] += cost[m];
     }
-    const SegItem empty{0, 0, 0, 0, 1, 0, 0};
+    const SegItem empty{0, 0, 0, 0, 1, 0, 0, 0};
     // forward: one workgroup per tile
     std::vector<std::vector<SegItem>> fq(kXcd);
     for (int x = 0; x < kXcd; ++x)
         for (int m : xq[x])
             for (const SegItem &t : tiles_of[m]) fq[x].push_back(t);
-    // adjoint: one wavefront per (tile, segment), kAdjWaves per workgroup
+    // adjoint: one wavefront per (tile group, segment), kAdjWaves per workgroup; the item is the
+    // group's first tile with tile = the group index and nr2 = the rings of its second tile
     std::vector<std::vector<SegItem>> aq(kXcd);
     for (int x = 0; x < kXcd; ++x) {
-        for (int m : xq[x])
-            for (const SegItem &t : tiles_of[m])
-                for (int sg = 0; sg < t.nseg; ++sg) {
-                    SegItem w = t;
+        for (int m : xq[x]) {
+            const auto &tl = tiles_of[m];
+            for (size_t t0 = 0; t0 < tl.size(); t0 += tg) {
+                SegItem w = tl[t0];
+                w.tile = int32_t(t0 / tg);
+                w.nr2 = tg > 1 && t0 + 1 < tl.size() ? tl[t0 + 1].nr : 0;
+                for (int sg = 0; sg < w.nseg; ++sg) {
                     w.seg = sg;
                     aq[x].push_back(w);
                 }
+            }
+        }
         while (aq[x].size() % kAdjWaves) aq[x].push_back(empty);
     }
     auto interleave = [&](const std::vector<std::vector<SegItem>> &q, size_t group) {
@@ -268,6 +277,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     };
     std::vector<SegItem> fitems = interleave(fq, 1), aitems = interleave(aq, kAdjWaves);
     d.fwd_waves = fw;
+    d.adj_tg = tg;
     d.nfitems = int(fitems.size());
     d.naitems = int(aitems.size());
     d.nck = nck;

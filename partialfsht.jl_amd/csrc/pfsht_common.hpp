@@ -50,6 +50,7 @@ struct SegItem {
     int32_t nseg;  // degree segments of order m
     int32_t seg;   // adjoint items: this wavefront's segment
     int64_t ck;    // checkpoint row of segment 1 of this tile; segment s uses row ck + s - 1
+    int32_t nr2;   // adjoint items with two tiles: rings of the second tile (0: none)
 };
 
 constexpr int kSegMax = 8;     // segments per order (forward workgroup: one wavefront each)
@@ -87,8 +88,9 @@ struct RingDev {
     int64_t nck = 0;
     int64_t *pofs = nullptr;    // [mmax+1] offset of order m's adjoint partials (in double4 units)
     int32_t *ntiles = nullptr;  // [mmax+1]
-    double4 *partials = nullptr;  // adjoint per-tile partial sums: (re+, im+, re-, im-) per (tile, l)
+    double4 *partials = nullptr;  // adjoint partial sums: (re+, im+, re-, im-) per (tile group, l)
     int64_t npartials = 0;
+    int adj_tg = 1;             // 64-ring tiles per adjoint wavefront (a tile group); ntiles counts groups
 };
 
 // Spin-weighted ring engine (src/Spin.jl:170-545), used with a RingDev that supplies n, L, B, mmax,

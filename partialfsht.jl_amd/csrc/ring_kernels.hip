@@ -581,46 +581,46 @@ struct AdjPhase1<kSub, GUARD, SCALED> {
                                                int) {}
 };
 
-// Degrees ls .. le of order m. Segment 0 (ls = m) starts from the seed qs; a later segment from
-// its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}), aprev = a_{ls-1}. ls - m is a multiple of 64.
-// Phase 1 runs the rescaled recurrence of the forward sweep over each 16-degree sub-tile (gamma_l
-// and C_l of lane k of a row give degree l0 + k), and the ring sums of degree l are multiplied by
-// C_l once, after the reduction. For one or two transforms the ring's exponent goes into its G
-// operands (G 2^E, exact, refreshed when E changes at a block end) and the tile holds r; with more
-// transforms (registers) the tile holds r 2^E.
-template <int NB, bool NEG>
-__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const double2 *__restrict__ gpc,
-                                          const double2 *__restrict__ gmc, int64_t gstride, int nr, double x,
-                                          double qs, double q1, double q2, double aprev, int E, int ls, int le,
-                                          int m, int L, int B, double (*W)[kWS], double4 *__restrict__ out,
-                                          int &sink) {
-    const int lane = threadIdx.x & 63;
-    const bool active = lane < nr;
-    // this lane's ring (phase-1 layout) for every transform: (re, im) of orders +m and -m
-    double g[NB][4];
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-        const double2 p = active ? gpc[u * gstride + lane] : make_double2(0.0, 0.0);
-        const double2 q = (NEG && active) ? gmc[u * gstride + lane] : make_double2(0.0, 0.0);
-        g[u][0] = p.x;
-        g[u][1] = p.y;
-        g[u][2] = q.x;
-        g[u][3] = q.y;
-    }
-    const int t = lane & 15, row = lane >> 4;
-    const int wcol = lane + (lane >> 4);
-    const double *__restrict__ wr = &W[t][17 * row];
-    constexpr bool kScaleG = NB <= 2;
-    constexpr int NG = kScaleG ? NB : 1;
-    double gs[NG][4];  // G 2^E
-    auto scale_g = [&]() {
+// Per-lane state of one 64-ring tile in the adjoint sweep.
+template <int NB>
+struct AdjTile {
+    static constexpr bool kScaleG = NB <= 2;
+    static constexpr int NG = kScaleG ? NB : 1;
+    double x, r1, r2;   // cos theta; the recurrence pair
+    double w0, w1;      // segment 0: the seed degrees m and m + 1 as stored in the tile
+    int E;              // the ring's exponent
+    double g[NB][4];    // (re, im) of orders +m and -m per transform
+    double gs[NG][4];   // G 2^E
+    __device__ __forceinline__ void scale_g() {
         if (kScaleG)
 #pragma unroll
             for (int u = 0; u < NG; ++u)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) gs[u][c] = ldexp(g[u][c], E);
-    };
-    scale_g();
+    }
+};
+
+// Degrees ls .. le of order m for TG 64-ring tiles of that order (the second may be empty). Segment 0
+// (ls = m) starts from the seed; a later segment from its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}),
+// aprev = a_{ls-1}. ls - m is a multiple of 64. Phase 1 runs the rescaled recurrence of the forward
+// sweep over each 16-degree sub-tile (gamma_l and C_l of lane k of a row give degree l0 + k); phase
+// 2 adds the tile's ring sums into the row accumulators; the tiles take turns in the one LDS tile,
+// and the accumulators, their sum over the four rows and the partial stores are shared. The ring
+// sums of degree l are multiplied by C_l once, after the reduction. For one or two transforms the
+// ring's exponent goes into its G operands (G 2^E, exact, refreshed when E changes at a block end)
+// and the tile holds r; with more transforms (registers) the tile holds r 2^E.
+template <int NB, bool NEG, int TG>
+__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTile<NB> (&T)[TG], bool two,
+                                          double aprev, int ls, int le, int m, int L, int B, double (*W)[kWS],
+                                          double4 *__restrict__ out, int &sink, const double (&qs)[TG],
+                                          const double (&q1)[TG], const double (&q2)[TG]) {
+    constexpr bool kScaleG = AdjTile<NB>::kScaleG;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 15, row = lane >> 4;
+    const int wcol = lane + (lane >> 4);
+    const double *__restrict__ wr = &W[t][17 * row];
+#pragma unroll
+    for (int k = 0; k < TG; ++k) T[k].scale_g();
 
     // (gamma, C) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
     // i < 16 owns the line at byte offset 128 i of a 128-degree window
@@ -632,23 +632,27 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const
     const bool has_tab = m + 2 <= L;
     auto coef = [&](int l0) { return has_tab ? gcm[min(max(l0 + t, m + 2), L)] : make_double2(0.0, 1.0); };
     double2 gcv = coef(ls);
-    double r1, r2;
     int kfirst = 0;
-    if (ls == m) {  // seed degrees m and m + 1, stored as q (q 2^E)
-        W[0][wcol] = kScaleG ? qs : ldexp(qs, E);
-        kfirst = 1;
-        r1 = qs;
-        r2 = 0.0;
-        if (m < L) {
-            const double q = first_step(m, x, qs);
-            W[1][wcol] = kScaleG ? q : ldexp(q, E);
-            kfirst = 2;
-            r1 = q;
-            r2 = qs * sqrt(double(2 * m + 3));
+#pragma unroll
+    for (int k = 0; k < TG; ++k) {
+        AdjTile<NB> &A = T[k];
+        if (ls == m) {  // seed degrees m and m + 1, stored as q (q 2^E)
+            A.w0 = kScaleG ? qs[k] : ldexp(qs[k], A.E);
+            kfirst = 1;
+            A.r1 = qs[k];
+            A.r2 = 0.0;
+            A.w1 = 0.0;
+            if (m < L) {
+                const double q = first_step(m, A.x, qs[k]);
+                A.w1 = kScaleG ? q : ldexp(q, A.E);
+                kfirst = 2;
+                A.r1 = q;
+                A.r2 = qs[k] * sqrt(double(2 * m + 3));
+            }
+        } else {
+            A.r1 = q1[k];
+            A.r2 = q2[k] * aprev;
         }
-    } else {
-        r1 = q1;
-        r2 = q2 * aprev;
     }
     for (int l0 = ls; l0 <= le; l0 += kSub) {
         const int tl = min(kSub, le - l0 + 1);
@@ -660,31 +664,42 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const
         const double2 gcn = coef(l0 + kSub);
         const double ng = -gcv.x, Cc = gcv.y;
         const double cmul = t < kfirst ? 1.0 : Cc;  // the seed degrees are stored unscaled
-        // phase 1: lanes = rings
-        if (tl == kSub && kfirst == 0) AdjPhase1<0, false, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, 0, kSub);
-        else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, 2, kSub);
-        else AdjPhase1<0, true, !kScaleG>::run(x, r1, r2, ng, E, W, wcol, kfirst, tl);
-        // back to the q scale: C of the sub-tile's last degree (the seed-only sub-tile ends the sweep)
+        // back to the q scale after the sub-tile: C of its last degree (the seed-only sub-tile ends
+        // the sweep)
         const int last = tl - 1;
         const double Ce = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(Cc), last),
                                            __builtin_amdgcn_readlane(__double2loint(Cc), last));
-        r1 *= Ce;
-        r2 *= Ce;
-        if (((l0 + kSub - m) & 63) == 0) {  // renormalise at the end of every 64-degree block
-            const int s = renorm_shift(r1, r2);
-            r1 = ldexp(r1, -s);
-            r2 = ldexp(r2, -s);
-            E += s;
-        }
-        wave_lds_order();
-        // phase 2: lane t of row r sums rings 16 r .. 16 r + 15 of degree l0 + t
+        const bool block_end = ((l0 + kSub - m) & 63) == 0;  // renormalise at every 64-degree block end
         double a[NB][4];
 #pragma unroll
         for (int u = 0; u < NB; ++u) a[u][0] = a[u][1] = a[u][2] = a[u][3] = 0.0;
-        if constexpr (kScaleG) RowSweep<0, NB, NEG>::run(wr, a, gs);
-        else RowSweep<0, NB, NEG>::run(wr, a, g);
-        if (((l0 + kSub - m) & 63) == 0) scale_g();  // the next sub-tile's r carry the new exponent
-        wave_lds_order();
+#pragma unroll
+        for (int k = 0; k < TG; ++k) {
+            if (k > 0 && !two) break;  // wave-uniform: the group has one tile
+            AdjTile<NB> &A = T[k];
+            // phase 1: lanes = rings
+            if (kfirst > 0) {
+                W[0][wcol] = A.w0;
+                if (kfirst == 2) W[1][wcol] = A.w1;
+            }
+            if (tl == kSub && kfirst == 0) AdjPhase1<0, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 0, kSub);
+            else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 2, kSub);
+            else AdjPhase1<0, true, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, kfirst, tl);
+            A.r1 *= Ce;
+            A.r2 *= Ce;
+            if (block_end) {
+                const int sh = renorm_shift(A.r1, A.r2);
+                A.r1 = ldexp(A.r1, -sh);
+                A.r2 = ldexp(A.r2, -sh);
+                A.E += sh;
+            }
+            wave_lds_order();
+            // phase 2: lane t of row r adds rings 16 r .. 16 r + 15 of degree l0 + t
+            if constexpr (kScaleG) RowSweep<0, NB, NEG>::run(wr, a, A.gs);
+            else RowSweep<0, NB, NEG>::run(wr, a, A.g);
+            if (block_end) A.scale_g();  // the next sub-tile's r carry the new exponent
+            wave_lds_order();
+        }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             a[u][0] = row_sum4(a[u][0]);
@@ -697,7 +712,7 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const
         {
             // every lane stores: lane t of row r writes component r of degree l0 + t (the row sums
             // are in every row). Degrees past the order's last (t >= tl, last sub-tile only) land in
-            // the tile's padding. No branch around the stores, so the wait for the next sub-tile's
+            // the group's padding. No branch around the stores, so the wait for the next sub-tile's
             // coefficients does not have to cover them.
             double *__restrict__ od = reinterpret_cast<double *>(out);
 #pragma unroll
@@ -713,9 +728,10 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, const
     sink ^= pending;
 }
 
-// Workgroups of kAdjWaves wavefronts; each wavefront is one item: one degree segment of one
-// 64-ring tile of one order, writing that tile's partials for the segment's degrees.
-template <int NB>
+// Workgroups of kAdjWaves wavefronts; each wavefront is one item: one degree segment of TG
+// consecutive 64-ring tiles of one order (the second may be empty), writing that tile group's
+// partials for the segment's degrees.
+template <int NB, int TG>
 __global__ void __launch_bounds__(64 * kAdjWaves)
 ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
@@ -730,41 +746,59 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     const int lane = threadIdx.x & 63;
     const int nr = __builtin_amdgcn_readfirstlane(it.nr);
     if (nr <= 0) return;  // an empty slot (wave-uniform): the whole wavefront leaves
+    const int nr2 = TG > 1 ? __builtin_amdgcn_readfirstlane(it.nr2) : 0;
     const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
     const int sg = __builtin_amdgcn_readfirstlane(it.seg);
-    const int tile = __builtin_amdgcn_readfirstlane(it.tile);
-    const int jt = it.j0;  // first ring of the tile
+    const int group = __builtin_amdgcn_readfirstlane(it.tile);
     double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds_adj + wave * kSub * kWS);
-    const int j = jt + (lane < nr ? lane : 0);
-
-    const double x = costh[j];
-    double qs = 0.0, q1 = 0.0, q2 = 0.0;
-    int E;
-    if (sg == 0) {
-        seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], qs, E);
-        if (lane >= nr) qs = 0.0;
-    } else {
-        const int64_t row = (it.ck + sg - 1) * 64 + lane;
-        const double2 c = ckq[row];
-        q1 = c.x;
-        q2 = c.y;
-        E = ckE[row];
-    }
     const int ls = seg_begin(m, L, nseg, sg);
     const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
-
     const int64_t ncol = 2 * mmax + 1;
     const int64_t gstride = ncol * n;  // next transform
-    const double2 *gpc = Gd + (int64_t(b0) * ncol + (mmax + m)) * n + jt;
-    const double2 *gmc = Gd + (int64_t(b0) * ncol + (mmax - m)) * n + jt;
+
+    AdjTile<NB> T[TG];
+    double qs[TG], q1[TG], q2[TG];
+#pragma unroll
+    for (int k = 0; k < TG; ++k) {
+        const int nrk = k == 0 ? nr : nr2;
+        const int jt = it.j0 + 64 * k;  // first ring of the tile
+        const bool active = lane < nrk;
+        const int j = active ? jt + lane : it.j0;
+        AdjTile<NB> &A = T[k];
+        A.x = costh[j];
+        qs[k] = q1[k] = q2[k] = 0.0;
+        if (sg == 0) {
+            seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], qs[k], A.E);
+            if (!active) qs[k] = 0.0;
+        } else {
+            // tile k's checkpoint rows follow tile 0's: one row per segment >= 1 per tile
+            const int64_t rowk = (it.ck + int64_t(k) * (nseg - 1) + sg - 1) * 64 + lane;
+            const double2 c = (k == 0 || nrk > 0) ? ckq[rowk] : make_double2(0.0, 0.0);
+            q1[k] = active ? c.x : 0.0;
+            q2[k] = active ? c.y : 0.0;
+            A.E = (k == 0 || nrk > 0) ? ckE[rowk] : 0;
+        }
+        const double2 *gpc = Gd + (int64_t(b0) * ncol + (mmax + m)) * n + jt;
+        const double2 *gmc = Gd + (int64_t(b0) * ncol + (mmax - m)) * n + jt;
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const double2 p = active ? gpc[u * gstride + lane] : make_double2(0.0, 0.0);
+            const double2 q = (m > 0 && active) ? gmc[u * gstride + lane] : make_double2(0.0, 0.0);
+            A.g[u][0] = p.x;
+            A.g[u][1] = p.y;
+            A.g[u][2] = q.x;
+            A.g[u][3] = q.y;
+        }
+    }
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
     const int nl = adj_rows_padded(m, L);
-    double4 *__restrict__ out = partials + pofs[m] + (int64_t(tile) * nl) * B + b0;
+    double4 *__restrict__ out = partials + pofs[m] + (int64_t(group) * nl) * B + b0;
     int sink = 0;
     const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
     const double aprev = sg == 0 ? 0.0 : abm[ls - 1].x;
-    if (m > 0) adj_sweep<NB, true>(gcm, gpc, gmc, gstride, nr, x, qs, q1, q2, aprev, E, ls, le, m, L, B, W, out, sink);
-    else adj_sweep<NB, false>(gcm, gpc, gmc, gstride, nr, x, qs, q1, q2, aprev, E, ls, le, m, L, B, W, out, sink);
+    const bool two = nr2 > 0;
+    if (m > 0) adj_sweep<NB, true, TG>(gcm, T, two, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+    else adj_sweep<NB, false, TG>(gcm, T, two, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
@@ -809,15 +843,22 @@ void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
         for_transform_groups(r.B, [&](int nb, int b0) {
             dim3 grid(r.naitems / kAdjWaves);
             const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
-#define PFSHT_ADJ(NB)                                                                                      \
-    ring_adj_kernel<NB><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, \
-                                                          r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, r.mmax,  \
-                                                          r.B, b0, kNeverSink)
+#define PFSHT_ADJ(NB, TG)                                                                                    \
+    ring_adj_kernel<NB, TG><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,  \
+                                                              r.abofs, r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, \
+                                                              r.mmax, r.B, b0, kNeverSink)
+            // the item list was built for r.adj_tg tiles per wavefront (two when ntrans <= 2)
             switch (nb) {
-                case 8: PFSHT_ADJ(8); break;
-                case 4: PFSHT_ADJ(4); break;
-                case 2: PFSHT_ADJ(2); break;
-                default: PFSHT_ADJ(1); break;
+                case 8: PFSHT_ADJ(8, 1); break;
+                case 4: PFSHT_ADJ(4, 1); break;
+                case 2:
+                    if (r.adj_tg == 2) PFSHT_ADJ(2, 2);
+                    else PFSHT_ADJ(2, 1);
+                    break;
+                default:
+                    if (r.adj_tg == 2) PFSHT_ADJ(1, 2);
+                    else PFSHT_ADJ(1, 1);
+                    break;
             }
 #undef PFSHT_ADJ
         });
