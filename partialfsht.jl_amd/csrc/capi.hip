@@ -733,19 +733,23 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
         u.r0 = best > 0 ? best_end : 0;
         u.nR = u.N1 - best;
     }
-    // Non-empty super-tiles, and per (slot, local bin) the start of its point range: a slot's bins
-    // are contiguous in the sorted order and empty super-tiles own no points, so bo[q + 1] is the
-    // end of range q for every q, with bo[nst * 16] = M.
+    // Non-empty super-tiles, and per (slot, local bin) the start of its point range: a super-tile's
+    // bins are contiguous in the sorted order, so bin q's end is the next bin's start, and the
+    // super-tile's end follows its 16 starts.
     std::vector<int32_t> st_list, st_slot(size_t(u.ns1) * u.ns2, -1);
-    for (int s = 0; s < u.ns1 * u.ns2; ++s) {
-        if (cnt[int64_t(s) * 16 + 16] == cnt[int64_t(s) * 16]) continue;
-        st_slot[s] = int32_t(st_list.size());
-        st_list.push_back(s);
-    }
-    std::vector<int32_t> bo(st_list.size() * 16 + 1);
+    for (int s = 0; s < u.ns1 * u.ns2; ++s)
+        if (cnt[int64_t(s) * 16 + 16] != cnt[int64_t(s) * 16]) st_list.push_back(s);
+    // Slots in order of decreasing point count (ties by super-tile): the spreading and interpolation
+    // workgroups then start with the heaviest super-tiles and the light ones fill in behind them,
+    // instead of the densest rows of a cap arriving last as a tail. Per slot, bin_ofs holds the 16
+    // bin starts and the super-tile's end (slots are no longer consecutive in point order).
+    std::stable_sort(st_list.begin(), st_list.end(), [&](int32_t a, int32_t b) {
+        return cnt[int64_t(a) * 16 + 16] - cnt[int64_t(a) * 16] > cnt[int64_t(b) * 16 + 16] - cnt[int64_t(b) * 16];
+    });
+    for (size_t slot = 0; slot < st_list.size(); ++slot) st_slot[st_list[slot]] = int32_t(slot);
+    std::vector<int32_t> bo(std::max<size_t>(st_list.size(), 1) * 17);
     for (size_t slot = 0; slot < st_list.size(); ++slot)
-        for (int lb = 0; lb < 16; ++lb) bo[slot * 16 + lb] = int32_t(cnt[int64_t(st_list[slot]) * 16 + lb]);
-    bo[st_list.size() * 16] = int32_t(M);
+        for (int lb = 0; lb <= 16; ++lb) bo[slot * 17 + lb] = int32_t(cnt[int64_t(st_list[slot]) * 16 + lb]);
     u.nst = int(st_list.size());
     // pc1[i] = phase[i] * h1/psihat1(k_i), phase = cis(-k shift)/(nth nph) (src/Stretch.jl:229-233)
     std::vector<double> c1, c2;

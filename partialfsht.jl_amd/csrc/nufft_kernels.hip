@@ -305,7 +305,7 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + (y0 + c) % N2];
     }
     __syncthreads();
-    const int start = bin_ofs[slot * 16], end = bin_ofs[slot * 16 + 16];
+    const int start = bin_ofs[slot * 17], end = bin_ofs[slot * 17 + 16];
     for (int p = start + int(threadIdx.x); p < end; p += 256) {
         const double x1 = X1[p], x2 = X2[p];
         const int b1 = int(ceil(x1 - 0.5 * W));
@@ -370,7 +370,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const double *__restrict__ Xa = axis == 0 ? X1 : X2;
     const double2 *__restrict__ fb = f + int64_t(b) * M;
     for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_double2(0.0, 0.0);
-    const int32_t *__restrict__ bo = bin_ofs + slot * 16;
+    const int32_t *__restrict__ bo = bin_ofs + slot * 17;
     // the bin being accumulated is always tile 0; after each bin the tiles rotate by one, so after
     // all of its bins tile k holds bin wave + kSpreadWaves k again
     constexpr int NBW = 16 / kSpreadWaves;  // bins per wavefront

@@ -168,7 +168,7 @@ struct NufftDev {
     int nst = 0;                          // non-empty super-tiles
     int32_t *st_list = nullptr;           // [nst] super-tile ids (s1 * ns2 + s2)
     int32_t *st_slot = nullptr;           // [ns1*ns2] slot in Wst or -1 when empty
-    int32_t *bin_ofs = nullptr;           // [nst*16 + 1] point ranges per (super-tile, local bin)
+    int32_t *bin_ofs = nullptr;           // [nst*17] per slot: the 16 local bins' starts, then the end
     void *Wst = nullptr;                  // [B][nst][win][win] super-tile windows (grid storage type)
 };
 // LDS of one interpolation workgroup: the super-tile window of the fine grid.
