@@ -107,7 +107,7 @@ def main():
         print(json.dumps({"copy_bytes": 1 << 28, "launches": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (collectives are out of "
+        print("bench.py measures one GPU: the library has no multi-GPU path (collectives are out of "
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

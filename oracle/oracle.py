@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/pfsht_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg as the checker. The product library never imports this module.
+cpu_baseline leg, as the reference results are compared against. The product library never
+imports this module.
 
 Arrays follow the reference's column-major layouts (Layout.jl:13-20; Stretch.jl:215-220):
 numpy arrays here are Fortran-ordered complex128, alm of shape (L+1, 2L+1[, B]) and f of shape

@@ -1,7 +1,7 @@
 /*
  * pfsht_oracle.c — CPU restatement of PartialFSHT.jl's stretched-leaf apply path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker that tests/, the smoke check and the benchmark's
+ * TEST INFRASTRUCTURE ONLY: the CPU results that tests/, the smoke check and the benchmark's
  * CPU-baseline leg compare the HIP library against. It is never linked into, loaded by or called
  * from the product.
  *

@@ -1,7 +1,7 @@
 """Device parity of libpfsht_hip against the CPU oracle, through the C ABI.
 
 Mirrors gpu/runtests.jl of the reference (ring pass forward/adjoint, end-to-end device vs host,
-adjoint identity on device) and the north-star checks: synthesis and adjoint against the oracle's
+adjoint identity on device) and the acceptance checks: synthesis and adjoint against the oracle's
 stretched leaf on the same inputs (within tol), against the FP64 Direct oracle (<= 10 tol at
 small sizes, <= 1e-8 at lmax 2048, M = 2e6), the adjoint identity to round-off, and bit-identical
 results run to run. Tolerances are written in each test.
@@ -159,7 +159,8 @@ def test_config1_synth_adjoint_vs_oracle(torch, P, O, rng):
     ao = po.adjoint(f)
     assert rel_modes(O, ag, ao, L) < 1e-11
     assert np.all(ag[~O.valid_mask(L)] == 0)  # padding overwritten with zeros
-    # adjoint identity <Y a, f> = <a, Y' f> to round-off (north star ~1e-13; reference gate 1e-12)
+    # adjoint identity <Y a, f> = <a, Y' f> to round-off (target ~1e-13; the reference's gate is
+    # 1e-12, test/test_transforms.jl:111-123)
     lhs = np.vdot(f, fg)
     rhs = np.vdot(ag, a)
     assert abs(lhs - rhs) / abs(lhs) < 1e-12
