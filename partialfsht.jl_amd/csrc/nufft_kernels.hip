@@ -39,23 +39,6 @@ __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOM
 // in SGPRs where they are used), never loaded.
 #include "es_horner_tables.inc"
 
-// A coefficient passed through an empty asm: inside a loop its materialisation cannot be hoisted
-// out (where, with all of a row's coefficients live at once, they would be parked in VGPRs).
-__device__ __forceinline__ double launder(double c) {
-    asm volatile("" : "+s"(c));
-    return c;
-}
-template <int W, int Off, int N, int... J>
-__device__ __forceinline__ double horner_seq_l(double x, std::integer_sequence<int, J...>) {
-    double acc = launder(EsHorner<W>::c[Off + N - 1]);
-    ((acc = fma(acc, x, launder(EsHorner<W>::c[Off + N - 2 - J]))), ...);
-    return acc;
-}
-template <int W, int Off, int N>
-__device__ __forceinline__ double horner_l(double x) {
-    return horner_seq_l<W, Off, N>(x, std::make_integer_sequence<int, N - 1>{});
-}
-
 template <int W, int Off, int N, int... J>
 __device__ __forceinline__ double horner_seq(double x, std::integer_sequence<int, J...>) {
     double acc = EsHorner<W>::c[Off + N - 1];
@@ -83,16 +66,50 @@ __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
 }
 
 // Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
-// v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2). Coefficients are
-// laundered: this is evaluated inside the spreading chunk loop.
-template <int W, int... T>
-__device__ __forceinline__ void es_half_interior(double v, double *k, std::integer_sequence<int, T...>) {
-    ((k[T + 1] = horner_l<W, T * EsHorner<W>::nint, EsHorner<W>::nint>(v)), ...);
+// v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2): the edge
+// polynomial (k[0]) and the interior ones, evaluated by Horner in lockstep, so that the polynomials
+// end on the same step and one step of all of them issues together.
+//
+// A coefficient reaches its FMA through an empty asm that also takes the polynomial's running value:
+// the asm cannot move above the previous step, so each coefficient is materialised in SGPRs just
+// before its use. Left free, the compiler materialises the coefficients of many steps at once, which
+// exceeds the SGPR file inside the spreading loop, and hoisting them out of the loop would park them
+// in VGPRs.
+__device__ __forceinline__ double coef_after(double c, double acc) {
+    asm volatile("" : "+s"(c) : "v"(acc));
+    return c;
 }
 template <int W>
+struct HalfRow {
+    static constexpr int NH = (W + 1) / 2;  // cells per half row: the edge cell and NH - 1 interior ones
+    static constexpr int ni = EsHorner<W>::nint, ne = EsHorner<W>::nedge;
+    static constexpr int EOFF = (W - 2) * ni;                      // edge coefficients
+    static constexpr int NS = (ne > ni ? ne : ni) - 1;             // Horner steps of the longest
+    static constexpr int S0E = NS - (ne - 1), S0I = NS - (ni - 1);  // first step of each polynomial
+
+    template <int S, int... T>
+    static __device__ __forceinline__ void step(double e, double v, double (&a)[NH], std::integer_sequence<int, T...>) {
+        if constexpr (S >= S0E)
+            a[0] = fma(a[0], e, coef_after(EsHorner<W>::c[EOFF + ne - 2 - (S - S0E)], a[0]));
+        if constexpr (S >= S0I)
+            ((a[T + 1] = fma(a[T + 1], v, coef_after(EsHorner<W>::c[T * ni + ni - 2 - (S - S0I)], a[T + 1]))), ...);
+    }
+    template <int... S>
+    static __device__ __forceinline__ void steps(double e, double v, double (&a)[NH], std::integer_sequence<int, S...>) {
+        (step<S>(e, v, a, std::make_integer_sequence<int, NH - 1>{}), ...);
+    }
+    template <int... T>
+    static __device__ __forceinline__ void init(double (&a)[NH], std::integer_sequence<int, T...>) {
+        a[0] = coef_after(EsHorner<W>::c[EOFF + ne - 1], 0.0);
+        ((a[T + 1] = coef_after(EsHorner<W>::c[T * ni + ni - 1], 0.0)), ...);
+    }
+};
+template <int W>
 __device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2]) {
-    k[0] = horner_l<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 + v)) - 1.0);
-    es_half_interior<W>(v, k, std::make_integer_sequence<int, (W + 1) / 2 - 1>{});
+    using H = HalfRow<W>;
+    const double e = 2.0 * sqrt(0.5 * (1.0 + v)) - 1.0;
+    H::init(k, std::make_integer_sequence<int, H::NH - 1>{});
+    H::steps(e, v, k, std::make_integer_sequence<int, H::NS>{});
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -359,7 +376,8 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const int st = st_list[slot];
     const int S4 = 4 * S;
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // rows are kRowPad doubles apart so that the 16 lanes writing one entry of 16 rows, and the
     // MFMA operand reads, fall in different LDS banks
     double *K1 = lds_sp + 2 * win * win + wave * kSpreadStage;  // [16][kRowPad] row kernel
@@ -371,9 +389,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const double2 *__restrict__ fb = f + int64_t(b) * M;
     for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_double2(0.0, 0.0);
     const int32_t *__restrict__ bo = bin_ofs + slot * 17;
-    // the bin being accumulated is always tile 0; after each bin the tiles rotate by one, so after
-    // all of its bins tile k holds bin wave + kSpreadWaves k again
-    constexpr int NBW = 16 / kSpreadWaves;  // bins per wavefront
+    constexpr int NBW = 16 / kSpreadWaves;  // bins per wavefront; tile k accumulates bin wave + kSpreadWaves k
     double4v dr[NBW], di[NBW];
 #pragma unroll
     for (int k = 0; k < NBW; ++k) {
@@ -386,105 +402,85 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         bs[k] = bo[wave + kSpreadWaves * k];
         be[k] = bo[wave + kSpreadWaves * k + 1];
     }
-    // first chunk: (k, c) with c the chunk start in bin k; k = NBW when none is left
-    int k = 0, c = bs[0];
-    while (k < NBW && c >= be[k]) {
-        ++k;
-        if (k < NBW) c = bs[k];
-    }
-    // one rotation per empty bin skipped (all NBW, an identity, when the wave has no points)
+    // Chunk cursors: (k1, c1) is the next chunk (bin k1, first point c1), (k2, c2) the one after;
+    // k = NBW when none is left. The next chunk's coordinate and f value are in flight while a chunk
+    // is worked on, and so is the point index of the one after, so the f gather never waits on the
+    // index load. The loads are unconditional (a lane past a chunk's end reads the super-tile's last
+    // point and its value is never used), so that waiting for one chunk's inputs never waits for the
+    // loads issued after them.
+    auto advance = [&](int &k, int &c) {
+        while (k < NBW && c >= be[k]) {
+            ++k;
+            if (k < NBW) c = bs[k];
+        }
+    };
+    const int plast = max(bo[16] - 1, 0);
+    int k1 = 0, c1 = bs[0];
+    advance(k1, c1);
+    int k2 = k1, c2 = c1 + 16;
+    advance(k2, c2);
+    double x1 = Xa[min(c1 + pa, plast)];
+    double2 f1 = fb[perm[min(c1 + pa, plast)]];
+    int p2 = perm[min(c2 + pa, plast)];
+    // Bins in a static order, so each accumulates into a fixed register tile.
 #pragma unroll
-    for (int r = 0; r < NBW; ++r)
-        if (r < k) {
-            const double4v tr = dr[0], ti = di[0];
-#pragma unroll
-            for (int q = 0; q + 1 < NBW; ++q) {
-                dr[q] = dr[q + 1];
-                di[q] = di[q + 1];
-            }
-            dr[NBW - 1] = tr;
-            di[NBW - 1] = ti;
-        }
-    double xn = 0.0;
-    double2 fn = make_double2(0.0, 0.0);
-    if (k < NBW && c + pa < be[k]) {
-        xn = Xa[c + pa];
-        if (axis == 0 && h == 0) fn = fb[perm[c + pa]];
-    }
-    while (k < NBW) {
-        const int n = min(16, be[k] - c);
-        const double x = xn;
-        const double2 fv = fn;
-        // the next chunk, possibly in a later bin
-        int k2 = k, c2 = c + 16;
-        while (k2 < NBW && c2 >= be[k2]) {
-            ++k2;
-            if (k2 < NBW) c2 = bs[k2];
-        }
-        if (k2 < NBW && c2 + pa < be[k2]) {
-            xn = Xa[c2 + pa];
-            if (axis == 0 && h == 0) fn = fb[perm[c2 + pa]];
-        }
-        if (pa < n) {
-            const int bc = int(ceil(x - 0.5 * W));
-            const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
-            const double u = 2.0 * (double(bc) - x) + double(W - 1);
-            double kv[NH];
-            es_half_row<W>(h == 0 ? u : -u, kv);
+    for (int kb = 0; kb < NBW; ++kb) {
+        while (k1 == kb) {  // wave-uniform
+            const int n = min(16, be[kb] - c1);
+            const double x = x1;
+            const double2 fv = f1;
+            // shift the cursors by one chunk and issue the loads of the new pair
+            k1 = k2;
+            c1 = c2;
+            c2 += 16;
+            advance(k2, c2);
+            x1 = Xa[min(c1 + pa, plast)];
+            f1 = fb[p2];
+            p2 = perm[min(c2 + pa, plast)];
+            // Staging rows of every lane are cleared, so rows and f of points q >= n are zero and
+            // their products add exact zeros; lanes of points q < n then write their kernel values.
             double *row = (axis == 0 ? K1 : K2) + pa * kRowPad;
 #pragma unroll
             for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0;
-            // the middle cell of an odd W is written by the left half only
+            if (pa < n) {
+                const int bc = int(ceil(x - 0.5 * W));
+                const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
+                const double u = 2.0 * (double(bc) - x) + double(W - 1);
+                double kv[NH];
+                es_half_row<W>(h == 0 ? u : -u, kv);
+                // the middle cell of an odd W is written by the left half only
 #pragma unroll
-            for (int t = 0; t < NH; ++t)
-                if (h == 0) row[o + t] = kv[t];
-                else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
-            if (axis == 0 && h == 0) Fc[pa] = fv;
-        }
-        wave_lds_order();
-        // all operands of the chunk first (one LDS wait), then up to four MFMA pairs; staging rows of
-        // points q >= n hold stale values and are selected away, never multiplied
-        double av[4], cv[4];
-        double2 fq[4];
-#pragma unroll
-        for (int gi = 0; gi < 4; ++gi) {
-            const int q = 4 * gi + kq;
-            av[gi] = K1[q * kRowPad + ii];
-            cv[gi] = K2[q * kRowPad + ii];
-            fq[gi] = Fc[q];
-        }
-#pragma unroll
-        for (int gi = 0; gi < 4; ++gi) {
-            if (4 * gi >= n) break;  // wave-uniform
-            const bool ok = 4 * gi + kq < n;
-            const double are = ok ? av[gi] * fq[gi].x : 0.0;
-            const double aim = ok ? av[gi] * fq[gi].y : 0.0;
-            const double cc = ok ? cv[gi] : 0.0;
-            dr[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, cc, dr[0], 0, 0, 0);
-            di[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, cc, di[0], 0, 0, 0);
-        }
-        wave_lds_order();
-        // one rotation per bin finished (the current one and any empty ones skipped)
-#pragma unroll
-        for (int r = 0; r < NBW; ++r)
-            if (r < k2 - k) {
-                const double4v tr = dr[0], ti = di[0];
-#pragma unroll
-                for (int q = 0; q + 1 < NBW; ++q) {
-                    dr[q] = dr[q + 1];
-                    di[q] = di[q + 1];
-                }
-                dr[NBW - 1] = tr;
-                di[NBW - 1] = ti;
+                for (int t = 0; t < NH; ++t)
+                    if (h == 0) row[o + t] = kv[t];
+                    else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
             }
-        k = k2;
-        c = c2;
+            if (axis == 0 && h == 0) Fc[pa] = pa < n ? fv : make_double2(0.0, 0.0);
+            wave_lds_order();
+            // all operands of the chunk first (one LDS wait), then up to four MFMA pairs
+            double av[4], cv[4];
+            double2 fq[4];
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi) {
+                const int q = 4 * gi + kq;
+                av[gi] = K1[q * kRowPad + ii];
+                cv[gi] = K2[q * kRowPad + ii];
+                fq[gi] = Fc[q];
+            }
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi) {
+                if (4 * gi >= n) break;  // wave-uniform
+                dr[kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].x, cv[gi], dr[kb], 0, 0, 0);
+                di[kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].y, cv[gi], di[kb], 0, 0, 0);
+            }
+            wave_lds_order();
+        }
     }
     __syncthreads();
     // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds row
     // i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Bin windows overlap
     // (16 > S), so the waves add into LDS one at a time, in wave order, each wave its bins in
     // order: race-free and a fixed summation order.
+#pragma unroll 1
     for (int turn = 0; turn < kSpreadWaves; ++turn) {
         if (wave == turn) {
 #pragma unroll
