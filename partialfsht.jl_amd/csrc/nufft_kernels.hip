@@ -87,29 +87,45 @@ struct HalfRow {
     static constexpr int NS = (ne > ni ? ne : ni) - 1;             // Horner steps of the longest
     static constexpr int S0E = NS - (ne - 1), S0I = NS - (ni - 1);  // first step of each polynomial
 
+    // The edge polynomial's coefficients stay in VGPRs for the whole kernel (loaded once, pinned
+    // by an empty asm so they are not rematerialised): a third fewer scalar moves per chunk.
+    struct Edge {
+        double c[ne];
+    };
+    static __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+    template <int... I>
+    static __device__ __forceinline__ Edge load_edge(std::integer_sequence<int, I...>) {
+        Edge ec;
+        ((ec.c[I] = EsHorner<W>::c[EOFF + I]), ...);
+        (pin(ec.c[I]), ...);
+        return ec;
+    }
+    static __device__ __forceinline__ Edge edge() { return load_edge(std::make_integer_sequence<int, ne>{}); }
+
     template <int S, int... T>
-    static __device__ __forceinline__ void step(double e, double v, double (&a)[NH], std::integer_sequence<int, T...>) {
-        if constexpr (S >= S0E)
-            a[0] = fma(a[0], e, coef_after(EsHorner<W>::c[EOFF + ne - 2 - (S - S0E)], a[0]));
+    static __device__ __forceinline__ void step(double e, double v, double (&a)[NH], const Edge &ec,
+                                                std::integer_sequence<int, T...>) {
+        if constexpr (S >= S0E) a[0] = fma(a[0], e, ec.c[ne - 2 - (S - S0E)]);
         if constexpr (S >= S0I)
             ((a[T + 1] = fma(a[T + 1], v, coef_after(EsHorner<W>::c[T * ni + ni - 2 - (S - S0I)], a[T + 1]))), ...);
     }
     template <int... S>
-    static __device__ __forceinline__ void steps(double e, double v, double (&a)[NH], std::integer_sequence<int, S...>) {
-        (step<S>(e, v, a, std::make_integer_sequence<int, NH - 1>{}), ...);
+    static __device__ __forceinline__ void steps(double e, double v, double (&a)[NH], const Edge &ec,
+                                                 std::integer_sequence<int, S...>) {
+        (step<S>(e, v, a, ec, std::make_integer_sequence<int, NH - 1>{}), ...);
     }
     template <int... T>
-    static __device__ __forceinline__ void init(double (&a)[NH], std::integer_sequence<int, T...>) {
-        a[0] = coef_after(EsHorner<W>::c[EOFF + ne - 1], 0.0);
+    static __device__ __forceinline__ void init(double (&a)[NH], const Edge &ec, std::integer_sequence<int, T...>) {
+        a[0] = ec.c[ne - 1];
         ((a[T + 1] = coef_after(EsHorner<W>::c[T * ni + ni - 1], 0.0)), ...);
     }
 };
 template <int W>
-__device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2]) {
+__device__ __forceinline__ void es_half_row(double v, double (&k)[(W + 1) / 2], const typename HalfRow<W>::Edge &ec) {
     using H = HalfRow<W>;
     const double e = 2.0 * sqrt(0.5 * (1.0 + v)) - 1.0;
-    H::init(k, std::make_integer_sequence<int, H::NH - 1>{});
-    H::steps(e, v, k, std::make_integer_sequence<int, H::NS>{});
+    H::init(k, ec, std::make_integer_sequence<int, H::NH - 1>{});
+    H::steps(e, v, k, ec, std::make_integer_sequence<int, H::NS>{});
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -422,6 +438,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     double x1 = Xa[min(c1 + pa, plast)];
     double2 f1 = fb[perm[min(c1 + pa, plast)]];
     int p2 = perm[min(c2 + pa, plast)];
+    const typename HalfRow<W>::Edge ec = HalfRow<W>::edge();
     // Bins in a static order, so each accumulates into a fixed register tile.
 #pragma unroll
     for (int kb = 0; kb < NBW; ++kb) {
@@ -447,7 +464,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
                 const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
                 const double u = 2.0 * (double(bc) - x) + double(W - 1);
                 double kv[NH];
-                es_half_row<W>(h == 0 ? u : -u, kv);
+                es_half_row<W>(h == 0 ? u : -u, kv, ec);
                 // the middle cell of an odd W is written by the left half only
 #pragma unroll
                 for (int t = 0; t < NH; ++t)
