@@ -65,6 +65,68 @@ __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
     k[W - 1] = horner<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 - u)) - 1.0);
 }
 
+// A coefficient passed through an empty asm that also takes a polynomial's running value: the asm
+// cannot move above the previous Horner step, so the coefficient is materialised in SGPRs just
+// before its use instead of many steps ahead (which overflows the SGPR file in a loop).
+__device__ __forceinline__ double coef_after(double c, double acc) {
+    asm volatile("" : "+s"(c) : "v"(acc));
+    return c;
+}
+
+// Both rows of a point (axes 1 and 2) in lockstep, for interpolation: the W - 2 interior polynomials
+// in u and the edge polynomial at the two end cells, all running their Horner steps together, so
+// each coefficient is materialised once (tied to the previous step, as in the half row below) and
+// serves the FMAs of both axes.
+template <int W>
+struct FullRows {
+    static constexpr int ni = EsHorner<W>::nint, ne = EsHorner<W>::nedge;
+    static constexpr int EOFF = (W - 2) * ni;
+    static constexpr int NS = (ne > ni ? ne : ni) - 1;
+    static constexpr int S0E = NS - (ne - 1), S0I = NS - (ni - 1);
+    // a[0] / a[W - 1]: left / right edge; a[T + 1]: interior cell T + 1
+    template <int S, int... T>
+    static __device__ __forceinline__ void step(const double (&e1)[2], const double (&e2)[2], double u1, double u2,
+                                                double (&a1)[W], double (&a2)[W], std::integer_sequence<int, T...>) {
+        if constexpr (S >= S0E) {
+            const double c = coef_after(EsHorner<W>::c[EOFF + ne - 2 - (S - S0E)], a1[0]);
+            a1[0] = fma(a1[0], e1[0], c);
+            a1[W - 1] = fma(a1[W - 1], e1[1], c);
+            a2[0] = fma(a2[0], e2[0], c);
+            a2[W - 1] = fma(a2[W - 1], e2[1], c);
+        }
+        if constexpr (S >= S0I) (inner<S, T>(u1, u2, a1, a2), ...);
+    }
+    template <int S, int T>
+    static __device__ __forceinline__ void inner(double u1, double u2, double (&a1)[W], double (&a2)[W]) {
+        const double c = coef_after(EsHorner<W>::c[T * ni + ni - 2 - (S - S0I)], a1[T + 1]);
+        a1[T + 1] = fma(a1[T + 1], u1, c);
+        a2[T + 1] = fma(a2[T + 1], u2, c);
+    }
+    template <int... S>
+    static __device__ __forceinline__ void steps(const double (&e1)[2], const double (&e2)[2], double u1, double u2,
+                                                 double (&a1)[W], double (&a2)[W], std::integer_sequence<int, S...>) {
+        (step<S>(e1, e2, u1, u2, a1, a2, std::make_integer_sequence<int, W - 2>{}), ...);
+    }
+    template <int... T>
+    static __device__ __forceinline__ void init(double (&a)[W], std::integer_sequence<int, T...>) {
+        a[0] = a[W - 1] = EsHorner<W>::c[EOFF + ne - 1];
+        ((a[T + 1] = EsHorner<W>::c[T * ni + ni - 1]), ...);
+    }
+};
+// The kernel values of a point at (x1, x2) with base cells (b1, b2): k1 along axis 1, k2 along axis 2,
+// the same values as es_row on each axis.
+template <int W>
+__device__ __forceinline__ void es_rows(double x1, int b1, double x2, int b2, double (&k1)[W], double (&k2)[W]) {
+    using F = FullRows<W>;
+    const double u1 = 2.0 * (double(b1) - x1) + double(W - 1);
+    const double u2 = 2.0 * (double(b2) - x2) + double(W - 1);
+    const double e1[2] = {2.0 * sqrt(0.5 * (1.0 + u1)) - 1.0, 2.0 * sqrt(0.5 * (1.0 - u1)) - 1.0};
+    const double e2[2] = {2.0 * sqrt(0.5 * (1.0 + u2)) - 1.0, 2.0 * sqrt(0.5 * (1.0 - u2)) - 1.0};
+    F::init(k1, std::make_integer_sequence<int, W - 2>{});
+    F::init(k2, std::make_integer_sequence<int, W - 2>{});
+    F::steps(e1, e2, u1, u2, k1, k2, std::make_integer_sequence<int, F::NS>{});
+}
+
 // Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
 // v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2): the edge
 // polynomial (k[0]) and the interior ones, evaluated by Horner in lockstep, so that the polynomials
@@ -75,10 +137,6 @@ __device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
 // before its use. Left free, the compiler materialises the coefficients of many steps at once, which
 // exceeds the SGPR file inside the spreading loop, and hoisting them out of the loop would park them
 // in VGPRs.
-__device__ __forceinline__ double coef_after(double c, double acc) {
-    asm volatile("" : "+s"(c) : "v"(acc));
-    return c;
-}
 template <int W>
 struct HalfRow {
     static constexpr int NH = (W + 1) / 2;  // cells per half row: the edge cell and NH - 1 interior ones
@@ -344,8 +402,7 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         const int b1 = int(ceil(x1 - 0.5 * W));
         const int b2 = int(ceil(x2 - 0.5 * W));
         double k1[W], k2[W];
-        es_row<W>(x1, b1, k1);
-        es_row<W>(x2, b2, k2);
+        es_rows<W>(x1, b1, x2, b2, k1, k2);
         const C *__restrict__ wrow = lw + (b1 - x0) * win + (b2 - y0);
         double ar = 0.0, ai = 0.0;
 #pragma unroll
