@@ -581,23 +581,28 @@ __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__
                                     int nst, int r0, int nR) {
     const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
     const int pr = blockIdx.y;  // stored row
-    const int l1 = (r0 + pr) % N1;
+    int l1 = r0 + pr;
+    if (l1 >= N1) l1 -= N1;  // r0 < N1, pr < N1
     const int b = blockIdx.z;
     if (l2 >= N2) return;
     const int q1 = l1 / S4, q2 = l2 / S4;
+    // Wraps by one period at most: q - d >= -(ns - 1) for d < min(3, ns), and the offset of a cell
+    // from a super-tile origin lies in (-(ns - 1) S4, N) with (ns - 1) S4 < N.
     double sr = 0.0, si = 0.0;
     for (int d1 = 0; d1 < 3 && d1 < ns1; ++d1) {
-        const int s1 = ((q1 - d1) % ns1 + ns1) % ns1;
+        int s1 = q1 - d1;
+        if (s1 < 0) s1 += ns1;
         const int wh1 = min(S4, N1 - s1 * S4) + w - 1;
-        int o1 = (l1 - s1 * S4) % N1;
+        int o1 = l1 - s1 * S4;
         if (o1 < 0) o1 += N1;
         for (; o1 < wh1; o1 += N1) {
             for (int d2 = 0; d2 < 3 && d2 < ns2; ++d2) {
-                const int s2 = ((q2 - d2) % ns2 + ns2) % ns2;
+                int s2 = q2 - d2;
+                if (s2 < 0) s2 += ns2;
                 const int slot = st_slot[s1 * ns2 + s2];
                 if (slot < 0) continue;
                 const int wh2 = min(S4, N2 - s2 * S4) + w - 1;
-                int o2 = (l2 - s2 * S4) % N2;
+                int o2 = l2 - s2 * S4;
                 if (o2 < 0) o2 += N2;
                 for (; o2 < wh2; o2 += N2) {
                     const double2 v = ldc(Wst + ((int64_t(b) * nst + slot) * win + o1) * win + o2);
