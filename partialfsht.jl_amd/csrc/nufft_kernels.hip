@@ -24,12 +24,6 @@ namespace pfsht {
 
 typedef double double4v __attribute__((ext_vector_type(4)));
 
-// The ES kernel evaluated directly (spreading evaluates one cell per lane).
-__device__ __forceinline__ double es_kernel(double z, double beta) {
-    const double zz = 1.0 - z * z;
-    return zz >= 0.0 ? exp(beta * (sqrt(zz) - 1.0)) : 0.0;
-}
-
 // Wavefront-private ordering of LDS traffic: a wavefront's LDS operations complete in issue
 // order, so a wavefront reading back what it wrote needs only a compiler barrier.
 __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
@@ -38,32 +32,6 @@ __device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOM
 // coefficients are compile-time constants, so they are immediate operands of the FMAs (materialised
 // in SGPRs where they are used), never loaded.
 #include "es_horner_tables.inc"
-
-template <int W, int Off, int N, int... J>
-__device__ __forceinline__ double horner_seq(double x, std::integer_sequence<int, J...>) {
-    double acc = EsHorner<W>::c[Off + N - 1];
-    ((acc = fma(acc, x, EsHorner<W>::c[Off + N - 2 - J])), ...);
-    return acc;
-}
-template <int W, int Off, int N>
-__device__ __forceinline__ double horner(double x) {
-    return horner_seq<W, Off, N>(x, std::make_integer_sequence<int, N - 1>{});
-}
-template <int W, int... T>
-__device__ __forceinline__ void es_interior(double u, double (&k)[W], std::integer_sequence<int, T...>) {
-    ((k[T + 1] = horner<W, T * EsHorner<W>::nint, EsHorner<W>::nint>(u)), ...);
-}
-
-// The W kernel values of a point at x with base cell b (cells b .. b + W - 1): W - 2 interior
-// polynomials in u = 2 (b - x) + W - 1 and the edge polynomial, in e = 2 sqrt((1 -+ u)/2) - 1, for
-// the two end cells.
-template <int W>
-__device__ __forceinline__ void es_row(double x, int b, double (&k)[W]) {
-    const double u = 2.0 * (double(b) - x) + double(W - 1);
-    es_interior<W>(u, k, std::make_integer_sequence<int, W - 2>{});
-    k[0] = horner<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 + u)) - 1.0);
-    k[W - 1] = horner<W, (W - 2) * EsHorner<W>::nint, EsHorner<W>::nedge>(2.0 * sqrt(0.5 * (1.0 - u)) - 1.0);
-}
 
 // A coefficient passed through an empty asm that also takes a polynomial's running value: the asm
 // cannot move above the previous Horner step, so the coefficient is materialised in SGPRs just
@@ -130,13 +98,8 @@ __device__ __forceinline__ void es_rows(double x1, int b1, double x2, int b2, do
 // Half a row, from the left end for v = u and, through phi(-z) = phi(z), from the right end for
 // v = -u: k[t] is the kernel at cell t (left) or W - 1 - t (right), t < ceil(W/2): the edge
 // polynomial (k[0]) and the interior ones, evaluated by Horner in lockstep, so that the polynomials
-// end on the same step and one step of all of them issues together.
-//
-// A coefficient reaches its FMA through an empty asm that also takes the polynomial's running value:
-// the asm cannot move above the previous step, so each coefficient is materialised in SGPRs just
-// before its use. Left free, the compiler materialises the coefficients of many steps at once, which
-// exceeds the SGPR file inside the spreading loop, and hoisting them out of the loop would park them
-// in VGPRs.
+// end on the same step and one step of all of them issues together. Interior coefficients go
+// through coef_after; hoisting them out of the spreading loop instead would park them in VGPRs.
 template <int W>
 struct HalfRow {
     static constexpr int NH = (W + 1) / 2;  // cells per half row: the edge cell and NH - 1 interior ones
