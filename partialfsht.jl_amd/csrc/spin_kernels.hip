@@ -76,6 +76,46 @@ __global__ void spin_tab_kernel(int L, int s, const int64_t *__restrict__ tofs, 
 
 // Forward (spin_rings_from_alm!, Spin.jl:316-322): one wavefront per (order, 64-ring tile), the
 // coefficients of transforms b0 .. b0 + nb - 1 (nb <= NB) accumulated per lane in scaled form.
+// One degree step of a single-transform spin sweep with the step's coefficients broadcast from
+// lane K of the 16-lane row by DPP on the FMAs: d_{l+1} = A1 (x d_l) + A2 d_l + A0 d_{l-1} with
+// A1 = a1, A2 = -sgn a2, A0 = -a0, then acc += (a_lm N) d_{l+1}. s_nop 1: a DPP source must not be
+// written by VALU in the two instructions before its use.
+template <int K>
+__device__ __forceinline__ double spin_dpp_step(double x, double d1, double d0, double A1, double A2, double A0,
+                                                double ANr, double ANi, double &accr, double &acci) {
+    double t, dn;
+    asm("s_nop 1\n\t"
+        "v_mul_f64 %0, %4, %5\n\t"
+        "v_mov_b64 %1, 0\n\t"
+        "v_fmac_f64_dpp %1, %7, %0 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %8, %5 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %9, %6 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %2, %10, %1 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %3, %11, %1 row_newbcast:%12 row_mask:0xf bank_mask:0xf"
+        : "=&v"(t), "=&v"(dn), "+v"(accr), "+v"(acci)
+        : "v"(x), "v"(d1), "v"(d0), "v"(A1), "v"(A2), "v"(A0), "v"(ANr), "v"(ANi), "n"(K));
+    return dn;
+}
+
+// Steps K .. 15 of a 16-step group; GUARD: only K < kend.
+template <int K, bool GUARD>
+struct SpinDppGroup {
+    static __device__ __forceinline__ void run(double x, double &d1, double &d0, double A1, double A2, double A0,
+                                               double ANr, double ANi, double &accr, double &acci, int kend) {
+        if (!GUARD || K < kend) {
+            const double dn = spin_dpp_step<K>(x, d1, d0, A1, A2, A0, ANr, ANi, accr, acci);
+            d0 = d1;
+            d1 = dn;
+        }
+        SpinDppGroup<K + 1, GUARD>::run(x, d1, d0, A1, A2, A0, ANr, ANi, accr, acci, kend);
+    }
+};
+template <bool GUARD>
+struct SpinDppGroup<16, GUARD> {
+    static __device__ __forceinline__ void run(double, double &, double &, double, double, double, double, double,
+                                               double &, double &, int) {}
+};
+
 template <int NB>
 __global__ void __launch_bounds__(64)
 spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh, const double *__restrict__ l2c,
@@ -123,6 +163,38 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
             acc[b].y = fma(a[b].y, w, acc[b].y);
         }
     };
+    if constexpr (NB == 1) {
+        // One transform: groups of 16 steps, lane k of every row holding step q0 + k's table entry
+        // and coefficient (vector loads, one group ahead, at indices clamped into the order's
+        // steps: a lane past the last step loads the last one and is never broadcast);
+        // renormalised every 64 steps.
+        const int nsteps = L - l0, kq = lane & 15;
+        const double4 *__restrict__ t0 = tab + tofs[am];
+        const double2 *__restrict__ a0p = col + (l0 + 1);
+        double4 cn = make_double4(0.0, 0.0, 0.0, 0.0);
+        double2 an = make_double2(0.0, 0.0);
+        if (nsteps > 0) {
+            cn = t0[min(kq, nsteps - 1)];
+            an = a0p[min(kq, nsteps - 1)];
+        }
+        for (int q0 = 0; q0 < nsteps; q0 += 16) {
+            const double4 c = cn;
+            const double2 a = an;
+            if (q0 + 16 < nsteps) {
+                const int qn = min(q0 + 16 + kq, nsteps - 1);
+                cn = t0[qn];
+                an = a0p[qn];
+            }
+            const double A1 = c.x, A2 = -sg2 * c.y, A0 = -c.z, ANr = a.x * c.w, ANi = a.y * c.w;
+            const int kend = min(16, nsteps - q0);
+            if (kend == 16) SpinDppGroup<0, false>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc[0].x, acc[0].y, 16);
+            else SpinDppGroup<0, true>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc[0].x, acc[0].y, kend);
+            if (((q0 + 16) & 63) == 0) {
+                const int sh = spin_renorm(d1, d0, E);
+                if (sh != 0) acc[0] = make_double2(ldexp(acc[0].x, -sh), ldexp(acc[0].y, -sh));
+            }
+        }
+    } else {
     // kG degrees per round, their table entries and coefficients loaded (uniform loads) before any
     // is used, so one load latency covers kG steps; renormalised every 64 steps
     int Lc = l0, steps = 0;
@@ -153,6 +225,7 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
 #pragma unroll
         for (int b = 0; b < NB; ++b) a[b] = (NB == 1 || b < nb) ? ap[b * astride] : make_double2(0.0, 0.0);
         step(*tp, a);
+    }
     }
     if (lane < nr) {
         const int64_t ncol = 2 * mmax + 1;
