@@ -116,6 +116,47 @@ struct SpinDppGroup<16, GUARD> {
                                                double &, double &, int) {}
 };
 
+// The recurrence step alone (the adjoint's phase 1), the same operations as spin_dpp_step:
+// t = x d_l, d_{l+1} = A1 t (rounded), + A2 d_l, + A0 d_{l-1} (FMAs).
+template <int K>
+__device__ __forceinline__ double spin_dpp_rstep(double x, double d1, double d0, double A1, double A2, double A0) {
+    double t, dn;
+    asm("s_nop 1\n\t"
+        "v_mul_f64 %0, %2, %3\n\t"
+        "v_mov_b64 %1, 0\n\t"
+        "v_fmac_f64_dpp %1, %5, %0 row_newbcast:%8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %6, %3 row_newbcast:%8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %7, %4 row_newbcast:%8 row_mask:0xf bank_mask:0xf"
+        : "=&v"(t), "=&v"(dn)
+        : "v"(x), "v"(d1), "v"(d0), "v"(A1), "v"(A2), "v"(A0), "n"(K));
+    return dn;
+}
+// The same step with per-lane coefficients (no broadcast), for guarded sub-tiles.
+__device__ __forceinline__ double spin_rstep(double x, double d1, double d0, double A1, double A2, double A0) {
+#pragma clang fp contract(off)
+    const double t = x * d1;
+    double dn = A1 * t;
+    dn = fma(A2, d1, dn);
+    return fma(A0, d0, dn);
+}
+// Steps K .. 15 of a full sub-tile, row K of the LDS tile taking d_{lb + K}.
+template <int K>
+struct SpinAdjPhase1 {
+    static __device__ __forceinline__ void run(double x, double &d1, double &d0, double A1, double A2, double A0,
+                                               double (*W)[kWS], int wcol) {
+        const double dn = spin_dpp_rstep<K>(x, d1, d0, A1, A2, A0);
+        d0 = d1;
+        d1 = dn;
+        W[K][wcol] = dn;
+        SpinAdjPhase1<K + 1>::run(x, d1, d0, A1, A2, A0, W, wcol);
+    }
+};
+template <>
+struct SpinAdjPhase1<kSub> {
+    static __device__ __forceinline__ void run(double, double &, double &, double, double, double, double (*)[kWS],
+                                               int) {}
+};
+
 template <int NB>
 __global__ void __launch_bounds__(64)
 spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh, const double *__restrict__ l2c,
@@ -289,23 +330,18 @@ spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__rest
     const double4 *__restrict__ tb = tab + (tofs[am] - l0);
     const double sg2 = m * s <= 0 ? 1.0 : -1.0;
     int since = 0;  // degrees since the last renormalisation
+    // Full sub-tiles (lb > l0, 16 degrees): lane k of every row holds step lb - 1 + k's entry, a
+    // vector load one sub-tile ahead at an index clamped into the order's steps, read by the
+    // recurrence through DPP row broadcasts, the same operations as the forward pass.
+    const int kq = lane & 15;
+    double4 cn = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int lb = l0; lb <= L; lb += kSub) {
         const int tl = min(kSub, L - lb + 1);
+        const double4 cf = cn;  // this sub-tile's entries when it is full (loaded one sub-tile ago)
+        // the next sub-tile, when it is full (lb + 31 <= L): entries lb + 15 .. lb + 30
+        if (lb + 2 * kSub - 1 <= L) cn = tb[min(lb + kSub - 1 + kq, L - 1)];
         if (tl == kSub && lb > l0) {  // a full sub-tile of steps: entries lb - 1 .. lb + 14
-            const double4 *__restrict__ tp = tb + (lb - 1);
-#pragma unroll
-            for (int k0 = 0; k0 < kSub; k0 += kG) {
-                double4 c[kG];
-#pragma unroll
-                for (int q = 0; q < kG; ++q) c[q] = tp[k0 + q];
-#pragma unroll
-                for (int q = 0; q < kG; ++q) {
-                    const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
-                    d0 = d1;
-                    d1 = dn;
-                    W[k0 + q][wcol] = d1;
-                }
-            }
+            SpinAdjPhase1<0>::run(x, d1, d0, cf.x, -sg2 * cf.y, -cf.z, W, wcol);
         } else {  // the seed's sub-tile and the last one: guarded, entries clamped in range
 #pragma unroll
             for (int k0 = 0; k0 < kSub; k0 += kG) {
@@ -317,7 +353,7 @@ spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__rest
                     const int k = k0 + q, l = lb + k;
                     if (k < tl) {
                         if (l > l0) {
-                            const double dn = (c[q].x * x - sg2 * c[q].y) * d1 - c[q].z * d0;
+                            const double dn = spin_rstep(x, d1, d0, c[q].x, -sg2 * c[q].y, -c[q].z);
                             d0 = d1;
                             d1 = dn;
                         }
