@@ -193,8 +193,8 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
     // running pointers: the table entry of the step Lc -> Lc + 1 and the coefficient of degree Lc + 1
     const double4 *__restrict__ tp = tab + tofs[am];
     const double2 *__restrict__ ap = col + (l0 + 1);
-    auto step = [&](const double4 c, const double2 (&a)[NB]) {
-        const double dn = (c.x * x - sg2 * c.y) * d1 - c.z * d0;
+    auto step = [&](const double4 c, const double2 (&a)[NB]) {  // the same operations as spin_dpp_step
+        const double dn = spin_rstep(x, d1, d0, c.x, -sg2 * c.y, -c.z);
         d0 = d1;
         d1 = dn;
         const double w = c.w * dn;
@@ -236,37 +236,36 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
             }
         }
     } else {
-    // kG degrees per round, their table entries and coefficients loaded (uniform loads) before any
-    // is used, so one load latency covers kG steps; renormalised every 64 steps
-    int Lc = l0, steps = 0;
-    for (; Lc + kG <= L; Lc += kG, tp += kG, ap += kG) {
-        double4 c[kG];
-        double2 a[kG][NB];
+        // kG degrees per round, their table entries and coefficients loaded (uniform loads) before any
+        // is used, so one load latency covers kG steps; renormalised every 64 steps
+        int Lc = l0, steps = 0;
+        for (; Lc + kG <= L; Lc += kG, tp += kG, ap += kG) {
+            double4 c[kG];
+            double2 a[kG][NB];
 #pragma unroll
-        for (int q = 0; q < kG; ++q) {
-            c[q] = tp[q];
+            for (int q = 0; q < kG; ++q) {
+                c[q] = tp[q];
 #pragma unroll
-            for (int b = 0; b < NB; ++b)  // one transform per launch of the NB = 1 kernel: no test
-                a[q][b] = (NB == 1 || b < nb) ? ap[q + b * astride] : make_double2(0.0, 0.0);
-        }
+                for (int b = 0; b < NB; ++b) a[q][b] = b < nb ? ap[q + b * astride] : make_double2(0.0, 0.0);
+            }
 #pragma unroll
-        for (int q = 0; q < kG; ++q) step(c[q], a[q]);
-        steps += kG;
-        if (steps == 64) {
-            steps = 0;
-            const int sh = spin_renorm(d1, d0, E);
-            if (sh != 0) {
+            for (int q = 0; q < kG; ++q) step(c[q], a[q]);
+            steps += kG;
+            if (steps == 64) {
+                steps = 0;
+                const int sh = spin_renorm(d1, d0, E);
+                if (sh != 0) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
+                    for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
+                }
             }
         }
-    }
-    for (; Lc < L; ++Lc, ++tp, ++ap) {  // fewer than kG steps left
-        double2 a[NB];
+        for (; Lc < L; ++Lc, ++tp, ++ap) {  // fewer than kG steps left
+            double2 a[NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) a[b] = (NB == 1 || b < nb) ? ap[b * astride] : make_double2(0.0, 0.0);
-        step(*tp, a);
-    }
+            for (int b = 0; b < NB; ++b) a[b] = b < nb ? ap[b * astride] : make_double2(0.0, 0.0);
+            step(*tp, a);
+        }
     }
     if (lane < nr) {
         const int64_t ncol = 2 * mmax + 1;
