@@ -54,7 +54,7 @@ struct SegItem {
 };
 
 constexpr int kSegMax = 8;     // segments per order (forward workgroup: one wavefront each)
-constexpr int kSegBlocks = 4;  // default target of 64-degree blocks per segment
+constexpr int kSegBlocks = 3;  // default target of 64-degree blocks per segment (2..8 measured: 3 best)
 constexpr int kAdjWaves = 4;   // adjoint workgroup: 4 wavefronts, one 16 x 64 LDS tile each
 
 // First degree of segment s of order m (s = nseg gives a degree past L).
