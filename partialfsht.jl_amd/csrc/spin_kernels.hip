@@ -157,6 +157,37 @@ struct SpinAdjPhase1<kSub> {
                                                int) {}
 };
 
+// acc += (a_lm N) d_{l+1} for one more transform, the coefficient broadcast from lane K.
+template <int K>
+__device__ __forceinline__ void spin_dpp_acc(double &accr, double &acci, double ANr, double ANi, double dn) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f64_dpp %0, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f64_dpp %1, %3, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+        : "+v"(accr), "+v"(acci)
+        : "v"(ANr), "v"(ANi), "v"(dn), "n"(K));
+}
+// Steps K .. 15 of a 16-step group for NB transforms; GUARD: only K < kend.
+template <int K, bool GUARD, int NB>
+struct SpinDppGroupN {
+    static __device__ __forceinline__ void run(double x, double &d1, double &d0, double A1, double A2, double A0,
+                                               const double (&ANr)[NB], const double (&ANi)[NB], double2 (&acc)[NB],
+                                               int kend) {
+        if (!GUARD || K < kend) {
+            const double dn = spin_dpp_rstep<K>(x, d1, d0, A1, A2, A0);
+            d0 = d1;
+            d1 = dn;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) spin_dpp_acc<K>(acc[b].x, acc[b].y, ANr[b], ANi[b], dn);
+        }
+        SpinDppGroupN<K + 1, GUARD, NB>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc, kend);
+    }
+};
+template <bool GUARD, int NB>
+struct SpinDppGroupN<16, GUARD, NB> {
+    static __device__ __forceinline__ void run(double, double &, double &, double, double, double,
+                                               const double (&)[NB], const double (&)[NB], double2 (&)[NB], int) {}
+};
+
 template <int NB>
 __global__ void __launch_bounds__(64)
 spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh, const double *__restrict__ l2c,
@@ -190,81 +221,52 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
         }
     }
     const double sg2 = m * s <= 0 ? 1.0 : -1.0;  // sign of m n with n = -s
-    // running pointers: the table entry of the step Lc -> Lc + 1 and the coefficient of degree Lc + 1
-    const double4 *__restrict__ tp = tab + tofs[am];
-    const double2 *__restrict__ ap = col + (l0 + 1);
-    auto step = [&](const double4 c, const double2 (&a)[NB]) {  // the same operations as spin_dpp_step
-        const double dn = spin_rstep(x, d1, d0, c.x, -sg2 * c.y, -c.z);
-        d0 = d1;
-        d1 = dn;
-        const double w = c.w * dn;
+    // Groups of 16 steps, lane k of every row holding step q0 + k's table entry and coefficients
+    // (vector loads, one group ahead, at indices clamped into the order's steps: a lane past the
+    // last step loads the last one and is never broadcast), read by the FMAs through DPP row
+    // broadcasts; renormalised every 64 steps.
+    const int nsteps = L - l0, kq = lane & 15;
+    const double4 *__restrict__ t0 = tab + tofs[am];
+    const double2 *__restrict__ a0p = col + (l0 + 1);
+    double4 cn = make_double4(0.0, 0.0, 0.0, 0.0);
+    double2 an[NB];
+    auto group_load = [&](int q) {
+        cn = t0[q];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            acc[b].x = fma(a[b].x, w, acc[b].x);
-            acc[b].y = fma(a[b].y, w, acc[b].y);
-        }
+        for (int b = 0; b < NB; ++b)
+            an[b] = (NB == 1 || b < nb) ? a0p[q + b * astride] : make_double2(0.0, 0.0);
     };
-    if constexpr (NB == 1) {
-        // One transform: groups of 16 steps, lane k of every row holding step q0 + k's table entry
-        // and coefficient (vector loads, one group ahead, at indices clamped into the order's
-        // steps: a lane past the last step loads the last one and is never broadcast);
-        // renormalised every 64 steps.
-        const int nsteps = L - l0, kq = lane & 15;
-        const double4 *__restrict__ t0 = tab + tofs[am];
-        const double2 *__restrict__ a0p = col + (l0 + 1);
-        double4 cn = make_double4(0.0, 0.0, 0.0, 0.0);
-        double2 an = make_double2(0.0, 0.0);
-        if (nsteps > 0) {
-            cn = t0[min(kq, nsteps - 1)];
-            an = a0p[min(kq, nsteps - 1)];
-        }
-        for (int q0 = 0; q0 < nsteps; q0 += 16) {
-            const double4 c = cn;
-            const double2 a = an;
-            if (q0 + 16 < nsteps) {
-                const int qn = min(q0 + 16 + kq, nsteps - 1);
-                cn = t0[qn];
-                an = a0p[qn];
-            }
-            const double A1 = c.x, A2 = -sg2 * c.y, A0 = -c.z, ANr = a.x * c.w, ANi = a.y * c.w;
-            const int kend = min(16, nsteps - q0);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) an[b] = make_double2(0.0, 0.0);
+    if (nsteps > 0) group_load(min(kq, nsteps - 1));
+    for (int q0 = 0; q0 < nsteps; q0 += 16) {
+        const double4 c = cn;
+        double2 a[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) a[b] = an[b];
+        if (q0 + 16 < nsteps) group_load(min(q0 + 16 + kq, nsteps - 1));
+        const double A1 = c.x, A2 = -sg2 * c.y, A0 = -c.z;
+        const int kend = min(16, nsteps - q0);
+        if constexpr (NB == 1) {
+            const double ANr = a[0].x * c.w, ANi = a[0].y * c.w;
             if (kend == 16) SpinDppGroup<0, false>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc[0].x, acc[0].y, 16);
             else SpinDppGroup<0, true>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc[0].x, acc[0].y, kend);
-            if (((q0 + 16) & 63) == 0) {
-                const int sh = spin_renorm(d1, d0, E);
-                if (sh != 0) acc[0] = make_double2(ldexp(acc[0].x, -sh), ldexp(acc[0].y, -sh));
+        } else {
+            double ANr[NB], ANi[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                ANr[b] = a[b].x * c.w;
+                ANi[b] = a[b].y * c.w;
             }
+            if (kend == 16) SpinDppGroupN<0, false, NB>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc, 16);
+            else SpinDppGroupN<0, true, NB>::run(x, d1, d0, A1, A2, A0, ANr, ANi, acc, kend);
         }
-    } else {
-        // kG degrees per round, their table entries and coefficients loaded (uniform loads) before any
-        // is used, so one load latency covers kG steps; renormalised every 64 steps
-        int Lc = l0, steps = 0;
-        for (; Lc + kG <= L; Lc += kG, tp += kG, ap += kG) {
-            double4 c[kG];
-            double2 a[kG][NB];
+        if (((q0 + 16) & 63) == 0) {
+            const int sh = spin_renorm(d1, d0, E);
+            if (sh != 0) {
 #pragma unroll
-            for (int q = 0; q < kG; ++q) {
-                c[q] = tp[q];
-#pragma unroll
-                for (int b = 0; b < NB; ++b) a[q][b] = b < nb ? ap[q + b * astride] : make_double2(0.0, 0.0);
+                for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
             }
-#pragma unroll
-            for (int q = 0; q < kG; ++q) step(c[q], a[q]);
-            steps += kG;
-            if (steps == 64) {
-                steps = 0;
-                const int sh = spin_renorm(d1, d0, E);
-                if (sh != 0) {
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) acc[b] = make_double2(ldexp(acc[b].x, -sh), ldexp(acc[b].y, -sh));
-                }
-            }
-        }
-        for (; Lc < L; ++Lc, ++tp, ++ap) {  // fewer than kG steps left
-            double2 a[NB];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) a[b] = b < nb ? ap[b * astride] : make_double2(0.0, 0.0);
-            step(*tp, a);
         }
     }
     if (lane < nr) {
