@@ -717,8 +717,14 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
         PFSHT_HIP_TRY(hipMemcpy(on.data(), don, u.N1, hipMemcpyDeviceToHost));
         if (herr) { set_error("point maps outside the fine grid"); return kEArg; }
     }
-    std::vector<int64_t> cnt(int64_t(nkeys) + 1, 0);
-    for (int k = 0; k < nkeys; ++k) cnt[k + 1] = cnt[k] + cnt32[k];
+    std::vector<int64_t> cnt;
+    {
+        std::string bmsg;
+        if (!bin_ranges(cnt32, M, cnt, bmsg)) {  // every sorted position lies in one bin range < M
+            set_error("point binning is inconsistent: " + bmsg);
+            return kEHip;
+        }
+    }
     // Fine-grid rows the kernels reach: the apply stores and transforms only the smallest cyclic
     // interval holding them (about half of the rows for a polar cap, whose points fill half the
     // doubled colatitude period).

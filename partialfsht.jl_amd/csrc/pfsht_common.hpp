@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "geometry.hpp"
+#include "point_bins.hpp"
 
 namespace pfsht {
 
@@ -178,6 +179,10 @@ inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(
 constexpr int kRowPad = 17;
 constexpr int kSpreadStage = 2 * 16 * kRowPad + 2 * 16;
 constexpr int kSpreadWaves = 8;  // wavefronts per spreading workgroup: two of the 16 bins each
+// Wavefront w accumulates bins w + kSpreadWaves k, k < 16 / kSpreadWaves: the 16 bins of a super-tile
+// are covered exactly once only when the wave count divides 16 (a bin index past 15 would read the
+// point ranges of another super-tile, or past the end of bin_ofs).
+static_assert(16 % kSpreadWaves == 0 && kSpreadWaves >= 1, "kSpreadWaves must divide the 16 bins of a super-tile");
 inline size_t spread_lds_bytes(int win) {
     return nufft_window_lds_bytes(win) + sizeof(double) * kSpreadWaves * kSpreadStage;
 }
@@ -218,14 +223,6 @@ void launch_gather_field(double2 *fg, const double2 *f, const int32_t *idx, int6
                          hipStream_t s);
 
 // plan construction on the device
-struct PointMap {  // theta -> fine-grid row coordinate, phi -> column coordinate, and the bin keys
-    int kind, flip, north;  // geometry kind; southern reflection; pole cap at the north pole
-    double thmax;           // pole: h (1 + epsapo)
-    double lo, period;      // interior: thlo - h epsapo, and the period
-    double s1, s2;          // N1 / (2 pi), N2 / (2 pi)
-    double half;            // w / 2
-    int N1, N2, S, S4, ns2, w;
-};
 void launch_coef_tables(int L, int mmax, const int64_t *abofs, double2 *ab, double2 *gc, hipStream_t s);
 void launch_map_points(const double *theta, const double *phi, int64_t M, const PointMap &q, double *X1, double *X2,
                        int32_t *key, int32_t *idx, int32_t *cnt, uint8_t *on, int32_t *err, hipStream_t s);

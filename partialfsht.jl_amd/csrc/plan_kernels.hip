@@ -51,44 +51,25 @@ void launch_coef_tables(int L, int mmax, const int64_t *abofs, double2 *ab, doub
     coef_tables_kernel<<<dim3((ngroups + 63) / 64, mmax + 1), 64, 0, s>>>(L, abofs, ab, gc);
 }
 
-// theta -> stretched coordinate (theta_to_stretch) -> fine-grid units, phi mod 2 pi -> fine-grid
-// units; the first bin touched in each direction, the point's (super-tile, local bin) key, its
-// count, and the fine rows its kernel reaches. err[0] = 1 when a point maps outside the grid.
+// One thread per point: map_point (point_bins.hpp), the key's count, and the fine rows the point's
+// kernel reaches. err[0] = 1 when a point is refused (NaN, infinite or off the grid).
 __global__ void map_points_kernel(const double *__restrict__ theta, const double *__restrict__ phi, int64_t M,
                                   PointMap q, double *__restrict__ X1, double *__restrict__ X2,
                                   int32_t *__restrict__ key, int32_t *__restrict__ idx, int32_t *__restrict__ cnt,
                                   uint8_t *__restrict__ on, int32_t *__restrict__ err) {
-#pragma clang fp contract(off)
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    const double t0 = theta[i];
-    const double t = q.flip ? kPi - t0 : t0;
-    double T;
-    if (q.kind == kFull) T = t;
-    else if (q.kind == kPole) T = kPi * (q.north ? t : kPi - t) / q.thmax;
-    else T = 2 * kPi * (t - q.lo) / q.period;
-    double ph = fmod(phi[i], 2 * kPi);
-    if (ph != 0 && ph < 0) ph += 2 * kPi;
-    double x1 = T * q.s1, x2 = ph * q.s2;
-    x1 -= floor(x1 / q.N1) * q.N1;
-    x2 -= floor(x2 / q.N2) * q.N2;
-    int b1 = int(ceil(x1 - q.half)), b2 = int(ceil(x2 - q.half));
-    if (b1 < 0) { x1 += q.N1; b1 = int(ceil(x1 - q.half)); }
-    if (b2 < 0) { x2 += q.N2; b2 = int(ceil(x2 - q.half)); }
+    double x1, x2;
+    int k, b1;
     idx[i] = int32_t(i);
-    if (!(x1 == x1) || !(x2 == x2) || b1 < 0 || b1 >= q.N1 || b2 < 0 || b2 >= q.N2) {  // NaN or off the grid
-        err[0] = 1;
-        X1[i] = 0.0;
-        X2[i] = 0.0;
-        key[i] = 0;
-        return;
-    }
+    const bool ok = map_point(q, theta[i], phi[i], x1, x2, k, b1);
     X1[i] = x1;
     X2[i] = x2;
-    const int s1 = b1 / q.S4, s2 = b2 / q.S4;
-    const int lb = ((b1 % q.S4) / q.S) * 4 + (b2 % q.S4) / q.S;
-    const int k = (s1 * q.ns2 + s2) * 16 + lb;
     key[i] = k;
+    if (!ok) {  // NaN, infinite or off the grid: the plan is refused
+        err[0] = 1;
+        return;
+    }
     atomicAdd(&cnt[k], 1);
     for (int r = 0; r < q.w; ++r) on[(b1 + r) % q.N1] = 1;
 }

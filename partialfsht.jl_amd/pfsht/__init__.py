@@ -178,14 +178,31 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+def _column_major(x) -> bool:
+    """True when a torch tensor's memory is its column-major (Fortran) layout: stride 1 along the
+    first axis and each later stride the product of the earlier sizes (size-1 axes are free)."""
+    want = 1
+    for n, s in zip(x.shape, x.stride()):
+        if n != 1 and s != want:
+            return False
+        want *= n
+    return True
+
+
 def _buf(x, count: int, what: str):
-    """(pointer, where) of a complex128 buffer of `count` elements in column-major order."""
+    """(pointer, where) of a complex128 buffer of `count` elements in column-major order. A torch
+    tensor must be 1-D and contiguous, or hold its column-major layout (a row-major 2-D or 3-D
+    tensor would be read transposed, so it is refused)."""
     if _is_torch(x):
         import torch
         if x.dtype != torch.complex128:
             raise ArgumentError(f"{what} must be complex128")
-        if not x.is_contiguous():
-            raise ArgumentError(f"{what} must be contiguous (column-major data in a contiguous tensor)")
+        if x.dim() <= 1:
+            if not x.is_contiguous():
+                raise ArgumentError(f"{what} must be contiguous")
+        elif not _column_major(x):
+            raise ArgumentError(f"{what} is a {x.dim()}-D tensor with strides {tuple(x.stride())}: pass a flat "
+                                f"tensor or one whose memory is column-major (e.g. t.T.contiguous().T)")
         if x.numel() != count:
             raise DimensionMismatch(f"{what} has {x.numel()} elements, expected {count}")
         return C.c_void_p(x.data_ptr()), (WHERE_DEVICE if x.is_cuda else WHERE_HOST)

@@ -75,3 +75,28 @@ def test_server_argument_errors():
     done = C.c_int32()
     assert L.pfsht_server_query(None, 0, C.byref(done), None) == 2
     assert L.pfsht_server_get_stats(None, None) == 2
+
+
+def test_torch_layout_checked():
+    # a 2-D / 3-D torch tensor is read as column-major memory: a row-major one is refused instead of
+    # being read transposed; 1-D tensors and tensors holding the column-major layout are accepted
+    import torch
+    import pfsht
+    L, B = 4, 3
+    n = (L + 1) * (2 * L + 1)
+    rowmajor = torch.zeros((L + 1, 2 * L + 1), dtype=torch.complex128)
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht._buf(rowmajor, n, "alm")
+    colmajor = torch.zeros((2 * L + 1, L + 1), dtype=torch.complex128).T
+    assert tuple(colmajor.shape) == (L + 1, 2 * L + 1)
+    pfsht._buf(colmajor, n, "alm")
+    pfsht._buf(torch.zeros(n, dtype=torch.complex128), n, "alm")
+    cube = torch.zeros((B, 2 * L + 1, L + 1), dtype=torch.complex128).permute(2, 1, 0)
+    pfsht._buf(cube, n * B, "alm")
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht._buf(torch.zeros((L + 1, 2 * L + 1, B), dtype=torch.complex128), n * B, "alm")
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht._buf(torch.zeros(2 * n, dtype=torch.complex128)[::2], n, "alm")  # strided 1-D
+    field = torch.zeros((7, B), dtype=torch.complex128)  # M x B row-major
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht._buf(field, 7 * B, "f")
