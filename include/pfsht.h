@@ -91,6 +91,37 @@ int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int3
  * Wiener prior) return PFSHT_E_UNSUPPORTED on a spin plan. spin = 0 is pfsht_plan_create. */
 int pfsht_plan_create_spin(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
                            int32_t ntrans, int32_t spin, int32_t method, int32_t precision, pfsht_plan_t *out);
+/* The remaining keywords of Plan(θ, φ; lmax, tol, spin, ntrans, method, maxboxes, oversample_θ,
+ * oversample_φ, εapo_max) — src/Plan.jl:84-97, forwarded into the leaves as src/Plan.jl:110-135 does:
+ * oversample_θ inflates the stretched ring count (src/Stretch.jl:164), oversample_φ the longitude
+ * grid (src/Stretch.jl:194), εapo_max bounds the apodisation fraction a stretched geometry may use
+ * (src/Windows.jl:84-141), maxboxes bounds the planner's partition (src/Box.jl:171-235). A zero field
+ * takes the reference's default. `packed` has no counterpart in the reference: it makes the plan
+ * keep FFT plans for every power-of-two transform count, so that a column apply of nb < ntrans
+ * transforms (pfsht_synth_cols) does only nb transforms' work (servers packing requests set it). */
+typedef struct {
+    int32_t method;          /* PFSHT_METHOD_* (AUTO when 0) */
+    int32_t precision;       /* PFSHT_PREC_* */
+    int32_t spin;            /* spin weight; 0 scalar */
+    int32_t maxboxes;        /* 0: 8 */
+    double oversample_theta; /* 0: 1 */
+    double oversample_phi;   /* 0: 1 */
+    double epsapo_max;       /* 0: 1.0 */
+    int32_t packed;          /* nonzero: FFT plans for partial transform counts */
+    int32_t reserved;        /* 0 */
+} pfsht_plan_opts;
+/* Fills *opts with the defaults (all fields zero). */
+void pfsht_plan_opts_default(pfsht_plan_opts *opts);
+/* Plan(θ, φ; lmax, tol, ntrans, keywords...) with every keyword of src/Plan.jl:84-97 (opts may be
+ * NULL for the defaults). oversample_θ, oversample_φ >= 1 and εapo_max in (0, 1] (src/Windows.jl). */
+int pfsht_plan_create_opts(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                           int32_t ntrans, const pfsht_plan_opts *opts, pfsht_plan_t *out);
+/* The device bytes pfsht_plan_create_opts would allocate for these arguments, computed on the host
+ * from the same sizing, without building anything: an upper estimate (it counts the NUFFT windows of
+ * super-tiles no point reaches), for callers admitting plans against a memory budget. No counterpart
+ * in the reference. */
+int pfsht_plan_estimate_bytes(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                              int32_t ntrans, const pfsht_plan_opts *opts, int64_t *bytes);
 void pfsht_plan_destroy(pfsht_plan_t plan);
 int pfsht_plan_info(pfsht_plan_t plan, pfsht_info *info);
 
@@ -102,6 +133,18 @@ int pfsht_synth(pfsht_plan_t plan, const void *alm, void *f, int32_t where, void
 /* mul!(alm, plan', f) — src/Plan.jl:143-148 -> adjoint_stretch! (src/Stretch.jl:279-295).
  * Overwrites alm, padding slots included (src/Legendre.jl:338). */
 int pfsht_adjoint(pfsht_plan_t plan, const void *f, void *alm, int32_t where, void *stream);
+
+/* mul!(f, plan, alm) and mul!(alm, plan', f) — src/Plan.jl:137-148 — over nb <= ntrans transforms whose
+ * columns lie anywhere in device memory: alm_cols[b] is the (lmax+1) x (2 lmax+1) coefficient array of
+ * transform b (sph_mode layout, or the spin layout on a spin plan) and f_cols[b] its npts field values.
+ * The pointer arrays are host memory, read before the call returns; the call is asynchronous on the
+ * stream like the device-memory applies. Transform b of the result is the apply of column b alone
+ * (to round-off), whatever the other columns hold. No counterpart in the reference, whose batches are
+ * one contiguous array (src/Layout.jl:37-56): it is for callers that pack independent requests into
+ * one apply without copying them together (the serving runtime does). */
+int pfsht_synth_cols(pfsht_plan_t plan, int32_t nb, const void *const *alm_cols, void *const *f_cols, void *stream);
+int pfsht_adjoint_cols(pfsht_plan_t plan, int32_t nb, const void *const *f_cols, void *const *alm_cols,
+                       void *stream);
 
 /* RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255. theta sorted
  * ascending, host memory. */
@@ -190,37 +233,68 @@ int pfsht_solve(pfsht_plan_t plan, const void *f, const double *w, int32_t mode,
 /* ---------------------------------------------------------------------------------------------
  * Serving runtime. No counterpart in the reference, whose plans are built and applied by their
  * caller (src/Plan.jl:84-148); it is for services answering requests on recurring point sets.
- * A server keeps plans resident in HBM under caller-chosen 64-bit keys, builds a plan on a miss and
- * evicts least-recently-used plans whose last apply has finished when the resident plans would
- * exceed hbm_budget_bytes. Applies go round-robin over nstreams HIP streams (default priority);
- * applies of one plan stay ordered (a plan orders its own applies across streams). All calls are
- * thread-safe.
+ *
+ * A server keeps plans resident in HBM under caller-chosen 64-bit keys. pfsht_server_plan registers a
+ * geometry and the number of transforms a request of it carries (ntrans); the plan is built with
+ * room for max(ntrans, pack) transforms, with FP32 grids when tol >= 1e-6 (PFSHT_PREC_MIXED, which
+ * meets such tolerances), after its size has been estimated and least-recently-used plans with no
+ * request queued or running have been evicted to fit the HBM budget (nothing is allocated for a plan
+ * that cannot fit).
+ *
+ * pfsht_server_submit queues one request and returns at once. A dispatcher thread issues the queue
+ * in order of arrival: whenever one of the nstreams streams has fewer than `depth` launches in
+ * flight, it takes the oldest request and every later queued request of the same key and direction
+ * that fits the plan's room, and applies them together as one column apply (pfsht_synth_cols), in
+ * place in the callers' buffers; with graphs on, each (plan, direction, transform count) is captured
+ * once in a hipGraph and replayed. Requests that arrive while others run join the next launch of
+ * their plan. Each request's result is the same as its apply alone (to round-off). Applies of one
+ * plan are ordered; streams run at the default priority. All calls are thread-safe.
  * ------------------------------------------------------------------------------------------- */
 typedef struct pfsht_server_s *pfsht_server_t;
 
 typedef struct {
+    int32_t nstreams;         /* streams of the pool (1..64; default 4) */
+    int32_t depth;            /* launches in flight per stream (1..16; default 2) */
+    int32_t pack;             /* transforms a plan's launch can hold (1..64; default 8) */
+    int32_t graphs;           /* nonzero: replay captured hipGraphs (default 1) */
+    int64_t hbm_budget_bytes; /* resident plan bytes (default 200e9) */
+} pfsht_server_opts;
+
+typedef struct {
     int64_t hits, misses, evictions, rejected; /* plan lookups; plans refused for lack of HBM */
-    int64_t submitted;                          /* applies enqueued */
+    int64_t submitted;                          /* requests queued */
     int64_t resident_bytes, budget_bytes, plans;
+    int64_t launches;           /* packed applies issued */
+    int64_t packed_transforms;  /* transforms those applies carried */
+    int64_t graph_launches;     /* of which replayed from a hipGraph */
+    int64_t graph_failures;     /* plans whose capture failed (applied eagerly instead) */
+    int64_t failed;             /* requests whose launch returned an error */
+    int64_t queued;             /* requests waiting for a launch now */
 } pfsht_server_stats;
 
+void pfsht_server_opts_default(pfsht_server_opts *opts);
+int pfsht_server_create_ex(const pfsht_server_opts *opts, pfsht_server_t *out);
+/* pfsht_server_create_ex with the defaults, nstreams and the budget. */
 int pfsht_server_create(int32_t nstreams, int64_t hbm_budget_bytes, pfsht_server_t *out);
+/* Waits for the server's work, stops its dispatcher and frees its plans. */
 void pfsht_server_destroy(pfsht_server_t srv);
-/* The plan cached under key, built (method, FP64) on a miss; *hit (may be NULL) tells which. The
- * plan stays owned by the server: do not destroy it. */
+/* Registers the geometry under key (built with `method` on a miss; *hit, may be NULL, tells which) and
+ * returns its plan. The plan stays owned by the server and is valid until a later pfsht_server_plan
+ * call evicts it: do not destroy it. PFSHT_E_NOMEM when it cannot fit the budget. ntrans 1..64. */
 int pfsht_server_plan(pfsht_server_t srv, uint64_t key, const double *theta, const double *phi, int64_t npts,
                       int32_t lmax, double tol, int32_t ntrans, int32_t method, pfsht_plan_t *plan, int32_t *hit);
-/* Enqueue one apply of the plan under key: dir 0 = mul!(out = f, plan, in = alm), dir 1 =
- * mul!(out = alm, plan', in = f). in/out are device buffers; returns a ticket. */
+/* Queues one request of the plan under key: dir 0 = mul!(out = f, plan, in = alm), dir 1 =
+ * mul!(out = alm, plan', in = f), each with the key's ntrans transforms in the reference's contiguous
+ * layouts, device memory. in and out must stay valid until the ticket completes. Returns a ticket. */
 int pfsht_server_submit(pfsht_server_t srv, uint64_t key, int32_t dir, const void *in, void *out, int64_t *ticket);
-/* *done = 1 once the apply has finished; *device_ms (may be NULL) is then its time on its stream. */
+/* *done = 1 once the request's apply has finished; *device_ms (may be NULL) is then its launch's time on
+ * its stream (a packed launch's time is shared by its requests). */
 int pfsht_server_query(pfsht_server_t srv, int64_t ticket, int32_t *done, double *device_ms);
 int pfsht_server_wait(pfsht_server_t srv, int64_t ticket, double *device_ms);
 int pfsht_server_get_stats(pfsht_server_t srv, pfsht_server_stats *stats);
 /* Device clock: pfsht_server_epoch records a reference event and waits for it; pfsht_server_times
- * then gives a ticket's start (reached on its stream) and end, in ms after the epoch, waiting for
- * the apply to finish. With the host clock read right after the epoch, request latencies follow
- * without polling. */
+ * then gives a request's launch start and end, in ms after the epoch, waiting for it to finish. With
+ * the host clock read right after the epoch, request latencies (arrival to end) follow without polling. */
 int pfsht_server_epoch(pfsht_server_t srv);
 int pfsht_server_times(pfsht_server_t srv, int64_t ticket, double *start_ms, double *end_ms);
 

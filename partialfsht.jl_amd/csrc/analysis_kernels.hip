@@ -144,6 +144,46 @@ namespace pfsht {
 
 // Multi-box applies (src/MultiBox.jl:83-144): f[b][idx[j]] = fg[b][j] and fg[b][j] = f[b][idx[j]]
 // for the points of one box.
+__global__ void scatter_field_kernel(OutCols f, const double2 *__restrict__ fg, const int32_t *__restrict__ idx,
+                                     int64_t Mg) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (j < Mg) gstore(f.col(b) + idx[j], fg[int64_t(b) * Mg + j]);
+}
+
+__global__ void gather_field_kernel(double2 *__restrict__ fg, InCols f, const int32_t *__restrict__ idx, int64_t Mg) {
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (j < Mg) fg[int64_t(b) * Mg + j] = gload(f.col(b) + idx[j]);
+}
+
+__global__ void cols_add_kernel(OutCols a, const double2 *__restrict__ acc, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= n) return;
+    const gptr p = a.col(b) + i;
+    const double2 v = acc[int64_t(b) * n + i], u = gload(p);
+    gstore(p, make_double2(u.x + v.x, u.y + v.y));
+}
+
+void launch_scatter_field(OutCols f, const double2 *fg, const int32_t *idx, int64_t Mg, int nb, hipStream_t s) {
+    if (Mg > 0 && nb > 0) scatter_field_kernel<<<dim3(unsigned((Mg + 255) / 256), nb), 256, 0, s>>>(f, fg, idx, Mg);
+}
+
+void launch_gather_field(double2 *fg, InCols f, const int32_t *idx, int64_t Mg, int nb, hipStream_t s) {
+    if (Mg > 0 && nb > 0) gather_field_kernel<<<dim3(unsigned((Mg + 255) / 256), nb), 256, 0, s>>>(fg, f, idx, Mg);
+}
+
+void launch_cols_add(OutCols a, const double2 *acc, int64_t n, int nb, hipStream_t s) {
+    if (n > 0 && nb > 0) cols_add_kernel<<<dim3(unsigned((n + 255) / 256), nb), 256, 0, s>>>(a, acc, n);
+}
+
+}  // namespace pfsht
+
+namespace pfsht {
+
+// Multi-box applies (src/MultiBox.jl:83-144): f[b][idx[j]] = fg[b][j] and fg[b][j] = f[b][idx[j]]
+// for the points of one box.
 __global__ void scatter_field_kernel(double2 *__restrict__ f, const double2 *__restrict__ fg,
                                      const int32_t *__restrict__ idx, int64_t Mg, int64_t M) {
     const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -88,13 +88,12 @@ static rocfft_status g_rocfft_setup = rocfft_status_success;
 struct FftPlan {
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
-    void *work = nullptr;
     size_t work_bytes = 0;
 };
 
-// single: FP32 data (the grid buffers of a mixed-precision plan), else FP64.
-static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, size_t batch, int64_t &bytes,
-                       bool single = false) {
+// single: FP32 data (the grid buffers of a mixed-precision plan), else FP64. The work buffer is set
+// later (fft_attach_work): every FFT of a plan shares one, since a plan's FFTs run one after another.
+static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, size_t batch, bool single = false) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     if (g_rocfft_setup != rocfft_status_success) { set_error("rocfft_setup failed"); return kEHip; }
     size_t lengths[2] = {n_fast, n_slow};
@@ -110,18 +109,12 @@ static Status fft_make(FftPlan &fp, bool forward, size_t n_fast, size_t n_slow, 
         set_error("rocfft_execution_info_create failed");
         return kEHip;
     }
-    if (fp.work_bytes) {
-        Status st = dalloc(reinterpret_cast<char **>(&fp.work), fp.work_bytes, bytes);
-        if (st != kOk) return st;
-        rocfft_execution_info_set_work_buffer(fp.info, fp.work, fp.work_bytes);
-    }
     return kOk;
 }
 
 static void fft_free(FftPlan &fp) {
     if (fp.plan) rocfft_plan_destroy(fp.plan);
     if (fp.info) rocfft_execution_info_destroy(fp.info);
-    if (fp.work) (void)hipFree(fp.work);
     fp = FftPlan{};
 }
 
@@ -131,6 +124,50 @@ static Status fft_exec(FftPlan &fp, void *buf, hipStream_t s) {
     if (rocfft_execute(fp.plan, in, nullptr, fp.info) != rocfft_status_success) {
         set_error("rocfft_execute failed");
         return kEHip;
+    }
+    return kOk;
+}
+
+// The FFTs of one grid stage for 1 .. B transforms: a plan per transform count in `counts` (B, and
+// with partial applies enabled every power of two below it), each over count x per batch entries of
+// length n. An apply of nb transforms runs the largest counts that fit nb on consecutive transform
+// blocks; with only B available it runs B (the extra transforms' buffers hold data no output reads).
+struct FftSet {
+    std::vector<int> counts;  // descending
+    std::vector<FftPlan> plans;
+    size_t stride_bytes = 0;  // bytes of one transform's data
+};
+
+static Status fftset_make(FftSet &fs, bool forward, size_t n, size_t per, int B, bool partial, bool single) {
+    fs.counts.clear();
+    fs.counts.push_back(B);
+    if (partial)
+        for (int c = 1; c < B; c *= 2) fs.counts.push_back(c);
+    std::sort(fs.counts.rbegin(), fs.counts.rend());
+    fs.plans.assign(fs.counts.size(), FftPlan{});
+    for (size_t i = 0; i < fs.counts.size(); ++i) {
+        Status st = fft_make(fs.plans[i], forward, n, 0, size_t(fs.counts[i]) * per, single);
+        if (st != kOk) return st;
+    }
+    fs.stride_bytes = n * per * (single ? sizeof(float2) : sizeof(double2));
+    return kOk;
+}
+
+static void fftset_free(FftSet &fs) {
+    for (auto &p : fs.plans) fft_free(p);
+    fs.plans.clear();
+}
+
+static Status fftset_exec(FftSet &fs, void *buf, int nb, hipStream_t s) {
+    int done = 0;
+    while (done < nb) {
+        const int rem = nb - done;
+        size_t k = 0;
+        while (k < fs.counts.size() && fs.counts[k] > rem) ++k;
+        if (k == fs.counts.size()) k = fs.counts.size() - 1;  // only larger counts: the smallest (done is 0)
+        Status st = fft_exec(fs.plans[k], static_cast<char *>(buf) + size_t(done) * fs.stride_bytes, s);
+        if (st != kOk) return st;
+        done += fs.counts[k];
     }
     return kOk;
 }
@@ -156,7 +193,7 @@ struct pfsht_rings_s {
 
     ~pfsht_rings_s() {
         dfree(d.costh); dfree(d.log2sin); dfree(d.log2c); dfree(d.ab); dfree(d.gc); dfree(d.abofs);
-        dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
+        dfree(d.jlo); dfree(d.jhi); dfree(d.fitems); dfree(d.aitems); dfree(d.mitems); dfree(d.ckq); dfree(d.ckE); dfree(d.pofs); dfree(d.ntiles); dfree(d.partials);
         dfree(Gd); dfree(alm_stage); dfree(G_stage);
         dfree(sd.l2c); dfree(sd.l2s); dfree(sd.seed); dfree(sd.Nl); dfree(sd.tab); dfree(sd.tofs); dfree(sd.items);
         dfree(sd.pofs); dfree(sd.ntiles); dfree(sd.partials);
@@ -197,7 +234,21 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     // Work items: one per (order, 64-ring tile), with the order's degree segments; checkpoint rows
     // for segments 1.. of every tile. An adjoint wavefront takes a group of tg consecutive tiles of
     // an order (two when ntrans <= 2, which the registers allow); partials are indexed by the group.
-    const int tg = B <= 2 ? 2 : 1;
+    // The MFMA adjoint (two or more transforms) takes every order's tiles in one workgroup: four
+    // wavefronts of one or two tiles, or eight of two, so up to 16 tiles (1024 rings) per order.
+    int ntmax = 0;
+    for (int m = 0; m <= mmax; ++m)
+        if (r.jhi[m] >= r.jlo[m]) ntmax = std::max(ntmax, (r.jhi[m] - r.jlo[m] + 64) / 64);
+    const bool mfma_adj = B >= 2 && ntmax <= 16;
+    int mw = 0, mtg = 1;
+    if (mfma_adj) {
+        if (ntmax <= 4) mw = ntmax;
+        else if (ntmax <= 8) { mw = 4; mtg = 2; }
+        else { mw = 8; mtg = 2; }
+    }
+    // partials serve the adjoints the MFMA path does not: single transforms when it exists
+    const int pb = mfma_adj ? 1 : B;
+    const int tg = (B <= 2 || mfma_adj) ? 2 : 1;
     std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
     std::vector<int32_t> ntiles(mmax + 1, 0);
     std::vector<int64_t> pofs(mmax + 1, 0), cost(mmax + 1, 0);
@@ -223,7 +274,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         }
         const int ngroups = (nt + tg - 1) / tg;
         ntiles[m] = ngroups;
-        np += int64_t(ngroups) * adj_rows_padded(m, L) * B;  // group rows padded to whole sub-tiles
+        np += int64_t(ngroups) * adj_rows_padded(m, L) * pb;  // group rows padded to whole sub-tiles
     }
     // Orders are dealt to the eight XCDs, longest first onto the least-loaded one, and workgroup i
     // is laid out for XCD i % 8 (the hardware deals consecutive workgroups round-robin): the tiles
@@ -264,18 +315,36 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         }
         while (aq[x].size() % kAdjWaves) aq[x].push_back(empty);
     }
-    auto interleave = [&](const std::vector<std::vector<SegItem>> &q, size_t group) {
+    auto interleave = [&](const auto &q, size_t group, const auto &none) {
         size_t rounds = 0;
         for (const auto &v : q) rounds = std::max(rounds, v.size() / group);
-        std::vector<SegItem> out;
+        std::vector<std::decay_t<decltype(none)>> out;
         out.reserve(rounds * kXcd * group);
         for (size_t i = 0; i < rounds; ++i)
             for (int x = 0; x < kXcd; ++x)
                 for (size_t g = 0; g < group; ++g)
-                    out.push_back((i + 1) * group <= q[x].size() ? q[x][i * group + g] : empty);
+                    out.push_back((i + 1) * group <= q[x].size() ? q[x][i * group + g] : none);
         return out;
     };
-    std::vector<SegItem> fitems = interleave(fq, 1), aitems = interleave(aq, kAdjWaves);
+    std::vector<SegItem> fitems = interleave(fq, 1, empty);
+    std::vector<SegItem> aitems = interleave(aq, kAdjWaves, empty);
+    // MFMA adjoint: one workgroup per (order, segment), orders on the XCD they were dealt to
+    std::vector<OrderSeg> mitems;
+    if (mfma_adj) {
+        std::vector<std::vector<OrderSeg>> mq(kXcd);
+        for (int x = 0; x < kXcd; ++x)
+            for (int m : xq[x]) {
+                const auto &tl = tiles_of[m];
+                const int nt = int(tl.size());
+                for (int sg = 0; sg < tl[0].nseg; ++sg)
+                    mq[x].push_back({m, tl[0].nseg, sg, nt, tl[0].j0, tl[nt - 1].nr, tl[0].ck});
+            }
+        mitems = interleave(mq, 1, OrderSeg{0, 1, 0, 0, 0, 0, 0});
+    }
+    d.mfma_waves = mw;
+    d.mfma_tg = mtg;
+    d.part_b = pb;
+    d.nmitems = int(mitems.size());
     d.fwd_waves = fw;
     d.adj_tg = tg;
     d.nfitems = int(fitems.size());
@@ -296,6 +365,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.jhi, jhi32, by)) != kOk) return st;
     if ((st = upload(&d.fitems, fitems, by)) != kOk) return st;
     if ((st = upload(&d.aitems, aitems, by)) != kOk) return st;
+    if ((st = upload(&d.mitems, mitems, by)) != kOk) return st;
     if ((st = dalloc(&d.ckq, size_t(nck) * 64, by)) != kOk) return st;
     if ((st = dalloc(&d.ckE, size_t(nck) * 64, by)) != kOk) return st;
     if ((st = upload(&d.pofs, pofs, by)) != kOk) return st;
@@ -411,14 +481,14 @@ static Status rings_build_spin(const double *theta, int n, int L, int spin, doub
     return kOk;
 }
 
-// The ring passes of an engine of either kind.
-static void ring_forward(pfsht_rings_s &r, const double2 *alm, hipStream_t s) {
-    if (r.spin) launch_spin_fwd(r.d, r.sd, alm, r.Gd, s);
-    else launch_ring_fwd(r.d, alm, r.Gd, s);
+// The ring passes of an engine of either kind over nb transforms.
+static void ring_forward(pfsht_rings_s &r, InCols alm, int nb, hipStream_t s) {
+    if (r.spin) launch_spin_fwd(r.d, r.sd, alm, nb, r.Gd, s);
+    else launch_ring_fwd(r.d, alm, nb, r.Gd, s);
 }
-static void ring_adjoint(pfsht_rings_s &r, double2 *alm, hipStream_t s) {
-    if (r.spin) launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
-    else launch_ring_adj(r.d, r.Gd, alm, s);
+static void ring_adjoint(pfsht_rings_s &r, OutCols alm, int nb, hipStream_t s) {
+    if (r.spin) launch_spin_adj(r.d, r.sd, r.Gd, alm, nb, s);
+    else launch_ring_adj(r.d, r.Gd, alm, nb, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -454,7 +524,10 @@ struct pfsht_plan_s {
     // order-column layout [b][m + mmax][l1]. Elements are double2, or float2 in a mixed-precision
     // plan (the NUFFT seam is FP64 only).
     void *Ft = nullptr, *U = nullptr, *Ut = nullptr;
-    FftPlan th_fwd, th_bwd, fine_bwd, fine_fwd, col_bwd, col_fwd, row_bwd, row_fwd;
+    FftSet th_fwd, th_bwd, col_bwd, col_fwd, row_bwd, row_fwd;
+    FftPlan fine_bwd, fine_fwd;  // the NUFFT seam's 2-D transforms (all transforms)
+    void *fft_work = nullptr;    // rocFFT work buffer shared by the plan's FFTs
+    bool partial = false;        // FFT plans for every power-of-two transform count (packed applies)
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     // Direct leaf: theta-sorted points as rings
     double *dphi = nullptr;
@@ -482,6 +555,14 @@ struct pfsht_plan_s {
     double th_min = 0, th_max = 0;  // colatitude range of the points
     bool box_ready = false;
     Box box;
+    // Column tables of pfsht_synth_cols / pfsht_adjoint_cols: kColSlots slots of 2 B device pointers
+    // (the inputs, then the outputs), each filled from pinned host memory by an asynchronous copy on
+    // the apply's stream; a slot is reused only once the apply that read it has finished.
+    static constexpr int kColSlots = 8;
+    void **col_dev = nullptr, **col_host = nullptr;
+    hipEvent_t col_ev[kColSlots] = {};
+    bool col_used[kColSlots] = {};
+    int col_next = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> pool;
@@ -502,14 +583,28 @@ struct pfsht_plan_s {
         for (auto *q : boxes) delete q;
         for (auto *q : box_idx) dfree(q);
         dfree(mb_f); dfree(mb_acc);
-        fft_free(th_fwd); fft_free(th_bwd);
-        fft_free(fine_bwd); fft_free(fine_fwd); fft_free(col_bwd); fft_free(col_fwd);
-        fft_free(row_bwd); fft_free(row_fwd);
+        fftset_free(th_fwd); fftset_free(th_bwd);
+        fft_free(fine_bwd); fft_free(fine_fwd); fftset_free(col_bwd); fftset_free(col_fwd);
+        fftset_free(row_bwd); fftset_free(row_fwd);
+        dfree(fft_work);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
+        dfree(col_dev);
+        if (col_host) (void)hipHostFree(col_host);
+        for (auto e : col_ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
+
+static Status cols_alloc(pfsht_plan_s &p) {
+    const size_t n = size_t(pfsht_plan_s::kColSlots) * 2 * p.B;
+    Status st;
+    if ((st = dalloc(&p.col_dev, n, p.bytes)) != kOk) return st;
+    PFSHT_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&p.col_host), n * sizeof(void *)));
+    for (auto &e : p.col_ev) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return kOk;
+}
 
 // Orders an apply on stream s after the plan's previous apply (on any stream).
 static Status apply_begin(pfsht_plan_s &p, hipStream_t s) {
@@ -525,7 +620,11 @@ static Status apply_end(pfsht_plan_s &p, hipStream_t s) {
 }
 
 static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi, int64_t M, int L, double tol, int B,
-                         int method, int spin) {
+                         const pfsht_plan_opts &o) {
+    int method = o.method;
+    const int spin = o.spin;
+    const int maxboxes = o.maxboxes;
+    p.partial = o.packed != 0 && B > 1;
     if (M < 1) { set_error("need at least one point"); return kEArg; }
     if (M > INT32_MAX) { set_error("more than 2^31-1 points per plan is not supported"); return kEArg; }
     if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
@@ -552,7 +651,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
         double best = cd;
         method = PFSHT_METHOD_DIRECT;
         if (cs < best) { method = PFSHT_METHOD_STRETCH; best = cs; }
-        part = partition_boxes(theta, phi, M, L, B, tol, kMaxBoxes);
+        part = partition_boxes(theta, phi, M, L, B, tol, maxboxes);
         parted = true;
         if (part.groups.size() > 1) {
             double cm = 0;
@@ -563,7 +662,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     p.method = method;
     if (method == PFSHT_METHOD_MULTIBOX) {
         // MultiBoxLeaf (src/MultiBox.jl:30-62): a stretched plan per box of the partition
-        if (!parted) part = partition_boxes(theta, phi, M, L, B, tol, kMaxBoxes);
+        if (!parted) part = partition_boxes(theta, phi, M, L, B, tol, maxboxes);
         p.geom.kind = -2;
         int64_t &by = p.bytes, maxg = 0;
         for (const auto &g : part.groups) {
@@ -574,7 +673,9 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
             if (!q) { set_error("host allocation failed"); return kENomem; }
             q->precision = p.precision;
             p.boxes.push_back(q);
-            Status st = plan_build(*q, tg.data(), pg.data(), Mg, L, tol, B, PFSHT_METHOD_STRETCH, spin);
+            pfsht_plan_opts ob = o;
+            ob.method = PFSHT_METHOD_STRETCH;
+            Status st = plan_build(*q, tg.data(), pg.data(), Mg, L, tol, B, ob);
             if (st != kOk) return st;
             int32_t *idx = nullptr;
             if ((st = upload(&idx, g, by)) != kOk) return st;
@@ -617,17 +718,17 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     std::string msg;
     const double tiny = std::sqrt(DBL_EPSILON);
     Geom g0;
-    int rc = stretch_geometry(thlo, thhi, L, tol, g0, msg);
+    int rc = stretch_geometry(thlo, thhi, L, tol, g0, msg, o.epsapo_max);
     if (rc != 0) { set_error(msg); return rc < 0 ? kEArg : kEGeom; }
     p.flip = g0.kind == kPole && g0.thhi >= kPi - tiny && g0.thlo > tiny;
     if (p.flip) {
-        rc = stretch_geometry(kPi - thhi, kPi - thlo, L, tol, p.geom, msg);
+        rc = stretch_geometry(kPi - thhi, kPi - thlo, L, tol, p.geom, msg, o.epsapo_max);
         if (rc != 0) { set_error(msg); return rc < 0 ? kEArg : kEGeom; }
     } else {
         p.geom = g0;
     }
     // sizing (src/Stretch.jl:164-171)
-    const int Kt = p.geom.K + kKMargin;
+    const int Kt = int(std::ceil(double(p.geom.K + kKMargin) * o.oversample_theta));  // src/Stretch.jl:164
     if (p.geom.kind == kInterior) { p.nth = int(next_smooth(2 * Kt + 1)); p.nh = p.nth; }
     else { p.nh = int(next_smooth(Kt + 1)); p.nth = 2 * p.nh; }
     std::vector<double> th, Th;
@@ -638,7 +739,8 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
                      : rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings);
     if (st != kOk) return st;
     p.mmax = p.rings->d.mmax;
-    p.nph = int(next_smooth_even(2 * int64_t(std::max(p.mmax, std::abs(spin))) + 1));  // src/Stretch.jl:194
+    p.nph = int(next_smooth_even(int64_t(std::ceil(double(2 * int64_t(std::max(p.mmax, std::abs(spin))) + 1) *
+                                                    o.oversample_phi))));  // src/Stretch.jl:194
     int64_t &by = p.bytes;
     std::vector<double> wth(p.nh);
     for (int i = 0; i < p.nh; ++i) wth[i] = stretch_window(p.geom, Th[i]);
@@ -654,11 +756,11 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     const int S4 = 4 * u.S;
     u.ns1 = (u.N1 + S4 - 1) / S4;
     u.ns2 = (u.N2 + S4 - 1) / S4;
-    u.win = 3 * u.S + 16;
+    u.win = 3 * u.S + 16;  // the point kernels size their per-thread staging from this relation
     {
         int maxlds = 65536;
         (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, p.dev);
-        if (spread_lds_bytes(u.win) > size_t(maxlds)) {
+        if (spread_lds_bytes(u.win, kSpreadGroup) > size_t(maxlds) || 2 * nufft_window_lds_bytes(u.win) > size_t(maxlds)) {
             set_error("the NUFFT super-tile window does not fit in LDS at this tolerance (tol >= 1e-3)");
             return kEUnsupported;
         }
@@ -784,26 +886,38 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if ((st = galloc(&p.U, size_t(B) * u.N1 * u.N2)) != kOk) return st;
     // rocFFT plans (row-major, contiguous; lengths fastest first)
     // colatitude FFTs of the order columns (the longitude FFT pair of the reference cancels)
-    const size_t ncols = size_t(B) * (2 * p.mmax + 1);
-    if ((st = fft_make(p.th_fwd, true, p.nth, 0, ncols, by, single)) != kOk) return st;
-    if ((st = fft_make(p.th_bwd, false, p.nth, 0, ncols, by, single)) != kOk) return st;
+    const size_t ncol1 = size_t(2 * p.mmax + 1);
+    if ((st = fftset_make(p.th_fwd, true, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
+    if ((st = fftset_make(p.th_bwd, false, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
     if (!single) {  // the NUFFT seam
-        if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B, by)) != kOk) return st;
-        if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B, by)) != kOk) return st;
+        if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B)) != kOk) return st;
+        if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B)) != kOk) return st;
     }
     // the apply's pruned fine transform: colatitude FFTs of the nonzero columns, longitude FFTs of rows
-    if ((st = galloc(&p.Ut, ncols * u.N1)) != kOk) return st;
-    if ((st = fft_make(p.col_bwd, false, u.N1, 0, ncols, by, single)) != kOk) return st;
-    if ((st = fft_make(p.col_fwd, true, u.N1, 0, ncols, by, single)) != kOk) return st;
-    if ((st = fft_make(p.row_bwd, false, u.N2, 0, size_t(B) * u.nR, by, single)) != kOk) return st;
-    if ((st = fft_make(p.row_fwd, true, u.N2, 0, size_t(B) * u.nR, by, single)) != kOk) return st;
+    if ((st = galloc(&p.Ut, size_t(B) * ncol1 * u.N1)) != kOk) return st;
+    if ((st = fftset_make(p.col_bwd, false, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
+    if ((st = fftset_make(p.col_fwd, true, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
+    if ((st = fftset_make(p.row_bwd, false, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
+    if ((st = fftset_make(p.row_fwd, true, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
+    {
+        std::vector<FftPlan *> all = {&p.fine_bwd, &p.fine_fwd};
+        for (FftSet *fs : {&p.th_fwd, &p.th_bwd, &p.col_bwd, &p.col_fwd, &p.row_bwd, &p.row_fwd})
+            for (auto &q : fs->plans) all.push_back(&q);
+        size_t wb = 0;
+        for (FftPlan *q : all) wb = std::max(wb, q->work_bytes);
+        if (wb) {
+            if ((st = dalloc(reinterpret_cast<char **>(&p.fft_work), wb, by)) != kOk) return st;
+            for (FftPlan *q : all)
+                if (q->info && q->work_bytes) rocfft_execution_info_set_work_buffer(q->info, p.fft_work, q->work_bytes);
+        }
+    }
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
 }
 
-static GridDims dims_of(const pfsht_plan_s &p) {
+static GridDims dims_of(const pfsht_plan_s &p, int nb) {
     GridDims g;
-    g.B = p.B; g.nh = p.nh; g.nth = p.nth; g.nph = p.nph; g.mmax = p.mmax; g.L = p.L; g.kind = p.geom.kind;
+    g.B = nb; g.nh = p.nh; g.nth = p.nth; g.nph = p.nph; g.mmax = p.mmax; g.L = p.L; g.kind = p.geom.kind;
     g.spin = p.spin;
     return g;
 }
@@ -855,85 +969,93 @@ static void timing_drain(pfsht_plan_s &p) {
 }
 
 // The grid part of the stretched synthesis, from the ring buffer to the points, with the grid
-// buffers stored as C (stages 1-7).
+// buffers stored as C (stages 1-7), for nb transforms.
 template <class C>
-static Status synth_grid(pfsht_plan_s &p, TimedApply *t, double2 *f, hipStream_t s) {
-    const GridDims g = dims_of(p);
+static Status synth_grid(pfsht_plan_s &p, TimedApply *t, OutCols f, int nb, hipStream_t s) {
+    const GridDims g = dims_of(p, nb);
+    NufftDev u = p.nu;
+    u.B = nb;
     C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
     launch_spec_dfs(g, p.rings->d, p.rings->Gd, p.wth, Ft, s);
     tmark(p, t, 2, s);  // stages 2 and 3 (longitude FFT, DFS) have no work of their own
-    Status st = fft_exec(p.th_fwd, Ft, s);
+    Status st = fftset_exec(p.th_fwd, Ft, nb, s);
     if (st != kOk) return st;
     tmark(p, t, 5, s);
-    launch_deconv_pad_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ft), Ut, s);
+    launch_deconv_pad_ut(u, u.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ft), Ut, s);
     tmark(p, t, 6, s);
-    if ((st = fft_exec(p.col_bwd, Ut, s)) != kOk) return st;
-    launch_ut_to_rows(p.nu, p.mmax, static_cast<const C *>(Ut), U, s);
-    if ((st = fft_exec(p.row_bwd, U, s)) != kOk) return st;
+    if ((st = fftset_exec(p.col_bwd, Ut, nb, s)) != kOk) return st;
+    launch_ut_to_rows(u, p.mmax, static_cast<const C *>(Ut), U, s);
+    if ((st = fftset_exec(p.row_bwd, U, nb, s)) != kOk) return st;
     tmark(p, t, 7, s);
-    launch_interp(p.nu, static_cast<const C *>(U), f, s);
+    launch_interp(u, static_cast<const C *>(U), f, s);
     tmark(p, t, 8, s);  // stage 8 is the adjoint's
     return kOk;
 }
 
 // The grid part of the stretched adjoint, from the points to the ring buffer (stages 1-6).
 template <class C>
-static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, const double2 *f, hipStream_t s) {
-    const GridDims g = dims_of(p);
+static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, InCols f, int nb, hipStream_t s) {
+    const GridDims g = dims_of(p, nb);
+    NufftDev u = p.nu;
+    u.B = nb;
     C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
-    launch_spread(p.nu, f, U, s);
+    launch_spread(u, f, U, s);
     tmark(p, t, 1, s);
-    Status st = fft_exec(p.row_fwd, U, s);
+    Status st = fftset_exec(p.row_fwd, U, nb, s);
     if (st != kOk) return st;
-    launch_rows_to_ut(p.nu, p.mmax, static_cast<const C *>(U), Ut, s);
-    if ((st = fft_exec(p.col_fwd, Ut, s)) != kOk) return st;
+    launch_rows_to_ut(u, p.mmax, static_cast<const C *>(U), Ut, s);
+    if ((st = fftset_exec(p.col_fwd, Ut, nb, s)) != kOk) return st;
     tmark(p, t, 2, s);
-    launch_deconv_trunc_ut(p.nu, p.nu.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ut), Ft, s);
+    launch_deconv_trunc_ut(u, u.pc1, p.mmax, p.nph, double(p.nph), static_cast<const C *>(Ut), Ft, s);
     tmark(p, t, 3, s);
-    if ((st = fft_exec(p.th_bwd, Ft, s)) != kOk) return st;
+    if ((st = fftset_exec(p.th_bwd, Ft, nb, s)) != kOk) return st;
     tmark(p, t, 4, s);
     launch_spec_fold(g, static_cast<const C *>(Ft), p.wth, p.rings->Gd, s);
     return kOk;
 }
 
-static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s);
-static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s);
+static Status synth_cols(pfsht_plan_s &p, InCols alm, OutCols f, int nb, hipStream_t s);
+static Status adjoint_cols(pfsht_plan_s &p, InCols f, OutCols alm, int nb, hipStream_t s);
+
+static int64_t alm_len(const pfsht_plan_s &p) { return int64_t(p.L + 1) * (2 * p.L + 1); }
 
 // synthesize_multibox! / adjoint_multibox! (src/MultiBox.jl:83-112): every point belongs to one
 // box, so the synthesis writes each of f's entries once; the adjoint adds the boxes' coefficients
 // in box order.
-static Status multibox_synth(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+static Status multibox_synth(pfsht_plan_s &p, InCols alm, OutCols f, int nb, hipStream_t s) {
     for (size_t g = 0; g < p.boxes.size(); ++g) {
-        Status st = synth_impl(*p.boxes[g], alm, p.mb_f, s);
+        const int64_t Mg = p.box_npts[g];
+        Status st = synth_cols(*p.boxes[g], alm, out_cols(p.mb_f, Mg), nb, s);
         if (st != kOk) return st;
-        launch_scatter_field(f, p.mb_f, p.box_idx[g], p.box_npts[g], p.M, p.B, s);
+        launch_scatter_field(f, p.mb_f, p.box_idx[g], Mg, nb, s);
     }
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
 
-static Status multibox_adjoint(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
-    const int64_t n = int64_t(p.L + 1) * (2 * p.L + 1) * p.B;
+static Status multibox_adjoint(pfsht_plan_s &p, InCols f, OutCols alm, int nb, hipStream_t s) {
+    const int64_t n = alm_len(p);
     for (size_t g = 0; g < p.boxes.size(); ++g) {
-        launch_gather_field(p.mb_f, f, p.box_idx[g], p.box_npts[g], p.M, p.B, s);
-        Status st = adjoint_impl(*p.boxes[g], p.mb_f, g == 0 ? alm : p.mb_acc, s);
+        const int64_t Mg = p.box_npts[g];
+        launch_gather_field(p.mb_f, f, p.box_idx[g], Mg, nb, s);
+        Status st = adjoint_cols(*p.boxes[g], in_cols(p.mb_f, Mg), g == 0 ? alm : out_cols(p.mb_acc, n), nb, s);
         if (st != kOk) return st;
-        if (g > 0) launch_zaxpy(alm, p.mb_acc, make_double2(1.0, 0.0), n, s);
+        if (g > 0) launch_cols_add(alm, p.mb_acc, n, nb, s);
     }
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
 
-static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
-    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_synth(p, alm, f, s);
+static Status synth_body(pfsht_plan_s &p, InCols alm, OutCols f, int nb, hipStream_t s) {
+    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_synth(p, alm, f, nb, s);
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 0);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:60-91
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
-        ring_forward(r, alm, s);
+        ring_forward(r, alm, nb, s);
         tmark(p, t, 1, s);
-        launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, p.B, s);
+        launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, f, p.M, p.mmax, nb, s);
         tmark(p, t, 2, s);  // the phase sums are timed as stage 1
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
@@ -941,66 +1063,75 @@ static Status synth_body(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStr
     // src/Stretch.jl:262-277: ring pass, window + DFS (in longitude-spectral space), colatitude FFT,
     // phase and deconvolution into the fine grid, fine inverse FFT, interpolation
     tmark(p, t, 0, s);
-    ring_forward(*p.rings, alm, s);
+    ring_forward(*p.rings, alm, nb, s);
     tmark(p, t, 1, s);
-    Status st = p.precision == PFSHT_PREC_MIXED ? synth_grid<float2>(p, t, f, s) : synth_grid<double2>(p, t, f, s);
+    Status st = p.precision == PFSHT_PREC_MIXED ? synth_grid<float2>(p, t, f, nb, s)
+                                                : synth_grid<double2>(p, t, f, nb, s);
     if (st != kOk) return st;
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
 
-static Status adjoint_body(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
-    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_adjoint(p, f, alm, s);
+// The adjoint ring pass as timed stages: the pass (stage 0 of the adjoint order), then the partial
+// reduction (stage 8, empty on the MFMA path and for spin engines, which finish in one launch).
+static void ring_adjoint_timed(pfsht_plan_s &p, TimedApply *t, OutCols alm, int nb, hipStream_t s) {
+    pfsht_rings_s &r = *p.rings;
+    if (r.spin) {
+        launch_spin_adj(r.d, r.sd, r.Gd, alm, nb, s);
+        tmark(p, t, 8, s);
+    } else {
+        launch_ring_adj_pass(r.d, r.Gd, alm, nb, s);
+        tmark(p, t, 8, s);
+        launch_ring_adj_finish(r.d, alm, nb, s);
+    }
+    tmark(p, t, 9, s);
+}
+
+static Status adjoint_body(pfsht_plan_s &p, InCols f, OutCols alm, int nb, hipStream_t s) {
+    if (p.method == PFSHT_METHOD_MULTIBOX) return multibox_adjoint(p, f, alm, nb, s);
     if (p.timing && p.pending.size() >= 256) timing_drain(p);
     TimedApply *t = timing_begin(p, 1);
     if (p.method == PFSHT_METHOD_DIRECT) {  // src/Direct.jl:94-117
         pfsht_rings_s &r = *p.rings;
         tmark(p, t, 0, s);
-        launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, p.B, s);
+        launch_direct_adjoint(f, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, nb, s);
         tmark(p, t, 7, s);  // the phase products are timed as stage 1 (adjoint order)
-        if (r.spin) {
-            launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
-            tmark(p, t, 8, s);
-        } else {
-            launch_ring_adj_tiles(r.d, r.Gd, s);
-            tmark(p, t, 8, s);
-            launch_ring_adj_finalize(r.d, alm, s);
-        }
-        tmark(p, t, 9, s);
+        ring_adjoint_timed(p, t, alm, nb, s);
         PFSHT_HIP_TRY(hipGetLastError());
         return kOk;
     }
     // src/Stretch.jl:279-295, the exact transpose of the synthesis
-    pfsht_rings_s &r = *p.rings;
     tmark(p, t, 0, s);
-    Status st = p.precision == PFSHT_PREC_MIXED ? adjoint_grid<float2>(p, t, f, s) : adjoint_grid<double2>(p, t, f, s);
+    Status st = p.precision == PFSHT_PREC_MIXED ? adjoint_grid<float2>(p, t, f, nb, s)
+                                                : adjoint_grid<double2>(p, t, f, nb, s);
     if (st != kOk) return st;
     tmark(p, t, 7, s);
-    if (r.spin) {
-        launch_spin_adj(r.d, r.sd, r.Gd, alm, s);
-        tmark(p, t, 8, s);
-    } else {
-        launch_ring_adj_tiles(r.d, r.Gd, s);
-        tmark(p, t, 8, s);
-        launch_ring_adj_finalize(r.d, alm, s);
-    }
-    tmark(p, t, 9, s);
+    ring_adjoint_timed(p, t, alm, nb, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
 
-static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+static Status synth_cols(pfsht_plan_s &p, InCols alm, OutCols f, int nb, hipStream_t s) {
     Status st = apply_begin(p, s);
-    if (st == kOk) st = synth_body(p, alm, f, s);
+    if (st == kOk) st = synth_body(p, alm, f, nb, s);
     if (st == kOk) st = apply_end(p, s);
     return st;
 }
 
-static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+static Status adjoint_cols(pfsht_plan_s &p, InCols f, OutCols alm, int nb, hipStream_t s) {
     Status st = apply_begin(p, s);
-    if (st == kOk) st = adjoint_body(p, f, alm, s);
+    if (st == kOk) st = adjoint_body(p, f, alm, nb, s);
     if (st == kOk) st = apply_end(p, s);
     return st;
+}
+
+// All B transforms in the reference's contiguous layouts.
+static Status synth_impl(pfsht_plan_s &p, const double2 *alm, double2 *f, hipStream_t s) {
+    return synth_cols(p, in_cols(alm, alm_len(p)), out_cols(f, p.M), p.B, s);
+}
+
+static Status adjoint_impl(pfsht_plan_s &p, const double2 *f, double2 *alm, hipStream_t s) {
+    return adjoint_cols(p, in_cols(f, p.M), out_cols(alm, alm_len(p)), p.B, s);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1276,20 +1407,84 @@ int pfsht_plan_create(const double *theta, const double *phi, int64_t npts, int3
 
 int pfsht_plan_create_spin(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
                            int32_t ntrans, int32_t spin, int32_t method, int32_t precision, pfsht_plan_t *out) {
-    if (!out) { set_error("out is NULL"); return kEArg; }
-    *out = nullptr;
-    if (!theta || !phi) { set_error("theta/phi is NULL"); return kEArg; }
-    if (precision != PFSHT_PREC_FP64 && precision != PFSHT_PREC_MIXED) { set_error("unknown precision"); return kEArg; }
-    if (precision == PFSHT_PREC_MIXED && !(tol >= 1e-6)) {
+    pfsht_plan_opts o;
+    pfsht_plan_opts_default(&o);
+    o.method = method;
+    o.precision = precision;
+    o.spin = spin;
+    return pfsht_plan_create_opts(theta, phi, npts, lmax, tol, ntrans, &o, out);
+}
+
+void pfsht_plan_opts_default(pfsht_plan_opts *o) {
+    if (o) std::memset(o, 0, sizeof *o);
+}
+
+}  // extern "C"
+
+// The caller's options with the reference's defaults for the keywords left at zero (src/Plan.jl:84-97),
+// checked.
+static Status opts_normalise(const pfsht_plan_opts *opts, double tol, pfsht_plan_opts &o) {
+    pfsht_plan_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.maxboxes == 0) o.maxboxes = kMaxBoxes;
+    if (o.oversample_theta == 0) o.oversample_theta = 1.0;
+    if (o.oversample_phi == 0) o.oversample_phi = 1.0;
+    if (o.epsapo_max == 0) o.epsapo_max = kEpsApoMax;
+    if (o.precision != PFSHT_PREC_FP64 && o.precision != PFSHT_PREC_MIXED) { set_error("unknown precision"); return kEArg; }
+    if (o.precision == PFSHT_PREC_MIXED && !(tol >= 1e-6)) {
         set_error("mixed precision needs tol >= 1e-6");
         return kEArg;
     }
+    if (o.method < PFSHT_METHOD_AUTO || o.method > PFSHT_METHOD_MULTIBOX) { set_error("unknown method"); return kEArg; }
+    if (o.maxboxes < 1) { set_error("maxboxes must be >= 1"); return kEArg; }
+    if (!(o.oversample_theta >= 1) || !(o.oversample_phi >= 1) || !(o.oversample_theta <= 16) ||
+        !(o.oversample_phi <= 16)) {
+        set_error("oversample_theta and oversample_phi must lie in [1, 16]");
+        return kEArg;
+    }
+    if (!(o.epsapo_max > 0) || !(o.epsapo_max <= 1)) { set_error("epsapo_max must lie in (0, 1]"); return kEArg; }
+    return kOk;
+}
+
+namespace pfsht {
+int64_t plan_estimate_bytes(const double *theta, const double *phi, int64_t M, int L, double tol, int B,
+                            const pfsht_plan_opts &o);
+}
+
+extern "C" {
+
+int pfsht_plan_estimate_bytes(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                              int32_t ntrans, const pfsht_plan_opts *opts, int64_t *bytes) {
+    if (!theta || !phi || !bytes) { set_error("NULL argument"); return kEArg; }
+    if (npts < 1 || lmax < 0 || ntrans < 1 || !(tol > 0) || !(tol < 1)) { set_error("bad argument"); return kEArg; }
+    pfsht_plan_opts o;
+    Status st = opts_normalise(opts, tol, o);
+    if (st != kOk) return st;
+    try {
+        *bytes = pfsht::plan_estimate_bytes(theta, phi, npts, lmax, tol, ntrans, o);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return kENomem;
+    }
+    if (*bytes < 0) { set_error("the plan's size could not be estimated"); return kEArg; }
+    return kOk;
+}
+
+int pfsht_plan_create_opts(const double *theta, const double *phi, int64_t npts, int32_t lmax, double tol,
+                           int32_t ntrans, const pfsht_plan_opts *opts, pfsht_plan_t *out) {
+    if (!out) { set_error("out is NULL"); return kEArg; }
+    *out = nullptr;
+    if (!theta || !phi) { set_error("theta/phi is NULL"); return kEArg; }
+    pfsht_plan_opts o;
+    Status st0 = opts_normalise(opts, tol, o);
+    if (st0 != kOk) return st0;
     auto *p = new (std::nothrow) pfsht_plan_s;
     if (!p) { set_error("host allocation failed"); return kENomem; }
-    p->precision = precision;
+    p->precision = o.precision;
     Status st;
     try {
-        st = plan_build(*p, theta, phi, npts, lmax, tol, ntrans, method, spin);
+        st = plan_build(*p, theta, phi, npts, lmax, tol, ntrans, o);
+        if (st == kOk) st = cols_alloc(*p);
     } catch (const std::bad_alloc &) {
         set_error("host allocation failed");
         st = kENomem;
@@ -1388,6 +1583,49 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
     return kOk;
 }
 
+// The column apply shared by pfsht_synth_cols (dir 0) and pfsht_adjoint_cols (dir 1).
+static int cols_apply(pfsht_plan_t p, int dir, int32_t nb, const void *const *in, void *const *out, void *stream) {
+    if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
+    if (nb < 1 || nb > p->B) {
+        set_error("nb = " + std::to_string(nb) + " outside 1:" + std::to_string(p->B) + " (the plan's ntrans)");
+        return kEArg;
+    }
+    for (int b = 0; b < nb; ++b)
+        if (!in[b] || !out[b]) { set_error("NULL column pointer"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int slot = p->col_next;
+    p->col_next = (slot + 1) % pfsht_plan_s::kColSlots;
+    if (p->col_used[slot]) PFSHT_HIP_TRY(hipEventSynchronize(p->col_ev[slot]));
+    const size_t B = size_t(p->B);
+    void **h = p->col_host + size_t(slot) * 2 * B, **d = p->col_dev + size_t(slot) * 2 * B;
+    for (int b = 0; b < nb; ++b) {
+        h[b] = const_cast<void *>(in[b]);
+        h[B + b] = out[b];
+    }
+    PFSHT_HIP_TRY(hipMemcpyAsync(d, h, 2 * B * sizeof(void *), hipMemcpyHostToDevice, s));
+    InCols ic;
+    ic.tab = reinterpret_cast<const double2 *const *>(d);
+    OutCols oc;
+    oc.tab = reinterpret_cast<double2 *const *>(d + B);
+    Status st = kOk;
+    if (dir == 0) st = synth_cols(*p, ic, oc, nb, s);
+    else st = adjoint_cols(*p, ic, oc, nb, s);
+    if (st != kOk) return st;
+    PFSHT_HIP_TRY(hipEventRecord(p->col_ev[slot], s));
+    p->col_used[slot] = true;
+    return kOk;
+}
+
+int pfsht_synth_cols(pfsht_plan_t p, int32_t nb, const void *const *alm_cols, void *const *f_cols, void *stream) {
+    return cols_apply(p, 0, nb, alm_cols, f_cols, stream);
+}
+
+int pfsht_adjoint_cols(pfsht_plan_t p, int32_t nb, const void *const *f_cols, void *const *alm_cols, void *stream) {
+    return cols_apply(p, 1, nb, f_cols, alm_cols, stream);
+}
+
 int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, int32_t where, void *stream) {
     if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
     if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }
@@ -1426,9 +1664,9 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
         launch_transpose(static_cast<const double2 *>(in_d), Ft, p->nph, p->nth, p->B, s);
         launch_deconv_pad(u, u.c1, Ft, U, s);
         if ((st = fft_exec(p->fine_bwd, U, s)) != kOk) return st;
-        launch_interp(u, static_cast<const double2 *>(U), static_cast<double2 *>(out_d), s);
+        launch_interp(u, static_cast<const double2 *>(U), out_cols(static_cast<double2 *>(out_d), p->M), s);
     } else {
-        launch_spread(u, static_cast<const double2 *>(in_d), U, s);
+        launch_spread(u, in_cols(static_cast<const double2 *>(in_d), p->M), U, s);
         if ((st = fft_exec(p->fine_fwd, U, s)) != kOk) return st;
         launch_deconv_trunc(u, u.c1, U, Ft, s);
         launch_transpose(Ft, static_cast<double2 *>(out_d), p->nth, p->nph, p->B, s);
@@ -1507,7 +1745,7 @@ int pfsht_rings_from_alm(pfsht_rings_t r, const void *alm, void *G, int32_t wher
         a = r->alm_stage;
         Gout = r->G_stage;
     }
-    ring_forward(*r, a, s);
+    ring_forward(*r, in_cols(a, int64_t(r->d.L + 1) * (2 * r->d.L + 1)), r->d.B, s);
     launch_gd_to_ref(r->d, r->Gd, Gout, s);
     PFSHT_HIP_TRY(hipGetLastError());
     if (where == PFSHT_WHERE_HOST) {
@@ -1534,7 +1772,7 @@ int pfsht_alm_from_rings(pfsht_rings_t r, const void *G, void *alm, int32_t wher
         aout = r->alm_stage;
     }
     launch_ref_to_gd(r->d, Gin, r->Gd, s);
-    ring_adjoint(*r, aout, s);
+    ring_adjoint(*r, out_cols(aout, int64_t(r->d.L + 1) * (2 * r->d.L + 1)), r->d.B, s);
     PFSHT_HIP_TRY(hipGetLastError());
     if (where == PFSHT_WHERE_HOST) {
         PFSHT_HIP_TRY(hipMemcpyAsync(alm, r->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
@@ -1567,3 +1805,127 @@ int pfsht_stage_times(pfsht_plan_t p, int32_t dir, double *ms, int64_t *counts, 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// Hooks of the serving runtime (serve.hip): a plan's size before it is built, and its column apply
+// split into the cross-stream ordering and the launches (so that the launches can be captured in a
+// graph while the ordering events stay outside it).
+// ------------------------------------------------------------------------------------------------
+namespace pfsht {
+
+// Device bytes of a stretched leaf over colatitudes [thlo, thhi] with npts points: the ring
+// engine's tables, checkpoints, partials and ring buffer, the grids, the points and the super-tile
+// windows (bounded by the super-tile count), from the host sizing of plan_build; -1 when the stretch
+// is refused. An upper estimate: the windows of super-tiles no point reaches are counted.
+static int64_t estimate_stretch(double thlo, double thhi, int64_t M, int L, double tol, int B,
+                                const pfsht_plan_opts &o, bool single) {
+    std::string msg;
+    Geom g0, g;
+    if (stretch_geometry(thlo, thhi, L, tol, g0, msg, o.epsapo_max) != 0) return -1;
+    const double tiny = std::sqrt(DBL_EPSILON);
+    const bool flip = g0.kind == kPole && g0.thhi >= kPi - tiny && g0.thlo > tiny;
+    if (flip) {
+        if (stretch_geometry(kPi - thhi, kPi - thlo, L, tol, g, msg, o.epsapo_max) != 0) return -1;
+    } else {
+        g = g0;
+    }
+    const int Kt = int(std::ceil(double(g.K + kKMargin) * o.oversample_theta));
+    int nh, nth;
+    if (g.kind == kInterior) { nth = int(next_smooth(2 * Kt + 1)); nh = nth; }
+    else { nh = int(next_smooth(Kt + 1)); nth = 2 * nh; }
+    std::vector<double> th, Th;
+    stretch_rings(g, nh, th, Th);
+    const int as = std::abs(o.spin);
+    std::vector<int> mlim(nh);
+    int mmax = 0;
+    for (int j = 0; j < nh; ++j) {
+        mlim[j] = std::max(polar_mmax(th[j], L, tol), as);
+        mmax = std::max(mmax, mlim[j]);
+    }
+    std::vector<int> jlo, jhi;
+    if (!nested_ranges(mlim, L, jlo, jhi, msg)) return -1;
+    int64_t by = 0;
+    for (int m = 0; m <= mmax; ++m) {
+        by += int64_t(std::max(0, L - m - 1)) * 32 * (o.spin ? 2 : 1);  // coefficient tables
+        const int nj = jhi[m] - jlo[m] + 1;
+        if (nj <= 0) continue;
+        const int nt = (nj + 63) / 64, nseg = seg_count(m, L, kSegBlocks);
+        by += int64_t(nt) * (nseg - 1) * 64 * 20;                                   // checkpoints
+        by += int64_t((nt + 1) / 2) * adj_rows_padded(m, L) * 32 * (B >= 2 ? 1 : B); // partials
+    }
+    const int nph = int(next_smooth_even(int64_t(std::ceil(double(2 * int64_t(std::max(mmax, as)) + 1) * o.oversample_phi))));
+    int w;
+    double beta;
+    es_params(tol, w, beta);
+    const int N1 = es_fine_size(nth, w), N2 = es_fine_size(nph, w);
+    const int S4 = 4 * (17 - w), win = 3 * (17 - w) + 16;
+    const int64_t nst = std::min<int64_t>(int64_t((N1 + S4 - 1) / S4) * ((N2 + S4 - 1) / S4), M);
+    const int64_t csz = single ? 8 : 16, ncol = 2 * mmax + 1;
+    by += int64_t(B) * ncol * nh * 16;                                  // ring buffer
+    by += int64_t(B) * csz * (int64_t(nth) * nph + int64_t(N1) * N2 + ncol * N1 + nst * win * win);  // grids, windows
+    by += M * 20 + int64_t(nh) * 24 + nst * 80;                        // points, permutation, tables
+    return by;
+}
+
+int64_t plan_estimate_bytes(const double *theta, const double *phi, int64_t M, int L, double tol, int B,
+                            const pfsht_plan_opts &o) {
+    if (M < 1 || L < 0 || B < 1) return -1;
+    double thlo = INFINITY, thhi = -INFINITY;
+    for (int64_t i = 0; i < M; ++i) { thlo = std::min(thlo, theta[i]); thhi = std::max(thhi, theta[i]); }
+    thlo = std::max(thlo, 0.0);
+    thhi = std::min(thhi, kPi);
+    const bool single = o.precision == PFSHT_PREC_MIXED;
+    const int64_t direct = int64_t(B) * (2 * L + 1) * M * 16 + ((M + 63) / 64) * int64_t(L + 1) * (L + 2) / 2 * 32 *
+                           (B >= 2 ? 1 : B) + M * 32;
+    int method = o.method;
+    if (method == PFSHT_METHOD_AUTO) {
+        const double cd = cost_direct(M, L, B), cs = cost_stretch(thlo, thhi, M, L, B, tol);
+        method = cs < cd ? PFSHT_METHOD_STRETCH : PFSHT_METHOD_DIRECT;
+        const Partition part = partition_boxes(theta, phi, M, L, B, tol, o.maxboxes);
+        if (part.groups.size() > 1) {
+            double cm = 0;
+            for (double c : part.costs) cm += c;
+            if (cm < std::min(cd, cs)) {
+                int64_t by = int64_t(L + 1) * (2 * L + 1) * B * 16;
+                for (const auto &grp : part.groups) {
+                    double lo = INFINITY, hi = -INFINITY;
+                    for (int32_t i : grp) { lo = std::min(lo, theta[i]); hi = std::max(hi, theta[i]); }
+                    const int64_t b = estimate_stretch(std::max(lo, 0.0), std::min(hi, kPi), int64_t(grp.size()), L,
+                                                       tol, B, o, single);
+                    if (b < 0) return direct;
+                    by += b + int64_t(grp.size()) * 4 + int64_t(grp.size()) * B * 16;
+                }
+                return by;
+            }
+        }
+    }
+    if (method == PFSHT_METHOD_DIRECT) return direct;
+    const int64_t b = estimate_stretch(thlo, thhi, M, L, tol, B, o, single);
+    return b < 0 ? direct : b;
+}
+
+// Cross-stream ordering of a plan's applies: an apply on stream s waits for the plan's previous one.
+Status plan_order_begin(pfsht_plan_t p, hipStream_t s) { return apply_begin(*p, s); }
+Status plan_order_end(pfsht_plan_t p, hipStream_t s) { return apply_end(*p, s); }
+// The launches of a column apply (dir 0 synthesis, 1 adjoint), with no event recorded or waited for.
+Status plan_launches(pfsht_plan_t p, int dir, InCols in, OutCols out, int nb, hipStream_t s) {
+    return dir == 0 ? synth_body(*p, in, out, nb, s) : adjoint_body(*p, in, out, nb, s);
+}
+std::mutex &plan_mutex(pfsht_plan_t p) { return p->mu; }
+void plan_shape(pfsht_plan_t p, int &L, int64_t &M, int &B, int &dev) {
+    L = p->L;
+    M = p->M;
+    B = p->B;
+    dev = p->dev;
+}
+// Frees a plan once its last apply has finished, without synchronising the whole device.
+void plan_release(pfsht_plan_t p) {
+    if (!p) return;
+    DeviceGuard dg(p->dev);
+    if (p->done && p->has_done) (void)hipEventSynchronize(p->done);
+    for (int i = 0; i < pfsht_plan_s::kColSlots; ++i)
+        if (p->col_used[i]) (void)hipEventSynchronize(p->col_ev[i]);
+    delete p;
+}
+
+}  // namespace pfsht

@@ -10,7 +10,7 @@ namespace pfsht {
 // than by a running product (src/Direct.jl:69-88).
 __global__ void direct_synth_kernel(const double2 *__restrict__ Gd, const double *__restrict__ phis,
                                     const int32_t *__restrict__ mlim, const int32_t *__restrict__ perm,
-                                    double2 *__restrict__ f, int64_t M, int mmax) {
+                                    OutCols f, int64_t M, int mmax) {
     const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
     const int b = blockIdx.y;
     if (j >= M) return;
@@ -29,12 +29,12 @@ __global__ void direct_synth_kernel(const double2 *__restrict__ Gd, const double
         ar += (gp.x * cs - gp.y * sn) + (gm.x * cs + gm.y * sn);
         ai += (gp.x * sn + gp.y * cs) + (gm.y * cs - gm.x * sn);
     }
-    f[int64_t(perm[j]) + int64_t(b) * M] = make_double2(ar, ai);
+    gstore(f.col(b) + perm[j], make_double2(ar, ai));
 }
 
 // G[j, m] = f_j conj(cis(m phi_j)), G[j, -m] = f_j cis(m phi_j) on the orders ring j retains
 // (src/Direct.jl:94-117); alm_from_rings! then reads exactly those entries.
-__global__ void direct_adjoint_kernel(const double2 *__restrict__ f, const double *__restrict__ phis,
+__global__ void direct_adjoint_kernel(InCols f, const double *__restrict__ phis,
                                       const int32_t *__restrict__ mlim, const int32_t *__restrict__ perm,
                                       double2 *__restrict__ Gd, int64_t M, int mmax) {
     const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -42,7 +42,7 @@ __global__ void direct_adjoint_kernel(const double2 *__restrict__ f, const doubl
     if (j >= M) return;
     const int64_t ncol = 2 * mmax + 1;
     double2 *G = Gd + int64_t(b) * ncol * M + j;
-    const double2 fj = f[int64_t(perm[j]) + int64_t(b) * M];
+    const double2 fj = gload(f.col(b) + perm[j]);
     const double phi = phis[j];
     const int mt = min(mlim[j], mmax);
     G[int64_t(mmax) * M] = fj;
@@ -55,14 +55,14 @@ __global__ void direct_adjoint_kernel(const double2 *__restrict__ f, const doubl
 }
 
 void launch_direct_synth(const double2 *Gd, const double *phis, const int32_t *mlim, const int32_t *perm,
-                         double2 *f, int64_t M, int mmax, int B, hipStream_t s) {
-    dim3 grid(unsigned((M + 255) / 256), B);
+                         OutCols f, int64_t M, int mmax, int nb, hipStream_t s) {
+    dim3 grid(unsigned((M + 255) / 256), nb);
     direct_synth_kernel<<<grid, 256, 0, s>>>(Gd, phis, mlim, perm, f, M, mmax);
 }
 
-void launch_direct_adjoint(const double2 *f, const double *phis, const int32_t *mlim, const int32_t *perm,
-                           double2 *Gd, int64_t M, int mmax, int B, hipStream_t s) {
-    dim3 grid(unsigned((M + 255) / 256), B);
+void launch_direct_adjoint(InCols f, const double *phis, const int32_t *mlim, const int32_t *perm,
+                           double2 *Gd, int64_t M, int mmax, int nb, hipStream_t s) {
+    dim3 grid(unsigned((M + 255) / 256), nb);
     direct_adjoint_kernel<<<grid, 256, 0, s>>>(f, phis, mlim, perm, Gd, M, mmax);
 }
 

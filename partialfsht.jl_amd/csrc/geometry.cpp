@@ -4,6 +4,7 @@
 #include "geometry.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -88,7 +89,7 @@ double es_cdf(double beta, double u) {
     return acc;
 }
 
-int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &out, std::string &msg) {
+int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &out, std::string &msg, double epsapo_max) {
     const double lo = thlo, hi = thhi;
     if (!(lo <= hi)) { msg = "thlo exceeds thhi"; return -1; }
     if (!(lo >= 0 && hi <= kPi)) { msg = "colatitudes must lie in [0, pi]"; return -1; }
@@ -112,7 +113,7 @@ int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &out, std
         const double h = c.h;
         if (!(h > 0)) continue;
         const double Lh = double(L) * h;
-        if (!(Lh >= 2 * beta / (kEpsApoMax * kEpsApoMax))) continue;
+        if (!(Lh >= 2 * beta / (epsapo_max * epsapo_max))) continue;
         const double ea = std::sqrt(2 * beta / Lh);
         const double pad = h * ea;
         if (c.kind == kInterior) {
@@ -134,8 +135,13 @@ int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &out, std
         }
     }
     if (!found) {
-        msg = "stretched transform is degenerate for this colatitude range at this lmax and tol: every "
-              "periodisation needs an apodisation wider than the data; use a direct evaluation";
+        char buf[320];
+        std::snprintf(buf, sizeof buf,
+                      "no stretched geometry for colatitudes [%.6g, %.6g] at lmax %d and tol %.3g: each candidate "
+                      "needs an apodisation fraction above epsapo_max = %.3g or a pad past a pole; the Direct "
+                      "leaf serves such point sets",
+                      lo, hi, L, eps, epsapo_max);
+        msg = buf;
         return 1;
     }
     return 0;

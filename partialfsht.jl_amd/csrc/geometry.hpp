@@ -35,8 +35,10 @@ bool nested_ranges(const std::vector<int> &mlim, int L, std::vector<int> &jlo, s
                    std::string &msg);
 // src/Windows.jl:208-226
 double es_cdf(double beta, double u);
-// src/Windows.jl:84-141: 0 ok, 1 degenerate, -1 bad argument (msg set)
-int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &g, std::string &msg);
+// src/Windows.jl:84-141: 0 ok, 1 degenerate, -1 bad argument (msg set); epsapo_max is the largest
+// apodisation fraction admitted (Plan's εapo_max, src/Plan.jl:97)
+int stretch_geometry(double thlo, double thhi, int L, double eps, Geom &g, std::string &msg,
+                     double epsapo_max = kEpsApoMax);
 // src/Windows.jl:237-250, 272-298, 307-320
 double stretch_window(const Geom &g, double Theta);
 double stretch_to_theta(const Geom &g, double Theta);

@@ -41,6 +41,18 @@ __device__ __forceinline__ double coef_after(double c, double acc) {
     return c;
 }
 
+// The same, with the coefficient's two 32-bit halves written by volatile scalar moves tied to the
+// running value: a volatile asm is never hoisted out of a loop, so inside a loop over points the
+// coefficients are not kept live across iterations (which spills the SGPR file).
+template <int I, class T>
+__device__ __forceinline__ double coef_now(double acc) {
+    constexpr uint64_t bits = __builtin_bit_cast(uint64_t, T::c[I]);
+    uint32_t lo, hi;
+    asm volatile("s_mov_b32 %0, %1" : "=s"(lo) : "i"(uint32_t(bits)), "v"(acc));
+    asm volatile("s_mov_b32 %0, %1" : "=s"(hi) : "i"(uint32_t(bits >> 32)), "v"(acc));
+    return __hiloint2double(int(hi), int(lo));
+}
+
 // Both rows of a point (axes 1 and 2) in lockstep, for interpolation: the W - 2 interior polynomials
 // in u and the edge polynomial at the two end cells, all running their Horner steps together, so
 // each coefficient is materialised once (tied to the previous step, as in the half row below) and
@@ -56,7 +68,7 @@ struct FullRows {
     static __device__ __forceinline__ void step(const double (&e1)[2], const double (&e2)[2], double u1, double u2,
                                                 double (&a1)[W], double (&a2)[W], std::integer_sequence<int, T...>) {
         if constexpr (S >= S0E) {
-            const double c = coef_after(EsHorner<W>::c[EOFF + ne - 2 - (S - S0E)], a1[0]);
+            const double c = coef_now<EOFF + ne - 2 - (S - S0E), EsHorner<W>>(a1[0]);
             a1[0] = fma(a1[0], e1[0], c);
             a1[W - 1] = fma(a1[W - 1], e1[1], c);
             a2[0] = fma(a2[0], e2[0], c);
@@ -66,7 +78,7 @@ struct FullRows {
     }
     template <int S, int T>
     static __device__ __forceinline__ void inner(double u1, double u2, double (&a1)[W], double (&a2)[W]) {
-        const double c = coef_after(EsHorner<W>::c[T * ni + ni - 2 - (S - S0I)], a1[T + 1]);
+        const double c = coef_now<T * ni + ni - 2 - (S - S0I), EsHorner<W>>(a1[T + 1]);
         a1[T + 1] = fma(a1[T + 1], u1, c);
         a2[T + 1] = fma(a2[T + 1], u2, c);
     }
@@ -76,9 +88,9 @@ struct FullRows {
         (step<S>(e1, e2, u1, u2, a1, a2, std::make_integer_sequence<int, W - 2>{}), ...);
     }
     template <int... T>
-    static __device__ __forceinline__ void init(double (&a)[W], std::integer_sequence<int, T...>) {
-        a[0] = a[W - 1] = EsHorner<W>::c[EOFF + ne - 1];
-        ((a[T + 1] = EsHorner<W>::c[T * ni + ni - 1]), ...);
+    static __device__ __forceinline__ void init(double (&a)[W], double dep, std::integer_sequence<int, T...>) {
+        a[0] = a[W - 1] = coef_now<EOFF + ne - 1, EsHorner<W>>(dep);
+        ((a[T + 1] = coef_now<T * ni + ni - 1, EsHorner<W>>(dep)), ...);
     }
 };
 // The kernel values of a point at (x1, x2) with base cells (b1, b2): k1 along axis 1, k2 along axis 2,
@@ -90,8 +102,8 @@ __device__ __forceinline__ void es_rows(double x1, int b1, double x2, int b2, do
     const double u2 = 2.0 * (double(b2) - x2) + double(W - 1);
     const double e1[2] = {2.0 * sqrt(0.5 * (1.0 + u1)) - 1.0, 2.0 * sqrt(0.5 * (1.0 - u1)) - 1.0};
     const double e2[2] = {2.0 * sqrt(0.5 * (1.0 + u2)) - 1.0, 2.0 * sqrt(0.5 * (1.0 - u2)) - 1.0};
-    F::init(k1, std::make_integer_sequence<int, W - 2>{});
-    F::init(k2, std::make_integer_sequence<int, W - 2>{});
+    F::init(k1, u1, std::make_integer_sequence<int, W - 2>{});
+    F::init(k2, u2, std::make_integer_sequence<int, W - 2>{});
     F::steps(e1, e2, u1, u2, k1, k2, std::make_integer_sequence<int, F::NS>{});
 }
 
@@ -335,16 +347,16 @@ void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U
 }
 
 // ------------------------------------------------------------------------------------------------
-// Interpolation (type 2): one workgroup per non-empty super-tile. The super-tile's window of the
-// fine grid (4 S + w - 1 cells square, wrapped periodically) is staged in LDS with coalesced row
-// reads, then each thread interpolates bin-sorted points of the super-tile from LDS.
+// Interpolation (type 2). One or two transforms: one workgroup per (non-empty super-tile,
+// transform); the super-tile's window of the fine grid (4 S + w - 1 cells square, wrapped
+// periodically) is staged in LDS with coalesced row reads, then each thread interpolates bin-sorted
+// points of the super-tile from LDS, evaluating each point's kernel rows.
 // ------------------------------------------------------------------------------------------------
 template <int W, class C>
 __global__ void __launch_bounds__(256)
 interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
-                 const C *__restrict__ U, double2 *__restrict__ f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, int64_t M, int N1, int N2, int S4, int ns2, int win, int r0,
-                 int nR) {
+                 const C *__restrict__ U, OutCols f, const int32_t *__restrict__ st_list,
+                 const int32_t *__restrict__ bin_ofs, int N1, int N2, int S4, int ns2, int win, int r0, int nR) {
     extern __shared__ __align__(16) unsigned char lds_ip_raw[];
     C *lw = reinterpret_cast<C *>(lds_ip_raw);
     const int slot = blockIdx.x;
@@ -352,6 +364,7 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const int st = st_list[slot];
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
     const C *__restrict__ Ub = U + int64_t(b) * nR * N2;
+    const gptr fcol = f.col(b);
     for (int r = threadIdx.x >> 6; r < win; r += 4) {
         int pr = (x0 + r) % N1 - r0;
         if (pr < 0) pr += N1;
@@ -380,35 +393,138 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             ar = fma(k1[t1], rr, ar);
             ai = fma(k1[t1], ri, ai);
         }
-        f[int64_t(perm[p]) + int64_t(b) * M] = make_double2(ar, ai);
+        gstore(fcol + perm[p], make_double2(ar, ai));
+    }
+}
+
+// Three or more transforms: one workgroup of 512 threads per (non-empty super-tile, group of up to
+// kt transforms). Each thread takes one point of the super-tile at a time and evaluates its kernel
+// rows once for the whole group; the transforms' windows go through two LDS buffers, the next one's
+// cells loaded into registers (up to kIpCells per thread) while the current one is interpolated,
+// so the loads overlap the arithmetic and one barrier per transform suffices.
+constexpr int kIpThreads = 512;
+// window cells per thread: the window edge is win = 3 S + 16 with S = 17 - W (the plan's sizing)
+constexpr int ip_cells(int W) { return ((3 * (17 - W) + 16) * (3 * (17 - W) + 16) + kIpThreads - 1) / kIpThreads; }
+
+template <int W, class C>
+__global__ void __launch_bounds__(kIpThreads)
+interp_st_multi_kernel(const double *__restrict__ X1, const double *__restrict__ X2,
+                       const int32_t *__restrict__ perm, const C *__restrict__ U, OutCols f,
+                       const int32_t *__restrict__ st_list, const int32_t *__restrict__ bin_ofs, int N1, int N2,
+                       int S4, int ns2, int win, int r0, int nR, int nb, int kt) {
+    extern __shared__ __align__(16) unsigned char lds_ipm_raw[];
+    C *lw0 = reinterpret_cast<C *>(lds_ipm_raw);
+    const int slot = blockIdx.x;
+    const int bfirst = blockIdx.y * kt, blast = min(nb, bfirst + kt);
+    const int st = st_list[slot];
+    const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
+    const int wsz = win * win;
+    constexpr int kIpCells = ip_cells(W);
+    // this thread's window cells q = tid + kIpThreads i: their offset in a transform's grid (or -1)
+    int64_t src[kIpCells];
+#pragma unroll
+    for (int i = 0; i < kIpCells; ++i) {
+        const int q = int(threadIdx.x) + kIpThreads * i;
+        src[i] = -1;
+        if (q < wsz) {
+            const int r = q / win, c = q - r * win;
+            int pr = (x0 + r) % N1 - r0;
+            if (pr < 0) pr += N1;
+            if (pr < nR) src[i] = int64_t(pr) * N2 + (y0 + c) % N2;  // rows no kernel reaches stay unread
+        }
+    }
+    using E = decltype(C::x);  // the component type, kept in plain registers
+    E rx[kIpCells], ry[kIpCells];
+    auto fetch = [&](int b) {
+        const C *__restrict__ Ub = U + int64_t(b) * nR * N2;
+#pragma unroll
+        for (int i = 0; i < kIpCells; ++i) {
+            const C v = Ub[src[i] >= 0 ? src[i] : 0];
+            rx[i] = v.x;
+            ry[i] = v.y;
+        }
+    };
+    auto put = [&](C *lw) {
+#pragma unroll
+        for (int i = 0; i < kIpCells; ++i)
+            if (src[i] >= 0) {
+                C v;
+                v.x = rx[i];
+                v.y = ry[i];
+                lw[threadIdx.x + kIpThreads * i] = v;
+            }
+    };
+    const int start = bin_ofs[slot * 17], end = bin_ofs[slot * 17 + 16];
+    int it = 0;  // windows staged so far; buffer it % 2 is the current one
+#pragma unroll 1
+    for (int c0 = start; c0 < end; c0 += kIpThreads) {
+        const int p = c0 + int(threadIdx.x);
+        const bool valid = p < end;
+        const int pc = valid ? p : end - 1;
+        const double x1 = X1[pc], x2 = X2[pc];
+        const int b1 = int(ceil(x1 - 0.5 * W));
+        const int b2 = int(ceil(x2 - 0.5 * W));
+        double k1[W], k2[W];
+        if (valid) es_rows<W>(x1, b1, x2, b2, k1, k2);
+        const int off = (b1 - x0) * win + (b2 - y0);
+        const int64_t fi = perm[pc];
+        fetch(bfirst);
+        put(lw0 + (it & 1) * wsz);
+        __syncthreads();
+#pragma unroll 1
+        for (int b = bfirst; b < blast; ++b, ++it) {
+            const C *lw = lw0 + (it & 1) * wsz;
+            if (b + 1 < blast) fetch(b + 1);  // in flight during this transform's interpolation
+            const gptr fcol = f.col(b);  // uniform: one scalar address per transform
+            if (valid) {
+                const C *__restrict__ wrow = lw + off;
+                double ar = 0.0, ai = 0.0;
+#pragma unroll
+                for (int t1 = 0; t1 < W; ++t1) {
+                    double rr = 0.0, ri = 0.0;
+#pragma unroll
+                    for (int t2 = 0; t2 < W; ++t2) {
+                        const double2 v = ldc(wrow + t1 * win + t2);
+                        rr = fma(k2[t2], v.x, rr);
+                        ri = fma(k2[t2], v.y, ri);
+                    }
+                    ar = fma(k1[t1], rr, ar);
+                    ai = fma(k1[t1], ri, ai);
+                }
+                gstore(fcol + fi, make_double2(ar, ai));
+            }
+            if (b + 1 < blast) put(lw0 + ((it + 1) & 1) * wsz);
+            __syncthreads();
+        }
     }
 }
 
 // ------------------------------------------------------------------------------------------------
 // Spreading (type 1): MFMA super-tiles + ordered merge.
 // ------------------------------------------------------------------------------------------------
-// Spreading: one workgroup per non-empty super-tile; wave w owns bins w + kSpreadWaves k and takes
-// each bin's points 16 at a time. For such a chunk the wave first evaluates the kernel rows once per
-// point: lane (p, axis, h) evaluates the cells t < ceil(W/2) of its point and axis from the left
-// (h = 0) or, through phi(-z) = phi(z), from the right (h = 1), so all lanes use the same
-// polynomial coefficients (scalar operands). The rows go to a wave-private LDS staging area at the
-// point's offset within its bin's 16-cell window, zero elsewhere, with f at the point. The wave then
-// accumulates the chunk on FP64 MFMA, four points per instruction (A = f psi_row, B = psi_col),
-// keeping each of its bins' 16 x 16 tiles in registers. The inputs of the next chunk (its
-// coordinates and its f values) are loaded while the current chunk is evaluated and accumulated. No workgroup barrier until the end, where the tiles are added
-// into the LDS window wave after wave (bin windows overlap) and the window is written out for the
-// merge pass.
-template <int W, class C>
+// Spreading: one workgroup per (non-empty super-tile, group of G transforms); wave w owns bins
+// w + kSpreadWaves k and takes each bin's points 16 at a time. For such a chunk the wave first
+// evaluates the kernel rows once per point, for all G transforms: lane (p, axis, h) evaluates the
+// cells t < ceil(W/2) of its point and axis from the left (h = 0) or, through phi(-z) = phi(z), from
+// the right (h = 1), so all lanes use the same polynomial coefficients (scalar operands). The rows
+// go to a wave-private LDS staging area at the point's offset within its bin's 16-cell window, zero
+// elsewhere, with the G values of f at the point. The wave then accumulates the chunk on FP64 MFMA,
+// four points per instruction (A = f psi_row, B = psi_col), keeping each of its bins' 16 x 16 tiles
+// per transform in registers. The inputs of the next chunk (its coordinates and its f values) are
+// loaded while the current chunk is evaluated and accumulated. No workgroup barrier until the end,
+// where, transform by transform, the tiles are added into the LDS window wave after wave (bin
+// windows overlap) and the window is written out for the merge pass.
+template <int W, int G, class C>
 __global__ void __launch_bounds__(64 * kSpreadWaves)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
-                 const double2 *__restrict__ f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, C *__restrict__ Wst, int64_t M, int S, int ns2, int win,
-                 int nst) {
+                 InCols f, const int32_t *__restrict__ st_list, const int32_t *__restrict__ bin_ofs,
+                 C *__restrict__ Wst, int S, int ns2, int win, int nst, int nb) {
     constexpr int NH = (W + 1) / 2;  // cells per half row
     extern __shared__ double lds_sp[];
     double2 *lw = reinterpret_cast<double2 *>(lds_sp);  // [win][win]
     const int slot = blockIdx.x;
-    const int b = blockIdx.y;
+    const int b0 = blockIdx.y * G;
+    const int gc = min(G, nb - b0);  // transforms of this workgroup
     const int st = st_list[slot];
     const int S4 = 4 * S;
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
@@ -416,22 +532,26 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     // rows are kRowPad doubles apart so that the 16 lanes writing one entry of 16 rows, and the
     // MFMA operand reads, fall in different LDS banks
-    double *K1 = lds_sp + 2 * win * win + wave * kSpreadStage;  // [16][kRowPad] row kernel
-    double *K2 = K1 + 16 * kRowPad;                              // [16][kRowPad] column kernel
-    double2 *Fc = reinterpret_cast<double2 *>(K2 + 16 * kRowPad);  // [16]
-    const int ii = lane & 15, kq = lane >> 4;                    // MFMA operand layout
-    const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;  // kernel-row evaluation
+    double *K1 = lds_sp + 2 * win * win + wave * spread_stage_doubles(G);  // [16][kRowPad] row kernel
+    double *K2 = K1 + 16 * kRowPad;                                      // [16][kRowPad] column kernel
+    double2 *Fc = reinterpret_cast<double2 *>(K2 + 16 * kRowPad);         // [G][16]
+    const int ii = lane & 15, kq = lane >> 4;                            // MFMA operand layout
+    const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;     // kernel-row evaluation
     const double *__restrict__ Xa = axis == 0 ? X1 : X2;
-    const double2 *__restrict__ fb = f + int64_t(b) * M;
+    gcptr fb[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) fb[g] = f.col(b0 + min(g, gc - 1));
     for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_double2(0.0, 0.0);
     const int32_t *__restrict__ bo = bin_ofs + slot * 17;
     constexpr int NBW = 16 / kSpreadWaves;  // bins per wavefront; tile k accumulates bin wave + kSpreadWaves k
-    double4v dr[NBW], di[NBW];
+    double4v dr[NBW][G], di[NBW][G];
 #pragma unroll
-    for (int k = 0; k < NBW; ++k) {
-        dr[k] = double4v{0.0, 0.0, 0.0, 0.0};
-        di[k] = double4v{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int k = 0; k < NBW; ++k)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            dr[k][g] = double4v{0.0, 0.0, 0.0, 0.0};
+            di[k][g] = double4v{0.0, 0.0, 0.0, 0.0};
+        }
     int bs[NBW], be[NBW];  // point range [bs[k], be[k]) of this wave's bin k
 #pragma unroll
     for (int k = 0; k < NBW; ++k) {
@@ -439,11 +559,11 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         be[k] = bo[wave + kSpreadWaves * k + 1];
     }
     // Chunk cursors: (k1, c1) is the next chunk (bin k1, first point c1), (k2, c2) the one after;
-    // k = NBW when none is left. The next chunk's coordinate and f value are in flight while a chunk
-    // is worked on, and so is the point index of the one after, so the f gather never waits on the
-    // index load. The loads are unconditional (a lane past a chunk's end reads the super-tile's last
-    // point and its value is never used), so that waiting for one chunk's inputs never waits for the
-    // loads issued after them.
+    // k = NBW when none is left. The next chunk's coordinate and f values are in flight while a
+    // chunk is worked on, and so is the point index of the one after, so the f gathers never wait on
+    // the index load. The loads are unconditional (a lane past a chunk's end reads the super-tile's
+    // last point and its value is never used), so that waiting for one chunk's inputs never waits
+    // for the loads issued after them.
     auto advance = [&](int &k, int &c) {
         while (k < NBW && c >= be[k]) {
             ++k;
@@ -456,7 +576,12 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     int k2 = k1, c2 = c1 + 16;
     advance(k2, c2);
     double x1 = Xa[min(c1 + pa, plast)];
-    double2 f1 = fb[perm[min(c1 + pa, plast)]];
+    double2 f1[G];
+    {
+        const int q1 = perm[min(c1 + pa, plast)];
+#pragma unroll
+        for (int g = 0; g < G; ++g) f1[g] = gload(fb[g] + q1);
+    }
     int p2 = perm[min(c2 + pa, plast)];
     const typename HalfRow<W>::Edge ec = HalfRow<W>::edge();
     // Bins in a static order, so each accumulates into a fixed register tile.
@@ -465,14 +590,17 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
         while (k1 == kb) {  // wave-uniform
             const int n = min(16, be[kb] - c1);
             const double x = x1;
-            const double2 fv = f1;
+            double2 fv[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) fv[g] = f1[g];
             // shift the cursors by one chunk and issue the loads of the new pair
             k1 = k2;
             c1 = c2;
             c2 += 16;
             advance(k2, c2);
             x1 = Xa[min(c1 + pa, plast)];
-            f1 = fb[p2];
+#pragma unroll
+            for (int g = 0; g < G; ++g) f1[g] = gload(fb[g] + p2);
             p2 = perm[min(c2 + pa, plast)];
             // Staging rows of every lane are cleared, so rows and f of points q >= n are zero and
             // their products add exact zeros; lanes of points q < n then write their kernel values.
@@ -491,23 +619,30 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
                     if (h == 0) row[o + t] = kv[t];
                     else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
             }
-            if (axis == 0 && h == 0) Fc[pa] = pa < n ? fv : make_double2(0.0, 0.0);
+            if (axis == 0 && h == 0)
+#pragma unroll
+                for (int g = 0; g < G; ++g) Fc[g * 16 + pa] = pa < n ? fv[g] : make_double2(0.0, 0.0);
             wave_lds_order();
-            // all operands of the chunk first (one LDS wait), then up to four MFMA pairs
+            // all operands of the chunk first (one LDS wait), then up to four MFMA pairs per transform
             double av[4], cv[4];
-            double2 fq[4];
 #pragma unroll
             for (int gi = 0; gi < 4; ++gi) {
                 const int q = 4 * gi + kq;
                 av[gi] = K1[q * kRowPad + ii];
                 cv[gi] = K2[q * kRowPad + ii];
-                fq[gi] = Fc[q];
             }
 #pragma unroll
-            for (int gi = 0; gi < 4; ++gi) {
-                if (4 * gi >= n) break;  // wave-uniform
-                dr[kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].x, cv[gi], dr[kb], 0, 0, 0);
-                di[kb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].y, cv[gi], di[kb], 0, 0, 0);
+            for (int g = 0; g < G; ++g) {
+                if (g >= gc) break;  // wave-uniform
+                double2 fq[4];
+#pragma unroll
+                for (int gi = 0; gi < 4; ++gi) fq[gi] = Fc[g * 16 + 4 * gi + kq];
+#pragma unroll
+                for (int gi = 0; gi < 4; ++gi) {
+                    if (4 * gi >= n) break;  // wave-uniform
+                    dr[kb][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].x, cv[gi], dr[kb][g], 0, 0, 0);
+                    di[kb][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[gi] * fq[gi].y, cv[gi], di[kb][g], 0, 0, 0);
+                }
             }
             wave_lds_order();
         }
@@ -516,26 +651,34 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     // D[i][j] of v_mfma_f64_16x16x4f64: lane holds column j = lane % 16, register r holds row
     // i = lane / 16 + 4 r (checked on gfx950 with a one-wave layout probe). Bin windows overlap
     // (16 > S), so the waves add into LDS one at a time, in wave order, each wave its bins in
-    // order: race-free and a fixed summation order.
+    // order: race-free and a fixed summation order. One transform after another.
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (g >= gc) break;  // uniform over the workgroup
 #pragma unroll 1
-    for (int turn = 0; turn < kSpreadWaves; ++turn) {
-        if (wave == turn) {
+        for (int turn = 0; turn < kSpreadWaves; ++turn) {
+            if (wave == turn) {
 #pragma unroll
-            for (int k = 0; k < NBW; ++k) {
-                const int lb = wave + kSpreadWaves * k;
-                const int bi = lb >> 2, bj = lb & 3;
+                for (int k = 0; k < NBW; ++k) {
+                    const int lb = wave + kSpreadWaves * k;
+                    const int bi = lb >> 2, bj = lb & 3;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    double2 &cell = lw[(bi * S + kq + 4 * r) * win + bj * S + ii];
-                    cell.x += dr[k][r];
-                    cell.y += di[k][r];
+                    for (int r = 0; r < 4; ++r) {
+                        double2 &cell = lw[(bi * S + kq + 4 * r) * win + bj * S + ii];
+                        cell.x += dr[k][g][r];
+                        cell.y += di[k][g][r];
+                    }
                 }
             }
+            __syncthreads();
+        }
+        C *out = Wst + (int64_t(b0 + g) * nst + slot) * int64_t(win) * win;
+        for (int q = tid; q < win * win; q += 64 * kSpreadWaves) {
+            stc(out + q, lw[q]);
+            lw[q] = make_double2(0.0, 0.0);  // each thread clears the cells it wrote out
         }
         __syncthreads();
     }
-    C *out = Wst + (int64_t(b) * nst + slot) * int64_t(win) * win;
-    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) stc(out + q, lw[q]);
 }
 
 template <class C>
@@ -578,21 +721,38 @@ __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__
     stc(U + (int64_t(b) * nR + pr) * N2 + l2, make_double2(sr, si));
 }
 
+// transforms per multi-transform interpolation workgroup (each evaluates the kernel rows once per point)
+constexpr int kInterpTransforms = 8;
+
 template <int W, class C>
-static void interp_w(const NufftDev &u, const C *U, double2 *f, hipStream_t s) {
-    if (u.nst == 0) return;
-    dim3 grid(u.nst, u.B);
-    const size_t lds = sizeof(C) * size_t(u.win) * u.win;
-    interp_st_kernel<W, C><<<grid, 256, lds, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list, u.bin_ofs, u.M, u.N1, u.N2,
-                                                  4 * u.S, u.ns2, u.win, u.r0, u.nR);
+static void interp_w(const NufftDev &u, const C *U, OutCols f, hipStream_t s) {
+    if (u.nst == 0 || u.B <= 0) return;
+    const size_t win_bytes = sizeof(C) * size_t(u.win) * u.win;
+    if (u.B <= 2) {
+        interp_st_kernel<W, C><<<dim3(u.nst, u.B), 256, win_bytes, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list,
+                                                                       u.bin_ofs, u.N1, u.N2, 4 * u.S, u.ns2,
+                                                                       u.win, u.r0, u.nR);
+        return;
+    }
+    const int kt = std::min(u.B, kInterpTransforms);
+    dim3 grid(u.nst, (u.B + kt - 1) / kt);
+    interp_st_multi_kernel<W, C><<<grid, kIpThreads, 2 * win_bytes, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list,
+                                                                         u.bin_ofs, u.N1, u.N2, 4 * u.S, u.ns2,
+                                                                         u.win, u.r0, u.nR, u.B, kt);
 }
 
 template <int W, class C>
-static void spread_w(const NufftDev &u, const double2 *f, C *Wst, hipStream_t s) {
-    if (u.nst == 0) return;
-    dim3 grid(u.nst, u.B);
-    spread_st_kernel<W, C><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win), s>>>(
-        u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.M, u.S, u.ns2, u.win, u.nst);
+static void spread_w(const NufftDev &u, InCols f, C *Wst, hipStream_t s) {
+    if (u.nst == 0 || u.B <= 0) return;
+    if (u.B >= 2) {
+        constexpr int G = kSpreadGroup;
+        dim3 grid(u.nst, (u.B + G - 1) / G);
+        spread_st_kernel<W, G, C><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win, G), s>>>(
+            u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
+    } else {
+        spread_st_kernel<W, 1, C><<<dim3(u.nst, 1), 64 * kSpreadWaves, spread_lds_bytes(u.win, 1), s>>>(
+            u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
+    }
 }
 
 #define PFSHT_W_SWITCH(w, FN, ...)                 \
@@ -615,12 +775,12 @@ static void spread_w(const NufftDev &u, const double2 *f, C *Wst, hipStream_t s)
     }
 
 template <class C>
-void launch_interp(const NufftDev &u, const C *U, double2 *f, hipStream_t s) {
+void launch_interp(const NufftDev &u, const C *U, OutCols f, hipStream_t s) {
     PFSHT_W_SWITCH(u.w, interp_w, u, U, f, s);
 }
 
 template <class C>
-void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s) {
+void launch_spread(const NufftDev &u, InCols f, C *U, hipStream_t s) {
     C *Wst = static_cast<C *>(u.Wst);
     PFSHT_W_SWITCH(u.w, spread_w, u, f, Wst, s);
     dim3 grid((u.N2 + 255) / 256, u.nR, u.B);
@@ -635,8 +795,8 @@ void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s) {
                                             hipStream_t);                                                       \
     template void launch_ut_to_rows<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
     template void launch_rows_to_ut<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
-    template void launch_interp<C>(const NufftDev &, const C *, double2 *, hipStream_t);                        \
-    template void launch_spread<C>(const NufftDev &, const double2 *, C *, hipStream_t);
+    template void launch_interp<C>(const NufftDev &, const C *, OutCols, hipStream_t);                          \
+    template void launch_spread<C>(const NufftDev &, InCols, C *, hipStream_t);
 PFSHT_INSTANTIATE(double2)
 PFSHT_INSTANTIATE(float2)
 #undef PFSHT_INSTANTIATE

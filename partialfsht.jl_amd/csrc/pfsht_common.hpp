@@ -28,6 +28,57 @@ __device__ __forceinline__ void stc(float2 *p, double2 v) { *p = make_float2(flo
 // Status propagated through the C ABI (values of the PFSHT_E_* codes in include/pfsht.h).
 enum Status { kOk = 0, kEDim = 1, kEArg = 2, kEGeom = 3, kENomem = 4, kEHip = 5, kEUnsupported = 6 };
 
+// The transform columns of an apply's input or output in the caller's memory: column b (b < nb) at
+// base + b * stride elements, or at tab[b] when tab is set (a device array of column pointers: an
+// apply that packs the columns of several requests reads and writes them in place).
+// Columns live in device (global) memory, and col() returns pointers of that address space, so
+// that loads and stores through them are global instructions: a generic (flat) access also counts
+// against the LDS wait counter, and a kernel's waits for its LDS operations would then wait for its
+// global prefetches as well.
+#define PFSHT_GLOBAL __attribute__((address_space(1)))
+typedef const PFSHT_GLOBAL double2 *gcptr;
+typedef PFSHT_GLOBAL double2 *gptr;
+// complex loads and stores through global pointers (the vector type's copy operations take generic
+// references only)
+__device__ __forceinline__ double2 gload(gcptr p) {
+    const PFSHT_GLOBAL double *q = (const PFSHT_GLOBAL double *)p;
+    return make_double2(q[0], q[1]);
+}
+__device__ __forceinline__ void gstore(gptr p, double2 v) {
+    PFSHT_GLOBAL double *q = (PFSHT_GLOBAL double *)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+struct InCols {
+    const double2 *base = nullptr;
+    int64_t stride = 0;
+    const double2 *const *tab = nullptr;
+    __device__ __forceinline__ gcptr col(int b) const {
+        return tab ? (gcptr)tab[b] : (gcptr)(base + int64_t(b) * stride);
+    }
+};
+struct OutCols {
+    double2 *base = nullptr;
+    int64_t stride = 0;
+    double2 *const *tab = nullptr;
+    __device__ __forceinline__ gptr col(int b) const {
+        return tab ? (gptr)tab[b] : (gptr)(base + int64_t(b) * stride);
+    }
+};
+inline InCols in_cols(const double2 *base, int64_t stride) { InCols c; c.base = base; c.stride = stride; return c; }
+inline OutCols out_cols(double2 *base, int64_t stride) { OutCols c; c.base = base; c.stride = stride; return c; }
+// a window of columns: columns b0 .. of c
+inline InCols cols_from(InCols c, int b0) {
+    if (c.tab) c.tab += b0;
+    else c.base += int64_t(b0) * c.stride;
+    return c;
+}
+inline OutCols cols_from(OutCols c, int b0) {
+    if (c.tab) c.tab += b0;
+    else c.base += int64_t(b0) * c.stride;
+    return c;
+}
+
 #define PFSHT_HIP_TRY(expr)                                                                       \
     do {                                                                                          \
         hipError_t _e = (expr);                                                                   \
@@ -57,6 +108,20 @@ struct SegItem {
 constexpr int kSegMax = 8;     // segments per order (forward workgroup: one wavefront each)
 constexpr int kSegBlocks = 3;  // default target of 64-degree blocks per segment (2..8 measured: 3 best)
 constexpr int kAdjWaves = 4;   // adjoint workgroup: 4 wavefronts, one 16 x 64 LDS tile each
+
+// The MFMA adjoint (two or more transforms on plans whose orders have at most 16 ring tiles): one
+// workgroup per (order, degree segment) with the order's 64-ring tiles dealt to its wavefronts, tile
+// w + waves k for k < tg; the tiles' ring sums are added in LDS in tile order, so no partials leave
+// the workgroup.
+struct OrderSeg {
+    int32_t m;      // order
+    int32_t nseg;   // degree segments of the order
+    int32_t seg;    // this workgroup's segment
+    int32_t ntile;  // 64-ring tiles of the order (0: an empty slot)
+    int32_t j0;     // first ring of tile 0
+    int32_t nrl;    // rings of the last tile
+    int64_t ck;     // checkpoint row of tile 0 at segment 1; tile t, segment s >= 1: ck + t (nseg - 1) + s - 1
+};
 
 // First degree of segment s of order m (s = nseg gives a degree past L).
 __host__ __device__ inline int seg_begin(int m, int L, int nseg, int s) {
@@ -92,6 +157,12 @@ struct RingDev {
     double4 *partials = nullptr;  // adjoint partial sums: (re+, im+, re-, im-) per (tile group, l)
     int64_t npartials = 0;
     int adj_tg = 1;             // 64-ring tiles per adjoint wavefront (a tile group); ntiles counts groups
+    OrderSeg *mitems = nullptr;  // MFMA adjoint workgroups, XCD-interleaved
+    int nmitems = 0;
+    int mfma_waves = 0;         // wavefronts per MFMA adjoint workgroup (0: no MFMA adjoint)
+    int mfma_tg = 1;            // tiles per wavefront
+    int part_b = 1;             // transform stride of the partials (1 when only single-transform
+                                // adjoints take the partials path)
 };
 
 // Spin-weighted ring engine (src/Spin.jl:170-545), used with a RingDev that supplies n, L, B, mmax,
@@ -111,9 +182,9 @@ struct SpinDev {
     double2 *partials = nullptr;  // adjoint per-(tile, l, b) ring sums, l >= max(|m|, |s|)
 };
 void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s);
-void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, double2 *Gd, hipStream_t s);
+void launch_spin_fwd(const RingDev &r, const SpinDev &sd, InCols alm, int nb, double2 *Gd, hipStream_t s);
 // writes every slot of alm (zero outside the retained modes)
-void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, double2 *alm, hipStream_t s);
+void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, OutCols alm, int nb, hipStream_t s);
 
 // Degrees of one adjoint tile's partial rows for order m: L - m + 1 rounded up to whole 16-degree
 // sub-tiles, so that a sub-tile's stores never need a bounds test.
@@ -124,11 +195,16 @@ void launch_ring_checkpoints(const RingDev &r, hipStream_t s);
 
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
 // Gd is the internal ring buffer [b][m + mmax][j].
-void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s);
-void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s);
-// the two launches of launch_ring_adj: per-tile partial sums, then their ordered reduction into alm
-void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s);
-void launch_ring_adj_finalize(const RingDev &r, double2 *alm, hipStream_t s);
+// nb transforms (nb <= r.B) in columns alm.col(0 .. nb - 1); Gd holds B transforms, the first nb used.
+void launch_ring_fwd(const RingDev &r, InCols alm, int nb, double2 *Gd, hipStream_t s);
+void launch_ring_adj(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s);
+// The adjoint as timed stages: the ring pass (stage 0), then, on the partials path only, the ordered
+// reduction of the per-tile partials into alm (launch_ring_adj_finish; nothing on the MFMA path,
+// which reduces in LDS and writes alm itself).
+void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s);
+void launch_ring_adj_finish(const RingDev &r, OutCols alm, int nb, hipStream_t s);
+// true when the adjoint of nb transforms runs on the MFMA path (no partials)
+bool ring_adj_uses_mfma(const RingDev &r, int nb);
 void launch_gd_to_ref(const RingDev &r, const double2 *Gd, double2 *G, hipStream_t s);
 void launch_ref_to_gd(const RingDev &r, const double2 *G, double2 *Gd, hipStream_t s);
 
@@ -145,9 +221,9 @@ void launch_spec_fold(const GridDims &g, const C *Ds, const double *wth, double2
 
 // Direct leaf: phase sums over the ring-engine output of theta-sorted points.
 void launch_direct_synth(const double2 *Gd, const double *phis, const int32_t *mlim, const int32_t *perm,
-                         double2 *f, int64_t M, int mmax, int B, hipStream_t s);
-void launch_direct_adjoint(const double2 *f, const double *phis, const int32_t *mlim, const int32_t *perm,
-                           double2 *Gd, int64_t M, int mmax, int B, hipStream_t s);
+                         OutCols f, int64_t M, int mmax, int nb, hipStream_t s);
+void launch_direct_adjoint(InCols f, const double *phis, const int32_t *mlim, const int32_t *perm,
+                           double2 *Gd, int64_t M, int mmax, int nb, hipStream_t s);
 
 // ES NUFFT on the stretched torus.
 struct NufftDev {
@@ -177,14 +253,16 @@ inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(
 // LDS of one spreading workgroup: the window plus, per wavefront, the staging of one 16-point chunk
 // (two 16-row kernel tiles with rows kRowPad doubles apart, and 16 complex f values), in doubles.
 constexpr int kRowPad = 17;
-constexpr int kSpreadStage = 2 * 16 * kRowPad + 2 * 16;
+// staging of one wavefront for G transforms: two row tiles and G x 16 complex f values, in doubles
+constexpr int spread_stage_doubles(int G) { return 2 * 16 * kRowPad + 2 * 16 * G; }
+constexpr int kSpreadGroup = 2;  // transforms per spreading workgroup when there are two or more
 constexpr int kSpreadWaves = 8;  // wavefronts per spreading workgroup: two of the 16 bins each
 // Wavefront w accumulates bins w + kSpreadWaves k, k < 16 / kSpreadWaves: the 16 bins of a super-tile
 // are covered exactly once only when the wave count divides 16 (a bin index past 15 would read the
 // point ranges of another super-tile, or past the end of bin_ofs).
 static_assert(16 % kSpreadWaves == 0 && kSpreadWaves >= 1, "kSpreadWaves must divide the 16 bins of a super-tile");
-inline size_t spread_lds_bytes(int win) {
-    return nufft_window_lds_bytes(win) + sizeof(double) * kSpreadWaves * kSpreadStage;
+inline size_t spread_lds_bytes(int win, int G) {
+    return nufft_window_lds_bytes(win) + sizeof(double) * kSpreadWaves * spread_stage_doubles(G);
 }
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
@@ -202,10 +280,11 @@ template <class C>
 void launch_rows_to_ut(const NufftDev &u, int mmax, const C *U, C *Ut, hipStream_t s);
 // out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
 void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
+// The point kernels over u.B transforms (an apply passes a copy with B = nb).
 template <class C>
-void launch_interp(const NufftDev &u, const C *U, double2 *f, hipStream_t s);
+void launch_interp(const NufftDev &u, const C *U, OutCols f, hipStream_t s);
 template <class C>
-void launch_spread(const NufftDev &u, const double2 *f, C *U, hipStream_t s);
+void launch_spread(const NufftDev &u, InCols f, C *U, hipStream_t s);
 
 // Iterative callers on a plan (analysis_kernels.hip). launch_cdot writes sum conj(a) b to *out
 // through kDotBlocks partial sums in part, in a fixed order.
@@ -217,10 +296,10 @@ void launch_zxpby(double2 *p, const double2 *r, double2 beta, int64_t n, hipStre
 void launch_add_prior(double2 *Ap, const double2 *p, const double *reg, int L, int B, hipStream_t s);
 void launch_spectrum(const double2 *alm, int L, double *C, hipStream_t s);
 // multi-box plans: the points of one box in and out of the caller's field
-void launch_scatter_field(double2 *f, const double2 *fg, const int32_t *idx, int64_t Mg, int64_t M, int B,
-                          hipStream_t s);
-void launch_gather_field(double2 *fg, const double2 *f, const int32_t *idx, int64_t Mg, int64_t M, int B,
-                         hipStream_t s);
+void launch_scatter_field(OutCols f, const double2 *fg, const int32_t *idx, int64_t Mg, int nb, hipStream_t s);
+void launch_gather_field(double2 *fg, InCols f, const int32_t *idx, int64_t Mg, int nb, hipStream_t s);
+// alm.col(b)[i] += acc[b * n + i] for b < nb
+void launch_cols_add(OutCols alm, const double2 *acc, int64_t n, int nb, hipStream_t s);
 
 // plan construction on the device
 void launch_coef_tables(int L, int mmax, const int64_t *abofs, double2 *ab, double2 *gc, hipStream_t s);

@@ -135,94 +135,6 @@ void launch_ring_checkpoints(const RingDev &r, hipStream_t s) {
 // ------------------------------------------------------------------------------------------------
 // Forward
 // ------------------------------------------------------------------------------------------------
-// Degrees l .. le of order m feeding NB coefficient sets (transforms b0 .. b0 + NB - 1): the
-// recurrence is computed once per ring and reused for every transform. NEG: m > 0, so the order
-// -m column is accumulated too. The accumulators share the exponent E of (q1, q2).
-template <int NB, bool NEG>
-__device__ __forceinline__ void fwd_sweep(const double2 *__restrict__ abm, const double2 *__restrict__ cp0,
-                                          const double2 *__restrict__ cm0, int64_t LB, double x, double q1,
-                                          double q2, int &E, int l, int le, int m, double (&ar)[NB],
-                                          double (&ai)[NB], double (&br)[NB], double (&bi)[NB], int &sink) {
-    // transform t's order-m column is cp0 + t LB (row l - m holds degree l), order -m likewise
-    const int lane = threadIdx.x & 63;
-    // Prefetch lanes: 8 lines of 128 B cover a 64-degree run of one 16-byte stream. Stream 0 is
-    // the (a, b) row, streams 1..NB the +m columns and NB+1..2NB the -m columns (NEG only).
-    constexpr int NS = 1 + 2 * NB;
-    constexpr int NCH = (8 * NS + 63) / 64;
-    int voff[NCH];
-    bool isab[NCH], used[NCH];
-    int lstart[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        const int k = 64 * c + lane, st = k >> 3, line = k & 7;
-        lstart[c] = 8 * line;  // degree offset of the line within the run
-        isab[c] = st == 0;
-        used[c] = st < (NEG ? NS : 1 + NB);
-        int64_t o = 128 * line;
-        if (st >= 1 && st <= NB) o += int64_t(st - 1) * LB * 16;
-        if (st > NB) o += int64_t(st - 1 - NB) * LB * 16 + (reinterpret_cast<const char *>(cm0) - reinterpret_cast<const char *>(cp0));
-        voff[c] = int(o);
-    }
-    int pending = 0;  // prefetched words of the previous block, folded into `sink` one block later
-    // the first two 64-degree runs now, then the run 128 degrees ahead at every block
-    for (int pb = 0; pb < 2; ++pb) {
-        const int lp = l + 64 * pb;
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-            const bool on = used[c] && lp + lstart[c] <= le;
-            pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
-                                voff[c], on);
-        }
-    }
-    while (l <= le) {
-        const int lend = min(le, block_end(m, l));
-        {
-            sink ^= pending;  // the previous block's prefetch has long completed
-            pending = 0;
-            const int lp = l + kPrefetchAhead;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                const bool on = used[c] && lp + lstart[c] <= le;
-                pending ^= l2_touch(isab[c] ? reinterpret_cast<const char *>(abm + lp) : reinterpret_cast<const char *>(cp0 + lp),
-                                    voff[c], on);
-            }
-        }
-#pragma unroll 4
-        for (; l <= lend; ++l) {
-            const double q = rec_step(abm[l], x, q1, q2);
-            q2 = q1;
-            q1 = q;
-#pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                const double2 a = cp0[t * LB + l];
-                ar[t] = fma(a.x, q, ar[t]);
-                ai[t] = fma(a.y, q, ai[t]);
-                if (NEG) {
-                    const double2 d = cm0[t * LB + l];
-                    br[t] = fma(d.x, q, br[t]);
-                    bi[t] = fma(d.y, q, bi[t]);
-                }
-            }
-        }
-        if (l <= le) {  // renormalise at the block end (not after the segment's last degree)
-            const int s = renorm_shift(q1, q2);
-            q1 = ldexp(q1, -s);
-            q2 = ldexp(q2, -s);
-#pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                ar[t] = ldexp(ar[t], -s);
-                ai[t] = ldexp(ai[t], -s);
-                if (NEG) {
-                    br[t] = ldexp(br[t], -s);
-                    bi[t] = ldexp(bi[t], -s);
-                }
-            }
-            E += s;
-        }
-    }
-    sink ^= pending;
-}
-
 // Single-transform sweep in the rescaled form. Degrees run in groups of 16 aligned at m (group 0
 // of segment 0 starts at m + 2). Within a group r_l = q_l / C_l with C_l the product of a_k over
 // the group's degrees up to l, so that
@@ -281,8 +193,8 @@ struct DppGroup<16, GUARD, NEG> {
 // Degrees l .. le (l = m + 2, or a 64-degree block start) from (q1, q2) = (q_{l-1}, q_{l-2}) with
 // exponent E; aprev = a_{l-1} (sqrt(2m + 3) for l = m + 2). gcm[l] = (gamma_l, C_l).
 template <bool NEG>
-__device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, const double2 *__restrict__ cp0,
-                                              const double2 *__restrict__ cm0, double x, double q1, double q2,
+__device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, gcptr __restrict__ cp0,
+                                              gcptr __restrict__ cm0, double x, double q1, double q2,
                                               double aprev, int &E, int l, int le, int m, int L,
                                               double (&acc)[4], int &sink) {
     const int lane = threadIdx.x & 63, k = lane & 15;
@@ -295,18 +207,18 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, c
     auto group_load = [&](int gs, double2 &gcv, double2 &ap, double2 &am) {
         const int lc = min(max(gs + k, m + 2), L);
         gcv = gcm[lc];
-        ap = cp0[lc];
-        if (NEG) am = cm0[lc];
+        ap = gload(cp0 + lc);
+        if (NEG) am = gload(cm0 + lc);
     };
     // L2 prefetch of the run two 64-degree blocks ahead, at every block start: lane i touches line
     // i % 8 of stream (i / 8) % 3 (gamma/C, a_lm of +m, of -m); the word read is folded into `sink`
     // one block later, so its wait never stalls the sweep
     const int pst = (lane >> 3) % (NEG ? 3 : 2), pline = lane & 7;
-    const char *pbase = pst == 0 ? reinterpret_cast<const char *>(gcm)
-                                 : reinterpret_cast<const char *>(pst == 1 ? cp0 : cm0);
+    const PFSHT_GLOBAL char *pbase = pst == 0 ? (const PFSHT_GLOBAL char *)gcm
+                                              : (const PFSHT_GLOBAL char *)(pst == 1 ? cp0 : cm0);
     auto touch = [&](int lp) {
         const int lt = min(max(lp + 8 * pline, m + 2), L);
-        return *reinterpret_cast<const int *>(pbase + int64_t(lt) * 16);
+        return *(const PFSHT_GLOBAL int *)(pbase + int64_t(lt) * 16);
     };
     int pa = 0, pb = touch(l + 64);
     double2 gcv, ap, am = make_double2(0.0, 0.0);
@@ -349,16 +261,17 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, c
     sink ^= pa ^ pb;
 }
 
-// One workgroup per 64-ring tile of one order; wavefront s sweeps degree segment s. Segment 0
-// starts from the seed, the others from their checkpoints; the segment sums, scaled to plain
-// doubles, are added in segment order through LDS and wavefront 0 stores the tile's ring values.
-template <int NB>
-__device__ __forceinline__ void ring_fwd_body(const SegItem *__restrict__ items, const double *__restrict__ costh,
-                const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
-                const double2 *__restrict__ ckq,
-                const int32_t *__restrict__ ckE, const double2 *__restrict__ alm, double2 *__restrict__ Gd, int L,
-                int n, int mmax, int b0, int never) {
+// One workgroup per 64-ring tile of one order; wavefront s sweeps degree segment s of one
+// transform (column b0 of alm). Segment 0 starts from the seed, the others from their checkpoints;
+// the segment sums, scaled to plain doubles, are added in segment order through LDS and wavefront 0
+// stores the tile's ring values. The sweep holds the DPP operands of two groups; six wavefronts per
+// SIMD fit its registers.
+__global__ void __launch_bounds__(64 * kSegMax) __attribute__((amdgpu_waves_per_eu(6)))
+ring_fwd1_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                 const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
+                 const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, InCols alm, int b0,
+                 double2 *__restrict__ Gd, int L, int n, int mmax, int never) {
     __shared__ double red[kSegMax - 1][4][64];
     const SegItem it = items[blockIdx.x];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
@@ -370,14 +283,13 @@ __device__ __forceinline__ void ring_fwd_body(const SegItem *__restrict__ items,
     const bool active = lane < nr;
     const int j = it.j0 + (active ? lane : 0);
 
-    const int64_t LB = int64_t(L + 1) * (2 * L + 1);
-    // column of order m (row l - m holds degree l) and of order -m of transform b0; transform
-    // b0 + t is LB further on
-    const double2 *__restrict__ cp0 = alm + b0 * LB + int64_t(2 * m) * (L + 1) - m;
-    const double2 *__restrict__ cm0 = alm + b0 * LB + int64_t(2 * m - 1) * (L + 1) - m;
+    // column of order m (row l - m holds degree l) and of order -m
+    const gcptr acol = alm.col(b0);
+    const gcptr cp0 = acol + (int64_t(2 * m) * (L + 1) - m);
+    const gcptr cm0 = acol + (int64_t(2 * m - 1) * (L + 1) - m);
     const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);  // abm[l] for l = m+2..L
 
-    double ar[NB], ai[NB], br[NB], bi[NB];
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
     int sink = 0;
     if (sg < nseg) {
         const double x = costh[j];
@@ -387,26 +299,22 @@ __device__ __forceinline__ void ring_fwd_body(const SegItem *__restrict__ items,
             seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], q2, E);
             if (!active) q2 = 0.0;
             q1 = m < L ? first_step(m, x, q2) : 0.0;
-#pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                const double2 a0 = cp0[t * LB + m];
-                ar[t] = a0.x * q2;
-                ai[t] = a0.y * q2;
-                br[t] = bi[t] = 0.0;
+            const double2 a0 = gload(cp0 + m);
+            ar = a0.x * q2;
+            ai = a0.y * q2;
+            if (m > 0) {
+                const double2 c0 = gload(cm0 + m);
+                br = c0.x * q2;
+                bi = c0.y * q2;
+            }
+            if (m < L) {
+                const double2 a1 = gload(cp0 + (m + 1));
+                ar = fma(a1.x, q1, ar);
+                ai = fma(a1.y, q1, ai);
                 if (m > 0) {
-                    const double2 c0 = cm0[t * LB + m];
-                    br[t] = c0.x * q2;
-                    bi[t] = c0.y * q2;
-                }
-                if (m < L) {
-                    const double2 a1 = cp0[t * LB + m + 1];
-                    ar[t] = fma(a1.x, q1, ar[t]);
-                    ai[t] = fma(a1.y, q1, ai[t]);
-                    if (m > 0) {
-                        const double2 c1 = cm0[t * LB + m + 1];
-                        br[t] = fma(c1.x, q1, br[t]);
-                        bi[t] = fma(c1.y, q1, bi[t]);
-                    }
+                    const double2 c1 = gload(cm0 + (m + 1));
+                    br = fma(c1.x, q1, br);
+                    bi = fma(c1.y, q1, bi);
                 }
             }
             l = m + 2;
@@ -416,122 +324,47 @@ __device__ __forceinline__ void ring_fwd_body(const SegItem *__restrict__ items,
             q1 = c.x;
             q2 = c.y;
             E = ckE[row];
-#pragma unroll
-            for (int t = 0; t < NB; ++t) ar[t] = ai[t] = br[t] = bi[t] = 0.0;
             l = seg_begin(m, L, nseg, sg);
         }
         const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
-        if (NB == 1) {
-            if (l <= le) {
-                const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
-                const double aprev = l == m + 2 ? sqrt(double(2 * m + 3)) : abm[l - 1].x;
-                double acc[4] = {ar[0], ai[0], br[0], bi[0]};
-                if (m > 0) fwd_sweep_dpp<true>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
-                else fwd_sweep_dpp<false>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
-                ar[0] = acc[0];
-                ai[0] = acc[1];
-                br[0] = acc[2];
-                bi[0] = acc[3];
-            }
-        } else if (m > 0) {
-            fwd_sweep<NB, true>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
-        } else {
-            fwd_sweep<NB, false>(abm, cp0, cm0, LB, x, q1, q2, E, l, le, m, ar, ai, br, bi, sink);
+        if (l <= le) {
+            const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
+            const double aprev = l == m + 2 ? sqrt(double(2 * m + 3)) : abm[l - 1].x;
+            double acc[4] = {ar, ai, br, bi};
+            if (m > 0) fwd_sweep_dpp<true>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
+            else fwd_sweep_dpp<false>(gcm, cp0, cm0, x, q1, q2, aprev, E, l, le, m, L, acc, sink);
+            ar = acc[0];
+            ai = acc[1];
+            br = acc[2];
+            bi = acc[3];
         }
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-            ar[t] = ldexp(ar[t], E);
-            ai[t] = ldexp(ai[t], E);
-            br[t] = ldexp(br[t], E);
-            bi[t] = ldexp(bi[t], E);
-        }
+        ar = ldexp(ar, E);
+        ai = ldexp(ai, E);
+        br = ldexp(br, E);
+        bi = ldexp(bi, E);
     }
     if (sink == never) Gd[j] = make_double2(0.0, 0.0);  // keeps the prefetch loads; never true
     if (nseg > 1) {
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-            if (sg > 0 && sg < nseg) {
-                red[sg - 1][0][lane] = ar[t];
-                red[sg - 1][1][lane] = ai[t];
-                red[sg - 1][2][lane] = br[t];
-                red[sg - 1][3][lane] = bi[t];
-            }
-            __syncthreads();
-            if (sg == 0)
-                for (int k = 0; k < nseg - 1; ++k) {
-                    ar[t] += red[k][0][lane];
-                    ai[t] += red[k][1][lane];
-                    br[t] += red[k][2][lane];
-                    bi[t] += red[k][3][lane];
-                }
-            __syncthreads();
+        if (sg > 0 && sg < nseg) {
+            red[sg - 1][0][lane] = ar;
+            red[sg - 1][1][lane] = ai;
+            red[sg - 1][2][lane] = br;
+            red[sg - 1][3][lane] = bi;
         }
+        __syncthreads();
+        if (sg == 0)
+            for (int k = 0; k < nseg - 1; ++k) {
+                ar += red[k][0][lane];
+                ai += red[k][1][lane];
+                br += red[k][2][lane];
+                bi += red[k][3][lane];
+            }
     }
     if (sg != 0 || !active) return;
     const double sgn = (m & 1) ? -1.0 : 1.0;
     const int64_t ncol = 2 * mmax + 1;
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-        double2 *gp = Gd + (int64_t(b0 + t) * ncol + (mmax + m)) * n + j;
-        *gp = make_double2(sgn * ar[t], sgn * ai[t]);
-        if (m > 0) {
-            double2 *gm = Gd + (int64_t(b0 + t) * ncol + (mmax - m)) * n + j;
-            *gm = make_double2(br[t], bi[t]);
-        }
-    }
-}
-
-// The single-transform sweep holds the DPP operands of two groups; six wavefronts per SIMD fit
-// its registers.
-__global__ void __launch_bounds__(64 * kSegMax) __attribute__((amdgpu_waves_per_eu(6)))
-ring_fwd1_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
-                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                 const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
-                 const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, const double2 *__restrict__ alm,
-                 double2 *__restrict__ Gd, int L, int n, int mmax, int b0, int never) {
-    ring_fwd_body<1>(items, costh, log2sin, log2c, ab, gc, abofs, ckq, ckE, alm, Gd, L, n, mmax, b0, never);
-}
-
-template <int NB>
-__global__ void __launch_bounds__(64 * kSegMax)
-ring_fwd_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
-                const double *__restrict__ log2sin, const double *__restrict__ log2c,
-                const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
-                const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, const double2 *__restrict__ alm,
-                double2 *__restrict__ Gd, int L, int n, int mmax, int b0, int never) {
-    ring_fwd_body<NB>(items, costh, log2sin, log2c, ab, gc, abofs, ckq, ckE, alm, Gd, L, n, mmax, b0, never);
-}
-
-// Transform groups of 8, 4, 2 and 1 cover any ntrans exactly with at most four launches.
-template <class F>
-static void for_transform_groups(int B, F &&launch) {
-    int b0 = 0;
-    for (int nb : {8, 4, 2, 1})
-        while (B - b0 >= nb) {
-            launch(nb, b0);
-            b0 += nb;
-        }
-}
-
-void launch_ring_fwd(const RingDev &r, const double2 *alm, double2 *Gd, hipStream_t s) {
-    if (r.nfitems == 0) return;
-    for_transform_groups(r.B, [&](int nb, int b0) {
-        dim3 grid(r.nfitems);
-#define PFSHT_FWD(NB)                                                                                         \
-    ring_fwd_kernel<NB><<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, \
-                                                          r.ckq, r.ckE, alm, Gd, r.L, r.n, r.mmax, b0, kNeverSink)
-        switch (nb) {
-            case 8: PFSHT_FWD(8); break;
-            case 4: PFSHT_FWD(4); break;
-            case 2: PFSHT_FWD(2); break;
-            default:
-                ring_fwd1_kernel<<<grid, 64 * r.fwd_waves, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,
-                                                                  r.abofs, r.ckq, r.ckE, alm, Gd, r.L, r.n, r.mmax, b0,
-                                                                  kNeverSink);
-                break;
-        }
-#undef PFSHT_FWD
-    });
+    Gd[(int64_t(b0) * ncol + (mmax + m)) * n + j] = make_double2(sgn * ar, sgn * ai);
+    if (m > 0) Gd[(int64_t(b0) * ncol + (mmax - m)) * n + j] = make_double2(br, bi);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -802,26 +635,26 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
-// Writes every alm slot: the tile-ordered sum of the partials where (l, m) is retained, zero in
-// the padding and for unretained orders (the adjoint overwrites alm, src/Legendre.jl:338). One
-// thread owns row l - |m| of the column pair (2|m|, 2|m| - 1), reading each partial once.
+// Writes every alm slot of transforms 0 .. nb - 1: the tile-ordered sum of the partials where (l, m)
+// is retained, zero in the padding and for unretained orders (the adjoint overwrites alm,
+// src/Legendre.jl:338). One thread owns row l - |m| of the column pair (2|m|, 2|m| - 1), reading each
+// partial once. pb is the partials' transform stride.
 __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_t *__restrict__ ntiles,
-                                  const double4 *__restrict__ partials, double2 *__restrict__ alm, int L,
-                                  int mmax, int B) {
+                                  const double4 *__restrict__ partials, OutCols alm, int L, int mmax, int pb) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     const int am = blockIdx.y;
     const int b = blockIdx.z;
     if (row > L) return;
     const int64_t nrow = L + 1;
-    const int64_t base = int64_t(b) * nrow * (2 * L + 1);
+    const gptr a = alm.col(b);
     const int l = row + am;
     double pr = 0, pi = 0, nr_ = 0, ni = 0;
     if (l <= L && am <= mmax) {
         const int nt = ntiles[am];
         const int nl = adj_rows_padded(am, L);
-        const double4 *p = partials + pofs[am] + int64_t(l - am) * B + b;
+        const double4 *p = partials + pofs[am] + int64_t(l - am) * pb + b;
         for (int t = 0; t < nt; ++t) {
-            const double4 q = p[int64_t(t) * nl * B];
+            const double4 q = p[int64_t(t) * nl * pb];
             pr += q.x;
             pi += q.y;
             nr_ += q.z;
@@ -829,26 +662,469 @@ __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_
         }
         if (am & 1) { pr = -pr; pi = -pi; }
     }
-    alm[base + int64_t(2 * am) * nrow + row] = make_double2(pr, pi);
-    if (am > 0) alm[base + int64_t(2 * am - 1) * nrow + row] = make_double2(nr_, ni);
+    gstore(a + (int64_t(2 * am) * nrow + row), make_double2(pr, pi));
+    if (am > 0) gstore(a + (int64_t(2 * am - 1) * nrow + row), make_double2(nr_, ni));
 }
 
-void launch_ring_adj(const RingDev &r, const double2 *Gd, double2 *alm, hipStream_t s) {
-    launch_ring_adj_tiles(r, Gd, s);
-    launch_ring_adj_finalize(r, alm, s);
+// ------------------------------------------------------------------------------------------------
+// FP64 MFMA ring passes for two or more transforms. The per-order accumulation is a contraction of
+// a (degree x ring) tile of the recurrence against the coefficient columns (forward) or the ring
+// columns (adjoint), on v_mfma_f64_16x16x4f64. Column c of a 16-column block is component c % 4 of
+// transform c / 4: (re, im) of order +m, then (re, im) of order -m. The recurrence runs on VALU
+// with lanes over rings, exactly as the single-transform adjoint's phase 1, and stores r 2^E in the
+// wavefront's LDS tile (r the rescaled recurrence value, E the ring's exponent); C_l, the rescale of
+// degree l, goes on the coefficient side (forward) or multiplies the ring sum (adjoint), so every
+// product is a_lm Pbar_l^m(x_j) or Pbar_l^m(x_j) G_j as plain doubles.
+//
+// Operand layouts of v_mfma_f64_16x16x4f64 (gfx950): A (16 x 4) lane 16 k + i holds A[i][k]; B
+// (4 x 16) lane 16 k + j holds B[k][j]; D (16 x 16) register r of lane 16 k + j holds D[4 r + k][j].
+// ------------------------------------------------------------------------------------------------
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfma_d4 mfma_f64(double a, double b, mfma_d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
+// LDS tile column of ring rho (0..63): rows are kWS doubles apart, ring rho at rho + rho / 16
+__device__ __forceinline__ int wcol_of(int rho) { return rho + (rho >> 4); }
+
+// Forward: one workgroup per 64-ring tile of one order, wavefront s sweeping degree segment s (as
+// the single-transform pass), 4 NCB transforms b0 .. b0 + nt - 1. D[rb][cb] holds rings 16 rb ..
+// 16 rb + 15 against column block cb: sum over degrees of (r 2^E)(a_lm C_l). The segments' sums are
+// added in segment order through LDS and the tile is written to Gd with (-1)^m on the +m columns.
+template <int NCB>
+__global__ void __launch_bounds__(64 * kSegMax)
+ring_fwd_mfma_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
+                     const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                     const double2 *__restrict__ ab, const double2 *__restrict__ gc,
+                     const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                     const int32_t *__restrict__ ckE, InCols alm, int b0, int nt, double2 *__restrict__ Gd, int L,
+                     int n, int mmax) {
+    extern __shared__ double lds_fm[];
+    constexpr int NC = 16 * NCB, NTR = 4 * NCB;
+    constexpr int kRS = 65;  // row stride of the reduction buffer [column][ring]
+    const SegItem it = items[blockIdx.x];
+    const int m = __builtin_amdgcn_readfirstlane(it.m);
+    const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
+    const int nr = __builtin_amdgcn_readfirstlane(it.nr);
+    if (nr <= 0) return;  // an empty slot: the whole workgroup leaves
+    const int sg = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63, kq = lane >> 4, jc = lane & 15;
+    const bool active = lane < nr;
+    const int j = it.j0 + (active ? lane : 0);
+    const int wcol = wcol_of(lane);
+    double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds_fm + sg * kSub * kWS);
+
+    // the coefficient each lane feeds to column block cb: transform t, component comp
+    const PFSHT_GLOBAL double *bcol[NCB];
+    bool bok[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) {
+        const int c = 16 * cb + jc, t = c >> 2, comp = c & 3;
+        bok[cb] = t < nt && (comp < 2 || m > 0);
+        const bool neg = comp >= 2 && m > 0;  // order -m has no column when m = 0: point at +m, unused
+        const gcptr acol = alm.col(b0 + min(t, nt - 1)) + (int64_t(neg ? 2 * m - 1 : 2 * m) * (L + 1) - m);
+        bcol[cb] = (const PFSHT_GLOBAL double *)acol + (comp & 1);  // degree l at bcol[2 l]
+    }
+    mfma_d4 D[4][NCB];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) D[rb][cb] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+
+    if (sg < nseg) {
+        const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
+        const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
+        const bool has_tab = m + 2 <= L;
+        const int ls = seg_begin(m, L, nseg, sg);
+        const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
+        const double x = costh[j];
+        double r1, r2, w0 = 0.0, w1 = 0.0;
+        int E, kfirst = 0;
+        if (sg == 0) {
+            double qs;
+            seed(m == 0 ? log2c[0] : double(m) * log2sin[j] + log2c[m], qs, E);
+            if (!active) qs = 0.0;
+            w0 = ldexp(qs, E);
+            kfirst = 1;
+            r1 = qs;
+            r2 = 0.0;
+            if (m < L) {
+                const double q = first_step(m, x, qs);
+                w1 = ldexp(q, E);
+                kfirst = 2;
+                r1 = q;
+                r2 = qs * sqrt(double(2 * m + 3));
+            }
+        } else {
+            const int64_t row = (it.ck + sg - 1) * 64 + lane;
+            const double2 c = ckq[row];
+            E = ckE[row];
+            r1 = c.x;
+            r2 = c.y * abm[ls - 1].x;
+        }
+#pragma unroll
+        for (int k = 0; k < kSub; ++k) W[k][wcol] = 0.0;
+        // The next sub-tile's operands are loaded raw, unconditionally and at indices clamped into the
+        // order's table (entry 0 of the table when the order has none), and are masked and scaled
+        // only when used: a load whose value feeds a select at issue is sunk into a branch and waited
+        // for on the spot, while raw loads are counted and waited for one sub-tile later.
+        const double2 *__restrict__ gct = has_tab ? gcm : gc;
+        auto gidx = [&](int l) { return has_tab ? min(max(l, m + 2), L) : 0; };
+        double braw[4][NCB], craw[4];
+        double2 graw;
+        auto load_next = [&](int l0) {
+            graw = gct[gidx(l0 + jc)];  // (gamma, C) of degree l0 + (lane % 16)
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int dc = min(l0 + 4 * st + kq, L);
+                craw[st] = gct[gidx(dc)].y;
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) braw[st][cb] = bcol[cb][2 * dc];
+            }
+        };
+        load_next(ls);
+        for (int l0 = ls; l0 <= le; l0 += kSub) {
+            const int tl = min(kSub, le - l0 + 1);
+            // B operands: step s, block cb holds degree l0 + 4 s + kq times C_l, zero past le
+            double bv[4][NCB];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int deg = l0 + 4 * st + kq;
+                const double C = (deg < m + 2 || !has_tab) ? 1.0 : craw[st];
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) bv[st][cb] = (deg <= le && bok[cb]) ? braw[st][cb] * C : 0.0;
+            }
+            const double2 gcv = has_tab ? graw : make_double2(0.0, 1.0);
+            load_next(l0 + kSub);
+            const double ng = -gcv.x, Cc = gcv.y;
+            const int last = tl - 1;
+            const double Ce = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(Cc), last),
+                                               __builtin_amdgcn_readlane(__double2loint(Cc), last));
+            if (kfirst > 0) {
+                W[0][wcol] = w0;
+                if (kfirst == 2) W[1][wcol] = w1;
+            }
+            if (tl == kSub && kfirst == 0) AdjPhase1<0, false, true>::run(x, r1, r2, ng, E, W, wcol, 0, kSub);
+            else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, true>::run(x, r1, r2, ng, E, W, wcol, 2, kSub);
+            else AdjPhase1<0, true, true>::run(x, r1, r2, ng, E, W, wcol, kfirst, tl);
+            r1 *= Ce;
+            r2 *= Ce;
+            if (((l0 + kSub - m) & 63) == 0) {  // renormalise at every 64-degree block end
+                const int sh = renorm_shift(r1, r2);
+                r1 = ldexp(r1, -sh);
+                r2 = ldexp(r2, -sh);
+                E += sh;
+            }
+            wave_lds_order();
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                double a[4];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) a[rb] = W[4 * st + kq][17 * rb + jc];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    if (16 * rb >= nr) break;  // wave-uniform: ring blocks past the tile's rings are empty
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb) D[rb][cb] = mfma_f64(a[rb], bv[st][cb], D[rb][cb]);
+                }
+            }
+            wave_lds_order();
+            kfirst = 0;
+        }
+    }
+    // segment sums in segment order; the W tiles are free now and the buffer overlays them
+    __syncthreads();
+    double *red = lds_fm;  // [NC][kRS]
+    for (int turn = 0; turn < nseg; ++turn) {
+        if (sg == turn) {
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        double &cell = red[(16 * cb + jc) * kRS + 16 * rb + 4 * r + kq];
+                        cell = turn == 0 ? D[rb][cb][r] : cell + D[rb][cb][r];
+                    }
+        }
+        __syncthreads();
+    }
+    const int64_t ncol = 2 * mmax + 1;
+    const double sgn = (m & 1) ? -1.0 : 1.0;
+    for (int q = threadIdx.x; q < NTR * 2 * 64; q += blockDim.x) {
+        const int rho = q & 63, pm = (q >> 6) & 1, t = q >> 7;
+        if (t >= nt || rho >= nr || (pm && m == 0)) continue;
+        const double re = red[(4 * t + 2 * pm) * kRS + rho], im = red[(4 * t + 2 * pm + 1) * kRS + rho];
+        Gd[(int64_t(b0 + t) * ncol + (pm ? mmax - m : mmax + m)) * n + it.j0 + rho] =
+            pm ? make_double2(re, im) : make_double2(sgn * re, sgn * im);
+    }
+}
+
+// Adjoint: one workgroup per (order, degree segment); wavefront w takes tiles w + nw k (k < TG) of the
+// order, with the tile's ring columns G (the B operands, ring 4 s + k of quad s against column
+// block cb) in registers for the whole sweep. Per 16-degree sub-tile each tile's recurrence fills
+// the wavefront's LDS tile and 16 ring quads are contracted into D (16 degrees x NC columns); the
+// wavefronts' D are then added in LDS in wavefront order (tiles w, w + nw, ... in tile order within
+// a wavefront), multiplied by C_l, and written to alm: no partials, no float atomics, bit-identical
+// run to run.
+template <int NCB, int TG>
+__global__ void __launch_bounds__(64 * 8)
+ring_adj_mfma_kernel(const OrderSeg *__restrict__ items, const double *__restrict__ costh,
+                     const double *__restrict__ log2sin, const double *__restrict__ log2c,
+                     const double2 *__restrict__ ab, const double2 *__restrict__ gc,
+                     const int64_t *__restrict__ abofs, const double2 *__restrict__ ckq,
+                     const int32_t *__restrict__ ckE, const double2 *__restrict__ Gd, OutCols alm, int b0, int nt,
+                     int L, int n, int mmax) {
+    extern __shared__ double lds_am[];
+    constexpr int NC = 16 * NCB, NTR = 4 * NCB, kRS = NC + 1;
+    const OrderSeg it = items[blockIdx.x];
+    const int m = __builtin_amdgcn_readfirstlane(it.m);
+    const int ntile = __builtin_amdgcn_readfirstlane(it.ntile);
+    if (ntile <= 0) return;  // an empty slot: the whole workgroup leaves
+    const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
+    const int sg = __builtin_amdgcn_readfirstlane(it.seg);
+    const int nw = int(blockDim.x) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63, kq = lane >> 4, jc = lane & 15;
+    const int wcol = wcol_of(lane);
+    double(*W)[kWS] = reinterpret_cast<double(*)[kWS]>(lds_am + wave * kSub * kWS);
+    double *red = lds_am + nw * kSub * kWS;  // [2][nw][16][kRS]
+    const int ls = seg_begin(m, L, nseg, sg);
+    const int le = min(L, seg_begin(m, L, nseg, sg + 1) - 1);
+    const int64_t ncol = 2 * mmax + 1;
+    const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
+    const double2 *__restrict__ abm = ab + abofs[m] - (m + 2);
+    const bool has_tab = m + 2 <= L;
+
+    double x[TG], r1[TG], r2[TG], w0[TG], w1[TG];
+    int E[TG], nrt[TG];
+    bool on[TG];
+    double gB[TG][16][NCB];
+    int kfirst = 0;
+#pragma unroll
+    for (int k = 0; k < TG; ++k) {
+        const int tile = wave + nw * k;
+        on[k] = tile < ntile;
+        const int nrk = on[k] ? (tile == ntile - 1 ? it.nrl : 64) : 0;
+        nrt[k] = nrk;
+        const int jt = it.j0 + 64 * tile;
+        const bool act = lane < nrk;
+        const int jj = act ? jt + lane : it.j0;
+        x[k] = costh[jj];
+        w0[k] = w1[k] = 0.0;
+        E[k] = 0;
+        r1[k] = r2[k] = 0.0;
+        if (on[k]) {
+            if (sg == 0) {
+                double qs;
+                seed(m == 0 ? log2c[0] : double(m) * log2sin[jj] + log2c[m], qs, E[k]);
+                if (!act) qs = 0.0;
+                w0[k] = ldexp(qs, E[k]);
+                kfirst = 1;
+                r1[k] = qs;
+                if (m < L) {
+                    const double q = first_step(m, x[k], qs);
+                    w1[k] = ldexp(q, E[k]);
+                    kfirst = 2;
+                    r1[k] = q;
+                    r2[k] = qs * sqrt(double(2 * m + 3));
+                }
+            } else {
+                const int64_t row = (it.ck + int64_t(tile) * (nseg - 1) + sg - 1) * 64 + lane;
+                const double2 c = ckq[row];
+                E[k] = ckE[row];
+                r1[k] = act ? c.x : 0.0;
+                r2[k] = act ? c.y * abm[ls - 1].x : 0.0;
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < 16; ++st) {
+            const int rho = 4 * st + kq;
+            const bool rok = rho < nrk;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int c = 16 * cb + jc, t = c >> 2, comp = c & 3;
+                const bool ok = rok && t < nt && (comp < 2 || m > 0);
+                double v = 0.0;
+                if (ok) {
+                    const double2 g = Gd[(int64_t(b0 + t) * ncol + (comp < 2 ? mmax + m : mmax - m)) * n + jt + rho];
+                    v = (comp & 1) ? g.y : g.x;
+                }
+                gB[k][st][cb] = v;
+            }
+        }
+    }
+    if (sg == 0) kfirst = m < L ? 2 : 1;  // wave-uniform (a wavefront without tiles joins in)
+#pragma unroll
+    for (int k = 0; k < kSub; ++k) W[k][wcol] = 0.0;
+    // (gamma, C) of degree l0 + (lane % 16), loaded raw one sub-tile ahead, unconditionally at a
+    // clamped index (entry 0 of the table when the order has none), and selected when used
+    const double2 *__restrict__ gct = has_tab ? gcm : gc;
+    auto coef_raw = [&](int l0) { return gct[has_tab ? min(max(l0 + jc, m + 2), L) : 0]; };
+    double2 graw = coef_raw(ls);
+    int buf = 0;
+    const int nwt = min(nw, ntile);  // wavefronts holding tiles
+    for (int l0 = ls; l0 <= le; l0 += kSub) {
+        const int tl = min(kSub, le - l0 + 1);
+        const double2 gcv = has_tab ? graw : make_double2(0.0, 1.0);
+        graw = coef_raw(l0 + kSub);
+        const double ng = -gcv.x, Cc = gcv.y;
+        const int last = tl - 1;
+        const double Ce = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(Cc), last),
+                                           __builtin_amdgcn_readlane(__double2loint(Cc), last));
+        const bool bend = ((l0 + kSub - m) & 63) == 0;
+        mfma_d4 D[NCB];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) D[cb] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < TG; ++k) {
+            if (!on[k]) break;  // wave-uniform; tiles of a wavefront are in increasing order
+            if (kfirst > 0) {
+                W[0][wcol] = w0[k];
+                if (kfirst == 2) W[1][wcol] = w1[k];
+            }
+            if (tl == kSub && kfirst == 0) AdjPhase1<0, false, true>::run(x[k], r1[k], r2[k], ng, E[k], W, wcol, 0, kSub);
+            else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, true>::run(x[k], r1[k], r2[k], ng, E[k], W, wcol, 2, kSub);
+            else AdjPhase1<0, true, true>::run(x[k], r1[k], r2[k], ng, E[k], W, wcol, kfirst, tl);
+            r1[k] *= Ce;
+            r2[k] *= Ce;
+            if (bend) {
+                const int sh = renorm_shift(r1[k], r2[k]);
+                r1[k] = ldexp(r1[k], -sh);
+                r2[k] = ldexp(r2[k], -sh);
+                E[k] += sh;
+            }
+            wave_lds_order();
+#pragma unroll
+            for (int st = 0; st < 16; ++st) {
+                if (4 * st >= nrt[k]) break;  // wave-uniform: ring quads past the tile's rings are empty
+                const double a = W[jc][wcol_of(4 * st + kq)];
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) D[cb] = mfma_f64(a, gB[k][st][cb], D[cb]);
+            }
+            wave_lds_order();
+        }
+        double *rw = red + (buf * nw + wave) * 16 * kRS;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rw[(4 * r + kq) * kRS + 16 * cb + jc] = D[cb][r];
+        __syncthreads();
+        // the wavefronts' sums in wavefront order, times C_l, into alm: thread (d, t, pm)
+        for (int q = threadIdx.x; q < 32 * NTR; q += blockDim.x) {
+            const int d = q & 15, pm = (q >> 4) & 1, t = q >> 5;
+            const int l = l0 + d;
+            if (l <= le && t < nt && (pm == 0 || m > 0)) {
+                const double *rr = red + (buf * nw) * 16 * kRS + d * kRS + 4 * t + 2 * pm;
+                double re = 0.0, im = 0.0;
+                for (int w = 0; w < nwt; ++w) {
+                    re += rr[w * 16 * kRS];
+                    im += rr[w * 16 * kRS + 1];
+                }
+                const double cm = l < m + 2 ? 1.0 : gcm[l].y;
+                re *= cm;
+                im *= cm;
+                if (pm == 0 && (m & 1)) { re = -re; im = -im; }
+                gstore(alm.col(b0 + t) + (int64_t(pm ? 2 * m - 1 : 2 * m) * (L + 1) + (l - m)), make_double2(re, im));
+            }
+        }
+        buf ^= 1;
+        kfirst = 0;
+    }
+}
+
+// alm entries the MFMA adjoint does not write: the padding rows of every column and every row of
+// the orders without retained rings (the adjoint overwrites alm, src/Legendre.jl:338).
+__global__ void __launch_bounds__(256)
+alm_zero_unwritten_kernel(OutCols alm, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi, int L,
+                          int mmax) {
+    const int c = blockIdx.x, b = blockIdx.y;
+    const int am = (c + 1) >> 1;  // column c = 2|m| (m >= 0) or 2|m| - 1 (m < 0)
+    const bool retained = am <= mmax && jhi[am] >= jlo[am];
+    const int first = retained ? L - am + 1 : 0;  // rows first .. L are not written by the ring pass
+    const gptr col = alm.col(b) + int64_t(c) * (L + 1);
+    for (int row = first + int(threadIdx.x); row <= L; row += 256) gstore(col + row, make_double2(0.0, 0.0));
+}
+
+// Transform groups of 8, 4, 2 and 1 cover any count exactly with at most four launches.
+template <class F>
+static void for_transform_groups(int B, F &&launch) {
+    int b0 = 0;
+    for (int nb : {8, 4, 2, 1})
+        while (B - b0 >= nb) {
+            launch(nb, b0);
+            b0 += nb;
+        }
+}
+
+// MFMA launches cover nb transforms in chunks of up to 8 (two column blocks), a last chunk of up
+// to 4 on one block.
+template <class F>
+static void for_mfma_chunks(int nb, F &&launch) {
+    for (int b0 = 0; b0 < nb;) {
+        const int rem = nb - b0, ncb = rem > 4 ? 2 : 1, nt = std::min(rem, 4 * ncb);
+        launch(ncb, b0, nt);
+        b0 += nt;
+    }
+}
+
+void launch_ring_fwd(const RingDev &r, InCols alm, int nb, double2 *Gd, hipStream_t s) {
+    if (r.nfitems == 0 || nb <= 0) return;
+    dim3 grid(r.nfitems), block(64 * r.fwd_waves);
+    if (nb == 1) {
+        ring_fwd1_kernel<<<grid, block, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, r.ckq,
+                                                r.ckE, alm, 0, Gd, r.L, r.n, r.mmax, kNeverSink);
+        return;
+    }
+    const size_t lds = sizeof(double) * std::max(r.fwd_waves * kSub * kWS, 32 * 65);
+    for_mfma_chunks(nb, [&](int ncb, int b0, int nt) {
+        if (ncb == 2)
+            ring_fwd_mfma_kernel<2><<<grid, block, lds, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,
+                                                             r.abofs, r.ckq, r.ckE, alm, b0, nt, Gd, r.L, r.n, r.mmax);
+        else
+            ring_fwd_mfma_kernel<1><<<grid, block, lds, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,
+                                                             r.abofs, r.ckq, r.ckE, alm, b0, nt, Gd, r.L, r.n, r.mmax);
+    });
+}
+
+bool ring_adj_uses_mfma(const RingDev &r, int nb) { return nb >= 2 && r.mfma_waves > 0 && r.nmitems > 0; }
+
+static size_t ring_adj_mfma_lds(int nw, int ncb) {
+    return sizeof(double) * (size_t(nw) * kSub * kWS + 2 * size_t(nw) * 16 * (16 * ncb + 1));
+}
+
+void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s) {
+    if (nb <= 0) return;
+    if (ring_adj_uses_mfma(r, nb)) {
+        alm_zero_unwritten_kernel<<<dim3(2 * r.L + 1, nb), 256, 0, s>>>(alm, r.jlo, r.jhi, r.L, r.mmax);
+        dim3 grid(r.nmitems), block(64 * r.mfma_waves);
+        for_mfma_chunks(nb, [&](int ncb, int b0, int nt) {
+#define PFSHT_ADJ_MFMA(NCB, TG)                                                                              \
+    ring_adj_mfma_kernel<NCB, TG><<<grid, block, ring_adj_mfma_lds(r.mfma_waves, NCB), s>>>(                  \
+        r.mitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, r.ckq, r.ckE, Gd, alm, b0, nt, r.L, r.n, r.mmax)
+            if (ncb == 2) {
+                if (r.mfma_tg == 2) PFSHT_ADJ_MFMA(2, 2);
+                else PFSHT_ADJ_MFMA(2, 1);
+            } else {
+                if (r.mfma_tg == 2) PFSHT_ADJ_MFMA(1, 2);
+                else PFSHT_ADJ_MFMA(1, 1);
+            }
+#undef PFSHT_ADJ_MFMA
+        });
+        return;
+    }
     if (r.naitems > 0)
-        for_transform_groups(r.B, [&](int nb, int b0) {
+        for_transform_groups(nb, [&](int ng, int b0) {
             dim3 grid(r.naitems / kAdjWaves);
             const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
 #define PFSHT_ADJ(NB, TG)                                                                                    \
     ring_adj_kernel<NB, TG><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,  \
                                                               r.abofs, r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, \
-                                                              r.mmax, r.B, b0, kNeverSink)
-            // the item list was built for r.adj_tg tiles per wavefront (two when ntrans <= 2)
-            switch (nb) {
+                                                              r.mmax, r.part_b, b0, kNeverSink)
+            // the item list was built for r.adj_tg tiles per wavefront (two when only one or two
+            // transforms use this path)
+            switch (ng) {
                 case 8: PFSHT_ADJ(8, 1); break;
                 case 4: PFSHT_ADJ(4, 1); break;
                 case 2:
@@ -864,9 +1140,15 @@ void launch_ring_adj_tiles(const RingDev &r, const double2 *Gd, hipStream_t s) {
         });
 }
 
-void launch_ring_adj_finalize(const RingDev &r, double2 *alm, hipStream_t s) {
-    dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, r.B);
-    ring_adj_finalize<<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.B);
+void launch_ring_adj_finish(const RingDev &r, OutCols alm, int nb, hipStream_t s) {
+    if (nb <= 0 || ring_adj_uses_mfma(r, nb)) return;
+    dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, nb);
+    ring_adj_finalize<<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.part_b);
+}
+
+void launch_ring_adj(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s) {
+    launch_ring_adj_pass(r, Gd, alm, nb, s);
+    launch_ring_adj_finish(r, alm, nb, s);
 }
 
 // ------------------------------------------------------------------------------------------------

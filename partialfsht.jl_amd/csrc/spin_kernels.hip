@@ -192,7 +192,7 @@ template <int NB>
 __global__ void __launch_bounds__(64)
 spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh, const double *__restrict__ l2c,
                 const double *__restrict__ l2s, const double4 *__restrict__ seed, const double *__restrict__ Nl,
-                const double4 *__restrict__ tab, const int64_t *__restrict__ tofs, const double2 *__restrict__ alm,
+                const double4 *__restrict__ tab, const int64_t *__restrict__ tofs, InCols alm,
                 double2 *__restrict__ Gd, int n, int L, int s, int mmax, int b0, int nb) {
     const int4 it = items[blockIdx.x];
     const int m = it.x, j0 = it.y, nr = it.z;
@@ -206,8 +206,10 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
     int E;
     spin_seed(seed[m + L], l2c[j], l2s[j], d1, E);
     double d0 = 0.0;
-    const int64_t astride = int64_t(L + 1) * (2 * L + 1);
-    const double2 *col = alm + int64_t(m + L) * (L + 1) + int64_t(b0) * astride;
+    // column m + L of transform b0 + b (a transform past nb reads the last one's, unused)
+    gcptr col[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) col[b] = alm.col(b0 + min(b, nb - 1)) + int64_t(m + L) * (L + 1);
     double2 acc[NB];
     {
         const double w = Nl[l0] * d1;
@@ -215,7 +217,7 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
         for (int b = 0; b < NB; ++b) {
             acc[b] = make_double2(0.0, 0.0);
             if (b < nb) {
-                const double2 a = col[b * astride + l0];
+                const double2 a = gload(col[b] + l0);
                 acc[b] = make_double2(a.x * w, a.y * w);
             }
         }
@@ -227,14 +229,17 @@ spin_fwd_kernel(const int4 *__restrict__ items, const double *__restrict__ costh
     // broadcasts; renormalised every 64 steps.
     const int nsteps = L - l0, kq = lane & 15;
     const double4 *__restrict__ t0 = tab + tofs[am];
-    const double2 *__restrict__ a0p = col + (l0 + 1);
+    const int la = l0 + 1;
     double4 cn = make_double4(0.0, 0.0, 0.0, 0.0);
     double2 an[NB];
     auto group_load = [&](int q) {
         cn = t0[q];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-            an[b] = (NB == 1 || b < nb) ? a0p[q + b * astride] : make_double2(0.0, 0.0);
+        {
+            const double2 v = gload(col[b] + (la + q));
+            an[b] = (NB == 1 || b < nb) ? v : make_double2(0.0, 0.0);
+        }
     };
 #pragma unroll
     for (int b = 0; b < NB; ++b) an[b] = make_double2(0.0, 0.0);
@@ -396,7 +401,7 @@ spin_adj_kernel(const int4 *__restrict__ items, int nitems, const double *__rest
 // retained (l >= max(|m|, |s|), |m| <= mmax), zero elsewhere (the adjoint overwrites alm).
 __global__ void spin_adj_finalize(const int64_t *__restrict__ pofs, const int32_t *__restrict__ ntiles,
                                   const double2 *__restrict__ partials, const double *__restrict__ Nl,
-                                  double2 *__restrict__ alm, int L, int s, int mmax, int B) {
+                                  OutCols alm, int L, int s, int mmax, int B) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const int b = blockIdx.z;
@@ -416,7 +421,7 @@ __global__ void spin_adj_finalize(const int64_t *__restrict__ pofs, const int32_
         vx *= N;
         vy *= N;
     }
-    alm[(int64_t(b) * (2 * L + 1) + c) * (L + 1) + l] = make_double2(vx, vy);
+    gstore(alm.col(b) + (int64_t(c) * (L + 1) + l), make_double2(vx, vy));
 }
 
 void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s) {
@@ -424,10 +429,10 @@ void launch_spin_tables(const RingDev &r, const SpinDev &sd, hipStream_t s) {
     spin_tab_kernel<<<dim3((r.L + 63) / 64, r.L + 1), 64, 0, s>>>(r.L, sd.s, sd.tofs, sd.Nl, sd.tab);
 }
 
-void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, double2 *Gd, hipStream_t s) {
+void launch_spin_fwd(const RingDev &r, const SpinDev &sd, InCols alm, int nbt, double2 *Gd, hipStream_t s) {
     if (sd.nitems == 0) return;
-    for (int b0 = 0; b0 < r.B; b0 += 4) {
-        const int nb = std::min(4, r.B - b0);
+    for (int b0 = 0; b0 < nbt; b0 += 4) {
+        const int nb = std::min(4, nbt - b0);
         if (nb == 1)
             spin_fwd_kernel<1><<<sd.nitems, 64, 0, s>>>(sd.items, r.costh, sd.l2c, sd.l2s, sd.seed, sd.Nl, sd.tab,
                                                          sd.tofs, alm, Gd, r.n, r.L, sd.s, r.mmax, b0, nb);
@@ -437,11 +442,11 @@ void launch_spin_fwd(const RingDev &r, const SpinDev &sd, const double2 *alm, do
     }
 }
 
-void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, double2 *alm, hipStream_t s) {
+void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, OutCols alm, int nbt, hipStream_t s) {
     if (sd.nitems > 0) {
         const unsigned grid = unsigned((sd.nitems + kSpinAdjWaves - 1) / kSpinAdjWaves);
-        for (int b0 = 0; b0 < r.B; b0 += 4) {
-            const int nb = std::min(4, r.B - b0);
+        for (int b0 = 0; b0 < nbt; b0 += 4) {
+            const int nb = std::min(4, nbt - b0);
 #define PFSHT_SPIN_ADJ(NB)                                                                                   \
     spin_adj_kernel<NB><<<grid, 64 * kSpinAdjWaves, 0, s>>>(sd.items, sd.nitems, r.costh, sd.l2c, sd.l2s, sd.seed, \
                                                             sd.tab, sd.tofs, sd.pofs, Gd, sd.partials, r.n, r.L, sd.s, \
@@ -452,7 +457,7 @@ void launch_spin_adj(const RingDev &r, const SpinDev &sd, const double2 *Gd, dou
 #undef PFSHT_SPIN_ADJ
         }
     }
-    dim3 fgrid((r.L + 1 + 255) / 256, 2 * r.L + 1, r.B);
+    dim3 fgrid((r.L + 1 + 255) / 256, 2 * r.L + 1, nbt);
     spin_adj_finalize<<<fgrid, 256, 0, s>>>(sd.pofs, sd.ntiles, sd.partials, sd.Nl, alm, r.L, sd.s, r.mmax, r.B);
 }
 

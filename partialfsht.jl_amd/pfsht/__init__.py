@@ -53,6 +53,12 @@ class InfoC(C.Structure):
                 ("es_rows", C.c_int32), ("nboxes", C.c_int32), ("spin", C.c_int32)]
 
 
+class PlanOptsC(C.Structure):
+    _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("spin", C.c_int32), ("maxboxes", C.c_int32),
+                ("oversample_theta", C.c_double), ("oversample_phi", C.c_double), ("epsapo_max", C.c_double),
+                ("packed", C.c_int32), ("reserved", C.c_int32)]
+
+
 class BoxC(C.Structure):
     _fields_ = [("thlo", C.c_double), ("thhi", C.c_double), ("phistart", C.c_double), ("phiext", C.c_double),
                 ("touches_north", C.c_int32), ("touches_south", C.c_int32), ("fsky", C.c_double),
@@ -79,6 +85,13 @@ EXPORTS = {
                                     C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "pfsht_plan_create_spin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double,
                                          C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_plan_opts_default": (None, [C.c_void_p]),
+    "pfsht_plan_create_opts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_int32,
+                                         C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pfsht_plan_estimate_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_int32,
+                                            C.c_void_p, C.POINTER(C.c_int64)]),
+    "pfsht_synth_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pfsht_adjoint_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pfsht_plan_destroy": (None, [C.c_void_p]),
     "pfsht_plan_info": (C.c_int, [C.c_void_p, C.POINTER(InfoC)]),
     "pfsht_synth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
@@ -104,6 +117,8 @@ EXPORTS = {
                                         C.c_void_p]),
     "pfsht_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_void_p,
                               C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(SolveInfoC)]),
+    "pfsht_server_opts_default": (None, [C.c_void_p]),
+    "pfsht_server_create_ex": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "pfsht_server_create": (C.c_int, [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
     "pfsht_server_destroy": (None, [C.c_void_p]),
     "pfsht_server_plan": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32,
@@ -283,6 +298,21 @@ def partition_boxes(theta, phi, lmax: int, ntrans: int = 1, tol: float = 1e-10, 
     return [np.flatnonzero(gid == g) for g in range(ng.value)]
 
 
+def plan_estimate_bytes(theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1, method: str = "auto",
+                        precision: str = "fp64", spin: int = 0, packed: bool = False) -> int:
+    """Device bytes a Plan with these arguments would take (an upper estimate), without building it."""
+    th = np.ascontiguousarray(theta, np.float64)
+    ph = np.ascontiguousarray(phi, np.float64)
+    if th.shape != ph.shape or th.ndim != 1:
+        raise DimensionMismatch("theta and phi must have the same length")
+    o = PlanOptsC(method=METHODS[method], precision={"fp64": 0, "mixed": 1}[precision], spin=int(spin),
+                  packed=int(bool(packed)))
+    out = C.c_int64()
+    _check(lib().pfsht_plan_estimate_bytes(th.ctypes.data, ph.ctypes.data, len(th), int(lmax), float(tol),
+                                           int(ntrans), C.byref(o), C.byref(out)))
+    return int(out.value)
+
+
 def _weights(w, M: int, where: int):
     """(pointer, keep-alive) of npts float64 weights on the same side as the other arrays."""
     if w is None:
@@ -306,7 +336,9 @@ class Plan:
     """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116."""
 
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
-                 method: str = "auto", precision: str = "fp64", spin: int = 0):
+                 method: str = "auto", precision: str = "fp64", spin: int = 0, maxboxes: int = 8,
+                 oversample_theta: float = 1.0, oversample_phi: float = 1.0, epsapo_max: float = 1.0,
+                 packed: bool = False):
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
         ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
         if th.shape != ph.shape or th.ndim != 1:
@@ -320,9 +352,11 @@ class Plan:
         self.lmax, self.ntrans, self.tol, self.npts = int(lmax), int(ntrans), float(tol), len(th)
         self.spin = int(spin)
         h = C.c_void_p()
-        _check(lib().pfsht_plan_create_spin(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
-                                            len(th), self.lmax, self.tol, self.ntrans, self.spin, METHODS[method],
-                                            {"fp64": 0, "mixed": 1}[precision], C.byref(h)))
+        o = PlanOptsC(method=METHODS[method], precision={"fp64": 0, "mixed": 1}[precision], spin=self.spin,
+                      maxboxes=int(maxboxes), oversample_theta=float(oversample_theta),
+                      oversample_phi=float(oversample_phi), epsapo_max=float(epsapo_max), packed=int(bool(packed)))
+        _check(lib().pfsht_plan_create_opts(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
+                                            len(th), self.lmax, self.tol, self.ntrans, C.byref(o), C.byref(h)))
         self._h = h
         inf = InfoC()
         _check(lib().pfsht_plan_info(self._h, C.byref(inf)))
@@ -384,6 +418,35 @@ class Plan:
             raise ArgumentError("alm and f must both be host or both be device arrays")
         _check(lib().pfsht_adjoint(self._h, pf, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
         return alm
+
+    def _cols(self, ins, outs, nin: int, nout: int, stream):
+        nb = len(ins)
+        if len(outs) != nb:
+            raise DimensionMismatch(f"{nb} input columns but {len(outs)} output columns")
+        ptrs_in = (C.c_void_p * max(nb, 1))()
+        ptrs_out = (C.c_void_p * max(nb, 1))()
+        for b, (x, y) in enumerate(zip(ins, outs)):
+            px, wx = _buf(x, nin, "input column")
+            py, wy = _buf(y, nout, "output column")
+            if wx != WHERE_DEVICE or wy != WHERE_DEVICE:
+                raise ArgumentError("column applies take device (CUDA tensor) columns")
+            ptrs_in[b], ptrs_out[b] = px, py
+        return nb, ptrs_in, ptrs_out
+
+    def synth_cols(self, f_cols, alm_cols, stream=None):
+        """mul!(f, plan, alm) over len(alm_cols) <= ntrans transforms whose columns are separate device
+        tensors (each (lmax+1)(2lmax+1) coefficients in; npts values out)."""
+        L, M = self.lmax, self.npts
+        nb, pi, po = self._cols(alm_cols, f_cols, (L + 1) * (2 * L + 1), M, stream)
+        _check(lib().pfsht_synth_cols(self._h, nb, pi, po, _stream_ptr(stream)))
+        return f_cols
+
+    def adjoint_cols(self, alm_cols, f_cols, stream=None):
+        """mul!(alm, plan', f) over len(f_cols) <= ntrans transforms in separate device tensors."""
+        L, M = self.lmax, self.npts
+        nb, pi, po = self._cols(f_cols, alm_cols, M, (L + 1) * (2 * L + 1), stream)
+        _check(lib().pfsht_adjoint_cols(self._h, nb, pi, po, _stream_ptr(stream)))
+        return alm_cols
 
     def __matmul__(self, alm):
         """plan * alm (src/Plan.jl:190-193): a new field, host or device like alm."""
@@ -509,31 +572,51 @@ class AdjointPlan:
 class ServerStatsC(C.Structure):
     _fields_ = [("hits", C.c_int64), ("misses", C.c_int64), ("evictions", C.c_int64),
                 ("rejected", C.c_int64), ("submitted", C.c_int64), ("resident_bytes", C.c_int64),
-                ("budget_bytes", C.c_int64), ("plans", C.c_int64)]
+                ("budget_bytes", C.c_int64), ("plans", C.c_int64), ("launches", C.c_int64),
+                ("packed_transforms", C.c_int64), ("graph_launches", C.c_int64), ("graph_failures", C.c_int64),
+                ("failed", C.c_int64), ("queued", C.c_int64)]
+
+
+class ServerOptsC(C.Structure):
+    _fields_ = [("nstreams", C.c_int32), ("depth", C.c_int32), ("pack", C.c_int32), ("graphs", C.c_int32),
+                ("hbm_budget_bytes", C.c_int64)]
 
 
 class Server:
     """Serving runtime over pfsht_server_* (include/pfsht.h): plans cached in HBM under integer keys
-    with an HBM budget and LRU eviction, applies spread over nstreams streams, completion tickets.
-    Inputs and outputs are device (torch) buffers."""
+    with an HBM budget and LRU eviction; requests queued and packed by a dispatcher thread into column
+    applies of their plan (continuous batching), replayed from hipGraphs. Inputs and outputs are device
+    (torch) tensors holding the key's ntrans transforms in the reference's layouts; they are kept
+    alive here until their request has completed."""
 
-    def __init__(self, nstreams: int = 4, hbm_budget_bytes: int = 200 * 2**30):
+    def __init__(self, nstreams: int = 4, hbm_budget_bytes: int = 200 * 10**9, depth: int = 2, pack: int = 8,
+                 graphs: bool = True):
+        o = ServerOptsC(nstreams=int(nstreams), depth=int(depth), pack=int(pack), graphs=int(bool(graphs)),
+                        hbm_budget_bytes=int(hbm_budget_bytes))
         h = C.c_void_p()
-        _check(lib().pfsht_server_create(int(nstreams), int(hbm_budget_bytes), C.byref(h)))
+        _check(lib().pfsht_server_create_ex(C.byref(o), C.byref(h)))
         self._h = h
-        self._plans = {}
+        self._shape = {}   # key -> (lmax, npts, ntrans)
+        self._live = {}    # ticket -> (input, output): kept alive until the request completes
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "_h", None):
             lib().pfsht_server_destroy(self._h)
             self._h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def plan(self, key: int, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
              method: str = "auto"):
         """The plan cached under key (built on a miss). Returns (info dict, hit)."""
         th = np.ascontiguousarray(theta, np.float64)
         ph = np.ascontiguousarray(phi, np.float64)
-        if th.shape != ph.shape:
+        if th.shape != ph.shape or th.ndim != 1:
             raise DimensionMismatch("theta and phi must have the same length")
         hp = C.c_void_p()
         hit = C.c_int32()
@@ -542,27 +625,47 @@ class Server:
                                        C.byref(hit)))
         inf = InfoC()
         _check(lib().pfsht_plan_info(hp, C.byref(inf)))
+        self._shape[int(key)] = (int(lmax), len(th), int(ntrans))
         return {k: getattr(inf, k) for k, _ in InfoC._fields_}, bool(hit.value)
 
     def submit(self, key: int, direction: str, inp, out) -> int:
-        """Enqueue mul!(out, plan, inp) ("synth") or mul!(out, plan', inp) ("adjoint"); returns a
-        ticket."""
+        """Queue mul!(out, plan, inp) ("synth") or mul!(out, plan', inp) ("adjoint"); returns a ticket."""
+        if direction not in ("synth", "adjoint"):
+            raise ArgumentError(f"direction must be 'synth' or 'adjoint', got {direction!r}")
+        if int(key) not in self._shape:
+            raise ArgumentError(f"no plan was registered under key {key}")
+        L, M, B = self._shape[int(key)]
+        na, nf = (L + 1) * (2 * L + 1) * B, M * B
+        pi_, wi = _buf(inp, na if direction == "synth" else nf, "input")
+        po, wo = _buf(out, nf if direction == "synth" else na, "output")
+        if wi != WHERE_DEVICE or wo != WHERE_DEVICE:
+            raise ArgumentError("server requests take device (CUDA) tensors")
         t = C.c_int64()
-        d = 0 if direction == "synth" else 1
-        _check(lib().pfsht_server_submit(self._h, C.c_uint64(key), d, C.c_void_p(inp.data_ptr()),
-                                         C.c_void_p(out.data_ptr()), C.byref(t)))
+        _check(lib().pfsht_server_submit(self._h, C.c_uint64(key), 0 if direction == "synth" else 1, pi_, po,
+                                         C.byref(t)))
+        self._live[t.value] = (inp, out)
+        if len(self._live) > 4096:
+            self._prune()
         return t.value
+
+    def _prune(self):
+        for t in list(self._live):
+            if self.query(t)[0]:
+                self._live.pop(t, None)
 
     def query(self, ticket: int):
         """(done, device_ms) of a ticket, without waiting."""
         done = C.c_int32()
         ms = C.c_double(-1.0)
         _check(lib().pfsht_server_query(self._h, int(ticket), C.byref(done), C.byref(ms)))
+        if done.value:
+            self._live.pop(int(ticket), None)
         return bool(done.value), ms.value
 
     def wait(self, ticket: int) -> float:
         ms = C.c_double(-1.0)
         _check(lib().pfsht_server_wait(self._h, int(ticket), C.byref(ms)))
+        self._live.pop(int(ticket), None)
         return ms.value
 
     def epoch(self):
@@ -570,9 +673,10 @@ class Server:
         _check(lib().pfsht_server_epoch(self._h))
 
     def times(self, ticket: int):
-        """(start_ms, end_ms) of a ticket after the epoch, on the device clock (waits for it)."""
+        """(start_ms, end_ms) of a ticket's launch after the epoch, on the device clock (waits for it)."""
         a, b = C.c_double(), C.c_double()
         _check(lib().pfsht_server_times(self._h, int(ticket), C.byref(a), C.byref(b)))
+        self._live.pop(int(ticket), None)
         return a.value, b.value
 
     def stats(self) -> dict:

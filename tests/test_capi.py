@@ -100,3 +100,23 @@ def test_torch_layout_checked():
     field = torch.zeros((7, B), dtype=torch.complex128)  # M x B row-major
     with pytest.raises(pfsht.ArgumentError):
         pfsht._buf(field, 7 * B, "f")
+
+
+def test_plan_estimate_on_host():
+    # the size estimate runs on the host only: larger for more transforms, finite for a refused
+    # stretch (the Direct leaf's size), argument errors as codes
+    import pfsht
+    rng = np.random.default_rng(3)
+    ts = np.deg2rad(20)
+    th = np.arccos(rng.uniform(np.cos(ts), 1.0, 5000))
+    ph = rng.uniform(0, 2 * np.pi, 5000)
+    b1 = pfsht.plan_estimate_bytes(th, ph, 256, 1e-8, 1, "stretch")
+    b8 = pfsht.plan_estimate_bytes(th, ph, 256, 1e-8, 8, "stretch")
+    bm = pfsht.plan_estimate_bytes(th, ph, 256, 1e-5, 8, "stretch", "mixed")
+    assert 0 < b1 < b8 and bm < b8
+    t5 = np.arccos(rng.uniform(np.cos(np.deg2rad(5)), 1.0, 2000))
+    assert pfsht.plan_estimate_bytes(t5, ph[:2000], 256, 1e-8, 1, "auto") > 0
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht.plan_estimate_bytes(th, ph, 256, 1e-8, 0, "stretch")
+    with pytest.raises(pfsht.ArgumentError):
+        pfsht.plan_estimate_bytes(th, ph, 256, 1e-8, 1, "stretch", "mixed")  # mixed needs tol >= 1e-6

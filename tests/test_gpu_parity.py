@@ -377,59 +377,6 @@ def test_auto_selection_matches_oracle(torch, P, O, rng):
 # --------------------------------------------------------------------------------------------
 # Serving runtime: plan cache, admission, applies over several streams
 # --------------------------------------------------------------------------------------------
-def test_server_cache_admission_and_streams(torch, P, O, rng):
-    L = 256
-    geoms = []
-    for k in range(3):
-        th, ph = cap_points(rng, 20 + 5 * k, 3000 + 500 * k)
-        geoms.append((th, ph))
-    # budget that holds two of the three plans
-    probe = P.Server(nstreams=1, hbm_budget_bytes=1 << 40)
-    sizes = [probe.plan(k, th, ph, L, 1e-8, 2, "stretch")[0]["device_bytes"] for k, (th, ph) in enumerate(geoms)]
-    del probe
-    budget = sizes[0] + sizes[1] + min(sizes) // 2
-    srv = P.Server(nstreams=3, hbm_budget_bytes=budget)
-    info, hit = srv.plan(0, *geoms[0], L, 1e-8, 2, "stretch")
-    assert not hit
-    assert srv.plan(0, *geoms[0], L, 1e-8, 2, "stretch")[1]
-    srv.plan(1, *geoms[1], L, 1e-8, 2, "stretch")
-    srv.plan(0, *geoms[0], L, 1e-8, 2, "stretch")       # 0 becomes most recently used
-    srv.plan(2, *geoms[2], L, 1e-8, 2, "stretch")       # evicts 1, the least recently used
-    st = srv.stats()
-    assert st["evictions"] == 1 and st["plans"] == 2 and st["resident_bytes"] <= budget
-    assert st["hits"] == 2 and st["misses"] == 3
-    # many applies of two plans over three streams, several in flight on the same plan: every
-    # output equals the same apply made alone on a private plan (applies of one plan are ordered)
-    ref = {k: P.Plan(*geoms[k], lmax=L, tol=1e-8, ntrans=2, method="stretch") for k in (0, 2)}
-    jobs = []
-    for i in range(12):
-        k = (0, 2)[i % 2]
-        M = len(geoms[k][0])
-        a = O.rand_alm_fast(L, rng, ntrans=2)
-        f = np.asfortranarray(rng.standard_normal((M, 2)) + 1j * rng.standard_normal((M, 2)))
-        if i % 3 == 2:
-            inp, out = dev(torch, f), torch.empty(a.size, dtype=torch.complex128, device="cuda")
-            expect = ref[k].H @ f
-            t = srv.submit(k, "adjoint", inp, out)
-        else:
-            inp, out = dev(torch, a), torch.empty(M * 2, dtype=torch.complex128, device="cuda")
-            expect = ref[k] @ a
-            t = srv.submit(k, "synth", inp, out)
-        jobs.append((t, inp, out, expect))
-    for t, inp, out, expect in jobs:
-        ms = srv.wait(t)
-        assert ms >= 0
-        got = host(out, expect.shape)
-        assert np.array_equal(got, expect)
-    assert srv.query(jobs[-1][0])[0]
-    srv.epoch()
-    t = srv.submit(0, "synth", jobs[0][1], jobs[0][2])
-    t_start, t_end = srv.times(t)
-    assert 0.0 <= t_start <= t_end
-    with pytest.raises(P.ArgumentError):
-        srv.submit(1, "synth", jobs[0][1], jobs[0][2])   # evicted: no plan under key 1
-
-
 # --------------------------------------------------------------------------------------------
 # Full size: lmax 2048, M = 2e6 (the benchmark workload)
 # --------------------------------------------------------------------------------------------
