@@ -163,6 +163,14 @@ int pfsht_rings_from_alm(pfsht_rings_t rings, const void *alm, void *G, int32_t 
 /* alm_from_rings!(alm, G, eng) — src/Legendre.jl:332-341; alm overwritten. */
 int pfsht_alm_from_rings(pfsht_rings_t rings, const void *G, void *alm, int32_t where, void *stream);
 
+/* alm_from_rings!(alm, G, eng) as pfsht_alm_from_rings on device buffers, allocating nothing: the
+ * internal ring buffer and the ring pass's partial sums are taken from a file-scope device array of
+ * 64 MiB that each GPU holds for the life of the process, instead of the engine's own buffers. No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops). Engines needing more than 64 MiB of such scratch are refused (PFSHT_E_UNSUPPORTED).
+ * Calls that overlap on one GPU share the array and race: order them (one stream, or events). */
+int pfsht_alm_from_rings_noalloc(pfsht_rings_t rings, const void *G, void *alm, void *stream);
+
 /* The plan's NUFFT on its stretched points — the seam _nufft_exec!(plan, in, out) of
  * src/NUFFT.jl:17-25 with the two plans a StretchLeaf builds (src/Stretch.jl:235-238):
  *   type 2 (iflag +1): in = modes nth x nph x ntrans (column-major, FFT order), out = f (npts x ntrans);

@@ -187,6 +187,7 @@ struct pfsht_rings_s {
     std::vector<int> jlo, jhi, mlim;
     int64_t work = 0;
     int64_t bytes = 0;
+    int64_t part_bytes = 0;                   // the adjoint ring pass's partial sums
     double2 *Gd = nullptr;                    // [B][2 mmax + 1][n]
     double2 *alm_stage = nullptr, *G_stage = nullptr;  // for host-memory calls
     std::mutex mu;
@@ -371,6 +372,7 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     if ((st = upload(&d.pofs, pofs, by)) != kOk) return st;
     if ((st = upload(&d.ntiles, ntiles, by)) != kOk) return st;
     if ((st = dalloc(&d.partials, size_t(std::max<int64_t>(np, 1)), by)) != kOk) return st;
+    r.part_bytes = int64_t(sizeof(double4)) * std::max<int64_t>(np, 1);
     if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
     launch_ring_checkpoints(d, nullptr);
     PFSHT_HIP_TRY(hipGetLastError());
@@ -474,6 +476,7 @@ static Status rings_build_spin(const double *theta, int n, int L, int spin, doub
     if ((st = upload(&sd.pofs, spofs, by)) != kOk) return st;
     if ((st = upload(&sd.ntiles, sntiles, by)) != kOk) return st;
     if ((st = dalloc(&sd.partials, size_t(std::max<int64_t>(nsp, 1)), by)) != kOk) return st;
+    r.part_bytes = int64_t(sizeof(double2)) * std::max<int64_t>(nsp, 1);
     if ((st = dalloc(&r.Gd, size_t(B) * (2 * mmax + 1) * n, by)) != kOk) return st;
     launch_spin_tables(d, sd, nullptr);
     PFSHT_HIP_TRY(hipGetLastError());
@@ -1778,6 +1781,75 @@ int pfsht_alm_from_rings(pfsht_rings_t r, const void *G, void *alm, int32_t wher
         PFSHT_HIP_TRY(hipMemcpyAsync(alm, r->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
         PFSHT_HIP_TRY(hipStreamSynchronize(s));
     }
+    return kOk;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// The allocation-free adjoint ring pass. Its scratch, the internal ring buffer and the ring pass's
+// partial sums, lives in one file-scope device array; each GPU the library runs on holds its own copy
+// until the process ends. Host code never passes the array's name to a kernel: the device address is
+// looked up once per device (hipGetSymbolAddress, with that device current) and used from then on.
+// Both buffers are overwritten completely by every call, so nothing has to be cleared first; calls
+// that overlap on one GPU share the array and race.
+// ------------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kScratchDevices] = {};  // per device, nullptr until looked up
+
+static Status scratch_address(int dev, unsigned char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= kScratchDevices) { set_error("device index outside 0..63"); return kEHip; }
+    if (!g_scratch_ptr[dev]) {
+        DeviceGuard dg(dev);  // the device whose copy of the array is looked up
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_error(std::string("hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(e));
+            return kEHip;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return kOk;
+}
+
+extern "C" {
+
+int pfsht_alm_from_rings_noalloc(pfsht_rings_t r, const void *G, void *alm, void *stream) {
+    if (!r || !alm || !G) { set_error("NULL argument"); return kEArg; }
+    std::lock_guard<std::mutex> lk(r->mu);
+    DeviceGuard dg(r->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t gd_bytes = sizeof(double2) * size_t(r->d.B) * (2 * r->d.mmax + 1) * r->d.n;
+    const size_t gd_pad = (gd_bytes + 255) & ~size_t(255);
+    const size_t need = gd_pad + size_t(r->part_bytes);
+    if (need > kScratchBytes) {
+        set_error("the ring buffer and partial sums of this engine need " + std::to_string(need) +
+                  " bytes, more than the allocation-free scratch holds (" + std::to_string(kScratchBytes) +
+                  "); use pfsht_alm_from_rings");
+        return kEUnsupported;
+    }
+    unsigned char *base = nullptr;
+    Status st = scratch_address(r->dev, &base);
+    if (st != kOk) return st;
+    double2 *Gd = reinterpret_cast<double2 *>(base);
+    RingDev d = r->d;  // the engine's tables, with its scratch redirected to the array
+    d.partials = reinterpret_cast<double4 *>(base + gd_pad);
+    const OutCols out = out_cols(static_cast<double2 *>(alm), int64_t(d.L + 1) * (2 * d.L + 1));
+    launch_ref_to_gd(d, static_cast<const double2 *>(G), Gd, s);
+    if (r->spin) {
+        SpinDev sd = r->sd;
+        sd.partials = reinterpret_cast<double2 *>(base + gd_pad);
+        launch_spin_adj(d, sd, Gd, out, d.B, s);
+    } else {
+        launch_ring_adj(d, Gd, out, d.B, s);
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
 }
 

@@ -105,6 +105,7 @@ EXPORTS = {
     "pfsht_rings_mmax": (C.c_int32, [C.c_void_p]),
     "pfsht_rings_from_alm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_alm_from_rings": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_alm_from_rings_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pfsht_plan_rings": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "pfsht_plan_nufft": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -747,6 +748,16 @@ class Rings:
         if wa != wg:
             raise ArgumentError("alm and G must both be host or both be device arrays")
         _check(lib().pfsht_alm_from_rings(self._h, pg, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        return alm
+
+    def alm_from_rings_noalloc(self, alm, G, stream=None):
+        """alm_from_rings! on device tensors with the library's static scratch (allocates nothing)."""
+        L, B = self.lmax, self.ntrans
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
+        pg, wg = _buf(G, self.n * (2 * L + 1) * B, "G")
+        if wa != WHERE_DEVICE or wg != WHERE_DEVICE:
+            raise ArgumentError("the allocation-free adjoint takes device (CUDA) tensors")
+        _check(lib().pfsht_alm_from_rings_noalloc(self._h, pg, pa, _stream_ptr(stream)))
         return alm
 
 
