@@ -981,89 +981,175 @@ void or_dfs_fold(const or_cplx *Ft, or_cplx *F, int nth, int nph, int B, int s) 
 }
 
 /* ---------------------------------------------------------------------------------------------
- * Mixed-radix FFT (stand-in for FFTW, Stretch.jl:222-225): X_k = sum_n x_n exp(sign 2 pi i k n / N).
+ * Mixed-radix FFT (stand-in for FFTW, Stretch.jl:222-225): X_k = sum_n x_n exp(sign 2 pi i k n / N)
+ * for N = 2^a 3^b 5^c (the sizes next_smooth makes, Stretch.jl:61-81).
+ *
+ * Stockham self-sorting decimation in frequency (Van Loan, "Computational Frameworks for the FFT",
+ * 1992, sec. 1.7): s interleaved sequences, element j of sequence q at x[q + s j]. A radix-r stage
+ * of current length n = r m maps a_k = x[q + s (p + k m)] to
+ *   y[q + s (r p + u)] = W_n^(p u) sum_k a_k W_r^(k u),  W_n = exp(sign 2 pi i / n),
+ * then continues on length m with stride r s; the output comes out in natural order. The q loop is
+ * contiguous, so transforms along the second dimension of a column-major grid run on blocks of rows
+ * at once (s = rows in the block), which keeps the data in cache.
  * ------------------------------------------------------------------------------------------- */
-static void fft_rec(or_cplx *out, const or_cplx *in, int n, int s, int N, const or_cplx *tw, int sign) {
-    if (n == 1) { out[0] = in[0]; return; }
-    int p = 0;
-    if (n % 2 == 0) p = 2;
-    else if (n % 3 == 0) p = 3;
-    else if (n % 5 == 0) p = 5;
-    else p = n;
-    int q = n / p;
-    for (int r = 0; r < p; r++) fft_rec(out + r * q, in + r * s, q, s * p, N, tw, sign);
-    int step = N / n; /* twiddle W_n^j = W_N^(j*step) */
-    or_cplx tmp[64];
-    or_cplx *t = p <= 64 ? tmp : malloc(sizeof(or_cplx) * p);
-    for (int k = 0; k < q; k++) {
-        for (int r = 0; r < p; r++) {
-            int64_t idx = (int64_t)r * k * step % N;
-            or_cplx w = tw[idx];
-            if (sign < 0) w = cconj(w);
-            t[r] = cmul(out[r * q + k], w);
-        }
-        for (int u = 0; u < p; u++) {
-            or_cplx acc = cx(0, 0);
-            for (int r = 0; r < p; r++) {
-                int64_t idx = (int64_t)r * u * q * step % N;
-                or_cplx w = tw[idx];
-                if (sign < 0) w = cconj(w);
-                acc = cadd(acc, cmul(t[r], w));
+typedef struct {
+    int n, nst, rad[64]; /* 64 stages cover any int length */
+    or_cplx *tw; /* tw[k] = exp(+2 pi i k / n), k < n */
+} fft_plan_t;
+
+static void fft_plan_init(fft_plan_t *pl, int n) {
+    pl->n = n; pl->nst = 0;
+    int r = n;
+    while (r % 4 == 0) { pl->rad[pl->nst++] = 4; r /= 4; }
+    while (r % 2 == 0) { pl->rad[pl->nst++] = 2; r /= 2; }
+    while (r % 3 == 0) { pl->rad[pl->nst++] = 3; r /= 3; }
+    while (r % 5 == 0) { pl->rad[pl->nst++] = 5; r /= 5; }
+    if (r > 1) pl->rad[pl->nst++] = r; /* any other factor: a plain DFT stage */
+    pl->tw = malloc(sizeof(or_cplx) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) pl->tw[j] = ccis(2 * PI * (double)j / (double)n);
+}
+
+static void fft_plan_free(fft_plan_t *pl) { free(pl->tw); }
+
+static inline or_cplx tw_get(const fft_plan_t *pl, int64_t k, int sign) {
+    or_cplx w = pl->tw[k];
+    return sign < 0 ? cconj(w) : w;
+}
+
+/* one stage: length n (a divisor of pl->n) = r m, stride s; x -> y */
+static void fft_stage(const fft_plan_t *pl, int r, int n, int64_t s, const or_cplx *x, or_cplx *y,
+                      int sign) {
+    int m = n / r;
+    int64_t step = pl->n / n; /* W_n^j = W_N^(j step) */
+    double sg = sign < 0 ? -1.0 : 1.0;
+    for (int p = 0; p < m; p++) {
+        or_cplx w1 = tw_get(pl, (int64_t)p * step, sign);
+        or_cplx w2 = tw_get(pl, (int64_t)2 * p * step, sign);
+        or_cplx w3 = r >= 4 ? tw_get(pl, (int64_t)3 * p * step, sign) : cx(1, 0);
+        const or_cplx *xp = x + s * p;
+        or_cplx *yp = y + s * (int64_t)r * p;
+        if (r == 2) {
+            for (int64_t q = 0; q < s; q++) {
+                or_cplx a0 = xp[q], a1 = xp[q + s * m];
+                yp[q] = cadd(a0, a1);
+                yp[q + s] = cmul(cx(a0.re - a1.re, a0.im - a1.im), w1);
             }
-            out[u * q + k] = acc;
+        } else if (r == 4) {
+            for (int64_t q = 0; q < s; q++) {
+                or_cplx a0 = xp[q], a1 = xp[q + s * m], a2 = xp[q + 2 * s * m], a3 = xp[q + 3 * s * m];
+                or_cplx s02 = cadd(a0, a2), d02 = cx(a0.re - a2.re, a0.im - a2.im);
+                or_cplx s13 = cadd(a1, a3), d13 = cx(a1.re - a3.re, a1.im - a3.im);
+                or_cplx jd = cx(-sg * d13.im, sg * d13.re); /* W_4 d13, W_4 = sign i */
+                yp[q] = cadd(s02, s13);
+                yp[q + s] = cmul(cadd(d02, jd), w1);
+                yp[q + 2 * s] = cmul(cx(s02.re - s13.re, s02.im - s13.im), w2);
+                yp[q + 3 * s] = cmul(cx(d02.re - jd.re, d02.im - jd.im), w3);
+            }
+        } else if (r == 3) {
+            const double h = 0.86602540378443864676; /* sqrt(3) / 2 */
+            for (int64_t q = 0; q < s; q++) {
+                or_cplx a0 = xp[q], a1 = xp[q + s * m], a2 = xp[q + 2 * s * m];
+                or_cplx t = cadd(a1, a2), d = cx(a1.re - a2.re, a1.im - a2.im);
+                or_cplx c = cx(a0.re - 0.5 * t.re, a0.im - 0.5 * t.im);
+                or_cplx jd = cx(-sg * h * d.im, sg * h * d.re);
+                yp[q] = cadd(a0, t);
+                yp[q + s] = cmul(cadd(c, jd), w1);
+                yp[q + 2 * s] = cmul(cx(c.re - jd.re, c.im - jd.im), w2);
+            }
+        } else {
+            or_cplx wrs[64], as[64];
+            or_cplx *wr = r <= 64 ? wrs : malloc(sizeof(or_cplx) * (size_t)r);
+            or_cplx *a = r <= 64 ? as : malloc(sizeof(or_cplx) * (size_t)r);
+            int64_t rstep = pl->n / r;
+            for (int u = 0; u < r; u++) wr[u] = tw_get(pl, (int64_t)u * rstep, sign);
+            for (int64_t q = 0; q < s; q++) {
+                for (int k = 0; k < r; k++) a[k] = xp[q + s * (int64_t)k * m];
+                for (int u = 0; u < r; u++) {
+                    or_cplx acc = cx(0, 0);
+                    for (int k = 0; k < r; k++) acc = cadd(acc, cmul(a[k], wr[(k * u) % r]));
+                    yp[q + s * u] = u ? cmul(acc, tw_get(pl, (int64_t)p * u * step, sign)) : acc;
+                }
+            }
+            if (wr != wrs) { free(wr); free(a); }
         }
     }
-    if (t != tmp) free(t);
 }
 
-static or_cplx *twiddles(int N) {
-    or_cplx *tw = malloc(sizeof(or_cplx) * N);
-    for (int j = 0; j < N; j++) tw[j] = ccis(2 * PI * (double)j / (double)N);
-    return tw;
-}
-
-static void fft_line(or_cplx *x, int n, int sign, const or_cplx *tw, or_cplx *buf) {
-    memcpy(buf, x, sizeof(or_cplx) * n);
-    fft_rec(x, buf, n, 1, n, tw, sign);
+/* s interleaved transforms of length pl->n in x (work array y of the same size) */
+static void fft_run(const fft_plan_t *pl, or_cplx *x, or_cplx *y, int64_t s, int sign) {
+    int n = pl->n;
+    or_cplx *a = x, *b = y;
+    for (int st = 0; st < pl->nst; st++) {
+        int r = pl->rad[st];
+        fft_stage(pl, r, n, s, a, b, sign);
+        n /= r; s *= r;
+        or_cplx *t = a; a = b; b = t;
+    }
+    if (a != x) memcpy(x, a, sizeof(or_cplx) * (size_t)s);
 }
 
 void or_fft1(or_cplx *x, int n, int sign) {
-    or_cplx *tw = twiddles(n), *buf = malloc(sizeof(or_cplx) * n);
-    fft_line(x, n, sign, tw, buf);
-    free(tw); free(buf);
+    fft_plan_t pl;
+    fft_plan_init(&pl, n);
+    or_cplx *buf = malloc(sizeof(or_cplx) * (size_t)(n > 0 ? n : 1));
+    fft_run(&pl, x, buf, 1, sign);
+    free(buf);
+    fft_plan_free(&pl);
 }
+
+enum { FFT_ROWS = 8 }; /* rows per block for transforms along dim 2 */
 
 /* 1-D transforms along dim 2 of column-major n1 x n2 x B */
 static void fft_dim2(or_cplx *x, int n1, int n2, int B, int sign, int nthreads) {
-    or_cplx *tw = twiddles(n2);
+    fft_plan_t pl;
+    fft_plan_init(&pl, n2);
     int nt = nthreads < 1 ? 1 : nthreads;
+    int64_t nblk = ((int64_t)n1 + FFT_ROWS - 1) / FFT_ROWS;
 #pragma omp parallel num_threads(nt)
     {
-        or_cplx *line = malloc(sizeof(or_cplx) * n2), *buf = malloc(sizeof(or_cplx) * n2);
+        or_cplx *blk = malloc(sizeof(or_cplx) * (size_t)n2 * FFT_ROWS);
+        or_cplx *buf = malloc(sizeof(or_cplx) * (size_t)n2 * FFT_ROWS);
 #pragma omp for schedule(static)
-        for (int64_t r = 0; r < (int64_t)n1 * B; r++) {
-            int64_t b = r / n1, i = r % n1;
-            or_cplx *base = x + b * (int64_t)n1 * n2 + i;
-            for (int j = 0; j < n2; j++) line[j] = base[(int64_t)j * n1];
-            fft_line(line, n2, sign, tw, buf);
-            for (int j = 0; j < n2; j++) base[(int64_t)j * n1] = line[j];
+        for (int64_t t = 0; t < nblk * B; t++) {
+            int64_t b = t / nblk, i0 = (t % nblk) * FFT_ROWS;
+            int v = (int)(n1 - i0 < FFT_ROWS ? n1 - i0 : FFT_ROWS);
+            or_cplx *base = x + b * (int64_t)n1 * n2 + i0;
+            for (int j = 0; j < n2; j++)
+                for (int i = 0; i < v; i++) blk[i + (int64_t)j * v] = base[i + (int64_t)j * n1];
+            fft_run(&pl, blk, buf, v, sign);
+            for (int j = 0; j < n2; j++)
+                for (int i = 0; i < v; i++) base[i + (int64_t)j * n1] = blk[i + (int64_t)j * v];
         }
-        free(line); free(buf);
+        free(blk); free(buf);
     }
-    free(tw);
+    fft_plan_free(&pl);
 }
 
+/* 1-D transforms along dim 1: blocks of FFT_ROWS columns are interleaved (a small transpose) so
+ * that every stage runs over FFT_ROWS sequences at once */
 static void fft_dim1(or_cplx *x, int n1, int n2, int B, int sign, int nthreads) {
-    or_cplx *tw = twiddles(n1);
+    fft_plan_t pl;
+    fft_plan_init(&pl, n1);
     int nt = nthreads < 1 ? 1 : nthreads;
+    int64_t ncol = (int64_t)n2 * B, nblk = (ncol + FFT_ROWS - 1) / FFT_ROWS;
 #pragma omp parallel num_threads(nt)
     {
-        or_cplx *buf = malloc(sizeof(or_cplx) * n1);
+        or_cplx *blk = malloc(sizeof(or_cplx) * (size_t)n1 * FFT_ROWS);
+        or_cplx *buf = malloc(sizeof(or_cplx) * (size_t)n1 * FFT_ROWS);
 #pragma omp for schedule(static)
-        for (int64_t c = 0; c < (int64_t)n2 * B; c++) fft_line(x + c * n1, n1, sign, tw, buf);
-        free(buf);
+        for (int64_t t = 0; t < nblk; t++) {
+            int64_t c0 = t * FFT_ROWS;
+            int v = (int)(ncol - c0 < FFT_ROWS ? ncol - c0 : FFT_ROWS);
+            or_cplx *base = x + c0 * n1;
+            for (int c = 0; c < v; c++)
+                for (int i = 0; i < n1; i++) blk[c + (int64_t)i * v] = base[i + (int64_t)c * n1];
+            fft_run(&pl, blk, buf, v, sign);
+            for (int c = 0; c < v; c++)
+                for (int i = 0; i < n1; i++) base[i + (int64_t)c * n1] = blk[c + (int64_t)i * v];
+        }
+        free(blk); free(buf);
     }
-    free(tw);
+    fft_plan_free(&pl);
 }
 
 void or_fft2(or_cplx *x, int n1, int n2, int B, int sign, int nthreads) {
@@ -1193,22 +1279,64 @@ static void es_corr(int n, int N, int w, double beta, double *corr) {
     }
 }
 
-/* kernel values at point x (radians) on a fine grid of N cells: base cell l0 and w weights */
-static inline int es_weights(double x, int N, int w, double beta, double *kv) {
+/* base cell of the w-cell window of point x (radians) on a fine grid of N cells, and the point's
+ * fine-grid coordinate X in [0, N) */
+static inline int es_base(double x, int N, int w, double *Xout) {
     double X = x * ((double)N / (2 * PI));
     X = X - floor(X / N) * N; /* [0, N) */
-    double l0 = ceil(X - 0.5 * w);
-    double inv = 2.0 / w;
-    for (int t = 0; t < w; t++) {
-        double z = (l0 + t - X) * inv;
-        double zz = 1 - z * z;
-        kv[t] = zz >= 0 ? exp(beta * (sqrt(zz) - 1)) : 0.0;
-    }
-    return (int)l0;
+    *Xout = X;
+    return (int)ceil(X - 0.5 * w);
 }
 
-int or_nufft_es(int type, int n1, int n2, int iflag, int B, const double *x, const double *y,
-                int64_t M, double tol, const or_cplx *in, or_cplx *out, int nthreads) {
+/* kernel values at point x (radians) on a fine grid of N cells: base cell l0 (in [-w/2, N)), the w
+ * weights, and the w window cells wrapped into [0, N) */
+static inline int es_weights(double x, int N, int w, double beta, double *kv, int *cell) {
+    double X;
+    int l0 = es_base(x, N, w, &X);
+    double inv = 2.0 / w;
+    for (int t = 0; t < w; t++) {
+        double z = ((double)l0 + t - X) * inv;
+        double zz = 1 - z * z;
+        kv[t] = zz >= 0 ? exp(beta * (sqrt(zz) - 1)) : 0.0;
+        int c = l0 + t;
+        cell[t] = c < 0 ? c + N : (c >= N ? c - N : c);
+    }
+    return l0;
+}
+
+typedef struct { int64_t key, j; } es_key_t;
+
+static int es_key_cmp(const void *a, const void *b) {
+    const es_key_t *x = a, *y = b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->j < y->j ? -1 : (x->j > y->j);
+}
+
+/* Points ordered by the fine-grid cell of their window (second coordinate first), as a fast NUFFT
+ * sorts them when it is planned; a pure reordering of the per-point work, which changes no value
+ * of type 2 and only the summation order of type 1. */
+static int64_t *es_point_order(int n1, int n2, double tol, const double *x, const double *y, int64_t M) {
+    int w;
+    double beta, X;
+    or_es_params(tol, &w, &beta);
+    int N1 = es_fine_size(n1, w), N2 = es_fine_size(n2, w);
+    es_key_t *k = malloc(sizeof(es_key_t) * (size_t)(M > 0 ? M : 1));
+    for (int64_t j = 0; j < M; j++) {
+        int64_t b1 = es_base(x[j], N1, w, &X) + w, b2 = es_base(y[j], N2, w, &X) + w;
+        k[j].key = b2 * (int64_t)(N1 + 2 * w) + b1;
+        k[j].j = j;
+    }
+    qsort(k, (size_t)M, sizeof(es_key_t), es_key_cmp);
+    int64_t *ord = malloc(sizeof(int64_t) * (size_t)(M > 0 ? M : 1));
+    for (int64_t j = 0; j < M; j++) ord[j] = k[j].j;
+    free(k);
+    return ord;
+}
+
+/* the ES NUFFT over the points in the given order (NULL: index order) */
+static int nufft_es_ordered(int type, int n1, int n2, int iflag, int B, const double *x, const double *y,
+                            int64_t M, double tol, const or_cplx *in, or_cplx *out, int nthreads,
+                            const int64_t *order) {
     int w;
     double beta;
     or_es_params(tol, &w, &beta);
@@ -1234,57 +1362,75 @@ int or_nufft_es(int type, int n1, int n2, int iflag, int B, const double *x, con
             }
             or_fft2(u, N1, N2, 1, iflag, nt);
 #pragma omp parallel for num_threads(nt) schedule(static)
-            for (int64_t j = 0; j < M; j++) {
+            for (int64_t jj = 0; jj < M; jj++) {
+                int64_t j = order ? order[jj] : jj;
                 double k1v[16], k2v[16];
-                int b1 = es_weights(x[j], N1, w, beta, k1v);
-                int b2 = es_weights(y[j], N2, w, beta, k2v);
+                int i1v[16], i2v[16];
+                es_weights(x[j], N1, w, beta, k1v, i1v);
+                es_weights(y[j], N2, w, beta, k2v, i2v);
                 or_cplx acc = cx(0, 0);
                 for (int t2 = 0; t2 < w; t2++) {
-                    int l2 = ((b2 + t2) % N2 + N2) % N2;
+                    const or_cplx *col = u + (int64_t)i2v[t2] * N1;
                     or_cplx row = cx(0, 0);
-                    for (int t1 = 0; t1 < w; t1++) {
-                        int l1 = ((b1 + t1) % N1 + N1) % N1;
-                        row = cadd(row, cscale(u[l1 + (int64_t)l2 * N1], k1v[t1]));
-                    }
+                    for (int t1 = 0; t1 < w; t1++) row = cadd(row, cscale(col[i1v[t1]], k1v[t1]));
                     acc = cadd(acc, cscale(row, k2v[t2]));
                 }
                 out[j + (int64_t)b * M] = acc;
             }
         } else if (type == 1) {
             const or_cplx *fb = in + (int64_t)b * M;
-            /* per-thread private grids summed in thread order: deterministic for a fixed count */
-            or_cplx **priv = malloc(sizeof(or_cplx *) * nt);
-            for (int t = 0; t < nt; t++) priv[t] = t == 0 ? u : malloc(sizeof(or_cplx) * NN);
+            /* thread t spreads a contiguous run of the ordered points into a private grid, summed
+             * into u in thread order: deterministic for a fixed thread count. With ordered points
+             * a run covers a band of columns, and only that band is kept. */
+            or_cplx **priv = calloc((size_t)nt, sizeof(or_cplx *));
+            int *lo = calloc((size_t)nt, sizeof(int)), *nc = calloc((size_t)nt, sizeof(int));
 #pragma omp parallel num_threads(nt)
             {
                 int tid = 0;
 #ifdef _OPENMP
                 tid = omp_get_thread_num();
 #endif
-                or_cplx *g = priv[tid];
-                memset(g, 0, sizeof(or_cplx) * NN);
                 int64_t chunk = (M + nt - 1) / nt, j0 = tid * chunk, j1 = j0 + chunk < M ? j0 + chunk : M;
-                for (int64_t j = j0; j < j1; j++) {
+                int clo = 0, chi = N2 - 1;
+                if (order && j1 > j0) { /* ordered: the band between the first and last window */
+                    double X;
+                    clo = es_base(y[order[j0]], N2, w, &X);
+                    chi = es_base(y[order[j1 - 1]], N2, w, &X) + w - 1;
+                    if (chi - clo + 1 >= N2) { clo = 0; chi = N2 - 1; }
+                }
+                int ncol = j1 > j0 ? chi - clo + 1 : 0;
+                or_cplx *g = ncol ? calloc((size_t)ncol * N1, sizeof(or_cplx)) : NULL;
+                priv[tid] = g; lo[tid] = clo; nc[tid] = ncol;
+                int banded = order && ncol && !(clo == 0 && chi == N2 - 1);
+                for (int64_t jj = j0; jj < j1; jj++) {
+                    int64_t j = order ? order[jj] : jj;
                     double k1v[16], k2v[16];
-                    int b1 = es_weights(x[j], N1, w, beta, k1v);
-                    int b2 = es_weights(y[j], N2, w, beta, k2v);
+                    int i1v[16], i2v[16];
+                    es_weights(x[j], N1, w, beta, k1v, i1v);
+                    int b2 = es_weights(y[j], N2, w, beta, k2v, i2v);
                     or_cplx fj = fb[j];
                     for (int t2 = 0; t2 < w; t2++) {
-                        int l2 = ((b2 + t2) % N2 + N2) % N2;
+                        int c = banded ? b2 + t2 - clo : i2v[t2];
+                        or_cplx *col = g + (int64_t)c * N1;
                         or_cplx v2 = cscale(fj, k2v[t2]);
-                        for (int t1 = 0; t1 < w; t1++) {
-                            int l1 = ((b1 + t1) % N1 + N1) % N1;
-                            int64_t o = l1 + (int64_t)l2 * N1;
-                            g[o] = cadd(g[o], cscale(v2, k1v[t1]));
-                        }
+                        for (int t1 = 0; t1 < w; t1++) col[i1v[t1]] = cadd(col[i1v[t1]], cscale(v2, k1v[t1]));
                     }
                 }
             }
-            for (int t = 1; t < nt; t++) {
-                for (int64_t i = 0; i < NN; i++) u[i] = cadd(u[i], priv[t][i]);
+            memset(u, 0, sizeof(or_cplx) * NN);
+            for (int t = 0; t < nt; t++) {
+                if (!priv[t]) continue;
+                int full = nc[t] == N2 && lo[t] == 0;
+                for (int c = 0; c < nc[t]; c++) {
+                    int l2 = full ? c : lo[t] + c;
+                    l2 = l2 < 0 ? l2 + N2 : (l2 >= N2 ? l2 - N2 : l2);
+                    or_cplx *dst = u + (int64_t)l2 * N1;
+                    const or_cplx *src = priv[t] + (int64_t)c * N1;
+                    for (int i = 0; i < N1; i++) dst[i] = cadd(dst[i], src[i]);
+                }
                 free(priv[t]);
             }
-            free(priv);
+            free(priv); free(lo); free(nc);
             or_fft2(u, N1, N2, 1, iflag, nt);
             or_cplx *cb = out + (int64_t)b * n1 * n2;
             for (int i2 = 0; i2 < n2; i2++) {
@@ -1306,6 +1452,11 @@ int or_nufft_es(int type, int n1, int n2, int iflag, int B, const double *x, con
     return 0;
 }
 
+int or_nufft_es(int type, int n1, int n2, int iflag, int B, const double *x, const double *y,
+                int64_t M, double tol, const or_cplx *in, or_cplx *out, int nthreads) {
+    return nufft_es_ordered(type, n1, n2, iflag, B, x, y, M, tol, in, out, nthreads, NULL);
+}
+
 /* ---------------------------------------------------------------------------------------------
  * Plan (Plan.jl:84-116), planner (Cost.jl:104-140,174-201), leaves (Stretch.jl, Direct.jl).
  * ------------------------------------------------------------------------------------------- */
@@ -1320,6 +1471,7 @@ struct or_plan {
     int spin;            /* spin weight; nonzero plans use the spin engine (Spin.jl) */
     or_spin_rings *seng;
     double *wth, *Tpts, *ppts;
+    int64_t *es_order;  /* ES NUFFT point order, fixed at plan time */
     or_cplx *phase, *G, *F, *Ft, *Fhat, *fbuf;
     /* direct */
     int64_t *perm;
@@ -1397,6 +1549,7 @@ void or_plan_destroy(or_plan *p) {
     or_spin_rings_destroy(p->seng);
     free(p->wth); free(p->Tpts); free(p->ppts); free(p->phase); free(p->G); free(p->F);
     free(p->Ft); free(p->Fhat); free(p->fbuf); free(p->perm); free(p->ths); free(p->phs);
+    free(p->es_order);
     free(p);
 }
 
@@ -1480,6 +1633,7 @@ or_plan *or_plan_create_spin(const double *theta, const double *phi, int64_t M, 
         p->ppts[i] = jmod(phi[i], 2 * PI);
     }
     int nth = p->nth, nph = p->nph, B = ntrans;
+    if (nufft_kind != 1) p->es_order = es_point_order(nth, nph, tol, p->Tpts, p->ppts, M);
     p->G = calloc((size_t)nh * (2 * L + 1) * B, sizeof(or_cplx));
     p->F = calloc((size_t)nh * nph * B, sizeof(or_cplx));
     p->Ft = calloc((size_t)nth * nph * B, sizeof(or_cplx));
@@ -1536,7 +1690,8 @@ static void unpack_orders(or_cplx *G, const or_cplx *F, int nh, int nph, int L, 
 static int run_nufft(or_plan *p, int type, int iflag, const or_cplx *in, or_cplx *out) {
     if (p->nufft_kind == 1)
         return or_nufft_direct(type, p->nth, p->nph, iflag, p->B, p->Tpts, p->ppts, p->M, in, out);
-    return or_nufft_es(type, p->nth, p->nph, iflag, p->B, p->Tpts, p->ppts, p->M, p->tol, in, out, p->nthreads);
+    return nufft_es_ordered(type, p->nth, p->nph, iflag, p->B, p->Tpts, p->ppts, p->M, p->tol, in, out, p->nthreads,
+                            p->es_order);
 }
 
 /* Stretch.jl:262-277 / Direct.jl:60-91 */
@@ -1569,7 +1724,7 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     else or_rings_from_alm(p->eng, alm, p->G, p->nthreads);
     double t1 = now_s();
     pack_orders(p->F, p->G, nh, nph, L, B);
-    fft_dim2(p->F, nh, nph, B, +1, 1);
+    fft_dim2(p->F, nh, nph, B, +1, p->nthreads);
     for (int b = 0; b < B; b++)
         for (int k = 0; k < nph; k++)
             for (int j = 0; j < nh; j++) {
@@ -1579,7 +1734,7 @@ int or_synth(or_plan *p, const or_cplx *alm, or_cplx *f) {
     if (p->geom.kind == OR_KIND_INTERIOR) memcpy(p->Ft, p->F, sizeof(or_cplx) * (size_t)nth * nph * B);
     else or_dfs_double(p->F, p->Ft, nh, nph, B, p->spin);
     memcpy(p->Fhat, p->Ft, sizeof(or_cplx) * (size_t)nth * nph * B);
-    or_fft2(p->Fhat, nth, nph, B, -1, 1);
+    or_fft2(p->Fhat, nth, nph, B, -1, p->nthreads);
     for (int b = 0; b < B; b++)
         for (int k = 0; k < nph; k++)
             for (int i = 0; i < nth; i++) {
@@ -1634,7 +1789,7 @@ int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
                 p->Fhat[q] = cmul(p->Fhat[q], cconj(p->phase[i]));
             }
     memcpy(p->Ft, p->Fhat, sizeof(or_cplx) * (size_t)nth * nph * B);
-    or_fft2(p->Ft, nth, nph, B, +1, 1);
+    or_fft2(p->Ft, nth, nph, B, +1, p->nthreads);
     if (p->geom.kind == OR_KIND_INTERIOR) memcpy(p->F, p->Ft, sizeof(or_cplx) * (size_t)nth * nph * B);
     else or_dfs_fold(p->Ft, p->F, nh, nph, B, p->spin);
     for (int b = 0; b < B; b++)
@@ -1643,7 +1798,7 @@ int or_adjoint(or_plan *p, const or_cplx *f, or_cplx *alm) {
                 int64_t i = j + (int64_t)k * nh + (int64_t)b * nh * nph;
                 p->F[i] = cscale(p->F[i], p->wth[j]);
             }
-    fft_dim2(p->F, nh, nph, B, -1, 1);
+    fft_dim2(p->F, nh, nph, B, -1, p->nthreads);
     unpack_orders(p->G, p->F, nh, nph, L, B);
     double t2 = now_s();
     if (p->spin) or_spin_alm_from_rings(p->seng, p->G, alm, p->nthreads);

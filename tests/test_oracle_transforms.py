@@ -152,12 +152,16 @@ def test_dfs_double_is_fold_transpose(O, rng):
 
 
 def test_fft_matches_numpy(O, rng):
-    for n in (1, 2, 6, 30, 360, 1920, 720):
+    # radix-4/2/3/5 stages, and the plain-DFT stage for other primes (7, 67, 131)
+    for n in (1, 2, 6, 30, 360, 1920, 720, 4, 8, 12, 25, 375, 7, 420, 67, 262):
         x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
         assert relmax(O.fft1(x, -1), np.fft.fft(x)) < 1e-13
         assert relmax(O.fft1(x, +1), np.fft.ifft(x) * n) < 1e-13
-    X = rng.standard_normal((12, 20, 2)) + 1j * rng.standard_normal((12, 20, 2))
-    assert relmax(O.fft2(X, -1), np.fft.fft2(X, axes=(0, 1))) < 1e-13
+    # second-dimension transforms run on blocks of 8 rows: row counts below, at and off a multiple
+    for shape in ((12, 20, 2), (8, 45, 1), (3, 64, 3), (45, 96, 1), (17, 50, 2)):
+        X = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        assert relmax(O.fft2(X, -1), np.fft.fft2(X, axes=(0, 1))) < 1e-13
+        assert relmax(O.fft2(X, +1, nthreads=3), np.fft.ifft2(X, axes=(0, 1)) * shape[0] * shape[1]) < 1e-13
 
 
 @pytest.mark.parametrize("tol", [1e-6, 1e-10])
