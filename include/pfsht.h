@@ -134,6 +134,15 @@ int pfsht_synth(pfsht_plan_t plan, const void *alm, void *f, int32_t where, void
  * Overwrites alm, padding slots included (src/Legendre.jl:338). */
 int pfsht_adjoint(pfsht_plan_t plan, const void *f, void *alm, int32_t where, void *stream);
 
+/* mul!(f, plan, alm) and mul!(alm, plan', f) on a Float32 plan — src/Plan.jl:84-116 with
+ * T = Float32, as gpu/runtests.jl:122-139 and test/test_transforms.jl:158-190 run it: alm and f hold
+ * ComplexF32 (interleaved float pairs) in the same layouts and counts as the FP64 calls, and the
+ * result is ComplexF32. The plan computes as it was created (a PFSHT_PREC_MIXED plan, tol >= 1e-6,
+ * keeps the recurrence FP64 and the grids FP32; an FP64 plan is FP64 throughout); the input is widened
+ * into the plan's FP64 staging and the result narrowed on the way out. */
+int pfsht_synth_f32(pfsht_plan_t plan, const void *alm, void *f, int32_t where, void *stream);
+int pfsht_adjoint_f32(pfsht_plan_t plan, const void *f, void *alm, int32_t where, void *stream);
+
 /* mul!(f, plan, alm) and mul!(alm, plan', f) — src/Plan.jl:137-148 — over nb <= ntrans transforms whose
  * columns lie anywhere in device memory: alm_cols[b] is the (lmax+1) x (2 lmax+1) coefficient array of
  * transform b (sph_mode layout, or the spin layout on a spin plan) and f_cols[b] its npts field values.

@@ -122,6 +122,35 @@ void launch_cdot(const double2 *a, const double2 *b, int64_t n, double2 *part, d
     cdot_final_kernel<<<1, kRedThreads, 0, s>>>(part, kDotBlocks, out);
 }
 
+// ComplexF32 <-> ComplexF64 for the single-precision applies: four elements per thread.
+__global__ void widen_c64_kernel(const float2 *__restrict__ in, double2 *__restrict__ out, int64_t n) {
+    const int64_t i0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < n) {
+            const float2 v = in[i0 + k];
+            out[i0 + k] = make_double2(double(v.x), double(v.y));
+        }
+}
+
+__global__ void narrow_c128_kernel(const double2 *__restrict__ in, float2 *__restrict__ out, int64_t n) {
+    const int64_t i0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < n) {
+            const double2 v = in[i0 + k];
+            out[i0 + k] = make_float2(float(v.x), float(v.y));
+        }
+}
+
+void launch_widen_c64(const float2 *in, double2 *out, int64_t n, hipStream_t s) {
+    if (n > 0) widen_c64_kernel<<<unsigned((n + 1023) / 1024), 256, 0, s>>>(in, out, n);
+}
+
+void launch_narrow_c128(const double2 *in, float2 *out, int64_t n, hipStream_t s) {
+    if (n > 0) narrow_c128_kernel<<<unsigned((n + 1023) / 1024), 256, 0, s>>>(in, out, n);
+}
+
 void launch_zaxpy(double2 *y, const double2 *x, double2 a, int64_t n, hipStream_t s) {
     if (n > 0) zaxpy_kernel<<<unsigned((n + 255) / 256), 256, 0, s>>>(y, x, a, n);
 }

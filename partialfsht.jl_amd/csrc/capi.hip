@@ -532,6 +532,7 @@ struct pfsht_plan_s {
     void *fft_work = nullptr;    // rocFFT work buffer shared by the plan's FFTs
     bool partial = false;        // FFT plans for every power-of-two transform count (packed applies)
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
+    float2 *c64_stage = nullptr;  // ComplexF32 applies on host memory: input, then output
     // Direct leaf: theta-sorted points as rings
     double *dphi = nullptr;
     int32_t *dperm = nullptr, *dmlim = nullptr;
@@ -578,7 +579,7 @@ struct pfsht_plan_s {
         dfree(wth);
         dfree(nu.X1); dfree(nu.X2); dfree(nu.perm); dfree(nu.pc1); dfree(nu.c1); dfree(nu.c2);
         dfree(nu.st_list); dfree(nu.st_slot); dfree(nu.bin_ofs); dfree(nu.Wst);
-        dfree(Ft); dfree(U); dfree(Ut); dfree(alm_stage); dfree(f_stage); dfree(modes_stage);
+        dfree(Ft); dfree(U); dfree(Ut); dfree(alm_stage); dfree(f_stage); dfree(modes_stage); dfree(c64_stage);
         dfree(dphi); dfree(dperm); dfree(dmlim);
         dfree(aw_rhs); dfree(aw_r); dfree(aw_p); dfree(aw_Ap); dfree(aw_x); dfree(aw_f); dfree(aw_f2);
         dfree(aw_part); dfree(aw_dot); dfree(aw_w); dfree(aw_reg); dfree(aw_C);
@@ -1584,6 +1585,57 @@ int pfsht_adjoint(pfsht_plan_t p, const void *f, void *alm, int32_t where, void 
     PFSHT_HIP_TRY(hipMemcpyAsync(alm, p->alm_stage, alm_bytes, hipMemcpyDeviceToHost, s));
     PFSHT_HIP_TRY(hipStreamSynchronize(s));
     return kOk;
+}
+
+// ComplexF32 applies (dir 0: synthesis, 1: adjoint), the device path of a Float32 plan: the input
+// is widened into the plan's FP64 staging, applied by the plan's own pipeline (FP64, or FP64
+// recurrence with FP32 grids for a mixed plan), and the result narrowed into the caller's buffer.
+static int apply_c64(pfsht_plan_t p, int dir, const void *in, void *out, int32_t where, void *stream) {
+    if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
+    if (where != PFSHT_WHERE_HOST && where != PFSHT_WHERE_DEVICE) { set_error("bad where"); return kEArg; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    DeviceGuard dg(p->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t na = size_t(p->L + 1) * (2 * p->L + 1) * p->B;
+    const size_t nf = size_t(p->M) * p->B;
+    const size_t nin = dir == 0 ? na : nf, nout = dir == 0 ? nf : na;
+    Status st;
+    if ((st = apply_begin(*p, s)) != kOk) return st;  // the staging buffers belong to the plan
+    if ((st = stage_in(reinterpret_cast<void **>(&p->alm_stage), sizeof(double2) * na, nullptr, s, p->bytes)) != kOk)
+        return st;
+    if ((st = stage_in(reinterpret_cast<void **>(&p->f_stage), sizeof(double2) * nf, nullptr, s, p->bytes)) != kOk)
+        return st;
+    const float2 *src = static_cast<const float2 *>(in);
+    float2 *dst = static_cast<float2 *>(out);
+    if (where == PFSHT_WHERE_HOST) {
+        if ((st = stage_in(reinterpret_cast<void **>(&p->c64_stage), sizeof(float2) * std::max(na, nf), nullptr, s,
+                           p->bytes)) != kOk)
+            return st;
+        PFSHT_HIP_TRY(hipMemcpyAsync(p->c64_stage, in, sizeof(float2) * nin, hipMemcpyHostToDevice, s));
+        src = dst = p->c64_stage;
+    }
+    double2 *win = dir == 0 ? p->alm_stage : p->f_stage;
+    double2 *wout = dir == 0 ? p->f_stage : p->alm_stage;
+    launch_widen_c64(src, win, int64_t(nin), s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    st = dir == 0 ? synth_impl(*p, win, wout, s) : adjoint_impl(*p, win, wout, s);
+    if (st != kOk) return st;
+    launch_narrow_c128(wout, dst, int64_t(nout), s);
+    PFSHT_HIP_TRY(hipGetLastError());
+    if ((st = apply_end(*p, s)) != kOk) return st;
+    if (where == PFSHT_WHERE_HOST) {
+        PFSHT_HIP_TRY(hipMemcpyAsync(out, p->c64_stage, sizeof(float2) * nout, hipMemcpyDeviceToHost, s));
+        PFSHT_HIP_TRY(hipStreamSynchronize(s));
+    }
+    return kOk;
+}
+
+int pfsht_synth_f32(pfsht_plan_t p, const void *alm, void *f, int32_t where, void *stream) {
+    return apply_c64(p, 0, alm, f, where, stream);
+}
+
+int pfsht_adjoint_f32(pfsht_plan_t p, const void *f, void *alm, int32_t where, void *stream) {
+    return apply_c64(p, 1, f, alm, where, stream);
 }
 
 // The column apply shared by pfsht_synth_cols (dir 0) and pfsht_adjoint_cols (dir 1).

@@ -292,6 +292,9 @@ constexpr int kDotBlocks = 1024;
 void launch_weight_field(double2 *f, const double *w, int64_t M, int B, hipStream_t s);
 void launch_cdot(const double2 *a, const double2 *b, int64_t n, double2 *part, double2 *out, hipStream_t s);
 void launch_zaxpy(double2 *y, const double2 *x, double2 a, int64_t n, hipStream_t s);
+// ComplexF32 -> ComplexF64 and back (n complex elements)
+void launch_widen_c64(const float2 *in, double2 *out, int64_t n, hipStream_t s);
+void launch_narrow_c128(const double2 *in, float2 *out, int64_t n, hipStream_t s);
 void launch_zxpby(double2 *p, const double2 *r, double2 beta, int64_t n, hipStream_t s);
 void launch_add_prior(double2 *Ap, const double2 *p, const double *reg, int L, int B, hipStream_t s);
 void launch_spectrum(const double2 *alm, int L, double *C, hipStream_t s);

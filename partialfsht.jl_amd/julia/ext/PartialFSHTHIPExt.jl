@@ -13,6 +13,11 @@ coefficients and fields left in HBM.
     mul!(f, plan, alm)            # f, alm :: ROCArray{ComplexF64}
     mul!(alm, plan', f)
 
+Float32 nodes make a Float32 plan (`T = float(eltype(θ))`, `src/Plan.jl:104`) whose applies take and
+return `ROCArray{ComplexF32}`; at `tol >= 1e-6` it keeps the FP64 recurrence with FP32 grids. The
+leaf keywords `oversample_θ`, `oversample_φ`, `εapo_max` and `maxboxes` reach the library
+(`src/Plan.jl:120-135`).
+
 The NUFFT lives inside the library, so the `nufft` keyword only has to resolve
 (`src/NUFFT.jl:131-142`); `DirectSumBackend()` always does. Dispatch is on the node array type,
 through the leaf seam `_build_leaf` / `_synthesize!` / `_adjoint_apply!` (`src/Plan.jl:118-135,
@@ -36,7 +41,20 @@ const PFSHT_METHOD_DIRECT = Cint(1)
 const PFSHT_METHOD_STRETCH = Cint(2)
 const PFSHT_METHOD_MULTIBOX = Cint(3)
 const PFSHT_PREC_FP64 = Cint(0)
+const PFSHT_PREC_MIXED = Cint(1)
 const PFSHT_WHERE_DEVICE = Cint(1)
+
+struct PfshtPlanOpts          # pfsht_plan_opts, include/pfsht.h
+    method::Int32
+    precision::Int32
+    spin::Int32
+    maxboxes::Int32
+    oversample_theta::Float64
+    oversample_phi::Float64
+    epsapo_max::Float64
+    packed::Int32
+    reserved::Int32
+end
 
 struct PfshtInfo              # pfsht_info, include/pfsht.h
     method::Int32
@@ -99,19 +117,26 @@ mutable struct HIPMultiBoxLeaf <: AbstractHIPLeaf
     info::PfshtInfo
 end
 
-# pfsht_plan_create_spin: spin = 0 is the scalar plan; a nonzero spin builds SpinDirectLeaf /
+# pfsht_plan_create_opts: spin = 0 is the scalar plan; a nonzero spin builds SpinDirectLeaf /
 # StretchLeaf over SpinRings / per-box spin plans (src/Plan.jl:118-133), coefficients in the
-# spin_coeff_index layout (src/Layout.jl:28-29).
+# spin_coeff_index layout (src/Layout.jl:28-29). `opts` is the reference's leaf keyword tuple
+# (src/Plan.jl:108-110); a Float32 plan at tol >= 1e-6 keeps FP32 grids over the FP64 recurrence.
 function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVector, lmax::Int,
-                 ntrans::Int, tol::Real, spin::Integer = 0) where {LT<:AbstractHIPLeaf}
+                 ntrans::Int, tol::Real, spin::Integer = 0, opts = nothing) where {LT<:AbstractHIPLeaf}
     θh = Vector{Float64}(PartialFSHT._host(θ))
     φh = Vector{Float64}(PartialFSHT._host(φ))
     length(θh) == length(φh) || throw(DimensionMismatch("θ and φ must have the same length"))
+    T = float(promote_type(eltype(θ), eltype(φ)))
+    prec = (T === Float32 && tol >= 1e-6) ? PFSHT_PREC_MIXED : PFSHT_PREC_FP64
+    get_opt(k, d) = (opts !== nothing && hasproperty(opts, k)) ? getproperty(opts, k) : d
+    o = PfshtPlanOpts(method, prec, spin, Int32(get_opt(:maxboxes, 8)),
+                      Float64(get_opt(:oversample_θ, 1)), Float64(get_opt(:oversample_φ, 1)),
+                      Float64(get_opt(:εapo_max, 1.0)), Int32(0), Int32(0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    _check(ccall((:pfsht_plan_create_spin, libpfsht), Cint,
-                 (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Int32, Int32, Int32,
+    _check(ccall((:pfsht_plan_create_opts, libpfsht), Cint,
+                 (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Ref{PfshtPlanOpts},
                   Ref{Ptr{Cvoid}}),
-                 θh, φh, length(θh), lmax, tol, ntrans, spin, method, PFSHT_PREC_FP64, h))
+                 θh, φh, length(θh), lmax, Float64(tol), ntrans, o, h))
     info = Ref{PfshtInfo}()
     _check(ccall((:pfsht_plan_info, libpfsht), Cint, (Ptr{Cvoid}, Ref{PfshtInfo}), h[], info))
     leaf = LT(h[], info[])
@@ -122,12 +147,12 @@ function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVecto
     return leaf
 end
 
-HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0) =
-    _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol, spin)
+HIPStretchLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0, opts = nothing) =
+    _create(HIPStretchLeaf, PFSHT_METHOD_STRETCH, θ, φ, lmax, ntrans, tol, spin, opts)
 HIPDirectLeaf(θ, φ, lmax::Int, ntrans::Int, spin::Integer = 0) =
     _create(HIPDirectLeaf, PFSHT_METHOD_DIRECT, θ, φ, lmax, ntrans, 1e-10, spin)
-HIPMultiBoxLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0) =
-    _create(HIPMultiBoxLeaf, PFSHT_METHOD_MULTIBOX, θ, φ, lmax, ntrans, tol, spin)
+HIPMultiBoxLeaf(θ, φ, lmax::Int, ntrans::Int, tol::Real, spin::Integer = 0, opts = nothing) =
+    _create(HIPMultiBoxLeaf, PFSHT_METHOD_MULTIBOX, θ, φ, lmax, ntrans, tol, spin, opts)
 
 PartialFSHT.algorithm(::HIPStretchLeaf) = PartialFSHT.StretchAlgorithm()
 PartialFSHT.algorithm(::HIPDirectLeaf) = PartialFSHT.DirectAlgorithm()
@@ -138,18 +163,18 @@ PartialFSHT.ring_work(leaf::AbstractHIPLeaf) = Int(leaf.info.ring_work)
 # algorithm.
 PartialFSHT._build_leaf(::PartialFSHT.StretchAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
-    HIPStretchLeaf(θ, φ, lmax, ntrans, tol, spin)
+    HIPStretchLeaf(θ, φ, lmax, ntrans, tol, spin, opts)
 PartialFSHT._build_leaf(::PartialFSHT.DirectAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
     HIPDirectLeaf(θ, φ, lmax, ntrans, spin)
 # src/Plan.jl:131-132: the library repeats partition_boxes on the host copy of the nodes
 PartialFSHT._build_leaf(::PartialFSHT.MultiBoxAlgorithm, θ::ROCVector, φ::ROCVector, lmax, ntrans,
                         spin, tol, nufft_b, opts) =
-    HIPMultiBoxLeaf(θ, φ, lmax, ntrans, tol, spin)
+    HIPMultiBoxLeaf(θ, φ, lmax, ntrans, tol, spin, opts)
 
 _stream() = AMDGPU.stream().stream   # hipStream_t of the task-local AMDGPU stream
 
-function _device_ptr(x::ROCArray{ComplexF64})
+function _device_ptr(x::ROCArray{<:Union{ComplexF64, ComplexF32}})
     return reinterpret(Ptr{Cvoid}, pointer(x))
 end
 
@@ -166,6 +191,23 @@ end
 function PartialFSHT._adjoint_apply!(alm::ROCArray{ComplexF64}, f::ROCArray{ComplexF64},
                                      leaf::AbstractHIPLeaf, ::PartialFSHT.Plan)
     _check(ccall((:pfsht_adjoint, libpfsht), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                 leaf.handle, _device_ptr(f), _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream()))
+    return alm
+end
+
+# Float32 plans (gpu/runtests.jl:122-139): ComplexF32 coefficients and fields in HBM
+function PartialFSHT._synthesize!(f::ROCArray{ComplexF32}, alm::ROCArray{ComplexF32},
+                                  leaf::AbstractHIPLeaf, ::PartialFSHT.Plan)
+    _check(ccall((:pfsht_synth_f32, libpfsht), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+                 leaf.handle, _device_ptr(alm), _device_ptr(f), PFSHT_WHERE_DEVICE, _stream()))
+    return f
+end
+
+function PartialFSHT._adjoint_apply!(alm::ROCArray{ComplexF32}, f::ROCArray{ComplexF32},
+                                     leaf::AbstractHIPLeaf, ::PartialFSHT.Plan)
+    _check(ccall((:pfsht_adjoint_f32, libpfsht), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Cvoid}),
                  leaf.handle, _device_ptr(f), _device_ptr(alm), PFSHT_WHERE_DEVICE, _stream()))
     return alm
@@ -243,8 +285,10 @@ end
 # A host array handed to a device leaf is refused, as the reference refuses the converse
 # (ext/PartialFSHTKernelAbstractionsExt.jl:266-271).
 PartialFSHT._synthesize!(f, alm, ::AbstractHIPLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
-    "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
+    "a HIP plan applies to ROCArray{ComplexF64} or ROCArray{ComplexF32} coefficients and fields " *
+    "of one element type"))
 PartialFSHT._adjoint_apply!(alm, f, ::AbstractHIPLeaf, ::PartialFSHT.Plan) = throw(ArgumentError(
-    "a HIP plan applies to ROCArray{ComplexF64} coefficients and fields"))
+    "a HIP plan applies to ROCArray{ComplexF64} or ROCArray{ComplexF32} coefficients and fields " *
+    "of one element type"))
 
 end

@@ -90,6 +90,8 @@ EXPORTS = {
                                          C.c_void_p, C.POINTER(C.c_void_p)]),
     "pfsht_plan_estimate_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_int32,
                                             C.c_void_p, C.POINTER(C.c_int64)]),
+    "pfsht_synth_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "pfsht_adjoint_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_synth_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pfsht_adjoint_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pfsht_plan_destroy": (None, [C.c_void_p]),
@@ -205,14 +207,23 @@ def _column_major(x) -> bool:
     return True
 
 
-def _buf(x, count: int, what: str):
-    """(pointer, where) of a complex128 buffer of `count` elements in column-major order. A torch
-    tensor must be 1-D and contiguous, or hold its column-major layout (a row-major 2-D or 3-D
-    tensor would be read transposed, so it is refused)."""
+def _is_c64(x) -> bool:
+    """True for a complex64 (ComplexF32) array or tensor."""
     if _is_torch(x):
         import torch
-        if x.dtype != torch.complex128:
-            raise ArgumentError(f"{what} must be complex128")
+        return x.dtype == torch.complex64
+    return isinstance(x, np.ndarray) and x.dtype == np.complex64
+
+
+def _buf(x, count: int, what: str, c64: bool = False):
+    """(pointer, where) of a complex128 buffer (complex64 when c64) of `count` elements in
+    column-major order. A torch tensor must be 1-D and contiguous, or hold its column-major layout (a
+    row-major 2-D or 3-D tensor would be read transposed, so it is refused)."""
+    want = "complex64" if c64 else "complex128"
+    if _is_torch(x):
+        import torch
+        if x.dtype != (torch.complex64 if c64 else torch.complex128):
+            raise ArgumentError(f"{what} must be {want}")
         if x.dim() <= 1:
             if not x.is_contiguous():
                 raise ArgumentError(f"{what} must be contiguous")
@@ -222,8 +233,8 @@ def _buf(x, count: int, what: str):
         if x.numel() != count:
             raise DimensionMismatch(f"{what} has {x.numel()} elements, expected {count}")
         return C.c_void_p(x.data_ptr()), (WHERE_DEVICE if x.is_cuda else WHERE_HOST)
-    if not isinstance(x, np.ndarray) or x.dtype != np.complex128:
-        raise ArgumentError(f"{what} must be a complex128 numpy array or torch tensor")
+    if not isinstance(x, np.ndarray) or x.dtype != (np.complex64 if c64 else np.complex128):
+        raise ArgumentError(f"{what} must be a {want} numpy array or torch tensor")
     if x.size != count:
         raise DimensionMismatch(f"{what} has shape {x.shape}, expected {count} elements")
     if not (x.flags.f_contiguous or x.ndim <= 1):
@@ -333,13 +344,33 @@ def _weights(w, M: int, where: int):
     return C.c_void_p(wa.ctypes.data), wa
 
 
+def _is_f32(x) -> bool:
+    if _is_torch(x):
+        import torch
+        return x.dtype == torch.float32
+    return np.asarray(x).dtype == np.float32
+
+
 class Plan:
-    """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116."""
+    """Plan(theta, phi; lmax, tol=1e-10, ntrans=1, method) — src/Plan.jl:84-116.
+
+    As in the reference, the element type follows the nodes (T = float(eltype(theta))): float32
+    theta and phi make a Float32 plan, whose `@` products return complex64 (gpu/runtests.jl:122-139).
+    Its default precision is "mixed" (FP64 recurrence, FP32 grids) at tol >= 1e-6, else "fp64". Any
+    plan applies to complex128 arrays, and to complex64 ones through pfsht_synth_f32 /
+    pfsht_adjoint_f32."""
 
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
-                 method: str = "auto", precision: str = "fp64", spin: int = 0, maxboxes: int = 8,
+                 method: str = "auto", precision: str | None = None, spin: int = 0, maxboxes: int = 8,
                  oversample_theta: float = 1.0, oversample_phi: float = 1.0, epsapo_max: float = 1.0,
                  packed: bool = False):
+        self.float32 = _is_f32(theta) and _is_f32(phi)
+        if precision is None:
+            precision = "mixed" if self.float32 and tol >= 1e-6 else "fp64"
+        if _is_torch(theta):
+            theta = theta.detach().cpu().numpy()
+        if _is_torch(phi):
+            phi = phi.detach().cpu().numpy()
         th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64))
         ph = np.ascontiguousarray(np.asarray(phi, dtype=np.float64))
         if th.shape != ph.shape or th.ndim != 1:
@@ -352,6 +383,9 @@ class Plan:
             raise ArgumentError(f"(|spin| = {abs(spin)}) > lmax = {lmax}")
         self.lmax, self.ntrans, self.tol, self.npts = int(lmax), int(ntrans), float(tol), len(th)
         self.spin = int(spin)
+        if precision not in ("fp64", "mixed"):
+            raise ArgumentError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        self.precision = precision
         h = C.c_void_p()
         o = PlanOptsC(method=METHODS[method], precision={"fp64": 0, "mixed": 1}[precision], spin=self.spin,
                       maxboxes=int(maxboxes), oversample_theta=float(oversample_theta),
@@ -397,27 +431,32 @@ class Plan:
         return int(self.info["ring_work"])
 
     def synth(self, f, alm, stream=None):
-        """mul!(f, plan, alm): overwrites f."""
+        """mul!(f, plan, alm): overwrites f. complex64 alm and f take the ComplexF32 path."""
         L, B, M = self.lmax, self.ntrans, self.npts
         _assert_coeffs(alm, L, B)
         _assert_field(f, M, B)
-        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
-        pf, wf = _buf(f, M * B, "f")
+        c64 = _is_c64(alm)
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm", c64)
+        pf, wf = _buf(f, M * B, "f", c64)
         if wa != wf:
             raise ArgumentError("alm and f must both be host or both be device arrays")
-        _check(lib().pfsht_synth(self._h, pa, pf, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        fn = lib().pfsht_synth_f32 if c64 else lib().pfsht_synth
+        _check(fn(self._h, pa, pf, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
         return f
 
     def adjoint(self, alm, f, stream=None):
-        """mul!(alm, plan', f): overwrites alm, padding included."""
+        """mul!(alm, plan', f): overwrites alm, padding included. complex64 arrays take the
+        ComplexF32 path."""
         L, B, M = self.lmax, self.ntrans, self.npts
         _assert_coeffs(alm, L, B)
         _assert_field(f, M, B)
-        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm")
-        pf, wf = _buf(f, M * B, "f")
+        c64 = _is_c64(f)
+        pa, wa = _buf(alm, (L + 1) * (2 * L + 1) * B, "alm", c64)
+        pf, wf = _buf(f, M * B, "f", c64)
         if wa != wf:
             raise ArgumentError("alm and f must both be host or both be device arrays")
-        _check(lib().pfsht_adjoint(self._h, pf, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
+        fn = lib().pfsht_adjoint_f32 if c64 else lib().pfsht_adjoint
+        _check(fn(self._h, pf, pa, wa, _stream_ptr(stream) if wa else C.c_void_p(0)))
         return alm
 
     def _cols(self, ins, outs, nin: int, nout: int, stream):
@@ -452,11 +491,12 @@ class Plan:
     def __matmul__(self, alm):
         """plan * alm (src/Plan.jl:190-193): a new field, host or device like alm."""
         B = self.ntrans
+        c64 = _is_c64(alm)
         if _is_torch(alm):
             import torch
-            f = torch.empty(self.npts * B, dtype=torch.complex128, device=alm.device)
+            f = torch.empty(self.npts * B, dtype=torch.complex64 if c64 else torch.complex128, device=alm.device)
             return self.synth(f, alm)
-        f = np.zeros((self.npts,) if B == 1 else (self.npts, B), np.complex128, order="F")
+        f = np.zeros((self.npts,) if B == 1 else (self.npts, B), np.complex64 if c64 else np.complex128, order="F")
         return self.synth(f, alm)
 
     def set_timing(self, on: bool = True):
@@ -566,7 +606,14 @@ class AdjointPlan:
 
     def __matmul__(self, f):
         p = self.parent
+        if _is_torch(f):
+            import torch
+            alm = torch.zeros((p.lmax + 1) * (2 * p.lmax + 1) * p.ntrans,
+                              dtype=torch.complex64 if _is_c64(f) else torch.complex128, device=f.device)
+            return p.adjoint(alm, f)
         alm = zeros_alm(p.lmax, p.ntrans)
+        if _is_c64(f):
+            alm = alm.astype(np.complex64, order="F")
         return p.adjoint(alm, f)
 
 
