@@ -114,7 +114,7 @@ def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", choices=["plain", "batched", "cap30", "equatorial", "cap15at60", "spin", "twocaps",
-                                         "serve"],
+                                         "serve", "tune"],
                     default="plain",
                     help="plain: the 20 deg cap at lmax 2048, M 2e6 (default, the headline workload); "
                          "batched: the same cap at lmax 1024, M 5.1e5, ntrans 16, against one-at-a-time "
@@ -123,7 +123,8 @@ def main():
                          "around the equator, 15 deg around colatitude 60 deg) at lmax 2048, tol 1e-8; "
                          "these three also report the error against the Direct leaf on 1e4 points; "
                          "serve: a Poisson request stream over 32 recurring geometries through the "
-                         "serving runtime (plan cache, HBM admission, 4 streams), p50/p99 latency")
+                         "serving runtime (plan cache, HBM admission, 4 streams), p50/p99 latency; "
+                         "tune: the ring-pass shapes the plan-time tuner picks at lmax 256 / 1024 / 2048")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -158,6 +159,8 @@ def main():
         return run_geometry(args)
     if args.config == "serve":
         return run_serve(args)
+    if args.config == "tune":
+        return run_tune(args)
     return run_plain(args)
 
 
@@ -551,6 +554,73 @@ def run_serve(args):
                                   "{1e-8,1e-5}, synthesis/adjoint 50/50), 4 streams, 200 GB HBM budget; value "
                                   "= closed-loop transforms/s, latency from Poisson arrivals at 70% of capacity"},
            "served": r, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out))
+
+
+def run_tune(args):
+    """The plan-time tuner at lmax 256, 1024 and 2048 (20 deg polar caps, M = 4 (L theta*)^2 up to 2e6,
+    tol 1e-8, ntrans 1; and lmax 1024 at ntrans 16): the candidates each plan timed, the shape it
+    chose, and a synthesis + adjoint step with the chosen shape against the default shape."""
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    rows = []
+    for L, B in ((256, 1), (1024, 1), (2048, 1), (1024, 16)):
+        M = min(int(round(4 * (L * np.deg2rad(20.0)) ** 2)), 2_000_000)
+        rng = np.random.default_rng(L + B)
+        th, ph = cap_points(rng, 20, M)
+        pfsht.tune_clear()
+        tuned = pfsht.Plan(th, ph, lmax=L, tol=1e-8, ntrans=B, method="stretch", tune=1)
+        table = tuned.tune_table()
+        pfsht.tune_clear()
+        default = pfsht.Plan(th, ph, lmax=L, tol=1e-8, ntrans=B, method="stretch", tune=-1)
+        a = pfsht.to_device(np.asfortranarray(np.stack([rand_alm(L, rng) for _ in range(B)], axis=2)))
+        f = torch.randn(M * B, dtype=torch.complex128, device="cuda")
+        fo = torch.empty_like(f)
+        ao = torch.empty_like(a)
+        s = torch.cuda.current_stream()
+        plans = (("tuned", tuned), ("default", default))
+        for _, p in plans:
+            for _ in range(args.warmup):
+                p.synth(fo, a, s)
+                p.adjoint(ao, f, s)
+            p.set_timing(True)
+            p.stage_times("synth", reset=True)
+            p.stage_times("adjoint", reset=True)
+        # the two plans in alternating rounds, so that clock and cache state do not favour one
+        el = {name: 0.0 for name, _ in plans}
+        rounds = 4
+        for _ in range(rounds):
+            for name, p in plans:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(max(1, args.steps // rounds)):
+                    p.synth(fo, a, s)
+                    p.adjoint(ao, f, s)
+                torch.cuda.synchronize()
+                el[name] += time.perf_counter() - t0
+        nst = rounds * max(1, args.steps // rounds)
+        res = {}
+        for name, p in plans:
+            p.set_timing(False)
+            ss, sa = p.stage_times("synth"), p.stage_times("adjoint")
+            ring = ss["ring"][0] / max(ss["ring"][1], 1) + sa["ring"][0] / max(sa["ring"][1], 1) + \
+                sa["ring_reduce"][0] / max(sa["ring_reduce"][1], 1)
+            res[name] = {"shape": [p.info["seg_blocks"], p.info["adj_tiles"], p.info["mfma_tiles"]],
+                         "ms_per_step": el[name] / nst * 1e3, "ring_ms_per_step": ring}
+        row = {"lmax": L, "ntrans": B, "npts": M, "nrings": tuned.info["nth_half"],
+               "candidates": table, **res, "ring_speedup": res["default"]["ring_ms_per_step"] /
+               res["tuned"]["ring_ms_per_step"]}
+        log(f"lmax {L} ntrans {B}: shape {res['tuned']['shape']} ring {res['tuned']['ring_ms_per_step']:.4f} ms vs "
+            f"default {res['default']['shape']} {res['default']['ring_ms_per_step']:.4f} ms; step "
+            f"{res['tuned']['ms_per_step']:.4f} vs {res['default']['ms_per_step']:.4f} ms")
+        rows.append(row)
+        del tuned, default
+    out = {"metric": "ring-pass launch shape chosen by the plan-time tuner", "value": None, "unit": None,
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "20deg polar caps, tol 1e-8, shapes timed at plan time (tune = 1)"},
+           "rows": rows, "roofline": None, "cpu_baseline": None}
     print(json.dumps(out))
 
 

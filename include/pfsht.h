@@ -67,6 +67,7 @@ typedef struct {
     int32_t es_rows;   /* fine-grid rows the apply stores: those the points' kernels reach */
     int32_t nboxes;    /* boxes of a multi-box plan (1 otherwise) */
     int32_t spin;      /* spin weight (0: scalar) */
+    int32_t seg_blocks, adj_tiles, mfma_tiles; /* the ring passes' launch shape (pfsht_ring_shape) */
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */
@@ -108,9 +109,11 @@ typedef struct {
     double oversample_phi;   /* 0: 1 */
     double epsapo_max;       /* 0: 1.0 */
     int32_t packed;          /* nonzero: FFT plans for partial transform counts */
-    int32_t reserved;        /* 0 */
+    int32_t tune;            /* ring-pass shape: 0 time candidates when the shape is not cached,
+                                -1 never time (cached shape or the default), 1 always time */
 } pfsht_plan_opts;
-/* Fills *opts with the defaults (all fields zero). */
+/* Fills *opts with the defaults (all fields zero). `tune` has no counterpart in the reference, whose
+ * ring pass has no launch shape: see pfsht_ring_shape. */
 void pfsht_plan_opts_default(pfsht_plan_opts *opts);
 /* Plan(θ, φ; lmax, tol, ntrans, keywords...) with every keyword of src/Plan.jl:84-97 (opts may be
  * NULL for the defaults). oversample_θ, oversample_φ >= 1 and εapo_max in (0, 1] (src/Windows.jl). */
@@ -124,6 +127,36 @@ int pfsht_plan_estimate_bytes(const double *theta, const double *phi, int64_t np
                               int32_t ntrans, const pfsht_plan_opts *opts, int64_t *bytes);
 void pfsht_plan_destroy(pfsht_plan_t plan);
 int pfsht_plan_info(pfsht_plan_t plan, pfsht_info *info);
+
+/* The launch shape of a plan's ring passes (no counterpart in the reference, whose ring pass is a
+ * threaded loop over orders, src/Legendre.jl:284-341). A stretched scalar plan times candidate shapes
+ * on its own ring engine when it is built (a few milliseconds; opts->tune) and keeps the fastest;
+ * the choice is cached per (device architecture, lmax, rings, largest order, ntrans), so later plans
+ * of the shape reuse it. PFSHT_TUNE_CACHE names a file the cache is read from once per process and
+ * written back to after each new timing. */
+typedef struct {
+    int32_t seg_blocks; /* target of 64-degree blocks per degree segment (1..64; 0: default 3) */
+    int32_t adj_tiles;  /* 64-ring tiles per wavefront of the single-transform adjoint (1, 2; 0: default) */
+    int32_t mfma_tiles; /* 64-ring tiles per wavefront of the FP64 MFMA adjoint (1, 2; 0: default) */
+    int32_t reserved;
+    double ms;          /* ring forward + adjoint time of the shape when it was timed */
+} pfsht_ring_shape;
+/* The cache, host only (no device work): the shape stored under a key (*found = 0 when none), a new
+ * entry, merging and writing cache files, clearing, and the number of entries. arch is the device's
+ * architecture and compute-unit count as plans record it, e.g. "gfx950:256". */
+int pfsht_tune_get(const char *arch, int32_t lmax, int32_t nrings, int32_t mmax, int32_t ntrans,
+                   pfsht_ring_shape *shape, int32_t *found);
+int pfsht_tune_put(const char *arch, int32_t lmax, int32_t nrings, int32_t mmax, int32_t ntrans,
+                   const pfsht_ring_shape *shape);
+int pfsht_tune_load(const char *path);
+int pfsht_tune_save(const char *path);
+void pfsht_tune_clear(void);
+int64_t pfsht_tune_size(void);
+/* The candidates a plan timed when it was built (*n = 0 when its shape came from the cache or the
+ * default), at most cap of them; and the key its ring engine is cached under (arch: cap bytes). */
+int pfsht_plan_tune_table(pfsht_plan_t plan, pfsht_ring_shape *rows, int32_t cap, int32_t *n);
+int pfsht_plan_tune_key(pfsht_plan_t plan, char *arch, int32_t cap, int32_t *lmax, int32_t *nrings, int32_t *mmax,
+                        int32_t *ntrans);
 
 /* mul!(f, plan, alm) — src/Plan.jl:137-141 -> synthesize_stretch! (src/Stretch.jl:262-277).
  * Overwrites f. stream: a hipStream_t or NULL for the null stream; the call is asynchronous

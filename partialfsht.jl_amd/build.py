@@ -21,7 +21,7 @@ LIB = os.path.join(LIBDIR, "libpfsht_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["geometry.cpp", "ring_kernels.hip", "grid_kernels.hip", "nufft_kernels.hip", "direct_kernels.hip",
            "analysis_kernels.hip", "plan_kernels.hip", "spin_kernels.hip", "capi.hip",
-           "serve.hip"]
+           "serve.hip", "tune_cache.cpp"]
 
 # No packed FP32 instructions anywhere; FP64 FMA contraction is left on for the kernels.
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -15,6 +15,7 @@
 
 #include "../../include/pfsht.h"
 #include "pfsht_common.hpp"
+#include "tune_cache.hpp"
 
 namespace pfsht {
 
@@ -188,6 +189,7 @@ struct pfsht_rings_s {
     int64_t work = 0;
     int64_t bytes = 0;
     int64_t part_bytes = 0;                   // the adjoint ring pass's partial sums
+    RingShape shape;                          // the launch shape it was built with
     double2 *Gd = nullptr;                    // [B][2 mmax + 1][n]
     double2 *alm_stage = nullptr, *G_stage = nullptr;  // for host-memory calls
     std::mutex mu;
@@ -201,7 +203,8 @@ struct pfsht_rings_s {
     }
 };
 
-static Status rings_build(const double *theta, int n, int L, double eps, int B, bool reflect, pfsht_rings_s &r) {
+static Status rings_build(const double *theta, int n, int L, double eps, int B, bool reflect, pfsht_rings_s &r,
+                          const RingShape &shape = RingShape()) {
     if (L < 0) { set_error("lmax must be nonnegative"); return kEArg; }
     if (B < 1) { set_error("ntrans must be >= 1"); return kEArg; }
     if (n < 1) { set_error("need at least one ring"); return kEArg; }
@@ -246,19 +249,23 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
         if (ntmax <= 4) mw = ntmax;
         else if (ntmax <= 8) { mw = 4; mtg = 2; }
         else { mw = 8; mtg = 2; }
+        // a tuned shape: one tile per wavefront (up to eight wavefronts), or two
+        if (shape.mfma_tg == 1 && ntmax <= 8) { mw = ntmax; mtg = 1; }
+        else if (shape.mfma_tg == 2) { mw = (ntmax + 1) / 2; mtg = 2; }
     }
     // partials serve the adjoints the MFMA path does not: single transforms when it exists
     const int pb = mfma_adj ? 1 : B;
-    const int tg = (B <= 2 || mfma_adj) ? 2 : 1;
+    // one or two tiles per wavefront for up to two transforms (the kernels with two are built for
+    // those counts only); a tuned shape may take one
+    int tg = (B <= 2 || mfma_adj) ? 2 : 1;
+    if (tg == 2 && shape.adj_tg == 1) tg = 1;
     std::vector<std::vector<SegItem>> tiles_of(mmax + 1);
     std::vector<int32_t> ntiles(mmax + 1, 0);
     std::vector<int64_t> pofs(mmax + 1, 0), cost(mmax + 1, 0);
     int64_t np = 0, nck = 0;
     r.work = 0;
     int fw = 1;
-    // PFSHT_SEG_BLOCKS (1..64) overrides the target segment length, for tuning
-    int per = kSegBlocks;
-    if (const char *e = std::getenv("PFSHT_SEG_BLOCKS")) per = std::min(64, std::max(1, std::atoi(e)));
+    const int per = shape.per > 0 ? std::min(64, shape.per) : kSegBlocks;
     for (int m = 0; m <= mmax; ++m) {
         pofs[m] = np;
         const int lo = r.jlo[m], hi = r.jhi[m];
@@ -342,6 +349,10 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
             }
         mitems = interleave(mq, 1, OrderSeg{0, 1, 0, 0, 0, 0, 0});
     }
+    r.shape = shape;
+    r.shape.per = per;
+    r.shape.adj_tg = tg;
+    r.shape.mfma_tg = mfma_adj ? mtg : 0;
     d.mfma_waves = mw;
     d.mfma_tg = mtg;
     d.part_b = pb;
@@ -379,6 +390,102 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     PFSHT_HIP_TRY(hipStreamSynchronize(nullptr));
     return kOk;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Plan-time tuning of the ring passes' launch shape (cached in tune_cache.cpp).
+// ---------------------------------------------------------------------------------------------
+// "gfx950:256": the architecture without its feature suffix, and the compute units
+static std::string device_arch(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    const size_t c = a.find(':');
+    if (c != std::string::npos) a.resize(c);
+    return a + ":" + std::to_string(prop.multiProcessorCount);
+}
+
+// Time candidate shapes on temporary engines of the ring set: the default first, the segment
+// target over {2, 4, 6} next, then, at the best target, the other tile count per wavefront of the
+// adjoint that the engine uses for its B transforms. A candidate replaces the best only when it is
+// 3% faster (timing noise keeps the default). Score: the mean time of a forward plus an adjoint ring
+// pass over all B transforms, on zero inputs (the passes' work does not depend on the values).
+static Status tune_rings(const double *theta, int n, int L, double eps, int B, bool reflect, RingShape &best,
+                         std::vector<RingShape> &table) {
+    const size_t LB = size_t(L + 1) * (2 * L + 1);
+    struct Res {
+        double2 *alm = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Res() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+            dfree(alm);
+        }
+    } res;
+    int64_t acct = 0;
+    Status st = dalloc(&res.alm, LB * size_t(B), acct);
+    if (st != kOk) return st;
+    PFSHT_HIP_TRY(hipStreamCreateWithFlags(&res.s, hipStreamNonBlocking));
+    PFSHT_HIP_TRY(hipEventCreate(&res.e0));
+    PFSHT_HIP_TRY(hipEventCreate(&res.e1));
+    PFSHT_HIP_TRY(hipMemsetAsync(res.alm, 0, sizeof(double2) * LB * size_t(B), res.s));
+    const double work = double(n) * L * L;
+    const int reps = int(std::min(20.0, std::max(2.0, 2e8 / std::max(work, 1.0))));
+    const InCols in = in_cols(res.alm, int64_t(LB));
+    const OutCols out = out_cols(res.alm, int64_t(LB));
+    auto time_shape = [&](const RingShape &want, RingShape &got) -> Status {
+        pfsht_rings_s r;
+        Status st2 = rings_build(theta, n, L, eps, B, reflect, r, want);
+        if (st2 != kOk) return st2;
+        launch_ring_fwd(r.d, in, B, r.Gd, res.s);  // warm-up
+        launch_ring_adj(r.d, r.Gd, out, B, res.s);
+        PFSHT_HIP_TRY(hipEventRecord(res.e0, res.s));
+        for (int i = 0; i < reps; ++i) {
+            launch_ring_fwd(r.d, in, B, r.Gd, res.s);
+            launch_ring_adj(r.d, r.Gd, out, B, res.s);
+        }
+        PFSHT_HIP_TRY(hipEventRecord(res.e1, res.s));
+        PFSHT_HIP_TRY(hipGetLastError());
+        PFSHT_HIP_TRY(hipEventSynchronize(res.e1));
+        float ms = 0;
+        PFSHT_HIP_TRY(hipEventElapsedTime(&ms, res.e0, res.e1));
+        got = r.shape;
+        got.ms = double(ms) / reps;
+        table.push_back(got);
+        return kOk;
+    };
+    RingShape got;
+    if ((st = time_shape(RingShape(), best)) != kOk) return st;
+    auto consider = [&](const RingShape &want) -> Status {
+        for (const RingShape &t : table)  // the engine may not take the shape asked for
+            if (t.per == want.per && (want.adj_tg == 0 || t.adj_tg == want.adj_tg) &&
+                (want.mfma_tg == 0 || t.mfma_tg == want.mfma_tg))
+                return kOk;
+        Status st3 = time_shape(want, got);
+        if (st3 != kOk) return st3;
+        if (got.ms < 0.97 * best.ms) best = got;
+        return kOk;
+    };
+    for (int per : {2, 4, 6}) {
+        RingShape w;
+        w.per = per;
+        if ((st = consider(w)) != kOk) return st;
+    }
+    RingShape w;
+    w.per = best.per;
+    if (best.mfma_tg == 0 && B <= 2 && best.adj_tg == 2) w.adj_tg = 1;  // the single-transform adjoint
+    else if (best.mfma_tg == 2) w.mfma_tg = 1;                         // the MFMA adjoint
+    else if (best.mfma_tg == 1) w.mfma_tg = 2;
+    if ((w.adj_tg || w.mfma_tg) && (st = consider(w)) != kOk) return st;
+    return kOk;
+}
+
+// The shape a plan's ring engine is built with: the cached one for its key, else (mode >= 0, and an
+// engine large enough for the timing to matter) the fastest timed candidate, recorded in the cache;
+// mode 1 times even when the key is cached, mode -1 never times.
+static Status choose_ring_shape(pfsht_plan_s &p, const double *theta, int n, int L, double eps, int B, bool reflect,
+                                int mode, RingShape &shape);
 
 // SpinRings (src/Spin.jl:192-270) on the device: the half-angle logarithms per ring (swapped, and
 // cos theta negated, for a reflected ring set, so the spin weight stays as given), retained orders
@@ -520,6 +627,8 @@ struct pfsht_plan_s {
     Geom geom;
     int flip = 0, nh = 0, nth = 0, nph = 0, mmax = 0;
     pfsht_rings_s *rings = nullptr;
+    TuneKey tkey;                        // the ring engine's tuning key (L = 0: not a tuned engine)
+    std::vector<RingShape> tune_table;   // the candidates timed when the plan was built
     double *wth = nullptr;
     NufftDev nu;
     // Ft: the periodised grid, either in order-column layout [b][m + mmax][i] (the apply) or
@@ -620,6 +729,29 @@ static Status apply_end(pfsht_plan_s &p, hipStream_t s) {
     PFSHT_HIP_TRY(hipEventRecord(p.done, s));
     p.has_done = true;
     p.last_stream = s;
+    return kOk;
+}
+
+static Status choose_ring_shape(pfsht_plan_s &p, const double *theta, int n, int L, double eps, int B, bool reflect,
+                                int mode, RingShape &shape) {
+    TuneKey key;
+    key.arch = device_arch(p.dev);
+    key.L = L;
+    key.n = n;
+    for (int j = 0; j < n; ++j) key.mmax = std::max(key.mmax, polar_mmax(theta[j], L, eps));
+    key.B = B;
+    p.tkey = key;
+    tune_load_env();
+    if (mode != 1 && tune_lookup(key, shape)) return kOk;
+    if (mode < 0 || double(n) * L * L < 2e6) {  // small engines keep the default: timing costs more than it saves
+        shape = RingShape();
+        return kOk;
+    }
+    RingShape best;
+    Status st = tune_rings(theta, n, L, eps, B, reflect, best, p.tune_table);
+    if (st != kOk) return st;
+    tune_store(key, best);
+    shape = best;
     return kOk;
 }
 
@@ -738,9 +870,14 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     std::vector<double> th, Th;
     stretch_rings(p.geom, p.nh, th, Th);
     if (!std::is_sorted(th.begin(), th.end())) { set_error("ring colatitudes are not ascending"); return kEArg; }
+    RingShape shape;
+    if (!spin) {
+        Status st0 = choose_ring_shape(p, th.data(), p.nh, L, tol, B, p.flip != 0, o.tune, shape);
+        if (st0 != kOk) return st0;
+    }
     p.rings = new pfsht_rings_s;
     Status st = spin ? rings_build_spin(th.data(), p.nh, L, spin, tol, B, p.flip != 0, *p.rings)
-                     : rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings);
+                     : rings_build(th.data(), p.nh, L, tol, B, p.flip != 0, *p.rings, shape);
     if (st != kOk) return st;
     p.mmax = p.rings->d.mmax;
     p.nph = int(next_smooth_even(int64_t(std::ceil(double(2 * int64_t(std::max(p.mmax, std::abs(spin))) + 1) *
@@ -1401,6 +1538,32 @@ int pfsht_solve(pfsht_plan_t p, const void *f, const double *w, int32_t mode, do
 }
 
 
+int pfsht_plan_tune_table(pfsht_plan_t p, pfsht_ring_shape *rows, int32_t cap, int32_t *n) {
+    if (!p || !n || (cap > 0 && !rows)) { set_error("NULL argument"); return kEArg; }
+    const int k = int(std::min<size_t>(p->tune_table.size(), size_t(std::max(cap, 0))));
+    for (int i = 0; i < k; ++i) {
+        const RingShape &t = p->tune_table[i];
+        rows[i].seg_blocks = t.per;
+        rows[i].adj_tiles = t.adj_tg;
+        rows[i].mfma_tiles = t.mfma_tg;
+        rows[i].reserved = 0;
+        rows[i].ms = t.ms;
+    }
+    *n = k;
+    return kOk;
+}
+
+int pfsht_plan_tune_key(pfsht_plan_t p, char *arch, int32_t cap, int32_t *lmax, int32_t *nrings, int32_t *mmax,
+                        int32_t *ntrans) {
+    if (!p || !arch || cap < 1 || !lmax || !nrings || !mmax || !ntrans) { set_error("NULL argument"); return kEArg; }
+    std::snprintf(arch, size_t(cap), "%s", p->tkey.arch.c_str());
+    *lmax = p->tkey.L;
+    *nrings = p->tkey.n;
+    *mmax = p->tkey.mmax;
+    *ntrans = p->tkey.B;
+    return kOk;
+}
+
 const char *pfsht_last_error(void) { return g_error.c_str(); }
 const char *pfsht_version(void) { return "pfsht-hip 0.1.0 (gfx950)"; }
 
@@ -1447,6 +1610,7 @@ static Status opts_normalise(const pfsht_plan_opts *opts, double tol, pfsht_plan
         return kEArg;
     }
     if (!(o.epsapo_max > 0) || !(o.epsapo_max <= 1)) { set_error("epsapo_max must lie in (0, 1]"); return kEArg; }
+    if (o.tune < -1 || o.tune > 1) { set_error("tune must be -1, 0 or 1"); return kEArg; }
     return kOk;
 }
 
@@ -1528,6 +1692,11 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
     info->es_n2 = p->nu.N2;
     info->npts = p->M;
     info->ring_work = p->rings ? p->rings->work : 0;
+    if (p->rings) {
+        info->seg_blocks = p->rings->shape.per;
+        info->adj_tiles = p->rings->shape.adj_tg;
+        info->mfma_tiles = p->rings->shape.mfma_tg;
+    }
     info->device_bytes = p->bytes + (p->rings ? p->rings->bytes : 0);
     for (auto *q : p->boxes) {
         info->ring_work += q->rings ? q->rings->work : 0;

@@ -53,7 +53,7 @@ struct PfshtPlanOpts          # pfsht_plan_opts, include/pfsht.h
     oversample_phi::Float64
     epsapo_max::Float64
     packed::Int32
-    reserved::Int32
+    tune::Int32
 end
 
 struct PfshtInfo              # pfsht_info, include/pfsht.h
@@ -78,6 +78,9 @@ struct PfshtInfo              # pfsht_info, include/pfsht.h
     es_rows::Int32
     nboxes::Int32
     spin::Int32
+    seg_blocks::Int32
+    adj_tiles::Int32
+    mfma_tiles::Int32
 end
 
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))

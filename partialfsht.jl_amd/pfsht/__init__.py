@@ -50,13 +50,19 @@ class InfoC(C.Structure):
                 ("mmax", C.c_int32), ("es_w", C.c_int32), ("es_n1", C.c_int32),
                 ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double),
-                ("es_rows", C.c_int32), ("nboxes", C.c_int32), ("spin", C.c_int32)]
+                ("es_rows", C.c_int32), ("nboxes", C.c_int32), ("spin", C.c_int32),
+                ("seg_blocks", C.c_int32), ("adj_tiles", C.c_int32), ("mfma_tiles", C.c_int32)]
+
+
+class RingShapeC(C.Structure):
+    _fields_ = [("seg_blocks", C.c_int32), ("adj_tiles", C.c_int32), ("mfma_tiles", C.c_int32),
+                ("reserved", C.c_int32), ("ms", C.c_double)]
 
 
 class PlanOptsC(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("spin", C.c_int32), ("maxboxes", C.c_int32),
                 ("oversample_theta", C.c_double), ("oversample_phi", C.c_double), ("epsapo_max", C.c_double),
-                ("packed", C.c_int32), ("reserved", C.c_int32)]
+                ("packed", C.c_int32), ("tune", C.c_int32)]
 
 
 class BoxC(C.Structure):
@@ -90,6 +96,16 @@ EXPORTS = {
                                          C.c_void_p, C.POINTER(C.c_void_p)]),
     "pfsht_plan_estimate_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_int32,
                                             C.c_void_p, C.POINTER(C.c_int64)]),
+    "pfsht_tune_get": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                 C.POINTER(C.c_int32)]),
+    "pfsht_tune_put": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
+    "pfsht_tune_load": (C.c_int, [C.c_char_p]),
+    "pfsht_tune_save": (C.c_int, [C.c_char_p]),
+    "pfsht_tune_clear": (None, []),
+    "pfsht_tune_size": (C.c_int64, []),
+    "pfsht_plan_tune_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "pfsht_plan_tune_key": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "pfsht_synth_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_adjoint_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_synth_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -325,6 +341,40 @@ def plan_estimate_bytes(theta, phi, lmax: int, tol: float = 1e-10, ntrans: int =
     return int(out.value)
 
 
+def tune_get(arch: str, lmax: int, nrings: int, mmax: int, ntrans: int):
+    """The ring-pass shape cached under the key, or None (pfsht_tune_get; host only)."""
+    sh = RingShapeC()
+    found = C.c_int32()
+    _check(lib().pfsht_tune_get(arch.encode(), int(lmax), int(nrings), int(mmax), int(ntrans), C.byref(sh),
+                                C.byref(found)))
+    if not found.value:
+        return None
+    return {k: getattr(sh, k) for k in ("seg_blocks", "adj_tiles", "mfma_tiles", "ms")}
+
+
+def tune_put(arch: str, lmax: int, nrings: int, mmax: int, ntrans: int, seg_blocks: int = 0, adj_tiles: int = 0,
+             mfma_tiles: int = 0, ms: float = 0.0):
+    """Record a ring-pass shape under the key (pfsht_tune_put; host only)."""
+    sh = RingShapeC(seg_blocks=int(seg_blocks), adj_tiles=int(adj_tiles), mfma_tiles=int(mfma_tiles), ms=float(ms))
+    _check(lib().pfsht_tune_put(arch.encode(), int(lmax), int(nrings), int(mmax), int(ntrans), C.byref(sh)))
+
+
+def tune_load(path: str):
+    _check(lib().pfsht_tune_load(str(path).encode()))
+
+
+def tune_save(path: str):
+    _check(lib().pfsht_tune_save(str(path).encode()))
+
+
+def tune_clear():
+    lib().pfsht_tune_clear()
+
+
+def tune_size() -> int:
+    return int(lib().pfsht_tune_size())
+
+
 def _weights(w, M: int, where: int):
     """(pointer, keep-alive) of npts float64 weights on the same side as the other arrays."""
     if w is None:
@@ -363,7 +413,7 @@ class Plan:
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
                  method: str = "auto", precision: str | None = None, spin: int = 0, maxboxes: int = 8,
                  oversample_theta: float = 1.0, oversample_phi: float = 1.0, epsapo_max: float = 1.0,
-                 packed: bool = False):
+                 packed: bool = False, tune: int = 0):
         self.float32 = _is_f32(theta) and _is_f32(phi)
         if precision is None:
             precision = "mixed" if self.float32 and tol >= 1e-6 else "fp64"
@@ -389,7 +439,8 @@ class Plan:
         h = C.c_void_p()
         o = PlanOptsC(method=METHODS[method], precision={"fp64": 0, "mixed": 1}[precision], spin=self.spin,
                       maxboxes=int(maxboxes), oversample_theta=float(oversample_theta),
-                      oversample_phi=float(oversample_phi), epsapo_max=float(epsapo_max), packed=int(bool(packed)))
+                      oversample_phi=float(oversample_phi), epsapo_max=float(epsapo_max), packed=int(bool(packed)),
+                      tune=int(tune))
         _check(lib().pfsht_plan_create_opts(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
                                             len(th), self.lmax, self.tol, self.ntrans, C.byref(o), C.byref(h)))
         self._h = h
@@ -415,6 +466,21 @@ class Plan:
                "multibox": "MultiBoxAlgorithm"}[self.algorithm]
         return (f"pfsht.Plan(lmax={self.lmax}, npts={self.npts}, ntrans={self.ntrans}, spin={self.spin}, "
                 f"algorithm={alg}, kind={self.info['kind_name']})")
+
+    def tune_table(self):
+        """The ring-pass shapes timed when the plan was built (empty when its shape came from the
+        cache or the default): dicts with seg_blocks, adj_tiles, mfma_tiles and ms."""
+        rows = (RingShapeC * 16)()
+        n = C.c_int32()
+        _check(lib().pfsht_plan_tune_table(self._h, rows, 16, C.byref(n)))
+        return [{k: getattr(rows[i], k) for k in ("seg_blocks", "adj_tiles", "mfma_tiles", "ms")} for i in range(n.value)]
+
+    def tune_key(self):
+        """(arch, lmax, nrings, mmax, ntrans): the key the plan's ring shape is cached under."""
+        arch = C.create_string_buffer(128)
+        v = [C.c_int32() for _ in range(4)]
+        _check(lib().pfsht_plan_tune_key(self._h, arch, 128, *[C.byref(x) for x in v]))
+        return (arch.value.decode(), *[x.value for x in v])
 
     @property
     def algorithm(self) -> str:
