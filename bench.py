@@ -114,7 +114,7 @@ def cpu_baseline(th, ph, L, tol, alm, f, nthreads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", choices=["plain", "batched", "cap30", "equatorial", "cap15at60", "spin", "twocaps",
-                                         "serve", "tune"],
+                                         "serve", "tune", "smallplans"],
                     default="plain",
                     help="plain: the 20 deg cap at lmax 2048, M 2e6 (default, the headline workload); "
                          "batched: the same cap at lmax 1024, M 5.1e5, ntrans 16, against one-at-a-time "
@@ -124,7 +124,8 @@ def main():
                          "these three also report the error against the Direct leaf on 1e4 points; "
                          "serve: a Poisson request stream over 32 recurring geometries through the "
                          "serving runtime (plan cache, HBM admission, 4 streams), p50/p99 latency; "
-                         "tune: the ring-pass shapes the plan-time tuner picks at lmax 256 / 1024 / 2048")
+                         "tune: the ring-pass shapes the plan-time tuner picks at lmax 256 / 1024 / 2048; "
+                         "smallplans: 64 small plans (lmax 256) as one plan group against one at a time")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -161,6 +162,8 @@ def main():
         return run_serve(args)
     if args.config == "tune":
         return run_tune(args)
+    if args.config == "smallplans":
+        return run_smallplans(args)
     return run_plain(args)
 
 
@@ -554,6 +557,80 @@ def run_serve(args):
                                   "{1e-8,1e-5}, synthesis/adjoint 50/50), 4 streams, 200 GB HBM budget; value "
                                   "= closed-loop transforms/s, latency from Poisson arrivals at 70% of capacity"},
            "served": r, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out))
+
+
+def run_smallplans(args):
+    """64 small plans (lmax 256, caps of radius U(5, 20) deg around random centres, M = 4 (L theta*)^2
+    area-uniform points, tol 1e-8, the planner's method, ntrans 1): one synthesis and one adjoint of
+    every plan per step, applied one at a time on one stream against the same applies as one plan
+    group (pfsht_group_*: the ring passes of all 64 in one launch over a (plan, order, ring tile) work
+    list, the other stages on concurrent streams, replayed from hipGraphs). value = applies/s of the
+    group."""
+    import torch
+    import pfsht
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L, n = 256, 64
+    rng = np.random.default_rng(64)
+    plans, alms, fs = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(n):
+        deg = rng.uniform(5.0, 20.0)
+        tc, pc = np.arccos(rng.uniform(-1.0, 1.0)), rng.uniform(0.0, 2 * np.pi)
+        M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
+        th, ph = centered_cap(rng, deg, M, tc, pc)
+        plans.append(pfsht.Plan(th, ph, lmax=L, tol=1e-8, method="auto"))
+        alms.append(pfsht.to_device(rand_alm(L, rng)))
+        fs.append(torch.randn(M, dtype=torch.complex128, device="cuda"))
+    torch.cuda.synchronize()
+    t_plans = time.perf_counter() - t0
+    fo = [torch.empty_like(f) for f in fs]
+    ao = [torch.empty_like(a) for a in alms]
+    g = pfsht.Group(plans)
+    s = torch.cuda.current_stream()
+    log(f"{n} plans in {t_plans:.1f}s ({sum(p.algorithm == 'direct' for p in plans)} Direct), "
+        f"{sum(p.npts for p in plans)} points")
+
+    def one_at_a_time():
+        for p, a, f, x, y in zip(plans, alms, fs, fo, ao):
+            p.synth(x, a, s)
+            p.adjoint(y, f, s)
+
+    def grouped():
+        g.synth(fo, alms, s)
+        g.adjoint(ao, fs, s)
+
+    for _ in range(args.warmup):
+        one_at_a_time()
+        grouped()
+    torch.cuda.synchronize()
+    el = {"one_at_a_time": 0.0, "group": 0.0}
+    rounds = 4
+    per = max(1, args.steps // rounds)
+    for _ in range(rounds):  # alternating rounds
+        for name, fn in (("one_at_a_time", one_at_a_time), ("group", grouped)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(per):
+                fn()
+            torch.cuda.synchronize()
+            el[name] += time.perf_counter() - t0
+    steps = rounds * per
+    rate = {k: 2 * n * steps / v for k, v in el.items()}
+    log(f"one at a time {rate['one_at_a_time']:.0f} applies/s, group {rate['group']:.0f} applies/s, ratio "
+        f"{rate['group'] / rate['one_at_a_time']:.2f}")
+    out = {"metric": "applies/s over 64 small plans (lmax 256, caps 5-20 deg, tol 1e-8), synth+adjoint",
+           "value": rate["group"], "unit": "applies/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": el["group"] / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "64 small plans: lmax 256, cap radius U(5,20) deg, random centres, M = 4 (L theta*)^2, "
+                                  "tol 1e-8, ntrans 1, one synthesis + one adjoint of each per step",
+                      "plans": n, "direct_plans": sum(p.algorithm == "direct" for p in plans),
+                      "points": sum(p.npts for p in plans)},
+           "one_at_a_time_applies_s": rate["one_at_a_time"],
+           "group_over_one_at_a_time": rate["group"] / rate["one_at_a_time"],
+           "one_at_a_time_ms_per_step": el["one_at_a_time"] / steps * 1e3,
+           "roofline": None, "cpu_baseline": None}
     print(json.dumps(out))
 
 

@@ -188,6 +188,23 @@ int pfsht_synth_cols(pfsht_plan_t plan, int32_t nb, const void *const *alm_cols,
 int pfsht_adjoint_cols(pfsht_plan_t plan, int32_t nb, const void *const *f_cols, void *const *alm_cols,
                        void *stream);
 
+/* Plan groups: `mul!` over many independent single-transform plans at once — the farmed apply of
+ * independent plans in the reference (ext/PartialFSHTOhMyThreadsExt.jl:151-159 over
+ * src/Plan.jl:201-219), for many small plans that one at a time leave most of the GPU idle. The
+ * ring passes of all members run as one launch over a (plan, order, 64-ring tile) work list; the
+ * members' grid, FFT and point stages run concurrently on the group's streams; each direction is
+ * replayed from a hipGraph after its first call. Members: scalar stretched or Direct plans with
+ * ntrans = 1 on one device (others: PFSHT_E_UNSUPPORTED); a plan belongs to at most one group in
+ * use at a time, and must not be applied on its own while a group apply is in flight.
+ * alm[i] and f[i] are member i's device buffers, in its layouts; the results equal the members'
+ * own applies. The pointer arrays are host memory read before the call returns; the call is
+ * asynchronous on stream. */
+typedef struct pfsht_group_s *pfsht_group_t;
+int pfsht_group_create(const pfsht_plan_t *plans, int32_t nplans, pfsht_group_t *out);
+void pfsht_group_destroy(pfsht_group_t group);
+int pfsht_group_synth(pfsht_group_t group, const void *const *alm, void *const *f, void *stream);
+int pfsht_group_adjoint(pfsht_group_t group, const void *const *f, void *const *alm, void *stream);
+
 /* RingLegendre(theta, lmax, eps; ntrans, reflect) — src/Legendre.jl:194-255. theta sorted
  * ascending, host memory. */
 int pfsht_rings_create(const double *theta, int32_t nrings, int32_t lmax, double eps, int32_t ntrans,

@@ -1850,6 +1850,268 @@ int pfsht_adjoint_cols(pfsht_plan_t p, int32_t nb, const void *const *f_cols, vo
     return cols_apply(p, 1, nb, f_cols, alm_cols, stream);
 }
 
+// ------------------------------------------------------------------------------------------------
+// Plan groups: many single-transform plans applied together. The ring passes of every member run
+// as one launch over a work list of (plan, order, 64-ring tile) items (and one for the adjoint's
+// partial reduction); the members' grid, FFT and point stages run concurrently, dealt round-robin
+// to the group's streams and joined back into the caller's stream. Each (direction, pointer slot)
+// sequence is captured once into a hipGraph and replayed; the caller's buffers reach the kernels
+// through a device pointer table written before each launch, so a graph stays valid across calls.
+// ------------------------------------------------------------------------------------------------
+struct pfsht_group_s {
+    static constexpr int kSlots = 4;    // pointer tables (and graphs) in flight
+    static constexpr int kStreams = 4;  // concurrent member streams
+    int dev = 0;
+    std::vector<pfsht_plan_t> plans;
+    int np = 0;
+    int64_t bytes = 0;
+    RingTab *tabs = nullptr;                  // [kSlots][np]
+    int2 *fwork = nullptr, *awork[2] = {};    // forward items; adjoint workgroups for 1 / 2 tiles per wave
+    int nfwork = 0, nawork[2] = {0, 0}, fwaves = 1;
+    int4 *zwork = nullptr;                    // finalize work
+    int nzwork = 0;
+    void **ptr_dev = nullptr, **ptr_host = nullptr;  // [kSlots][2 np]: inputs, then outputs
+    hipEvent_t slot_ev[kSlots] = {};
+    bool slot_used[kSlots] = {};
+    int slot_next = 0;
+    hipStream_t streams[kStreams] = {};
+    hipStream_t main = nullptr;  // the group's work runs here (capturable whatever the caller's stream)
+    hipEvent_t fork = nullptr, join[kStreams] = {}, enter = nullptr, leave = nullptr;
+    hipGraphExec_t graph[2][kSlots] = {};
+    std::mutex mu;
+
+    ~pfsht_group_s() {
+        DeviceGuard dg(dev);
+        (void)hipDeviceSynchronize();
+        for (auto &gd : graph)
+            for (auto g : gd)
+                if (g) (void)hipGraphExecDestroy(g);
+        for (auto e : slot_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (enter) (void)hipEventDestroy(enter);
+        if (leave) (void)hipEventDestroy(leave);
+        for (auto q : streams)
+            if (q) (void)hipStreamDestroy(q);
+        if (main) (void)hipStreamDestroy(main);
+        dfree(tabs); dfree(fwork); dfree(awork[0]); dfree(awork[1]); dfree(zwork); dfree(ptr_dev);
+        if (ptr_host) (void)hipHostFree(ptr_host);
+    }
+};
+
+static Status group_build(pfsht_group_s &g, const pfsht_plan_t *plans, int np) {
+    if (np < 1) { set_error("a group needs at least one plan"); return kEArg; }
+    g.np = np;
+    g.dev = plans[0] ? plans[0]->dev : 0;
+    for (int i = 0; i < np; ++i) {
+        pfsht_plan_s *p = plans[i];
+        if (!p) { set_error("NULL plan in the group"); return kEArg; }
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == p) { set_error("a plan appears twice in the group"); return kEArg; }
+        if (p->dev != g.dev) { set_error("the plans of a group must live on one device"); return kEArg; }
+        if (p->B != 1) { set_error("group members are single-transform plans (ntrans = 1)"); return kEUnsupported; }
+        if (p->spin || (p->method != PFSHT_METHOD_STRETCH && p->method != PFSHT_METHOD_DIRECT) || !p->rings) {
+            set_error("group members are scalar stretched or Direct plans (no spin, no multi-box)");
+            return kEUnsupported;
+        }
+        g.plans.push_back(p);
+    }
+    DeviceGuard dg(g.dev);
+    // work lists: each engine's item lists are whole rounds of eight (one per XCD), so concatenating
+    // them keeps item i on XCD i % 8
+    std::vector<int2> fw, aw[2];
+    std::vector<int4> zw;
+    for (int i = 0; i < np; ++i) {
+        const RingDev &d = g.plans[i]->rings->d;
+        for (int k = 0; k < d.nfitems; ++k) fw.push_back(make_int2(i, k));
+        const int tgi = d.adj_tg == 2 ? 1 : 0;
+        for (int k = 0; k < d.naitems / kAdjWaves; ++k) aw[tgi].push_back(make_int2(i, k));
+        for (int am = 0; am <= d.L; ++am)
+            for (int rb = 0; rb < (d.L + 1 + 255) / 256; ++rb) zw.push_back(make_int4(i, rb, am, 0));
+        g.fwaves = std::max(g.fwaves, d.fwd_waves);
+    }
+    Status st;
+    if ((st = upload(&g.fwork, fw, g.bytes)) != kOk) return st;
+    if ((st = upload(&g.awork[0], aw[0], g.bytes)) != kOk) return st;
+    if ((st = upload(&g.awork[1], aw[1], g.bytes)) != kOk) return st;
+    if ((st = upload(&g.zwork, zw, g.bytes)) != kOk) return st;
+    g.nfwork = int(fw.size());
+    g.nawork[0] = int(aw[0].size());
+    g.nawork[1] = int(aw[1].size());
+    g.nzwork = int(zw.size());
+    const size_t nptr = size_t(pfsht_group_s::kSlots) * 2 * np;
+    if ((st = dalloc(&g.ptr_dev, nptr, g.bytes)) != kOk) return st;
+    PFSHT_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.ptr_host), nptr * sizeof(void *)));
+    // engine tables per slot: the coefficient columns are entries of that slot's pointer table
+    std::vector<RingTab> tabs(size_t(pfsht_group_s::kSlots) * np);
+    for (int k = 0; k < pfsht_group_s::kSlots; ++k)
+        for (int i = 0; i < np; ++i) {
+            const RingDev &d = g.plans[i]->rings->d;
+            RingTab &t = tabs[size_t(k) * np + i];
+            t.fitems = d.fitems; t.aitems = d.aitems; t.costh = d.costh; t.log2sin = d.log2sin; t.log2c = d.log2c;
+            t.ab = d.ab; t.gc = d.gc; t.abofs = d.abofs; t.ckq = d.ckq; t.ckE = d.ckE; t.pofs = d.pofs;
+            t.ntiles = d.ntiles; t.partials = d.partials; t.Gd = g.plans[i]->rings->Gd;
+            void **tbl = g.ptr_dev + size_t(k) * 2 * np;
+            t.alm_in = InCols();
+            t.alm_in.tab = reinterpret_cast<const double2 *const *>(tbl + i);
+            t.alm_out = OutCols();
+            t.alm_out.tab = reinterpret_cast<double2 *const *>(tbl + np + i);
+            t.L = d.L; t.n = d.n; t.mmax = d.mmax; t.part_b = d.part_b;
+        }
+    if ((st = upload(&g.tabs, tabs, g.bytes)) != kOk) return st;
+    for (auto &e : g.slot_ev) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &q : g.streams) PFSHT_HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    PFSHT_HIP_TRY(hipStreamCreateWithFlags(&g.main, hipStreamNonBlocking));
+    PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming));
+    PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.enter, hipEventDisableTiming));
+    PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.leave, hipEventDisableTiming));
+    for (auto &e : g.join) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    PFSHT_HIP_TRY(hipDeviceSynchronize());
+    return kOk;
+}
+
+// The work of one group apply (dir 0 synthesis, 1 adjoint) with pointer slot k, on stream s.
+static Status group_enqueue(pfsht_group_s &g, int dir, int k, hipStream_t s) {
+    const int np = g.np;
+    const RingTab *tabs = g.tabs + size_t(k) * np;
+    void **tbl = g.ptr_dev + size_t(k) * 2 * np;
+    auto member = [&](int i, hipStream_t q) -> Status {
+        pfsht_plan_s &p = *g.plans[i];
+        InCols in;
+        in.tab = reinterpret_cast<const double2 *const *>(tbl + i);
+        OutCols out;
+        out.tab = reinterpret_cast<double2 *const *>(tbl + np + i);
+        pfsht_rings_s &r = *p.rings;
+        if (p.method == PFSHT_METHOD_DIRECT) {
+            if (dir == 0) launch_direct_synth(r.Gd, p.dphi, p.dmlim, p.dperm, out, p.M, p.mmax, 1, q);
+            else launch_direct_adjoint(in, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, 1, q);
+            return kOk;
+        }
+        const bool mixed = p.precision == PFSHT_PREC_MIXED;
+        if (dir == 0) return mixed ? synth_grid<float2>(p, nullptr, out, 1, q) : synth_grid<double2>(p, nullptr, out, 1, q);
+        return mixed ? adjoint_grid<float2>(p, nullptr, in, 1, q) : adjoint_grid<double2>(p, nullptr, in, 1, q);
+    };
+    auto members = [&]() -> Status {
+        PFSHT_HIP_TRY(hipEventRecord(g.fork, s));
+        const int ns = std::min(np, pfsht_group_s::kStreams);
+        for (int j = 0; j < ns; ++j) PFSHT_HIP_TRY(hipStreamWaitEvent(g.streams[j], g.fork, 0));
+        for (int i = 0; i < np; ++i) {
+            Status st = member(i, g.streams[i % ns]);
+            if (st != kOk) return st;
+        }
+        for (int j = 0; j < ns; ++j) {
+            PFSHT_HIP_TRY(hipEventRecord(g.join[j], g.streams[j]));
+            PFSHT_HIP_TRY(hipStreamWaitEvent(s, g.join[j], 0));
+        }
+        return kOk;
+    };
+    Status st;
+    if (dir == 0) {
+        launch_ring_fwd_group(tabs, g.fwork, g.nfwork, g.fwaves, s);
+        if ((st = members()) != kOk) return st;
+    } else {
+        if ((st = members()) != kOk) return st;
+        launch_ring_adj_group(tabs, g.awork[1], g.nawork[1], 2, s);
+        launch_ring_adj_group(tabs, g.awork[0], g.nawork[0], 1, s);
+        launch_ring_finalize_group(tabs, g.zwork, g.nzwork, s);
+    }
+    PFSHT_HIP_TRY(hipGetLastError());
+    return kOk;
+}
+
+static int group_apply(pfsht_group_t g, int dir, const void *const *in, void *const *out, void *stream) {
+    if (!g || !in || !out) { set_error("NULL argument"); return kEArg; }
+    const int np = g->np;
+    for (int i = 0; i < np; ++i)
+        if (!in[i] || !out[i]) { set_error("NULL buffer pointer"); return kEArg; }
+    std::lock_guard<std::mutex> lk(g->mu);
+    // the members' own locks, in address order
+    std::vector<pfsht_plan_s *> order(g->plans.begin(), g->plans.end());
+    std::sort(order.begin(), order.end());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(order.size());
+    for (auto *p : order) locks.emplace_back(p->mu);
+    DeviceGuard dg(g->dev);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int k = g->slot_next;
+    g->slot_next = (k + 1) % pfsht_group_s::kSlots;
+    if (g->slot_used[k]) PFSHT_HIP_TRY(hipEventSynchronize(g->slot_ev[k]));
+    void **h = g->ptr_host + size_t(k) * 2 * np;
+    for (int i = 0; i < np; ++i) {
+        h[i] = const_cast<void *>(in[i]);
+        h[np + i] = out[i];
+    }
+    PFSHT_HIP_TRY(hipMemcpyAsync(g->ptr_dev + size_t(k) * 2 * np, h, 2 * size_t(np) * sizeof(void *),
+                                 hipMemcpyHostToDevice, s));
+    Status st;
+    for (auto *p : g->plans)
+        if ((st = apply_begin(*p, s)) != kOk) return st;
+    // the group's own stream joins the caller's stream, runs the apply, and the caller's stream
+    // waits for it
+    hipStream_t gs = g->main;
+    PFSHT_HIP_TRY(hipEventRecord(g->enter, s));
+    PFSHT_HIP_TRY(hipStreamWaitEvent(gs, g->enter, 0));
+    hipGraphExec_t &ge = g->graph[dir][k];
+    if (!ge) {
+        // capture the sequence once; where capture fails the sequence runs eagerly
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            st = group_enqueue(*g, dir, k, gs);
+            const hipError_t e = hipStreamEndCapture(gs, &graph);
+            if (st == kOk && e == hipSuccess && graph &&
+                hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0) != hipSuccess)
+                ge = nullptr;
+            if (graph) (void)hipGraphDestroy(graph);
+            if (st != kOk) return st;
+        }
+        (void)hipGetLastError();  // a refused capture leaves its error behind
+    }
+    if (ge) {
+        PFSHT_HIP_TRY(hipGraphLaunch(ge, gs));
+    } else if ((st = group_enqueue(*g, dir, k, gs)) != kOk) {
+        return st;
+    }
+    PFSHT_HIP_TRY(hipEventRecord(g->leave, gs));
+    PFSHT_HIP_TRY(hipStreamWaitEvent(s, g->leave, 0));
+    for (auto *p : g->plans)
+        if ((st = apply_end(*p, s)) != kOk) return st;
+    PFSHT_HIP_TRY(hipEventRecord(g->slot_ev[k], s));
+    g->slot_used[k] = true;
+    return kOk;
+}
+
+int pfsht_group_create(const pfsht_plan_t *plans, int32_t nplans, pfsht_group_t *out) {
+    if (!plans || !out) { set_error("NULL argument"); return kEArg; }
+    *out = nullptr;
+    auto *g = new (std::nothrow) pfsht_group_s;
+    if (!g) { set_error("host allocation failed"); return kENomem; }
+    Status st;
+    try {
+        st = group_build(*g, plans, nplans);
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        st = kENomem;
+    }
+    if (st != kOk) {
+        delete g;
+        return st;
+    }
+    *out = g;
+    return kOk;
+}
+
+void pfsht_group_destroy(pfsht_group_t g) { delete g; }
+
+int pfsht_group_synth(pfsht_group_t g, const void *const *alm, void *const *f, void *stream) {
+    return group_apply(g, 0, alm, f, stream);
+}
+
+int pfsht_group_adjoint(pfsht_group_t g, const void *const *f, void *const *alm, void *stream) {
+    return group_apply(g, 1, f, alm, stream);
+}
+
 int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, int32_t where, void *stream) {
     if (!p || !in || !out) { set_error("NULL argument"); return kEArg; }
     if (type != 1 && type != 2) { set_error("NUFFT type must be 1 or 2"); return kEArg; }

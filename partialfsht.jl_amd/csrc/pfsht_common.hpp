@@ -196,6 +196,28 @@ void launch_ring_checkpoints(const RingDev &r, hipStream_t s);
 // Launchers (stream-ordered, asynchronous). alm and G use the reference's column-major layouts;
 // Gd is the internal ring buffer [b][m + mmax][j].
 // nb transforms (nb <= r.B) in columns alm.col(0 .. nb - 1); Gd holds B transforms, the first nb used.
+// One single-transform engine of a plan group, as the grouped ring launches read it (device memory).
+struct RingTab {
+    const SegItem *fitems, *aitems;
+    const double *costh, *log2sin, *log2c;
+    const double2 *ab, *gc;
+    const int64_t *abofs;
+    const double2 *ckq;
+    const int32_t *ckE;
+    const int64_t *pofs;
+    const int32_t *ntiles;
+    double4 *partials;
+    double2 *Gd;
+    InCols alm_in;    // the synthesis's coefficients
+    OutCols alm_out;  // the adjoint's coefficients
+    int L, n, mmax, part_b;
+};
+// Grouped ring passes: work lists of {engine, forward item} and {engine, adjoint workgroup} (waves:
+// the largest segment count of the engines; tg: the adjoint tiles per wavefront, the same for every
+// engine in one launch), and of {engine, row block, |m|, 0} for the finalize.
+void launch_ring_fwd_group(const RingTab *tabs, const int2 *work, int nwork, int waves, hipStream_t s);
+void launch_ring_adj_group(const RingTab *tabs, const int2 *work, int nwork, int tg, hipStream_t s);
+void launch_ring_finalize_group(const RingTab *tabs, const int4 *work, int nwork, hipStream_t s);
 void launch_ring_fwd(const RingDev &r, InCols alm, int nb, double2 *Gd, hipStream_t s);
 void launch_ring_adj(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s);
 // The adjoint as timed stages: the ring pass (stage 0), then, on the partials path only, the ordered

@@ -266,14 +266,27 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, g
 // the segment sums, scaled to plain doubles, are added in segment order through LDS and wavefront 0
 // stores the tile's ring values. The sweep holds the DPP operands of two groups; six wavefronts per
 // SIMD fit its registers.
+// GROUP: a launch over many single-transform engines (plan groups): workgroup i takes item
+// work[i].y of engine work[i].x, whose tables come from tabs[work[i].x] instead of the arguments.
+template <bool GROUP>
 __global__ void __launch_bounds__(64 * kSegMax) __attribute__((amdgpu_waves_per_eu(6)))
 ring_fwd1_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                  const double *__restrict__ log2sin, const double *__restrict__ log2c,
                  const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
                  const double2 *__restrict__ ckq, const int32_t *__restrict__ ckE, InCols alm, int b0,
-                 double2 *__restrict__ Gd, int L, int n, int mmax, int never) {
+                 double2 *__restrict__ Gd, int L, int n, int mmax, int never, const RingTab *__restrict__ tabs,
+                 const int2 *__restrict__ work) {
     __shared__ double red[kSegMax - 1][4][64];
-    const SegItem it = items[blockIdx.x];
+    int wi = blockIdx.x;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const RingTab &T = tabs[w.x];
+        items = T.fitems; costh = T.costh; log2sin = T.log2sin; log2c = T.log2c; ab = T.ab; gc = T.gc;
+        abofs = T.abofs; ckq = T.ckq; ckE = T.ckE; alm = T.alm_in; Gd = T.Gd; L = T.L; n = T.n; mmax = T.mmax;
+        b0 = 0;
+        wi = w.y;
+    }
+    const SegItem it = items[wi];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int nseg = __builtin_amdgcn_readfirstlane(it.nseg);
     const int nr = __builtin_amdgcn_readfirstlane(it.nr);
@@ -564,17 +577,29 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
 // Workgroups of kAdjWaves wavefronts; each wavefront is one item: one degree segment of TG
 // consecutive 64-ring tiles of one order (the second may be empty), writing that tile group's
 // partials for the segment's degrees.
-template <int NB, int TG>
+// GROUP: as the forward's; work[i].y is the engine's workgroup index (kAdjWaves items each).
+template <int NB, int TG, bool GROUP = false>
 __global__ void __launch_bounds__(64 * kAdjWaves)
 ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ costh,
                 const double *__restrict__ log2sin, const double *__restrict__ log2c,
                 const double2 *__restrict__ ab, const double2 *__restrict__ gc, const int64_t *__restrict__ abofs,
                 const double2 *__restrict__ ckq,
                 const int32_t *__restrict__ ckE, const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
-                double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never) {
+                double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never,
+                const RingTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
     extern __shared__ double lds_adj[];
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-    const SegItem it = items[blockIdx.x * kAdjWaves + wave];
+    int wg = blockIdx.x;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const RingTab &T = tabs[w.x];
+        items = T.aitems; costh = T.costh; log2sin = T.log2sin; log2c = T.log2c; ab = T.ab; gc = T.gc;
+        abofs = T.abofs; ckq = T.ckq; ckE = T.ckE; pofs = T.pofs; Gd = T.Gd; partials = T.partials;
+        L = T.L; n = T.n; mmax = T.mmax; B = T.part_b;
+        b0 = 0;
+        wg = w.y;
+    }
+    const SegItem it = items[wg * kAdjWaves + wave];
     const int m = __builtin_amdgcn_readfirstlane(it.m);
     const int lane = threadIdx.x & 63;
     const int nr = __builtin_amdgcn_readfirstlane(it.nr);
@@ -639,11 +664,22 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
 // is retained, zero in the padding and for unretained orders (the adjoint overwrites alm,
 // src/Legendre.jl:338). One thread owns row l - |m| of the column pair (2|m|, 2|m| - 1), reading each
 // partial once. pb is the partials' transform stride.
+// GROUP: one workgroup per work entry {engine, row block, |m|} of many single-transform engines.
+template <bool GROUP = false>
 __global__ void ring_adj_finalize(const int64_t *__restrict__ pofs, const int32_t *__restrict__ ntiles,
-                                  const double4 *__restrict__ partials, OutCols alm, int L, int mmax, int pb) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int am = blockIdx.y;
-    const int b = blockIdx.z;
+                                  const double4 *__restrict__ partials, OutCols alm, int L, int mmax, int pb,
+                                  const RingTab *__restrict__ tabs = nullptr, const int4 *__restrict__ work = nullptr) {
+    int rb = blockIdx.x, am = blockIdx.y, b = blockIdx.z;
+    if constexpr (GROUP) {
+        const int4 w = work[blockIdx.x];
+        const RingTab &T = tabs[w.x];
+        pofs = T.pofs; ntiles = T.ntiles; partials = T.partials; alm = T.alm_out; L = T.L; mmax = T.mmax;
+        pb = T.part_b;
+        rb = w.y;
+        am = w.z;
+        b = 0;
+    }
+    const int row = rb * blockDim.x + threadIdx.x;
     if (row > L) return;
     const int64_t nrow = L + 1;
     const gptr a = alm.col(b);
@@ -1073,8 +1109,9 @@ void launch_ring_fwd(const RingDev &r, InCols alm, int nb, double2 *Gd, hipStrea
     if (r.nfitems == 0 || nb <= 0) return;
     dim3 grid(r.nfitems), block(64 * r.fwd_waves);
     if (nb == 1) {
-        ring_fwd1_kernel<<<grid, block, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs, r.ckq,
-                                                r.ckE, alm, 0, Gd, r.L, r.n, r.mmax, kNeverSink);
+        ring_fwd1_kernel<false><<<grid, block, 0, s>>>(r.fitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc, r.abofs,
+                                                       r.ckq, r.ckE, alm, 0, Gd, r.L, r.n, r.mmax, kNeverSink,
+                                                       nullptr, nullptr);
         return;
     }
     const size_t lds = sizeof(double) * std::max(r.fwd_waves * kSub * kWS, 32 * 65);
@@ -1143,7 +1180,33 @@ void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int 
 void launch_ring_adj_finish(const RingDev &r, OutCols alm, int nb, hipStream_t s) {
     if (nb <= 0 || ring_adj_uses_mfma(r, nb)) return;
     dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, nb);
-    ring_adj_finalize<<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.part_b);
+    ring_adj_finalize<false><<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.part_b);
+}
+
+// Plan groups: the single-transform ring passes of many engines in one launch each.
+void launch_ring_fwd_group(const RingTab *tabs, const int2 *work, int nwork, int waves, hipStream_t s) {
+    if (nwork <= 0) return;
+    ring_fwd1_kernel<true><<<dim3(nwork), dim3(64 * waves), 0, s>>>(
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, InCols(), 0, nullptr, 0, 0,
+        0, kNeverSink, tabs, work);
+}
+
+void launch_ring_adj_group(const RingTab *tabs, const int2 *work, int nwork, int tg, hipStream_t s) {
+    if (nwork <= 0) return;
+    const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
+    if (tg == 2)
+        ring_adj_kernel<1, 2, true><<<dim3(nwork), 64 * kAdjWaves, lds, s>>>(
+            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, 0, 0, 0, 1, 0, kNeverSink, tabs, work);
+    else
+        ring_adj_kernel<1, 1, true><<<dim3(nwork), 64 * kAdjWaves, lds, s>>>(
+            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, 0, 0, 0, 1, 0, kNeverSink, tabs, work);
+}
+
+void launch_ring_finalize_group(const RingTab *tabs, const int4 *work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return;
+    ring_adj_finalize<true><<<dim3(nwork), 256, 0, s>>>(nullptr, nullptr, nullptr, OutCols(), 0, 0, 1, tabs, work);
 }
 
 void launch_ring_adj(const RingDev &r, const double2 *Gd, OutCols alm, int nb, hipStream_t s) {

@@ -106,6 +106,10 @@ EXPORTS = {
     "pfsht_plan_tune_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "pfsht_plan_tune_key": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "pfsht_group_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pfsht_group_destroy": (None, [C.c_void_p]),
+    "pfsht_group_synth": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pfsht_group_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pfsht_synth_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_adjoint_f32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "pfsht_synth_cols": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -797,6 +801,57 @@ class Server:
         st = ServerStatsC()
         _check(lib().pfsht_server_get_stats(self._h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in ServerStatsC._fields_}
+
+
+class Group:
+    """Many single-transform plans applied as one (pfsht_group_*): their ring passes in one launch
+    over a (plan, order, ring tile) work list, their other stages concurrently, each direction replayed
+    from a hipGraph. synth(fs, alms) / adjoint(alms, fs) take one device tensor per member."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p._h for p in self.plans])
+        h = C.c_void_p()
+        _check(lib().pfsht_group_create(arr, len(self.plans), C.byref(h)))
+        self._h = h
+        self._live = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pfsht_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _tables(self, ins, outs, synth: bool):
+        n = len(self.plans)
+        if len(ins) != n or len(outs) != n:
+            raise DimensionMismatch(f"the group has {n} plans: pass {n} inputs and {n} outputs")
+        pi = (C.c_void_p * n)()
+        po = (C.c_void_p * n)()
+        for i, (p, x, y) in enumerate(zip(self.plans, ins, outs)):
+            na, nf = (p.lmax + 1) * (2 * p.lmax + 1), p.npts
+            px, wx = _buf(x, na if synth else nf, "input")
+            py, wy = _buf(y, nf if synth else na, "output")
+            if wx != WHERE_DEVICE or wy != WHERE_DEVICE:
+                raise ArgumentError("group applies take device (CUDA) tensors")
+            pi[i], po[i] = px, py
+        self._live = [ins, outs]  # kept alive while the call may still run
+        return pi, po
+
+    def synth(self, fs, alms, stream=None):
+        pi, po = self._tables(alms, fs, True)
+        _check(lib().pfsht_group_synth(self._h, pi, po, _stream_ptr(stream)))
+        return fs
+
+    def adjoint(self, alms, fs, stream=None):
+        pi, po = self._tables(fs, alms, False)
+        _check(lib().pfsht_group_adjoint(self._h, pi, po, _stream_ptr(stream)))
+        return alms
 
 
 def mul(out, op, x, stream=None):
