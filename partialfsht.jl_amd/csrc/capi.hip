@@ -1112,7 +1112,7 @@ static void timing_drain(pfsht_plan_s &p) {
 // The grid part of the stretched synthesis, from the ring buffer to the points, with the grid
 // buffers stored as C (stages 1-7), for nb transforms.
 template <class C>
-static Status synth_grid(pfsht_plan_s &p, TimedApply *t, OutCols f, int nb, hipStream_t s) {
+static Status synth_grid(pfsht_plan_s &p, TimedApply *t, OutCols f, int nb, hipStream_t s, bool interp = true) {
     const GridDims g = dims_of(p, nb);
     NufftDev u = p.nu;
     u.B = nb;
@@ -1128,19 +1128,19 @@ static Status synth_grid(pfsht_plan_s &p, TimedApply *t, OutCols f, int nb, hipS
     launch_ut_to_rows(u, p.mmax, static_cast<const C *>(Ut), U, s);
     if ((st = fftset_exec(p.row_bwd, U, nb, s)) != kOk) return st;
     tmark(p, t, 7, s);
-    launch_interp(u, static_cast<const C *>(U), f, s);
+    if (interp) launch_interp(u, static_cast<const C *>(U), f, s);
     tmark(p, t, 8, s);  // stage 8 is the adjoint's
     return kOk;
 }
 
 // The grid part of the stretched adjoint, from the points to the ring buffer (stages 1-6).
 template <class C>
-static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, InCols f, int nb, hipStream_t s) {
+static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, InCols f, int nb, hipStream_t s, bool spread = true) {
     const GridDims g = dims_of(p, nb);
     NufftDev u = p.nu;
     u.B = nb;
     C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
-    launch_spread(u, f, U, s);
+    if (spread) launch_spread(u, f, U, s);
     tmark(p, t, 1, s);
     Status st = fftset_exec(p.row_fwd, U, nb, s);
     if (st != kOk) return st;
@@ -1870,6 +1870,16 @@ struct pfsht_group_s {
     int nfwork = 0, nawork[2] = {0, 0}, fwaves = 1;
     int4 *zwork = nullptr;                    // finalize work
     int nzwork = 0;
+    // the stretched members' NUFFT stages, one launch per class of (kernel width, grid type)
+    NufftTab *ntabs = nullptr;                // [kSlots][np]
+    struct NuClass {
+        int w = 0, win = 0;
+        bool single = false;
+        int2 *work = nullptr;   // {member, super-tile}
+        int4 *mwork = nullptr;  // {member, column block, stored row, 0}
+        int nwork = 0, nmwork = 0;
+    };
+    std::vector<NuClass> classes;
     void **ptr_dev = nullptr, **ptr_host = nullptr;  // [kSlots][2 np]: inputs, then outputs
     hipEvent_t slot_ev[kSlots] = {};
     bool slot_used[kSlots] = {};
@@ -1896,7 +1906,11 @@ struct pfsht_group_s {
         for (auto q : streams)
             if (q) (void)hipStreamDestroy(q);
         if (main) (void)hipStreamDestroy(main);
-        dfree(tabs); dfree(fwork); dfree(awork[0]); dfree(awork[1]); dfree(zwork); dfree(ptr_dev);
+        dfree(tabs); dfree(fwork); dfree(awork[0]); dfree(awork[1]); dfree(zwork); dfree(ptr_dev); dfree(ntabs);
+        for (auto &c : classes) {
+            dfree(c.work);
+            dfree(c.mwork);
+        }
         if (ptr_host) (void)hipHostFree(ptr_host);
     }
 };
@@ -1961,6 +1975,49 @@ static Status group_build(pfsht_group_s &g, const pfsht_plan_t *plans, int np) {
             t.L = d.L; t.n = d.n; t.mmax = d.mmax; t.part_b = d.part_b;
         }
     if ((st = upload(&g.tabs, tabs, g.bytes)) != kOk) return st;
+    // NUFFT classes and tables of the stretched members
+    std::vector<NufftTab> ntabs(size_t(pfsht_group_s::kSlots) * np);
+    std::vector<std::vector<int2>> cw;
+    std::vector<std::vector<int4>> cm;
+    for (int i = 0; i < np; ++i) {
+        pfsht_plan_s &p = *g.plans[i];
+        if (p.method != PFSHT_METHOD_STRETCH) continue;
+        const NufftDev &u = p.nu;
+        const bool single = p.precision == PFSHT_PREC_MIXED;
+        size_t c = 0;
+        while (c < g.classes.size() && !(g.classes[c].w == u.w && g.classes[c].single == single)) ++c;
+        if (c == g.classes.size()) {
+            pfsht_group_s::NuClass nc;
+            nc.w = u.w;
+            nc.win = u.win;
+            nc.single = single;
+            g.classes.push_back(nc);
+            cw.emplace_back();
+            cm.emplace_back();
+        }
+        for (int q = 0; q < u.nst; ++q) cw[c].push_back(make_int2(i, q));
+        for (int r = 0; r < u.nR; ++r)
+            for (int cb = 0; cb < (u.N2 + 255) / 256; ++cb) cm[c].push_back(make_int4(i, cb, r, 0));
+        for (int k = 0; k < pfsht_group_s::kSlots; ++k) {
+            NufftTab &t = ntabs[size_t(k) * np + i];
+            t.X1 = u.X1; t.X2 = u.X2; t.perm = u.perm; t.st_list = u.st_list; t.st_slot = u.st_slot;
+            t.bin_ofs = u.bin_ofs; t.U = p.U; t.Wst = u.Wst;
+            void **tbl = g.ptr_dev + size_t(k) * 2 * np;
+            t.fin = InCols();
+            t.fin.tab = reinterpret_cast<const double2 *const *>(tbl + i);
+            t.fout = OutCols();
+            t.fout.tab = reinterpret_cast<double2 *const *>(tbl + np + i);
+            t.N1 = u.N1; t.N2 = u.N2; t.S = u.S; t.ns1 = u.ns1; t.ns2 = u.ns2; t.win = u.win; t.w = u.w;
+            t.nst = u.nst; t.r0 = u.r0; t.nR = u.nR;
+        }
+    }
+    for (size_t c = 0; c < g.classes.size(); ++c) {
+        if ((st = upload(&g.classes[c].work, cw[c], g.bytes)) != kOk) return st;
+        if ((st = upload(&g.classes[c].mwork, cm[c], g.bytes)) != kOk) return st;
+        g.classes[c].nwork = int(cw[c].size());
+        g.classes[c].nmwork = int(cm[c].size());
+    }
+    if ((st = upload(&g.ntabs, ntabs, g.bytes)) != kOk) return st;
     for (auto &e : g.slot_ev) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &q : g.streams) PFSHT_HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     PFSHT_HIP_TRY(hipStreamCreateWithFlags(&g.main, hipStreamNonBlocking));
@@ -1989,9 +2046,25 @@ static Status group_enqueue(pfsht_group_s &g, int dir, int k, hipStream_t s) {
             else launch_direct_adjoint(in, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, 1, q);
             return kOk;
         }
+        // the interpolation and spreading run grouped, after and before the members' stages
         const bool mixed = p.precision == PFSHT_PREC_MIXED;
-        if (dir == 0) return mixed ? synth_grid<float2>(p, nullptr, out, 1, q) : synth_grid<double2>(p, nullptr, out, 1, q);
-        return mixed ? adjoint_grid<float2>(p, nullptr, in, 1, q) : adjoint_grid<double2>(p, nullptr, in, 1, q);
+        if (dir == 0)
+            return mixed ? synth_grid<float2>(p, nullptr, out, 1, q, false)
+                         : synth_grid<double2>(p, nullptr, out, 1, q, false);
+        return mixed ? adjoint_grid<float2>(p, nullptr, in, 1, q, false)
+                     : adjoint_grid<double2>(p, nullptr, in, 1, q, false);
+    };
+    const NufftTab *ntabs = g.ntabs + size_t(k) * np;
+    auto nufft = [&]() {
+        for (const auto &c : g.classes) {
+            if (dir == 0) {
+                if (c.single) launch_interp_group<float2>(ntabs, c.work, c.nwork, c.w, c.win, s);
+                else launch_interp_group<double2>(ntabs, c.work, c.nwork, c.w, c.win, s);
+            } else {
+                if (c.single) launch_spread_group<float2>(ntabs, c.work, c.nwork, c.mwork, c.nmwork, c.w, c.win, s);
+                else launch_spread_group<double2>(ntabs, c.work, c.nwork, c.mwork, c.nmwork, c.w, c.win, s);
+            }
+        }
     };
     auto members = [&]() -> Status {
         PFSHT_HIP_TRY(hipEventRecord(g.fork, s));
@@ -2011,7 +2084,9 @@ static Status group_enqueue(pfsht_group_s &g, int dir, int k, hipStream_t s) {
     if (dir == 0) {
         launch_ring_fwd_group(tabs, g.fwork, g.nfwork, g.fwaves, s);
         if ((st = members()) != kOk) return st;
+        nufft();
     } else {
+        nufft();
         if ((st = members()) != kOk) return st;
         launch_ring_adj_group(tabs, g.awork[1], g.nawork[1], 2, s);
         launch_ring_adj_group(tabs, g.awork[0], g.nawork[0], 1, s);

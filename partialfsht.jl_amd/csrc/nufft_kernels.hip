@@ -352,15 +352,25 @@ void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U
 // periodically) is staged in LDS with coalesced row reads, then each thread interpolates bin-sorted
 // points of the super-tile from LDS, evaluating each point's kernel rows.
 // ------------------------------------------------------------------------------------------------
-template <int W, class C>
+// GROUP: one launch over many single-transform plans; workgroup i takes super-tile work[i].y of
+// plan work[i].x, whose tables come from tabs[work[i].x].
+template <int W, class C, bool GROUP = false>
 __global__ void __launch_bounds__(256)
 interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  const C *__restrict__ U, OutCols f, const int32_t *__restrict__ st_list,
-                 const int32_t *__restrict__ bin_ofs, int N1, int N2, int S4, int ns2, int win, int r0, int nR) {
+                 const int32_t *__restrict__ bin_ofs, int N1, int N2, int S4, int ns2, int win, int r0, int nR,
+                 const NufftTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
     extern __shared__ __align__(16) unsigned char lds_ip_raw[];
     C *lw = reinterpret_cast<C *>(lds_ip_raw);
-    const int slot = blockIdx.x;
-    const int b = blockIdx.y;
+    int slot = blockIdx.x, b = blockIdx.y;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const NufftTab &T = tabs[w.x];
+        X1 = T.X1; X2 = T.X2; perm = T.perm; U = static_cast<const C *>(T.U); f = T.fout; st_list = T.st_list;
+        bin_ofs = T.bin_ofs; N1 = T.N1; N2 = T.N2; S4 = 4 * T.S; ns2 = T.ns2; win = T.win; r0 = T.r0; nR = T.nR;
+        slot = w.y;
+        b = 0;
+    }
     const int st = st_list[slot];
     const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
     const C *__restrict__ Ub = U + int64_t(b) * nR * N2;
@@ -514,16 +524,25 @@ interp_st_multi_kernel(const double *__restrict__ X1, const double *__restrict__
 // loaded while the current chunk is evaluated and accumulated. No workgroup barrier until the end,
 // where, transform by transform, the tiles are added into the LDS window wave after wave (bin
 // windows overlap) and the window is written out for the merge pass.
-template <int W, int G, class C>
+// GROUP (G = 1): as the interpolation's.
+template <int W, int G, class C, bool GROUP = false>
 __global__ void __launch_bounds__(64 * kSpreadWaves)
 spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
                  InCols f, const int32_t *__restrict__ st_list, const int32_t *__restrict__ bin_ofs,
-                 C *__restrict__ Wst, int S, int ns2, int win, int nst, int nb) {
+                 C *__restrict__ Wst, int S, int ns2, int win, int nst, int nb,
+                 const NufftTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
     constexpr int NH = (W + 1) / 2;  // cells per half row
     extern __shared__ double lds_sp[];
     double2 *lw = reinterpret_cast<double2 *>(lds_sp);  // [win][win]
-    const int slot = blockIdx.x;
-    const int b0 = blockIdx.y * G;
+    int slot = blockIdx.x, b0 = blockIdx.y * G;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const NufftTab &T = tabs[w.x];
+        X1 = T.X1; X2 = T.X2; perm = T.perm; f = T.fin; st_list = T.st_list; bin_ofs = T.bin_ofs;
+        Wst = static_cast<C *>(T.Wst); S = T.S; ns2 = T.ns2; win = T.win; nst = T.nst; nb = 1;
+        slot = w.y;
+        b0 = 0;
+    }
     const int gc = min(G, nb - b0);  // transforms of this workgroup
     const int st = st_list[slot];
     const int S4 = 4 * S;
@@ -681,15 +700,25 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     }
 }
 
-template <class C>
+// GROUP: workgroup i merges column block work[i].y of stored row work[i].z of plan work[i].x.
+template <class C, bool GROUP = false>
 __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__restrict__ st_slot,
                                     C *__restrict__ U, int N1, int N2, int S4, int ns1, int ns2, int win, int w,
-                                    int nst, int r0, int nR) {
-    const int l2 = blockIdx.x * blockDim.x + threadIdx.x;
-    const int pr = blockIdx.y;  // stored row
+                                    int nst, int r0, int nR, const NufftTab *__restrict__ tabs = nullptr,
+                                    const int4 *__restrict__ work = nullptr) {
+    int cb = blockIdx.x, pr = blockIdx.y, b = blockIdx.z;  // column block, stored row, transform
+    if constexpr (GROUP) {
+        const int4 q = work[blockIdx.x];
+        const NufftTab &T = tabs[q.x];
+        Wst = static_cast<const C *>(T.Wst); st_slot = T.st_slot; U = static_cast<C *>(T.U); N1 = T.N1; N2 = T.N2;
+        S4 = 4 * T.S; ns1 = T.ns1; ns2 = T.ns2; win = T.win; w = T.w; nst = T.nst; r0 = T.r0; nR = T.nR;
+        cb = q.y;
+        pr = q.z;
+        b = 0;
+    }
+    const int l2 = cb * blockDim.x + threadIdx.x;
     int l1 = r0 + pr;
     if (l1 >= N1) l1 -= N1;  // r0 < N1, pr < N1
-    const int b = blockIdx.z;
     if (l2 >= N2) return;
     const int q1 = l1 / S4, q2 = l2 / S4;
     // Wraps by one period at most: q - d >= -(ns - 1) for d < min(3, ns), and the offset of a cell
@@ -788,6 +817,37 @@ void launch_spread(const NufftDev &u, InCols f, C *U, hipStream_t s) {
                                                 u.nst, u.r0, u.nR);
 }
 
+// Plan groups: one launch per stage over the super-tiles (rows, for the merge) of many
+// single-transform plans of one kernel width and grid type.
+template <int W, class C>
+static void interp_group_w(const NufftTab *tabs, const int2 *work, int nwork, int win, hipStream_t s) {
+    interp_st_kernel<W, C, true><<<dim3(nwork), 256, sizeof(C) * size_t(win) * win, s>>>(
+        nullptr, nullptr, nullptr, nullptr, OutCols(), nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, tabs, work);
+}
+
+template <int W, class C>
+static void spread_group_w(const NufftTab *tabs, const int2 *work, int nwork, int win, hipStream_t s) {
+    spread_st_kernel<W, 1, C, true><<<dim3(nwork), 64 * kSpreadWaves, spread_lds_bytes(win, 1), s>>>(
+        nullptr, nullptr, nullptr, InCols(), nullptr, nullptr, nullptr, 0, 0, 0, 0, 1, tabs, work);
+}
+
+template <class C>
+void launch_interp_group(const NufftTab *tabs, const int2 *work, int nwork, int w, int win, hipStream_t s) {
+    if (nwork <= 0) return;
+    PFSHT_W_SWITCH(w, interp_group_w, tabs, work, nwork, win, s);
+}
+
+template <class C>
+void launch_spread_group(const NufftTab *tabs, const int2 *work, int nwork, const int4 *mwork, int nmwork, int w,
+                         int win, hipStream_t s) {
+    if (nwork > 0) {
+        PFSHT_W_SWITCH(w, spread_group_w, tabs, work, nwork, win, s);
+    }
+    if (nmwork > 0)
+        spread_merge_kernel<C, true><<<dim3(nmwork), 256, 0, s>>>(nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, 0,
+                                                                  0, 0, tabs, mwork);
+}
+
 #define PFSHT_INSTANTIATE(C)                                                                                   \
     template void launch_deconv_pad_ut<C>(const NufftDev &, const double2 *, int, int, double, const C *, C *,  \
                                           hipStream_t);                                                         \
@@ -796,7 +856,10 @@ void launch_spread(const NufftDev &u, InCols f, C *U, hipStream_t s) {
     template void launch_ut_to_rows<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
     template void launch_rows_to_ut<C>(const NufftDev &, int, const C *, C *, hipStream_t);                     \
     template void launch_interp<C>(const NufftDev &, const C *, OutCols, hipStream_t);                          \
-    template void launch_spread<C>(const NufftDev &, InCols, C *, hipStream_t);
+    template void launch_spread<C>(const NufftDev &, InCols, C *, hipStream_t);                                 \
+    template void launch_interp_group<C>(const NufftTab *, const int2 *, int, int, int, hipStream_t);           \
+    template void launch_spread_group<C>(const NufftTab *, const int2 *, int, const int4 *, int, int, int,      \
+                                         hipStream_t);
 PFSHT_INSTANTIATE(double2)
 PFSHT_INSTANTIATE(float2)
 #undef PFSHT_INSTANTIATE

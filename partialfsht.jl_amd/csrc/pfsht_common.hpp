@@ -270,6 +270,23 @@ struct NufftDev {
     int32_t *bin_ofs = nullptr;           // [nst*17] per slot: the 16 local bins' starts, then the end
     void *Wst = nullptr;                  // [B][nst][win][win] super-tile windows (grid storage type)
 };
+// One single-transform plan's NUFFT as the plan-group launches read it (device memory); U and Wst
+// hold the plan's grid type.
+struct NufftTab {
+    const double *X1, *X2;
+    const int32_t *perm, *st_list, *st_slot, *bin_ofs;
+    void *U, *Wst;
+    InCols fin;    // the adjoint's field
+    OutCols fout;  // the synthesis's field
+    int N1, N2, S, ns1, ns2, win, w, nst, r0, nR;
+};
+// Plan groups: interpolation over {plan, super-tile} work, spreading over {plan, super-tile} then the
+// merge over {plan, column block, stored row, 0}; every plan of one launch has kernel width w.
+template <class C>
+void launch_interp_group(const NufftTab *tabs, const int2 *work, int nwork, int w, int win, hipStream_t s);
+template <class C>
+void launch_spread_group(const NufftTab *tabs, const int2 *work, int nwork, const int4 *mwork, int nmwork, int w,
+                         int win, hipStream_t s);
 // LDS of one interpolation workgroup: the super-tile window of the fine grid.
 inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(win) * win; }
 // LDS of one spreading workgroup: the window plus, per wavefront, the staging of one 16-point chunk
