@@ -83,6 +83,7 @@ def lib():
             "or_stretch_to_theta": (_D, [C.POINTER(GeomC), _D]),
             "or_next_smooth": (_I64, [_I64]),
             "or_next_smooth_even": (_I64, [_I64]),
+            "or_es_fine_size": (_I, [_I, _I]),
             "or_dfs_double": (None, [_P, _P, _I, _I, _I, _I]),
             "or_dfs_fold": (None, [_P, _P, _I, _I, _I, _I]),
             "or_fft1": (None, [_P, _I, _I]),
@@ -324,6 +325,11 @@ def next_smooth(n):
 
 def next_smooth_even(n):
     return int(lib().or_next_smooth_even(int(n)))
+
+
+def es_fine_size(n, w):
+    """Fine-grid length of the ES NUFFT for a length-n axis at kernel width w."""
+    return int(lib().or_es_fine_size(int(n), int(w)))
 
 
 # --- DFS.jl, FFT, NUFFT.jl -------------------------------------------------------------------

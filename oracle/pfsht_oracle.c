@@ -1207,8 +1207,8 @@ int or_nufft_direct(int type, int n1, int n2, int iflag, int B, const double *x,
 
 /* ---------------------------------------------------------------------------------------------
  * ES-kernel NUFFT (stand-in for FINUFFT). Kernel psi(x) = phi(x / alpha), phi(z) =
- * exp(beta (sqrt(1 - z^2) - 1)) on |z| < 1, alpha = w h / 2, fine grid N = next_smooth_even(2 n)
- * (at least 2w), h = 2 pi / N. Type 2 with sign s:
+ * exp(beta (sqrt(1 - z^2) - 1)) on |z| < 1, alpha = w h / 2, fine grid N >= 2 n (es_fine_size),
+ * h = 2 pi / N. Type 2 with sign s:
  *   u_l = sum_k c_k (h / psihat(k)) e^{i s k l h},  f_j = sum_l psi(x_j - l h) u_l;
  * type 1 (sign s) is the exact transpose: spread, FFT with sign s, multiply by h / psihat(k).
  * Shape beta = 2.30 w as published for upsampling 2; the width is one wider than the published
@@ -1227,11 +1227,22 @@ void or_es_params(double tol, int *w, double *beta) {
     *beta = bo * ww;
 }
 
+/* The fine length is a free parameter of the ES NUFFT (any N >= 2n; a larger upsampling factor only
+ * lowers the aliasing error at the same width). The oracle takes the library's choice so that the
+ * two compare at round-off: next_smooth_even(max(2n, 2w)), and above 4096 the next of 6144, 8192,
+ * 9216, 10000, 10240 (the lengths rocFFT runs in one kernel there). The ES checks against the direct
+ * sum hold at every N. */
 static int es_fine_size(int n, int w) {
-    int64_t N = or_next_smooth_even(2 * (int64_t)n);
-    if (N < 2 * w) N = or_next_smooth_even(2 * w);
+    int64_t m = 2 * (int64_t)n > 2 * w ? 2 * (int64_t)n : 2 * w;
+    int64_t N = or_next_smooth_even(m);
+    static const int64_t fast[5] = {6144, 8192, 9216, 10000, 10240};
+    if (N > 4096)
+        for (int i = 0; i < 5; i++)
+            if (fast[i] >= N) return (int)fast[i];
     return (int)N;
 }
+
+int or_es_fine_size(int n, int w) { return es_fine_size(n, w); }
 
 /* Gauss-Legendre nodes on [-1,1] by Newton on P_n. */
 static void gauss_legendre(int n, double *x, double *w) {

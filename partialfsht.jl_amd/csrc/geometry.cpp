@@ -588,9 +588,17 @@ void es_params(double tol, int &w, double &beta) {
     beta = bo * ww;
 }
 
+// The fine grid is any length N >= 2n (a larger upsampling factor only lowers the kernel's aliasing
+// error at the same width), chosen for rocFFT on gfx950: up to 4096 the next even 5-smooth length
+// (one kernel per transform at every such length); above 4096 rocFFT runs most lengths as a
+// multi-kernel plan with transposes through a work buffer at a third to a fifth of the single-kernel
+// bandwidth, so the grid takes the next of the lengths it still runs in one kernel (measured over
+// all even 5-smooth lengths up to 10240, FP32 and FP64).
 int es_fine_size(int n, int w) {
-    int64_t N = next_smooth_even(2 * int64_t(n));
-    if (N < 2 * w) N = next_smooth_even(2 * w);
+    int64_t N = next_smooth_even(std::max<int64_t>(2 * int64_t(n), 2 * w));
+    if (N > 4096)
+        for (int64_t f : {6144, 8192, 9216, 10000, 10240})
+            if (f >= N) return int(f);
     return int(N);
 }
 

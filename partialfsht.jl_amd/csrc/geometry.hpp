@@ -87,6 +87,7 @@ double cost_stretch(double thlo, double thhi, int64_t npts, int L, int B, double
 // Exponential-of-semicircle NUFFT kernel (Barnett, Magland & af Klinteberg 2019), upsampling 2:
 // width w = ceil(log10(1/tol)) + 2, shape beta = 2.30 w.
 void es_params(double tol, int &w, double &beta);
+// Fine-grid length for a length-n axis (at least 2n and 2w; see geometry.cpp).
 int es_fine_size(int n, int w);
 // corr[i] = h / psihat(k_i): deconvolution factors for the modes of a length-n axis in FFT order.
 void es_corr(int n, int N, int w, double beta, std::vector<double> &corr);

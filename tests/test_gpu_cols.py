@@ -155,6 +155,17 @@ def test_plan_keywords(torch, P, O, rng):
     sub = rng.choice(M, 300, replace=False)
     pd = O.Plan(th[sub], ph[sub], L, method="direct", nthreads=NT)
     assert relmax((over @ a)[sub], pd.synth(a)) < 10 * tol
+    # a longitude grid past 2048 (oversample_phi 12): the fine length is past 4096 and takes the next
+    # single-kernel rocFFT length; both directions stay within 10 tol of Direct
+    wide = P.Plan(th, ph, lmax=L, tol=tol, method="stretch", oversample_phi=12.0)
+    assert wide.info["nph"] > 2048 and wide.info["es_n2"] == O.es_fine_size(wide.info["nph"], wide.info["es_w"])
+    assert wide.info["es_n2"] in (6144, 8192, 9216, 10000, 10240)
+    assert relmax((wide @ a)[sub], pd.synth(a)) < 10 * tol
+    fw = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    fz = np.zeros(M, complex)
+    fz[sub] = fw[sub]
+    m = O.valid_mask(L)
+    assert relmax((wide.H @ fz)[m], pd.adjoint(fw[sub])[m]) < 10 * tol
     # εapo_max: a cap the default admits is refused below its apodisation fraction
     ea = g.epsapo
     assert O.stretch_geometry(th.min(), th.max(), L, tol, epsapo_max=0.9 * ea) is None

@@ -182,6 +182,38 @@ def test_es_nufft_vs_direct_sum(O, rng, tol, typ):
     assert relmax(got, ref) < 200 * tol
 
 
+def test_es_fine_size_rule(O):
+    # at least 2n and 2w; the next even 5-smooth length up to 4096, then the next of the lengths
+    # rocFFT runs in one kernel; past 10240 the next even 5-smooth length again
+    cases = {(720, 10): 1440, (1920, 10): 3840, (2048, 10): 4096, (2049, 10): 6144, (2430, 7): 6144,
+             (3072, 7): 6144, (3073, 7): 8192, (4097, 10): 9216, (4700, 10): 10000, (5060, 7): 10240,
+             (5200, 7): O.next_smooth_even(10400), (3, 10): 20, (60, 10): 120}
+    for (n, w), N in cases.items():
+        assert O.es_fine_size(n, w) == N, (n, w)
+    for n in range(1, 6000, 37):
+        N = O.es_fine_size(n, 10)
+        assert N >= 2 * n and N >= 20 and N % 2 == 0
+
+
+@pytest.mark.parametrize("typ", [1, 2])
+def test_es_nufft_vs_direct_sum_long_axis(O, rng, typ):
+    # an axis whose fine length (6144 for n = 2430) is not the next smooth length: upsampling above
+    # 2 keeps the published accuracy
+    n1, n2, M, tol = 6, 2430, 120, 1e-8
+    assert O.es_fine_size(n2, 10) == 6144
+    x = rng.uniform(0, np.pi, M)
+    y = rng.uniform(0, 2 * np.pi, M)
+    if typ == 2:
+        c = rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2))
+        ref = O.nufft_direct(2, n1, n2, +1, x, y, c)
+        got = O.nufft_es(2, n1, n2, +1, x, y, c, tol)
+    else:
+        f = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+        ref = O.nufft_direct(1, n1, n2, -1, x, y, f)
+        got = O.nufft_es(1, n1, n2, -1, x, y, f, tol)
+    assert relmax(got, ref) < 200 * tol
+
+
 def test_es_nufft_adjoint_pair(O, rng):
     # type 1 (iflag -1) is the exact adjoint of type 2 (iflag +1)
     n1, n2, M = 30, 44, 500
