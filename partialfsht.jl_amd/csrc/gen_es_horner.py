@@ -23,14 +23,21 @@ over the cells, sampled at 801 points, is within target(w) = max(2e-16, 1e-5 tol
 kernel's own truncation error at that tolerance stays far above the polynomial's. The tables
 are compile-time constants of the device code, so the coefficients are immediate operands and
 never loaded.
+
+EsHorner32<w> (w = 2..8, the widths a mixed-precision plan can pick, tol >= 1e-6) holds the same
+polynomials for FP32 evaluation: the degrees are the smallest whose coefficients, rounded to float
+and run through Horner in float arithmetic, stay within TARGET32 of the kernel on the same samples.
 """
 import os
+import struct
 
 import mpmath as mp
 
 mp.mp.dps = 60
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEG_INT, DEG_EDGE = 14, 22  # largest degrees
+TARGET32 = 3e-7  # FP32 tables: a few float roundings of the kernel's peak value 1
+W32 = range(2, 9)
 
 
 def target(w):
@@ -54,6 +61,31 @@ def _fit(fns, dmax, tgt):
         cs = [cheb_monomial(fn, d) for fn in fns]
         if d == dmax or max(_maxerr(fn, c) for fn, c in zip(fns, cs)) <= tgt:
             return cs
+
+
+def _f32(x):
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+def _maxerr32(fn, c):
+    """Largest error of Horner in float arithmetic (each step rounded, as an FMA rounds once) with
+    the coefficients rounded to float, at float-rounded arguments."""
+    cf = [_f32(a) for a in c]
+    worst = 0.0
+    for i in range(801):
+        x = _f32(-1 + 2 * i / 800)
+        acc = 0.0
+        for a in reversed(cf):
+            acc = _f32(acc * x + a)
+        worst = max(worst, abs(acc - float(fn(mp.mpf(x)))))
+    return worst
+
+
+def _fit32(fns, dmax):
+    for d in range(3, dmax + 1):
+        cs = [cheb_monomial(fn, d) for fn in fns]
+        if d == dmax or max(_maxerr32(fn, c) for fn, c in zip(fns, cs)) <= TARGET32:
+            return [[_f32(a) for a in c] for c in cs]
 
 
 def es_beta(w):
@@ -97,7 +129,7 @@ def cheb_monomial(fn, d):
     return [float(x) for x in a]
 
 
-def tables(w):
+def tables(w, single=False):
     beta = mp.mpf(es_beta(w))
     fns = []
     for t in range(1, w - 1):
@@ -111,6 +143,9 @@ def tables(w):
     def edge(e):
         v = (e + 1) / 2 * vmax
         return mp.e ** (beta * (v * mp.sqrt(2 - v * v) - 1))
+    if single:
+        ints = _fit32(fns, DEG_INT) if fns else []
+        return ints, _fit32([edge], DEG_EDGE)[0]
     ints = _fit(fns, DEG_INT, target(w)) if fns else []
     return ints, _fit([edge], DEG_EDGE, target(w))[0]
 
@@ -131,6 +166,18 @@ def main():
                    f"    static constexpr double beta = {es_beta(w).hex()};\n"
                    f"    static constexpr double c[{len(vals)}] = {{\n        {body}}};\n}};")
         print(f"w {w}: interior degree {nint - 1}, edge degree {len(edge) - 1}", flush=True)
+    out.append("// EsHorner32<w>: the same polynomials for FP32 evaluation (float coefficients, degrees fitted")
+    out.append("// for Horner in float arithmetic), for the widths of mixed-precision plans.")
+    out.append("template <int W> struct EsHorner32;")
+    for w in W32:
+        ints, edge = tables(w, single=True)
+        nint = len(ints[0]) if ints else 1
+        vals = [x for p in ints for x in p] + edge
+        body = ",\n        ".join(", ".join(v.hex() + "f" for v in vals[i:i + 3]) for i in range(0, len(vals), 3))
+        out.append(f"template <> struct EsHorner32<{w}> {{\n"
+                   f"    static constexpr int nint = {nint}, nedge = {len(edge)};\n"
+                   f"    static constexpr float c[{len(vals)}] = {{\n        {body}}};\n}};")
+        print(f"w {w} (FP32): interior degree {nint - 1}, edge degree {len(edge) - 1}", flush=True)
     with open(os.path.join(HERE, "es_horner_tables.inc"), "w") as fh:
         fh.write("\n".join(out) + "\n")
 

@@ -16,6 +16,7 @@
 // waves compute four bins' windows concurrently, then add them into the LDS super-tile window one
 // wave after another, so every cell is summed in a fixed order. A merge pass then sums, for every
 // fine-grid cell, the (at most 2 x 2) super-tile windows covering it in a fixed order.
+#include <type_traits>
 #include <utility>
 
 #include "pfsht_common.hpp"
@@ -750,13 +751,300 @@ __global__ void spread_merge_kernel(const C *__restrict__ Wst, const int32_t *__
     stc(U + (int64_t(b) * nR + pr) * N2 + l2, make_double2(sr, si));
 }
 
+// ------------------------------------------------------------------------------------------------
+// FP32 arithmetic for mixed-precision plans (tol >= 1e-6, so kernel width w <= 8; FP32 grids). The
+// kernel values come from EsHorner32 by Horner in float (one v_fmaak_f32 per step, the coefficient a
+// literal operand: no scalar moves); interpolation sums the FP32 window in float; spreading
+// accumulates on v_mfma_f32_16x16x4f32 into FP32 tiles and an FP32 LDS window. The coordinates and
+// base cells stay FP64 (the cell index is exact); the offset u within the cell is rounded to float.
+// The fields in the caller's memory stay complex FP64 (loads narrow, stores widen).
+// ------------------------------------------------------------------------------------------------
+template <int W, int OFF, int N>
+__device__ __forceinline__ float horner32(float x) {
+    using T = EsHorner32<W>;
+    float a = T::c[OFF + N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) a = fmaf(a, x, T::c[OFF + i]);
+    return a;
+}
+
+// edge-polynomial argument e = 2 sqrt((1 + v) / 2) - 1 (v_sqrt_f32: 1 ulp)
+__device__ __forceinline__ float es_edge_arg32(float v) {
+    return 2.0f * __builtin_amdgcn_sqrtf(0.5f * (1.0f + v)) - 1.0f;
+}
+
+template <int W, int... T>
+__device__ __forceinline__ void es_interior32(float u, float (&k)[W], std::integer_sequence<int, T...>) {
+    constexpr int ni = EsHorner32<W>::nint;
+    ((k[T + 1] = horner32<W, T * ni, ni>(u)), ...);
+}
+
+// All W kernel values of a point on one axis (cell t at k[t]), as es_rows does in FP64.
+template <int W>
+__device__ __forceinline__ void es_row32(float u, float (&k)[W]) {
+    using T = EsHorner32<W>;
+    constexpr int EOFF = (W - 2) * T::nint;
+    k[0] = horner32<W, EOFF, T::nedge>(es_edge_arg32(u));
+    k[W - 1] = horner32<W, EOFF, T::nedge>(es_edge_arg32(-u));
+    es_interior32<W>(u, k, std::make_integer_sequence<int, W - 2>{});
+}
+
+template <int W, int... T>
+__device__ __forceinline__ void es_half_interior32(float v, float (&k)[(W + 1) / 2],
+                                                   std::integer_sequence<int, T...>) {
+    constexpr int ni = EsHorner32<W>::nint;
+    ((k[T + 1] = horner32<W, T * ni, ni>(v)), ...);
+}
+
+// Half a row (cells t < ceil(W/2)) from the left for v = u, from the right for v = -u, as es_half_row.
+template <int W>
+__device__ __forceinline__ void es_half_row32(float v, float (&k)[(W + 1) / 2]) {
+    using T = EsHorner32<W>;
+    constexpr int NH = (W + 1) / 2, EOFF = (W - 2) * T::nint;
+    k[0] = horner32<W, EOFF, T::nedge>(es_edge_arg32(v));
+    es_half_interior32<W>(v, k, std::make_integer_sequence<int, NH - 1>{});
+}
+
+// Interpolation in FP32, one or two transforms: as interp_st_kernel (one workgroup per super-tile
+// and transform, the window staged in LDS), with the window's float2 cells summed in float.
+template <int W, bool GROUP = false>
+__global__ void __launch_bounds__(256)
+interp_st32_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
+                   const float2 *__restrict__ U, OutCols f, const int32_t *__restrict__ st_list,
+                   const int32_t *__restrict__ bin_ofs, int N1, int N2, int S4, int ns2, int win, int r0, int nR,
+                   const NufftTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
+    extern __shared__ __align__(16) unsigned char lds_ip32_raw[];
+    float2 *lw = reinterpret_cast<float2 *>(lds_ip32_raw);
+    int slot = blockIdx.x, b = blockIdx.y;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const NufftTab &T = tabs[w.x];
+        X1 = T.X1; X2 = T.X2; perm = T.perm; U = static_cast<const float2 *>(T.U); f = T.fout;
+        st_list = T.st_list; bin_ofs = T.bin_ofs; N1 = T.N1; N2 = T.N2; S4 = 4 * T.S; ns2 = T.ns2; win = T.win;
+        r0 = T.r0; nR = T.nR;
+        slot = w.y;
+        b = 0;
+    }
+    const int st = st_list[slot];
+    const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
+    const float2 *__restrict__ Ub = U + int64_t(b) * nR * N2;
+    const gptr fcol = f.col(b);
+    for (int r = threadIdx.x >> 6; r < win; r += 4) {
+        int pr = (x0 + r) % N1 - r0;
+        if (pr < 0) pr += N1;
+        if (pr >= nR) continue;  // a row that no point's kernel reaches is neither stored nor read
+        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + (y0 + c) % N2];
+    }
+    __syncthreads();
+    const int start = bin_ofs[slot * 17], end = bin_ofs[slot * 17 + 16];
+    for (int p = start + int(threadIdx.x); p < end; p += 256) {
+        const double x1 = X1[p], x2 = X2[p];
+        const int b1 = int(ceil(x1 - 0.5 * W));
+        const int b2 = int(ceil(x2 - 0.5 * W));
+        float k1[W], k2[W];
+        es_row32<W>(float(2.0 * (double(b1) - x1) + double(W - 1)), k1);
+        es_row32<W>(float(2.0 * (double(b2) - x2) + double(W - 1)), k2);
+        const float2 *__restrict__ wrow = lw + (b1 - x0) * win + (b2 - y0);
+        float ar = 0.0f, ai = 0.0f;
+#pragma unroll
+        for (int t1 = 0; t1 < W; ++t1) {
+            float rr = 0.0f, ri = 0.0f;
+#pragma unroll
+            for (int t2 = 0; t2 < W; ++t2) {
+                const float2 v = wrow[t1 * win + t2];
+                rr = fmaf(k2[t2], v.x, rr);
+                ri = fmaf(k2[t2], v.y, ri);
+            }
+            ar = fmaf(k1[t1], rr, ar);
+            ai = fmaf(k1[t1], ri, ai);
+        }
+        gstore(fcol + perm[p], make_double2(double(ar), double(ai)));
+    }
+}
+
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+// Spreading in FP32 for G transforms per workgroup: the structure of spread_st_kernel (bins per
+// wavefront, 16-point chunks with the next chunk's inputs in flight, zeroed staging rows, wave-ordered
+// adds into the LDS window), with FP32 kernel rows, FP32 MFMA tiles and an FP32 window.
+// D[i][j] of v_mfma_f32_16x16x4f32: lane 16 q + j, register r holds row i = 4 q + r (A and B as for
+// the FP64 instruction: A[i][k] in lane 16 k + i, B[k][j] in lane 16 k + j; checked on gfx950 with a
+// one-wave layout probe).
+template <int W, int G, bool GROUP = false>
+__global__ void __launch_bounds__(64 * kSpreadWaves)
+spread_st32_kernel(const double *__restrict__ X1, const double *__restrict__ X2, const int32_t *__restrict__ perm,
+                   InCols f, const int32_t *__restrict__ st_list, const int32_t *__restrict__ bin_ofs,
+                   float2 *__restrict__ Wst, int S, int ns2, int win, int nst, int nb,
+                   const NufftTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
+    constexpr int NH = (W + 1) / 2;
+    extern __shared__ float lds_sp32[];
+    float2 *lw = reinterpret_cast<float2 *>(lds_sp32);  // [win][win]
+    int slot = blockIdx.x, b0 = blockIdx.y * G;
+    if constexpr (GROUP) {
+        const int2 w = work[blockIdx.x];
+        const NufftTab &T = tabs[w.x];
+        X1 = T.X1; X2 = T.X2; perm = T.perm; f = T.fin; st_list = T.st_list; bin_ofs = T.bin_ofs;
+        Wst = static_cast<float2 *>(T.Wst); S = T.S; ns2 = T.ns2; win = T.win; nst = T.nst; nb = 1;
+        slot = w.y;
+        b0 = 0;
+    }
+    const int gc = min(G, nb - b0);
+    const int st = st_list[slot];
+    const int S4 = 4 * S;
+    const int x0 = (st / ns2) * S4, y0 = (st % ns2) * S4;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    float *K1 = lds_sp32 + 2 * win * win + wave * spread_stage_floats(G);  // [16][kRowPad32] row kernel
+    float *K2 = K1 + 16 * kRowPad32;                                      // [16][kRowPad32] column kernel
+    float2 *Fc = reinterpret_cast<float2 *>(K2 + 16 * kRowPad32);          // [G][16]
+    const int ii = lane & 15, kq = lane >> 4;
+    const int pa = lane & 15, axis = (lane >> 4) & 1, h = lane >> 5;
+    const double *__restrict__ Xa = axis == 0 ? X1 : X2;
+    gcptr fb[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) fb[g] = f.col(b0 + min(g, gc - 1));
+    for (int q = tid; q < win * win; q += 64 * kSpreadWaves) lw[q] = make_float2(0.0f, 0.0f);
+    const int32_t *__restrict__ bo = bin_ofs + slot * 17;
+    constexpr int NBW = 16 / kSpreadWaves;
+    float4v dr[NBW][G], di[NBW][G];
+#pragma unroll
+    for (int k = 0; k < NBW; ++k)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            dr[k][g] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+            di[k][g] = float4v{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    int bs[NBW], be[NBW];
+#pragma unroll
+    for (int k = 0; k < NBW; ++k) {
+        bs[k] = bo[wave + kSpreadWaves * k];
+        be[k] = bo[wave + kSpreadWaves * k + 1];
+    }
+    auto advance = [&](int &k, int &c) {
+        while (k < NBW && c >= be[k]) {
+            ++k;
+            if (k < NBW) c = bs[k];
+        }
+    };
+    const int plast = max(bo[16] - 1, 0);
+    int k1 = 0, c1 = bs[0];
+    advance(k1, c1);
+    int k2 = k1, c2 = c1 + 16;
+    advance(k2, c2);
+    double x1 = Xa[min(c1 + pa, plast)];
+    double2 f1[G];
+    {
+        const int q1 = perm[min(c1 + pa, plast)];
+#pragma unroll
+        for (int g = 0; g < G; ++g) f1[g] = gload(fb[g] + q1);
+    }
+    int p2 = perm[min(c2 + pa, plast)];
+#pragma unroll
+    for (int kb = 0; kb < NBW; ++kb) {
+        while (k1 == kb) {  // wave-uniform
+            const int n = min(16, be[kb] - c1);
+            const double x = x1;
+            float2 fv[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) fv[g] = make_float2(float(f1[g].x), float(f1[g].y));
+            k1 = k2;
+            c1 = c2;
+            c2 += 16;
+            advance(k2, c2);
+            x1 = Xa[min(c1 + pa, plast)];
+#pragma unroll
+            for (int g = 0; g < G; ++g) f1[g] = gload(fb[g] + p2);
+            p2 = perm[min(c2 + pa, plast)];
+            float *row = (axis == 0 ? K1 : K2) + pa * kRowPad32;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0f;
+            if (pa < n) {
+                const int bc = int(ceil(x - 0.5 * W));
+                const int o = (bc - (axis == 0 ? x0 : y0)) % S;
+                const float u = float(2.0 * (double(bc) - x) + double(W - 1));
+                float kv[NH];
+                es_half_row32<W>(h == 0 ? u : -u, kv);
+#pragma unroll
+                for (int t = 0; t < NH; ++t)
+                    if (h == 0) row[o + t] = kv[t];
+                    else if (W - 1 - t != t) row[o + W - 1 - t] = kv[t];
+            }
+            if (axis == 0 && h == 0)
+#pragma unroll
+                for (int g = 0; g < G; ++g) Fc[g * 16 + pa] = pa < n ? fv[g] : make_float2(0.0f, 0.0f);
+            wave_lds_order();
+            float av[4], cv[4];
+#pragma unroll
+            for (int gi = 0; gi < 4; ++gi) {
+                const int q = 4 * gi + kq;
+                av[gi] = K1[q * kRowPad32 + ii];
+                cv[gi] = K2[q * kRowPad32 + ii];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g >= gc) break;  // wave-uniform
+                float2 fq[4];
+#pragma unroll
+                for (int gi = 0; gi < 4; ++gi) fq[gi] = Fc[g * 16 + 4 * gi + kq];
+#pragma unroll
+                for (int gi = 0; gi < 4; ++gi) {
+                    if (4 * gi >= n) break;  // wave-uniform
+                    dr[kb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[gi] * fq[gi].x, cv[gi], dr[kb][g], 0, 0, 0);
+                    di[kb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[gi] * fq[gi].y, cv[gi], di[kb][g], 0, 0, 0);
+                }
+            }
+            wave_lds_order();
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (g >= gc) break;  // uniform over the workgroup
+#pragma unroll 1
+        for (int turn = 0; turn < kSpreadWaves; ++turn) {
+            if (wave == turn) {
+#pragma unroll
+                for (int k = 0; k < NBW; ++k) {
+                    const int lb = wave + kSpreadWaves * k;
+                    const int bi = lb >> 2, bj = lb & 3;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        float2 &cell = lw[(bi * S + 4 * kq + r) * win + bj * S + ii];
+                        cell.x += dr[k][g][r];
+                        cell.y += di[k][g][r];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        float2 *out = Wst + (int64_t(b0 + g) * nst + slot) * int64_t(win) * win;
+        for (int q = tid; q < win * win; q += 64 * kSpreadWaves) {
+            out[q] = lw[q];
+            lw[q] = make_float2(0.0f, 0.0f);
+        }
+        __syncthreads();
+    }
+}
+
 // transforms per multi-transform interpolation workgroup (each evaluates the kernel rows once per point)
 constexpr int kInterpTransforms = 8;
+
+// FP32 grids (mixed-precision plans) at the widths EsHorner32 covers take the FP32 kernels.
+template <int W, class C>
+constexpr bool kFp32Math = std::is_same_v<C, float2> && W <= 8;
 
 template <int W, class C>
 static void interp_w(const NufftDev &u, const C *U, OutCols f, hipStream_t s) {
     if (u.nst == 0 || u.B <= 0) return;
     const size_t win_bytes = sizeof(C) * size_t(u.win) * u.win;
+    if constexpr (kFp32Math<W, C>) {
+        if (u.B <= 2) {
+            interp_st32_kernel<W><<<dim3(u.nst, u.B), 256, win_bytes, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list,
+                                                                          u.bin_ofs, u.N1, u.N2, 4 * u.S, u.ns2,
+                                                                          u.win, u.r0, u.nR);
+            return;
+        }
+    }
     if (u.B <= 2) {
         interp_st_kernel<W, C><<<dim3(u.nst, u.B), 256, win_bytes, s>>>(u.X1, u.X2, u.perm, U, f, u.st_list,
                                                                        u.bin_ofs, u.N1, u.N2, 4 * u.S, u.ns2,
@@ -773,6 +1061,19 @@ static void interp_w(const NufftDev &u, const C *U, OutCols f, hipStream_t s) {
 template <int W, class C>
 static void spread_w(const NufftDev &u, InCols f, C *Wst, hipStream_t s) {
     if (u.nst == 0 || u.B <= 0) return;
+    if constexpr (kFp32Math<W, C>) {
+        if (u.B >= 2) {
+            constexpr int G = kSpreadGroup;
+            spread_st32_kernel<W, G><<<dim3(u.nst, (u.B + G - 1) / G), 64 * kSpreadWaves,
+                                       spread32_lds_bytes(u.win, G), s>>>(u.X1, u.X2, u.perm, f, u.st_list,
+                                                                         u.bin_ofs, Wst, u.S, u.ns2, u.win,
+                                                                         u.nst, u.B);
+        } else {
+            spread_st32_kernel<W, 1><<<dim3(u.nst, 1), 64 * kSpreadWaves, spread32_lds_bytes(u.win, 1), s>>>(
+                u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
+        }
+        return;
+    }
     if (u.B >= 2) {
         constexpr int G = kSpreadGroup;
         dim3 grid(u.nst, (u.B + G - 1) / G);
@@ -821,14 +1122,22 @@ void launch_spread(const NufftDev &u, InCols f, C *U, hipStream_t s) {
 // single-transform plans of one kernel width and grid type.
 template <int W, class C>
 static void interp_group_w(const NufftTab *tabs, const int2 *work, int nwork, int win, hipStream_t s) {
-    interp_st_kernel<W, C, true><<<dim3(nwork), 256, sizeof(C) * size_t(win) * win, s>>>(
-        nullptr, nullptr, nullptr, nullptr, OutCols(), nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, tabs, work);
+    if constexpr (kFp32Math<W, C>)
+        interp_st32_kernel<W, true><<<dim3(nwork), 256, sizeof(C) * size_t(win) * win, s>>>(
+            nullptr, nullptr, nullptr, nullptr, OutCols(), nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, tabs, work);
+    else
+        interp_st_kernel<W, C, true><<<dim3(nwork), 256, sizeof(C) * size_t(win) * win, s>>>(
+            nullptr, nullptr, nullptr, nullptr, OutCols(), nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0, tabs, work);
 }
 
 template <int W, class C>
 static void spread_group_w(const NufftTab *tabs, const int2 *work, int nwork, int win, hipStream_t s) {
-    spread_st_kernel<W, 1, C, true><<<dim3(nwork), 64 * kSpreadWaves, spread_lds_bytes(win, 1), s>>>(
-        nullptr, nullptr, nullptr, InCols(), nullptr, nullptr, nullptr, 0, 0, 0, 0, 1, tabs, work);
+    if constexpr (kFp32Math<W, C>)
+        spread_st32_kernel<W, 1, true><<<dim3(nwork), 64 * kSpreadWaves, spread32_lds_bytes(win, 1), s>>>(
+            nullptr, nullptr, nullptr, InCols(), nullptr, nullptr, nullptr, 0, 0, 0, 0, 1, tabs, work);
+    else
+        spread_st_kernel<W, 1, C, true><<<dim3(nwork), 64 * kSpreadWaves, spread_lds_bytes(win, 1), s>>>(
+            nullptr, nullptr, nullptr, InCols(), nullptr, nullptr, nullptr, 0, 0, 0, 0, 1, tabs, work);
 }
 
 template <class C>

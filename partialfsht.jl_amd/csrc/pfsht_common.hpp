@@ -303,6 +303,14 @@ static_assert(16 % kSpreadWaves == 0 && kSpreadWaves >= 1, "kSpreadWaves must di
 inline size_t spread_lds_bytes(int win, int G) {
     return nufft_window_lds_bytes(win) + sizeof(double) * kSpreadWaves * spread_stage_doubles(G);
 }
+// The FP32 spreading kernel (mixed-precision plans): a float2 window and, per wavefront, two 16-row
+// float kernel tiles (rows kRowPad32 floats apart: the 16 lanes writing one entry of 16 rows fall in
+// different LDS banks) and G x 16 float2 f values.
+constexpr int kRowPad32 = 17;
+constexpr int spread_stage_floats(int G) { return 2 * 16 * kRowPad32 + 2 * 16 * G; }
+inline size_t spread32_lds_bytes(int win, int G) {
+    return sizeof(float2) * size_t(win) * win + sizeof(float) * kSpreadWaves * spread_stage_floats(G);
+}
 // pc1 selects the row factors: u.pc1 for the stretched apply, u.c1 for the bare NUFFT.
 void launch_deconv_pad(const NufftDev &u, const double2 *pc1, const double2 *Fhat, double2 *U, hipStream_t s);
 void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U, double2 *Fhat, hipStream_t s);

@@ -69,3 +69,48 @@ def test_tables_match_kernel(w):
             got = _horner(c[(w - 2) * hint:], e)
         worst = max(worst, float(np.abs(got.astype(np.longdouble) - exact).max()))
     assert worst < _bound(w), (w, worst)
+
+
+def _tables32():
+    txt = open(INC).read()
+    out = {}
+    for m in re.finditer(r"struct EsHorner32<(\d+)> \{\s*static constexpr int nint = (\d+), nedge = (\d+);\s*"
+                         r"static constexpr float c\[(\d+)\] = \{([^}]*)\}", txt):
+        w, hint, hedge = int(m.group(1)), int(m.group(2)), int(m.group(3))
+        vals = [float.fromhex(v.strip().rstrip("f")) for v in m.group(5).split(",")]
+        assert len(vals) == int(m.group(4)) == max(w - 2, 0) * hint + hedge
+        assert all(np.float32(v) == v for v in vals)  # exactly representable in float
+        out[w] = (hint, np.array(vals, np.float32))
+    return out
+
+
+def test_fp32_tables_cover_mixed_widths():
+    # mixed-precision plans take tol >= 1e-6, i.e. widths up to ceil(-log10 tol) + 2 = 8
+    assert sorted(_tables32()) == list(range(2, 9))
+
+
+@pytest.mark.parametrize("w", range(2, 9))
+def test_fp32_tables_match_kernel(w):
+    # the FP32 evaluation as the kernels run it: u rounded to float, e = 2 sqrt((1 +- u) / 2) - 1 and
+    # Horner in float arithmetic (each step rounded), against the kernel in extended precision; the
+    # bound is a few float roundings of the peak value 1 (FP32 kernels serve tol >= 1e-6 only)
+    hint, c = _tables32()[w]
+    beta = _beta_of(w)
+    x = 1000.0 + np.linspace(0.0, 1.0, 20001, endpoint=False)
+    b = np.ceil(x - 0.5 * w)
+    u = (2.0 * (b - x) + (w - 1)).astype(np.float32)
+    one, half, two = np.float32(1), np.float32(0.5), np.float32(2)
+    worst = 0.0
+    for t in range(w):
+        z = (b.astype(np.longdouble) + t - x) * (np.longdouble(2) / w)
+        exact = np.exp(np.longdouble(beta) * (np.sqrt(np.maximum(1 - z * z, 0)) - 1))
+        if 1 <= t <= w - 2:
+            cc, arg = c[(t - 1) * hint:t * hint], u
+        else:
+            s = one if t == 0 else -one
+            cc, arg = c[(w - 2) * hint:], two * np.sqrt(half * (one + s * u)) - one
+        acc = np.zeros_like(arg)
+        for a in reversed(cc):
+            acc = (acc * arg + a).astype(np.float32)
+        worst = max(worst, float(np.abs(acc.astype(np.longdouble) - exact).max()))
+    assert worst < 1e-6, (w, worst)
