@@ -21,6 +21,8 @@
 // and sums them against G with row-broadcast FMAs (see the adjoint section). Per-tile partials go
 // to a plan-owned buffer and a finalize kernel adds the tiles of each order in tile order: no
 // float atomics, bit-identical run to run.
+#include <type_traits>
+
 #include "pfsht_common.hpp"
 #include "adj_rows.hpp"
 
@@ -221,22 +223,31 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, g
         return *(const PFSHT_GLOBAL int *)(pbase + int64_t(lt) * 16);
     };
     int pa = 0, pb = touch(l + 64);
-    double2 gcv, ap, am = make_double2(0.0, 0.0);
-    group_load(g, gcv, ap, am);
-    while (g <= le) {
-        const int gn = g + 16;
-        double2 gcn, apn, amn = make_double2(0.0, 0.0);
-        group_load(gn, gcn, apn, amn);
+    // The coefficient lines of a group (gamma_l, C_l and a_lm of +m and -m), loaded one group ahead
+    // into the other of two buffers.
+    struct Lines {
+        double2 gcv, ap, am;
+    };
+    Lines A, B;
+    A.am = B.am = make_double2(0.0, 0.0);
+    group_load(g, A.gcv, A.ap, A.am);
+    // One group of degrees g .. g + 15 on the lines in `cur`: the next group's loads into `nxt`, the L2
+    // touch at a block start, the steps, the rescale to the q scale and the renormalisation at a
+    // block end. MODE 0: a full group; 1: a full group from step 2 (the first group of segment 0);
+    // 2: steps kfirst .. kend - 1 only.
+    auto group = [&](auto mode, const Lines &cur, Lines &nxt) {
+        constexpr int MODE = decltype(mode)::value;
+        group_load(g + 16, nxt.gcv, nxt.ap, nxt.am);
         if (((g - m) & 63) == 0) {
             sink ^= pa;
             pa = pb;
             pb = touch(g + kPrefetchAhead);
         }
-        const double C = gcv.y, ng = -gcv.x;
-        double c[4] = {ap.x * C, ap.y * C, am.x * C, am.y * C};
-        const int kend = min(16, le - g + 1);
-        if (kend == 16 && kfirst == 0) DppGroup<0, false, NEG>::run(x, r1, r2, ng, c, acc, 0, 16);
-        else if (kend == 16 && kfirst == 2) DppGroup<2, false, NEG>::run(x, r1, r2, ng, c, acc, 2, 16);
+        const double C = cur.gcv.y, ng = -cur.gcv.x;
+        double c[4] = {cur.ap.x * C, cur.ap.y * C, cur.am.x * C, cur.am.y * C};
+        const int kend = MODE == 2 ? min(16, le - g + 1) : 16;
+        if constexpr (MODE == 0) DppGroup<0, false, NEG>::run(x, r1, r2, ng, c, acc, 0, 16);
+        else if constexpr (MODE == 1) DppGroup<2, false, NEG>::run(x, r1, r2, ng, c, acc, 2, 16);
         else DppGroup<0, true, NEG>::run(x, r1, r2, ng, c, acc, kfirst, kend);
         // back to the q scale at the group end: C of the group's last degree, from lane kend - 1
         const int last = kend - 1;
@@ -244,7 +255,7 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, g
                                            __builtin_amdgcn_readlane(__double2loint(C), last));
         r1 *= Ce;
         r2 *= Ce;
-        g = gn;
+        g += 16;
         kfirst = 0;
         if (g <= le && ((g - m) & 63) == 0) {  // renormalise at the block end
             const int s = renorm_shift(r1, r2);
@@ -254,10 +265,26 @@ __device__ __forceinline__ void fwd_sweep_dpp(const double2 *__restrict__ gcm, g
             for (int u = 0; u < 4; ++u) acc[u] = ldexp(acc[u], -s);
             E += s;
         }
-        gcv = gcn;
-        ap = apn;
-        am = amn;
+    };
+    using Full = std::integral_constant<int, 0>;
+    using Guarded = std::integral_constant<int, 2>;
+    // The partial groups (the first of segment 0 starts at step 2; the order's last may end early)
+    // are peeled off, and the loop over full groups runs two at a time with the line buffers in
+    // alternation: one path, registers in place.
+    if (g <= le && (kfirst != 0 || g + 15 > le)) {
+        if (kfirst == 2 && g + 15 <= le) group(std::integral_constant<int, 1>{}, A, B);
+        else group(Guarded{}, A, B);
+        A = B;
     }
+    while (g + 31 <= le) {
+        group(Full{}, A, B);
+        group(Full{}, B, A);
+    }
+    if (g + 15 <= le) {
+        group(Full{}, A, B);
+        A = B;
+    }
+    if (g <= le) group(Guarded{}, A, B);
     sink ^= pa ^ pb;
 }
 
@@ -446,27 +473,30 @@ struct AdjTile {
     }
 };
 
-// Degrees ls .. le of order m for TG 64-ring tiles of that order (the second may be empty). Segment 0
-// (ls = m) starts from the seed; a later segment from its checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}),
-// aprev = a_{ls-1}. ls - m is a multiple of 64. Phase 1 runs the rescaled recurrence of the forward
-// sweep over each 16-degree sub-tile (gamma_l and C_l of lane k of a row give degree l0 + k); phase
-// 2 adds the tile's ring sums into the row accumulators; the tiles take turns in the one LDS tile,
-// and the accumulators, their sum over the four rows and the partial stores are shared. The ring
-// sums of degree l are multiplied by C_l once, after the reduction. For one or two transforms the
-// ring's exponent goes into its G operands (G 2^E, exact, refreshed when E changes at a block end)
-// and the tile holds r; with more transforms (registers) the tile holds r 2^E.
-template <int NB, bool NEG, int TG>
-__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTile<NB> (&T)[TG], bool two,
-                                          double aprev, int ls, int le, int m, int L, int B, double (*W)[kWS],
+// Degrees ls .. le of order m for the first NT of the TG 64-ring tiles in T (NT = 2 when the tile
+// group has a second tile). Segment 0 (ls = m) starts from the seed; a later segment from its
+// checkpoint (q1, q2) = (q_{ls-1}, q_{ls-2}), aprev = a_{ls-1}. ls - m is a multiple of 64. Phase 1
+// runs the rescaled recurrence of the forward sweep over each 16-degree sub-tile (gamma_l and C_l of
+// lane k of a row give degree l0 + k); phase 2 adds the tile's ring sums into the row accumulators;
+// the tiles take turns in the one LDS tile, and the accumulators, their sum over the four rows and
+// the partial stores are shared. The ring sums of degree l are multiplied by C_l once, after the
+// reduction. For one or two transforms the ring's exponent goes into its G operands (G 2^E, exact,
+// refreshed when E changes at a block end) and the tile holds r; with more transforms (registers)
+// the tile holds r 2^E. The sub-tiles holding the seed degrees or ending the order early are peeled
+// off the loop, so the loop over full sub-tiles has one path.
+template <int NB, bool NEG, int TG, int NT>
+__device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTile<NB> (&T)[TG], double aprev,
+                                          int ls, int le, int m, int L, int B, double (*W)[kWS],
                                           double4 *__restrict__ out, int &sink, const double (&qs)[TG],
                                           const double (&q1)[TG], const double (&q2)[TG]) {
+    static_assert(NT >= 1 && NT <= TG, "tiles in use");
     constexpr bool kScaleG = AdjTile<NB>::kScaleG;
     const int lane = threadIdx.x & 63;
     const int t = lane & 15, row = lane >> 4;
     const int wcol = lane + (lane >> 4);
     const double *__restrict__ wr = &W[t][17 * row];
 #pragma unroll
-    for (int k = 0; k < TG; ++k) T[k].scale_g();
+    for (int k = 0; k < NT; ++k) T[k].scale_g();
 
     // (gamma, C) lines of the first two 64-degree blocks, then two blocks ahead of the sweep: lane
     // i < 16 owns the line at byte offset 128 i of a 128-degree window
@@ -480,7 +510,7 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
     double2 gcv = coef(ls);
     int kfirst = 0;
 #pragma unroll
-    for (int k = 0; k < TG; ++k) {
+    for (int k = 0; k < NT; ++k) {
         AdjTile<NB> &A = T[k];
         if (ls == m) {  // seed degrees m and m + 1, stored as q (q 2^E)
             A.w0 = kScaleG ? qs[k] : ldexp(qs[k], A.E);
@@ -500,8 +530,12 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
             A.r2 = q2[k] * aprev;
         }
     }
-    for (int l0 = ls; l0 <= le; l0 += kSub) {
-        const int tl = min(kSub, le - l0 + 1);
+    int l0 = ls;
+    // One sub-tile of degrees l0 .. l0 + 15. MODE 0: full, no seed degrees; 1: full, seed degrees m
+    // and m + 1 in rows 0 and 1 (kfirst = 2); 2: rows kfirst .. tl - 1 only.
+    auto subtile = [&](auto mode) {
+        constexpr int MODE = decltype(mode)::value;
+        const int tl = MODE == 2 ? min(kSub, le - l0 + 1) : kSub;
         if (((l0 - m) & 63) == 0) {
             sink ^= pending;
             const int lp = l0 + 128;
@@ -509,7 +543,7 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
         }
         const double2 gcn = coef(l0 + kSub);
         const double ng = -gcv.x, Cc = gcv.y;
-        const double cmul = t < kfirst ? 1.0 : Cc;  // the seed degrees are stored unscaled
+        const double cmul = MODE == 0 ? Cc : (t < kfirst ? 1.0 : Cc);  // the seed degrees are stored unscaled
         // back to the q scale after the sub-tile: C of its last degree (the seed-only sub-tile ends
         // the sweep)
         const int last = tl - 1;
@@ -520,16 +554,17 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
 #pragma unroll
         for (int u = 0; u < NB; ++u) a[u][0] = a[u][1] = a[u][2] = a[u][3] = 0.0;
 #pragma unroll
-        for (int k = 0; k < TG; ++k) {
-            if (k > 0 && !two) break;  // wave-uniform: the group has one tile
+        for (int k = 0; k < NT; ++k) {
             AdjTile<NB> &A = T[k];
             // phase 1: lanes = rings
-            if (kfirst > 0) {
-                W[0][wcol] = A.w0;
-                if (kfirst == 2) W[1][wcol] = A.w1;
+            if constexpr (MODE != 0) {
+                if (kfirst > 0) {
+                    W[0][wcol] = A.w0;
+                    if (kfirst == 2) W[1][wcol] = A.w1;
+                }
             }
-            if (tl == kSub && kfirst == 0) AdjPhase1<0, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 0, kSub);
-            else if (tl == kSub && kfirst == 2) AdjPhase1<2, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 2, kSub);
+            if constexpr (MODE == 0) AdjPhase1<0, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 0, kSub);
+            else if constexpr (MODE == 1) AdjPhase1<2, false, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, 2, kSub);
             else AdjPhase1<0, true, !kScaleG>::run(A.x, A.r1, A.r2, ng, A.E, W, wcol, kfirst, tl);
             A.r1 *= Ce;
             A.r2 *= Ce;
@@ -570,7 +605,16 @@ __device__ __forceinline__ void adj_sweep(const double2 *__restrict__ gcm, AdjTi
         }
         gcv = gcn;
         kfirst = 0;
+        l0 += kSub;
+    };
+    using Full = std::integral_constant<int, 0>;
+    using Guarded = std::integral_constant<int, 2>;
+    if (l0 <= le && (kfirst > 0 || le - l0 + 1 < kSub)) {
+        if (kfirst == 2 && le - l0 + 1 >= kSub) subtile(std::integral_constant<int, 1>{});
+        else subtile(Guarded{});
     }
+    while (l0 + kSub - 1 <= le) subtile(Full{});
+    if (l0 <= le) subtile(Guarded{});
     sink ^= pending;
 }
 
@@ -655,8 +699,15 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     const double2 *__restrict__ gcm = gc + abofs[m] - (m + 2);
     const double aprev = sg == 0 ? 0.0 : abm[ls - 1].x;
     const bool two = nr2 > 0;
-    if (m > 0) adj_sweep<NB, true, TG>(gcm, T, two, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
-    else adj_sweep<NB, false, TG>(gcm, T, two, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+    // wave-uniform choices of the sweep: orders -m present, and whether the group has a second tile
+    if (two && TG > 1) {
+        constexpr int NT = TG > 1 ? 2 : 1;
+        if (m > 0) adj_sweep<NB, true, TG, NT>(gcm, T, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+        else adj_sweep<NB, false, TG, NT>(gcm, T, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+    } else {
+        if (m > 0) adj_sweep<NB, true, TG, 1>(gcm, T, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+        else adj_sweep<NB, false, TG, 1>(gcm, T, aprev, ls, le, m, L, B, W, out, sink, qs, q1, q2);
+    }
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
