@@ -630,8 +630,26 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
                 const double2 *__restrict__ ckq,
                 const int32_t *__restrict__ ckE, const int64_t *__restrict__ pofs, const double2 *__restrict__ Gd,
                 double4 *__restrict__ partials, int L, int n, int mmax, int B, int b0, int never,
-                const RingTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr) {
+                const RingTab *__restrict__ tabs = nullptr, const int2 *__restrict__ work = nullptr,
+                OutCols zalm = OutCols(), int nring = 0, int znb = 0) {
     extern __shared__ double lds_adj[];
+    if constexpr (!GROUP) {
+        // workgroups past the nring item workgroups (one launch per apply carries them) zero the
+        // columns of the orders above mmax in transforms 0 .. znb - 1, chunks of 2048 entries: launched
+        // last, they run in the pass's tail, and the partial reduction writes only orders <= mmax
+        if (nring > 0 && int(blockIdx.x) >= nring) {
+            const int64_t base = int64_t(2 * mmax + 1) * (L + 1), count = int64_t(2 * (L - mmax)) * (L + 1);
+            const int64_t nch = (count + 2047) / 2048;
+            const int64_t z = int64_t(blockIdx.x) - nring;
+            const int u = int(z / nch);
+            const int64_t c0 = (z % nch) * 2048;
+            if (u >= znb) return;
+            const gptr col = zalm.col(u) + base;
+            for (int i = threadIdx.x; i < 2048; i += 64 * kAdjWaves)
+                if (c0 + i < count) gstore(col + (c0 + i), make_double2(0.0, 0.0));
+            return;
+        }
+    }
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
     int wg = blockIdx.x;
     if constexpr (GROUP) {
@@ -711,8 +729,9 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
     if (sink == never) out[0] = make_double4(0.0, 0.0, 0.0, 0.0);  // never true; keeps the prefetch
 }
 
-// Writes every alm slot of transforms 0 .. nb - 1: the tile-ordered sum of the partials where (l, m)
-// is retained, zero in the padding and for unretained orders (the adjoint overwrites alm,
+// Writes the alm slots of orders |m| <= mmax (launched over those) or of every order (the plan
+// groups) in transforms 0 .. nb - 1: the tile-ordered sum of the partials where (l, m) is retained,
+// zero in the padding and for unretained orders (the adjoint overwrites alm,
 // src/Legendre.jl:338). One thread owns row l - |m| of the column pair (2|m|, 2|m| - 1), reading each
 // partial once. pb is the partials' transform stride.
 // GROUP: one workgroup per work entry {engine, row block, |m|} of many single-transform engines.
@@ -1204,12 +1223,18 @@ void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int 
     }
     if (r.naitems > 0)
         for_transform_groups(nb, [&](int ng, int b0) {
-            dim3 grid(r.naitems / kAdjWaves);
+            // the first launch carries the workgroups that zero the orders above mmax of all nb
+            // transforms (ring_adj_finalize writes orders <= mmax only)
+            const int nring = r.naitems / kAdjWaves;
+            const int64_t zcount = int64_t(2 * (r.L - r.mmax)) * (r.L + 1);
+            const int nzero = b0 == 0 ? int((zcount + 2047) / 2048) * nb : 0;
+            dim3 grid(nring + nzero);
             const size_t lds = sizeof(double) * kSub * kWS * kAdjWaves;
 #define PFSHT_ADJ(NB, TG)                                                                                    \
     ring_adj_kernel<NB, TG><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,  \
                                                               r.abofs, r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, \
-                                                              r.mmax, r.part_b, b0, kNeverSink)
+                                                              r.mmax, r.part_b, b0, kNeverSink, nullptr, nullptr, alm, \
+                                                              nzero > 0 ? nring : 0, nb)
             // the item list was built for r.adj_tg tiles per wavefront (two when only one or two
             // transforms use this path)
             switch (ng) {
@@ -1230,7 +1255,7 @@ void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int 
 
 void launch_ring_adj_finish(const RingDev &r, OutCols alm, int nb, hipStream_t s) {
     if (nb <= 0 || ring_adj_uses_mfma(r, nb)) return;
-    dim3 fgrid((r.L + 1 + 255) / 256, r.L + 1, nb);
+    dim3 fgrid((r.L + 1 + 255) / 256, r.naitems > 0 ? r.mmax + 1 : r.L + 1, nb);
     ring_adj_finalize<false><<<fgrid, 256, 0, s>>>(r.pofs, r.ntiles, r.partials, alm, r.L, r.mmax, r.part_b);
 }
 
