@@ -58,12 +58,16 @@ def rel_modes(O, a, b, L):
 # --------------------------------------------------------------------------------------------
 # Ring engine (gpu/runtests.jl:42-66; test_legendre.jl:87-98)
 # --------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("L", [128, 512])
-def test_ring_forward_and_adjoint_vs_oracle(torch, P, O, rng, L):
-    th = np.sort(0.02 + 0.5 * rng.uniform(size=64))
-    a = O.rand_alm_fast(L, rng)
-    Gh = O.Rings(th, L, 1e-8).rings_from_alm(a)
-    eng = P.Rings(th, L, 1e-8)
+# (lmax, rings, transforms): one 64-ring tile; two-tile groups with several degree segments (200
+# rings: tiles 64, 64, 64, 8); an odd tile count, whose last group has one tile (150 rings); two
+# transforms (the FP64 MFMA passes, one column block) and five (two column blocks, four + one)
+@pytest.mark.parametrize("L,nr,B", [(128, 64, 1), (512, 64, 1), (300, 200, 1), (300, 150, 1), (300, 200, 2),
+                                    (260, 150, 5)])
+def test_ring_forward_and_adjoint_vs_oracle(torch, P, O, rng, L, nr, B):
+    th = np.sort(0.02 + 0.5 * rng.uniform(size=nr))
+    a = O.rand_alm_fast(L, rng, ntrans=B) if B > 1 else O.rand_alm_fast(L, rng)
+    Gh = O.Rings(th, L, 1e-8, ntrans=B).rings_from_alm(a)
+    eng = P.Rings(th, L, 1e-8, ntrans=B)
     assert eng.work == O.Rings(th, L, 1e-8).work and eng.mmax == O.Rings(th, L, 1e-8).mmax
     Gd = torch.zeros(Gh.size, dtype=torch.complex128, device="cuda")
     eng.rings_from_alm(Gd, dev(torch, a))
@@ -73,7 +77,8 @@ def test_ring_forward_and_adjoint_vs_oracle(torch, P, O, rng, L):
     assert relmax(G, Gh) < 1e-12
     assert np.all(G[Gh == 0] == 0)
     Gr = rng.standard_normal(Gh.shape) + 1j * rng.standard_normal(Gh.shape)
-    ah = O.Rings(th, L, 1e-8).alm_from_rings(Gr)
+    Gr[Gh == 0] = 0  # entries off the retained set are never read
+    ah = O.Rings(th, L, 1e-8, ntrans=B).alm_from_rings(Gr)
     ad = torch.zeros(ah.size, dtype=torch.complex128, device="cuda")
     eng.alm_from_rings(ad, dev(torch, Gr))
     assert relmax(host(ad, ah.shape), ah) <= 1e-12

@@ -35,8 +35,10 @@ def _rel_valid(O, a, b, L, s):
     return float(np.abs(a[m] - b[m]).max() / np.abs(b[m]).max())
 
 
+# B = 2 and B = 5 cover the batched DPP forward with a group of two and with groups of four and one
+# (the second launch at transform offset 4)
 @pytest.mark.parametrize("L,s,reflect,B", [(128, 2, False, 1), (512, -1, False, 1), (256, 3, True, 1),
-                                           (200, 2, False, 3)])
+                                           (200, 2, False, 3), (160, 1, False, 2), (200, -2, True, 5)])
 def test_spin_engine_vs_oracle(torch, P, O, rng, L, s, reflect, B):
     th = np.sort(0.02 + 0.9 * rng.uniform(size=150))
     eng = P.Rings(th, L, 1e-8, ntrans=B, reflect=reflect, spin=s)
