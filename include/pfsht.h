@@ -45,6 +45,14 @@ enum { PFSHT_WHERE_HOST = 0, PFSHT_WHERE_DEVICE = 1 };
 /* Arithmetic: FP64 throughout, or FP64 recurrence with FP32 accumulation/FFT/NUFFT (tol >= 1e-6). */
 enum { PFSHT_PREC_FP64 = 0, PFSHT_PREC_MIXED = 1 };
 
+/* The stretched apply's grid stage (the FFTs between the ring pass and the NUFFT, src/Stretch.jl:
+ * 262-295): LDS kernels that run the colatitude, fine-column and fine-row FFTs inside workgroups,
+ * fused with the window, periodisation, deconvolution and transposes around them, or rocFFT plans
+ * with separate kernels between them. AUTO takes the LDS kernels wherever the fine lengths fit a
+ * workgroup's LDS (up to 10240 FP64 / 16384 FP32 values), LDS insists on them, LIBRARY takes
+ * rocFFT. No counterpart in the reference, whose FFTs are FFTW plans. */
+enum { PFSHT_GRID_FFT_AUTO = 0, PFSHT_GRID_FFT_LDS = 1, PFSHT_GRID_FFT_LIBRARY = 2 };
+
 /* Periodisation kinds (src/Windows.jl:34-36); -1 for a Direct leaf, -2 for a multi-box plan. */
 enum { PFSHT_KIND_POLE = 0, PFSHT_KIND_INTERIOR = 1, PFSHT_KIND_FULL = 2 };
 
@@ -68,6 +76,8 @@ typedef struct {
     int32_t nboxes;    /* boxes of a multi-box plan (1 otherwise) */
     int32_t spin;      /* spin weight (0: scalar) */
     int32_t seg_blocks, adj_tiles, mfma_tiles; /* the ring passes' launch shape (pfsht_ring_shape) */
+    int32_t grid_fft;  /* the grid stage taken: PFSHT_GRID_FFT_LDS or _LIBRARY (0 for a Direct leaf; a
+                          multi-box plan reports LIBRARY when any box takes it) */
 } pfsht_info;
 
 /* Number of timed stages reported by pfsht_stage_times. */
@@ -111,9 +121,10 @@ typedef struct {
     int32_t packed;          /* nonzero: FFT plans for partial transform counts */
     int32_t tune;            /* ring-pass shape: 0 time candidates when the shape is not cached,
                                 -1 never time (cached shape or the default), 1 always time */
+    int32_t grid_fft;        /* PFSHT_GRID_FFT_* (AUTO when 0) */
 } pfsht_plan_opts;
 /* Fills *opts with the defaults (all fields zero). `tune` has no counterpart in the reference, whose
- * ring pass has no launch shape: see pfsht_ring_shape. */
+ * ring pass has no launch shape: see pfsht_ring_shape; nor has `grid_fft` (PFSHT_GRID_FFT_*). */
 void pfsht_plan_opts_default(pfsht_plan_opts *opts);
 /* Plan(θ, φ; lmax, tol, ntrans, keywords...) with every keyword of src/Plan.jl:84-97 (opts may be
  * NULL for the defaults). oversample_θ, oversample_φ >= 1 and εapo_max in (0, 1] (src/Windows.jl). */

@@ -19,8 +19,8 @@ OBJ = os.path.join(HERE, "build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpfsht_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["geometry.cpp", "ring_kernels.hip", "grid_kernels.hip", "nufft_kernels.hip", "direct_kernels.hip",
-           "analysis_kernels.hip", "plan_kernels.hip", "spin_kernels.hip", "capi.hip",
+SOURCES = ["geometry.cpp", "ring_kernels.hip", "grid_kernels.hip", "grid_fft_kernels.hip", "nufft_kernels.hip",
+           "direct_kernels.hip", "analysis_kernels.hip", "plan_kernels.hip", "spin_kernels.hip", "capi.hip",
            "serve.hip", "tune_cache.cpp"]
 
 # No packed FP32 instructions anywhere; FP64 FMA contraction is left on for the kernels.

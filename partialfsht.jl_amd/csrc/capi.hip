@@ -391,6 +391,32 @@ static Status rings_build(const double *theta, int n, int L, double eps, int B, 
     return kOk;
 }
 
+// One work buffer for FFT plans that run one after another, attached to each plan that needs one.
+static Status fft_attach_work(const std::vector<FftPlan *> &plans, void *&work, int64_t &bytes) {
+    size_t wb = 0;
+    for (const FftPlan *q : plans) wb = std::max(wb, q->work_bytes);
+    if (!wb) return kOk;
+    Status st = dalloc(reinterpret_cast<char **>(&work), wb, bytes);
+    if (st != kOk) return st;
+    for (FftPlan *q : plans)
+        if (q->info && q->work_bytes) rocfft_execution_info_set_work_buffer(q->info, work, q->work_bytes);
+    return kOk;
+}
+
+// Whether the LDS grid stage takes the lengths nth, N1, N2 (nth <= N1) with elements of csz bytes:
+// each factors into radix-8/4/2/3/5 passes and the longer fine length fits one workgroup's LDS and
+// kFftElems x 1024 threads. Fills g's lengths when it does.
+static bool grid_fits(int nth, int N1, int N2, size_t csz, int dev, GridFftDev *g) {
+    int maxlds = 65536;
+    (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    GridFftDev t;
+    if (!fft_len_plan(nth, t.th) || !fft_len_plan(N1, t.c1) || !fft_len_plan(N2, t.r2)) return false;
+    const int n = std::max(N1, N2);
+    if (nth > N1 || n > kFftElems * 1024 || size_t(lds_slots(n)) * csz > size_t(maxlds)) return false;
+    if (g) { g->th = t.th; g->c1 = t.c1; g->r2 = t.r2; }
+    return true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Plan-time tuning of the ring passes' launch shape (cached in tune_cache.cpp).
 // ---------------------------------------------------------------------------------------------
@@ -636,9 +662,15 @@ struct pfsht_plan_s {
     // order-column layout [b][m + mmax][l1]. Elements are double2, or float2 in a mixed-precision
     // plan (the NUFFT seam is FP64 only).
     void *Ft = nullptr, *U = nullptr, *Ut = nullptr;
+    // The grid stage: the LDS kernels of grid_fft_kernels.hip (fused; Ut holds their T1 / T2), or,
+    // for lengths past a workgroup's LDS, rocFFT plans of the colatitude, fine-column and fine-row
+    // FFTs with the transposes and deconvolution kernels between them.
+    bool fused = false;
+    GridFftDev gf;
     FftSet th_fwd, th_bwd, col_bwd, col_fwd, row_bwd, row_fwd;
-    FftPlan fine_bwd, fine_fwd;  // the NUFFT seam's 2-D transforms (all transforms)
-    void *fft_work = nullptr;    // rocFFT work buffer shared by the plan's FFTs
+    void *fft_work = nullptr;    // rocFFT work buffer shared by those FFTs
+    FftPlan fine_bwd, fine_fwd;  // the NUFFT seam's 2-D transforms (all transforms), made at first use
+    void *seam_work = nullptr;   // their work buffer
     bool partial = false;        // FFT plans for every power-of-two transform count (packed applies)
     double2 *alm_stage = nullptr, *f_stage = nullptr, *modes_stage = nullptr;
     float2 *c64_stage = nullptr;  // ComplexF32 applies on host memory: input, then output
@@ -699,7 +731,7 @@ struct pfsht_plan_s {
         fftset_free(th_fwd); fftset_free(th_bwd);
         fft_free(fine_bwd); fft_free(fine_fwd); fftset_free(col_bwd); fftset_free(col_fwd);
         fftset_free(row_bwd); fftset_free(row_fwd);
-        dfree(fft_work);
+        dfree(fft_work); dfree(seam_work); dfree(gf.tw);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
@@ -1022,38 +1054,76 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     const size_t csz = single ? sizeof(float2) : sizeof(double2);
     auto galloc = [&](void **q, size_t count) { return dalloc(reinterpret_cast<char **>(q), count * csz, by); };
     if ((st = galloc(&u.Wst, size_t(B) * std::max(u.nst, 1) * u.win * u.win)) != kOk) return st;
-    // work buffers
-    if ((st = galloc(&p.Ft, size_t(B) * p.nth * p.nph)) != kOk) return st;  // nph >= 2 mmax + 1
+    // work buffers: the fine grid's stored rows (the NUFFT seam uses all N1 rows), and the nonzero
+    // fine columns in order-column layout (T1 / T2 of the LDS grid stage); the NUFFT seam's
+    // periodised grid Ft is made at its first use when the LDS grid stage runs the apply
     if ((st = galloc(&p.U, size_t(B) * u.N1 * u.N2)) != kOk) return st;
-    // rocFFT plans (row-major, contiguous; lengths fastest first)
-    // colatitude FFTs of the order columns (the longitude FFT pair of the reference cancels)
     const size_t ncol1 = size_t(2 * p.mmax + 1);
-    if ((st = fftset_make(p.th_fwd, true, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
-    if ((st = fftset_make(p.th_bwd, false, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
-    if (!single) {  // the NUFFT seam
-        if ((st = fft_make(p.fine_bwd, false, u.N2, u.N1, B)) != kOk) return st;
-        if ((st = fft_make(p.fine_fwd, true, u.N2, u.N1, B)) != kOk) return st;
-    }
-    // the apply's pruned fine transform: colatitude FFTs of the nonzero columns, longitude FFTs of rows
     if ((st = galloc(&p.Ut, size_t(B) * ncol1 * u.N1)) != kOk) return st;
-    if ((st = fftset_make(p.col_bwd, false, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
-    if ((st = fftset_make(p.col_fwd, true, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
-    if ((st = fftset_make(p.row_bwd, false, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
-    if ((st = fftset_make(p.row_fwd, true, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
-    {
-        std::vector<FftPlan *> all = {&p.fine_bwd, &p.fine_fwd};
+    p.fused = o.grid_fft != PFSHT_GRID_FFT_LIBRARY && grid_fits(p.nth, u.N1, u.N2, csz, p.dev, &p.gf);
+    if (o.grid_fft == PFSHT_GRID_FFT_LDS && !p.fused) {
+        set_error("the grid lengths (" + std::to_string(p.nth) + ", " + std::to_string(u.N1) + ", " +
+                  std::to_string(u.N2) + ") do not fit the LDS grid stage");
+        return kEUnsupported;
+    }
+    if (p.fused) {
+        GridFftDev &g = p.gf;
+        g.nt_col = grid_fft_threads(u.N1);
+        g.nt_row = grid_fft_threads(u.N2);
+        g.args.nh = p.nh; g.args.nth = p.nth; g.args.mmax = p.mmax; g.args.spin = p.spin; g.args.nph = p.nph;
+        g.args.N1 = u.N1; g.args.N2 = u.N2; g.args.r0 = u.r0; g.args.nR = u.nR;
+        g.args.scale = double(p.nph);
+        // exp(-2 pi i k / n), k < n, for the three lengths back to back
+        g.th.tw = 0;
+        g.c1.tw = p.nth;
+        g.r2.tw = p.nth + u.N1;
+        const size_t ntw = size_t(p.nth) + u.N1 + u.N2;
+        std::vector<double2> tw(ntw);
+        for (const FftLen *f : {&g.th, &g.c1, &g.r2})
+            for (int k = 0; k < f->n; ++k) {
+                const long double a = -2.0L * 3.141592653589793238462643383279502884L * k / f->n;
+                tw[size_t(f->tw) + k] = make_double2(double(cosl(a)), double(sinl(a)));
+            }
+        if (single) {
+            std::vector<float2> tw32(ntw);
+            for (size_t k = 0; k < ntw; ++k) tw32[k] = make_float2(float(tw[k].x), float(tw[k].y));
+            float2 *d = nullptr;
+            if ((st = upload(&d, tw32, by)) != kOk) return st;
+            g.tw = d;
+        } else {
+            double2 *d = nullptr;
+            if ((st = upload(&d, tw, by)) != kOk) return st;
+            g.tw = d;
+        }
+    } else {
+        // rocFFT plans (row-major, contiguous; lengths fastest first): colatitude FFTs of the order
+        // columns (the longitude FFT pair of the reference cancels); the pruned fine transform:
+        // colatitude FFTs of the nonzero columns, longitude FFTs of the stored rows
+        if ((st = galloc(&p.Ft, size_t(B) * p.nth * p.nph)) != kOk) return st;  // nph >= 2 mmax + 1
+        if ((st = fftset_make(p.th_fwd, true, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
+        if ((st = fftset_make(p.th_bwd, false, p.nth, ncol1, B, p.partial, single)) != kOk) return st;
+        if ((st = fftset_make(p.col_bwd, false, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
+        if ((st = fftset_make(p.col_fwd, true, u.N1, ncol1, B, p.partial, single)) != kOk) return st;
+        if ((st = fftset_make(p.row_bwd, false, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
+        if ((st = fftset_make(p.row_fwd, true, u.N2, size_t(u.nR), B, p.partial, single)) != kOk) return st;
+        std::vector<FftPlan *> all;
         for (FftSet *fs : {&p.th_fwd, &p.th_bwd, &p.col_bwd, &p.col_fwd, &p.row_bwd, &p.row_fwd})
             for (auto &q : fs->plans) all.push_back(&q);
-        size_t wb = 0;
-        for (FftPlan *q : all) wb = std::max(wb, q->work_bytes);
-        if (wb) {
-            if ((st = dalloc(reinterpret_cast<char **>(&p.fft_work), wb, by)) != kOk) return st;
-            for (FftPlan *q : all)
-                if (q->info && q->work_bytes) rocfft_execution_info_set_work_buffer(q->info, p.fft_work, q->work_bytes);
-        }
+        if ((st = fft_attach_work(all, p.fft_work, by)) != kOk) return st;
     }
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
+}
+
+// The NUFFT seam's periodised grid and 2-D rocFFT plans, made at its first call.
+static Status seam_ready(pfsht_plan_s &p) {
+    Status st;
+    if (!p.Ft && (st = dalloc(reinterpret_cast<double2 **>(&p.Ft), size_t(p.B) * p.nth * p.nph, p.bytes)) != kOk)
+        return st;
+    if (p.fine_bwd.plan) return kOk;
+    if ((st = fft_make(p.fine_bwd, false, p.nu.N2, p.nu.N1, p.B)) != kOk) return st;
+    if ((st = fft_make(p.fine_fwd, true, p.nu.N2, p.nu.N1, p.B)) != kOk) return st;
+    return fft_attach_work({&p.fine_bwd, &p.fine_fwd}, p.seam_work, p.bytes);
 }
 
 static GridDims dims_of(const pfsht_plan_s &p, int nb) {
@@ -1117,6 +1187,15 @@ static Status synth_grid(pfsht_plan_s &p, TimedApply *t, OutCols f, int nb, hipS
     NufftDev u = p.nu;
     u.B = nb;
     C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
+    if (p.fused) {  // stages 1-5 in the column kernel, the fine longitude FFT in the row kernel
+        launch_grid_synth_cols(p.gf, p.rings->d, p.rings->Gd, p.wth, u.pc1, u.c2, Ut, nb, s);
+        tmark(p, t, 5, s);
+        launch_grid_synth_rows(p.gf, static_cast<const C *>(Ut), U, nb, s);
+        tmark(p, t, 7, s);
+        if (interp) launch_interp(u, static_cast<const C *>(U), f, s);
+        tmark(p, t, 8, s);
+        return kOk;
+    }
     launch_spec_dfs(g, p.rings->d, p.rings->Gd, p.wth, Ft, s);
     tmark(p, t, 2, s);  // stages 2 and 3 (longitude FFT, DFS) have no work of their own
     Status st = fftset_exec(p.th_fwd, Ft, nb, s);
@@ -1142,6 +1221,13 @@ static Status adjoint_grid(pfsht_plan_s &p, TimedApply *t, InCols f, int nb, hip
     C *Ft = static_cast<C *>(p.Ft), *Ut = static_cast<C *>(p.Ut), *U = static_cast<C *>(p.U);
     if (spread) launch_spread(u, f, U, s);
     tmark(p, t, 1, s);
+    if (p.fused) {  // the fine longitude FFT in the row kernel, stages 2-6 in the column kernel
+        launch_grid_adjoint_rows(p.gf, static_cast<const C *>(U), Ut, nb, s);
+        tmark(p, t, 2, s);
+        launch_grid_adjoint_cols(p.gf, static_cast<const C *>(Ut), p.wth, u.pc1, u.c2, p.rings->Gd, nb, s);
+        tmark(p, t, 4, s);
+        return kOk;
+    }
     Status st = fftset_exec(p.row_fwd, U, nb, s);
     if (st != kOk) return st;
     launch_rows_to_ut(u, p.mmax, static_cast<const C *>(U), Ut, s);
@@ -1611,6 +1697,10 @@ static Status opts_normalise(const pfsht_plan_opts *opts, double tol, pfsht_plan
     }
     if (!(o.epsapo_max > 0) || !(o.epsapo_max <= 1)) { set_error("epsapo_max must lie in (0, 1]"); return kEArg; }
     if (o.tune < -1 || o.tune > 1) { set_error("tune must be -1, 0 or 1"); return kEArg; }
+    if (o.grid_fft < PFSHT_GRID_FFT_AUTO || o.grid_fft > PFSHT_GRID_FFT_LIBRARY) {
+        set_error("grid_fft must be 0 (auto), 1 (LDS) or 2 (library)");
+        return kEArg;
+    }
     return kOk;
 }
 
@@ -1698,9 +1788,11 @@ int pfsht_plan_info(pfsht_plan_t p, pfsht_info *info) {
         info->mfma_tiles = p->rings->shape.mfma_tg;
     }
     info->device_bytes = p->bytes + (p->rings ? p->rings->bytes : 0);
+    if (p->method == PFSHT_METHOD_STRETCH) info->grid_fft = p->fused ? PFSHT_GRID_FFT_LDS : PFSHT_GRID_FFT_LIBRARY;
     for (auto *q : p->boxes) {
         info->ring_work += q->rings ? q->rings->work : 0;
         info->device_bytes += q->bytes + (q->rings ? q->rings->bytes : 0);
+        info->grid_fft = std::max(info->grid_fft, q->fused ? int32_t(PFSHT_GRID_FFT_LDS) : int32_t(PFSHT_GRID_FFT_LIBRARY));
     }
     info->tol = p->tol;
     info->es_beta = p->nu.beta;
@@ -2196,6 +2288,8 @@ int pfsht_plan_nufft(pfsht_plan_t p, int32_t type, const void *in, void *out, in
     std::lock_guard<std::mutex> lk(p->mu);
     DeviceGuard dg(p->dev);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    Status st0 = seam_ready(*p);
+    if (st0 != kOk) return st0;
     NufftDev u = p->nu;  // the seam works on the whole fine grid
     u.r0 = 0;
     u.nR = u.N1;
@@ -2491,8 +2585,12 @@ static int64_t estimate_stretch(double thlo, double thhi, int64_t M, int L, doub
     const int S4 = 4 * (17 - w), win = 3 * (17 - w) + 16;
     const int64_t nst = std::min<int64_t>(int64_t((N1 + S4 - 1) / S4) * ((N2 + S4 - 1) / S4), M);
     const int64_t csz = single ? 8 : 16, ncol = 2 * mmax + 1;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const bool fused = o.grid_fft != PFSHT_GRID_FFT_LIBRARY && grid_fits(nth, N1, N2, size_t(csz), dev, nullptr);
     by += int64_t(B) * ncol * nh * 16;                                  // ring buffer
-    by += int64_t(B) * csz * (int64_t(nth) * nph + int64_t(N1) * N2 + ncol * N1 + nst * win * win);  // grids, windows
+    by += int64_t(B) * csz * ((fused ? 0 : int64_t(nth) * nph) + int64_t(N1) * N2 + ncol * N1 + nst * win * win);
+    by += fused ? csz * (int64_t(nth) + N1 + N2) : 0;                  // twiddles of the LDS grid stage
     by += M * 20 + int64_t(nh) * 24 + nst * 80;                        // points, permutation, tables
     return by;
 }

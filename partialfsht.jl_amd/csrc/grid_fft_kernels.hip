@@ -1,0 +1,283 @@
+// grid_fft_kernels.hip — the stretched apply's grid stage (src/Stretch.jl:262-295 between the ring
+// pass and the NUFFT) as two kernels per direction, each running its FFTs in LDS (lds_fft.hpp):
+//
+// synthesis  column kernel, one workgroup per (order column c, transform b): window + DFS
+//            periodisation of the ring column (src/DFS.jl:13-56 in longitude-spectral space, as
+//            spec_dfs_kernel), colatitude FFT of length nth, phase + deconvolution into the fine
+//            colatitude layout (src/NUFFT.jl type-2 deconvolution, as deconv_pad_ut_kernel), inverse
+//            FFT of length N1, and the stored fine rows of the column out to T1 [b][c][pr];
+//            row kernel, one workgroup per (stored row pr, b): the row's retained columns from T1
+//            (zero elsewhere), inverse FFT of length N2, the row out to U [b][pr][l2].
+// adjoint    row kernel: forward FFT of a stored row of U, its retained columns out to
+//            T2 [b][pr][c]; column kernel: the column from T2 (rows not stored are zero), forward FFT
+//            of length N1, deconvolution back to the nth colatitude modes (deconv_trunc_ut_kernel),
+//            inverse FFT of length nth and the DFS fold + window into the ring column
+//            (spec_fold_kernel).
+//
+// Each kernel reads and writes its data once in HBM; the transposes between the column and row
+// orders happen in those reads: the synthesis row kernel and the adjoint column kernel gather
+// elements 16 B (8 B) apart in lines shared by neighbouring rows (columns), and their workgroups are
+// numbered so that neighbours run on one XCD (hardware dispatch sends workgroup w to XCD w mod 8),
+// so each line comes from HBM once into that XCD's L2.
+#include "lds_fft.hpp"
+#include "pfsht_common.hpp"
+
+namespace pfsht {
+
+namespace {
+
+template <class C> __device__ __forceinline__ C czero() { return cx<C>(0, 0); }
+
+// workgroup w of a grid of 8 ceil(n/8) -> item (w mod 8) ceil(n/8) + w / 8: consecutive items on
+// one XCD
+__device__ __forceinline__ int xcd_item(int w, int per) { return (w & 7) * per + (w >> 3); }
+
+__device__ __forceinline__ int fine_mode(int l, int n, int N) {
+    if (l < (n + 1) / 2) return l;
+    if (l >= N - n / 2) return l - N + n;
+    return -1;
+}
+
+template <class C, int NT, int E>
+__global__ void __launch_bounds__(NT)
+grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
+                      const double *__restrict__ wth, const double2 *__restrict__ pc1, const double *__restrict__ c2,
+                      C *__restrict__ T1, const C *__restrict__ tw, FftLen fth, FftLen fc1, GridFftArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_gf_raw[];
+    C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int c = blockIdx.x, b = blockIdx.y;
+    const int64_t ncol = 2 * a.mmax + 1;
+    const int m = c - a.mmax, am = m < 0 ? -m : m;
+    const int lo = jlo[am], hi = jhi[am];
+    const int nh = a.nh, nth = a.nth;
+    const double2 *G = Gd + (int64_t(b) * ncol + c) * nh;
+    const bool odd = (am + a.spin) & 1;
+    for (int i = threadIdx.x; i < nth; i += NT) {
+        const int j = i < nh ? i : nth - 1 - i;
+        double2 v = make_double2(0.0, 0.0);
+        if (j >= lo && j <= hi) {
+            const double2 g = G[j];
+            double w = wth[j];
+            if (i >= nh && odd) w = -w;
+            v = make_double2(g.x * w, g.y * w);
+        }
+        stc(x + lds_slot<true>(i), v);
+    }
+    __syncthreads();
+    lds_fft<false, NT, E, true>(x, fth, tw + fth.tw);
+    // the fine colatitude layout: mode i of the nth-point transform at fine index l1 (zero padding
+    // between), times the phase and the deconvolution
+    const int N1 = a.N1, n1 = nth;
+    C v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int l1 = int(threadIdx.x) + q * NT;
+        if (l1 < N1) {
+            const int i = fine_mode(l1, n1, N1);
+            v[q] = i >= 0 ? x[lds_slot<true>(i)] : czero<C>();
+        }
+    }
+    __syncthreads();
+    const double sc = c2[m >= 0 ? m : m + a.nph] * a.scale;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int l1 = int(threadIdx.x) + q * NT;
+        if (l1 < N1) {
+            const int i = fine_mode(l1, n1, N1);
+            double2 r = make_double2(0.0, 0.0);
+            if (i >= 0) {
+                const double2 F = ldc(&v[q]);
+                const double2 p = pc1[i];
+                r = make_double2((F.x * p.x - F.y * p.y) * sc, (F.x * p.y + F.y * p.x) * sc);
+            }
+            stc(x + lds_slot<true>(l1), r);
+        }
+    }
+    __syncthreads();
+    lds_fft<true, NT, E, true>(x, fc1, tw + fc1.tw);
+    C *out = T1 + (int64_t(b) * ncol + c) * a.nR;
+    for (int pr = threadIdx.x; pr < a.nR; pr += NT) {
+        int l1 = a.r0 + pr;
+        if (l1 >= N1) l1 -= N1;
+        out[pr] = x[lds_slot<true>(l1)];
+    }
+}
+
+template <class C, int NT, int E>
+__global__ void __launch_bounds__(NT)
+grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__restrict__ tw, FftLen fr2,
+                      GridFftArgs a, int per) {
+    extern __shared__ __align__(16) unsigned char lds_gf_raw[];
+    C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int pr = xcd_item(blockIdx.x, per), b = blockIdx.y;
+    if (pr >= a.nR) return;
+    const int64_t ncol = 2 * a.mmax + 1;
+    const int N2 = a.N2, mmax = a.mmax;
+    const C *in = T1 + int64_t(b) * ncol * a.nR + pr;
+    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) {
+        const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
+        x[lds_slot<true>(l2)] = m != INT32_MIN ? in[int64_t(m + mmax) * a.nR] : czero<C>();
+    }
+    __syncthreads();
+    lds_fft<true, NT, E, true>(x, fr2, tw + fr2.tw);
+    C *out = U + (int64_t(b) * a.nR + pr) * N2;
+    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) out[l2] = x[lds_slot<true>(l2)];
+}
+
+template <class C, int NT, int E>
+__global__ void __launch_bounds__(NT)
+grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__restrict__ tw, FftLen fr2,
+                    GridFftArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_gf_raw[];
+    C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int pr = blockIdx.x, b = blockIdx.y;
+    const int64_t ncol = 2 * a.mmax + 1;
+    const int N2 = a.N2;
+    const C *in = U + (int64_t(b) * a.nR + pr) * N2;
+    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) x[lds_slot<true>(l2)] = in[l2];
+    __syncthreads();
+    lds_fft<false, NT, E, true>(x, fr2, tw + fr2.tw);
+    C *out = T2 + (int64_t(b) * a.nR + pr) * ncol;
+    for (int c = threadIdx.x; c < ncol; c += NT) {
+        const int m = c - a.mmax;
+        out[c] = x[lds_slot<true>(m >= 0 ? m : m + N2)];
+    }
+}
+
+template <class C, int NT, int E>
+__global__ void __launch_bounds__(NT)
+grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const double *__restrict__ wth,
+                    const double2 *__restrict__ pc1, const double *__restrict__ c2, const C *__restrict__ tw,
+                    FftLen fth, FftLen fc1, GridFftArgs a, int per) {
+    extern __shared__ __align__(16) unsigned char lds_gf_raw[];
+    C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int64_t ncol = 2 * a.mmax + 1;
+    const int c = xcd_item(blockIdx.x, per), b = blockIdx.y;
+    if (c >= ncol) return;
+    const int m = c - a.mmax, am = m < 0 ? -m : m;
+    const int N1 = a.N1, nth = a.nth, n1 = nth, nh = a.nh;
+    const C *in = T2 + int64_t(b) * a.nR * ncol + c;
+    for (int l1 = threadIdx.x; l1 < N1; l1 += NT) {
+        int pr = l1 - a.r0;
+        if (pr < 0) pr += N1;
+        x[lds_slot<true>(l1)] = pr < a.nR ? in[int64_t(pr) * ncol] : czero<C>();
+    }
+    __syncthreads();
+    lds_fft<false, NT, E, true>(x, fc1, tw + fc1.tw);
+    // back to the nth colatitude modes, times the conjugate phase and the deconvolution
+    C v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int i = int(threadIdx.x) + q * NT;
+        if (i < n1) v[q] = x[lds_slot<true>(i < (n1 + 1) / 2 ? i : i - n1 + N1)];
+    }
+    __syncthreads();
+    const double sc = c2[m >= 0 ? m : m + a.nph] * a.scale;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int i = int(threadIdx.x) + q * NT;
+        if (i < n1) {
+            const double2 u = ldc(&v[q]);
+            const double2 p = pc1[i];
+            stc(x + lds_slot<true>(i), make_double2((u.x * p.x + u.y * p.y) * sc, (u.y * p.x - u.x * p.y) * sc));
+        }
+    }
+    __syncthreads();
+    lds_fft<true, NT, E, true>(x, fth, tw + fth.tw);
+    double2 *G = Gd + (int64_t(b) * ncol + c) * nh;
+    const bool odd = (am + a.spin) & 1;
+    for (int j = threadIdx.x; j < nh; j += NT) {
+        double2 v2 = ldc(x + lds_slot<true>(j));
+        if (nth > nh) {
+            const double2 r = ldc(x + lds_slot<true>(nth - 1 - j));
+            v2 = odd ? make_double2(v2.x - r.x, v2.y - r.y) : make_double2(v2.x + r.x, v2.y + r.y);
+        }
+        const double w = wth[j];
+        G[j] = make_double2(v2.x * w, v2.y * w);
+    }
+}
+
+template <class C>
+size_t lds_of(int n) { return sizeof(C) * size_t(lds_slots(n)); }
+
+}  // namespace
+
+// Eight values per thread (128 .. 1024 threads: lengths up to 8192), sixteen on 1024 threads above;
+// the LDS slots are swizzled (lds_slot). The smaller per-thread share measured 15-25% faster than
+// sixteen at the apply's lengths (more wavefronts in flight per transform, fewer registers).
+int grid_fft_threads(int n) {
+    int nt = 128;
+    while (nt < 1024 && n > 8 * nt) nt *= 2;
+    return nt;
+}
+
+#define PFSHT_GF_DISPATCH(NTV, N, LAUNCH)           \
+    switch (NTV) {                                  \
+    case 128: LAUNCH(128, 8); break;                \
+    case 256: LAUNCH(256, 8); break;                \
+    case 512: LAUNCH(512, 8); break;                \
+    default:                                        \
+        if ((N) <= 8 * 1024) LAUNCH(1024, 8);       \
+        else LAUNCH(1024, 16);                      \
+        break;                                      \
+    }
+
+template <class C>
+void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2 *Gd, const double *wth,
+                            const double2 *pc1, const double *c2, C *T1, int nb, hipStream_t s) {
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
+    const size_t lds = lds_of<C>(a.N1);
+#define PFSHT_GF_L(NTV, EV)                                                                                        \
+    grid_col_synth_kernel<C, NTV, EV><<<dim3(2 * a.mmax + 1, nb), NTV, lds, s>>>(Gd, r.jlo, r.jhi, wth, pc1, c2, T1, \
+                                                                              tw, g.th, g.c1, a)
+    PFSHT_GF_DISPATCH(g.nt_col, a.N1, PFSHT_GF_L)
+#undef PFSHT_GF_L
+}
+
+template <class C>
+void launch_grid_synth_rows(const GridFftDev &g, const C *T1, C *U, int nb, hipStream_t s) {
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
+    const size_t lds = lds_of<C>(a.N2);
+    const int per = (a.nR + 7) / 8;
+#define PFSHT_GF_L(NTV, EV) grid_row_synth_kernel<C, NTV, EV><<<dim3(8 * per, nb), NTV, lds, s>>>(T1, U, tw, g.r2, a, per)
+    PFSHT_GF_DISPATCH(g.nt_row, a.N2, PFSHT_GF_L)
+#undef PFSHT_GF_L
+}
+
+template <class C>
+void launch_grid_adjoint_rows(const GridFftDev &g, const C *U, C *T2, int nb, hipStream_t s) {
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
+    const size_t lds = lds_of<C>(a.N2);
+#define PFSHT_GF_L(NTV, EV) grid_row_adj_kernel<C, NTV, EV><<<dim3(a.nR, nb), NTV, lds, s>>>(U, T2, tw, g.r2, a)
+    PFSHT_GF_DISPATCH(g.nt_row, a.N2, PFSHT_GF_L)
+#undef PFSHT_GF_L
+}
+
+template <class C>
+void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wth, const double2 *pc1,
+                              const double *c2, double2 *Gd, int nb, hipStream_t s) {
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
+    const size_t lds = lds_of<C>(a.N1);
+    const int per = (2 * a.mmax + 1 + 7) / 8;
+#define PFSHT_GF_L(NTV, EV) \
+    grid_col_adj_kernel<C, NTV, EV><<<dim3(8 * per, nb), NTV, lds, s>>>(T2, Gd, wth, pc1, c2, tw, g.th, g.c1, a, per)
+    PFSHT_GF_DISPATCH(g.nt_col, a.N1, PFSHT_GF_L)
+#undef PFSHT_GF_L
+}
+
+#define PFSHT_GF_INST(C)                                                                                          \
+    template void launch_grid_synth_cols<C>(const GridFftDev &, const RingDev &, const double2 *, const double *, \
+                                            const double2 *, const double *, C *, int, hipStream_t);             \
+    template void launch_grid_synth_rows<C>(const GridFftDev &, const C *, C *, int, hipStream_t);                \
+    template void launch_grid_adjoint_rows<C>(const GridFftDev &, const C *, C *, int, hipStream_t);              \
+    template void launch_grid_adjoint_cols<C>(const GridFftDev &, const C *, const double *, const double2 *,     \
+                                              const double *, double2 *, int, hipStream_t);
+PFSHT_GF_INST(double2)
+PFSHT_GF_INST(float2)
+#undef PFSHT_GF_INST
+
+}  // namespace pfsht

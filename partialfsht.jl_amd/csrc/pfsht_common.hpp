@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "geometry.hpp"
+#include "lds_fft.hpp"
 #include "point_bins.hpp"
 
 namespace pfsht {
@@ -327,6 +328,35 @@ template <class C>
 void launch_rows_to_ut(const NufftDev &u, int mmax, const C *U, C *Ut, hipStream_t s);
 // out[b][c][r] = in[b][r][c] for a rows x cols complex matrix per batch entry.
 void launch_transpose(const double2 *in, double2 *out, int rows, int cols, int B, hipStream_t s);
+
+// The stretched apply's grid stage as two kernels per direction with their FFTs in LDS
+// (grid_fft_kernels.hip): from the ring buffer Gd to the stored fine rows U and back, through
+// T1 [b][c][pr] (synthesis) or T2 [b][pr][c] (adjoint) in the Ut buffer.
+struct GridFftArgs {
+    int nh = 0, nth = 0, mmax = 0, spin = 0, nph = 0;  // ring grid (nth colatitudes, nh rings)
+    int N1 = 0, N2 = 0, r0 = 0, nR = 0;                 // fine grid and its stored rows
+    double scale = 1.0;                                 // the skipped longitude FFT pair (nph)
+};
+struct GridFftDev {
+    FftLen th, c1, r2;     // lengths nth, N1, N2
+    void *tw = nullptr;    // twiddle tables of the three lengths (grid storage type)
+    int nt_col = 128, nt_row = 128;  // workgroup sizes of the column and row kernels
+    GridFftArgs args;
+};
+// workgroup size for an LDS transform of length n (n <= kFftElems * size)
+int grid_fft_threads(int n);
+// synthesis: the column kernel (ring buffer -> T1), then the row kernel (T1 -> U)
+template <class C>
+void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2 *Gd, const double *wth,
+                            const double2 *pc1, const double *c2, C *T1, int nb, hipStream_t s);
+template <class C>
+void launch_grid_synth_rows(const GridFftDev &g, const C *T1, C *U, int nb, hipStream_t s);
+// adjoint: the row kernel (U -> T2), then the column kernel (T2 -> ring buffer)
+template <class C>
+void launch_grid_adjoint_rows(const GridFftDev &g, const C *U, C *T2, int nb, hipStream_t s);
+template <class C>
+void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wth, const double2 *pc1,
+                              const double *c2, double2 *Gd, int nb, hipStream_t s);
 // The point kernels over u.B transforms (an apply passes a copy with B = nb).
 template <class C>
 void launch_interp(const NufftDev &u, const C *U, OutCols f, hipStream_t s);

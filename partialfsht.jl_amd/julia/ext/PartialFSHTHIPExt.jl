@@ -54,6 +54,7 @@ struct PfshtPlanOpts          # pfsht_plan_opts, include/pfsht.h
     epsapo_max::Float64
     packed::Int32
     tune::Int32
+    grid_fft::Int32           # PFSHT_GRID_FFT_* (0: auto)
 end
 
 struct PfshtInfo              # pfsht_info, include/pfsht.h
@@ -81,6 +82,7 @@ struct PfshtInfo              # pfsht_info, include/pfsht.h
     seg_blocks::Int32
     adj_tiles::Int32
     mfma_tiles::Int32
+    grid_fft::Int32
 end
 
 _last_error() = unsafe_string(ccall((:pfsht_last_error, libpfsht), Cstring, ()))
@@ -134,7 +136,7 @@ function _create(::Type{LT}, method::Cint, θ::AbstractVector, φ::AbstractVecto
     get_opt(k, d) = (opts !== nothing && hasproperty(opts, k)) ? getproperty(opts, k) : d
     o = PfshtPlanOpts(method, prec, spin, Int32(get_opt(:maxboxes, 8)),
                       Float64(get_opt(:oversample_θ, 1)), Float64(get_opt(:oversample_φ, 1)),
-                      Float64(get_opt(:εapo_max, 1.0)), Int32(0), Int32(0))
+                      Float64(get_opt(:εapo_max, 1.0)), Int32(0), Int32(0), Int32(0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     _check(ccall((:pfsht_plan_create_opts, libpfsht), Cint,
                  (Ptr{Float64}, Ptr{Float64}, Int64, Int32, Float64, Int32, Ref{PfshtPlanOpts},

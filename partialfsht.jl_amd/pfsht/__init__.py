@@ -51,7 +51,8 @@ class InfoC(C.Structure):
                 ("es_n2", C.c_int32), ("npts", C.c_int64), ("ring_work", C.c_int64),
                 ("device_bytes", C.c_int64), ("tol", C.c_double), ("es_beta", C.c_double),
                 ("es_rows", C.c_int32), ("nboxes", C.c_int32), ("spin", C.c_int32),
-                ("seg_blocks", C.c_int32), ("adj_tiles", C.c_int32), ("mfma_tiles", C.c_int32)]
+                ("seg_blocks", C.c_int32), ("adj_tiles", C.c_int32), ("mfma_tiles", C.c_int32),
+                ("grid_fft", C.c_int32)]
 
 
 class RingShapeC(C.Structure):
@@ -62,7 +63,7 @@ class RingShapeC(C.Structure):
 class PlanOptsC(C.Structure):
     _fields_ = [("method", C.c_int32), ("precision", C.c_int32), ("spin", C.c_int32), ("maxboxes", C.c_int32),
                 ("oversample_theta", C.c_double), ("oversample_phi", C.c_double), ("epsapo_max", C.c_double),
-                ("packed", C.c_int32), ("tune", C.c_int32)]
+                ("packed", C.c_int32), ("tune", C.c_int32), ("grid_fft", C.c_int32)]
 
 
 class BoxC(C.Structure):
@@ -82,6 +83,8 @@ NSTAGES = 9
 STAGES = ["ring", "pack", "phi_fft", "dfs", "grid_fft", "deconv", "fine_fft", "nufft", "ring_reduce"]
 METHODS = {"auto": 0, "direct": 1, "stretch": 2, "multibox": 3}
 KINDS = {0: "pole", 1: "interior", 2: "full", -1: "direct", -2: "multibox"}
+# the grid stage of a stretched apply: LDS kernels with fused FFTs, or rocFFT plans (PFSHT_GRID_FFT_*)
+GRID_FFT = {"auto": 0, "lds": 1, "library": 2}
 WHERE_HOST, WHERE_DEVICE = 0, 1
 
 EXPORTS = {
@@ -417,7 +420,7 @@ class Plan:
     def __init__(self, theta, phi, lmax: int, tol: float = 1e-10, ntrans: int = 1,
                  method: str = "auto", precision: str | None = None, spin: int = 0, maxboxes: int = 8,
                  oversample_theta: float = 1.0, oversample_phi: float = 1.0, epsapo_max: float = 1.0,
-                 packed: bool = False, tune: int = 0):
+                 packed: bool = False, tune: int = 0, grid_fft: str = "auto"):
         self.float32 = _is_f32(theta) and _is_f32(phi)
         if precision is None:
             precision = "mixed" if self.float32 and tol >= 1e-6 else "fp64"
@@ -440,11 +443,13 @@ class Plan:
         if precision not in ("fp64", "mixed"):
             raise ArgumentError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
         self.precision = precision
+        if grid_fft not in GRID_FFT:
+            raise ArgumentError(f"grid_fft must be one of {sorted(GRID_FFT)}, got {grid_fft!r}")
         h = C.c_void_p()
         o = PlanOptsC(method=METHODS[method], precision={"fp64": 0, "mixed": 1}[precision], spin=self.spin,
                       maxboxes=int(maxboxes), oversample_theta=float(oversample_theta),
                       oversample_phi=float(oversample_phi), epsapo_max=float(epsapo_max), packed=int(bool(packed)),
-                      tune=int(tune))
+                      tune=int(tune), grid_fft=GRID_FFT[grid_fft])
         _check(lib().pfsht_plan_create_opts(th.ctypes.data_as(C.c_void_p), ph.ctypes.data_as(C.c_void_p),
                                             len(th), self.lmax, self.tol, self.ntrans, C.byref(o), C.byref(h)))
         self._h = h
