@@ -404,16 +404,20 @@ static Status fft_attach_work(const std::vector<FftPlan *> &plans, void *&work, 
 }
 
 // Whether the LDS grid stage takes the lengths nth, N1, N2 (nth <= N1) with elements of csz bytes:
-// each factors into radix-8/4/2/3/5 passes and the longer fine length fits one workgroup's LDS and
-// kFftElems x 1024 threads. Fills g's lengths when it does.
+// each factors into radix-8/4/2/3/5 passes, the column length N1 fits kFftElems x 1024 threads, the
+// row length N2 that or, split in two (even, N2 / 2 <= kFftElems x 1024), twice that, and the longer
+// fits one workgroup's LDS. Fills g's lengths when it does.
 static bool grid_fits(int nth, int N1, int N2, size_t csz, int dev, GridFftDev *g) {
     int maxlds = 65536;
     (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    constexpr int cap = kFftElems * 1024;
     GridFftDev t;
+    t.split = N2 > cap;
     if (!fft_len_plan(nth, t.th) || !fft_len_plan(N1, t.c1) || !fft_len_plan(N2, t.r2)) return false;
+    if (t.split && (N2 % 2 != 0 || N2 / 2 > cap || !fft_len_plan(N2 / 2, t.r2h))) return false;
     const int n = std::max(N1, N2);
-    if (nth > N1 || n > kFftElems * 1024 || size_t(lds_slots(n)) * csz > size_t(maxlds)) return false;
-    if (g) { g->th = t.th; g->c1 = t.c1; g->r2 = t.r2; }
+    if (nth > N1 || N1 > cap || size_t(lds_slots(n)) * csz > size_t(maxlds)) return false;
+    if (g) { g->th = t.th; g->c1 = t.c1; g->r2 = t.r2; g->r2h = t.r2h; g->split = t.split; }
     return true;
 }
 
@@ -1069,17 +1073,20 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if (p.fused) {
         GridFftDev &g = p.gf;
         g.nt_col = grid_fft_threads(u.N1);
-        g.nt_row = grid_fft_threads(u.N2);
+        g.nt_row = g.split ? 1024 : grid_fft_threads(u.N2);
         g.args.nh = p.nh; g.args.nth = p.nth; g.args.mmax = p.mmax; g.args.spin = p.spin; g.args.nph = p.nph;
         g.args.N1 = u.N1; g.args.N2 = u.N2; g.args.r0 = u.r0; g.args.nR = u.nR;
         g.args.scale = double(p.nph);
-        // exp(-2 pi i k / n), k < n, for the three lengths back to back
+        // exp(-2 pi i k / n), k < n, for the lengths back to back (N2 / 2 last, when rows split)
         g.th.tw = 0;
         g.c1.tw = p.nth;
         g.r2.tw = p.nth + u.N1;
-        const size_t ntw = size_t(p.nth) + u.N1 + u.N2;
+        g.r2h.tw = p.nth + u.N1 + u.N2;
+        const size_t ntw = size_t(p.nth) + u.N1 + u.N2 + (g.split ? u.N2 / 2 : 0);
         std::vector<double2> tw(ntw);
-        for (const FftLen *f : {&g.th, &g.c1, &g.r2})
+        std::vector<const FftLen *> lens = {&g.th, &g.c1, &g.r2};
+        if (g.split) lens.push_back(&g.r2h);
+        for (const FftLen *f : lens)
             for (int k = 0; k < f->n; ++k) {
                 const long double a = -2.0L * 3.141592653589793238462643383279502884L * k / f->n;
                 tw[size_t(f->tw) + k] = make_double2(double(cosl(a)), double(sinl(a)));
@@ -2590,7 +2597,7 @@ static int64_t estimate_stretch(double thlo, double thhi, int64_t M, int L, doub
     const bool fused = o.grid_fft != PFSHT_GRID_FFT_LIBRARY && grid_fits(nth, N1, N2, size_t(csz), dev, nullptr);
     by += int64_t(B) * ncol * nh * 16;                                  // ring buffer
     by += int64_t(B) * csz * ((fused ? 0 : int64_t(nth) * nph) + int64_t(N1) * N2 + ncol * N1 + nst * win * win);
-    by += fused ? csz * (int64_t(nth) + N1 + N2) : 0;                  // twiddles of the LDS grid stage
+    by += fused ? csz * (int64_t(nth) + N1 + N2 + N2 / 2) : 0;         // twiddles of the LDS grid stage
     by += M * 20 + int64_t(nh) * 24 + nst * 80;                        // points, permutation, tables
     return by;
 }

@@ -26,6 +26,8 @@ namespace pfsht {
 
 namespace {
 
+constexpr int E = kFftElems;  // values a thread holds in a pass
+
 template <class C> __device__ __forceinline__ C czero() { return cx<C>(0, 0); }
 
 // workgroup w of a grid of 8 ceil(n/8) -> item (w mod 8) ceil(n/8) + w / 8: consecutive items on
@@ -38,7 +40,7 @@ __device__ __forceinline__ int fine_mode(int l, int n, int N) {
     return -1;
 }
 
-template <class C, int NT, int E>
+template <class C, int NT>
 __global__ void __launch_bounds__(NT)
 grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
                       const double *__restrict__ wth, const double2 *__restrict__ pc1, const double *__restrict__ c2,
@@ -103,12 +105,25 @@ grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict_
     }
 }
 
-template <class C, int NT, int E>
+// Rows of N2 <= 8192 values run lds_fft over swizzled slots; longer ones (SPLIT) the radix-2 split
+// over natural slots, whose outputs sit at x[k / 2] (k even) and x[N2 / 2 + k / 2] (k odd).
+template <bool SPLIT> __device__ __forceinline__ int row_slot(int k, int h) {
+    if (SPLIT) return (k & 1) ? h + (k >> 1) : (k >> 1);
+    return lds_slot<true>(k);
+}
+
+template <bool INV, bool SPLIT, int NT, class C>
+__device__ __forceinline__ void row_fft(C *x, const GridFftDev &g, const C *tw) {
+    if constexpr (SPLIT) lds_fft_split<INV, NT, E>(x, g.args.N2, g.r2h, tw + g.r2.tw, tw + g.r2h.tw);
+    else lds_fft<INV, NT, E, true>(x, g.r2, tw + g.r2.tw);
+}
+
+template <class C, int NT, bool SPLIT>
 __global__ void __launch_bounds__(NT)
-grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__restrict__ tw, FftLen fr2,
-                      GridFftArgs a, int per) {
+grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__restrict__ tw, GridFftDev g, int per) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const GridFftArgs &a = g.args;
     const int pr = xcd_item(blockIdx.x, per), b = blockIdx.y;
     if (pr >= a.nR) return;
     const int64_t ncol = 2 * a.mmax + 1;
@@ -116,35 +131,35 @@ grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__re
     const C *in = T1 + int64_t(b) * ncol * a.nR + pr;
     for (int l2 = threadIdx.x; l2 < N2; l2 += NT) {
         const int m = l2 <= mmax ? l2 : (l2 >= N2 - mmax ? l2 - N2 : INT32_MIN);
-        x[lds_slot<true>(l2)] = m != INT32_MIN ? in[int64_t(m + mmax) * a.nR] : czero<C>();
+        x[SPLIT ? l2 : lds_slot<true>(l2)] = m != INT32_MIN ? in[int64_t(m + mmax) * a.nR] : czero<C>();
     }
     __syncthreads();
-    lds_fft<true, NT, E, true>(x, fr2, tw + fr2.tw);
+    row_fft<true, SPLIT, NT>(x, g, tw);
     C *out = U + (int64_t(b) * a.nR + pr) * N2;
-    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) out[l2] = x[lds_slot<true>(l2)];
+    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) out[l2] = x[row_slot<SPLIT>(l2, N2 / 2)];
 }
 
-template <class C, int NT, int E>
+template <class C, int NT, bool SPLIT>
 __global__ void __launch_bounds__(NT)
-grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__restrict__ tw, FftLen fr2,
-                    GridFftArgs a) {
+grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__restrict__ tw, GridFftDev g) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const GridFftArgs &a = g.args;
     const int pr = blockIdx.x, b = blockIdx.y;
     const int64_t ncol = 2 * a.mmax + 1;
     const int N2 = a.N2;
     const C *in = U + (int64_t(b) * a.nR + pr) * N2;
-    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) x[lds_slot<true>(l2)] = in[l2];
+    for (int l2 = threadIdx.x; l2 < N2; l2 += NT) x[SPLIT ? l2 : lds_slot<true>(l2)] = in[l2];
     __syncthreads();
-    lds_fft<false, NT, E, true>(x, fr2, tw + fr2.tw);
+    row_fft<false, SPLIT, NT>(x, g, tw);
     C *out = T2 + (int64_t(b) * a.nR + pr) * ncol;
     for (int c = threadIdx.x; c < ncol; c += NT) {
         const int m = c - a.mmax;
-        out[c] = x[lds_slot<true>(m >= 0 ? m : m + N2)];
+        out[c] = x[row_slot<SPLIT>(m >= 0 ? m : m + N2, N2 / 2)];
     }
 }
 
-template <class C, int NT, int E>
+template <class C, int NT>
 __global__ void __launch_bounds__(NT)
 grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const double *__restrict__ wth,
                     const double2 *__restrict__ pc1, const double *__restrict__ c2, const C *__restrict__ tw,
@@ -202,24 +217,21 @@ size_t lds_of(int n) { return sizeof(C) * size_t(lds_slots(n)); }
 
 }  // namespace
 
-// Eight values per thread (128 .. 1024 threads: lengths up to 8192), sixteen on 1024 threads above;
-// the LDS slots are swizzled (lds_slot). The smaller per-thread share measured 15-25% faster than
-// sixteen at the apply's lengths (more wavefronts in flight per transform, fewer registers).
+// Eight values per thread, 128 .. 1024 threads (lengths up to 8192; longer rows split in two), over
+// swizzled LDS slots: measured 15-25% faster than sixteen values per thread at the apply's lengths
+// (more wavefronts in flight per transform, fewer registers; sixteen on 1024 threads spills).
 int grid_fft_threads(int n) {
     int nt = 128;
-    while (nt < 1024 && n > 8 * nt) nt *= 2;
+    while (nt < 1024 && n > E * nt) nt *= 2;
     return nt;
 }
 
-#define PFSHT_GF_DISPATCH(NTV, N, LAUNCH)           \
-    switch (NTV) {                                  \
-    case 128: LAUNCH(128, 8); break;                \
-    case 256: LAUNCH(256, 8); break;                \
-    case 512: LAUNCH(512, 8); break;                \
-    default:                                        \
-        if ((N) <= 8 * 1024) LAUNCH(1024, 8);       \
-        else LAUNCH(1024, 16);                      \
-        break;                                      \
+#define PFSHT_GF_DISPATCH(NTV, LAUNCH) \
+    switch (NTV) {                     \
+    case 128: LAUNCH(128); break;      \
+    case 256: LAUNCH(256); break;      \
+    case 512: LAUNCH(512); break;      \
+    default: LAUNCH(1024); break;      \
     }
 
 template <class C>
@@ -228,10 +240,10 @@ void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2
     const GridFftArgs &a = g.args;
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N1);
-#define PFSHT_GF_L(NTV, EV)                                                                                        \
-    grid_col_synth_kernel<C, NTV, EV><<<dim3(2 * a.mmax + 1, nb), NTV, lds, s>>>(Gd, r.jlo, r.jhi, wth, pc1, c2, T1, \
-                                                                              tw, g.th, g.c1, a)
-    PFSHT_GF_DISPATCH(g.nt_col, a.N1, PFSHT_GF_L)
+#define PFSHT_GF_L(NTV)                                                                                        \
+    grid_col_synth_kernel<C, NTV><<<dim3(2 * a.mmax + 1, nb), NTV, lds, s>>>(Gd, r.jlo, r.jhi, wth, pc1, c2, T1, \
+                                                                          tw, g.th, g.c1, a)
+    PFSHT_GF_DISPATCH(g.nt_col, PFSHT_GF_L)
 #undef PFSHT_GF_L
 }
 
@@ -241,8 +253,12 @@ void launch_grid_synth_rows(const GridFftDev &g, const C *T1, C *U, int nb, hipS
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
     const int per = (a.nR + 7) / 8;
-#define PFSHT_GF_L(NTV, EV) grid_row_synth_kernel<C, NTV, EV><<<dim3(8 * per, nb), NTV, lds, s>>>(T1, U, tw, g.r2, a, per)
-    PFSHT_GF_DISPATCH(g.nt_row, a.N2, PFSHT_GF_L)
+    if (g.split) {
+        grid_row_synth_kernel<C, 1024, true><<<dim3(8 * per, nb), 1024, lds, s>>>(T1, U, tw, g, per);
+        return;
+    }
+#define PFSHT_GF_L(NTV) grid_row_synth_kernel<C, NTV, false><<<dim3(8 * per, nb), NTV, lds, s>>>(T1, U, tw, g, per)
+    PFSHT_GF_DISPATCH(g.nt_row, PFSHT_GF_L)
 #undef PFSHT_GF_L
 }
 
@@ -251,8 +267,12 @@ void launch_grid_adjoint_rows(const GridFftDev &g, const C *U, C *T2, int nb, hi
     const GridFftArgs &a = g.args;
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
-#define PFSHT_GF_L(NTV, EV) grid_row_adj_kernel<C, NTV, EV><<<dim3(a.nR, nb), NTV, lds, s>>>(U, T2, tw, g.r2, a)
-    PFSHT_GF_DISPATCH(g.nt_row, a.N2, PFSHT_GF_L)
+    if (g.split) {
+        grid_row_adj_kernel<C, 1024, true><<<dim3(a.nR, nb), 1024, lds, s>>>(U, T2, tw, g);
+        return;
+    }
+#define PFSHT_GF_L(NTV) grid_row_adj_kernel<C, NTV, false><<<dim3(a.nR, nb), NTV, lds, s>>>(U, T2, tw, g)
+    PFSHT_GF_DISPATCH(g.nt_row, PFSHT_GF_L)
 #undef PFSHT_GF_L
 }
 
@@ -263,9 +283,9 @@ void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wt
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N1);
     const int per = (2 * a.mmax + 1 + 7) / 8;
-#define PFSHT_GF_L(NTV, EV) \
-    grid_col_adj_kernel<C, NTV, EV><<<dim3(8 * per, nb), NTV, lds, s>>>(T2, Gd, wth, pc1, c2, tw, g.th, g.c1, a, per)
-    PFSHT_GF_DISPATCH(g.nt_col, a.N1, PFSHT_GF_L)
+#define PFSHT_GF_L(NTV) \
+    grid_col_adj_kernel<C, NTV><<<dim3(8 * per, nb), NTV, lds, s>>>(T2, Gd, wth, pc1, c2, tw, g.th, g.c1, a, per)
+    PFSHT_GF_DISPATCH(g.nt_col, PFSHT_GF_L)
 #undef PFSHT_GF_L
 }
 

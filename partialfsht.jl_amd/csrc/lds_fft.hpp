@@ -13,7 +13,7 @@
 
 namespace pfsht {
 
-constexpr int kFftElems = 16;   // complex values a thread holds in a pass: n <= kFftElems * threads
+constexpr int kFftElems = 8;    // complex values a thread holds in a pass: n <= kFftElems * threads
 constexpr int kFftMaxPass = 16;
 
 // One transform length and its passes (kernel argument).
@@ -212,6 +212,25 @@ __device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw) {
         }
         p *= r;
     }
+}
+
+// The DFT of x[0 .. n), n even and n / 2 <= E * NT, through one radix-2 decimation-in-frequency step,
+// x[i], x[i + n/2] -> x[i] + x[i + n/2], (x[i] - x[i + n/2]) w^i (in place, no thread reads another's
+// values), and LDS transforms of the two halves: afterwards X[2m] is at x[m] and X[2m + 1] at
+// x[n/2 + m]. For lengths past E x 1024 values: each half keeps the per-thread share of E values.
+// Unswizzled slots. tw_n is the length-n table, half the length n/2 passes with their own table.
+template <bool INV, int NT, int E, class C>
+__device__ void lds_fft_split(C *x, int n, const FftLen &half, const C *__restrict__ tw_n,
+                              const C *__restrict__ tw_half) {
+    const int h = n / 2;
+    for (int i = threadIdx.x; i < h; i += NT) {
+        const C a = x[i], b = x[i + h];
+        x[i] = cadd(a, b);
+        x[i + h] = cmulw<INV>(csub(a, b), tw_n[i]);
+    }
+    __syncthreads();
+    lds_fft<INV, NT, E, false>(x, half, tw_half);
+    lds_fft<INV, NT, E, false>(x + h, half, tw_half);
 }
 
 // Host: the passes of length n (8s first, a 2 only when the power of two is 2; then 3s and 5s);

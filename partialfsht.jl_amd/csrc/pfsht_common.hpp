@@ -339,11 +339,13 @@ struct GridFftArgs {
 };
 struct GridFftDev {
     FftLen th, c1, r2;     // lengths nth, N1, N2
-    void *tw = nullptr;    // twiddle tables of the three lengths (grid storage type)
+    FftLen r2h;            // N2 / 2, when the rows split (N2 > kFftElems x 1024)
+    bool split = false;
+    void *tw = nullptr;    // twiddle tables of the lengths (grid storage type)
     int nt_col = 128, nt_row = 128;  // workgroup sizes of the column and row kernels
     GridFftArgs args;
 };
-// workgroup size for an LDS transform of length n (n <= kFftElems * size)
+// workgroup size for an LDS transform of length n (n <= kFftElems * size, size <= 1024)
 int grid_fft_threads(int n);
 // synthesis: the column kernel (ring buffer -> T1), then the row kernel (T1 -> U)
 template <class C>

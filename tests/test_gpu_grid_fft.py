@@ -2,9 +2,10 @@
 inside workgroups, fused with the window, periodisation, deconvolution and transposes) against the
 rocFFT path of the same plan geometry (grid_fft="library"), through the C ABI, on every periodisation
 kind, spin, batched columns, mixed precision, and fine lengths that take each workgroup size of the
-LDS transforms (128 ... 1024 threads, fine lengths up to 10000). The two paths compute the same
-transform with different FFT algorithms, so they agree to round-off: 1e-12 relative in FP64, 2e-5 in
-mixed precision (FP32 grids). The widest case is also held to the oracle's Direct sum at 10 tol."""
+LDS transforms (128 ... 1024 threads; rows past 8192 values split in two, here 10000). The two paths
+compute the same transform with different FFT algorithms, so they agree to round-off: 1e-12 relative
+in FP64, 2e-5 in mixed precision (FP32 grids). The widest case is also held to the oracle's Direct
+sum at 10 tol."""
 import os
 
 import numpy as np
