@@ -1339,7 +1339,7 @@ static Status adjoint_body(pfsht_plan_s &p, InCols f, OutCols alm, int nb, hipSt
     Status st = p.precision == PFSHT_PREC_MIXED ? adjoint_grid<float2>(p, t, f, nb, s)
                                                 : adjoint_grid<double2>(p, t, f, nb, s);
     if (st != kOk) return st;
-    tmark(p, t, 7, s);
+    if (!p.fused) tmark(p, t, 7, s);  // the LDS column kernel ends with the fold (stage 1)
     ring_adjoint_timed(p, t, alm, nb, s);
     PFSHT_HIP_TRY(hipGetLastError());
     return kOk;
