@@ -1073,7 +1073,7 @@ static Status plan_build(pfsht_plan_s &p, const double *theta, const double *phi
     if (p.fused) {
         GridFftDev &g = p.gf;
         g.nt_col = grid_fft_threads(u.N1);
-        g.nt_row = g.split ? 1024 : grid_fft_threads(u.N2);
+        g.nt_row = grid_fft_threads(g.split ? u.N2 / 2 : u.N2);
         g.args.nh = p.nh; g.args.nth = p.nth; g.args.mmax = p.mmax; g.args.spin = p.spin; g.args.nph = p.nph;
         g.args.N1 = u.N1; g.args.N2 = u.N2; g.args.r0 = u.r0; g.args.nR = u.nR;
         g.args.scale = double(p.nph);

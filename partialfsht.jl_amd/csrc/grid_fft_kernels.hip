@@ -19,6 +19,8 @@
 // elements 16 B (8 B) apart in lines shared by neighbouring rows (columns), and their workgroups are
 // numbered so that neighbours run on one XCD (hardware dispatch sends workgroup w to XCD w mod 8),
 // so each line comes from HBM once into that XCD's L2.
+#include <algorithm>
+
 #include "lds_fft.hpp"
 #include "pfsht_common.hpp"
 
@@ -40,13 +42,14 @@ __device__ __forceinline__ int fine_mode(int l, int n, int N) {
     return -1;
 }
 
-template <class C, int NT>
-__global__ void __launch_bounds__(NT)
+template <class C>
+__global__ void __launch_bounds__(1024)
 grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
                       const double *__restrict__ wth, const double2 *__restrict__ pc1, const double *__restrict__ c2,
                       C *__restrict__ T1, const C *__restrict__ tw, FftLen fth, FftLen fc1, GridFftArgs a) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int NT = int(blockDim.x);
     const int c = blockIdx.x, b = blockIdx.y;
     const int64_t ncol = 2 * a.mmax + 1;
     const int m = c - a.mmax, am = m < 0 ? -m : m;
@@ -66,7 +69,7 @@ grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict_
         stc(x + lds_slot<true>(i), v);
     }
     __syncthreads();
-    lds_fft<false, NT, E, true>(x, fth, tw + fth.tw);
+    lds_fft<false, E, true>(x, fth, tw + fth.tw, NT);
     // the fine colatitude layout: mode i of the nth-point transform at fine index l1 (zero padding
     // between), times the phase and the deconvolution
     const int N1 = a.N1, n1 = nth;
@@ -96,7 +99,7 @@ grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict_
         }
     }
     __syncthreads();
-    lds_fft<true, NT, E, true>(x, fc1, tw + fc1.tw);
+    lds_fft<true, E, true>(x, fc1, tw + fc1.tw, NT);
     C *out = T1 + (int64_t(b) * ncol + c) * a.nR;
     for (int pr = threadIdx.x; pr < a.nR; pr += NT) {
         int l1 = a.r0 + pr;
@@ -112,17 +115,18 @@ template <bool SPLIT> __device__ __forceinline__ int row_slot(int k, int h) {
     return lds_slot<true>(k);
 }
 
-template <bool INV, bool SPLIT, int NT, class C>
-__device__ __forceinline__ void row_fft(C *x, const GridFftDev &g, const C *tw) {
-    if constexpr (SPLIT) lds_fft_split<INV, NT, E>(x, g.args.N2, g.r2h, tw + g.r2.tw, tw + g.r2h.tw);
-    else lds_fft<INV, NT, E, true>(x, g.r2, tw + g.r2.tw);
+template <bool INV, bool SPLIT, class C>
+__device__ __forceinline__ void row_fft(C *x, const GridFftDev &g, const C *tw, int NT) {
+    if constexpr (SPLIT) lds_fft_split<INV, E>(x, g.args.N2, g.r2h, tw + g.r2.tw, tw + g.r2h.tw, NT);
+    else lds_fft<INV, E, true>(x, g.r2, tw + g.r2.tw, NT);
 }
 
-template <class C, int NT, bool SPLIT>
-__global__ void __launch_bounds__(NT)
+template <class C, bool SPLIT>
+__global__ void __launch_bounds__(1024)
 grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__restrict__ tw, GridFftDev g, int per) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int NT = int(blockDim.x);
     const GridFftArgs &a = g.args;
     const int pr = xcd_item(blockIdx.x, per), b = blockIdx.y;
     if (pr >= a.nR) return;
@@ -134,16 +138,17 @@ grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__re
         x[SPLIT ? l2 : lds_slot<true>(l2)] = m != INT32_MIN ? in[int64_t(m + mmax) * a.nR] : czero<C>();
     }
     __syncthreads();
-    row_fft<true, SPLIT, NT>(x, g, tw);
+    row_fft<true, SPLIT>(x, g, tw, NT);
     C *out = U + (int64_t(b) * a.nR + pr) * N2;
     for (int l2 = threadIdx.x; l2 < N2; l2 += NT) out[l2] = x[row_slot<SPLIT>(l2, N2 / 2)];
 }
 
-template <class C, int NT, bool SPLIT>
-__global__ void __launch_bounds__(NT)
+template <class C, bool SPLIT>
+__global__ void __launch_bounds__(1024)
 grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__restrict__ tw, GridFftDev g) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int NT = int(blockDim.x);
     const GridFftArgs &a = g.args;
     const int pr = blockIdx.x, b = blockIdx.y;
     const int64_t ncol = 2 * a.mmax + 1;
@@ -151,7 +156,7 @@ grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__rest
     const C *in = U + (int64_t(b) * a.nR + pr) * N2;
     for (int l2 = threadIdx.x; l2 < N2; l2 += NT) x[SPLIT ? l2 : lds_slot<true>(l2)] = in[l2];
     __syncthreads();
-    row_fft<false, SPLIT, NT>(x, g, tw);
+    row_fft<false, SPLIT>(x, g, tw, NT);
     C *out = T2 + (int64_t(b) * a.nR + pr) * ncol;
     for (int c = threadIdx.x; c < ncol; c += NT) {
         const int m = c - a.mmax;
@@ -159,13 +164,14 @@ grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__rest
     }
 }
 
-template <class C, int NT>
-__global__ void __launch_bounds__(NT)
+template <class C>
+__global__ void __launch_bounds__(1024)
 grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const double *__restrict__ wth,
                     const double2 *__restrict__ pc1, const double *__restrict__ c2, const C *__restrict__ tw,
                     FftLen fth, FftLen fc1, GridFftArgs a, int per) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
+    const int NT = int(blockDim.x);
     const int64_t ncol = 2 * a.mmax + 1;
     const int c = xcd_item(blockIdx.x, per), b = blockIdx.y;
     if (c >= ncol) return;
@@ -178,7 +184,7 @@ grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const do
         x[lds_slot<true>(l1)] = pr < a.nR ? in[int64_t(pr) * ncol] : czero<C>();
     }
     __syncthreads();
-    lds_fft<false, NT, E, true>(x, fc1, tw + fc1.tw);
+    lds_fft<false, E, true>(x, fc1, tw + fc1.tw, NT);
     // back to the nth colatitude modes, times the conjugate phase and the deconvolution
     C v[E];
 #pragma unroll
@@ -198,7 +204,7 @@ grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const do
         }
     }
     __syncthreads();
-    lds_fft<true, NT, E, true>(x, fth, tw + fth.tw);
+    lds_fft<true, E, true>(x, fth, tw + fth.tw, NT);
     double2 *G = Gd + (int64_t(b) * ncol + c) * nh;
     const bool odd = (am + a.spin) & 1;
     for (int j = threadIdx.x; j < nh; j += NT) {
@@ -217,34 +223,22 @@ size_t lds_of(int n) { return sizeof(C) * size_t(lds_slots(n)); }
 
 }  // namespace
 
-// Eight values per thread, 128 .. 1024 threads (lengths up to 8192; longer rows split in two), over
-// swizzled LDS slots: measured 15-25% faster than sixteen values per thread at the apply's lengths
-// (more wavefronts in flight per transform, fewer registers; sixteen on 1024 threads spills).
+// Eight values per thread on ceil(n / 8) threads rounded up to whole wavefronts (64 .. 1024: lengths
+// up to 8192; longer rows split in two), over swizzled LDS slots. Eight measured 15-25% faster than
+// sixteen at the apply's lengths (more wavefronts in flight per transform, fewer registers; sixteen
+// on 1024 threads spills); whole wavefronts rather than powers of two keep the lanes busy (1440
+// values: 192 threads, not 256).
 int grid_fft_threads(int n) {
-    int nt = 128;
-    while (nt < 1024 && n > E * nt) nt *= 2;
-    return nt;
+    const int t = (n + E - 1) / E;
+    return std::min(1024, std::max(64, (t + 63) / 64 * 64));
 }
-
-#define PFSHT_GF_DISPATCH(NTV, LAUNCH) \
-    switch (NTV) {                     \
-    case 128: LAUNCH(128); break;      \
-    case 256: LAUNCH(256); break;      \
-    case 512: LAUNCH(512); break;      \
-    default: LAUNCH(1024); break;      \
-    }
 
 template <class C>
 void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2 *Gd, const double *wth,
                             const double2 *pc1, const double *c2, C *T1, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
-    const C *tw = static_cast<const C *>(g.tw);
-    const size_t lds = lds_of<C>(a.N1);
-#define PFSHT_GF_L(NTV)                                                                                        \
-    grid_col_synth_kernel<C, NTV><<<dim3(2 * a.mmax + 1, nb), NTV, lds, s>>>(Gd, r.jlo, r.jhi, wth, pc1, c2, T1, \
-                                                                          tw, g.th, g.c1, a)
-    PFSHT_GF_DISPATCH(g.nt_col, PFSHT_GF_L)
-#undef PFSHT_GF_L
+    grid_col_synth_kernel<C><<<dim3(2 * a.mmax + 1, nb), g.nt_col, lds_of<C>(a.N1), s>>>(
+        Gd, r.jlo, r.jhi, wth, pc1, c2, T1, static_cast<const C *>(g.tw), g.th, g.c1, a);
 }
 
 template <class C>
@@ -253,13 +247,8 @@ void launch_grid_synth_rows(const GridFftDev &g, const C *T1, C *U, int nb, hipS
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
     const int per = (a.nR + 7) / 8;
-    if (g.split) {
-        grid_row_synth_kernel<C, 1024, true><<<dim3(8 * per, nb), 1024, lds, s>>>(T1, U, tw, g, per);
-        return;
-    }
-#define PFSHT_GF_L(NTV) grid_row_synth_kernel<C, NTV, false><<<dim3(8 * per, nb), NTV, lds, s>>>(T1, U, tw, g, per)
-    PFSHT_GF_DISPATCH(g.nt_row, PFSHT_GF_L)
-#undef PFSHT_GF_L
+    if (g.split) grid_row_synth_kernel<C, true><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, tw, g, per);
+    else grid_row_synth_kernel<C, false><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, tw, g, per);
 }
 
 template <class C>
@@ -267,26 +256,17 @@ void launch_grid_adjoint_rows(const GridFftDev &g, const C *U, C *T2, int nb, hi
     const GridFftArgs &a = g.args;
     const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
-    if (g.split) {
-        grid_row_adj_kernel<C, 1024, true><<<dim3(a.nR, nb), 1024, lds, s>>>(U, T2, tw, g);
-        return;
-    }
-#define PFSHT_GF_L(NTV) grid_row_adj_kernel<C, NTV, false><<<dim3(a.nR, nb), NTV, lds, s>>>(U, T2, tw, g)
-    PFSHT_GF_DISPATCH(g.nt_row, PFSHT_GF_L)
-#undef PFSHT_GF_L
+    if (g.split) grid_row_adj_kernel<C, true><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, tw, g);
+    else grid_row_adj_kernel<C, false><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, tw, g);
 }
 
 template <class C>
 void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wth, const double2 *pc1,
                               const double *c2, double2 *Gd, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
-    const C *tw = static_cast<const C *>(g.tw);
-    const size_t lds = lds_of<C>(a.N1);
     const int per = (2 * a.mmax + 1 + 7) / 8;
-#define PFSHT_GF_L(NTV) \
-    grid_col_adj_kernel<C, NTV><<<dim3(8 * per, nb), NTV, lds, s>>>(T2, Gd, wth, pc1, c2, tw, g.th, g.c1, a, per)
-    PFSHT_GF_DISPATCH(g.nt_col, PFSHT_GF_L)
-#undef PFSHT_GF_L
+    grid_col_adj_kernel<C><<<dim3(8 * per, nb), g.nt_col, lds_of<C>(a.N1), s>>>(
+        T2, Gd, wth, pc1, c2, static_cast<const C *>(g.tw), g.th, g.c1, a, per);
 }
 
 #define PFSHT_GF_INST(C)                                                                                          \

@@ -157,8 +157,8 @@ template <int R, class C> __device__ __forceinline__ void twiddle(C *v, C w1) {
 template <bool SW> __device__ __forceinline__ int lds_slot(int e) { return SW ? (e ^ ((e >> 3) & 7)) : e; }
 __host__ __device__ inline int lds_slots(int n) { return (n + 7) & ~7; }
 
-template <int R, bool INV, int NT, int E, bool SW, class C>
-__device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C *__restrict__ tw) {
+template <int R, bool INV, int E, bool SW, class C>
+__device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C *__restrict__ tw, int NT) {
     constexpr int K = (E + R - 1) / R;
     const int nbf = n / R;
     const float rp = 1.0f / float(p);
@@ -194,21 +194,21 @@ __device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C 
 }
 
 // The DFT of x[0 .. f.n) in place (forward: exp(-2 pi i jk/n); INV: exp(+2 pi i jk/n), unscaled).
-// Every thread of the NT-thread workgroup calls it after a barrier that publishes x; it returns after
-// one. tw is the length's own table (the plan's table + f.tw).
-// n <= E * NT (E: values a thread holds in a pass); SW: swizzled LDS slots (lds_slot).
-template <bool INV, int NT, int E = kFftElems, bool SW = false, class C>
-__device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw) {
+// Every thread of the NT-thread workgroup (NT = blockDim.x, n <= E * NT; E: values a thread holds in
+// a pass) calls it after a barrier that publishes x; it returns after one. tw is the length's own
+// table (the plan's table + f.tw); SW: swizzled LDS slots (lds_slot).
+template <bool INV, int E = kFftElems, bool SW = false, class C>
+__device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw, int NT) {
     int p = 1;
     for (int s = 0; s < f.npass; ++s) {
         const int r = f.rad[s];
         const int tstep = f.n / (p * r);
         switch (r) {
-        case 8: fft_pass<8, INV, NT, E, SW>(x, f.n, p, tstep, tw); break;
-        case 4: fft_pass<4, INV, NT, E, SW>(x, f.n, p, tstep, tw); break;
-        case 2: fft_pass<2, INV, NT, E, SW>(x, f.n, p, tstep, tw); break;
-        case 3: fft_pass<3, INV, NT, E, SW>(x, f.n, p, tstep, tw); break;
-        default: fft_pass<5, INV, NT, E, SW>(x, f.n, p, tstep, tw); break;
+        case 8: fft_pass<8, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
+        case 4: fft_pass<4, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
+        case 2: fft_pass<2, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
+        case 3: fft_pass<3, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
+        default: fft_pass<5, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
         }
         p *= r;
     }
@@ -219,9 +219,9 @@ __device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw) {
 // values), and LDS transforms of the two halves: afterwards X[2m] is at x[m] and X[2m + 1] at
 // x[n/2 + m]. For lengths past E x 1024 values: each half keeps the per-thread share of E values.
 // Unswizzled slots. tw_n is the length-n table, half the length n/2 passes with their own table.
-template <bool INV, int NT, int E, class C>
+template <bool INV, int E, class C>
 __device__ void lds_fft_split(C *x, int n, const FftLen &half, const C *__restrict__ tw_n,
-                              const C *__restrict__ tw_half) {
+                              const C *__restrict__ tw_half, int NT) {
     const int h = n / 2;
     for (int i = threadIdx.x; i < h; i += NT) {
         const C a = x[i], b = x[i + h];
@@ -229,8 +229,8 @@ __device__ void lds_fft_split(C *x, int n, const FftLen &half, const C *__restri
         x[i + h] = cmulw<INV>(csub(a, b), tw_n[i]);
     }
     __syncthreads();
-    lds_fft<INV, NT, E, false>(x, half, tw_half);
-    lds_fft<INV, NT, E, false>(x + h, half, tw_half);
+    lds_fft<INV, E, false>(x, half, tw_half, NT);
+    lds_fft<INV, E, false>(x + h, half, tw_half, NT);
 }
 
 // Host: the passes of length n (8s first, a 2 only when the power of two is 2; then 3s and 5s);
