@@ -242,14 +242,22 @@ def run_plain(args):
 
     W = info["ring_work"]
     nu_b = 2 * 16 * M + 16 * info["es_n1"] * info["es_n2"] + 4 * M  # points + f, fine grid, perm
+    # the grid stage (LDS column + row kernels): the ring columns, the column/row hand-off written and
+    # read once, the stored fine rows
+    ncol, nrow = 2 * info["mmax"] + 1, info["es_rows"]
+    grid_b = 16 * (ncol * info["nth_half"] + 2 * ncol * nrow + nrow * info["es_n2"])
+
+    def both(d, *keys):
+        return (sum(d[k][0] for k in keys), max(d[k][1] for k in keys))
+
     stages = {}
     for name, d, flops, bytes_ in (
             ("ring_synth", ss["ring"], 12.0 * W, None),
             ("ring_adjoint", sa["ring"], 12.0 * W, None),
             ("nufft_interp", ss["nufft"], None, nu_b),
             ("nufft_spread", sa["nufft"], None, nu_b),
-            ("fine_fft_synth", ss["fine_fft"], None, None),
-            ("fine_fft_adjoint", sa["fine_fft"], None, None)):
+            ("grid_synth", both(ss, "grid_fft", "fine_fft"), None, grid_b),
+            ("grid_adjoint", both(sa, "grid_fft", "fine_fft"), None, grid_b)):
         ms_avg = d[0] / max(d[1], 1)
         e = {"ms": ms_avg}
         if flops:
