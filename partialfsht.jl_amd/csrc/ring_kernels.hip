@@ -637,7 +637,7 @@ ring_adj_kernel(const SegItem *__restrict__ items, const double *__restrict__ co
         // workgroups past the nring item workgroups (one launch per apply carries them) zero the
         // columns of the orders above mmax in transforms 0 .. znb - 1, chunks of 2048 entries: launched
         // last, they run in the pass's tail, and the partial reduction writes only orders <= mmax
-        if (nring > 0 && int(blockIdx.x) >= nring) {
+        if (znb > 0 && int(blockIdx.x) >= nring) {
             const int64_t base = int64_t(2 * mmax + 1) * (L + 1), count = int64_t(2 * (L - mmax)) * (L + 1);
             const int64_t nch = (count + 2047) / 2048;
             const int64_t z = int64_t(blockIdx.x) - nring;
@@ -1234,7 +1234,7 @@ void launch_ring_adj_pass(const RingDev &r, const double2 *Gd, OutCols alm, int 
     ring_adj_kernel<NB, TG><<<grid, 64 * kAdjWaves, lds, s>>>(r.aitems, r.costh, r.log2sin, r.log2c, r.ab, r.gc,  \
                                                               r.abofs, r.ckq, r.ckE, r.pofs, Gd, r.partials, r.L, r.n, \
                                                               r.mmax, r.part_b, b0, kNeverSink, nullptr, nullptr, alm, \
-                                                              nzero > 0 ? nring : 0, nb)
+                                                              nring, nzero > 0 ? nb : 0)
             // the item list was built for r.adj_tg tiles per wavefront (two when only one or two
             // transforms use this path)
             switch (ng) {
