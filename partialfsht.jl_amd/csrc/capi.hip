@@ -1979,6 +1979,11 @@ struct pfsht_group_s {
         int nwork = 0, nmwork = 0;
     };
     std::vector<NuClass> classes;
+    // the grid stage of the members whose plans run it in LDS without split rows, grouped per
+    // storage type (0: FP64, 1: FP32); the other stretched members run it on the member streams
+    GridTab *gtabs = nullptr;     // [np]
+    GridGroupWork gw[2];
+    std::vector<char> grid_grouped;  // [np]
     void **ptr_dev = nullptr, **ptr_host = nullptr;  // [kSlots][2 np]: inputs, then outputs
     hipEvent_t slot_ev[kSlots] = {};
     bool slot_used[kSlots] = {};
@@ -2006,6 +2011,10 @@ struct pfsht_group_s {
             if (q) (void)hipStreamDestroy(q);
         if (main) (void)hipStreamDestroy(main);
         dfree(tabs); dfree(fwork); dfree(awork[0]); dfree(awork[1]); dfree(zwork); dfree(ptr_dev); dfree(ntabs);
+        dfree(gtabs);
+        for (auto &w : gw) {
+            dfree(w.col); dfree(w.row_s); dfree(w.row_a); dfree(w.col_a);
+        }
         for (auto &c : classes) {
             dfree(c.work);
             dfree(c.mwork);
@@ -2117,6 +2126,44 @@ static Status group_build(pfsht_group_s &g, const pfsht_plan_t *plans, int np) {
         g.classes[c].nmwork = int(cm[c].size());
     }
     if ((st = upload(&g.ntabs, ntabs, g.bytes)) != kOk) return st;
+    // grouped grid stage: work lists per storage type; the XCD-numbered lists (row synthesis, column
+    // adjoint) give each member whole rounds of eight workgroups
+    {
+        std::vector<GridTab> gt(np);
+        std::vector<int2> wl[2][4];
+        g.grid_grouped.assign(np, 0);
+        for (int i = 0; i < np; ++i) {
+            pfsht_plan_s &p = *g.plans[i];
+            if (p.method != PFSHT_METHOD_STRETCH || !p.fused || p.gf.split) continue;
+            g.grid_grouped[i] = 1;
+            const int cls = p.precision == PFSHT_PREC_MIXED ? 1 : 0;
+            const size_t csz = cls ? sizeof(float2) : sizeof(double2);
+            GridTab &t = gt[i];
+            t.g = p.gf;
+            t.Gd = p.rings->Gd; t.jlo = p.rings->d.jlo; t.jhi = p.rings->d.jhi; t.wth = p.wth;
+            t.pc1 = p.nu.pc1; t.c2 = p.nu.c2; t.Ut = p.Ut; t.U = p.U;
+            const int ncol = 2 * p.mmax + 1, nR = p.nu.nR;
+            for (int q = 0; q < ncol; ++q) wl[cls][0].push_back(make_int2(i, q));
+            for (int w = 0; w < 8 * ((nR + 7) / 8); ++w) wl[cls][1].push_back(make_int2(i, w));
+            for (int r = 0; r < nR; ++r) wl[cls][2].push_back(make_int2(i, r));
+            for (int w = 0; w < 8 * ((ncol + 7) / 8); ++w) wl[cls][3].push_back(make_int2(i, w));
+            GridGroupWork &gw = g.gw[cls];
+            gw.nt_col = std::max(gw.nt_col, p.gf.nt_col);
+            gw.nt_row = std::max(gw.nt_row, p.gf.nt_row);
+            gw.lds_col = std::max(gw.lds_col, csz * size_t(lds_slots(p.nu.N1)));
+            gw.lds_row = std::max(gw.lds_row, csz * size_t(lds_slots(p.nu.N2)));
+        }
+        if ((st = upload(&g.gtabs, gt, g.bytes)) != kOk) return st;
+        for (int c = 0; c < 2; ++c) {
+            GridGroupWork &gw = g.gw[c];
+            if ((st = upload(&gw.col, wl[c][0], g.bytes)) != kOk) return st;
+            if ((st = upload(&gw.row_s, wl[c][1], g.bytes)) != kOk) return st;
+            if ((st = upload(&gw.row_a, wl[c][2], g.bytes)) != kOk) return st;
+            if ((st = upload(&gw.col_a, wl[c][3], g.bytes)) != kOk) return st;
+            gw.ncol = int(wl[c][0].size()); gw.nrow_s = int(wl[c][1].size());
+            gw.nrow_a = int(wl[c][2].size()); gw.ncol_a = int(wl[c][3].size());
+        }
+    }
     for (auto &e : g.slot_ev) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto &q : g.streams) PFSHT_HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     PFSHT_HIP_TRY(hipStreamCreateWithFlags(&g.main, hipStreamNonBlocking));
@@ -2145,6 +2192,7 @@ static Status group_enqueue(pfsht_group_s &g, int dir, int k, hipStream_t s) {
             else launch_direct_adjoint(in, p.dphi, p.dmlim, p.dperm, r.Gd, p.M, p.mmax, 1, q);
             return kOk;
         }
+        if (g.grid_grouped[i]) return kOk;  // the grouped grid-stage launches cover it
         // the interpolation and spreading run grouped, after and before the members' stages
         const bool mixed = p.precision == PFSHT_PREC_MIXED;
         if (dir == 0)
@@ -2165,13 +2213,27 @@ static Status group_enqueue(pfsht_group_s &g, int dir, int k, hipStream_t s) {
             }
         }
     };
+    // the members' own stages on the member streams (Direct members, and the grid stage of members
+    // outside the grouped launches), the grouped grid-stage launches on s meanwhile
     auto members = [&]() -> Status {
-        PFSHT_HIP_TRY(hipEventRecord(g.fork, s));
-        const int ns = std::min(np, pfsht_group_s::kStreams);
-        for (int j = 0; j < ns; ++j) PFSHT_HIP_TRY(hipStreamWaitEvent(g.streams[j], g.fork, 0));
-        for (int i = 0; i < np; ++i) {
-            Status st = member(i, g.streams[i % ns]);
-            if (st != kOk) return st;
+        std::vector<int> own;
+        for (int i = 0; i < np; ++i)
+            if (!g.grid_grouped[i]) own.push_back(i);
+        const int ns = std::min(int(own.size()), pfsht_group_s::kStreams);
+        if (ns > 0) {
+            PFSHT_HIP_TRY(hipEventRecord(g.fork, s));
+            for (int j = 0; j < ns; ++j) PFSHT_HIP_TRY(hipStreamWaitEvent(g.streams[j], g.fork, 0));
+            for (size_t k2 = 0; k2 < own.size(); ++k2) {
+                Status st = member(own[k2], g.streams[k2 % ns]);
+                if (st != kOk) return st;
+            }
+        }
+        if (dir == 0) {
+            launch_grid_synth_group<double2>(g.gtabs, g.gw[0], s);
+            launch_grid_synth_group<float2>(g.gtabs, g.gw[1], s);
+        } else {
+            launch_grid_adjoint_group<double2>(g.gtabs, g.gw[0], s);
+            launch_grid_adjoint_group<float2>(g.gtabs, g.gw[1], s);
         }
         for (int j = 0; j < ns; ++j) {
             PFSHT_HIP_TRY(hipEventRecord(g.join[j], g.streams[j]));

@@ -42,15 +42,19 @@ __device__ __forceinline__ int fine_mode(int l, int n, int N) {
     return -1;
 }
 
+// Kernel bodies: item = the order column c (column kernels) or the stored row pr (row kernels), b =
+// the transform; g holds the lengths, the sizes and the twiddle table. The plain kernels take them
+// from the launch, the group kernels (a plan group's members in one launch) from a GridTab per plan.
 template <class C>
-__global__ void __launch_bounds__(1024)
-grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
-                      const double *__restrict__ wth, const double2 *__restrict__ pc1, const double *__restrict__ c2,
-                      C *__restrict__ T1, const C *__restrict__ tw, FftLen fth, FftLen fc1, GridFftArgs a) {
+__device__ __forceinline__ void col_synth_body(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo,
+                                               const int32_t *__restrict__ jhi, const double *__restrict__ wth,
+                                               const double2 *__restrict__ pc1, const double *__restrict__ c2,
+                                               C *__restrict__ T1, const GridFftDev &g, int c, int b) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
     const int NT = int(blockDim.x);
-    const int c = blockIdx.x, b = blockIdx.y;
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
     const int64_t ncol = 2 * a.mmax + 1;
     const int m = c - a.mmax, am = m < 0 ? -m : m;
     const int lo = jlo[am], hi = jhi[am];
@@ -61,15 +65,15 @@ grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict_
         const int j = i < nh ? i : nth - 1 - i;
         double2 v = make_double2(0.0, 0.0);
         if (j >= lo && j <= hi) {
-            const double2 g = G[j];
+            const double2 gv = G[j];
             double w = wth[j];
             if (i >= nh && odd) w = -w;
-            v = make_double2(g.x * w, g.y * w);
+            v = make_double2(gv.x * w, gv.y * w);
         }
         stc(x + lds_slot<true>(i), v);
     }
     __syncthreads();
-    lds_fft<false, E, true>(x, fth, tw + fth.tw, NT);
+    lds_fft<false, E, true>(x, g.th, tw + g.th.tw, NT);
     // the fine colatitude layout: mode i of the nth-point transform at fine index l1 (zero padding
     // between), times the phase and the deconvolution
     const int N1 = a.N1, n1 = nth;
@@ -99,7 +103,7 @@ grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict_
         }
     }
     __syncthreads();
-    lds_fft<true, E, true>(x, fc1, tw + fc1.tw, NT);
+    lds_fft<true, E, true>(x, g.c1, tw + g.c1.tw, NT);
     C *out = T1 + (int64_t(b) * ncol + c) * a.nR;
     for (int pr = threadIdx.x; pr < a.nR; pr += NT) {
         int l1 = a.r0 + pr;
@@ -122,14 +126,12 @@ __device__ __forceinline__ void row_fft(C *x, const GridFftDev &g, const C *tw, 
 }
 
 template <class C, bool SPLIT>
-__global__ void __launch_bounds__(1024)
-grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__restrict__ tw, GridFftDev g, int per) {
+__device__ __forceinline__ void row_synth_body(const C *__restrict__ T1, C *__restrict__ U, const GridFftDev &g,
+                                               int pr, int b) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
     const int NT = int(blockDim.x);
     const GridFftArgs &a = g.args;
-    const int pr = xcd_item(blockIdx.x, per), b = blockIdx.y;
-    if (pr >= a.nR) return;
     const int64_t ncol = 2 * a.mmax + 1;
     const int N2 = a.N2, mmax = a.mmax;
     const C *in = T1 + int64_t(b) * ncol * a.nR + pr;
@@ -138,25 +140,24 @@ grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, const C *__re
         x[SPLIT ? l2 : lds_slot<true>(l2)] = m != INT32_MIN ? in[int64_t(m + mmax) * a.nR] : czero<C>();
     }
     __syncthreads();
-    row_fft<true, SPLIT>(x, g, tw, NT);
+    row_fft<true, SPLIT>(x, g, static_cast<const C *>(g.tw), NT);
     C *out = U + (int64_t(b) * a.nR + pr) * N2;
     for (int l2 = threadIdx.x; l2 < N2; l2 += NT) out[l2] = x[row_slot<SPLIT>(l2, N2 / 2)];
 }
 
 template <class C, bool SPLIT>
-__global__ void __launch_bounds__(1024)
-grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__restrict__ tw, GridFftDev g) {
+__device__ __forceinline__ void row_adj_body(const C *__restrict__ U, C *__restrict__ T2, const GridFftDev &g,
+                                             int pr, int b) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
     const int NT = int(blockDim.x);
     const GridFftArgs &a = g.args;
-    const int pr = blockIdx.x, b = blockIdx.y;
     const int64_t ncol = 2 * a.mmax + 1;
     const int N2 = a.N2;
     const C *in = U + (int64_t(b) * a.nR + pr) * N2;
     for (int l2 = threadIdx.x; l2 < N2; l2 += NT) x[SPLIT ? l2 : lds_slot<true>(l2)] = in[l2];
     __syncthreads();
-    row_fft<false, SPLIT>(x, g, tw, NT);
+    row_fft<false, SPLIT>(x, g, static_cast<const C *>(g.tw), NT);
     C *out = T2 + (int64_t(b) * a.nR + pr) * ncol;
     for (int c = threadIdx.x; c < ncol; c += NT) {
         const int m = c - a.mmax;
@@ -165,16 +166,15 @@ grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, const C *__rest
 }
 
 template <class C>
-__global__ void __launch_bounds__(1024)
-grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const double *__restrict__ wth,
-                    const double2 *__restrict__ pc1, const double *__restrict__ c2, const C *__restrict__ tw,
-                    FftLen fth, FftLen fc1, GridFftArgs a, int per) {
+__device__ __forceinline__ void col_adj_body(const C *__restrict__ T2, double2 *__restrict__ Gd,
+                                             const double *__restrict__ wth, const double2 *__restrict__ pc1,
+                                             const double *__restrict__ c2, const GridFftDev &g, int c, int b) {
     extern __shared__ __align__(16) unsigned char lds_gf_raw[];
     C *x = reinterpret_cast<C *>(lds_gf_raw);
     const int NT = int(blockDim.x);
+    const GridFftArgs &a = g.args;
+    const C *tw = static_cast<const C *>(g.tw);
     const int64_t ncol = 2 * a.mmax + 1;
-    const int c = xcd_item(blockIdx.x, per), b = blockIdx.y;
-    if (c >= ncol) return;
     const int m = c - a.mmax, am = m < 0 ? -m : m;
     const int N1 = a.N1, nth = a.nth, n1 = nth, nh = a.nh;
     const C *in = T2 + int64_t(b) * a.nR * ncol + c;
@@ -184,7 +184,7 @@ grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const do
         x[lds_slot<true>(l1)] = pr < a.nR ? in[int64_t(pr) * ncol] : czero<C>();
     }
     __syncthreads();
-    lds_fft<false, E, true>(x, fc1, tw + fc1.tw, NT);
+    lds_fft<false, E, true>(x, g.c1, tw + g.c1.tw, NT);
     // back to the nth colatitude modes, times the conjugate phase and the deconvolution
     C v[E];
 #pragma unroll
@@ -204,7 +204,7 @@ grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const do
         }
     }
     __syncthreads();
-    lds_fft<true, E, true>(x, fth, tw + fth.tw, NT);
+    lds_fft<true, E, true>(x, g.th, tw + g.th.tw, NT);
     double2 *G = Gd + (int64_t(b) * ncol + c) * nh;
     const bool odd = (am + a.spin) & 1;
     for (int j = threadIdx.x; j < nh; j += NT) {
@@ -216,6 +216,92 @@ grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const do
         const double w = wth[j];
         G[j] = make_double2(v2.x * w, v2.y * w);
     }
+}
+
+// One plan: workgroup (item, transform) from the grid; the row synthesis and the column adjoint
+// number their items XCD-wise (xcd_item over 8 ceil(count / 8) workgroups).
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_col_synth_kernel(const double2 *__restrict__ Gd, const int32_t *__restrict__ jlo, const int32_t *__restrict__ jhi,
+                      const double *__restrict__ wth, const double2 *__restrict__ pc1, const double *__restrict__ c2,
+                      C *__restrict__ T1, GridFftDev g) {
+    col_synth_body<C>(Gd, jlo, jhi, wth, pc1, c2, T1, g, blockIdx.x, blockIdx.y);
+}
+
+template <class C, bool SPLIT>
+__global__ void __launch_bounds__(1024)
+grid_row_synth_kernel(const C *__restrict__ T1, C *__restrict__ U, GridFftDev g, int per) {
+    const int pr = xcd_item(blockIdx.x, per);
+    if (pr >= g.args.nR) return;
+    row_synth_body<C, SPLIT>(T1, U, g, pr, blockIdx.y);
+}
+
+template <class C, bool SPLIT>
+__global__ void __launch_bounds__(1024)
+grid_row_adj_kernel(const C *__restrict__ U, C *__restrict__ T2, GridFftDev g) {
+    row_adj_body<C, SPLIT>(U, T2, g, blockIdx.x, blockIdx.y);
+}
+
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_col_adj_kernel(const C *__restrict__ T2, double2 *__restrict__ Gd, const double *__restrict__ wth,
+                    const double2 *__restrict__ pc1, const double *__restrict__ c2, GridFftDev g, int per) {
+    const int c = xcd_item(blockIdx.x, per);
+    if (c >= 2 * g.args.mmax + 1) return;
+    col_adj_body<C>(T2, Gd, wth, pc1, c2, g, c, blockIdx.y);
+}
+
+// A plan group: workgroup i takes work[i] = (plan, item), the plan's lengths and buffers from
+// tabs[plan] (one transform per plan; no split rows). The row synthesis and column adjoint items
+// are XCD-numbered within each plan, whose item lists start at multiples of eight. The plan's
+// GridFftDev is copied to LDS first: the passes index its radix arrays at run time, which in a
+// register copy would go to scratch.
+__device__ __forceinline__ const GridFftDev &group_dev(const GridTab &T) {
+    constexpr int nw = int(sizeof(GridFftDev) / sizeof(int));
+    static_assert(sizeof(GridFftDev) % sizeof(int) == 0 && alignof(GridFftDev) <= 8, "copied as words");
+    __shared__ __align__(8) int gsh[nw];
+    const int *src = reinterpret_cast<const int *>(&T.g);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) gsh[i] = src[i];
+    __syncthreads();
+    return *reinterpret_cast<const GridFftDev *>(gsh);
+}
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_col_synth_group(const GridTab *__restrict__ tabs, const int2 *__restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    const GridTab &T = tabs[w.x];
+    col_synth_body<C>(T.Gd, T.jlo, T.jhi, T.wth, T.pc1, T.c2, static_cast<C *>(T.Ut), group_dev(T), w.y, 0);
+}
+
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_row_synth_group(const GridTab *__restrict__ tabs, const int2 *__restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    const GridTab &T = tabs[w.x];
+    const GridFftDev &g = group_dev(T);
+    const int pr = xcd_item(w.y, (g.args.nR + 7) / 8);
+    if (pr >= g.args.nR) return;
+    row_synth_body<C, false>(static_cast<const C *>(T.Ut), static_cast<C *>(T.U), g, pr, 0);
+}
+
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_row_adj_group(const GridTab *__restrict__ tabs, const int2 *__restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    const GridTab &T = tabs[w.x];
+    row_adj_body<C, false>(static_cast<const C *>(T.U), static_cast<C *>(T.Ut), group_dev(T), w.y, 0);
+}
+
+template <class C>
+__global__ void __launch_bounds__(1024)
+grid_col_adj_group(const GridTab *__restrict__ tabs, const int2 *__restrict__ work) {
+    const int2 w = work[blockIdx.x];
+    const GridTab &T = tabs[w.x];
+    const GridFftDev &g = group_dev(T);
+    const int ncol = 2 * g.args.mmax + 1;
+    const int c = xcd_item(w.y, (ncol + 7) / 8);
+    if (c >= ncol) return;
+    col_adj_body<C>(static_cast<const C *>(T.Ut), T.Gd, T.wth, T.pc1, T.c2, g, c, 0);
 }
 
 template <class C>
@@ -238,26 +324,24 @@ void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2
                             const double2 *pc1, const double *c2, C *T1, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
     grid_col_synth_kernel<C><<<dim3(2 * a.mmax + 1, nb), g.nt_col, lds_of<C>(a.N1), s>>>(
-        Gd, r.jlo, r.jhi, wth, pc1, c2, T1, static_cast<const C *>(g.tw), g.th, g.c1, a);
+        Gd, r.jlo, r.jhi, wth, pc1, c2, T1, g);
 }
 
 template <class C>
 void launch_grid_synth_rows(const GridFftDev &g, const C *T1, C *U, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
-    const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
     const int per = (a.nR + 7) / 8;
-    if (g.split) grid_row_synth_kernel<C, true><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, tw, g, per);
-    else grid_row_synth_kernel<C, false><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, tw, g, per);
+    if (g.split) grid_row_synth_kernel<C, true><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, g, per);
+    else grid_row_synth_kernel<C, false><<<dim3(8 * per, nb), g.nt_row, lds, s>>>(T1, U, g, per);
 }
 
 template <class C>
 void launch_grid_adjoint_rows(const GridFftDev &g, const C *U, C *T2, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
-    const C *tw = static_cast<const C *>(g.tw);
     const size_t lds = lds_of<C>(a.N2);
-    if (g.split) grid_row_adj_kernel<C, true><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, tw, g);
-    else grid_row_adj_kernel<C, false><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, tw, g);
+    if (g.split) grid_row_adj_kernel<C, true><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, g);
+    else grid_row_adj_kernel<C, false><<<dim3(a.nR, nb), g.nt_row, lds, s>>>(U, T2, g);
 }
 
 template <class C>
@@ -265,8 +349,23 @@ void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wt
                               const double *c2, double2 *Gd, int nb, hipStream_t s) {
     const GridFftArgs &a = g.args;
     const int per = (2 * a.mmax + 1 + 7) / 8;
-    grid_col_adj_kernel<C><<<dim3(8 * per, nb), g.nt_col, lds_of<C>(a.N1), s>>>(
-        T2, Gd, wth, pc1, c2, static_cast<const C *>(g.tw), g.th, g.c1, a, per);
+    grid_col_adj_kernel<C><<<dim3(8 * per, nb), g.nt_col, lds_of<C>(a.N1), s>>>(T2, Gd, wth, pc1, c2, g, per);
+}
+
+template <class C>
+void launch_grid_synth_group(const GridTab *tabs, const GridGroupWork &w, hipStream_t s) {
+    if (w.ncol > 0)
+        grid_col_synth_group<C><<<dim3(w.ncol), w.nt_col, w.lds_col, s>>>(tabs, w.col);
+    if (w.nrow_s > 0)
+        grid_row_synth_group<C><<<dim3(w.nrow_s), w.nt_row, w.lds_row, s>>>(tabs, w.row_s);
+}
+
+template <class C>
+void launch_grid_adjoint_group(const GridTab *tabs, const GridGroupWork &w, hipStream_t s) {
+    if (w.nrow_a > 0)
+        grid_row_adj_group<C><<<dim3(w.nrow_a), w.nt_row, w.lds_row, s>>>(tabs, w.row_a);
+    if (w.ncol_a > 0)
+        grid_col_adj_group<C><<<dim3(w.ncol_a), w.nt_col, w.lds_col, s>>>(tabs, w.col_a);
 }
 
 #define PFSHT_GF_INST(C)                                                                                          \
@@ -275,7 +374,9 @@ void launch_grid_adjoint_cols(const GridFftDev &g, const C *T2, const double *wt
     template void launch_grid_synth_rows<C>(const GridFftDev &, const C *, C *, int, hipStream_t);                \
     template void launch_grid_adjoint_rows<C>(const GridFftDev &, const C *, C *, int, hipStream_t);              \
     template void launch_grid_adjoint_cols<C>(const GridFftDev &, const C *, const double *, const double2 *,     \
-                                              const double *, double2 *, int, hipStream_t);
+                                              const double *, double2 *, int, hipStream_t);                    \
+    template void launch_grid_synth_group<C>(const GridTab *, const GridGroupWork &, hipStream_t);                \
+    template void launch_grid_adjoint_group<C>(const GridTab *, const GridGroupWork &, hipStream_t);
 PFSHT_GF_INST(double2)
 PFSHT_GF_INST(float2)
 #undef PFSHT_GF_INST

@@ -198,7 +198,7 @@ __device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C 
 // a pass) calls it after a barrier that publishes x; it returns after one. tw is the length's own
 // table (the plan's table + f.tw); SW: swizzled LDS slots (lds_slot).
 template <bool INV, int E = kFftElems, bool SW = false, class C>
-__device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw, int NT) {
+__device__ __forceinline__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw, int NT) {
     int p = 1;
     for (int s = 0; s < f.npass; ++s) {
         const int r = f.rad[s];
@@ -220,7 +220,7 @@ __device__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw, int NT)
 // x[n/2 + m]. For lengths past E x 1024 values: each half keeps the per-thread share of E values.
 // Unswizzled slots. tw_n is the length-n table, half the length n/2 passes with their own table.
 template <bool INV, int E, class C>
-__device__ void lds_fft_split(C *x, int n, const FftLen &half, const C *__restrict__ tw_n,
+__device__ __forceinline__ void lds_fft_split(C *x, int n, const FftLen &half, const C *__restrict__ tw_n,
                               const C *__restrict__ tw_half, int NT) {
     const int h = n / 2;
     for (int i = threadIdx.x; i < h; i += NT) {

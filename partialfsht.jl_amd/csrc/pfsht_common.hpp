@@ -347,6 +347,28 @@ struct GridFftDev {
 };
 // workgroup size for an LDS transform of length n (n <= kFftElems * size, size <= 1024)
 int grid_fft_threads(int n);
+// One single-transform plan of a plan group's grouped grid-stage launches (rows not split).
+struct GridTab {
+    GridFftDev g;
+    double2 *Gd = nullptr;  // the ring buffer
+    const int32_t *jlo = nullptr, *jhi = nullptr;
+    const double *wth = nullptr;
+    const double2 *pc1 = nullptr;
+    const double *c2 = nullptr;
+    void *Ut = nullptr, *U = nullptr;
+};
+// The grouped launches' work lists (device) of one storage type: {plan, item} per workgroup, and the
+// workgroup size and LDS of each kernel (the largest member's).
+struct GridGroupWork {
+    int2 *col = nullptr, *row_s = nullptr, *row_a = nullptr, *col_a = nullptr;
+    int ncol = 0, nrow_s = 0, nrow_a = 0, ncol_a = 0;
+    int nt_col = 64, nt_row = 64;
+    size_t lds_col = 0, lds_row = 0;
+};
+template <class C>
+void launch_grid_synth_group(const GridTab *tabs, const GridGroupWork &w, hipStream_t s);
+template <class C>
+void launch_grid_adjoint_group(const GridTab *tabs, const GridGroupWork &w, hipStream_t s);
 // synthesis: the column kernel (ring buffer -> T1), then the row kernel (T1 -> U)
 template <class C>
 void launch_grid_synth_cols(const GridFftDev &g, const RingDev &r, const double2 *Gd, const double *wth,
