@@ -202,9 +202,11 @@ int pfsht_adjoint_cols(pfsht_plan_t plan, int32_t nb, const void *const *f_cols,
 /* Plan groups: `mul!` over many independent single-transform plans at once — the farmed apply of
  * independent plans in the reference (ext/PartialFSHTOhMyThreadsExt.jl:151-159 over
  * src/Plan.jl:201-219), for many small plans that one at a time leave most of the GPU idle. The
- * ring passes of all members run as one launch over a (plan, order, 64-ring tile) work list; the
- * members' grid, FFT and point stages run concurrently on the group's streams; each direction is
- * replayed from a hipGraph after its first call. Members: scalar stretched or Direct plans with
+ * ring passes of all members run as one launch over a (plan, order, 64-ring tile) work list, and
+ * so do the interpolation and spreading (over (plan, bin) lists) and the LDS grid stage (over
+ * (plan, column) and (plan, row) lists); Direct members and grid stages past a workgroup's LDS run
+ * concurrently on the group's streams. Each direction is replayed from a hipGraph on the caller's
+ * stream after its first call. Members: scalar stretched or Direct plans with
  * ntrans = 1 on one device (others: PFSHT_E_UNSUPPORTED); a plan belongs to at most one group in
  * use at a time, and must not be applied on its own while a group apply is in flight.
  * alm[i] and f[i] are member i's device buffers, in its layouts; the results equal the members'

@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 
 #include "../../include/pfsht.h"
@@ -645,6 +646,26 @@ struct TimedApply {
     bool recorded[kStages + 1];
     int stage_of[kStages];
 };
+
+// An event that marks the end of a plan's last apply: the plan's own, or a plan group's, shared by
+// every member of the group (one record per group apply rather than one per member).
+struct SharedEvent {
+    hipEvent_t e = nullptr;
+    SharedEvent() = default;
+    SharedEvent(const SharedEvent &) = delete;
+    SharedEvent &operator=(const SharedEvent &) = delete;
+    ~SharedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+using EventRef = std::shared_ptr<SharedEvent>;
+
+static Status event_make(EventRef &r) {
+    auto ev = std::make_shared<SharedEvent>();
+    PFSHT_HIP_TRY(hipEventCreateWithFlags(&ev->e, hipEventDisableTiming));
+    r = std::move(ev);
+    return kOk;
+}
 }  // namespace
 
 struct pfsht_plan_s {
@@ -684,10 +705,11 @@ struct pfsht_plan_s {
     int64_t bytes = 0;
     std::mutex mu;
     // Applies of one plan share its work buffers, so they are ordered across streams: every apply
-    // records `done` on its stream, and an apply on another stream first waits for it.
-    hipEvent_t done = nullptr;
+    // leaves `last` marking its end on `last_stream` (the plan's own `done`, recorded by its apply,
+    // or the event a plan group records once for all its members), and an apply on another stream
+    // first waits for it.
+    EventRef done, last;
     hipStream_t last_stream = nullptr;
-    bool has_done = false;
     // multi-box plan (MultiBoxLeaf, src/MultiBox.jl:20-73): one stretched plan per box, the
     // points of each box (device indices into the caller's field), a box's field, and the
     // accumulator the adjoint sums the boxes' coefficients through
@@ -738,7 +760,6 @@ struct pfsht_plan_s {
         dfree(fft_work); dfree(seam_work); dfree(gf.tw);
         for (auto &t : pending) for (auto e : t.ev) pool.push_back(e);
         for (auto e : pool) (void)hipEventDestroy(e);
-        if (done) (void)hipEventDestroy(done);
         dfree(col_dev);
         if (col_host) (void)hipHostFree(col_host);
         for (auto e : col_ev)
@@ -757,13 +778,14 @@ static Status cols_alloc(pfsht_plan_s &p) {
 
 // Orders an apply on stream s after the plan's previous apply (on any stream).
 static Status apply_begin(pfsht_plan_s &p, hipStream_t s) {
-    if (!p.done) PFSHT_HIP_TRY(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
-    if (p.has_done && s != p.last_stream) PFSHT_HIP_TRY(hipStreamWaitEvent(s, p.done, 0));
+    if (p.last && s != p.last_stream) PFSHT_HIP_TRY(hipStreamWaitEvent(s, p.last->e, 0));
     return kOk;
 }
 static Status apply_end(pfsht_plan_s &p, hipStream_t s) {
-    PFSHT_HIP_TRY(hipEventRecord(p.done, s));
-    p.has_done = true;
+    Status st;
+    if (!p.done && (st = event_make(p.done)) != kOk) return st;
+    PFSHT_HIP_TRY(hipEventRecord(p.done->e, s));
+    p.last = p.done;
     p.last_stream = s;
     return kOk;
 }
@@ -1985,12 +2007,12 @@ struct pfsht_group_s {
     GridGroupWork gw[2];
     std::vector<char> grid_grouped;  // [np]
     void **ptr_dev = nullptr, **ptr_host = nullptr;  // [kSlots][2 np]: inputs, then outputs
-    hipEvent_t slot_ev[kSlots] = {};
+    EventRef slot_ev[kSlots];  // each slot's last apply; also its members' `last`
     bool slot_used[kSlots] = {};
     int slot_next = 0;
     hipStream_t streams[kStreams] = {};
-    hipStream_t main = nullptr;  // the group's work runs here (capturable whatever the caller's stream)
-    hipEvent_t fork = nullptr, join[kStreams] = {}, enter = nullptr, leave = nullptr;
+    hipStream_t main = nullptr;  // the replays are captured here (whatever the caller's stream)
+    hipEvent_t fork = nullptr, join[kStreams] = {};
     hipGraphExec_t graph[2][kSlots] = {};
     std::mutex mu;
 
@@ -2000,13 +2022,9 @@ struct pfsht_group_s {
         for (auto &gd : graph)
             for (auto g : gd)
                 if (g) (void)hipGraphExecDestroy(g);
-        for (auto e : slot_ev)
-            if (e) (void)hipEventDestroy(e);
         for (auto e : join)
             if (e) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
-        if (enter) (void)hipEventDestroy(enter);
-        if (leave) (void)hipEventDestroy(leave);
         for (auto q : streams)
             if (q) (void)hipStreamDestroy(q);
         if (main) (void)hipStreamDestroy(main);
@@ -2164,12 +2182,11 @@ static Status group_build(pfsht_group_s &g, const pfsht_plan_t *plans, int np) {
             gw.nrow_a = int(wl[c][2].size()); gw.ncol_a = int(wl[c][3].size());
         }
     }
-    for (auto &e : g.slot_ev) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : g.slot_ev)
+        if ((st = event_make(e)) != kOk) return st;
     for (auto &q : g.streams) PFSHT_HIP_TRY(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     PFSHT_HIP_TRY(hipStreamCreateWithFlags(&g.main, hipStreamNonBlocking));
     PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.fork, hipEventDisableTiming));
-    PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.enter, hipEventDisableTiming));
-    PFSHT_HIP_TRY(hipEventCreateWithFlags(&g.leave, hipEventDisableTiming));
     for (auto &e : g.join) PFSHT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     PFSHT_HIP_TRY(hipDeviceSynchronize());
     return kOk;
@@ -2273,7 +2290,7 @@ static int group_apply(pfsht_group_t g, int dir, const void *const *in, void *co
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int k = g->slot_next;
     g->slot_next = (k + 1) % pfsht_group_s::kSlots;
-    if (g->slot_used[k]) PFSHT_HIP_TRY(hipEventSynchronize(g->slot_ev[k]));
+    if (g->slot_used[k]) PFSHT_HIP_TRY(hipEventSynchronize(g->slot_ev[k]->e));
     void **h = g->ptr_host + size_t(k) * 2 * np;
     for (int i = 0; i < np; ++i) {
         h[i] = const_cast<void *>(in[i]);
@@ -2281,21 +2298,24 @@ static int group_apply(pfsht_group_t g, int dir, const void *const *in, void *co
     }
     PFSHT_HIP_TRY(hipMemcpyAsync(g->ptr_dev + size_t(k) * 2 * np, h, 2 * size_t(np) * sizeof(void *),
                                  hipMemcpyHostToDevice, s));
+    // order after each member's previous apply on another stream (members last applied together
+    // share one event)
+    std::vector<const SharedEvent *> waited;
+    for (auto *p : g->plans) {
+        if (!p->last || s == p->last_stream) continue;
+        if (std::find(waited.begin(), waited.end(), p->last.get()) != waited.end()) continue;
+        waited.push_back(p->last.get());
+        PFSHT_HIP_TRY(hipStreamWaitEvent(s, p->last->e, 0));
+    }
     Status st;
-    for (auto *p : g->plans)
-        if ((st = apply_begin(*p, s)) != kOk) return st;
-    // the group's own stream joins the caller's stream, runs the apply, and the caller's stream
-    // waits for it
-    hipStream_t gs = g->main;
-    PFSHT_HIP_TRY(hipEventRecord(g->enter, s));
-    PFSHT_HIP_TRY(hipStreamWaitEvent(gs, g->enter, 0));
     hipGraphExec_t &ge = g->graph[dir][k];
     if (!ge) {
-        // capture the sequence once; where capture fails the sequence runs eagerly
+        // capture the sequence once, on the group's own stream (capture runs nothing); where capture
+        // fails the sequence runs eagerly
         hipGraph_t graph = nullptr;
-        if (hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            st = group_enqueue(*g, dir, k, gs);
-            const hipError_t e = hipStreamEndCapture(gs, &graph);
+        if (hipStreamBeginCapture(g->main, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            st = group_enqueue(*g, dir, k, g->main);
+            const hipError_t e = hipStreamEndCapture(g->main, &graph);
             if (st == kOk && e == hipSuccess && graph &&
                 hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0) != hipSuccess)
                 ge = nullptr;
@@ -2304,16 +2324,18 @@ static int group_apply(pfsht_group_t g, int dir, const void *const *in, void *co
         }
         (void)hipGetLastError();  // a refused capture leaves its error behind
     }
+    // the replay (or the eager sequence) runs on the caller's stream
     if (ge) {
-        PFSHT_HIP_TRY(hipGraphLaunch(ge, gs));
-    } else if ((st = group_enqueue(*g, dir, k, gs)) != kOk) {
+        PFSHT_HIP_TRY(hipGraphLaunch(ge, s));
+    } else if ((st = group_enqueue(*g, dir, k, s)) != kOk) {
         return st;
     }
-    PFSHT_HIP_TRY(hipEventRecord(g->leave, gs));
-    PFSHT_HIP_TRY(hipStreamWaitEvent(s, g->leave, 0));
-    for (auto *p : g->plans)
-        if ((st = apply_end(*p, s)) != kOk) return st;
-    PFSHT_HIP_TRY(hipEventRecord(g->slot_ev[k], s));
+    // one event marks the end for the group and for every member
+    PFSHT_HIP_TRY(hipEventRecord(g->slot_ev[k]->e, s));
+    for (auto *p : g->plans) {
+        p->last = g->slot_ev[k];
+        p->last_stream = s;
+    }
     g->slot_used[k] = true;
     return kOk;
 }
@@ -2719,7 +2741,7 @@ void plan_shape(pfsht_plan_t p, int &L, int64_t &M, int &B, int &dev) {
 void plan_release(pfsht_plan_t p) {
     if (!p) return;
     DeviceGuard dg(p->dev);
-    if (p->done && p->has_done) (void)hipEventSynchronize(p->done);
+    if (p->last) (void)hipEventSynchronize(p->last->e);
     for (int i = 0; i < pfsht_plan_s::kColSlots; ++i)
         if (p->col_used[i]) (void)hipEventSynchronize(p->col_ev[i]);
     delete p;

@@ -93,3 +93,55 @@ def test_group_arguments(torch, P, rng):
         g.synth([torch.zeros(4999, dtype=torch.complex128, device="cuda")], [a])
     with pytest.raises(P.DimensionMismatch):
         g.synth([], [])
+
+
+def test_group_orders_member_applies_across_streams(torch, P, O, rng):
+    """A member applied on its own on another stream waits for the group apply that last used its
+    work buffers, and a group apply waits for a member's own apply on another stream: a long kernel
+    ahead of the first apply keeps the second one's stream from finishing until it ends."""
+    plans = members(P, rng, n=4)[:4]
+    g = P.Group(plans)
+    alms, fs = [], []
+    for p in plans:
+        a = O.rand_alm_fast(p.lmax, rng, cl_power=2.0)
+        alms.append(torch.from_numpy(np.asarray(a).ravel(order="F").copy()).cuda())
+        fs.append(torch.from_numpy(rng.standard_normal(p.npts) + 1j * rng.standard_normal(p.npts)).cuda())
+    f_ref = [p @ alms[i] for i, p in enumerate(plans)]
+    a_ref = [p.H @ fs[i] for i, p in enumerate(plans)]
+    fo = [torch.empty_like(f) for f in fs]
+    ao = [torch.empty_like(a) for a in alms]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(4):  # every slot's replay captured before the timed order checks
+        g.synth(fo, alms, s1)
+        g.adjoint(ao, fs, s1)
+    torch.cuda.synchronize()
+    done = torch.cuda.Event()
+
+    # group on s1 behind a long kernel, then member 0's own synthesis on s2
+    for x in fo:
+        x.fill_(np.nan)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s1):
+        torch.cuda._sleep(int(2e8))
+    g.synth(fo, alms, s1)
+    x0 = torch.empty_like(fs[0])
+    plans[0].synth(x0, alms[0], s2)
+    done.record(s2)
+    assert not done.query(), "the member's own apply did not wait for the group apply"
+    torch.cuda.synchronize()
+    assert rel(x0, f_ref[0]) < 1e-14
+    for i in range(len(plans)):
+        assert rel(fo[i], f_ref[i]) < 1e-14, i
+
+    # member 2's own adjoint on s2 behind a long kernel, then the group adjoint on s1
+    y2 = torch.empty_like(alms[2])
+    with torch.cuda.stream(s2):
+        torch.cuda._sleep(int(2e8))
+    plans[2].adjoint(y2, fs[2], s2)
+    g.adjoint(ao, fs, s1)
+    done.record(s1)
+    assert not done.query(), "the group apply did not wait for the member's own apply"
+    torch.cuda.synchronize()
+    assert rel(y2, a_ref[2]) < 1e-14
+    for i in range(len(plans)):
+        assert rel(ao[i], a_ref[i]) < 1e-14, i
