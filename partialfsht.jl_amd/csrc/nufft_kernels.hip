@@ -1058,31 +1058,25 @@ static void interp_w(const NufftDev &u, const C *U, OutCols f, hipStream_t s) {
                                                                          u.win, u.r0, u.nR, u.B, kt);
 }
 
-template <int W, class C>
-static void spread_w(const NufftDev &u, InCols f, C *Wst, hipStream_t s) {
-    if (u.nst == 0 || u.B <= 0) return;
-    if constexpr (kFp32Math<W, C>) {
-        if (u.B >= 2) {
-            constexpr int G = kSpreadGroup;
-            spread_st32_kernel<W, G><<<dim3(u.nst, (u.B + G - 1) / G), 64 * kSpreadWaves,
-                                       spread32_lds_bytes(u.win, G), s>>>(u.X1, u.X2, u.perm, f, u.st_list,
-                                                                         u.bin_ofs, Wst, u.S, u.ns2, u.win,
-                                                                         u.nst, u.B);
-        } else {
-            spread_st32_kernel<W, 1><<<dim3(u.nst, 1), 64 * kSpreadWaves, spread32_lds_bytes(u.win, 1), s>>>(
-                u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
-        }
-        return;
-    }
-    if (u.B >= 2) {
-        constexpr int G = kSpreadGroup;
-        dim3 grid(u.nst, (u.B + G - 1) / G);
+// G transforms per workgroup (grid.y = ceil(B / G) groups; the last may be partial)
+template <int W, int G, class C>
+static void spread_launch(const NufftDev &u, InCols f, C *Wst, hipStream_t s) {
+    const dim3 grid(u.nst, (u.B + G - 1) / G);
+    if constexpr (kFp32Math<W, C>)
+        spread_st32_kernel<W, G><<<grid, 64 * kSpreadWaves, spread32_lds_bytes(u.win, G), s>>>(
+            u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
+    else
         spread_st_kernel<W, G, C><<<grid, 64 * kSpreadWaves, spread_lds_bytes(u.win, G), s>>>(
             u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
-    } else {
-        spread_st_kernel<W, 1, C><<<dim3(u.nst, 1), 64 * kSpreadWaves, spread_lds_bytes(u.win, 1), s>>>(
-            u.X1, u.X2, u.perm, f, u.st_list, u.bin_ofs, Wst, u.S, u.ns2, u.win, u.nst, u.B);
-    }
+}
+
+template <int W, class C>
+static void spread_w(const NufftDev &u, InCols f, C *Wst, hipStream_t s) {
+    static_assert(kSpreadGroup == 4, "the group sizes below are 1, 2 and kSpreadGroup");
+    if (u.nst == 0 || u.B <= 0) return;
+    if (u.B >= 4) spread_launch<W, 4>(u, f, Wst, s);
+    else if (u.B >= 2) spread_launch<W, 2>(u, f, Wst, s);
+    else spread_launch<W, 1>(u, f, Wst, s);
 }
 
 #define PFSHT_W_SWITCH(w, FN, ...)                 \

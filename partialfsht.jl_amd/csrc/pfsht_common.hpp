@@ -295,7 +295,9 @@ inline size_t nufft_window_lds_bytes(int win) { return sizeof(double2) * size_t(
 constexpr int kRowPad = 17;
 // staging of one wavefront for G transforms: two row tiles and G x 16 complex f values, in doubles
 constexpr int spread_stage_doubles(int G) { return 2 * 16 * kRowPad + 2 * 16 * G; }
-constexpr int kSpreadGroup = 2;  // transforms per spreading workgroup when there are two or more
+// transforms per spreading workgroup: four at four or more transforms, two at two or three (the
+// accumulators of four take 238 VGPRs, two waves per SIMD); kSpreadGroup is the largest
+constexpr int kSpreadGroup = 4;
 constexpr int kSpreadWaves = 8;  // wavefronts per spreading workgroup: two of the 16 bins each
 // Wavefront w accumulates bins w + kSpreadWaves k, k < 16 / kSpreadWaves: the 16 bins of a super-tile
 // are covered exactly once only when the wave count divides 16 (a bin index past 15 would read the
