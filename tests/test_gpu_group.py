@@ -2,8 +2,8 @@
 
 Members are drawn as the small-plan batch is (lmax 256, caps of radius U(5, 20) deg around random
 centres, M = 4 (L theta*)^2 area-uniform points, tol 1e-8, the planner's method), plus a cap too
-small for the stretch, which the planner gives a Direct leaf, and a tol 1e-5 member in mixed
-precision. Every member's result equals its own apply (the group runs the same kernels in the same
+small for the stretch, which the planner gives a Direct leaf, a tol 1e-5 member in mixed
+precision, and a member whose grid stage takes rocFFT (outside the grouped grid-stage launches). Every member's result equals its own apply (the group runs the same kernels in the same
 order: the ring passes grouped into one launch, the rest on concurrent streams) in both directions,
 on repeated calls with new inputs (the hipGraph replays), with eager work of the members' own in
 between; the adjoint identity holds per member.
@@ -38,7 +38,8 @@ def members(P, rng, n=8, L=256):
         M = int(round(4 * (L * np.deg2rad(deg)) ** 2))
         th, ph = centered_cap_points(rng, deg, M, tc, pc)
         tol, prec = (1e-5, "mixed") if i == 1 else (1e-8, "fp64")
-        plans.append(P.Plan(th, ph, lmax=L, tol=tol, method="auto", precision=prec))
+        gf = "library" if i == 2 else "auto"  # a member whose grid stage runs on the member streams
+        plans.append(P.Plan(th, ph, lmax=L, tol=tol, method="auto", precision=prec, grid_fft=gf))
     th, ph = centered_cap_points(rng, 0.5, 60, 1.0, 2.0)  # below the stretch's reach: Direct
     plans.append(P.Plan(th, ph, lmax=L, tol=1e-8, method="auto"))
     return plans
