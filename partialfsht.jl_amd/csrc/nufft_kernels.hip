@@ -347,6 +347,13 @@ void launch_deconv_trunc(const NufftDev &u, const double2 *pc1, const double2 *U
     deconv_trunc_kernel<<<grid, 256, 0, s>>>(U, pc1, u.c2, Fhat, u.n1, u.n2, u.N1, u.N2);
 }
 
+// v mod n for 0 <= v (a super-tile origin plus a window offset: one subtraction at most, unless the
+// window is wider than the grid), without a run-time integer division
+__device__ __forceinline__ int wrap_index(int v, int n) {
+    while (v >= n) v -= n;
+    return v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Interpolation (type 2). One or two transforms: one workgroup per (non-empty super-tile,
 // transform); the super-tile's window of the fine grid (4 S + w - 1 cells square, wrapped
@@ -377,10 +384,10 @@ interp_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     const C *__restrict__ Ub = U + int64_t(b) * nR * N2;
     const gptr fcol = f.col(b);
     for (int r = threadIdx.x >> 6; r < win; r += 4) {
-        int pr = (x0 + r) % N1 - r0;
+        int pr = wrap_index(x0 + r, N1) - r0;
         if (pr < 0) pr += N1;
         if (pr >= nR) continue;  // a row that no point's kernel reaches is neither stored nor read
-        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + (y0 + c) % N2];
+        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + wrap_index(y0 + c, N2)];
     }
     __syncthreads();
     const int start = bin_ofs[slot * 17], end = bin_ofs[slot * 17 + 16];
@@ -607,6 +614,10 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
     // Bins in a static order, so each accumulates into a fixed register tile.
 #pragma unroll
     for (int kb = 0; kb < NBW; ++kb) {
+        // the bin's first base cell along each axis (wave-uniform): a point's base cell lies in its
+        // bin, as the plan keyed it, so its offset within the bin window is the difference
+        const int lbin = wave + kSpreadWaves * kb;
+        const int bx0 = x0 + (lbin >> 2) * S, by0 = y0 + (lbin & 3) * S;
         while (k1 == kb) {  // wave-uniform
             const int n = min(16, be[kb] - c1);
             const double x = x1;
@@ -629,7 +640,7 @@ spread_st_kernel(const double *__restrict__ X1, const double *__restrict__ X2, c
             for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0;
             if (pa < n) {
                 const int bc = int(ceil(x - 0.5 * W));
-                const int o = (bc - (axis == 0 ? x0 : y0)) % S;  // offset within the bin window
+                const int o = bc - (axis == 0 ? bx0 : by0);  // offset within the bin window, [0, S)
                 const double u = 2.0 * (double(bc) - x) + double(W - 1);
                 double kv[NH];
                 es_half_row<W>(h == 0 ? u : -u, kv, ec);
@@ -830,10 +841,10 @@ interp_st32_kernel(const double *__restrict__ X1, const double *__restrict__ X2,
     const float2 *__restrict__ Ub = U + int64_t(b) * nR * N2;
     const gptr fcol = f.col(b);
     for (int r = threadIdx.x >> 6; r < win; r += 4) {
-        int pr = (x0 + r) % N1 - r0;
+        int pr = wrap_index(x0 + r, N1) - r0;
         if (pr < 0) pr += N1;
         if (pr >= nR) continue;  // a row that no point's kernel reaches is neither stored nor read
-        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + (y0 + c) % N2];
+        for (int c = threadIdx.x & 63; c < win; c += 64) lw[r * win + c] = Ub[int64_t(pr) * N2 + wrap_index(y0 + c, N2)];
     }
     __syncthreads();
     const int start = bin_ofs[slot * 17], end = bin_ofs[slot * 17 + 16];
@@ -941,6 +952,10 @@ spread_st32_kernel(const double *__restrict__ X1, const double *__restrict__ X2,
     int p2 = perm[min(c2 + pa, plast)];
 #pragma unroll
     for (int kb = 0; kb < NBW; ++kb) {
+        // the bin's first base cell along each axis (wave-uniform): a point's base cell lies in its
+        // bin, as the plan keyed it, so its offset within the bin window is the difference
+        const int lbin = wave + kSpreadWaves * kb;
+        const int bx0 = x0 + (lbin >> 2) * S, by0 = y0 + (lbin & 3) * S;
         while (k1 == kb) {  // wave-uniform
             const int n = min(16, be[kb] - c1);
             const double x = x1;
@@ -960,7 +975,7 @@ spread_st32_kernel(const double *__restrict__ X1, const double *__restrict__ X2,
             for (int i = 0; i < 8; ++i) row[8 * h + i] = 0.0f;
             if (pa < n) {
                 const int bc = int(ceil(x - 0.5 * W));
-                const int o = (bc - (axis == 0 ? x0 : y0)) % S;
+                const int o = bc - (axis == 0 ? bx0 : by0);
                 const float u = float(2.0 * (double(bc) - x) + double(W - 1));
                 float kv[NH];
                 es_half_row32<W>(h == 0 ? u : -u, kv);
