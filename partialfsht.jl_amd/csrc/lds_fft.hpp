@@ -199,16 +199,18 @@ __device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C 
 // table (the plan's table + f.tw); SW: swizzled LDS slots (lds_slot).
 template <bool INV, int E = kFftElems, bool SW = false, class C>
 __device__ __forceinline__ void lds_fft(C *x, const FftLen &f, const C *__restrict__ tw, int NT) {
+    // q = n / (p r), the table step of a pass of radix r at span p, divided down by the pass's
+    // compile-time radix (no run-time integer division)
     int p = 1;
+    unsigned q = unsigned(f.n);
     for (int s = 0; s < f.npass; ++s) {
         const int r = f.rad[s];
-        const int tstep = f.n / (p * r);
         switch (r) {
-        case 8: fft_pass<8, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
-        case 4: fft_pass<4, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
-        case 2: fft_pass<2, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
-        case 3: fft_pass<3, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
-        default: fft_pass<5, INV, E, SW>(x, f.n, p, tstep, tw, NT); break;
+        case 8: q /= 8u; fft_pass<8, INV, E, SW>(x, f.n, p, int(q), tw, NT); break;
+        case 4: q /= 4u; fft_pass<4, INV, E, SW>(x, f.n, p, int(q), tw, NT); break;
+        case 2: q /= 2u; fft_pass<2, INV, E, SW>(x, f.n, p, int(q), tw, NT); break;
+        case 3: q /= 3u; fft_pass<3, INV, E, SW>(x, f.n, p, int(q), tw, NT); break;
+        default: q /= 5u; fft_pass<5, INV, E, SW>(x, f.n, p, int(q), tw, NT); break;
         }
         p *= r;
     }
