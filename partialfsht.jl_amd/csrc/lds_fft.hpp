@@ -164,16 +164,15 @@ __device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C 
     const float rp = 1.0f / float(p);
     C v[K][R];
     C w1[K];
+    int kk[K];  // butterfly i's position k = i mod p within its span, for the loads and the stores
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int i = int(threadIdx.x) + q * NT;
+        kk[q] = i - int((float(i) + 0.5f) * rp) * p;
         if (i < nbf) {
 #pragma unroll
             for (int j = 0; j < R; ++j) v[q][j] = x[lds_slot<SW>(i + j * nbf)];
-            if (p > 1) {
-                const int g = int((float(i) + 0.5f) * rp);
-                w1[q] = tw[(i - g * p) * tstep];
-            }
+            if (p > 1) w1[q] = tw[kk[q] * tstep];
         }
     }
     __syncthreads();
@@ -181,11 +180,10 @@ __device__ __forceinline__ void fft_pass(C *x, int n, int p, int tstep, const C 
     for (int q = 0; q < K; ++q) {
         const int i = int(threadIdx.x) + q * NT;
         if (i < nbf) {
-            const int g = int((float(i) + 0.5f) * rp);
-            const int k = i - g * p;
+            const int k = kk[q];
             if (p > 1) twiddle<R>(v[q], INV ? cx<C>(w1[q].x, -w1[q].y) : w1[q]);
             dft<R, INV>(v[q]);
-            const int base = g * p * R + k;
+            const int base = (i - k) * R + k;
 #pragma unroll
             for (int j = 0; j < R; ++j) x[lds_slot<SW>(base + j * p)] = v[q][j];
         }
